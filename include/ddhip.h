@@ -1,0 +1,90 @@
+/* ddhip: C-ABI of the MI355X (gfx950) batched steady-state drift-diffusion J-V solve.
+ *
+ * The reference (jaxdd 0.5.0) is pure Python on JAX and has no C/FFI boundary of its own; the
+ * entry points below are what its Python API for this path binds to, each named with the
+ * reference function it replaces. Every pointer argument is DEVICE memory unless stated;
+ * sizes are plain integers; no torch or C++ types cross the boundary. The caller allocates
+ * and frees every buffer, the workspace included (ddh_workspace_bytes); the library allocates
+ * nothing persistent. Errors: 0 on success, a negative code otherwise, with a message from
+ * ddh_last_error() (thread-local). No C++ exception crosses the ABI. Calls on distinct
+ * streams are re-entrant; there is no global mutable state besides the error message.
+ */
+#ifndef DDHIP_H
+#define DDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDH_ABI_VERSION 1
+
+enum ddh_status {
+    DDH_OK = 0,
+    DDH_E_SHAPE = -1,   /* a size or shape the kernels do not take */
+    DDH_E_HIP = -2,     /* a HIP runtime error (launch, sync) */
+    DDH_E_WORKSPACE = -3,
+    DDH_E_ARG = -4      /* a NULL pointer or an unknown option */
+};
+
+enum ddh_precision { DDH_FP64 = 0, DDH_MIXED = 1 };
+
+/* A packed batch of n devices on NP interior points each, lane-innermost
+ * (jaxdd_amd/csrc/ddhip_layout.h documents every field). Built by jaxdd_amd.batch.prepare,
+ * the counterpart of src/jaxdd/batch.py:151-223 (`prepare`). */
+typedef struct {
+    int32_t n;        /* devices */
+    int32_t NP;       /* interior grid points */
+    int32_t nseg;     /* segments per device (max over the batch) */
+    int32_t reserved;
+    const double *pt;   /* [NP + 2][4][n] */
+    const uint8_t *sid; /* [NP + 2][n] */
+    const double *seg;  /* [nseg][18][n] */
+    const double *sc;   /* [20][n] */
+} ddh_batch;
+
+/* Options of src/jaxdd/batch.py:1136-1137 (`solve_batch`) and src/jaxdd/solver.py:2041-2043
+ * (`solve_jv`). */
+typedef struct {
+    double tol;          /* Newton step tolerance (1e-10) */
+    int32_t max_iter;    /* Newton steps per voltage (60 batch, 200 jv) */
+    int32_t start_iter;  /* Poisson and start solves (max_iter) */
+    int32_t strict;      /* 1: only the step below tol or the round-off floor stop Newton */
+    int32_t bernoulli;   /* 0 exact B(x), 1 SIMsalabim's piecewise approximation */
+    int32_t precision;   /* ddh_precision */
+    int32_t jv_mode;     /* 0: solve_batch semantics; 1: solve_jv semantics (given order,
+                            no extrapolation, tol/stall/noise-floor stops, pre-bias must hold) */
+} ddh_opts;
+
+int ddh_abi_version(void);
+const char *ddh_last_error(void);
+
+/* Workspace bytes ddh_solve_batch needs for this shape and precision. */
+size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision);
+
+/* The J-V curves of a batch: replaces src/jaxdd/batch.py:1033-1047 (`_solve_batch`, the
+ * jit+vmap of `_solve_one` :937-1006) and, with opts.jv_mode = 1, src/jaxdd/solver.py:2105-2245
+ * (`_scan` of `solve_jv`) for each device. V: nV voltages (device memory), ascending in batch
+ * mode, the caller's order in jv mode. Outputs [n][nV]: J (A/m^2, without the shunt),
+ * converged, iterations. status: [n] int32, nonzero where a device's pre-bias failed in jv
+ * mode (the reference raises RuntimeError there). */
+int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
+                    double *J, uint8_t *ok, int32_t *its, int32_t *status, void *workspace,
+                    size_t workspace_bytes, void *hip_stream);
+
+/* The batched block-tridiagonal sweep: replaces src/jaxdd/sweep.py:313-330 (`block_sweep`,
+ * the custom_vmap operator over src/jaxdd/solver.py:1534-1560 `_block_thomas`). Solves
+ * A[i] x[i-1] + B[i] x[i] + C[i] x[i+1] = d[i] for n systems of NP blocks b x b (b = 3 or 5)
+ * with k <= 8 right-hand sides. Layout lane-innermost: A, B, C [NP][b*b][n]; d_inout
+ * [NP][b*k][n], overwritten by x. precision: DDH_FP64 (double arrays) or DDH_MIXED (float
+ * arrays). workspace: NP*(b*b + b*k)*n elements of the same type. */
+int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, const void *A, const void *B,
+                    const void *C, void *d_inout, int32_t precision, void *workspace,
+                    void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+"""ctypes binding of ``libddhip.so`` (include/ddhip.h). The product path calls the HIP kernels
+only through this module; a missing library is an error, never a fallback."""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libddhip.so"
+
+DDH_FP64, DDH_MIXED = 0, 1
+PRECISIONS = {"fp64": DDH_FP64, "mixed": DDH_MIXED}
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("NP", ctypes.c_int32), ("nseg", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("pt", ctypes.c_void_p), ("sid", ctypes.c_void_p),
+                ("seg", ctypes.c_void_p), ("sc", ctypes.c_void_p)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+                ("start_iter", ctypes.c_int32), ("strict", ctypes.c_int32),
+                ("bernoulli", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("jv_mode", ctypes.c_int32)]
+
+
+#: The symbols include/ddhip.h declares.
+SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solve_batch",
+           "ddh_block_sweep")
+
+_lib = None
+
+
+def load():
+    """The loaded library (once per process). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        raise RuntimeError(f"{LIB} is missing: build it first (python -m jaxdd_amd._build, or "
+                           "__graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(str(LIB))
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    lib.ddh_abi_version.restype = ctypes.c_int
+    lib.ddh_last_error.restype = ctypes.c_char_p
+    lib.ddh_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_workspace_bytes.argtypes = [i32, i32, i32, i32]
+    lib.ddh_solve_batch.restype = ctypes.c_int
+    lib.ddh_solve_batch.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts), vp, vp,
+                                    vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_block_sweep.restype = ctypes.c_int
+    lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
+    if lib.ddh_abi_version() != 1:
+        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 1")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().ddh_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")

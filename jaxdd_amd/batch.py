@@ -1,0 +1,199 @@
+"""Many devices at once: `prepare` and `solve_batch`, the reference's batched API
+(src/jaxdd/batch.py:151-223 `prepare`, :1136-1162 `solve_batch`, :1104-1108
+`external_voltages`), solved by the HIP kernels of ``libddhip.so``.
+
+`prepare` packs a list of devices into device-resident (HBM) torch tensors in the layout the
+kernels read (jaxdd_amd/csrc/ddhip_layout.h): per grid point the spacing, position,
+generation and fixed ions; per segment (a run of points with one material and one interval
+mobility) the material parameters; per device the scalars. The returned dict also holds
+the reference's per-device keys (``J_scale``, ``g_frac``, ``R_series``, ``G_shunt``, ...),
+which `solve_batch` reads at every call, so ``dict(batch, g_frac=...)`` works as there.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import Q
+from .device import Device
+
+__all__ = ["external_voltages", "pack", "prepare", "solve_batch"]
+
+# Field orders of jaxdd_amd/csrc/ddhip_layout.h.
+PT_FIELDS = ("h", "x", "G", "fix")
+SEG_FIELDS = ("eps", "offn", "offp", "Nc", "dop", "kd", "ni2", "Nt", "Cn", "Cp", "nt0", "pt0",
+              "am", "cm", "mun", "mup", "Ec", "ni")
+SC_FIELDS = ("L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_right",
+             "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
+             "G_shunt", "n_floor", "pre_cond", "fix_ions")
+DENSITY_FLOOR = 1e-12  # src/jaxdd/solver.py:54
+
+
+def _point_fields(dev: Device) -> np.ndarray:
+    """(NP + 2, 18) material parameters per point, in `SEG_FIELDS` order."""
+    return np.stack([
+        dev.eps, dev.offset_n, dev.offset_p, dev.N_c, dev.N_A - dev.N_D, dev.k_direct,
+        dev.ni ** 2, dev.srh_Nt, dev.srh_Cn, dev.srh_Cp, dev.srh_nt0, dev.srh_pt0,
+        np.asarray(dev.anion_mask, dtype=float), np.asarray(dev.cation_mask, dtype=float),
+        dev.mu_n, dev.mu_p, dev.E_c, dev.ni,
+    ], axis=-1).astype(np.float64)
+
+
+def j_scale(dev: Device) -> float:
+    """The current scale (src/jaxdd/batch.py:96, 115): the generation's current, >= 1 A/m^2."""
+    generated = Q * dev.L * np.sum(dev.h[:-1] * dev.G[:-1]) / np.sum(dev.h[:-1])
+    return max(float(generated), 1.0)
+
+
+def _scalars(dev: Device) -> list[float]:
+    return [
+        dev.L, dev.Vt, dev.G_frac, dev.V0, dev.VL, dev.W_L, dev.W_R,
+        np.log(dev.n_left), np.log(dev.n_right), np.log(dev.p_left), np.log(dev.p_right),
+        float(dev.left_elec), dev.anion_total, dev.cation_total, dev.V_pre, j_scale(dev),
+        1.0 / dev.R_shunt if dev.R_shunt > 0 else 0.0,
+        DENSITY_FLOOR * float(np.max(dev.N_c)), float(dev.pre_cond), float(dev.fix_ions),
+    ]
+
+
+def _check(devices) -> None:
+    for d in devices:
+        if d.statistics != "boltzmann" or any(d.free) or d.curr_diff:
+            raise NotImplementedError("outside this path: Boltzmann carriers, no free ends, the "
+                                      "integral current")
+
+
+def pack(devices: list[Device]) -> dict[str, np.ndarray]:
+    """The kernels' arrays of a list of devices (host NumPy, lane-innermost)."""
+    sizes = {len(d.x) for d in devices}
+    if len(sizes) != 1:
+        raise ValueError(f"a batch needs one grid size, got NP + 2 in {sorted(sizes)}")
+    _check(devices)
+    n = len(devices)
+    P = next(iter(sizes))
+    fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 18)
+    change = np.any(fields[:, 1:] != fields[:, :-1], axis=2)
+    sid = np.concatenate([np.zeros((n, 1), dtype=np.int64), np.cumsum(change, axis=1)], axis=1)
+    nseg = int(sid.max()) + 1
+    if nseg > 255:
+        raise NotImplementedError(f"{nseg} segments of constant material: at most 255")
+    seg = np.zeros((nseg, len(SEG_FIELDS), n))
+    first = np.concatenate([np.ones((n, 1), dtype=bool), change], axis=1)  # (n, P)
+    dev_idx, pt_idx = np.nonzero(first)
+    seg[sid[dev_idx, pt_idx], :, dev_idx] = fields[dev_idx, pt_idx]
+    # segments beyond a device's own count repeat its last (never read)
+    last = sid[:, -1]
+    for s in range(nseg):
+        pad = np.flatnonzero(last < s)
+        if pad.size:
+            seg[s, :, pad] = seg[last[pad], :, pad]
+    pt = np.stack([
+        np.stack([d.h, d.x, d.G, d.anion_fixed - d.cation_fixed], axis=0) for d in devices
+    ], axis=-1)  # (4, P, n)
+    pt = np.ascontiguousarray(pt.transpose(1, 0, 2))  # (P, 4, n)
+    sc = np.ascontiguousarray(np.array([_scalars(d) for d in devices]).T)  # (20, n)
+    return {"pt": pt, "sid": np.ascontiguousarray(sid.T.astype(np.uint8)), "seg": seg,
+            "sc": sc, "nseg": nseg}
+
+
+def prepare(devices: list[Device], device: str | torch.device = "cuda") -> dict:
+    """A batch (reference: src/jaxdd/batch.py:151-223): the devices packed for the kernels, in
+    HBM on ``device``, with the reference's per-device keys. Raises `ValueError` on devices of
+    different grid sizes, as the reference does."""
+    packed = pack(devices)
+    n = len(devices)
+    NP = len(devices[0].x) - 2
+    dev = torch.device(device)
+
+    def t(a, dtype=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
+
+    sc = packed["sc"]
+    col = {name: sc[k] for k, name in enumerate(SC_FIELDS)}
+    batch = {
+        "n": n, "NP": NP, "nseg": packed["nseg"],
+        "_pt": t(packed["pt"]), "_sid": t(packed["sid"], torch.uint8), "_seg": t(packed["seg"]),
+        "_sc": t(sc),
+        # the reference's per-device keys (read by solve_batch at each call)
+        "J_scale": t(col["J_scale"]), "g_frac": t(col["g_frac"]), "Vt": t(col["Vt"]),
+        "L": t(col["L"]), "V_pre": t(col["V_pre"]), "pre_cond": t(col["pre_cond"]),
+        "fix_ions": t(col["fix_ions"]), "left_elec": t(col["left_elec"]),
+        "G_shunt": t(col["G_shunt"]),
+        "R_series": t(np.array([d.R_series for d in devices])),
+        "x": t(np.stack([d.x for d in devices])), "h": t(np.stack([d.h for d in devices])),
+    }
+    return batch
+
+
+def _sc_now(batch: dict) -> torch.Tensor:
+    """The packed scalars with the per-device keys the caller may have replaced."""
+    sc = batch["_sc"].clone()
+    for name in ("g_frac", "J_scale", "V_pre"):
+        sc[SC_FIELDS.index(name)] = batch[name].to(sc.device, torch.float64)
+    return sc
+
+
+def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int):
+    """One call of ddh_solve_batch: (J, ok, its, status) as (n, nV) tensors on the batch's
+    device, V as given (sorted by the caller in batch mode)."""
+    lib = _lib.load()
+    dev = batch["_pt"].device
+    if dev.type != "cuda":
+        raise RuntimeError("solve_batch runs on the GPU: prepare(devices, device='cuda')")
+    n, NP, nV = batch["n"], batch["NP"], len(V)
+    sc = _sc_now(batch)
+    Vd = torch.as_tensor(np.ascontiguousarray(V, dtype=np.float64), device=dev)
+    J = torch.empty((n, nV), dtype=torch.float64, device=dev)
+    ok = torch.zeros((n, nV), dtype=torch.uint8, device=dev)
+    its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    nbytes = lib.ddh_workspace_bytes(n, NP, nV, precision)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
+                   batch["_seg"].data_ptr(), sc.data_ptr())
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib.ddh_solve_batch(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
+                             J.data_ptr(), ok.data_ptr(), its.data_ptr(), status.data_ptr(),
+                             ws.data_ptr(), nbytes, stream)
+    _lib.check(rc, "ddh_solve_batch")
+    # the workspace and the inputs live until the kernels are done
+    batch["_inflight"] = (ws, sc, Vd)
+    return J, ok, its, status
+
+
+def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
+                bernoulli_name="exact", strict=False, devices=None, precision="fp64"):
+    """J-V curves of a batch at ``voltages`` (reference: src/jaxdd/batch.py:1136-1162):
+    ``(J, converged, iterations)``, each batch x voltages, in the order of ``voltages``, as
+    torch tensors on the batch's GPU (float64, bool, int64).
+
+    Each device's scan runs outwards from its pre-bias, as the reference's. The reference's two
+    passes (``first_max_iter``, then the stragglers with ``max_iter``) give every device the
+    result of one pass with ``max_iter``; here each device runs on its own lane of a
+    persistent kernel, so no device waits for another and one pass with ``max_iter`` is run.
+    ``precision``: "fp64", or "mixed" (the Newton correction's block factorisation in FP32 on
+    row-equilibrated blocks; residual, exponentials, current and convergence tests in FP64).
+    ``devices`` is accepted for the reference's signature: one GPU solves the whole batch."""
+    del first_max_iter, devices
+    if precision not in _lib.PRECISIONS:
+        raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
+    if bernoulli_name not in ("exact", "simss"):
+        raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
+    V = np.asarray(voltages, dtype=float).ravel()
+    order = np.argsort(V, kind="stable")
+    opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
+                     0 if bernoulli_name == "exact" else 1, _lib.PRECISIONS[precision], 0)
+    J, ok, its, _ = _run(batch, V[order], opts, _lib.PRECISIONS[precision])
+    inv = torch.as_tensor(np.argsort(order, kind="stable"), device=J.device)
+    J, ok, its = J[:, inv], ok[:, inv].bool(), its[:, inv].long()
+    Vt = torch.as_tensor(V, device=J.device)
+    return J + batch["G_shunt"][:, None] * Vt[None, :], ok, its
+
+
+def external_voltages(batch, voltages, J):
+    """``V_ext = V_int + J * R_series`` (reference: src/jaxdd/batch.py:1104-1108)."""
+    Vt = torch.as_tensor(np.asarray(voltages, dtype=float), device=J.device)
+    return Vt[None, :] + J * batch["R_series"][:, None]

@@ -1,0 +1,949 @@
+// Per-lane logic of the batched J-V solve, shared by the HIP kernels (ddhip_solve.hip) and a
+// test-only CPU build of the same source (tests/kernel_emu), which replays each wavefront's
+// lanes in turn so that the kernels' algorithm can be debugged and checked on a host.
+//
+// One lane per device runs that device's whole sequence -- Poisson start, the start solves
+// with the ions' stages, the pre-bias and its freezing, and the voltage scan -- so no lane
+// waits for another device's Newton iterations (the reference's vmap runs every voltage until
+// its slowest lane has converged, src/jaxdd/batch.py:740-848, docs/batching.md).
+//
+// Each Newton step is two sweeps over the grid:
+//   elimination (right to left): the state's Scharfetter-Gummel residual and 3x3 Jacobian
+//     blocks in FP64 (src/jaxdd/solver.py:1021-1187), eliminated block by block as they are
+//     made, the factors (c', d') written in T (double, or float on row-equilibrated blocks);
+//     the ion border's Schur complement (src/jaxdd/batch.py:595-603) closes at the sweep's
+//     end through an adjoint recursion gamma_i = W_i - c'_{i+1}^T gamma_{i+1}, so that
+//     W^T x = sum_i gamma_i^T d'_i needs no second pass;
+//   back-substitution (left to right): the step, its caps (solver.py:59-88), the new state,
+//     the step size, and the integral current of the new state in the reference's prefix
+//     order (src/jaxdd/batch.py:302-403).
+#pragma once
+
+#include <stdint.h>
+
+#ifdef DDH_CPU_EMULATION
+#include <cmath>
+#include <cstring>
+#define DDH_FN inline
+#define DDH_CONST static const
+using std::isfinite;
+#else
+#include <hip/hip_runtime.h>
+#define DDH_FN __device__ inline
+#define DDH_CONST __constant__
+#endif
+
+#include "ddhip_layout.h"
+#include "ddhip_math.h"
+
+namespace ddh {
+
+// ---------------------------------------------------------------------------------------
+// Kernel arguments and per-lane state
+// ---------------------------------------------------------------------------------------
+enum Phase { PH_F = 0, PH_STAGE, PH_PRE, PH_SCAN, PH_DONE };
+
+struct LaneState {
+    int phase, stage, j, below, it, maxit, err;
+    int b_pois, b_f, b_c, b_work, b_last, b_before, b_start, b_src1, b_src2;
+    int fresh, freeze_now, mobile_now, mobile0, ok_f, fin_f, ok_c, pre_ok, fin_work;
+    int noise_state;  // jv mode: 0 no noise floor yet, 1 pending, 2 known
+    int need64;       // mixed precision: this solve has switched to FP64 factors
+    double Vint, share, ratio, first, V_last, V_before, before_share, noise_floor;
+    double lw[2], lpois[2], lf[2], lc[2], llast[2], lbefore[2], lstart[2];
+    double hist[4], Jh[4];
+    double z[2];
+};
+
+struct Args {
+    int n, NP, nV, nseg;
+    const double *pt;
+    const uint8_t *sid;
+    const double *seg;
+    const double *sc;
+    const double *V;
+    double *Y;        // [NBUF][NP][3][n]
+    void *fac;        // T [NP][NFAC][n]
+    double *fix;      // [NP][n] net fixed ions in use (frozen after the pre-bias)
+    LaneState *st;    // [n]
+    double *J;        // [n][nV]
+    uint8_t *ok;
+    int32_t *its;
+    int32_t *status;
+    double tol;
+    int max_iter, start_iter, strict, bern, jv;
+    int mixed;  // FP32 factors, a solve switching to FP64 after MIXED_STEPS steps
+};
+
+// Mixed precision: a Newton step uses FP32 factors only while the solve is far from its
+// solution; once a step falls below MIXED_SWITCH (V_T, relative densities), or after
+// MIXED_STEPS steps, the solve's remaining steps use FP64 factors. The steps that decide
+// convergence (src/jaxdd/batch.py:793-818: the quadratic and settled stops assume true Newton)
+// are then FP64 Newton steps, and an FP32 correction that no longer contracts (an
+// ill-conditioned Jacobian at high forward bias) cannot stall a solve.
+#ifndef DDH_MIXED_SWITCH
+#define DDH_MIXED_SWITCH 1e-2
+#endif
+constexpr double MIXED_SWITCH = DDH_MIXED_SWITCH;
+constexpr int MIXED_STEPS = 4;
+
+#define SC(f) a.sc[(size_t)(f) * a.n + lane]
+#define PT(j, f) a.pt[((size_t)(j) * PT_N + (f)) * a.n + lane]
+#define SID(j) a.sid[(size_t)(j) * a.n + lane]
+#define YB(b, j, f) a.Y[(((size_t)(b) * a.NP + ((j) - 1)) * 3 + (f)) * a.n + lane]
+#define FIX(j) a.fix[(size_t)((j) - 1) * a.n + lane]
+
+struct Seg {
+    int id;
+    double v[SEG_N];
+};
+
+DDH_FN void seg_at(const Args &a, int lane, int j, Seg &s) {
+    int id = SID(j);
+    if (id != s.id) {
+        s.id = id;
+#pragma unroll
+        for (int f = 0; f < SEG_N; ++f) s.v[f] = a.seg[((size_t)id * SEG_N + f) * a.n + lane];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Poisson's start: src/jaxdd/batch.py:486-503 with src/jaxdd/solver.py:1657-1800
+// (quasi-Fermi levels held, the ions re-fitted each step, the bordered step damped as a
+// whole) and the linear start src/jaxdd/batch.py:276-299. Scalar tridiagonal per lane.
+// ---------------------------------------------------------------------------------------
+struct IonSum {  // trapezoidal number of each species over its region (solver.py:1241-1252)
+    double num[2];
+};
+
+DDH_FN double contact_V(const Args &a, int lane, double Vint, bool right) {
+    return right ? SC(SC_VL) - 0.5 * SC(SC_ELEC) * Vint : SC(SC_V0) + 0.5 * SC(SC_ELEC) * Vint;
+}
+
+// Closed-form Boltzmann normalisations from the number integrals at logs = 0
+// (solver.py:1695-1712).
+DDH_FN void refit_logs(const double num0[2], const double tot[2], double out[2]) {
+    for (int k = 0; k < 2; ++k) out[k] = tot[k] > 0 ? log(tot[k] / num0[k]) : 0.0;
+}
+
+DDH_FN void poisson_lane(const Args &a, int lane) {
+    LaneState &S = a.st[lane];
+    const int NP = a.NP;
+    const double Vt = SC(SC_VT), L = SC(SC_L), L2 = L * L;
+    const double tot[2] = {SC(SC_AT), SC(SC_CT)};
+    const bool mobile = tot[0] > 0 || tot[1] > 0;
+    // the pre-bias, else the first voltage (the lowest in batch mode, where V is sorted)
+    const double first = SC(SC_PRE) > 0 ? SC(SC_VPRE) : a.V[0];
+    const double Vl = contact_V(a, lane, first, false), Vr = contact_V(a, lane, first, true);
+    const double elec = SC(SC_ELEC);
+    double *cfac = (double *)a.fac;  // [NP][4][n] reinterpreted: c', d'(3)
+#define PF(j, f) cfac[((size_t)((j) - 1) * 4 + (f)) * a.n + lane]
+
+    // the linear start and the ions' numbers at logs = 0 (for the first refit)
+    Seg sg; sg.id = -1;
+    double num0[2] = {0.0, 0.0};
+    double prevV = Vl, prevam = 0, prevcm = 0, prevh = 0;
+    {
+        seg_at(a, lane, 0, sg);
+        prevam = sg.v[SEG_AM]; prevcm = sg.v[SEG_CM]; prevh = PT(0, PT_H);
+    }
+    for (int j = 1; j <= NP + 1; ++j) {
+        seg_at(a, lane, j, sg);
+        double V;
+        if (j <= NP) {
+            double x = PT(j, PT_X);
+            V = SC(SC_V0) + 0.5 * elec * first + x * (SC(SC_VL) - SC(SC_V0) - elec * first) / L;
+            double EF = SC(SC_WL) - x * (SC(SC_WL) - SC(SC_WR)) / L;
+            double ln_n = log(sg.v[SEG_NC]) - (EF - sg.v[SEG_EC]) / Vt;
+            YB(0, j, 0) = V;
+            YB(0, j, 1) = ln_n;
+            YB(0, j, 2) = 2 * log(sg.v[SEG_NI]) - ln_n;
+            FIX(j) = PT(j, PT_FIX);
+        } else {
+            V = Vr;
+        }
+        if (mobile) {
+            double am = sg.v[SEG_AM], cm = sg.v[SEG_CM];
+            double hL = prevh * L;
+            num0[0] += 0.5 * (am * exp(V / Vt) + prevam * exp(prevV / Vt)) * (am * prevam) * hL;
+            num0[1] += 0.5 * (cm * exp(-V / Vt) + prevcm * exp(-prevV / Vt)) * (cm * prevcm) * hL;
+            prevam = am; prevcm = cm;
+        }
+        prevV = V;
+        prevh = PT(j, PT_H);
+    }
+    double logs[2] = {0.0, 0.0};
+    if (mobile) refit_logs(num0, tot, logs);
+
+    double size = INFINITY;
+    int it = 0;
+    while (size > POISSON_STOP && it < a.start_iter) {
+        // elimination, right to left: m_i = mid_i - up_i c'_{i+1}, c'_i = lo_i/m_i, d'_i =
+        // (r_i - up_i d'_{i+1})/m_i for r = (-F, U_anion, U_cation); gamma for the border.
+        double cprev = 0.0, dprev[3] = {0.0, 0.0, 0.0}, gam[2] = {0.0, 0.0};
+        double accF[2] = {0.0, 0.0}, accU[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        double numk[2] = {0.0, 0.0};
+        Seg s2; s2.id = -1;
+        // the right neighbour's values, rolled along: row i is assembled at its own point,
+        // its left neighbour's potential read ahead
+        seg_at(a, lane, NP + 1, s2);
+        double Vp = Vr, amp = s2.v[SEG_AM], cmp = s2.v[SEG_CM];
+        double ap = amp * exp(logs[0] + Vp / Vt), cp_ = cmp * exp(logs[1] - Vp / Vt);
+        for (int i = NP; i >= 1; --i) {
+            seg_at(a, lane, i, s2);
+            double V = YB(0, i, 0), ln_n = YB(0, i, 1), ln_p = YB(0, i, 2);
+            double n = exp(ln_n), p = exp(ln_p);
+            double h = PT(i, PT_H), hm = PT(i - 1, PT_H);
+            double Vm1 = (i == 1) ? Vl : YB(0, i - 1, 0);
+            double eps = s2.v[SEG_EPS];
+            const int idm = SID(i - 1);  // the left neighbour's segment
+            double epsm = (idm == s2.id) ? eps : a.seg[((size_t)idm * SEG_N + SEG_EPS) * a.n + lane];
+            double hs = h + hm;
+            double fac_m = (eps - epsm) / (hm * hm * L2) - 2 * eps * (1 / (h * hs * L2) + 1 / (hm * hs * L2));
+            double fac_l = 2 * eps / (hm * hs * L2) - (eps - epsm) / (hm * hm * L2);
+            double fac_u = 2 * eps / (h * hs * L2);
+            double sP = 1.0 / (Q_E * s2.v[SEG_NC]);
+            double am = s2.v[SEG_AM], cm = s2.v[SEG_CM];
+            double an = am * exp(logs[0] + V / Vt), ca = cm * exp(logs[1] - V / Vt);
+            double F = fac_l * Vm1 + fac_m * V + fac_u * Vp -
+                       Q_E * (n - p + s2.v[SEG_DOP] + FIX(i) + an - ca);
+            double lo = (i == 1) ? 0.0 : fac_l * sP;
+            double up = (i == NP) ? 0.0 : fac_u * sP;
+            double mid = (fac_m - Q_E * (n / Vt + p / Vt + an / Vt + ca / Vt)) * sP;
+            double r0 = -F * sP, r1 = -Q_E * an * sP * am, r2 = Q_E * ca * sP * cm;
+            double m = mid - up * cprev;
+            double rm = 1.0 / m;
+            double c = lo * rm;
+            double d0 = (r0 - up * dprev[0]) * rm;
+            double d1 = (r1 - up * dprev[1]) * rm;
+            double d2 = (r2 - up * dprev[2]) * rm;
+            PF(i, 0) = c; PF(i, 1) = d0; PF(i, 2) = d1; PF(i, 3) = d2;
+            // the constraints' gradient in V_i and the numbers (interval i, i+1)
+            double hL = h * L, hLm = hm * L;
+            double amm = (idm == s2.id) ? am : a.seg[((size_t)idm * SEG_N + SEG_AM) * a.n + lane];
+            double cmm = (idm == s2.id) ? cm : a.seg[((size_t)idm * SEG_N + SEG_CM) * a.n + lane];
+            double w0 = 0.0, w1 = 0.0;
+            if (mobile) {
+                double wsa = 0.5 * (am * amm * hLm + amp * am * hL);
+                double wsc = 0.5 * (cm * cmm * hLm + cmp * cm * hL);
+                w0 = tot[0] > 0 ? wsa * an * (1.0 / Vt) / tot[0] : 0.0;
+                w1 = tot[1] > 0 ? wsc * ca * (-1.0 / Vt) / tot[1] : 0.0;
+                numk[0] += 0.5 * (ap + an) * (amp * am) * hL;
+                numk[1] += 0.5 * (cp_ + ca) * (cmp * cm) * hL;
+            }
+            gam[0] = w0 - cprev * gam[0];
+            gam[1] = w1 - cprev * gam[1];
+            accF[0] += gam[0] * d0; accF[1] += gam[1] * d0;
+            accU[0][0] += gam[0] * d1; accU[0][1] += gam[0] * d2;
+            accU[1][0] += gam[1] * d1; accU[1][1] += gam[1] * d2;
+            cprev = c; dprev[0] = d0; dprev[1] = d1; dprev[2] = d2;
+            Vp = V; amp = am; cmp = cm; ap = an; cp_ = ca;
+        }
+        double z0 = 0.0, z1 = 0.0;
+        if (mobile) {
+            // the interval (0, 1) of the numbers
+            seg_at(a, lane, 0, s2);
+            double am0 = s2.v[SEG_AM], cm0 = s2.v[SEG_CM], hL0 = PT(0, PT_H) * L;
+            numk[0] += 0.5 * (ap + am0 * exp(logs[0] + Vl / Vt)) * (amp * am0) * hL0;
+            numk[1] += 0.5 * (cp_ + cm0 * exp(logs[1] - Vl / Vt)) * (cmp * cm0) * hL0;
+            double g[2], D[2];
+            for (int k = 0; k < 2; ++k) {
+                bool has = tot[k] > 0;
+                g[k] = has ? numk[k] / tot[k] - 1.0 : 0.0;
+                D[k] = has ? numk[k] / tot[k] : 1.0;
+            }
+            z0 = -g[0] - accF[0];
+            z1 = -g[1] - accF[1];
+            solve2(D[0] - accU[0][0], -accU[0][1], -accU[1][0], D[1] - accU[1][1], z0, z1);
+        }
+        // back-substitution, left to right; with mobile ions first the step's largest entry
+        double scale = 1.0;
+        if (mobile) {
+            double xprev = 0.0, mx = 0.0;
+            for (int i = 1; i <= NP; ++i) {
+                double x = PF(i, 1) - (PF(i, 2) * z0 + PF(i, 3) * z1) - PF(i, 0) * xprev;
+                mx = nmax(mx, fabs(x));
+                xprev = x;
+            }
+            double mm = mx / Vt;
+            scale = mm > 0 ? fmin(log1p(mm), CAP_V) / mm : 1.0;
+            if (mm != mm) scale = mm;
+        }
+        double xprev = 0.0, sz = 0.0, cap = CAP_V * Vt;
+        double nnum[2] = {0.0, 0.0};
+        Seg s4; s4.id = -1;
+        seg_at(a, lane, 0, s4);
+        double pV = Vl, pam = s4.v[SEG_AM], pcm = s4.v[SEG_CM], ph = PT(0, PT_H);
+        for (int i = 1; i <= NP + 1; ++i) {
+            double V;
+            seg_at(a, lane, i, s4);
+            if (i <= NP) {
+                double x = PF(i, 1) - (PF(i, 2) * z0 + PF(i, 3) * z1) - PF(i, 0) * xprev;
+                xprev = x;
+                double dV = mobile ? x * scale : clipd(x, -cap, cap);
+                double shift = dV / Vt;
+                V = YB(0, i, 0) + dV;
+                YB(0, i, 0) = V;
+                YB(0, i, 1) += shift;
+                YB(0, i, 2) -= shift;
+                sz = nmax(sz, fabs(shift));
+            } else {
+                V = Vr;
+            }
+            if (mobile) {
+                double am = s4.v[SEG_AM], cm = s4.v[SEG_CM], hL = ph * L;
+                nnum[0] += 0.5 * (am * exp(V / Vt) + pam * exp(pV / Vt)) * (am * pam) * hL;
+                nnum[1] += 0.5 * (cm * exp(-V / Vt) + pcm * exp(-pV / Vt)) * (cm * pcm) * hL;
+                pam = am; pcm = cm;
+            }
+            pV = V;
+            ph = PT(i, PT_H);
+        }
+        if (mobile) refit_logs(nnum, tot, logs);
+        size = sz;
+        ++it;
+    }
+#undef PF
+    S.lpois[0] = logs[0];
+    S.lpois[1] = logs[1];
+    S.first = first;
+    S.b_pois = 0;
+    S.mobile0 = mobile ? 1 : 0;
+    a.status[lane] = 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// The Newton sweeps.
+// ---------------------------------------------------------------------------------------
+struct Pt {
+    double V, ln_n, ln_p, n, p, offn, offp, h, eps, Nc, mumax, mun, mup;
+    double R, dRn, dRp, G, fix, dop, an, ca, am, cm;
+};
+
+struct Edge {  // Scharfetter-Gummel fluxes over one interval and their derivatives
+    double Fn, Fp, gn, gp, an0, an1, ap0, ap1;
+};
+
+DDH_FN Edge edge_flux(const Pt &l, const Pt &r, double Vt, int bern) {
+    Edge e;
+    double cn = Vt * l.mun, cp = Vt * l.mup;
+    double xn = ((r.V + r.offn) - (l.V + l.offn)) / Vt;
+    double xp = ((r.V + r.offp) - (l.V + l.offp)) / Vt;
+    double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
+    bernoulli_pair(xn, bern, Bnf, Bnb, dBnf, dBnb);
+    bernoulli_pair(xp, bern, Bpf, Bpb, dBpf, dBpb);
+    e.Fn = cn * (Bnf * r.n - Bnb * l.n);
+    e.gn = cn * (dBnf * r.n + dBnb * l.n) / Vt;
+    e.an0 = -cn * Bnb * l.n;
+    e.an1 = cn * Bnf * r.n;
+    e.Fp = cp * (Bpf * l.p - Bpb * r.p);
+    e.gp = cp * (dBpf * l.p + dBpb * r.p) / Vt;
+    e.ap0 = cp * Bpf * l.p;
+    e.ap1 = -cp * Bpb * r.p;
+    return e;
+}
+
+struct SweepCtl {  // what one lane's sweeps need of its state
+    double Vint, share, ratio, g_frac;
+    double logs[2];     // normalisations at the state being assembled
+    double lstart[2];   // the pre-bias's (for the freeze)
+    int fresh, freeze_now, mobile, src1, src2, work;
+};
+
+// Fills the point values of interior point j from the state buffers (forming the start of a
+// new solve where `fresh`) and its segment; contacts from the boundary values.
+DDH_FN void load_point(const Args &a, int lane, int j, const Seg &s, const SweepCtl &c,
+                                  Pt &q, double Vt, bool need_rates) {
+    const int NP = a.NP;
+    q.offn = s.v[SEG_OFFN]; q.offp = s.v[SEG_OFFP]; q.eps = s.v[SEG_EPS]; q.Nc = s.v[SEG_NC];
+    q.mun = s.v[SEG_MUN]; q.mup = s.v[SEG_MUP];
+    q.mumax = q.mun > q.mup ? q.mun : q.mup;
+    q.h = PT(j, PT_H);
+    q.dop = s.v[SEG_DOP];
+    q.am = c.mobile ? s.v[SEG_AM] : 0.0;
+    q.cm = c.mobile ? s.v[SEG_CM] : 0.0;
+    if (j == 0 || j == NP + 1) {
+        bool right = j == NP + 1;
+        q.V = contact_V(a, lane, c.Vint, right);
+        q.ln_n = right ? SC(SC_LNR) : SC(SC_LNL);
+        q.ln_p = right ? SC(SC_LPR) : SC(SC_LPL);
+        q.G = 0.0; q.fix = 0.0;
+    } else {
+        double y0, y1, y2;
+        if (c.fresh) {
+            y0 = YB(c.src1, j, 0); y1 = YB(c.src1, j, 1); y2 = YB(c.src1, j, 2);
+            if (c.freeze_now) {
+                // the pre-bias's mobile ions as fixed ones (src/jaxdd/batch.py:859-872)
+                double V = y0;
+                double fx = PT(j, PT_FIX) + s.v[SEG_AM] * exp(c.lstart[0] + V / Vt) -
+                            s.v[SEG_CM] * exp(c.lstart[1] - V / Vt);
+                FIX(j) = fx;
+            }
+            if (c.ratio != 0.0) {
+                double capv = CAP_V * Vt;
+                y0 += clipd(c.ratio * (y0 - YB(c.src2, j, 0)), -capv, capv);
+                y1 += clipd(c.ratio * (y1 - YB(c.src2, j, 1)), -CAP_N, CAP_N);
+                y2 += clipd(c.ratio * (y2 - YB(c.src2, j, 2)), -CAP_N, CAP_N);
+            }
+            YB(c.work, j, 0) = y0; YB(c.work, j, 1) = y1; YB(c.work, j, 2) = y2;
+        } else {
+            y0 = YB(c.work, j, 0); y1 = YB(c.work, j, 1); y2 = YB(c.work, j, 2);
+        }
+        q.V = y0; q.ln_n = y1; q.ln_p = y2;
+        q.G = PT(j, PT_G);
+        q.fix = FIX(j);
+    }
+    q.n = exp(q.ln_n);
+    q.p = exp(q.ln_p);
+    q.an = q.am != 0.0 ? q.am * exp(c.logs[0] + q.V / Vt) : 0.0;
+    q.ca = q.cm != 0.0 ? q.cm * exp(c.logs[1] - q.V / Vt) : 0.0;
+    if (need_rates && j >= 1 && j <= NP) {
+        q.R = recombination(q.n, q.p, s.v[SEG_KD], s.v[SEG_NI2], s.v[SEG_NT], s.v[SEG_CN],
+                            s.v[SEG_CP], s.v[SEG_NT0], s.v[SEG_PT0], q.dRn, q.dRp);
+    } else {
+        q.R = q.dRn = q.dRp = 0.0;
+    }
+}
+
+template <typename T>
+DDH_FN void to_t(const double *in, T *out, int m) {
+#pragma unroll
+    for (int q = 0; q < m; ++q) out[q] = (T)in[q];
+}
+
+// Elimination sweep (right to left). Returns the ion step z through c/out.
+template <typename T, int K>
+DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) {
+    const int NP = a.NP;
+    const double Vt = SC(SC_VT), L = SC(SC_L), L2 = L * L;
+    // the ions' totals (scaled in the start's stages); none once they are frozen (the slots
+    // are then held, src/jaxdd/batch.py:859-872)
+    const double tot[2] = {c.mobile ? SC(SC_AT) * c.share : 0.0, c.mobile ? SC(SC_CT) * c.share : 0.0};
+    T *fac = (T *)a.fac;
+#define FAC(j, f) fac[((size_t)((j) - 1) * NFAC + (f)) * a.n + lane]
+    Seg sg; sg.id = -1;
+    Pt right, mid, cur;
+    seg_at(a, lane, NP + 1, sg);
+    load_point(a, lane, NP + 1, sg, c, mid, Vt, false);
+    Edge eR = {0, 0, 0, 0, 0, 0, 0, 0};
+    right = mid;  // unused until two points are in
+    T cp[9], dp[3 * K];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) cp[q] = T(0);
+#pragma unroll
+    for (int q = 0; q < 3 * K; ++q) dp[q] = T(0);
+    double gam[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    double accF[2] = {0, 0}, accU[2][2] = {{0, 0}, {0, 0}};
+    double num[2] = {0, 0};
+    for (int j = NP; j >= 0; --j) {
+        seg_at(a, lane, j, sg);
+        load_point(a, lane, j, sg, c, cur, Vt, true);
+        Edge eL = edge_flux(cur, mid, Vt, a.bern);
+        if constexpr (K > 1) {
+            double hLj = cur.h * L;
+            num[0] += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
+            num[1] += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
+        }
+        if (j + 1 <= NP) {
+            // row i = j + 1: left = cur, mid = mid, right = right, edges eL (i-1,i), eR (i,i+1)
+            const int i = j + 1;
+            double hm = cur.h, hp = mid.h, hs = hp + hm;
+            double fac_ = 0.5 * L2 * hp * hm * hs;
+            double fac_m = (mid.eps - cur.eps) / (hm * hm * L2) -
+                           2 * mid.eps * (1 / (hp * hs * L2) + 1 / (hm * hs * L2));
+            double fac_l = 2 * mid.eps / (hm * hs * L2) - (mid.eps - cur.eps) / (hm * hm * L2);
+            double fac_u = 2 * mid.eps / (hp * hs * L2);
+            double sP = 1.0 / (Q_E * mid.Nc);
+            double sC = 1.0 / (mid.mumax * Vt * mid.Nc * hp);
+            double gen = mid.G * c.g_frac;
+            double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
+            double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
+            double pois = fac_l * cur.V + fac_m * mid.V + fac_u * right.V -
+                          Q_E * (mid.n - mid.p + mid.dop + c.share * mid.fix + mid.an - mid.ca);
+            double A[9], B[9], C[9], r[3 * K];
+            A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
+            A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
+            A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
+            B[0] = (fac_m - Q_E * (mid.an + mid.ca) / Vt) * sP;
+            B[1] = -Q_E * mid.n * sP;
+            B[2] = Q_E * mid.p * sP;
+            B[3] = (-hp * eL.gn - hm * eR.gn) * sC;
+            B[4] = (-hp * eL.an1 + hm * eR.an0 - fac_ * mid.dRn) * sC;
+            B[5] = (-fac_ * mid.dRp) * sC;
+            B[6] = (hp * eL.gp + hm * eR.gp) * sC;
+            B[7] = (-fac_ * mid.dRn) * sC;
+            B[8] = (hp * eL.ap1 - hm * eR.ap0 - fac_ * mid.dRp) * sC;
+            if (i == NP) {  // the coupling to the contact is not an unknown
+                C[0] = C[1] = C[2] = C[3] = C[4] = C[5] = C[6] = C[7] = C[8] = 0.0;
+            } else {
+                C[0] = fac_u * sP; C[1] = 0.0; C[2] = 0.0;
+                C[3] = hm * eR.gn * sC; C[4] = hm * eR.an1 * sC; C[5] = 0.0;
+                C[6] = -hm * eR.gp * sC; C[7] = 0.0; C[8] = -hm * eR.ap1 * sC;
+            }
+            r[0 * K] = -pois * sP; r[1 * K] = -elec * sC; r[2 * K] = -hole * sC;
+            if constexpr (K > 1) {
+                r[0 * K + 1] = -Q_E * mid.an * sP; r[0 * K + 2] = Q_E * mid.ca * sP;
+                r[1 * K + 1] = r[1 * K + 2] = r[2 * K + 1] = r[2 * K + 2] = 0.0;
+            }
+            if (sizeof(T) == 4) {
+                // row equilibration by powers of two (exact) before the rounding to FP32
+#pragma unroll
+                for (int row = 0; row < 3; ++row) {
+                    double mx = 0.0;
+#pragma unroll
+                    for (int col = 0; col < 3; ++col) {
+                        mx = fmax(mx, fabs(A[3 * row + col]));
+                        mx = fmax(mx, fabs(B[3 * row + col]));
+                        mx = fmax(mx, fabs(C[3 * row + col]));
+                    }
+                    if (mx > 0 && mx < INFINITY) {
+                        int e;
+                        frexp(mx, &e);
+                        double s = ldexp(1.0, -e);
+#pragma unroll
+                        for (int col = 0; col < 3; ++col) {
+                            A[3 * row + col] *= s; B[3 * row + col] *= s; C[3 * row + col] *= s;
+                        }
+#pragma unroll
+                        for (int q = 0; q < K; ++q) r[row * K + q] *= s;
+                    }
+                }
+            }
+            T At[9], Bt[9], Ct[9], rt[3 * K];
+            to_t(A, At, 9); to_t(B, Bt, 9); to_t(C, Ct, 9); to_t(r, rt, 3 * K);
+            // M = B - C c'_prev ; rhs = r - C d'_prev
+            T M[9], rh[3 * K];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    M[3 * p + q] = Bt[3 * p + q] - (Ct[3 * p + 0] * cp[0 + q] + Ct[3 * p + 1] * cp[3 + q] + Ct[3 * p + 2] * cp[6 + q]);
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    rh[p * K + q] = rt[p * K + q] - (Ct[3 * p + 0] * dp[0 * K + q] + Ct[3 * p + 1] * dp[1 * K + q] + Ct[3 * p + 2] * dp[2 * K + q]);
+            }
+            T Mi[9];
+            inverse3(M, Mi);
+            T cn_[9], dn_[3 * K];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    cn_[3 * p + q] = Mi[3 * p + 0] * At[0 + q] + Mi[3 * p + 1] * At[3 + q] + Mi[3 * p + 2] * At[6 + q];
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    dn_[p * K + q] = Mi[3 * p + 0] * rh[0 * K + q] + Mi[3 * p + 1] * rh[1 * K + q] + Mi[3 * p + 2] * rh[2 * K + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) FAC(i, q) = cn_[q];
+#pragma unroll
+            for (int q = 0; q < 3 * K; ++q) FAC(i, 9 + q) = dn_[q];
+            if constexpr (K > 1) {
+                // gamma_i = W_i - c'_{i+1}^T gamma_{i+1}; W_i only in the potential's row
+                double hLm = cur.h * L, hLp = mid.h * L;
+                double w[2] = {0.0, 0.0};
+                if (tot[0] > 0)
+                    w[0] = 0.5 * (mid.am * cur.am * hLm + right.am * mid.am * hLp) * mid.an * (1.0 / Vt) / tot[0];
+                if (tot[1] > 0)
+                    w[1] = 0.5 * (mid.cm * cur.cm * hLm + right.cm * mid.cm * hLp) * mid.ca * (-1.0 / Vt) / tot[1];
+                double ng[3][2];
+#pragma unroll
+                for (int row = 0; row < 3; ++row)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        ng[row][k] = (row == 0 ? w[k] : 0.0) -
+                                     ((double)cp[0 + row] * gam[0][k] + (double)cp[3 + row] * gam[1][k] + (double)cp[6 + row] * gam[2][k]);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double sF = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                    for (int row = 0; row < 3; ++row) {
+                        gam[row][k] = ng[row][k];
+                        sF += ng[row][k] * (double)dn_[row * K + 0];
+                        s1 += ng[row][k] * (double)dn_[row * K + 1];
+                        s2 += ng[row][k] * (double)dn_[row * K + 2];
+                    }
+                    accF[k] += sF; accU[k][0] += s1; accU[k][1] += s2;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) cp[q] = cn_[q];
+#pragma unroll
+            for (int q = 0; q < 3 * K; ++q) dp[q] = dn_[q];
+        }
+        right = mid;
+        mid = cur;
+        eR = eL;
+    }
+#undef FAC
+    z[0] = z[1] = 0.0;
+    if constexpr (K > 1) {
+        double g[2], D[2];
+        for (int k = 0; k < 2; ++k) {
+            bool has = tot[k] > 0;
+            g[k] = has ? num[k] / tot[k] - 1.0 : 0.0;
+            D[k] = has ? num[k] / tot[k] : 1.0;  // a slot without ions is held
+        }
+        z[0] = -g[0] - accF[0];
+        z[1] = -g[1] - accF[1];
+        solve2(D[0] - accU[0][0], -accU[0][1], -accU[1][0], D[1] - accU[1][1], z[0], z[1]);
+    }
+}
+
+// Back-substitution sweep (left to right): the step, the new state, its size and current.
+template <typename T, int K>
+DDH_FN void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const double z[2],
+                              double &size_out, double &J_out, int &finite_out) {
+    const int NP = a.NP;
+    const double Vt = SC(SC_VT), nfl = SC(SC_NFLOOR), lnfloor = log(SC(SC_NFLOOR));
+    const double capv = CAP_V * Vt;
+    T *fac = (T *)a.fac;
+#define FAC(j, f) fac[((size_t)((j) - 1) * NFAC + (f)) * a.n + lane]
+    Seg sg; sg.id = -1;
+    seg_at(a, lane, 0, sg);
+    // the left contact
+    double V0 = contact_V(a, lane, c.Vint, false);
+    double Vg0n = V0 + sg.v[SEG_OFFN], Vg0p = V0 + sg.v[SEG_OFFP];
+    double wn_prev = exp(-Vg0n / Vt) / sg.v[SEG_MUN];
+    double wp_prev = exp(Vg0p / Vt) / sg.v[SEG_MUP];
+    double endn0 = exp(SC(SC_LNL)) * exp(-Vg0n / Vt);
+    double endp0 = exp(SC(SC_LPL)) * exp(Vg0p / Vt);
+    double U_prev = 0.0, iu_prev = 0.0, x_prev = PT(0, PT_X);
+    double single_n = 0.0, double_n = 0.0, single_p = 0.0, double_p = 0.0;
+    double xs[3] = {0.0, 0.0, 0.0};
+    double size = 0.0;
+    int finite = 1;
+    for (int j = 1; j <= NP; ++j) {
+        seg_at(a, lane, j, sg);
+        double dF[3];
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            double v = (double)FAC(j, 9 + row * K + 0);
+            if constexpr (K > 1) v -= (double)FAC(j, 9 + row * K + 1) * z[0] + (double)FAC(j, 9 + row * K + 2) * z[1];
+            dF[row] = v;
+        }
+        double xn[3];
+#pragma unroll
+        for (int row = 0; row < 3; ++row)
+            xn[row] = dF[row] - ((double)FAC(j, 3 * row + 0) * xs[0] + (double)FAC(j, 3 * row + 1) * xs[1] +
+                                 (double)FAC(j, 3 * row + 2) * xs[2]);
+        xs[0] = xn[0]; xs[1] = xn[1]; xs[2] = xn[2];
+        double y0 = YB(c.work, j, 0), y1 = YB(c.work, j, 1), y2 = YB(c.work, j, 2);
+        double st0 = clipd(xn[0], -capv, capv);
+        double st[2];
+        double yo[2] = {y1, y2};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            double d = xn[1 + q];
+            st[q] = clipd(d, -CAP_N, CAP_N);
+            if (yo[q] < lnfloor) {  // density_steps (src/jaxdd/solver.py:74-88)
+                double in_n = d > -1 ? log1p(d) : -CAP_N;
+                st[q] = clipd(in_n, -FLOOR_DROP_CAP, CAP_N);
+            }
+        }
+        y0 += st0; y1 += st[0]; y2 += st[1];
+        YB(c.work, j, 0) = y0; YB(c.work, j, 1) = y1; YB(c.work, j, 2) = y2;
+        if (!(isfinite(y0) && isfinite(y1) && isfinite(y2))) finite = 0;
+        double n = exp(y1), p = exp(y2);
+        size = nmax(size, fabs(st0) / Vt);
+        size = nmax(size, fabs(st[0]) * (n / (n + nfl)));
+        size = nmax(size, fabs(st[1]) * (p / (p + nfl)));
+        // the integral current of the new state (src/jaxdd/batch.py:321-373)
+        double R = recombination_value(n, p, sg.v[SEG_KD], sg.v[SEG_NI2], sg.v[SEG_NT],
+                                       sg.v[SEG_CN], sg.v[SEG_CP], sg.v[SEG_NT0], sg.v[SEG_PT0]);
+        double U = R - PT(j, PT_G) * c.g_frac;
+        double wn = exp(-(y0 + sg.v[SEG_OFFN]) / Vt) / sg.v[SEG_MUN];
+        double wp = exp((y0 + sg.v[SEG_OFFP]) / Vt) / sg.v[SEG_MUP];
+        double xj = PT(j, PT_X);
+        double dx = xj - x_prev;
+        double iu = iu_prev + 0.5 * dx * (U_prev + U);
+        single_n += 0.5 * dx * (wn_prev + wn);
+        double_n += 0.5 * dx * (wn_prev * iu_prev + wn * iu);
+        single_p += 0.5 * dx * (wp_prev + wp);
+        double_p += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
+        wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
+    }
+    seg_at(a, lane, NP + 1, sg);
+    double VN = contact_V(a, lane, c.Vint, true);
+    double endnN = exp(SC(SC_LNR)) * exp(-(VN + sg.v[SEG_OFFN]) / Vt);
+    double endpN = exp(SC(SC_LPR)) * exp((VN + sg.v[SEG_OFFP]) / Vt);
+    // K = (V_T (sn dens_N ev_N - sn dens_0 ev_0) - sn double) / single, sn = -1 electrons
+    double Kn = (Vt * (-endnN + endn0) + double_n) / single_n;
+    double Kp = (Vt * (endpN - endp0) - double_p) / single_p;
+    // The cumulative recombination terms of the two carriers cancel exactly in the sum (the
+    // same U with opposite signs), so the device average is q (K_n + K_p), sign turned with
+    // the anode on the left.
+    J_out = -SC(SC_ELEC) * Q_E * (Kn + Kp);
+    size_out = size;
+    finite_out = finite;
+#undef FAC
+}
+
+// ---------------------------------------------------------------------------------------
+// The per-lane state machine: src/jaxdd/batch.py:875-1006 (_start, _solve_one) and
+// src/jaxdd/solver.py:2105-2245 (_scan of solve_jv).
+// ---------------------------------------------------------------------------------------
+DDH_CONST double ION_STAGES[4] = {1.0 / 64, 1.0 / 16, 1.0 / 4, 1.0};
+
+DDH_FN void begin_solve(const Args &a, LaneState &S, double Vint, double share,
+                                   int maxit, int src1, int src2, double ratio, const double lg[2],
+                                   int work) {
+    S.Vint = Vint; S.share = share; S.maxit = maxit;
+    S.b_src1 = src1; S.b_src2 = src2; S.ratio = ratio; S.b_work = work;
+    S.lw[0] = lg[0]; S.lw[1] = lg[1];
+    S.it = 0; S.fresh = 1; S.need64 = 0;
+    for (int k = 0; k < 4; ++k) { S.hist[k] = INFINITY; S.Jh[k] = INFINITY; }
+    S.noise_state = 0;
+}
+
+DDH_FN int free_buffer(const LaneState &S, int a0, int a1, int a2) {
+    for (int b = 0; b < NBUF; ++b)
+        if (b != a0 && b != a1 && b != a2) return b;
+    return NBUF - 1;
+}
+
+DDH_FN void begin_scan_voltage(const Args &a, int lane, LaneState &S) {
+    const int nV = a.nV;
+    int j = S.j;
+    int idx;
+    double ratio = 0.0;
+    if (a.jv) {
+        idx = j;
+    } else {
+        idx = j < nV - S.below ? S.below + j : nV - 1 - j;
+        if (j == nV - S.below) {  // the turn: from the pre-bias solution again
+            S.b_last = S.b_before = S.b_start;
+            S.llast[0] = S.lbefore[0] = S.lstart[0];
+            S.llast[1] = S.lbefore[1] = S.lstart[1];
+            S.V_last = S.V_before = S.first;
+        }
+        double Vi = a.V[idx];
+        ratio = S.V_last != S.V_before ? (Vi - S.V_last) / (S.V_last - S.V_before) : 0.0;
+    }
+    double Vi = a.V[idx];
+    double lg[2];
+    for (int k = 0; k < 2; ++k)
+        lg[k] = S.llast[k] + clipd(ratio * (S.llast[k] - S.lbefore[k]), -ION_LOG_CAP, ION_LOG_CAP);
+    int work = free_buffer(S, S.b_last, S.b_before, S.b_start);
+    begin_solve(a, S, Vi, 1.0, a.max_iter, S.b_last, S.b_before, ratio, lg, work);
+}
+
+DDH_FN void scan_setup(const Args &a, int lane, LaneState &S, int y_buf,
+                                  const double ly[2]) {
+    S.b_start = y_buf;
+    S.lstart[0] = ly[0]; S.lstart[1] = ly[1];
+    bool frozen = SC(SC_PRE) > 0 && SC(SC_FIX) > 0;
+    S.freeze_now = frozen ? 1 : 0;
+    S.mobile_now = frozen ? 0 : S.mobile0;
+    if (!a.jv) S.pre_ok = S.pre_ok || !frozen;
+    int below = 0;
+    if (!a.jv)
+        while (below < a.nV && a.V[below] < S.first) ++below;
+    S.below = below;
+    S.j = 0;
+    S.b_last = S.b_before = y_buf;
+    S.llast[0] = S.lbefore[0] = ly[0];
+    S.llast[1] = S.lbefore[1] = ly[1];
+    S.V_last = S.V_before = S.first;
+    S.phase = PH_SCAN;
+    begin_scan_voltage(a, lane, S);
+}
+
+// Where the start stands after the F solve or the stages: the pre-bias solve, or straight to
+// the scan where the start is already the solution.
+DDH_FN void to_pre(const Args &a, int lane, LaneState &S, int y_buf, const double ly[2],
+                              int solved) {
+    if (solved) {
+        S.pre_ok = 1;
+        scan_setup(a, lane, S, y_buf, ly);
+        return;
+    }
+    S.phase = PH_PRE;
+    int work = free_buffer(S, y_buf, -1, -1);
+    begin_solve(a, S, S.first, 1.0, a.jv ? a.max_iter : a.start_iter, y_buf, y_buf, 0.0, ly, work);
+}
+
+DDH_FN void begin_stage(const Args &a, LaneState &S) {
+    double share = ION_STAGES[S.stage];
+    double sh = log(share / S.before_share);
+    double lg[2] = {S.lc[0] + sh, S.lc[1] + sh};
+    S.lc[0] = lg[0]; S.lc[1] = lg[1];
+    int work = (S.b_c == 2) ? 3 : 2;
+    S.phase = PH_STAGE;
+    begin_solve(a, S, S.first, share, a.jv ? a.max_iter : a.start_iter, S.b_c, S.b_c, 0.0, lg, work);
+}
+
+// The lane's first solve after Poisson's start.
+DDH_FN void start_lane(const Args &a, int lane, LaneState &S) {
+    S.err = 0;
+    S.mobile_now = S.mobile0;
+    S.freeze_now = 0;
+    S.pre_ok = 1;
+    bool pre = SC(SC_PRE) > 0;
+    if (a.jv && !S.mobile0) {
+        // solve_jv: without mobile ions no solve at the first voltage (solver.py:2211-2226)
+        to_pre(a, lane, S, S.b_pois, S.lpois, pre ? 0 : 1);
+        return;
+    }
+    S.phase = PH_F;
+    begin_solve(a, S, S.first, 1.0, a.jv ? a.max_iter : a.start_iter, S.b_pois, S.b_pois, 0.0,
+                S.lpois, 1);
+}
+
+// One finished solve of the lane: the next solve, or the end.
+DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
+    const int fin = S.fin_work && isfinite(S.lw[0]) && isfinite(S.lw[1]);
+    switch (S.phase) {
+    case PH_F: {
+        S.ok_f = ok;
+        S.fin_f = fin;
+        S.b_f = S.b_work;
+        S.lf[0] = S.lw[0]; S.lf[1] = S.lw[1];
+        if (a.jv) {
+            if (ok) { to_pre(a, lane, S, S.b_f, S.lf, SC(SC_PRE) > 0 ? 0 : 1); return; }
+            S.stage = 0; S.b_c = S.b_pois; S.lc[0] = S.lpois[0]; S.lc[1] = S.lpois[1];
+            S.before_share = 1.0;
+            begin_stage(a, S);
+            return;
+        }
+        bool done = ok || !S.mobile0;
+        if (done) {
+            int yb = fin ? S.b_f : S.b_pois;
+            const double *ly = fin ? S.lf : S.lpois;
+            to_pre(a, lane, S, yb, ly, ok);
+            return;
+        }
+        S.stage = 0; S.b_c = S.b_pois; S.lc[0] = S.lpois[0]; S.lc[1] = S.lpois[1];
+        S.before_share = 1.0;
+        begin_stage(a, S);
+        return;
+    }
+    case PH_STAGE: {
+        if (a.jv) {
+            // solve_jv: each stage from the last, stopping at the first failure
+            S.b_c = S.b_work; S.lc[0] = S.lw[0]; S.lc[1] = S.lw[1];
+            S.before_share = ION_STAGES[S.stage];
+            S.ok_c = ok;
+            S.stage++;
+            if (ok && S.stage < 4) { begin_stage(a, S); return; }
+            if (ok) { to_pre(a, lane, S, S.b_c, S.lc, SC(SC_PRE) > 0 ? 0 : 1); return; }
+            to_pre(a, lane, S, S.b_pois, S.lpois, SC(SC_PRE) > 0 ? 0 : 1);
+            return;
+        }
+        if (fin) { S.b_c = S.b_work; S.lc[0] = S.lw[0]; S.lc[1] = S.lw[1]; }
+        S.ok_c = ok;
+        S.before_share = ION_STAGES[S.stage];
+        S.stage++;
+        if (S.stage < 4) { begin_stage(a, S); return; }
+        int staged = S.ok_c;
+        int yb; const double *ly;
+        if (staged) { yb = S.b_c; ly = S.lc; }
+        else if (S.fin_f) { yb = S.b_f; ly = S.lf; }
+        else { yb = S.b_pois; ly = S.lpois; }
+        to_pre(a, lane, S, yb, ly, staged);
+        return;
+    }
+    case PH_PRE: {
+        if (a.jv) {
+            if (!ok) {  // the reference raises RuntimeError at a failed pre-bias
+                S.err = 1;
+                a.status[lane] = 1;
+                S.phase = PH_DONE;
+                return;
+            }
+            S.pre_ok = 1;
+        } else {
+            S.pre_ok = ok;
+        }
+        scan_setup(a, lane, S, S.b_work, S.lw);
+        return;
+    }
+    case PH_SCAN: {
+        int idx = a.jv ? S.j : (S.j < a.nV - S.below ? S.below + S.j : a.nV - 1 - S.j);
+        size_t o = (size_t)lane * a.nV + idx;
+        a.J[o] = J;
+        a.ok[o] = (uint8_t)((ok && S.pre_ok) ? 1 : 0);
+        a.its[o] = S.it;
+        if (ok && fin) {  // only a converged, finite solution is carried on
+            S.b_before = S.b_last; S.b_last = S.b_work;
+            S.lbefore[0] = S.llast[0]; S.lbefore[1] = S.llast[1];
+            S.llast[0] = S.lw[0]; S.llast[1] = S.lw[1];
+            S.V_before = S.V_last; S.V_last = S.Vint;
+        }
+        S.freeze_now = 0;
+        S.j++;
+        if (S.j >= a.nV) { S.phase = PH_DONE; return; }
+        begin_scan_voltage(a, lane, S);
+        return;
+    }
+    default:
+        S.phase = PH_DONE;
+    }
+}
+
+// After a Newton step: the histories and the stop rules (src/jaxdd/batch.py:793-839; with
+// jv, src/jaxdd/solver.py:2150-2166 without the noise floor's extra solve).
+DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, double J) {
+    for (int k = 0; k < 3; ++k) { S.hist[k] = S.hist[k + 1]; S.Jh[k] = S.Jh[k + 1]; }
+    S.hist[3] = size;
+    S.Jh[3] = J;
+    S.it++;
+    bool anynan = false;
+    double hmax = 0.0, hmin_tail = INFINITY;
+    for (int k = 0; k < 4; ++k) { anynan |= S.hist[k] != S.hist[k]; hmax = fmax(hmax, S.hist[k]); }
+    for (int k = 1; k < 4; ++k) hmin_tail = fmin(hmin_tail, S.hist[k]);
+    bool stalled = S.it > STALL_STEPS && !anynan && hmax < ROUND_OFF_FLOOR && hmin_tail >= 0.1 * S.hist[0];
+    bool done;
+    if (a.jv || a.strict) {
+        done = size < a.tol || stalled;
+    } else {
+        double Jn = S.Jh[3], scale = SC(SC_JS) + fabs(Jn), jdev = 0.0;
+        bool jnan = false;
+        for (int k = 0; k < 4; ++k) {
+            double v = fabs(S.Jh[k] - Jn);
+            jnan |= v != v;
+            jdev = fmax(jdev, v);
+        }
+        bool settled = S.it > STALL_STEPS && size < CURRENT_STEP_CAP && !jnan && jdev < CURRENT_TOL * scale;
+        bool quadratic = S.it > 1 && size < QUADRATIC_TOL && size < QUADRATIC_RATIO * S.hist[2];
+        done = size < a.tol || stalled || settled || quadratic;
+    }
+    if (done) { end_solve(a, lane, S, 1, J); return; }
+    if (S.it >= S.maxit || size != size) { end_solve(a, lane, S, 0, J); return; }
+    S.fresh = 0;
+    if (a.mixed && (S.it >= MIXED_STEPS || size < MIXED_SWITCH)) S.need64 = 1;
+}
+
+// One Newton step of one lane (both sweeps and the state machine). `ions`: some lane of
+// the wavefront has mobile ions (the sweeps then carry the border's right-hand sides).
+template <typename T>
+DDH_FN void newton_lane_step(const Args &a, int lane, bool ions) {
+    LaneState &S = a.st[lane];
+    SweepCtl c;
+    c.Vint = S.Vint; c.share = S.share; c.ratio = S.ratio; c.g_frac = SC(SC_GFRAC);
+    c.logs[0] = S.lw[0]; c.logs[1] = S.lw[1];
+    c.lstart[0] = S.lstart[0]; c.lstart[1] = S.lstart[1];
+    c.fresh = S.fresh; c.freeze_now = S.freeze_now && S.fresh; c.mobile = S.mobile_now;
+    c.src1 = S.b_src1; c.src2 = S.b_src2; c.work = S.b_work;
+    double z[2];
+    if (ions) elim_sweep<T, 3>(a, lane, c, z);
+    else elim_sweep<T, 1>(a, lane, c, z);
+    double size, J;
+    int fin;
+    c.fresh = 0;
+    if (ions) backsub_sweep<T, 3>(a, lane, c, z, size, J, fin);
+    else backsub_sweep<T, 1>(a, lane, c, z, size, J, fin);
+    double dl0 = clipd(z[0], -ION_LOG_CAP, ION_LOG_CAP);
+    double dl1 = clipd(z[1], -ION_LOG_CAP, ION_LOG_CAP);
+    S.lw[0] += dl0;
+    S.lw[1] += dl1;
+    size = nmax(size, nmax(fabs(dl0), fabs(dl1)));
+    S.fin_work = fin;
+    after_step(a, lane, S, size, J);
+}
+
+DDH_FN bool lane_active(const Args &a, int lane) { return a.st[lane].phase != PH_DONE; }
+DDH_FN bool lane_mobile(const Args &a, int lane) { return a.st[lane].mobile_now != 0; }
+DDH_FN bool lane_needs64(const Args &a, int lane) { return a.st[lane].need64 != 0; }
+
+}  // namespace ddh

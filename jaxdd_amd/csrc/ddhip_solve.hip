@@ -1,0 +1,267 @@
+// HIP kernels of the batched steady-state J-V solve (gfx950) and the C-ABI of libddhip.so.
+// The per-lane logic is in ddhip_core.h; here the launches: Poisson's start (one lane per
+// device), the persistent Newton kernel (one lane per device, each wavefront looping until its
+// last lane has finished its scan), and the block-tridiagonal sweep operator.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "ddhip_core.h"
+
+namespace ddh {
+
+__global__ void __launch_bounds__(64) poisson_kernel(Args a) {
+    const int lane = blockIdx.x * 64 + threadIdx.x;
+    if (lane < a.n) poisson_lane(a, lane);
+}
+
+__global__ void __launch_bounds__(64) newton_kernel(Args a) {
+    const int lane = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = lane < a.n;
+    if (valid) start_lane(a, lane, a.st[lane]);
+    while (true) {
+        const bool act = valid && lane_active(a, lane);
+        if (!__any(act)) break;
+        const bool ions = __any(act && lane_mobile(a, lane));
+        const bool f64 = !a.mixed || __any(act && lane_needs64(a, lane));
+        if (act) {
+            if (f64) newton_lane_step<double>(a, lane, ions);
+            else newton_lane_step<float>(a, lane, ions);
+        }
+    }
+}
+
+}  // namespace ddh
+
+// ---------------------------------------------------------------------------------------
+// The block-tridiagonal sweep operator (src/jaxdd/sweep.py:313-330): one lane per system.
+// ---------------------------------------------------------------------------------------
+namespace ddh {
+
+template <typename T, int b>
+__device__ void inverse_b(const T *m, T *out) {
+    if (b == 3) { inverse3(m, out); return; }
+    // Gauss-Jordan with partial pivoting (5x5 blocks)
+    T t[b * b], inv[b * b];
+#pragma unroll
+    for (int q = 0; q < b * b; ++q) { t[q] = m[q]; inv[q] = (q / b == q % b) ? T(1) : T(0); }
+    for (int col = 0; col < b; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < b; ++r)
+            if (fabs((double)t[r * b + col]) > fabs((double)t[piv * b + col])) piv = r;
+        if (piv != col)
+            for (int q = 0; q < b; ++q) {
+                T s = t[col * b + q]; t[col * b + q] = t[piv * b + q]; t[piv * b + q] = s;
+                s = inv[col * b + q]; inv[col * b + q] = inv[piv * b + q]; inv[piv * b + q] = s;
+            }
+        T rp = T(1) / t[col * b + col];
+        for (int q = 0; q < b; ++q) { t[col * b + q] *= rp; inv[col * b + q] *= rp; }
+        for (int r = 0; r < b; ++r) {
+            if (r == col) continue;
+            T f = t[r * b + col];
+            for (int q = 0; q < b; ++q) { t[r * b + q] -= f * t[col * b + q]; inv[r * b + q] -= f * inv[col * b + q]; }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < b * b; ++q) out[q] = inv[q];
+}
+
+template <typename T, int b>
+__global__ void __launch_bounds__(64) block_sweep_kernel(int n, int NP, int k, const T *A,
+                                                         const T *B, const T *C, T *d, T *ws) {
+    const int lane = blockIdx.x * 64 + threadIdx.x;
+    if (lane >= n) return;
+    const int bb = b * b, bk = b * k;
+    // ws: c' [NP][bb][n]; d is overwritten by d' then x
+    T cprev[b * b], dprev[b * 8];
+#pragma unroll
+    for (int q = 0; q < bb; ++q) cprev[q] = T(0);
+    for (int q = 0; q < bk; ++q) dprev[q] = T(0);
+    for (int i = 0; i < NP; ++i) {
+        T a_[b * b], m[b * b], mi[b * b], rh[b * 8];
+#pragma unroll
+        for (int q = 0; q < bb; ++q) {
+            a_[q] = A[((size_t)i * bb + q) * n + lane];
+            m[q] = B[((size_t)i * bb + q) * n + lane];
+        }
+        for (int q = 0; q < bk; ++q) rh[q] = d[((size_t)i * bk + q) * n + lane];
+        if (i > 0) {
+#pragma unroll
+            for (int p = 0; p < b; ++p) {
+#pragma unroll
+                for (int q = 0; q < b; ++q) {
+                    T s = T(0);
+#pragma unroll
+                    for (int l = 0; l < b; ++l) s += a_[p * b + l] * cprev[l * b + q];
+                    m[p * b + q] -= s;
+                }
+                for (int q = 0; q < k; ++q) {
+                    T s = T(0);
+#pragma unroll
+                    for (int l = 0; l < b; ++l) s += a_[p * b + l] * dprev[l * k + q];
+                    rh[p * k + q] -= s;
+                }
+            }
+        }
+        inverse_b<T, b>(m, mi);
+        T c_[b * b];
+#pragma unroll
+        for (int q = 0; q < bb; ++q) c_[q] = C[((size_t)i * bb + q) * n + lane];
+#pragma unroll
+        for (int p = 0; p < b; ++p) {
+#pragma unroll
+            for (int q = 0; q < b; ++q) {
+                T s = T(0);
+#pragma unroll
+                for (int l = 0; l < b; ++l) s += mi[p * b + l] * c_[l * b + q];
+                cprev[p * b + q] = s;
+                ws[((size_t)i * bb + p * b + q) * n + lane] = s;
+            }
+            for (int q = 0; q < k; ++q) {
+                T s = T(0);
+#pragma unroll
+                for (int l = 0; l < b; ++l) s += mi[p * b + l] * rh[l * k + q];
+                dprev[p * k + q] = s;
+            }
+        }
+        for (int q = 0; q < bk; ++q) d[((size_t)i * bk + q) * n + lane] = dprev[q];
+    }
+    T xn[b * 8];
+    for (int q = 0; q < bk; ++q) xn[q] = T(0);
+    for (int i = NP - 1; i >= 0; --i) {
+        T x[b * 8];
+        for (int q = 0; q < bk; ++q) x[q] = d[((size_t)i * bk + q) * n + lane];
+#pragma unroll
+        for (int p = 0; p < b; ++p)
+            for (int q = 0; q < k; ++q) {
+                T s = T(0);
+#pragma unroll
+                for (int l = 0; l < b; ++l) s += ws[((size_t)i * bb + p * b + l) * n + lane] * xn[l * k + q];
+                x[p * k + q] -= s;
+            }
+        for (int q = 0; q < bk; ++q) {
+            d[((size_t)i * bk + q) * n + lane] = x[q];
+            xn[q] = x[q];
+        }
+    }
+}
+
+}  // namespace ddh
+
+// ---------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------
+#include "ddhip.h"
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char *msg) {
+    strncpy(g_err, msg, sizeof(g_err) - 1);
+    g_err[sizeof(g_err) - 1] = 0;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *where) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+    return DDH_E_HIP;
+}
+
+extern "C" int ddh_abi_version(void) { return DDH_ABI_VERSION; }
+extern "C" const char *ddh_last_error(void) { return g_err; }
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct WsLayout {
+    size_t y, fac, fix, st, total;
+};
+
+static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision) {
+    WsLayout w;
+    (void)precision;  // a mixed solve may switch a lane to FP64 factors: room for those
+    size_t tsize = 8;
+    size_t off = 0;
+    w.y = off; off = align256(off + (size_t)ddh::NBUF * NP * 3 * n * sizeof(double));
+    size_t facb = (size_t)NP * ddh::NFAC * n * tsize;
+    size_t poisb = (size_t)NP * 4 * n * sizeof(double);
+    w.fac = off; off = align256(off + (facb > poisb ? facb : poisb));
+    w.fix = off; off = align256(off + (size_t)NP * n * sizeof(double));
+    w.st = off; off = align256(off + (size_t)n * sizeof(ddh::LaneState));
+    w.total = off;
+    return w;
+}
+
+extern "C" size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision) {
+    (void)nV;
+    if (n <= 0 || NP < 2) return 0;
+    return ws_layout(n, NP, precision).total;
+}
+
+extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
+                               const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
+                               int32_t *status, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    if (!b || !V || !o || !J || !ok || !its || !status || !workspace)
+        return fail(DDH_E_ARG, "ddh_solve_batch: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || b->nseg < 1 || b->nseg > 255)
+        return fail(DDH_E_SHAPE, "ddh_solve_batch: n > 0, 2 <= NP <= 65536, nV >= 1, 1 <= nseg <= 255");
+    if (o->precision != DDH_FP64 && o->precision != DDH_MIXED)
+        return fail(DDH_E_ARG, "ddh_solve_batch: precision must be DDH_FP64 or DDH_MIXED");
+    if (o->bernoulli != 0 && o->bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_solve_batch: bernoulli must be 0 (exact) or 1 (simss)");
+    if (o->max_iter < 1 || o->start_iter < 1)
+        return fail(DDH_E_ARG, "ddh_solve_batch: max_iter and start_iter must be >= 1");
+    WsLayout w = ws_layout(b->n, b->NP, o->precision);
+    if (workspace_bytes < w.total) return fail(DDH_E_WORKSPACE, "ddh_solve_batch: workspace too small");
+    char *base = (char *)workspace;
+    ddh::Args a;
+    a.n = b->n; a.NP = b->NP; a.nV = nV; a.nseg = b->nseg;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc; a.V = V;
+    a.Y = (double *)(base + w.y);
+    a.fac = (void *)(base + w.fac);
+    a.fix = (double *)(base + w.fix);
+    a.st = (ddh::LaneState *)(base + w.st);
+    a.J = J; a.ok = ok; a.its = its; a.status = status;
+    a.tol = o->tol; a.max_iter = o->max_iter; a.start_iter = o->start_iter;
+    a.strict = o->strict; a.bern = o->bernoulli; a.jv = o->jv_mode;
+    a.mixed = o->precision == DDH_MIXED ? 1 : 0;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((b->n + 63) / 64), block(64);
+    hipLaunchKernelGGL(ddh::poisson_kernel, grid, block, 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "poisson_kernel launch");
+    hipLaunchKernelGGL(ddh::newton_kernel, grid, block, 0, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "newton_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, const void *A,
+                               const void *B, const void *C, void *d, int32_t precision,
+                               void *workspace, void *stream) {
+    if (!A || !B || !C || !d || !workspace) return fail(DDH_E_ARG, "ddh_block_sweep: NULL argument");
+    if (n <= 0 || NP < 1 || (b != 3 && b != 5) || k < 1 || k > 8)
+        return fail(DDH_E_SHAPE, "ddh_block_sweep: n > 0, NP >= 1, b in {3, 5}, 1 <= k <= 8");
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((n + 63) / 64), block(64);
+    if (precision == DDH_FP64) {
+        if (b == 3)
+            hipLaunchKernelGGL((ddh::block_sweep_kernel<double, 3>), grid, block, 0, s, n, NP, k,
+                               (const double *)A, (const double *)B, (const double *)C, (double *)d, (double *)workspace);
+        else
+            hipLaunchKernelGGL((ddh::block_sweep_kernel<double, 5>), grid, block, 0, s, n, NP, k,
+                               (const double *)A, (const double *)B, (const double *)C, (double *)d, (double *)workspace);
+    } else if (precision == DDH_MIXED) {
+        if (b == 3)
+            hipLaunchKernelGGL((ddh::block_sweep_kernel<float, 3>), grid, block, 0, s, n, NP, k,
+                               (const float *)A, (const float *)B, (const float *)C, (float *)d, (float *)workspace);
+        else
+            hipLaunchKernelGGL((ddh::block_sweep_kernel<float, 5>), grid, block, 0, s, n, NP, k,
+                               (const float *)A, (const float *)B, (const float *)C, (float *)d, (float *)workspace);
+    } else {
+        return fail(DDH_E_ARG, "ddh_block_sweep: precision must be DDH_FP64 or DDH_MIXED");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "block_sweep_kernel launch");
+    return DDH_OK;
+}

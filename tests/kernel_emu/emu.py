@@ -1,0 +1,57 @@
+"""ctypes binding of the test-only CPU build of the kernels' per-lane logic (emu.cpp).
+
+It runs the same per-lane code as the HIP kernels on packed host arrays (jaxdd_amd.batch.pack),
+so tests can compare the kernels' algorithm with the oracle without a GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libddemu.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(str(LIB))
+        vp, i = ctypes.c_void_p, ctypes.c_int
+        _lib.emu_solve_batch.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, ctypes.c_double, i, i,
+                                         i, i, i, i, vp, vp, vp, vp]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=False,
+                 bernoulli="exact", precision="fp64", jv=False):
+    """(J without the shunt, ok, its, status) for packed devices at voltages V (ascending in
+    batch mode, the given order in jv mode)."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    nV = len(V)
+    J = np.zeros((n, nV))
+    ok = np.zeros((n, nV), dtype=np.uint8)
+    its = np.zeros((n, nV), dtype=np.int32)
+    status = np.zeros(n, dtype=np.int32)
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    lib().emu_solve_batch(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV, tol,
+                          max_iter, start_iter or max_iter, int(strict),
+                          0 if bernoulli == "exact" else 1, 0 if precision == "fp64" else 1,
+                          int(jv), _p(J), _p(ok), _p(its), _p(status))
+    return J, ok.astype(bool), its, status
