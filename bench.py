@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Throughput of the batched steady-state J-V solve on one MI355X.
+
+A step is one `solve_batch` of the whole synthetic workload: 16384 devices (NP = 400) with
+mobile anions and cations in the perovskite, free at every voltage, 61 voltages from -0.2 to
+1.3 V, in mixed precision (FP32 block factorisation on row-equilibrated blocks while a solve is
+far from its solution, FP64 residual, exponentials, current and stop rules). The all-FP64
+form is timed beside it, and the mixed curves are checked against it (1e-6 of each curve's
+scale). One JSON line on stdout:
+
+    python bench.py [--steps K] [--warmup W] [--n 16384] [--precision mixed|fp64]
+
+``roofline``: the solve kernel's algorithmic HBM bytes (per needed Newton step per device
+B_step = NP*(16b + 8s + 2w(b^2 + bk)), b = 3, s = 22, k = 3 with the ion border, w the
+factor's bytes) over its duration from HIP events on its launch stream, against 8 TB/s.
+``cpu_baseline``: the FP64 CPU oracle (a C restatement of the reference's algorithm) timed
+on this host, one pinned single-threaded process per core, each on its own devices of the
+same workload, as the reference times its CPU path.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "J-V curves/s, 16384 devices NP=400, mixed vs FP64 at <=1e-6 J-scale error"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+SEGMENT_COEFFS = 22  # s: per-point coefficients of the reference's stencils
+CHAOTIC_STEPS = 20
+
+
+def b_step(NP: int, k: int, w: int, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
+    """Algorithmic bytes per Newton step per device: state read+write, coefficients once,
+    factors written by the elimination and read by the back-substitution."""
+    return NP * (16 * b + 8 * s + 2 * w * (b * b + b * k))
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--config", default="c3", choices=["c2", "c3"])
+    p.add_argument("--variant", default="free", choices=["free", "frozen"])
+    p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    p.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
+    p.add_argument("--workers", type=int, default=0, help="host processes building devices")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only copy MIB MiB device to device (HBM counter calibration)")
+    return p.parse_args()
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+# ---- CPU baseline: the oracle, one pinned single-threaded process per core -----------------
+def _cpu_worker(core, rows, config, variant, start_evt, ready, out):
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    try:
+        os.sched_setaffinity(0, {core})
+    except (AttributeError, OSError):
+        pass
+    from jaxdd_amd import synthetic as S
+    from oracle import ddoracle as O
+
+    O.lib()
+    t = S.draw(rows[1], S.SEED)
+    ions = config == "c3"
+    devs = [S.device_from_row(t, k, 400, ions, ions and variant == "frozen")
+            for k in range(*rows)]
+    O.solve_batch(devs[:1], S.VOLTAGES)  # warm-up
+    ready.put(core)
+    start_evt.wait()
+    t0 = time.perf_counter()
+    J, ok, _ = O.solve_batch(devs, S.VOLTAGES)
+    t1 = time.perf_counter()
+    out.put((t0, t1, len(devs), float(ok.mean())))
+
+
+def cpu_baseline(config, variant, per_proc):
+    import multiprocessing as mp
+
+    ctx = mp.get_context("fork")
+    cores = host_cores()
+    try:
+        core_ids = sorted(os.sched_getaffinity(0))[:cores]
+    except AttributeError:
+        core_ids = list(range(cores))
+    start, ready, out = ctx.Event(), ctx.Queue(), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker,
+                         args=(c, (i * per_proc, (i + 1) * per_proc), config, variant, start,
+                               ready, out))
+             for i, c in enumerate(core_ids)]
+    for p in procs:
+        p.start()
+    for _ in procs:  # every worker has built its devices and warmed up
+        ready.get(timeout=600)
+    start.set()
+    res = [out.get(timeout=600) for _ in procs]
+    for p in procs:
+        p.join()
+    t0 = min(r[0] for r in res)
+    t1 = max(r[1] for r in res)
+    curves = sum(r[2] for r in res)
+    return {"value": curves / (t1 - t0), "unit": "J-V curves/s", "cores": len(procs),
+            "kind": "port",
+            "sample": f"{curves} curves ({per_proc} devices per process) of the same workload, "
+                      f"FP64 C oracle with the reference's two passes, {len(procs)} pinned "
+                      "single-threaded processes",
+            "per_core": curves / (t1 - t0) / len(procs), "converged": float(np.mean([r[3] for r in res]))}
+
+
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of known size, for scaling rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch of the solve kernel from the committed rocprofv3 counter summary
+    (profiles/hbm_traffic.json), when it was measured on this workload."""
+    f = ROOT / "profiles" / "hbm_traffic.json"
+    if not f.exists():
+        return None
+    d = json.loads(f.read_text())
+    return d.get("bytes_per_launch") if d.get("workload") == workload else None
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the devices of a batch are independent, a batch "
+              "splits across GPUs with no communication (DESIGN.md, 'Sharding'), and this "
+              f"build measures one GPU's share; --gpus {args.gpus} is not supported.",
+              file=sys.stderr)
+        return 2
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
+    from jaxdd_amd import synthetic as S
+
+    workers = args.workers or host_cores()
+    t = time.perf_counter()
+    packed = S.packed(args.config, args.n, variant=args.variant, workers=workers)
+    build_s = time.perf_counter() - t
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.config, args.variant, args.cpu_curves)
+
+    import torch
+
+    from jaxdd_amd.batch import from_packed, solve_batch
+
+    batch = from_packed(packed, "cuda")
+    V = S.VOLTAGES
+    for _ in range(args.warmup):
+        solve_batch(batch, V, precision=args.precision)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        J, ok, its = solve_batch(batch, V, precision=args.precision, events=evs[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    J, ok, its = J.cpu().numpy(), ok.cpu().numpy(), its.cpu().numpy()
+
+    ions = args.config == "c3"
+    w = 4 if args.precision == "mixed" else 8
+    step_bytes = b_step(400, 3 if ions else 1, w)
+    launch_bytes = float(its.sum()) * step_bytes
+    achieved = launch_bytes / (kernel_ms * 1e-3) / 1e9
+    workload = f"{args.config}-{args.variant}-n{args.n}-{args.precision}"
+
+    out = {
+        "metric": METRIC,
+        "value": args.n * args.steps / elapsed,
+        "unit": "J-V curves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.config} {'free' if args.variant == 'free' else 'frozen-after-1.2V'}"
+                         f"-ion J-V: {args.n} devices, NP=400, 61 voltages -0.2..1.3 V, "
+                         f"3x3 blocks{' + 2-slot ion border' if ions else ''}, "
+                         f"{args.precision} precision, seed {S.SEED}"),
+            "devices": args.n, "NP": 400, "voltages": len(V), "precision": args.precision,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(workload),
+            "kernel": "ddh::solve_kernel", "kernel_ms": kernel_ms,
+            "algorithmic_bytes_per_launch": launch_bytes, "B_step": step_bytes,
+        },
+        "cpu_baseline": cpu,
+        "converged_frac": float(ok.mean()),
+        "newton_steps_per_voltage": float(its.mean()),
+        "host_build_s": build_s,
+    }
+    if not args.no_fp64 and args.precision == "mixed":
+        e64 = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        J64, ok64, its64 = solve_batch(batch, V, precision="fp64", events=e64)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        J64, ok64, its64 = J64.cpu().numpy(), ok64.cpu().numpy(), its64.cpu().numpy()
+        scale = batch["J_scale"].cpu().numpy()[:, None] + np.max(np.abs(J64), axis=1, keepdims=True)
+        both = ok & ok64
+        calm = (its < CHAOTIC_STEPS) & (its64 < CHAOTIC_STEPS)
+        out["fp64"] = {
+            "value": args.n / (b - a), "ms_per_step": (b - a) * 1e3,
+            "kernel_ms": e64[0].elapsed_time(e64[1]),
+            "speedup_mixed_over_fp64": (b - a) / (elapsed / args.steps),
+        }
+        out["max_err_vs_fp64"] = float(np.max((np.abs(J - J64) / scale)[both]))
+        out["flags_equal_calm"] = bool(np.array_equal(ok[calm], ok64[calm]))
+        out["chaotic_points"] = int((~calm).sum())
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -7,7 +7,6 @@ The reference's API for this path (src/jaxdd/__init__.py:14-66): SIMsalabim inpu
 called through a C-ABI; torch tensors hold the buffers.
 """
 
-from .batch import external_voltages, prepare, solve_batch
 from .device import Device, Structure, build_device, device_arrays, device_parameters, make_grid
 from .simss_input import (
     Layer,
@@ -17,8 +16,22 @@ from .simss_input import (
     setup_from_texts,
     simss_voltages,
 )
-from .solver import JVResult, bernoulli, bernoulli_simss, solve_jv
-from .sweep import block_sweep
+
+# The GPU entry points import torch; they are loaded on first use, so that the host-only
+# modules (the reader, the device builder, the synthetic workloads) import without it.
+_LAZY = {
+    "external_voltages": "batch", "prepare": "batch", "solve_batch": "batch",
+    "JVResult": "solver", "bernoulli": "solver", "bernoulli_simss": "solver", "solve_jv": "solver",
+    "block_sweep": "sweep",
+}
+
+
+def __getattr__(name):
+    if name in _LAZY:
+        import importlib
+
+        return getattr(importlib.import_module(f".{_LAZY[name]}", __name__), name)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
 
 __version__ = "0.1.0"
 

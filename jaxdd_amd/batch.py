@@ -21,7 +21,7 @@ from . import _lib
 from .constants import Q
 from .device import Device
 
-__all__ = ["external_voltages", "pack", "prepare", "solve_batch"]
+__all__ = ["external_voltages", "from_packed", "merge", "pack", "prepare", "solve_batch"]
 
 # Field orders of jaxdd_amd/csrc/ddhip_layout.h.
 PT_FIELDS = ("h", "x", "G", "fix")
@@ -96,24 +96,45 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     pt = np.ascontiguousarray(pt.transpose(1, 0, 2))  # (P, 4, n)
     sc = np.ascontiguousarray(np.array([_scalars(d) for d in devices]).T)  # (20, n)
     return {"pt": pt, "sid": np.ascontiguousarray(sid.T.astype(np.uint8)), "seg": seg,
-            "sc": sc, "nseg": nseg}
+            "sc": sc, "nseg": nseg, "R_series": np.array([d.R_series for d in devices])}
+
+
+def merge(parts: list[dict]) -> dict:
+    """Packed arrays of several device lists joined along the lane axis (the segment tables
+    padded to the largest count, the extra segments never read)."""
+    nseg = max(p["nseg"] for p in parts)
+    segs = []
+    for p in parts:
+        seg = p["seg"]
+        if seg.shape[0] < nseg:
+            seg = np.concatenate([seg, np.repeat(seg[-1:], nseg - seg.shape[0], axis=0)])
+        segs.append(seg)
+    out = {"nseg": nseg, "seg": np.concatenate(segs, axis=2)}
+    for k in ("pt", "sid", "sc"):
+        out[k] = np.ascontiguousarray(np.concatenate([p[k] for p in parts], axis=-1))
+    out["R_series"] = np.concatenate([p["R_series"] for p in parts])
+    return out
 
 
 def prepare(devices: list[Device], device: str | torch.device = "cuda") -> dict:
     """A batch (reference: src/jaxdd/batch.py:151-223): the devices packed for the kernels, in
     HBM on ``device``, with the reference's per-device keys. Raises `ValueError` on devices of
     different grid sizes, as the reference does."""
-    packed = pack(devices)
-    n = len(devices)
-    NP = len(devices[0].x) - 2
+    return from_packed(pack(devices), device)
+
+
+def from_packed(packed: dict, device: str | torch.device = "cuda") -> dict:
+    """The batch of packed arrays (`pack`, `merge`), moved to ``device``."""
     dev = torch.device(device)
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
 
     def t(a, dtype=torch.float64):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
 
     sc = packed["sc"]
     col = {name: sc[k] for k, name in enumerate(SC_FIELDS)}
-    batch = {
+    return {
         "n": n, "NP": NP, "nseg": packed["nseg"],
         "_pt": t(packed["pt"]), "_sid": t(packed["sid"], torch.uint8), "_seg": t(packed["seg"]),
         "_sc": t(sc),
@@ -121,11 +142,9 @@ def prepare(devices: list[Device], device: str | torch.device = "cuda") -> dict:
         "J_scale": t(col["J_scale"]), "g_frac": t(col["g_frac"]), "Vt": t(col["Vt"]),
         "L": t(col["L"]), "V_pre": t(col["V_pre"]), "pre_cond": t(col["pre_cond"]),
         "fix_ions": t(col["fix_ions"]), "left_elec": t(col["left_elec"]),
-        "G_shunt": t(col["G_shunt"]),
-        "R_series": t(np.array([d.R_series for d in devices])),
-        "x": t(np.stack([d.x for d in devices])), "h": t(np.stack([d.h for d in devices])),
+        "G_shunt": t(col["G_shunt"]), "R_series": t(packed["R_series"]),
+        "x": t(packed["pt"][:, 1, :].T), "h": t(packed["pt"][:, 0, :].T),
     }
-    return batch
 
 
 def _sc_now(batch: dict) -> torch.Tensor:
@@ -136,9 +155,10 @@ def _sc_now(batch: dict) -> torch.Tensor:
     return sc
 
 
-def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int):
+def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=None):
     """One call of ddh_solve_batch: (J, ok, its, status) as (n, nV) tensors on the batch's
-    device, V as given (sorted by the caller in batch mode)."""
+    device, V as given (sorted by the caller in batch mode). ``events``: a pair of
+    torch.cuda.Event recorded on the launch stream just before and after the solve kernel."""
     lib = _lib.load()
     dev = batch["_pt"].device
     if dev.type != "cuda":
@@ -155,17 +175,22 @@ def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int):
     b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
                    batch["_seg"].data_ptr(), sc.data_ptr())
     stream = torch.cuda.current_stream(dev).cuda_stream
+    if events is not None:
+        events[0].record()
     rc = lib.ddh_solve_batch(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
                              J.data_ptr(), ok.data_ptr(), its.data_ptr(), status.data_ptr(),
                              ws.data_ptr(), nbytes, stream)
     _lib.check(rc, "ddh_solve_batch")
+    if events is not None:
+        events[1].record()
     # the workspace and the inputs live until the kernels are done
     batch["_inflight"] = (ws, sc, Vd)
     return J, ok, its, status
 
 
 def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
-                bernoulli_name="exact", strict=False, devices=None, precision="fp64"):
+                bernoulli_name="exact", strict=False, devices=None, precision="fp64",
+                events=None):
     """J-V curves of a batch at ``voltages`` (reference: src/jaxdd/batch.py:1136-1162):
     ``(J, converged, iterations)``, each batch x voltages, in the order of ``voltages``, as
     torch tensors on the batch's GPU (float64, bool, int64).
@@ -176,7 +201,8 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     persistent kernel, so no device waits for another and one pass with ``max_iter`` is run.
     ``precision``: "fp64", or "mixed" (the Newton correction's block factorisation in FP32 on
     row-equilibrated blocks; residual, exponentials, current and convergence tests in FP64).
-    ``devices`` is accepted for the reference's signature: one GPU solves the whole batch."""
+    ``devices`` is accepted for the reference's signature: one GPU solves the whole batch.
+    ``events``: optional (start, end) torch.cuda.Event pair timing the solve kernel."""
     del first_max_iter, devices
     if precision not in _lib.PRECISIONS:
         raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
@@ -186,7 +212,7 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     order = np.argsort(V, kind="stable")
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
                      0 if bernoulli_name == "exact" else 1, _lib.PRECISIONS[precision], 0)
-    J, ok, its, _ = _run(batch, V[order], opts, _lib.PRECISIONS[precision])
+    J, ok, its, _ = _run(batch, V[order], opts, _lib.PRECISIONS[precision], events)
     inv = torch.as_tensor(np.argsort(order, kind="stable"), device=J.device)
     J, ok, its = J[:, inv], ok[:, inv].bool(), its[:, inv].long()
     Vt = torch.as_tensor(V, device=J.device)

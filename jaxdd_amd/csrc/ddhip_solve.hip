@@ -1,7 +1,7 @@
 // HIP kernels of the batched steady-state J-V solve (gfx950) and the C-ABI of libddhip.so.
-// The per-lane logic is in ddhip_core.h; here the launches: Poisson's start (one lane per
-// device), the persistent Newton kernel (one lane per device, each wavefront looping until its
-// last lane has finished its scan), and the block-tridiagonal sweep operator.
+// The per-lane logic is in ddhip_core.h; here the launches: the persistent solve kernel (one
+// lane per device runs its Poisson start, start solves, pre-bias and voltage scan) and the
+// block-tridiagonal sweep operator.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -11,15 +11,15 @@
 
 namespace ddh {
 
-__global__ void __launch_bounds__(64) poisson_kernel(Args a) {
-    const int lane = blockIdx.x * 64 + threadIdx.x;
-    if (lane < a.n) poisson_lane(a, lane);
-}
-
-__global__ void __launch_bounds__(64) newton_kernel(Args a) {
+// The whole solve of every device in one launch: each lane runs Poisson's start, then its
+// Newton state machine; each wavefront loops until its last lane has finished its scan.
+__global__ void __launch_bounds__(64) solve_kernel(Args a) {
     const int lane = blockIdx.x * 64 + threadIdx.x;
     const bool valid = lane < a.n;
-    if (valid) start_lane(a, lane, a.st[lane]);
+    if (valid) {
+        poisson_lane(a, lane);
+        start_lane(a, lane, a.st[lane]);
+    }
     while (true) {
         const bool act = valid && lane_active(a, lane);
         if (!__any(act)) break;
@@ -227,12 +227,9 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
     a.mixed = o->precision == DDH_MIXED ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((b->n + 63) / 64), block(64);
-    hipLaunchKernelGGL(ddh::poisson_kernel, grid, block, 0, s, a);
+    hipLaunchKernelGGL(ddh::solve_kernel, grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "poisson_kernel launch");
-    hipLaunchKernelGGL(ddh::newton_kernel, grid, block, 0, s, a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "newton_kernel launch");
+    if (e != hipSuccess) return hip_fail(e, "solve_kernel launch");
     return DDH_OK;
 }
 

@@ -236,6 +236,8 @@ def check_supported(setup: Setup) -> None:
         problems.append("genProfile = calc: the transfer-matrix generation is not on this path")
     for j, ly in enumerate(setup.layers, start=1):
         for name, off in _LAYER_OFF.items():
+            if name == "bulkTrapType" and float(ly.raw.get("N_t_bulk", "0")) == 0:
+                continue  # a trap type without traps does nothing
             value = ly.raw.get(name, off).strip()
             same = value.lower() == off if off == "none" else float(value) == float(off)
             if not same:

@@ -143,3 +143,30 @@ def devices(config: str, n: int, seed: int = SEED, variant: str = "free", start:
     ions = config == "c3"
     frozen = ions and variant == "frozen"
     return [device_from_row(t, k, 400, ions, frozen) for k in range(start, start + n)]
+
+
+def _pack_rows(args):
+    from .batch import pack
+
+    config, start, stop, seed, variant, NP = args
+    t = draw(stop, seed)
+    ions = config == "c3"
+    frozen = ions and variant == "frozen"
+    return pack([device_from_row(t, k, NP, ions, frozen) for k in range(start, stop)])
+
+
+def packed(config: str, n: int, seed: int = SEED, variant: str = "free", workers: int = 1,
+           chunk: int = 256) -> dict:
+    """The packed kernel arrays (`batch.pack`) of ``n`` devices of a workload, built in
+    ``workers`` processes. Call it before the process touches the GPU: the workers are forked."""
+    from .batch import merge
+
+    jobs = [(config, s, min(s + chunk, n), seed, variant, 400) for s in range(0, n, chunk)]
+    if workers <= 1:
+        parts = [_pack_rows(j) for j in jobs]
+    else:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(workers) as pool:
+            parts = pool.map(_pack_rows, jobs)
+    return merge(parts)

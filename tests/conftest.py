@@ -1,0 +1,80 @@
+"""Shared fixtures. GPU tests are marked ``gpu`` and run the HIP kernels through the C-ABI;
+everything else runs on the CPU (the oracle, the host logic, the kernels' CPU replay)."""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+for p in (ROOT, ROOT / "tests" / "kernel_emu"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+#: The SimSS 5.36 runs copied from the reference's tests/data, with the tolerance (of the
+#: curve's largest |J|) its own tests hold them to (src/../tests/test_solver.py:194-204,
+#: :254-272, :878-887, :1316-1358, :1556-1575).
+FIXTURES = {
+    "thin_l1_light": 1e-5,
+    "none_light": 1e-5,
+    "frozen_1e17_x1_thin_light": 2e-5,
+    "steady_1e17_thin_light": 2e-5,
+    "held_thin_light": 1e-9,
+    "frozen_thick_htl_dark": 2e-5,
+    "resist_light": 1e-5,
+}
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+def fixture_device(case: str):
+    from jaxdd_amd.device import build_device
+    from jaxdd_amd.simss_input import read_setup
+
+    return build_device(read_setup(GOLDEN / case / "simulation_setup.txt"), grid="simss")
+
+
+def fixture_jv(case: str):
+    return np.genfromtxt(GOLDEN / case / "JV.dat", names=True)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import ddoracle
+
+    ddoracle.build()
+    return ddoracle
+
+
+@pytest.fixture(scope="session")
+def emu():
+    import emu as _emu
+
+    _emu.build()
+    return _emu
+
+
+def curve_scale(dev, J) -> float:
+    """The reference's per-curve scale: the generation's current plus the curve's largest |J|."""
+    from jaxdd_amd.batch import j_scale
+
+    return j_scale(dev) + float(np.max(np.abs(J)))
+
+
+#: A solve that takes this many Newton steps wanders before it converges (or fails): there the
+#: outcome depends on the last bits of the arithmetic (a 1e-15 change of one input flips it in
+#: the oracle itself), so converged flags are compared on the solves both sides finish sooner.
+CHAOTIC_STEPS = 20
+
+
+def flags_agree(ok, its, ok_ref, its_ref):
+    """Converged flags equal wherever neither side took CHAOTIC_STEPS or more; returns
+    (agree, number of chaotic entries)."""
+    calm = (its < CHAOTIC_STEPS) & (its_ref < CHAOTIC_STEPS)
+    return bool(np.array_equal(ok[calm], ok_ref[calm])), int((~calm).sum())

@@ -1,6 +1,6 @@
 // Test-only CPU build of the HIP kernels' per-lane logic (jaxdd_amd/csrc/ddhip_core.h).
 //
-// It replays each 64-lane wavefront of the persistent Newton kernel lane by lane, with the
+// It replays each 64-lane wavefront of the persistent solve kernel lane by lane, with the
 // same "any lane active" / "any lane with mobile ions" wave votes, so that the kernels'
 // algorithm can be debugged and compared with the oracle on a host without a GPU. Nothing in
 // the product loads it.
@@ -13,10 +13,12 @@
 using namespace ddh;
 
 static void run(Args &a) {
-    for (int lane = 0; lane < a.n; ++lane) poisson_lane(a, lane);
     for (int w0 = 0; w0 < a.n; w0 += 64) {
         int w1 = w0 + 64 < a.n ? w0 + 64 : a.n;
-        for (int lane = w0; lane < w1; ++lane) start_lane(a, lane, a.st[lane]);
+        for (int lane = w0; lane < w1; ++lane) {
+            poisson_lane(a, lane);
+            start_lane(a, lane, a.st[lane]);
+        }
         for (;;) {
             bool any = false, ions = false, f64 = !a.mixed;
             for (int lane = w0; lane < w1; ++lane) {

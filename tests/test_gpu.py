@@ -1,0 +1,206 @@
+"""Parity of the HIP path (through the C-ABI, libddhip.so) with the CPU oracle and the SimSS
+fixtures, on an MI355X. Sizes the oracle finishes in seconds are compared point by point;
+the full 16384-device workload is checked through size-independent properties."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, curve_scale, fixture_device, fixture_jv, flags_agree
+
+pytestmark = pytest.mark.gpu
+
+V_FULL = np.round(np.arange(-0.2, 1.3 + 1e-9, 0.025), 6)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available(), "the GPU tests need the MI355X"
+    from jaxdd_amd import _lib
+
+    _lib.load()  # the HIP library must be there: no fallback
+    return torch
+
+
+# ---- the block sweep operator (reference tests test_solver.py:1620-1641, 1753-1773) ------
+@pytest.mark.parametrize("b,k", [(3, 1), (3, 3), (5, 4), (3, 8)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+def test_block_sweep_is_the_scan(torch_cuda, oracle, b, k, precision, tol):
+    from jaxdd_amd.sweep import block_sweep
+
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    n, NP = 70, 12  # not a whole number of wavefronts
+    A, C = rng.normal(size=(n, NP, b, b)), rng.normal(size=(n, NP, b, b))
+    B = rng.normal(size=(n, NP, b, b)) + 8 * np.eye(b)
+    d = rng.normal(size=(n, NP, b, k))
+    ref = np.stack([oracle.block_thomas(A[i], B[i], C[i], d[i]) for i in range(n)])
+    x = block_sweep(*(torch.as_tensor(t, device="cuda") for t in (A, B, C, d)), precision=precision)
+    assert np.max(np.abs(_np(x) - ref)) < tol * np.max(np.abs(ref))
+
+
+# ---- the fixtures: oracle and SimSS -----------------------------------------------------
+@pytest.fixture(scope="module")
+def fixtures_on_gpu(torch_cuda, oracle):
+    from jaxdd_amd.batch import prepare
+
+    cases = sorted(FIXTURES)
+    devs = [fixture_device(c) for c in cases]
+    ref = oracle.solve_batch(devs, V_FULL, bernoulli_name="simss", first_max_iter=None)
+    return cases, devs, prepare(devs), ref
+
+
+@pytest.mark.parametrize("precision,jtol", [("fp64", 1e-12), ("mixed", 1e-9)])
+def test_solve_batch_matches_oracle_on_fixtures(fixtures_on_gpu, precision, jtol):
+    from jaxdd_amd.batch import solve_batch
+
+    cases, devs, batch, (Jo, oko, itso) = fixtures_on_gpu
+    J, ok, its = solve_batch(batch, V_FULL, bernoulli_name="simss", precision=precision)
+    J, ok, its = _np(J), _np(ok), _np(its)
+    assert np.array_equal(ok, oko)
+    if precision == "fp64":
+        assert np.array_equal(its, itso)
+    for k, dev in enumerate(devs):
+        assert np.max(np.abs(J[k] - Jo[k])) < jtol * curve_scale(dev, Jo[k]), cases[k]
+
+
+@pytest.mark.parametrize("case", sorted(FIXTURES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_jv_reproduces_simss(torch_cuda, case, precision):
+    from jaxdd_amd.solver import solve_jv
+
+    dev = fixture_device(case)
+    ref = fixture_jv(case)
+    res = solve_jv(dev, ref["Vint"], bernoulli_name="simss", precision=precision)
+    assert res.converged.all()
+    scale = np.max(np.abs(ref["Jext"]))
+    assert np.max(np.abs(res.J - ref["Jext"])) < FIXTURES[case] * scale
+    if case == "resist_light":
+        assert np.max(np.abs(res.V_ext - ref["Vext"])) < 1e-4
+
+
+def test_solve_jv_matches_oracle(torch_cuda, oracle):
+    from jaxdd_amd.solver import solve_jv
+
+    for case in ("thin_l1_light", "frozen_1e17_x1_thin_light", "held_thin_light"):
+        dev = fixture_device(case)
+        V = V_FULL[::-1].copy()
+        Jo, oko, itso = oracle.solve_jv(dev, V, bernoulli_name="simss")
+        res = solve_jv(dev, V, bernoulli_name="simss")
+        assert np.array_equal(res.converged, oko)
+        assert np.array_equal(res.iterations, itso)
+        assert np.max(np.abs(res.J - Jo)) < 1e-12 * curve_scale(dev, Jo)
+
+
+def test_solve_jv_raises_on_a_failed_pre_bias(torch_cuda):
+    from jaxdd_amd.solver import solve_jv
+
+    dev = fixture_device("frozen_1e17_x1_thin_light")
+    with pytest.raises(RuntimeError, match="pre-bias"):
+        solve_jv(dev, V_FULL[:3], max_iter=1)
+
+
+def test_voltage_order_and_g_frac(torch_cuda, fixtures_on_gpu):
+    """Results come back in the caller's voltage order; dict(batch, g_frac=...) is honoured
+    (reference tests test_solver.py:1521-1553)."""
+    from jaxdd_amd.batch import solve_batch
+
+    _, devs, batch, _ = fixtures_on_gpu
+    up = [_np(a) for a in solve_batch(batch, V_FULL)]
+    down = [_np(a) for a in solve_batch(batch, V_FULL[::-1])]
+    for a, b in zip(up, down):
+        assert np.array_equal(a, b[:, ::-1])
+    dark = dict(batch, g_frac=batch["g_frac"] * 0.0)
+    J, ok, _ = solve_batch(dark, np.array([0.0, 0.4]))
+    assert _np(ok).all()
+    scale = _np(batch["J_scale"])
+    assert np.all(np.abs(_np(J)[:, 0]) < 1e-5 * scale)  # no light, no bias: no current
+
+
+def test_deterministic(torch_cuda, fixtures_on_gpu):
+    from jaxdd_amd.batch import solve_batch
+
+    _, _, batch, _ = fixtures_on_gpu
+    a = [_np(t) for t in solve_batch(batch, V_FULL, precision="mixed")]
+    b = [_np(t) for t in solve_batch(batch, V_FULL, precision="mixed")]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+# ---- the synthetic workloads against the oracle --------------------------------------------
+@pytest.mark.parametrize("config,variant,n", [("c2", "free", 48), ("c3", "free", 48),
+                                              ("c3", "frozen", 48)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_synthetic_matches_oracle(torch_cuda, oracle, config, variant, n, precision):
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import prepare, solve_batch
+
+    devs = S.devices(config, n, variant=variant)
+    Jo, oko, itso = oracle.solve_batch(devs, S.VOLTAGES, first_max_iter=None)
+    J, ok, its = (_np(t) for t in solve_batch(prepare(devs), S.VOLTAGES, precision=precision))
+    agree, chaotic = flags_agree(ok, its, oko, itso)
+    assert agree and chaotic <= 0.02 * ok.size
+    both = ok & oko
+    for k, dev in enumerate(devs):
+        err = np.max(np.abs(J[k] - Jo[k])[both[k]], initial=0.0)
+        assert err < 1e-6 * curve_scale(dev, Jo[k])  # the workload's accuracy bar
+
+
+def test_one_device_workload_c1(torch_cuda, oracle):
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.solver import solve_jv
+
+    dev = S.c1_device()
+    Jo, oko, itso = oracle.solve_jv(dev, S.VOLTAGES)
+    res = solve_jv(dev, S.VOLTAGES)
+    assert res.converged.all() and oko.all()
+    assert np.max(np.abs(res.J - Jo)) < 1e-10 * curve_scale(dev, Jo)
+
+
+def test_nan_input_ends_unconverged(torch_cuda):
+    import dataclasses
+
+    from jaxdd_amd.batch import prepare, solve_batch
+
+    good = fixture_device("thin_l1_light")
+    G = good.G.copy()
+    G[200] = np.nan
+    bad = dataclasses.replace(good, G=G)
+    J, ok, its = (_np(t) for t in solve_batch(prepare([bad, good]), V_FULL[::6].copy()))
+    assert not ok[0].any() and ok[1].all()
+    assert its[0].max() <= 1
+
+
+# ---- the full-size workload: size-independent properties ---------------------------------
+def test_full_batch_properties(torch_cuda):
+    """4096 devices of the ion workload: every voltage's current finite, the short-circuit
+    current never above the generation's, mixed within 1e-6 of the all-FP64 solve per
+    converged point, the same converged flags away from chaotic solves, and a second solve
+    bit-identical to the first."""
+    import os
+
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import from_packed, solve_batch
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    batch = from_packed(S.packed("c3", 4096, variant="free", workers=workers))
+    J64, ok64, it64 = (_np(t) for t in solve_batch(batch, S.VOLTAGES))
+    Jm, okm, itm = (_np(t) for t in solve_batch(batch, S.VOLTAGES, precision="mixed"))
+    Jm2 = _np(solve_batch(batch, S.VOLTAGES, precision="mixed")[0])
+    assert np.array_equal(Jm, Jm2)
+    assert np.isfinite(J64).all()
+    assert ok64.mean() > 0.99
+    agree, chaotic = flags_agree(okm, itm, ok64, it64)
+    assert agree and chaotic <= 0.02 * okm.size
+    scale = _np(batch["J_scale"])[:, None] + np.max(np.abs(J64), axis=1, keepdims=True)
+    both = okm & ok64
+    assert np.max((np.abs(Jm - J64) / scale)[both]) < 1e-6
+    i0 = int(np.argmin(np.abs(S.VOLTAGES)))
+    assert np.all(-J64[:, i0][ok64[:, i0]] <= _np(batch["J_scale"])[ok64[:, i0]] * (1 + 1e-9))

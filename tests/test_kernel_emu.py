@@ -1,0 +1,92 @@
+"""The HIP kernels' per-lane logic (jaxdd_amd/csrc/ddhip_core.h), built for the CPU by
+tests/kernel_emu and replayed wavefront by wavefront, against the oracle. These run without a
+GPU; the same comparisons on the GPU are in test_gpu.py."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, curve_scale, fixture_device, flags_agree
+
+V_FULL = np.round(np.arange(-0.2, 1.3 + 1e-9, 0.025), 6)
+
+
+@pytest.fixture(scope="module")
+def fixture_runs(oracle):
+    from jaxdd_amd.batch import pack
+
+    cases = sorted(FIXTURES)
+    devs = [fixture_device(c) for c in cases]
+    ref = oracle.solve_batch(devs, V_FULL, bernoulli_name="simss", first_max_iter=None)
+    shunt = np.array([1 / d.R_shunt if d.R_shunt > 0 else 0.0 for d in devs])
+    return cases, devs, pack(devs), (ref[0] - shunt[:, None] * V_FULL, ref[1], ref[2])
+
+
+@pytest.mark.parametrize("precision,jtol", [("fp64", 1e-12), ("mixed", 1e-9)])
+def test_replay_matches_oracle_on_fixtures(emu, fixture_runs, precision, jtol):
+    cases, devs, packed, (Jo, oko, itso) = fixture_runs
+    J, ok, its, status = emu.solve_packed(packed, V_FULL, bernoulli="simss", precision=precision)
+    assert not status.any()
+    assert np.array_equal(ok, oko)
+    if precision == "fp64":
+        assert np.array_equal(its, itso)  # the same Newton steps, voltage by voltage
+    for k, dev in enumerate(devs):
+        assert np.max(np.abs(J[k] - Jo[k])) < jtol * curve_scale(dev, Jo[k]), cases[k]
+
+
+@pytest.mark.parametrize("config,variant", [("c2", "free"), ("c3", "free"), ("c3", "frozen")])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_replay_matches_oracle_on_synthetic(emu, oracle, config, variant, precision):
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    devs = S.devices(config, 12, variant=variant)
+    Jo, oko, itso = oracle.solve_batch(devs, S.VOLTAGES, first_max_iter=None)
+    J, ok, its, _ = emu.solve_packed(pack(devs), S.VOLTAGES, precision=precision)
+    agree, chaotic = flags_agree(ok, its, oko, itso)
+    assert agree and chaotic <= 0.02 * ok.size
+    both = ok & oko
+    for k, dev in enumerate(devs):
+        err = np.max(np.abs(J[k] - Jo[k])[both[k]], initial=0.0)
+        assert err < 1e-9 * curve_scale(dev, Jo[k])
+
+
+def test_replay_solve_jv_mode_matches_oracle(emu, oracle):
+    from jaxdd_amd.batch import pack
+
+    for case in ("thin_l1_light", "frozen_1e17_x1_thin_light", "steady_1e17_thin_light"):
+        dev = fixture_device(case)
+        V = V_FULL[::-1].copy()  # the given order, not sorted
+        Jo, oko, itso = oracle.solve_jv(dev, V, bernoulli_name="simss")
+        J, ok, its, status = emu.solve_packed(pack([dev]), V, bernoulli="simss", jv=True,
+                                              max_iter=200)
+        assert not status.any()
+        assert np.array_equal(ok[0], oko)
+        assert np.max(np.abs(J[0] - Jo)) < 1e-10 * curve_scale(dev, Jo)
+
+
+def test_replay_pre_bias_failure_is_reported(emu):
+    """jv mode: a pre-bias that cannot converge (one Newton step allowed) sets the status the
+    host turns into RuntimeError, as the reference's solve_jv raises."""
+    from jaxdd_amd.batch import pack
+
+    dev = fixture_device("frozen_1e17_x1_thin_light")
+    _, _, _, status = emu.solve_packed(pack([dev]), V_FULL[:3], jv=True, max_iter=1)
+    assert status[0] == 1
+
+
+def test_replay_nan_input_ends_unconverged(emu):
+    """A NaN in a device's generation ends every Newton solve at its first non-number step:
+    the lane finishes, its flags False, the other lanes untouched."""
+    import dataclasses
+
+    from jaxdd_amd.batch import pack
+
+    good = fixture_device("thin_l1_light")
+    G = good.G.copy()
+    G[200] = np.nan
+    bad = dataclasses.replace(good, G=G)
+    J, ok, its, _ = emu.solve_packed(pack([bad, good]), V_FULL[::6].copy())
+    assert not ok[0].any() and ok[1].all()
+    assert its[0].max() <= 1

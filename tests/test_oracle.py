@@ -1,0 +1,223 @@
+"""The CPU oracle pinned before it is trusted: against the SimSS fixtures the reference's own
+tests hold (tests/golden), closed forms, dense solves and finite differences."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, curve_scale, fixture_device, fixture_jv
+
+
+# ---- Bernoulli (src/jaxdd/solver.py:95-121; reference tests test_solver.py:50-64) ----------
+def test_bernoulli_at_zero(oracle):
+    B, dB = oracle.bernoulli(0.0)
+    assert B[0] == 1.0
+    assert dB[0] == pytest.approx(-0.5)
+
+
+@pytest.mark.parametrize("x", [-30.0, -1.0, -1e-3, 1e-3, 0.3, 2.0, 30.0])
+def test_bernoulli_identity(oracle, x):
+    """B(-x) = B(x) + x, and B'(-x) = -B'(x) - 1."""
+    Bp, dBp = oracle.bernoulli(x)
+    Bm, dBm = oracle.bernoulli(-x)
+    assert Bm[0] == pytest.approx(Bp[0] + x, rel=1e-12, abs=1e-14)
+    assert dBm[0] == pytest.approx(-dBp[0] - 1.0, rel=1e-10, abs=1e-12)
+
+
+def test_bernoulli_closed_form_and_derivative(oracle):
+    """Against x/expm1(x) in extended precision and a central difference of B."""
+    x = np.concatenate([np.linspace(-40, 40, 161), [0.02, -0.02, 5e-3, -5e-3]])
+    B, dB = oracle.bernoulli(x)
+    xl = x.astype(np.longdouble)
+    safe = np.where(x == 0, 1, xl)
+    exact = np.where(x == 0, 1, safe / np.expm1(safe))
+    assert np.max(np.abs(B - exact.astype(float)) / np.abs(exact.astype(float))) < 1e-13
+    h = 1e-6
+    fd = (oracle.bernoulli(x + h)[0] - oracle.bernoulli(x - h)[0]) / (2 * h)
+    assert np.max(np.abs(dB - fd)) < 1e-8
+
+
+def test_simss_bernoulli_within_its_own_error(oracle):
+    x = np.linspace(-5, 5, 2001)
+    assert np.max(np.abs(oracle.bernoulli(x, "simss")[0] / oracle.bernoulli(x)[0] - 1)) < 1e-3
+
+
+def test_host_bernoulli_matches_oracle(oracle):
+    from jaxdd_amd.solver import bernoulli, bernoulli_simss
+
+    x = np.linspace(-20, 20, 801)
+    assert np.allclose(bernoulli(x), oracle.bernoulli(x)[0], rtol=1e-14, atol=0)
+    assert np.allclose(bernoulli_simss(x), oracle.bernoulli(x, "simss")[0], rtol=1e-14, atol=0)
+
+
+# ---- block Thomas vs a dense solve (reference tests test_solver.py:1753-1773) --------------
+@pytest.mark.parametrize("b,k", [(3, 1), (3, 3), (5, 4)])
+def test_block_thomas_is_the_dense_solve(oracle, b, k):
+    rng = np.random.default_rng(3)
+    NP = 12
+    A, C = rng.normal(size=(NP, b, b)), rng.normal(size=(NP, b, b))
+    B = rng.normal(size=(NP, b, b)) + 8 * np.eye(b)
+    d = rng.normal(size=(NP, b, k))
+    T = np.zeros((NP * b, NP * b))
+    for i in range(NP):
+        T[i * b:(i + 1) * b, i * b:(i + 1) * b] = B[i]
+        if i > 0:
+            T[i * b:(i + 1) * b, (i - 1) * b:i * b] = A[i]
+        if i < NP - 1:
+            T[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = C[i]
+    ref = np.linalg.solve(T, d.reshape(NP * b, k)).reshape(NP, b, k)
+    x = oracle.block_thomas(A, B, C, d)
+    assert np.max(np.abs(x - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+# ---- assembly: Jacobian by finite differences, residual by rows ---------------------------
+def _perturbed_state(dev, seed=1):
+    from tests_helpers import initial_full
+
+    rng = np.random.default_rng(seed)
+    full = initial_full(dev, 0.3)
+    NP = len(dev.x) - 2
+    noise = np.zeros_like(full)
+    noise[1:-1] = rng.normal(0, 1, (NP, 3)) * np.array([5 * dev.Vt, 2.0, 2.0])
+    noise[1:-1:7, 0] = 0.0  # some neighbours at equal potential: the series of B
+    return full + noise
+
+
+@pytest.mark.parametrize("case", ["thin_l1_light", "steady_1e17_thin_light"])
+@pytest.mark.parametrize("bern", ["exact", "simss"])
+def test_assembly_jacobian_is_the_residuals_derivative(oracle, case, bern):
+    """A, B, C and the ion columns U against central differences of the residual F
+    (the reference checks its by-edge assembly against jacfwd of its rows,
+    test_solver.py:1578-1617)."""
+    dev = fixture_device(case)
+    full = _perturbed_state(dev)
+    NP = len(dev.x) - 2
+    la, lc = 40.0, 41.0
+    F, A, B, C, U = oracle.assemble(dev, full, la=la, lc=lc, bernoulli_name=bern)
+    rng = np.random.default_rng(7)
+    for i in rng.choice(np.arange(1, NP + 1), 12, replace=False):
+        for col in range(3):
+            h = 1e-7 * max(1.0, abs(full[i, col]))
+            fp, fm = full.copy(), full.copy()
+            fp[i, col] += h
+            fm[i, col] -= h
+            dF = (oracle.assemble(dev, fp, la=la, lc=lc, bernoulli_name=bern)[0]
+                  - oracle.assemble(dev, fm, la=la, lc=lc, bernoulli_name=bern)[0]) / (2 * h)
+            r = i - 1
+            for row_pt, blk in ((r, B), (r + 1, A), (r - 1, C)):
+                if 0 <= row_pt < NP:
+                    got, want = blk[row_pt, :, col], dF[row_pt]
+                    scale = np.max(np.abs(blk[row_pt])) + 1e-300
+                    # the series switch of SimSS's B makes its derivative piecewise
+                    tol = 1e-5 if bern == "exact" else 2e-3
+                    assert np.max(np.abs(got - want)) < tol * scale, (case, i, col)
+    for k, l0 in ((0, la), (1, lc)):
+        h = 1e-6
+        args = [la, lc]
+        args[k] = l0 + h
+        Fp = oracle.assemble(dev, full, la=args[0], lc=args[1], bernoulli_name=bern)[0]
+        args[k] = l0 - h
+        Fm = oracle.assemble(dev, full, la=args[0], lc=args[1], bernoulli_name=bern)[0]
+        dF = (Fp - Fm) / (2 * h)
+        mask = (dev.anion_mask if k == 0 else dev.cation_mask)[1:-1]
+        got = U[:, :, k] * mask[:, None]
+        assert np.max(np.abs(got - dF * mask[:, None])) < 1e-6 * (np.max(np.abs(dF)) + 1e-300)
+
+
+def test_assembly_residual_by_rows(oracle):
+    """The residual by edges equals an independent row-by-row NumPy statement of the discrete
+    equations (src/jaxdd/solver.py:964-1006, `_row`)."""
+    from tests_helpers import residual_by_rows
+
+    dev = fixture_device("steady_1e17_thin_light")
+    full = _perturbed_state(dev, seed=5)
+    F = oracle.assemble(dev, full, la=40.0, lc=41.0)[0]
+    ref = residual_by_rows(dev, full, 40.0, 41.0)
+    scale = np.max(np.abs(ref), axis=0)
+    assert np.max(np.abs(F - ref) / scale) < 1e-11
+
+
+# ---- end to end against SimSS -------------------------------------------------------------
+@pytest.mark.parametrize("case", sorted(FIXTURES))
+def test_oracle_solve_jv_reproduces_simss(oracle, case):
+    dev = fixture_device(case)
+    ref = fixture_jv(case)
+    J, ok, its = oracle.solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    assert ok.all()
+    scale = np.max(np.abs(ref["Jext"]))
+    assert np.max(np.abs(J - ref["Jext"])) < FIXTURES[case] * scale
+
+
+@pytest.mark.parametrize("case", sorted(FIXTURES))
+def test_oracle_batch_reproduces_simss(oracle, case):
+    dev = fixture_device(case)
+    ref = fixture_jv(case)
+    J, ok, its = oracle.solve_batch([dev], ref["Vint"], bernoulli_name="simss")
+    assert ok.all()
+    scale = np.max(np.abs(ref["Jext"]))
+    assert np.max(np.abs(J[0] - ref["Jext"])) < FIXTURES[case] * scale
+
+
+def test_dark_current_is_monotone_and_zero_at_zero(oracle):
+    """frozen_thick_htl_dark (reference test_solver.py:1556-1575): monotone, |J(0)| below the
+    current tolerance of the scale."""
+    dev = fixture_device("frozen_thick_htl_dark")
+    ref = fixture_jv("frozen_thick_htl_dark")
+    J, ok, _ = oracle.solve_batch([dev], ref["Vint"], bernoulli_name="simss")
+    J = J[0]
+    scale = oracle.j_scale(dev)
+    assert ok.all()
+    assert np.all(np.diff(J) > -1e-5 * scale)
+    assert abs(J[np.argmin(np.abs(ref["Vint"]))]) < 1e-5 * scale
+
+
+def test_series_resistance_external_voltage(oracle):
+    dev = fixture_device("resist_light")
+    ref = fixture_jv("resist_light")
+    J, ok, _ = oracle.solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    V_ext = ref["Vint"] + J * dev.R_series
+    assert np.max(np.abs(V_ext - ref["Vext"])) < 1e-4
+
+
+def test_short_circuit_current_below_generation(oracle):
+    from jaxdd_amd.constants import Q
+
+    dev = fixture_device("thin_l1_light")
+    J, ok, _ = oracle.solve_jv(dev, np.array([0.0]))
+    generated = Q * np.sum(dev.h[:-1] * dev.G[:-1]) / np.sum(dev.h[:-1]) * dev.L
+    assert ok[0] and -J[0] <= generated * (1 + 1e-9)
+
+
+# ---- batch semantics (reference tests test_solver.py:1479-1532) ----------------------------
+BATCH_CASES = ["thin_l1_light", "frozen_1e17_x1_thin_light", "steady_1e17_thin_light"]
+V_COARSE = np.round(np.arange(-0.2, 1.3 + 1e-9, 0.1), 6)
+
+
+def test_batch_matches_one_device(oracle):
+    devs = [fixture_device(c) for c in BATCH_CASES]
+    J, ok, _ = oracle.solve_batch(devs, V_COARSE, bernoulli_name="simss")
+    assert ok.all()
+    for k, dev in enumerate(devs):
+        single, ok1, _ = oracle.solve_jv(dev, V_COARSE, bernoulli_name="simss")
+        assert ok1.all()
+        assert np.max(np.abs(J[k] - single)) < 1e-6 * np.max(np.abs(single))
+
+
+def test_two_passes_give_the_one_pass_result(oracle):
+    devs = [fixture_device(c) for c in BATCH_CASES]
+    devs = devs + devs[:2]
+    one = oracle.solve_batch(devs, V_COARSE, first_max_iter=None)
+    two = oracle.solve_batch(devs, V_COARSE, first_max_iter=6)
+    first = oracle.solve_batch(devs, V_COARSE, first_max_iter=6, max_iter=6)
+    assert 0 < (~first[1].all(axis=1)).sum() < len(devs)  # some devices are re-solved
+    assert np.array_equal(one[1], two[1]) and np.array_equal(one[2], two[2])
+    assert np.array_equal(one[0], two[0])
+
+
+def test_voltage_order_does_not_matter(oracle):
+    devs = [fixture_device("frozen_1e17_x1_thin_light")]
+    up = oracle.solve_batch(devs, V_COARSE)
+    down = oracle.solve_batch(devs, V_COARSE[::-1])
+    for a, b in zip(up, down):
+        assert np.array_equal(a, b[:, ::-1])
