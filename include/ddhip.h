@@ -41,7 +41,7 @@ typedef struct {
     int32_t reserved;
     const double *pt;   /* [NP + 2][4][n] */
     const uint8_t *sid; /* [NP + 2][n] */
-    const double *seg;  /* [nseg][18][n] */
+    const double *seg;  /* [nseg][22][n] */
     const double *sc;   /* [20][n] */
 } ddh_batch;
 

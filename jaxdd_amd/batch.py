@@ -26,7 +26,7 @@ __all__ = ["external_voltages", "from_packed", "merge", "pack", "prepare", "solv
 # Field orders of jaxdd_amd/csrc/ddhip_layout.h.
 PT_FIELDS = ("h", "x", "G", "fix")
 SEG_FIELDS = ("eps", "offn", "offp", "Nc", "dop", "kd", "ni2", "Nt", "Cn", "Cp", "nt0", "pt0",
-              "am", "cm", "mun", "mup", "Ec", "ni")
+              "am", "cm", "mun", "mup", "Ec", "ni", "sP", "sCs", "imun", "imup")
 SC_FIELDS = ("L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_right",
              "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
              "G_shunt", "n_floor", "pre_cond", "fix_ions")
@@ -34,13 +34,18 @@ DENSITY_FLOOR = 1e-12  # src/jaxdd/solver.py:54
 
 
 def _point_fields(dev: Device) -> np.ndarray:
-    """(NP + 2, 18) material parameters per point, in `SEG_FIELDS` order."""
-    return np.stack([
-        dev.eps, dev.offset_n, dev.offset_p, dev.N_c, dev.N_A - dev.N_D, dev.k_direct,
-        dev.ni ** 2, dev.srh_Nt, dev.srh_Cn, dev.srh_Cp, dev.srh_nt0, dev.srh_pt0,
-        np.asarray(dev.anion_mask, dtype=float), np.asarray(dev.cation_mask, dtype=float),
-        dev.mu_n, dev.mu_p, dev.E_c, dev.ni,
-    ], axis=-1).astype(np.float64)
+    """(NP + 2, 22) material parameters per point, in `SEG_FIELDS` order: the device's own
+    and, derived from them, the row scales (src/jaxdd/solver.py:245-248, the continuity
+    scale without its 1/h) and the inverse interval mobilities."""
+    with np.errstate(divide="ignore"):
+        return np.stack([
+            dev.eps, dev.offset_n, dev.offset_p, dev.N_c, dev.N_A - dev.N_D, dev.k_direct,
+            dev.ni ** 2, dev.srh_Nt, dev.srh_Cn, dev.srh_Cp, dev.srh_nt0, dev.srh_pt0,
+            np.asarray(dev.anion_mask, dtype=float), np.asarray(dev.cation_mask, dtype=float),
+            dev.mu_n, dev.mu_p, dev.E_c, dev.ni,
+            1.0 / (Q * dev.N_c), 1.0 / (np.maximum(dev.mu_n, dev.mu_p) * dev.Vt * dev.N_c),
+            1.0 / dev.mu_n, 1.0 / dev.mu_p,
+        ], axis=-1).astype(np.float64)
 
 
 def j_scale(dev: Device) -> float:
@@ -74,7 +79,7 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     _check(devices)
     n = len(devices)
     P = next(iter(sizes))
-    fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 18)
+    fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 22)
     change = np.any(fields[:, 1:] != fields[:, :-1], axis=2)
     sid = np.concatenate([np.zeros((n, 1), dtype=np.int64), np.cumsum(change, axis=1)], axis=1)
     nseg = int(sid.max()) + 1

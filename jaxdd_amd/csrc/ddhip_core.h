@@ -26,11 +26,15 @@
 #include <cstring>
 #define DDH_FN inline
 #define DDH_CONST static const
+#define DDH_GLOBAL
 using std::isfinite;
 #else
 #include <hip/hip_runtime.h>
 #define DDH_FN __device__ inline
 #define DDH_CONST __constant__
+// Pointers into device memory: global address space, so that a sweep compiled as a function
+// of its own loads and stores with global (not flat) instructions.
+#define DDH_GLOBAL __attribute__((address_space(1)))
 #endif
 
 #include "ddhip_layout.h"
@@ -63,7 +67,12 @@ struct Args {
     const double *sc;
     const double *V;
     double *Y;        // [NBUF][NP][3][n]
-    void *fac;        // T [NP][NFAC][n]
+    // Factors, one region per element type: lane-innermost arrays of different element sizes
+    // must not share memory (lane L's 4-byte element would overlay lane L/2's 8-byte one, and
+    // wavefronts run their phases and precisions concurrently).
+    double *fac64;    // [NP][NFAC][n] FP64 Newton factors
+    float *fac32;     // [NP][NFAC][n] FP32 Newton factors
+    double *pfac;     // [NP][4][n] Poisson's scalar factors
     double *fix;      // [NP][n] net fixed ions in use (frozen after the pre-bias)
     LaneState *st;    // [n]
     double *J;        // [n][nV]
@@ -136,7 +145,7 @@ DDH_FN void poisson_lane(const Args &a, int lane) {
     const double first = SC(SC_PRE) > 0 ? SC(SC_VPRE) : a.V[0];
     const double Vl = contact_V(a, lane, first, false), Vr = contact_V(a, lane, first, true);
     const double elec = SC(SC_ELEC);
-    double *cfac = (double *)a.fac;  // [NP][4][n] reinterpreted: c', d'(3)
+    double *cfac = a.pfac;  // [NP][4][n]: c', d' (3 right-hand sides)
 #define PF(j, f) cfac[((size_t)((j) - 1) * 4 + (f)) * a.n + lane]
 
     // the linear start and the ions' numbers at logs = 0 (for the first refit)
@@ -315,31 +324,39 @@ DDH_FN void poisson_lane(const Args &a, int lane) {
 // ---------------------------------------------------------------------------------------
 // The Newton sweeps.
 // ---------------------------------------------------------------------------------------
+// The values of one grid point a sweep carries from visit to visit.
 struct Pt {
-    double V, ln_n, ln_p, n, p, offn, offp, h, eps, Nc, mumax, mun, mup;
-    double R, dRn, dRp, G, fix, dop, an, ca, am, cm;
+    double V, n, p;        // the state
+    double offn, offp;     // V_gn - V, V_gp - V
+    double cn, cp;         // V_T mu of the interval after the point
+    double h, eps;
+    double sP, sCs;        // row scales (sCs / h is the continuity rows')
+    double R, dRn, dRp;    // recombination and its derivatives in ln n, ln p
+    double G, fix, dop;
+    double an, ca;         // mobile ion densities (0 where none or frozen)
+    double am, cm;         // the ions' region masks (0 once frozen)
 };
 
 struct Edge {  // Scharfetter-Gummel fluxes over one interval and their derivatives
     double Fn, Fp, gn, gp, an0, an1, ap0, ap1;
 };
 
-DDH_FN Edge edge_flux(const Pt &l, const Pt &r, double Vt, int bern) {
+// The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108).
+DDH_FN Edge edge_flux(const Pt &l, const Pt &r, double iVt, int bern) {
     Edge e;
-    double cn = Vt * l.mun, cp = Vt * l.mup;
-    double xn = ((r.V + r.offn) - (l.V + l.offn)) / Vt;
-    double xp = ((r.V + r.offp) - (l.V + l.offp)) / Vt;
+    double xn = ((r.V + r.offn) - (l.V + l.offn)) * iVt;
+    double xp = ((r.V + r.offp) - (l.V + l.offp)) * iVt;
     double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
     bernoulli_pair(xn, bern, Bnf, Bnb, dBnf, dBnb);
     bernoulli_pair(xp, bern, Bpf, Bpb, dBpf, dBpb);
-    e.Fn = cn * (Bnf * r.n - Bnb * l.n);
-    e.gn = cn * (dBnf * r.n + dBnb * l.n) / Vt;
-    e.an0 = -cn * Bnb * l.n;
-    e.an1 = cn * Bnf * r.n;
-    e.Fp = cp * (Bpf * l.p - Bpb * r.p);
-    e.gp = cp * (dBpf * l.p + dBpb * r.p) / Vt;
-    e.ap0 = cp * Bpf * l.p;
-    e.ap1 = -cp * Bpb * r.p;
+    e.an0 = -l.cn * Bnb * l.n;
+    e.an1 = l.cn * Bnf * r.n;
+    e.Fn = e.an1 + e.an0;
+    e.gn = l.cn * (dBnf * r.n + dBnb * l.n) * iVt;
+    e.ap0 = l.cp * Bpf * l.p;
+    e.ap1 = -l.cp * Bpb * r.p;
+    e.Fp = e.ap0 + e.ap1;
+    e.gp = l.cp * (dBpf * l.p + dBpb * r.p) * iVt;
     return e;
 }
 
@@ -350,60 +367,113 @@ struct SweepCtl {  // what one lane's sweeps need of its state
     int fresh, freeze_now, mobile, src1, src2, work;
 };
 
-// Fills the point values of interior point j from the state buffers (forming the start of a
-// new solve where `fresh`) and its segment; contacts from the boundary values.
-DDH_FN void load_point(const Args &a, int lane, int j, const Seg &s, const SweepCtl &c,
-                                  Pt &q, double Vt, bool need_rates) {
+// Per-lane views of the arrays a sweep walks: each advances by its row stride per point.
+typedef DDH_GLOBAL const double gcdouble;
+typedef DDH_GLOBAL double gdouble;
+typedef DDH_GLOBAL const uint8_t gcuint8;
+
+struct Lane {
+    gcdouble *pt;   // + j*PT_N*n + f*n
+    gcuint8 *sid;   // + j*n
+    gcdouble *seg;  // + (id*SEG_N + f)*n
+    gdouble *yw, *ys1, *ys2;  // state buffers: + (j-1)*3n + f*n
+    gdouble *fix;   // + (j-1)*n
+    size_t n;
+};
+
+DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
+    Lane L;
+    const size_t n = (size_t)a.n, ny = (size_t)a.NP * 3 * n;
+    L.n = n;
+    L.pt = (gcdouble *)(a.pt + lane);
+    L.sid = (gcuint8 *)(a.sid + lane);
+    L.seg = (gcdouble *)(a.seg + lane);
+    L.yw = (gdouble *)(a.Y + (size_t)c.work * ny + lane);
+    L.ys1 = (gdouble *)(a.Y + (size_t)c.src1 * ny + lane);
+    L.ys2 = (gdouble *)(a.Y + (size_t)c.src2 * ny + lane);
+    L.fix = (gdouble *)(a.fix + lane);
+    return L;
+}
+
+// The segment's fields one sweep needs, reloaded where the segment changes.
+struct SegE {
+    int id;
+    double eps, offn, offp, dop, kd, ni2, Nt, Cn, Cp, nt0, pt0, am, cm, mun, mup, sP, sCs;
+};
+
+DDH_FN void seg_elim(const Lane &L, int j, SegE &s) {
+    const int id = L.sid[(size_t)j * L.n];
+    if (id == s.id) return;
+    s.id = id;
+    gcdouble *q = L.seg + (size_t)id * SEG_N * L.n;
+    s.eps = q[SEG_EPS * L.n]; s.offn = q[SEG_OFFN * L.n]; s.offp = q[SEG_OFFP * L.n];
+    s.dop = q[SEG_DOP * L.n]; s.kd = q[SEG_KD * L.n]; s.ni2 = q[SEG_NI2 * L.n];
+    s.Nt = q[SEG_NT * L.n]; s.Cn = q[SEG_CN * L.n]; s.Cp = q[SEG_CP * L.n];
+    s.nt0 = q[SEG_NT0 * L.n]; s.pt0 = q[SEG_PT0 * L.n]; s.am = q[SEG_AM * L.n];
+    s.cm = q[SEG_CM * L.n]; s.mun = q[SEG_MUN * L.n]; s.mup = q[SEG_MUP * L.n];
+    s.sP = q[SEG_SP * L.n]; s.sCs = q[SEG_SCS * L.n];
+}
+
+// Point j's values: interior points from the state (forming a solve's start where `fresh`),
+// the contacts from their boundary values.
+DDH_FN void load_point(const Args &a, int lane, const Lane &L, int j, const SegE &s,
+                       const SweepCtl &c, double Vt, double iVt, Pt &q) {
     const int NP = a.NP;
-    q.offn = s.v[SEG_OFFN]; q.offp = s.v[SEG_OFFP]; q.eps = s.v[SEG_EPS]; q.Nc = s.v[SEG_NC];
-    q.mun = s.v[SEG_MUN]; q.mup = s.v[SEG_MUP];
-    q.mumax = q.mun > q.mup ? q.mun : q.mup;
-    q.h = PT(j, PT_H);
-    q.dop = s.v[SEG_DOP];
-    q.am = c.mobile ? s.v[SEG_AM] : 0.0;
-    q.cm = c.mobile ? s.v[SEG_CM] : 0.0;
+    const size_t n = L.n;
+    q.offn = s.offn; q.offp = s.offp; q.eps = s.eps; q.dop = s.dop;
+    q.cn = Vt * s.mun; q.cp = Vt * s.mup;
+    q.sP = s.sP; q.sCs = s.sCs;
+    q.h = L.pt[((size_t)j * PT_N + PT_H) * n];
+    q.am = c.mobile ? s.am : 0.0;
+    q.cm = c.mobile ? s.cm : 0.0;
+    double ln_n, ln_p;
     if (j == 0 || j == NP + 1) {
         bool right = j == NP + 1;
         q.V = contact_V(a, lane, c.Vint, right);
-        q.ln_n = right ? SC(SC_LNR) : SC(SC_LNL);
-        q.ln_p = right ? SC(SC_LPR) : SC(SC_LPL);
+        ln_n = right ? SC(SC_LNR) : SC(SC_LNL);
+        ln_p = right ? SC(SC_LPR) : SC(SC_LPL);
         q.G = 0.0; q.fix = 0.0;
+        q.n = exp(ln_n);
+        q.p = exp(ln_p);
+        q.R = q.dRn = q.dRp = 0.0;
     } else {
+        const size_t o = (size_t)(j - 1) * 3 * n;
         double y0, y1, y2;
         if (c.fresh) {
-            y0 = YB(c.src1, j, 0); y1 = YB(c.src1, j, 1); y2 = YB(c.src1, j, 2);
+            y0 = L.ys1[o]; y1 = L.ys1[o + n]; y2 = L.ys1[o + 2 * n];
             if (c.freeze_now) {
                 // the pre-bias's mobile ions as fixed ones (src/jaxdd/batch.py:859-872)
-                double V = y0;
-                double fx = PT(j, PT_FIX) + s.v[SEG_AM] * exp(c.lstart[0] + V / Vt) -
-                            s.v[SEG_CM] * exp(c.lstart[1] - V / Vt);
-                FIX(j) = fx;
+                double fx = L.pt[((size_t)j * PT_N + PT_FIX) * n] + s.am * exp(c.lstart[0] + y0 * iVt) -
+                            s.cm * exp(c.lstart[1] - y0 * iVt);
+                L.fix[(size_t)(j - 1) * n] = fx;
             }
             if (c.ratio != 0.0) {
                 double capv = CAP_V * Vt;
-                y0 += clipd(c.ratio * (y0 - YB(c.src2, j, 0)), -capv, capv);
-                y1 += clipd(c.ratio * (y1 - YB(c.src2, j, 1)), -CAP_N, CAP_N);
-                y2 += clipd(c.ratio * (y2 - YB(c.src2, j, 2)), -CAP_N, CAP_N);
+                y0 += clipd(c.ratio * (y0 - L.ys2[o]), -capv, capv);
+                y1 += clipd(c.ratio * (y1 - L.ys2[o + n]), -CAP_N, CAP_N);
+                y2 += clipd(c.ratio * (y2 - L.ys2[o + 2 * n]), -CAP_N, CAP_N);
             }
-            YB(c.work, j, 0) = y0; YB(c.work, j, 1) = y1; YB(c.work, j, 2) = y2;
+            L.yw[o] = y0; L.yw[o + n] = y1; L.yw[o + 2 * n] = y2;
         } else {
-            y0 = YB(c.work, j, 0); y1 = YB(c.work, j, 1); y2 = YB(c.work, j, 2);
+            y0 = L.yw[o]; y1 = L.yw[o + n]; y2 = L.yw[o + 2 * n];
         }
-        q.V = y0; q.ln_n = y1; q.ln_p = y2;
-        q.G = PT(j, PT_G);
-        q.fix = FIX(j);
+        q.V = y0;
+        q.G = L.pt[((size_t)j * PT_N + PT_G) * n];
+        q.fix = L.fix[(size_t)(j - 1) * n];
+        q.n = exp(y1);
+        q.p = exp(y2);
+        q.R = recombination(q.n, q.p, s.kd, s.ni2, s.Nt, s.Cn, s.Cp, s.nt0, s.pt0, q.dRn, q.dRp);
     }
-    q.n = exp(q.ln_n);
-    q.p = exp(q.ln_p);
-    q.an = q.am != 0.0 ? q.am * exp(c.logs[0] + q.V / Vt) : 0.0;
-    q.ca = q.cm != 0.0 ? q.cm * exp(c.logs[1] - q.V / Vt) : 0.0;
-    if (need_rates && j >= 1 && j <= NP) {
-        q.R = recombination(q.n, q.p, s.v[SEG_KD], s.v[SEG_NI2], s.v[SEG_NT], s.v[SEG_CN],
-                            s.v[SEG_CP], s.v[SEG_NT0], s.v[SEG_PT0], q.dRn, q.dRp);
-    } else {
-        q.R = q.dRn = q.dRp = 0.0;
-    }
+    q.an = q.am != 0.0 ? q.am * exp(c.logs[0] + q.V * iVt) : 0.0;
+    q.ca = q.cm != 0.0 ? q.cm * exp(c.logs[1] - q.V * iVt) : 0.0;
 }
+
+template <typename T>
+DDH_FN T *factors(const Args &a);
+template <>
+DDH_FN double *factors<double>(const Args &a) { return a.fac64; }
+template <>
+DDH_FN float *factors<float>(const Args &a) { return a.fac32; }
 
 template <typename T>
 DDH_FN void to_t(const double *in, T *out, int m) {
@@ -411,22 +481,33 @@ DDH_FN void to_t(const double *in, T *out, int m) {
     for (int q = 0; q < m; ++q) out[q] = (T)in[q];
 }
 
-// Elimination sweep (right to left). Returns the ion step z through c/out.
+#ifdef DDH_CPU_EMULATION
+#define DDH_SWEEP
+#else
+// Each sweep variant is its own function, so that its register allocation is its own and the
+// state machine's values wait in memory, not in registers, while a sweep runs.
+#define DDH_SWEEP __device__ __attribute__((noinline))
+#endif
+
+// Elimination sweep (right to left). Returns the ion step z.
 template <typename T, int K>
-DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) {
+DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z) {
     const int NP = a.NP;
-    const double Vt = SC(SC_VT), L = SC(SC_L), L2 = L * L;
+    const double Vt = SC(SC_VT), iVt = 1.0 / Vt, L = SC(SC_L), L2 = L * L;
     // the ions' totals (scaled in the start's stages); none once they are frozen (the slots
     // are then held, src/jaxdd/batch.py:859-872)
-    const double tot[2] = {c.mobile ? SC(SC_AT) * c.share : 0.0, c.mobile ? SC(SC_CT) * c.share : 0.0};
-    T *fac = (T *)a.fac;
-#define FAC(j, f) fac[((size_t)((j) - 1) * NFAC + (f)) * a.n + lane]
-    Seg sg; sg.id = -1;
-    Pt right, mid, cur;
-    seg_at(a, lane, NP + 1, sg);
-    load_point(a, lane, NP + 1, sg, c, mid, Vt, false);
+    const double tot0 = c.mobile ? SC(SC_AT) * c.share : 0.0;
+    const double tot1 = c.mobile ? SC(SC_CT) * c.share : 0.0;
+    const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
+    const Lane Lv = lane_view(a, lane, c);
+    const size_t n = Lv.n;
+    DDH_GLOBAL T *fac = (DDH_GLOBAL T *)(factors<T>(a) + lane);
+    SegE sg; sg.id = -1;
+    Pt mid, cur;
+    double rV = 0.0, ram = 0.0, rcm = 0.0;  // the row's right neighbour
+    seg_elim(Lv, NP + 1, sg);
+    load_point(a, lane, Lv, NP + 1, sg, c, Vt, iVt, mid);
     Edge eR = {0, 0, 0, 0, 0, 0, 0, 0};
-    right = mid;  // unused until two points are in
     T cp[9], dp[3 * K];
 #pragma unroll
     for (int q = 0; q < 9; ++q) cp[q] = T(0);
@@ -434,37 +515,40 @@ DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) 
     for (int q = 0; q < 3 * K; ++q) dp[q] = T(0);
     double gam[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     double accF[2] = {0, 0}, accU[2][2] = {{0, 0}, {0, 0}};
-    double num[2] = {0, 0};
+    double num0 = 0.0, num1 = 0.0;
     for (int j = NP; j >= 0; --j) {
-        seg_at(a, lane, j, sg);
-        load_point(a, lane, j, sg, c, cur, Vt, true);
-        Edge eL = edge_flux(cur, mid, Vt, a.bern);
+        seg_elim(Lv, j, sg);
+        load_point(a, lane, Lv, j, sg, c, Vt, iVt, cur);
+        const Edge eL = edge_flux(cur, mid, iVt, a.bern);
         if constexpr (K > 1) {
-            double hLj = cur.h * L;
-            num[0] += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
-            num[1] += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
+            const double hLj = cur.h * L;
+            num0 += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
+            num1 += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
         }
         if (j + 1 <= NP) {
-            // row i = j + 1: left = cur, mid = mid, right = right, edges eL (i-1,i), eR (i,i+1)
+            // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
             const int i = j + 1;
-            double hm = cur.h, hp = mid.h, hs = hp + hm;
-            double fac_ = 0.5 * L2 * hp * hm * hs;
-            double fac_m = (mid.eps - cur.eps) / (hm * hm * L2) -
-                           2 * mid.eps * (1 / (hp * hs * L2) + 1 / (hm * hs * L2));
-            double fac_l = 2 * mid.eps / (hm * hs * L2) - (mid.eps - cur.eps) / (hm * hm * L2);
-            double fac_u = 2 * mid.eps / (hp * hs * L2);
-            double sP = 1.0 / (Q_E * mid.Nc);
-            double sC = 1.0 / (mid.mumax * Vt * mid.Nc * hp);
-            double gen = mid.G * c.g_frac;
-            double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
-            double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
-            double pois = fac_l * cur.V + fac_m * mid.V + fac_u * right.V -
-                          Q_E * (mid.n - mid.p + mid.dop + c.share * mid.fix + mid.an - mid.ca);
+            const double hm = cur.h, hp = mid.h, hs = hp + hm;
+            const double D = 1.0 / (hp * hm * hs * L2);
+            const double fac_ = 0.5 * L2 * hp * hm * hs;
+            const double fu = 2 * mid.eps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
+            const double fl0 = 2 * mid.eps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
+            const double deps = mid.eps - cur.eps;     // the jump of eps between layers
+            const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
+            const double fac_m = jump - (fu + fl0);
+            const double fac_l = fl0 - jump;
+            const double sP = mid.sP;
+            const double sC = mid.sCs * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
+            const double gen = mid.G * c.g_frac;
+            const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
+            const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
+            const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
+                                Q_E * (mid.n - mid.p + mid.dop + c.share * mid.fix + mid.an - mid.ca);
             double A[9], B[9], C[9], r[3 * K];
             A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
             A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
             A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
-            B[0] = (fac_m - Q_E * (mid.an + mid.ca) / Vt) * sP;
+            B[0] = (fac_m - Q_E * (mid.an + mid.ca) * iVt) * sP;
             B[1] = -Q_E * mid.n * sP;
             B[2] = Q_E * mid.p * sP;
             B[3] = (-hp * eL.gn - hm * eR.gn) * sC;
@@ -474,18 +558,19 @@ DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) 
             B[7] = (-fac_ * mid.dRn) * sC;
             B[8] = (hp * eL.ap1 - hm * eR.ap0 - fac_ * mid.dRp) * sC;
             if (i == NP) {  // the coupling to the contact is not an unknown
-                C[0] = C[1] = C[2] = C[3] = C[4] = C[5] = C[6] = C[7] = C[8] = 0.0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) C[q] = 0.0;
             } else {
-                C[0] = fac_u * sP; C[1] = 0.0; C[2] = 0.0;
+                C[0] = fu * sP; C[1] = 0.0; C[2] = 0.0;
                 C[3] = hm * eR.gn * sC; C[4] = hm * eR.an1 * sC; C[5] = 0.0;
                 C[6] = -hm * eR.gp * sC; C[7] = 0.0; C[8] = -hm * eR.ap1 * sC;
             }
             r[0 * K] = -pois * sP; r[1 * K] = -elec * sC; r[2 * K] = -hole * sC;
             if constexpr (K > 1) {
                 r[0 * K + 1] = -Q_E * mid.an * sP; r[0 * K + 2] = Q_E * mid.ca * sP;
-                r[1 * K + 1] = r[1 * K + 2] = r[2 * K + 1] = r[2 * K + 2] = 0.0;
+                r[1 * K + 1] = 0.0; r[1 * K + 2] = 0.0; r[2 * K + 1] = 0.0; r[2 * K + 2] = 0.0;
             }
-            if (sizeof(T) == 4) {
+            if constexpr (sizeof(T) == 4) {
                 // row equilibration by powers of two (exact) before the rounding to FP32
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {
@@ -499,13 +584,13 @@ DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) 
                     if (mx > 0 && mx < INFINITY) {
                         int e;
                         frexp(mx, &e);
-                        double s = ldexp(1.0, -e);
+                        const double sc = ldexp(1.0, -e);
 #pragma unroll
                         for (int col = 0; col < 3; ++col) {
-                            A[3 * row + col] *= s; B[3 * row + col] *= s; C[3 * row + col] *= s;
+                            A[3 * row + col] *= sc; B[3 * row + col] *= sc; C[3 * row + col] *= sc;
                         }
 #pragma unroll
-                        for (int q = 0; q < K; ++q) r[row * K + q] *= s;
+                        for (int q = 0; q < K; ++q) r[row * K + q] *= sc;
                     }
                 }
             }
@@ -534,25 +619,23 @@ DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) 
                 for (int q = 0; q < K; ++q)
                     dn_[p * K + q] = Mi[3 * p + 0] * rh[0 * K + q] + Mi[3 * p + 1] * rh[1 * K + q] + Mi[3 * p + 2] * rh[2 * K + q];
             }
+            DDH_GLOBAL T *f = fac + (size_t)(i - 1) * NFAC * n;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) FAC(i, q) = cn_[q];
+            for (int q = 0; q < 9; ++q) f[q * n] = cn_[q];
 #pragma unroll
-            for (int q = 0; q < 3 * K; ++q) FAC(i, 9 + q) = dn_[q];
+            for (int q = 0; q < 3 * K; ++q) f[(9 + q) * n] = dn_[q];
             if constexpr (K > 1) {
                 // gamma_i = W_i - c'_{i+1}^T gamma_{i+1}; W_i only in the potential's row
-                double hLm = cur.h * L, hLp = mid.h * L;
-                double w[2] = {0.0, 0.0};
-                if (tot[0] > 0)
-                    w[0] = 0.5 * (mid.am * cur.am * hLm + right.am * mid.am * hLp) * mid.an * (1.0 / Vt) / tot[0];
-                if (tot[1] > 0)
-                    w[1] = 0.5 * (mid.cm * cur.cm * hLm + right.cm * mid.cm * hLp) * mid.ca * (-1.0 / Vt) / tot[1];
+                const double hLm = cur.h * L, hLp = mid.h * L;
+                const double w0 = 0.5 * (mid.am * cur.am * hLm + ram * mid.am * hLp) * mid.an * iVt * itot0;
+                const double w1 = -0.5 * (mid.cm * cur.cm * hLm + rcm * mid.cm * hLp) * mid.ca * iVt * itot1;
                 double ng[3][2];
 #pragma unroll
-                for (int row = 0; row < 3; ++row)
-#pragma unroll
-                    for (int k = 0; k < 2; ++k)
-                        ng[row][k] = (row == 0 ? w[k] : 0.0) -
-                                     ((double)cp[0 + row] * gam[0][k] + (double)cp[3 + row] * gam[1][k] + (double)cp[6 + row] * gam[2][k]);
+                for (int row = 0; row < 3; ++row) {
+                    const double c0 = (double)cp[0 + row], c1 = (double)cp[3 + row], c2 = (double)cp[6 + row];
+                    ng[row][0] = (row == 0 ? w0 : 0.0) - (c0 * gam[0][0] + c1 * gam[1][0] + c2 * gam[2][0]);
+                    ng[row][1] = (row == 0 ? w1 : 0.0) - (c0 * gam[0][1] + c1 * gam[1][1] + c2 * gam[2][1]);
+                }
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     double sF = 0.0, s1 = 0.0, s2 = 0.0;
@@ -571,112 +654,120 @@ DDH_FN void elim_sweep(const Args &a, int lane, const SweepCtl &c, double z[2]) 
 #pragma unroll
             for (int q = 0; q < 3 * K; ++q) dp[q] = dn_[q];
         }
-        right = mid;
+        rV = mid.V; ram = mid.am; rcm = mid.cm;
         mid = cur;
         eR = eL;
     }
-#undef FAC
     z[0] = z[1] = 0.0;
     if constexpr (K > 1) {
-        double g[2], D[2];
-        for (int k = 0; k < 2; ++k) {
-            bool has = tot[k] > 0;
-            g[k] = has ? num[k] / tot[k] - 1.0 : 0.0;
-            D[k] = has ? num[k] / tot[k] : 1.0;  // a slot without ions is held
-        }
-        z[0] = -g[0] - accF[0];
-        z[1] = -g[1] - accF[1];
-        solve2(D[0] - accU[0][0], -accU[0][1], -accU[1][0], D[1] - accU[1][1], z[0], z[1]);
+        // the constraints (solver.py:1255-1265) and their Schur complement; a slot without
+        // ions is held (an identity row)
+        const double g0 = tot0 > 0 ? num0 * itot0 - 1.0 : 0.0;
+        const double g1 = tot1 > 0 ? num1 * itot1 - 1.0 : 0.0;
+        const double D0 = tot0 > 0 ? num0 * itot0 : 1.0;
+        const double D1 = tot1 > 0 ? num1 * itot1 : 1.0;
+        z[0] = -g0 - accF[0];
+        z[1] = -g1 - accF[1];
+        solve2(D0 - accU[0][0], -accU[0][1], -accU[1][0], D1 - accU[1][1], z[0], z[1]);
     }
 }
 
 // Back-substitution sweep (left to right): the step, the new state, its size and current.
 template <typename T, int K>
-DDH_FN void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const double z[2],
-                              double &size_out, double &J_out, int &finite_out) {
+DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const double *z,
+                             double *out /* size, J, finite */) {
     const int NP = a.NP;
-    const double Vt = SC(SC_VT), nfl = SC(SC_NFLOOR), lnfloor = log(SC(SC_NFLOOR));
+    const double Vt = SC(SC_VT), iVt = 1.0 / Vt, nfl = SC(SC_NFLOOR), lnfloor = log(nfl);
     const double capv = CAP_V * Vt;
-    T *fac = (T *)a.fac;
-#define FAC(j, f) fac[((size_t)((j) - 1) * NFAC + (f)) * a.n + lane]
-    Seg sg; sg.id = -1;
-    seg_at(a, lane, 0, sg);
+    const Lane Lv = lane_view(a, lane, c);
+    const size_t n = Lv.n;
+    DDH_GLOBAL const T *fac = (DDH_GLOBAL const T *)(factors<T>(a) + lane);
+    const double z0 = z[0], z1 = z[1];
+    int id = -1;
+    double offn = 0, offp = 0, kd = 0, ni2 = 0, Nt = 0, Cn = 0, Cp = 0, nt0 = 0, pt0 = 0;
+    double imun = 0, imup = 0;
+    auto seg = [&](int j) {
+        const int s = Lv.sid[(size_t)j * n];
+        if (s == id) return;
+        id = s;
+        gcdouble *q = Lv.seg + (size_t)s * SEG_N * n;
+        offn = q[SEG_OFFN * n]; offp = q[SEG_OFFP * n]; kd = q[SEG_KD * n]; ni2 = q[SEG_NI2 * n];
+        Nt = q[SEG_NT * n]; Cn = q[SEG_CN * n]; Cp = q[SEG_CP * n]; nt0 = q[SEG_NT0 * n];
+        pt0 = q[SEG_PT0 * n]; imun = q[SEG_IMUN * n]; imup = q[SEG_IMUP * n];
+    };
+    seg(0);
     // the left contact
-    double V0 = contact_V(a, lane, c.Vint, false);
-    double Vg0n = V0 + sg.v[SEG_OFFN], Vg0p = V0 + sg.v[SEG_OFFP];
-    double wn_prev = exp(-Vg0n / Vt) / sg.v[SEG_MUN];
-    double wp_prev = exp(Vg0p / Vt) / sg.v[SEG_MUP];
-    double endn0 = exp(SC(SC_LNL)) * exp(-Vg0n / Vt);
-    double endp0 = exp(SC(SC_LPL)) * exp(Vg0p / Vt);
-    double U_prev = 0.0, iu_prev = 0.0, x_prev = PT(0, PT_X);
+    const double V0 = contact_V(a, lane, c.Vint, false);
+    const double Vg0n = V0 + offn, Vg0p = V0 + offp;
+    double wn_prev = exp(-Vg0n * iVt) * imun;
+    double wp_prev = exp(Vg0p * iVt) * imup;
+    const double endn0 = exp(SC(SC_LNL) - Vg0n * iVt);
+    const double endp0 = exp(SC(SC_LPL) + Vg0p * iVt);
+    double U_prev = 0.0, iu_prev = 0.0, x_prev = Lv.pt[PT_X * n];
     double single_n = 0.0, double_n = 0.0, single_p = 0.0, double_p = 0.0;
-    double xs[3] = {0.0, 0.0, 0.0};
+    double xs0 = 0.0, xs1 = 0.0, xs2 = 0.0;
     double size = 0.0;
     int finite = 1;
     for (int j = 1; j <= NP; ++j) {
-        seg_at(a, lane, j, sg);
+        seg(j);
+        DDH_GLOBAL const T *f = fac + (size_t)(j - 1) * NFAC * n;
         double dF[3];
 #pragma unroll
         for (int row = 0; row < 3; ++row) {
-            double v = (double)FAC(j, 9 + row * K + 0);
-            if constexpr (K > 1) v -= (double)FAC(j, 9 + row * K + 1) * z[0] + (double)FAC(j, 9 + row * K + 2) * z[1];
+            double v = (double)f[(9 + row * K + 0) * n];
+            if constexpr (K > 1) v -= (double)f[(9 + row * K + 1) * n] * z0 + (double)f[(9 + row * K + 2) * n] * z1;
             dF[row] = v;
         }
-        double xn[3];
-#pragma unroll
-        for (int row = 0; row < 3; ++row)
-            xn[row] = dF[row] - ((double)FAC(j, 3 * row + 0) * xs[0] + (double)FAC(j, 3 * row + 1) * xs[1] +
-                                 (double)FAC(j, 3 * row + 2) * xs[2]);
-        xs[0] = xn[0]; xs[1] = xn[1]; xs[2] = xn[2];
-        double y0 = YB(c.work, j, 0), y1 = YB(c.work, j, 1), y2 = YB(c.work, j, 2);
-        double st0 = clipd(xn[0], -capv, capv);
-        double st[2];
-        double yo[2] = {y1, y2};
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            double d = xn[1 + q];
-            st[q] = clipd(d, -CAP_N, CAP_N);
-            if (yo[q] < lnfloor) {  // density_steps (src/jaxdd/solver.py:74-88)
-                double in_n = d > -1 ? log1p(d) : -CAP_N;
-                st[q] = clipd(in_n, -FLOOR_DROP_CAP, CAP_N);
-            }
+        const double x0 = dF[0] - ((double)f[0] * xs0 + (double)f[n] * xs1 + (double)f[2 * n] * xs2);
+        const double x1 = dF[1] - ((double)f[3 * n] * xs0 + (double)f[4 * n] * xs1 + (double)f[5 * n] * xs2);
+        const double x2 = dF[2] - ((double)f[6 * n] * xs0 + (double)f[7 * n] * xs1 + (double)f[8 * n] * xs2);
+        xs0 = x0; xs1 = x1; xs2 = x2;
+        gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
+        double y0 = y[0], y1 = y[n], y2 = y[2 * n];
+        const double st0 = clipd(x0, -capv, capv);
+        double st1 = clipd(x1, -CAP_N, CAP_N), st2 = clipd(x2, -CAP_N, CAP_N);
+        if (y1 < lnfloor) {  // density_steps (src/jaxdd/solver.py:74-88)
+            const double in_n = x1 > -1 ? log1p(x1) : -CAP_N;
+            st1 = clipd(in_n, -FLOOR_DROP_CAP, CAP_N);
         }
-        y0 += st0; y1 += st[0]; y2 += st[1];
-        YB(c.work, j, 0) = y0; YB(c.work, j, 1) = y1; YB(c.work, j, 2) = y2;
+        if (y2 < lnfloor) {
+            const double in_n = x2 > -1 ? log1p(x2) : -CAP_N;
+            st2 = clipd(in_n, -FLOOR_DROP_CAP, CAP_N);
+        }
+        y0 += st0; y1 += st1; y2 += st2;
+        y[0] = y0; y[n] = y1; y[2 * n] = y2;
         if (!(isfinite(y0) && isfinite(y1) && isfinite(y2))) finite = 0;
-        double n = exp(y1), p = exp(y2);
-        size = nmax(size, fabs(st0) / Vt);
-        size = nmax(size, fabs(st[0]) * (n / (n + nfl)));
-        size = nmax(size, fabs(st[1]) * (p / (p + nfl)));
+        const double nn = exp(y1), pp = exp(y2);
+        size = nmax(size, fabs(st0) * iVt);
+        size = nmax(size, fabs(st1) * (nn / (nn + nfl)));
+        size = nmax(size, fabs(st2) * (pp / (pp + nfl)));
         // the integral current of the new state (src/jaxdd/batch.py:321-373)
-        double R = recombination_value(n, p, sg.v[SEG_KD], sg.v[SEG_NI2], sg.v[SEG_NT],
-                                       sg.v[SEG_CN], sg.v[SEG_CP], sg.v[SEG_NT0], sg.v[SEG_PT0]);
-        double U = R - PT(j, PT_G) * c.g_frac;
-        double wn = exp(-(y0 + sg.v[SEG_OFFN]) / Vt) / sg.v[SEG_MUN];
-        double wp = exp((y0 + sg.v[SEG_OFFP]) / Vt) / sg.v[SEG_MUP];
-        double xj = PT(j, PT_X);
-        double dx = xj - x_prev;
-        double iu = iu_prev + 0.5 * dx * (U_prev + U);
+        const double R = recombination_value(nn, pp, kd, ni2, Nt, Cn, Cp, nt0, pt0);
+        const double U = R - Lv.pt[((size_t)j * PT_N + PT_G) * n] * c.g_frac;
+        const double wn = exp(-(y0 + offn) * iVt) * imun;
+        const double wp = exp((y0 + offp) * iVt) * imup;
+        const double xj = Lv.pt[((size_t)j * PT_N + PT_X) * n];
+        const double dx = xj - x_prev;
+        const double iu = iu_prev + 0.5 * dx * (U_prev + U);
         single_n += 0.5 * dx * (wn_prev + wn);
         double_n += 0.5 * dx * (wn_prev * iu_prev + wn * iu);
         single_p += 0.5 * dx * (wp_prev + wp);
         double_p += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
     }
-    seg_at(a, lane, NP + 1, sg);
-    double VN = contact_V(a, lane, c.Vint, true);
-    double endnN = exp(SC(SC_LNR)) * exp(-(VN + sg.v[SEG_OFFN]) / Vt);
-    double endpN = exp(SC(SC_LPR)) * exp((VN + sg.v[SEG_OFFP]) / Vt);
+    seg(NP + 1);
+    const double VN = contact_V(a, lane, c.Vint, true);
+    const double endnN = exp(SC(SC_LNR) - (VN + offn) * iVt);
+    const double endpN = exp(SC(SC_LPR) + (VN + offp) * iVt);
     // K = (V_T (sn dens_N ev_N - sn dens_0 ev_0) - sn double) / single, sn = -1 electrons
-    double Kn = (Vt * (-endnN + endn0) + double_n) / single_n;
-    double Kp = (Vt * (endpN - endp0) - double_p) / single_p;
+    const double Kn = (Vt * (-endnN + endn0) + double_n) / single_n;
+    const double Kp = (Vt * (endpN - endp0) - double_p) / single_p;
     // The cumulative recombination terms of the two carriers cancel exactly in the sum (the
     // same U with opposite signs), so the device average is q (K_n + K_p), sign turned with
     // the anode on the left.
-    J_out = -SC(SC_ELEC) * Q_E * (Kn + Kp);
-    size_out = size;
-    finite_out = finite;
-#undef FAC
+    out[0] = size;
+    out[1] = -SC(SC_ELEC) * Q_E * (Kn + Kp);
+    out[2] = finite;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -925,21 +1016,19 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions) {
     c.lstart[0] = S.lstart[0]; c.lstart[1] = S.lstart[1];
     c.fresh = S.fresh; c.freeze_now = S.freeze_now && S.fresh; c.mobile = S.mobile_now;
     c.src1 = S.b_src1; c.src2 = S.b_src2; c.work = S.b_work;
-    double z[2];
+    double z[2], out[3];
     if (ions) elim_sweep<T, 3>(a, lane, c, z);
     else elim_sweep<T, 1>(a, lane, c, z);
-    double size, J;
-    int fin;
     c.fresh = 0;
-    if (ions) backsub_sweep<T, 3>(a, lane, c, z, size, J, fin);
-    else backsub_sweep<T, 1>(a, lane, c, z, size, J, fin);
+    if (ions) backsub_sweep<T, 3>(a, lane, c, z, out);
+    else backsub_sweep<T, 1>(a, lane, c, z, out);
     double dl0 = clipd(z[0], -ION_LOG_CAP, ION_LOG_CAP);
     double dl1 = clipd(z[1], -ION_LOG_CAP, ION_LOG_CAP);
     S.lw[0] += dl0;
     S.lw[1] += dl1;
-    size = nmax(size, nmax(fabs(dl0), fabs(dl1)));
-    S.fin_work = fin;
-    after_step(a, lane, S, size, J);
+    const double size = nmax(out[0], nmax(fabs(dl0), fabs(dl1)));
+    S.fin_work = out[2] != 0.0;
+    after_step(a, lane, S, size, out[1]);
 }
 
 DDH_FN bool lane_active(const Args &a, int lane) { return a.st[lane].phase != PH_DONE; }

@@ -41,6 +41,10 @@ enum SegField {
     SEG_MUP,
     SEG_EC,       // conduction band edge (eV), for the linear start
     SEG_NI,       // intrinsic density, for the linear start
+    SEG_SP,       // Poisson's row scale 1/(q N_c)
+    SEG_SCS,      // the continuity rows' scale times h: 1/(max(mu_n, mu_p) V_T N_c)
+    SEG_IMUN,     // 1/mu_n, 1/mu_p of the interval after the point (the current's integrals)
+    SEG_IMUP,
     SEG_N
 };
 

@@ -173,18 +173,19 @@ extern "C" const char *ddh_last_error(void) { return g_err; }
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct WsLayout {
-    size_t y, fac, fix, st, total;
+    size_t y, fac64, fac32, pfac, fix, st, total;
 };
 
+// One region per array: the state buffers, the FP64 and FP32 Newton factors, Poisson's
+// factors, the fixed ions in use and the lanes' state machines.
 static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision) {
     WsLayout w;
-    (void)precision;  // a mixed solve may switch a lane to FP64 factors: room for those
-    size_t tsize = 8;
     size_t off = 0;
     w.y = off; off = align256(off + (size_t)ddh::NBUF * NP * 3 * n * sizeof(double));
-    size_t facb = (size_t)NP * ddh::NFAC * n * tsize;
-    size_t poisb = (size_t)NP * 4 * n * sizeof(double);
-    w.fac = off; off = align256(off + (facb > poisb ? facb : poisb));
+    w.fac64 = off; off = align256(off + (size_t)NP * ddh::NFAC * n * sizeof(double));
+    w.fac32 = off;
+    if (precision == DDH_MIXED) off = align256(off + (size_t)NP * ddh::NFAC * n * sizeof(float));
+    w.pfac = off; off = align256(off + (size_t)NP * 4 * n * sizeof(double));
     w.fix = off; off = align256(off + (size_t)NP * n * sizeof(double));
     w.st = off; off = align256(off + (size_t)n * sizeof(ddh::LaneState));
     w.total = off;
@@ -218,7 +219,9 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
     a.n = b->n; a.NP = b->NP; a.nV = nV; a.nseg = b->nseg;
     a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc; a.V = V;
     a.Y = (double *)(base + w.y);
-    a.fac = (void *)(base + w.fac);
+    a.fac64 = (double *)(base + w.fac64);
+    a.fac32 = (float *)(base + w.fac32);
+    a.pfac = (double *)(base + w.pfac);
     a.fix = (double *)(base + w.fix);
     a.st = (ddh::LaneState *)(base + w.st);
     a.J = J; a.ok = ok; a.its = its; a.status = status;

@@ -1,9 +1,9 @@
 // Test-only CPU build of the HIP kernels' per-lane logic (jaxdd_amd/csrc/ddhip_core.h).
 //
-// It replays each 64-lane wavefront of the persistent solve kernel lane by lane, with the
-// same "any lane active" / "any lane with mobile ions" wave votes, so that the kernels'
-// algorithm can be debugged and compared with the oracle on a host without a GPU. Nothing in
-// the product loads it.
+// It replays the 64-lane wavefronts of the persistent solve kernel lane by lane, with the
+// same "any lane active" / "any lane with mobile ions" / "any lane needs FP64" wave votes and
+// the wavefronts interleaved step by step, so that the kernels' algorithm can be debugged and
+// compared with the oracle on a host without a GPU. Nothing in the product loads it.
 #define DDH_CPU_EMULATION 1
 #include "ddhip_core.h"
 
@@ -12,14 +12,25 @@
 
 using namespace ddh;
 
+// The wavefronts of the GPU run concurrently, each at its own phase (Poisson's start or a
+// Newton step) and precision: here they take turns, one unit of work each per round, so that
+// memory one wavefront's phase shares with another's would show here too.
 static void run(Args &a) {
-    for (int w0 = 0; w0 < a.n; w0 += 64) {
-        int w1 = w0 + 64 < a.n ? w0 + 64 : a.n;
-        for (int lane = w0; lane < w1; ++lane) {
-            poisson_lane(a, lane);
-            start_lane(a, lane, a.st[lane]);
-        }
-        for (;;) {
+    const int nw = (a.n + 63) / 64;
+    std::vector<char> started(nw, 0), done(nw, 0);
+    int remaining = nw;
+    while (remaining > 0) {
+        for (int w = 0; w < nw; ++w) {
+            if (done[w]) continue;
+            int w0 = w * 64, w1 = w0 + 64 < a.n ? w0 + 64 : a.n;
+            if (!started[w]) {
+                for (int lane = w0; lane < w1; ++lane) {
+                    poisson_lane(a, lane);
+                    start_lane(a, lane, a.st[lane]);
+                }
+                started[w] = 1;
+                continue;
+            }
             bool any = false, ions = false, f64 = !a.mixed;
             for (int lane = w0; lane < w1; ++lane) {
                 bool act = lane_active(a, lane);
@@ -27,7 +38,7 @@ static void run(Args &a) {
                 ions |= act && lane_mobile(a, lane);
                 f64 |= act && lane_needs64(a, lane);
             }
-            if (!any) break;
+            if (!any) { done[w] = 1; --remaining; continue; }
             for (int lane = w0; lane < w1; ++lane) {
                 if (!lane_active(a, lane)) continue;
                 if (f64) newton_lane_step<double>(a, lane, ions);
@@ -37,21 +48,32 @@ static void run(Args &a) {
     }
 }
 
+static int g_poison = 0;  // fill the workspace with this byte before a solve (0: zeros)
+extern "C" void emu_set_poison(int byte) { g_poison = byte; }
+
 extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                                const double *seg, const double *sc, const double *V, int nV,
                                double tol, int max_iter, int start_iter, int strict, int bern,
                                int precision, int jv, double *J, uint8_t *ok, int32_t *its,
                                int32_t *status) {
-    size_t tsize = 8;
     std::vector<double> Y((size_t)NBUF * NP * 3 * n);
-    size_t facb = (size_t)NP * NFAC * n * tsize, poisb = (size_t)NP * 4 * n * 8;
-    std::vector<char> fac(facb > poisb ? facb : poisb);
+    std::vector<double> fac64((size_t)NP * NFAC * n), pfac((size_t)NP * 4 * n);
+    std::vector<float> fac32((size_t)NP * NFAC * n);
     std::vector<double> fix((size_t)NP * n);
     std::vector<LaneState> st(n);
+    if (g_poison) {
+        memset(Y.data(), g_poison, Y.size() * sizeof(double));
+        memset(fac64.data(), g_poison, fac64.size() * sizeof(double));
+        memset(fac32.data(), g_poison, fac32.size() * sizeof(float));
+        memset(pfac.data(), g_poison, pfac.size() * sizeof(double));
+        memset(fix.data(), g_poison, fix.size() * sizeof(double));
+        memset((void *)st.data(), g_poison, st.size() * sizeof(LaneState));
+    }
     Args a;
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V;
-    a.Y = Y.data(); a.fac = fac.data(); a.fix = fix.data(); a.st = st.data();
+    a.Y = Y.data(); a.fac64 = fac64.data(); a.fac32 = fac32.data(); a.pfac = pfac.data();
+    a.fix = fix.data(); a.st = st.data();
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = tol; a.max_iter = max_iter; a.start_iter = start_iter; a.strict = strict;
     a.bern = bern; a.jv = jv; a.mixed = precision ? 1 : 0;
