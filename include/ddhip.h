@@ -21,6 +21,10 @@ extern "C" {
 
 #define DDH_ABI_VERSION 1
 
+/* Segments (runs of points of one material) per device: the solve keeps each wavefront's
+ * segment tables in the compute unit's 160 KB local data share. */
+#define DDH_MAX_SEGMENTS 14
+
 enum ddh_status {
     DDH_OK = 0,
     DDH_E_SHAPE = -1,   /* a size or shape the kernels do not take */
@@ -37,7 +41,7 @@ enum ddh_precision { DDH_FP64 = 0, DDH_MIXED = 1 };
 typedef struct {
     int32_t n;        /* devices */
     int32_t NP;       /* interior grid points */
-    int32_t nseg;     /* segments per device (max over the batch) */
+    int32_t nseg;     /* segments per device (max over the batch), <= DDH_MAX_SEGMENTS */
     int32_t reserved;
     const double *pt;   /* [NP + 2][4][n] */
     const uint8_t *sid; /* [NP + 2][n] */

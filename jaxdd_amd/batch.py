@@ -31,6 +31,7 @@ SC_FIELDS = ("L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_r
              "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
              "G_shunt", "n_floor", "pre_cond", "fix_ions")
 DENSITY_FLOOR = 1e-12  # src/jaxdd/solver.py:54
+MAX_SEGMENTS = 14  # DDH_MAX_SEGMENTS of include/ddhip.h
 
 
 def _point_fields(dev: Device) -> np.ndarray:
@@ -83,8 +84,9 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     change = np.any(fields[:, 1:] != fields[:, :-1], axis=2)
     sid = np.concatenate([np.zeros((n, 1), dtype=np.int64), np.cumsum(change, axis=1)], axis=1)
     nseg = int(sid.max()) + 1
-    if nseg > 255:
-        raise NotImplementedError(f"{nseg} segments of constant material: at most 255")
+    if nseg > MAX_SEGMENTS:
+        raise NotImplementedError(f"{nseg} segments of constant material: at most {MAX_SEGMENTS} "
+                                  "(the kernels keep the segment tables in LDS)")
     seg = np.zeros((nseg, len(SEG_FIELDS), n))
     first = np.concatenate([np.ones((n, 1), dtype=bool), change], axis=1)  # (n, P)
     dev_idx, pt_idx = np.nonzero(first)

@@ -27,6 +27,7 @@
 #define DDH_FN inline
 #define DDH_CONST static const
 #define DDH_GLOBAL
+#define DDH_LDS
 using std::isfinite;
 #else
 #include <hip/hip_runtime.h>
@@ -35,6 +36,8 @@ using std::isfinite;
 // Pointers into device memory: global address space, so that a sweep compiled as a function
 // of its own loads and stores with global (not flat) instructions.
 #define DDH_GLOBAL __attribute__((address_space(1)))
+// The wavefront's copy of its lanes' segment tables: local data share (LDS).
+#define DDH_LDS __attribute__((address_space(3)))
 #endif
 
 #include "ddhip_layout.h"
@@ -360,11 +363,24 @@ DDH_FN Edge edge_flux(const Pt &l, const Pt &r, double iVt, int bern) {
     return e;
 }
 
+// The segment table of a wavefront's lanes in LDS, copied once per launch: field f of
+// segment s of the lane at lseg[(s * SEG_N + f) * WAVE]. A segment change then reloads from
+// LDS, whose wait counter is not the one global stores count on, so the reload does not wait
+// for the sweep's outstanding factor stores.
+constexpr int WAVE = 64;
+typedef DDH_LDS double ldouble;
+typedef DDH_LDS const double lcdouble;
+
+DDH_FN void fill_segments(const Args &a, int lane, ldouble *lseg) {
+    for (int q = 0; q < a.nseg * SEG_N; ++q) lseg[(size_t)q * WAVE] = a.seg[(size_t)q * a.n + lane];
+}
+
 struct SweepCtl {  // what one lane's sweeps need of its state
     double Vint, share, ratio, g_frac;
     double logs[2];     // normalisations at the state being assembled
     double lstart[2];   // the pre-bias's (for the freeze)
     int fresh, freeze_now, mobile, src1, src2, work;
+    lcdouble *lseg;     // the lane's segment table in LDS
 };
 
 // Per-lane views of the arrays a sweep walks: each advances by its row stride per point.
@@ -375,7 +391,7 @@ typedef DDH_GLOBAL const uint8_t gcuint8;
 struct Lane {
     gcdouble *pt;   // + j*PT_N*n + f*n
     gcuint8 *sid;   // + j*n
-    gcdouble *seg;  // + (id*SEG_N + f)*n
+    lcdouble *seg;  // + (id*SEG_N + f)*WAVE (LDS)
     gdouble *yw, *ys1, *ys2;  // state buffers: + (j-1)*3n + f*n
     gdouble *fix;   // + (j-1)*n
     size_t n;
@@ -387,7 +403,7 @@ DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
     L.n = n;
     L.pt = (gcdouble *)(a.pt + lane);
     L.sid = (gcuint8 *)(a.sid + lane);
-    L.seg = (gcdouble *)(a.seg + lane);
+    L.seg = c.lseg;
     L.yw = (gdouble *)(a.Y + (size_t)c.work * ny + lane);
     L.ys1 = (gdouble *)(a.Y + (size_t)c.src1 * ny + lane);
     L.ys2 = (gdouble *)(a.Y + (size_t)c.src2 * ny + lane);
@@ -401,65 +417,90 @@ struct SegE {
     double eps, offn, offp, dop, kd, ni2, Nt, Cn, Cp, nt0, pt0, am, cm, mun, mup, sP, sCs;
 };
 
-DDH_FN void seg_elim(const Lane &L, int j, SegE &s) {
-    const int id = L.sid[(size_t)j * L.n];
+DDH_FN void seg_elim(const Lane &L, int id, SegE &s) {
     if (id == s.id) return;
     s.id = id;
-    gcdouble *q = L.seg + (size_t)id * SEG_N * L.n;
-    s.eps = q[SEG_EPS * L.n]; s.offn = q[SEG_OFFN * L.n]; s.offp = q[SEG_OFFP * L.n];
-    s.dop = q[SEG_DOP * L.n]; s.kd = q[SEG_KD * L.n]; s.ni2 = q[SEG_NI2 * L.n];
-    s.Nt = q[SEG_NT * L.n]; s.Cn = q[SEG_CN * L.n]; s.Cp = q[SEG_CP * L.n];
-    s.nt0 = q[SEG_NT0 * L.n]; s.pt0 = q[SEG_PT0 * L.n]; s.am = q[SEG_AM * L.n];
-    s.cm = q[SEG_CM * L.n]; s.mun = q[SEG_MUN * L.n]; s.mup = q[SEG_MUP * L.n];
-    s.sP = q[SEG_SP * L.n]; s.sCs = q[SEG_SCS * L.n];
+    lcdouble *q = L.seg + (size_t)id * SEG_N * WAVE;
+    s.eps = q[SEG_EPS * WAVE]; s.offn = q[SEG_OFFN * WAVE]; s.offp = q[SEG_OFFP * WAVE];
+    s.dop = q[SEG_DOP * WAVE]; s.kd = q[SEG_KD * WAVE]; s.ni2 = q[SEG_NI2 * WAVE];
+    s.Nt = q[SEG_NT * WAVE]; s.Cn = q[SEG_CN * WAVE]; s.Cp = q[SEG_CP * WAVE];
+    s.nt0 = q[SEG_NT0 * WAVE]; s.pt0 = q[SEG_PT0 * WAVE]; s.am = q[SEG_AM * WAVE];
+    s.cm = q[SEG_CM * WAVE]; s.mun = q[SEG_MUN * WAVE]; s.mup = q[SEG_MUP * WAVE];
+    s.sP = q[SEG_SP * WAVE]; s.sCs = q[SEG_SCS * WAVE];
 }
 
-// Point j's values: interior points from the state (forming a solve's start where `fresh`),
+// An interior point's loads. A sweep issues them one point ahead of their use, so that their
+// latency passes while the point before is computed: with one wavefront per SIMD nothing
+// else would hide it.
+struct Raw {
+    double h, G, fix, y0, y1, y2, o0, o1, o2;  // o: the older state of the start's extrapolation
+    int sid;
+};
+
+DDH_FN void fetch_point(const Lane &L, gdouble *s1, gdouble *s2, int j, Raw &r) {
+    const size_t n = L.n, o = (size_t)(j - 1) * 3 * n;
+    r.sid = L.sid[(size_t)j * n];
+    r.h = L.pt[((size_t)j * PT_N + PT_H) * n];
+    r.G = L.pt[((size_t)j * PT_N + PT_G) * n];
+    r.fix = L.fix[(size_t)(j - 1) * n];
+    r.y0 = s1[o]; r.y1 = s1[o + n]; r.y2 = s1[o + 2 * n];
+    r.o0 = s2[o]; r.o1 = s2[o + n]; r.o2 = s2[o + 2 * n];
+}
+
+// A contact's values, read before a sweep's loop.
+struct Contact {
+    double V, n, p, h;
+};
+
+DDH_FN Contact contact_point(const Args &a, int lane, const Lane &L, double Vint, bool right) {
+    Contact ct;
+    const int j = right ? a.NP + 1 : 0;
+    ct.h = L.pt[((size_t)j * PT_N + PT_H) * L.n];
+    ct.V = contact_V(a, lane, Vint, right);
+    ct.n = exp(right ? SC(SC_LNR) : SC(SC_LNL));
+    ct.p = exp(right ? SC(SC_LPR) : SC(SC_LPL));
+    return ct;
+}
+
+// Point j's values: interior points from their loads (forming a solve's start where `fresh`),
 // the contacts from their boundary values.
-DDH_FN void load_point(const Args &a, int lane, const Lane &L, int j, const SegE &s,
-                       const SweepCtl &c, double Vt, double iVt, Pt &q) {
-    const int NP = a.NP;
+DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, const Raw &r,
+                       const SegE &s, const SweepCtl &c, double Vt, double iVt, Pt &q) {
     const size_t n = L.n;
     q.offn = s.offn; q.offp = s.offp; q.eps = s.eps; q.dop = s.dop;
     q.cn = Vt * s.mun; q.cp = Vt * s.mup;
     q.sP = s.sP; q.sCs = s.sCs;
-    q.h = L.pt[((size_t)j * PT_N + PT_H) * n];
     q.am = c.mobile ? s.am : 0.0;
     q.cm = c.mobile ? s.cm : 0.0;
-    double ln_n, ln_p;
-    if (j == 0 || j == NP + 1) {
-        bool right = j == NP + 1;
-        q.V = contact_V(a, lane, c.Vint, right);
-        ln_n = right ? SC(SC_LNR) : SC(SC_LNL);
-        ln_p = right ? SC(SC_LPR) : SC(SC_LPL);
+    if (contact) {
+        q.h = ct.h;
+        q.V = ct.V;
         q.G = 0.0; q.fix = 0.0;
-        q.n = exp(ln_n);
-        q.p = exp(ln_p);
+        q.n = ct.n;
+        q.p = ct.p;
         q.R = q.dRn = q.dRp = 0.0;
     } else {
-        const size_t o = (size_t)(j - 1) * 3 * n;
-        double y0, y1, y2;
+        q.h = r.h;
+        double y0 = r.y0, y1 = r.y1, y2 = r.y2;
+        q.fix = r.fix;
         if (c.fresh) {
-            y0 = L.ys1[o]; y1 = L.ys1[o + n]; y2 = L.ys1[o + 2 * n];
             if (c.freeze_now) {
                 // the pre-bias's mobile ions as fixed ones (src/jaxdd/batch.py:859-872)
-                double fx = L.pt[((size_t)j * PT_N + PT_FIX) * n] + s.am * exp(c.lstart[0] + y0 * iVt) -
-                            s.cm * exp(c.lstart[1] - y0 * iVt);
-                L.fix[(size_t)(j - 1) * n] = fx;
+                q.fix = L.pt[((size_t)j * PT_N + PT_FIX) * n] + s.am * exp(c.lstart[0] + y0 * iVt) -
+                        s.cm * exp(c.lstart[1] - y0 * iVt);
+                L.fix[(size_t)(j - 1) * n] = q.fix;
             }
             if (c.ratio != 0.0) {
                 double capv = CAP_V * Vt;
-                y0 += clipd(c.ratio * (y0 - L.ys2[o]), -capv, capv);
-                y1 += clipd(c.ratio * (y1 - L.ys2[o + n]), -CAP_N, CAP_N);
-                y2 += clipd(c.ratio * (y2 - L.ys2[o + 2 * n]), -CAP_N, CAP_N);
+                y0 += clipd(c.ratio * (y0 - r.o0), -capv, capv);
+                y1 += clipd(c.ratio * (y1 - r.o1), -CAP_N, CAP_N);
+                y2 += clipd(c.ratio * (y2 - r.o2), -CAP_N, CAP_N);
             }
+            const size_t o = (size_t)(j - 1) * 3 * n;
             L.yw[o] = y0; L.yw[o + n] = y1; L.yw[o + 2 * n] = y2;
-        } else {
-            y0 = L.yw[o]; y1 = L.yw[o + n]; y2 = L.yw[o + 2 * n];
         }
         q.V = y0;
-        q.G = L.pt[((size_t)j * PT_N + PT_G) * n];
-        q.fix = L.fix[(size_t)(j - 1) * n];
+        q.G = r.G;
         q.n = exp(y1);
         q.p = exp(y2);
         q.R = recombination(q.n, q.p, s.kd, s.ni2, s.Nt, s.Cn, s.Cp, s.nt0, s.pt0, q.dRn, q.dRp);
@@ -491,8 +532,11 @@ DDH_FN void to_t(const double *in, T *out, int m) {
 
 // Elimination sweep (right to left). Returns the ion step z.
 template <typename T, int K>
-DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z) {
-    const int NP = a.NP;
+DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double *z) {
+    // everything the loop reads of the arguments, in registers: the loop's stores to global
+    // memory would otherwise keep these generic-pointer loads inside it
+    const SweepCtl c = cref;
+    const int NP = a.NP, bern = a.bern;
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, L = SC(SC_L), L2 = L * L;
     // the ions' totals (scaled in the start's stages); none once they are frozen (the slots
     // are then held, src/jaxdd/batch.py:859-872)
@@ -502,11 +546,19 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z)
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
     DDH_GLOBAL T *fac = (DDH_GLOBAL T *)(factors<T>(a) + lane);
+    // a solve's first step reads its start from the source buffers, later steps the iterate
+    gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
+    gdouble *s2 = (c.fresh && c.ratio != 0.0) ? Lv.ys2 : s1;
+    const Contact left = contact_point(a, lane, Lv, c.Vint, false);
+    const Contact right = contact_point(a, lane, Lv, c.Vint, true);
+    const int sid0 = Lv.sid[0];
     SegE sg; sg.id = -1;
     Pt mid, cur;
     double rV = 0.0, ram = 0.0, rcm = 0.0;  // the row's right neighbour
-    seg_elim(Lv, NP + 1, sg);
-    load_point(a, lane, Lv, NP + 1, sg, c, Vt, iVt, mid);
+    Raw nxt, now;
+    seg_elim(Lv, Lv.sid[(size_t)(NP + 1) * n], sg);
+    fetch_point(Lv, s1, s2, NP, nxt);
+    make_point(Lv, NP + 1, true, right, nxt, sg, c, Vt, iVt, mid);  // a contact: nxt unread
     Edge eR = {0, 0, 0, 0, 0, 0, 0, 0};
     T cp[9], dp[3 * K];
 #pragma unroll
@@ -516,16 +568,23 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z)
     double gam[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     double accF[2] = {0, 0}, accU[2][2] = {{0, 0}, {0, 0}};
     double num0 = 0.0, num1 = 0.0;
-    for (int j = NP; j >= 0; --j) {
-        seg_elim(Lv, j, sg);
-        load_point(a, lane, Lv, j, sg, c, Vt, iVt, cur);
-        const Edge eL = edge_flux(cur, mid, iVt, a.bern);
+    // Point j: its values, the edge to its right neighbour `mid`, and mid's row where mid is an
+    // interior point. Every visit of the loop issues the same memory operations (the next
+    // point's loads, then the row's factor stores), so that the wait for the loads never
+    // has to cover the stores.
+    auto visit = [&](const int j, const bool has_row) {
+        now = nxt;
+        // the segment first: its rare reload is then waited for without the next point's loads
+        seg_elim(Lv, j > 0 ? now.sid : sid0, sg);
+        fetch_point(Lv, s1, s2, j > 1 ? j - 1 : 1, nxt);
+        make_point(Lv, j, j == 0, left, now, sg, c, Vt, iVt, cur);
+        const Edge eL = edge_flux(cur, mid, iVt, bern);
         if constexpr (K > 1) {
             const double hLj = cur.h * L;
             num0 += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
             num1 += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
         }
-        if (j + 1 <= NP) {
+        if (has_row) {
             // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
             const int i = j + 1;
             const double hm = cur.h, hp = mid.h, hs = hp + hm;
@@ -657,7 +716,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z)
         rV = mid.V; ram = mid.am; rcm = mid.cm;
         mid = cur;
         eR = eL;
-    }
+    };
+    visit(NP, false);
+    for (int j = NP - 1; j >= 0; --j) visit(j, true);
     z[0] = z[1] = 0.0;
     if constexpr (K > 1) {
         // the constraints (solver.py:1255-1265) and their Schur complement; a slot without
@@ -674,8 +735,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &c, double *z)
 
 // Back-substitution sweep (left to right): the step, the new state, its size and current.
 template <typename T, int K>
-DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const double *z,
+DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, const double *z,
                              double *out /* size, J, finite */) {
+    const SweepCtl c = cref;  // in registers (as in the elimination)
     const int NP = a.NP;
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, nfl = SC(SC_NFLOOR), lnfloor = log(nfl);
     const double capv = CAP_V * Vt;
@@ -686,16 +748,34 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const d
     int id = -1;
     double offn = 0, offp = 0, kd = 0, ni2 = 0, Nt = 0, Cn = 0, Cp = 0, nt0 = 0, pt0 = 0;
     double imun = 0, imup = 0;
-    auto seg = [&](int j) {
-        const int s = Lv.sid[(size_t)j * n];
+    auto seg = [&](int s) {
         if (s == id) return;
         id = s;
-        gcdouble *q = Lv.seg + (size_t)s * SEG_N * n;
-        offn = q[SEG_OFFN * n]; offp = q[SEG_OFFP * n]; kd = q[SEG_KD * n]; ni2 = q[SEG_NI2 * n];
-        Nt = q[SEG_NT * n]; Cn = q[SEG_CN * n]; Cp = q[SEG_CP * n]; nt0 = q[SEG_NT0 * n];
-        pt0 = q[SEG_PT0 * n]; imun = q[SEG_IMUN * n]; imup = q[SEG_IMUP * n];
+        lcdouble *q = Lv.seg + (size_t)s * SEG_N * WAVE;
+        offn = q[SEG_OFFN * WAVE]; offp = q[SEG_OFFP * WAVE]; kd = q[SEG_KD * WAVE];
+        ni2 = q[SEG_NI2 * WAVE]; Nt = q[SEG_NT * WAVE]; Cn = q[SEG_CN * WAVE];
+        Cp = q[SEG_CP * WAVE]; nt0 = q[SEG_NT0 * WAVE]; pt0 = q[SEG_PT0 * WAVE];
+        imun = q[SEG_IMUN * WAVE]; imup = q[SEG_IMUP * WAVE];
     };
-    seg(0);
+    // point j's loads, issued one point ahead of their use (as in the elimination)
+    struct Next {
+        T f[9 + 3 * K];
+        double y0, y1, y2, G, x;
+        int sid;
+    };
+    auto fetch = [&](int j, Next &r) {
+        DDH_GLOBAL const T *f = fac + (size_t)(j - 1) * NFAC * n;
+#pragma unroll
+        for (int q = 0; q < 9 + 3 * K; ++q) r.f[q] = f[q * n];
+        gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
+        r.y0 = y[0]; r.y1 = y[n]; r.y2 = y[2 * n];
+        r.G = Lv.pt[((size_t)j * PT_N + PT_G) * n];
+        r.x = Lv.pt[((size_t)j * PT_N + PT_X) * n];
+        r.sid = Lv.sid[(size_t)j * n];
+    };
+    Next nxt, now;
+    fetch(1, nxt);
+    seg(Lv.sid[0]);
     // the left contact
     const double V0 = contact_V(a, lane, c.Vint, false);
     const double Vg0n = V0 + offn, Vg0p = V0 + offp;
@@ -709,21 +789,23 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const d
     double size = 0.0;
     int finite = 1;
     for (int j = 1; j <= NP; ++j) {
-        seg(j);
-        DDH_GLOBAL const T *f = fac + (size_t)(j - 1) * NFAC * n;
+        now = nxt;
+        seg(now.sid);
+        fetch(j < NP ? j + 1 : NP, nxt);  // the same loads every visit (as in the elimination)
+        const T *f = now.f;
         double dF[3];
 #pragma unroll
         for (int row = 0; row < 3; ++row) {
-            double v = (double)f[(9 + row * K + 0) * n];
-            if constexpr (K > 1) v -= (double)f[(9 + row * K + 1) * n] * z0 + (double)f[(9 + row * K + 2) * n] * z1;
+            double v = (double)f[9 + row * K + 0];
+            if constexpr (K > 1) v -= (double)f[9 + row * K + 1] * z0 + (double)f[9 + row * K + 2] * z1;
             dF[row] = v;
         }
-        const double x0 = dF[0] - ((double)f[0] * xs0 + (double)f[n] * xs1 + (double)f[2 * n] * xs2);
-        const double x1 = dF[1] - ((double)f[3 * n] * xs0 + (double)f[4 * n] * xs1 + (double)f[5 * n] * xs2);
-        const double x2 = dF[2] - ((double)f[6 * n] * xs0 + (double)f[7 * n] * xs1 + (double)f[8 * n] * xs2);
+        const double x0 = dF[0] - ((double)f[0] * xs0 + (double)f[1] * xs1 + (double)f[2] * xs2);
+        const double x1 = dF[1] - ((double)f[3] * xs0 + (double)f[4] * xs1 + (double)f[5] * xs2);
+        const double x2 = dF[2] - ((double)f[6] * xs0 + (double)f[7] * xs1 + (double)f[8] * xs2);
         xs0 = x0; xs1 = x1; xs2 = x2;
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
-        double y0 = y[0], y1 = y[n], y2 = y[2 * n];
+        double y0 = now.y0, y1 = now.y1, y2 = now.y2;
         const double st0 = clipd(x0, -capv, capv);
         double st1 = clipd(x1, -CAP_N, CAP_N), st2 = clipd(x2, -CAP_N, CAP_N);
         if (y1 < lnfloor) {  // density_steps (src/jaxdd/solver.py:74-88)
@@ -743,10 +825,10 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const d
         size = nmax(size, fabs(st2) * (pp / (pp + nfl)));
         // the integral current of the new state (src/jaxdd/batch.py:321-373)
         const double R = recombination_value(nn, pp, kd, ni2, Nt, Cn, Cp, nt0, pt0);
-        const double U = R - Lv.pt[((size_t)j * PT_N + PT_G) * n] * c.g_frac;
+        const double U = R - now.G * c.g_frac;
         const double wn = exp(-(y0 + offn) * iVt) * imun;
         const double wp = exp((y0 + offp) * iVt) * imup;
-        const double xj = Lv.pt[((size_t)j * PT_N + PT_X) * n];
+        const double xj = now.x;
         const double dx = xj - x_prev;
         const double iu = iu_prev + 0.5 * dx * (U_prev + U);
         single_n += 0.5 * dx * (wn_prev + wn);
@@ -755,7 +837,7 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &c, const d
         double_p += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
     }
-    seg(NP + 1);
+    seg(Lv.sid[(size_t)(NP + 1) * n]);
     const double VN = contact_V(a, lane, c.Vint, true);
     const double endnN = exp(SC(SC_LNR) - (VN + offn) * iVt);
     const double endpN = exp(SC(SC_LPR) + (VN + offp) * iVt);
@@ -1008,9 +1090,10 @@ DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, doubl
 // One Newton step of one lane (both sweeps and the state machine). `ions`: some lane of
 // the wavefront has mobile ions (the sweeps then carry the border's right-hand sides).
 template <typename T>
-DDH_FN void newton_lane_step(const Args &a, int lane, bool ions) {
+DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg) {
     LaneState &S = a.st[lane];
     SweepCtl c;
+    c.lseg = lseg;
     c.Vint = S.Vint; c.share = S.share; c.ratio = S.ratio; c.g_frac = SC(SC_GFRAC);
     c.logs[0] = S.lw[0]; c.logs[1] = S.lw[1];
     c.lstart[0] = S.lstart[0]; c.lstart[1] = S.lstart[1];

@@ -14,9 +14,12 @@ namespace ddh {
 // The whole solve of every device in one launch: each lane runs Poisson's start, then its
 // Newton state machine; each wavefront loops until its last lane has finished its scan.
 __global__ void __launch_bounds__(64) solve_kernel(Args a) {
+    extern __shared__ double lds[];  // a.nseg * SEG_N * WAVE doubles
     const int lane = blockIdx.x * 64 + threadIdx.x;
     const bool valid = lane < a.n;
+    ldouble *lseg = (ldouble *)lds + threadIdx.x;
     if (valid) {
+        fill_segments(a, lane, lseg);
         poisson_lane(a, lane);
         start_lane(a, lane, a.st[lane]);
     }
@@ -26,8 +29,8 @@ __global__ void __launch_bounds__(64) solve_kernel(Args a) {
         const bool ions = __any(act && lane_mobile(a, lane));
         const bool f64 = !a.mixed || __any(act && lane_needs64(a, lane));
         if (act) {
-            if (f64) newton_lane_step<double>(a, lane, ions);
-            else newton_lane_step<float>(a, lane, ions);
+            if (f64) newton_lane_step<double>(a, lane, ions, lseg);
+            else newton_lane_step<float>(a, lane, ions, lseg);
         }
     }
 }
@@ -204,8 +207,8 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
                                void *stream) {
     if (!b || !V || !o || !J || !ok || !its || !status || !workspace)
         return fail(DDH_E_ARG, "ddh_solve_batch: NULL argument");
-    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || b->nseg < 1 || b->nseg > 255)
-        return fail(DDH_E_SHAPE, "ddh_solve_batch: n > 0, 2 <= NP <= 65536, nV >= 1, 1 <= nseg <= 255");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_solve_batch: n > 0, 2 <= NP <= 65536, nV >= 1, 1 <= nseg <= 14");
     if (o->precision != DDH_FP64 && o->precision != DDH_MIXED)
         return fail(DDH_E_ARG, "ddh_solve_batch: precision must be DDH_FP64 or DDH_MIXED");
     if (o->bernoulli != 0 && o->bernoulli != 1)
@@ -230,7 +233,10 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
     a.mixed = o->precision == DDH_MIXED ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((b->n + 63) / 64), block(64);
-    hipLaunchKernelGGL(ddh::solve_kernel, grid, block, 0, s, a);
+    const size_t lds = (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
+    (void)hipFuncSetAttribute((const void *)ddh::solve_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(ddh::solve_kernel, grid, block, lds, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "solve_kernel launch");
     return DDH_OK;

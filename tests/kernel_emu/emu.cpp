@@ -18,6 +18,8 @@ using namespace ddh;
 static void run(Args &a) {
     const int nw = (a.n + 63) / 64;
     std::vector<char> started(nw, 0), done(nw, 0);
+    // each wavefront's local data share (the lanes' segment tables)
+    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * WAVE);
     int remaining = nw;
     while (remaining > 0) {
         for (int w = 0; w < nw; ++w) {
@@ -25,12 +27,14 @@ static void run(Args &a) {
             int w0 = w * 64, w1 = w0 + 64 < a.n ? w0 + 64 : a.n;
             if (!started[w]) {
                 for (int lane = w0; lane < w1; ++lane) {
+                    fill_segments(a, lane, &lds_all[(size_t)w * a.nseg * SEG_N * WAVE] + (lane - w0));
                     poisson_lane(a, lane);
                     start_lane(a, lane, a.st[lane]);
                 }
                 started[w] = 1;
                 continue;
             }
+            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
             bool any = false, ions = false, f64 = !a.mixed;
             for (int lane = w0; lane < w1; ++lane) {
                 bool act = lane_active(a, lane);
@@ -41,8 +45,8 @@ static void run(Args &a) {
             if (!any) { done[w] = 1; --remaining; continue; }
             for (int lane = w0; lane < w1; ++lane) {
                 if (!lane_active(a, lane)) continue;
-                if (f64) newton_lane_step<double>(a, lane, ions);
-                else newton_lane_step<float>(a, lane, ions);
+                if (f64) newton_lane_step<double>(a, lane, ions, lds + (lane - w0));
+                else newton_lane_step<float>(a, lane, ions, lds + (lane - w0));
             }
         }
     }

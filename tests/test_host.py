@@ -189,3 +189,8 @@ def test_library_rejects_bad_shapes_before_any_launch():
     assert b"NULL" in lib.ddh_last_error()
     rc = lib.ddh_block_sweep(4, 10, 4, 1, 1, 1, 1, 1, 0, 1, None)
     assert rc == -1
+    # more segments than the LDS holds: refused before any device memory is touched
+    b = _lib.Batch(64, 400, 15, 0, 1, 1, 1, 1)
+    o = _lib.Opts(1e-10, 60, 60, 0, 0, 1, 0)
+    rc = lib.ddh_solve_batch(ctypes.byref(b), 1, 61, ctypes.byref(o), 1, 1, 1, 1, 1, 1 << 40, None)
+    assert rc == -1 and b"nseg" in lib.ddh_last_error()
