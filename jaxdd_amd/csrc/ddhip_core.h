@@ -28,6 +28,7 @@
 #define DDH_CONST static const
 #define DDH_GLOBAL
 #define DDH_LDS
+#define DDH_UNIFORM(x) (x)
 using std::isfinite;
 #else
 #include <hip/hip_runtime.h>
@@ -38,6 +39,10 @@ using std::isfinite;
 #define DDH_GLOBAL __attribute__((address_space(1)))
 // The wavefront's copy of its lanes' segment tables: local data share (LDS).
 #define DDH_LDS __attribute__((address_space(3)))
+// A value every lane of the wavefront holds, moved to a scalar register: the sweeps are
+// functions of their own, whose arguments arrive in vector registers, and a grid index or
+// row stride kept there would make every address a vector computation.
+#define DDH_UNIFORM(x) __builtin_amdgcn_readfirstlane(x)
 #endif
 
 #include "ddhip_layout.h"
@@ -327,41 +332,21 @@ DDH_FN void poisson_lane(const Args &a, int lane) {
 // ---------------------------------------------------------------------------------------
 // The Newton sweeps.
 // ---------------------------------------------------------------------------------------
-// The values of one grid point a sweep carries from visit to visit.
+// The values of one grid point a sweep carries from visit to visit. Its material parameters
+// stay in the lane's LDS segment table and are read, through the point's segment id, where
+// they are used: registers hold only what changes from point to point.
 struct Pt {
     double V, n, p;        // the state
-    double offn, offp;     // V_gn - V, V_gp - V
-    double cn, cp;         // V_T mu of the interval after the point
-    double h, eps;
-    double sP, sCs;        // row scales (sCs / h is the continuity rows')
+    double h;
     double R, dRn, dRp;    // recombination and its derivatives in ln n, ln p
-    double G, fix, dop;
+    double G, fix;
     double an, ca;         // mobile ion densities (0 where none or frozen)
-    double am, cm;         // the ions' region masks (0 once frozen)
+    int sid;               // the point's segment
 };
 
 struct Edge {  // Scharfetter-Gummel fluxes over one interval and their derivatives
     double Fn, Fp, gn, gp, an0, an1, ap0, ap1;
 };
-
-// The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108).
-DDH_FN Edge edge_flux(const Pt &l, const Pt &r, double iVt, int bern) {
-    Edge e;
-    double xn = ((r.V + r.offn) - (l.V + l.offn)) * iVt;
-    double xp = ((r.V + r.offp) - (l.V + l.offp)) * iVt;
-    double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
-    bernoulli_pair(xn, bern, Bnf, Bnb, dBnf, dBnb);
-    bernoulli_pair(xp, bern, Bpf, Bpb, dBpf, dBpb);
-    e.an0 = -l.cn * Bnb * l.n;
-    e.an1 = l.cn * Bnf * r.n;
-    e.Fn = e.an1 + e.an0;
-    e.gn = l.cn * (dBnf * r.n + dBnb * l.n) * iVt;
-    e.ap0 = l.cp * Bpf * l.p;
-    e.ap1 = -l.cp * Bpb * r.p;
-    e.Fp = e.ap0 + e.ap1;
-    e.gp = l.cp * (dBpf * l.p + dBpb * r.p) * iVt;
-    return e;
-}
 
 // The segment table of a wavefront's lanes in LDS, copied once per launch: field f of
 // segment s of the lane at lseg[(s * SEG_N + f) * WAVE]. A segment change then reloads from
@@ -399,7 +384,7 @@ struct Lane {
 
 DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
     Lane L;
-    const size_t n = (size_t)a.n, ny = (size_t)a.NP * 3 * n;
+    const size_t n = (size_t)DDH_UNIFORM(a.n), ny = (size_t)DDH_UNIFORM(a.NP) * 3 * n;
     L.n = n;
     L.pt = (gcdouble *)(a.pt + lane);
     L.sid = (gcuint8 *)(a.sid + lane);
@@ -411,22 +396,28 @@ DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
     return L;
 }
 
-// The segment's fields one sweep needs, reloaded where the segment changes.
-struct SegE {
-    int id;
-    double eps, offn, offp, dop, kd, ni2, Nt, Cn, Cp, nt0, pt0, am, cm, mun, mup, sP, sCs;
-};
+// Field f of this lane's segment id (LDS).
+DDH_FN double segv(const Lane &L, int id, int f) { return L.seg[(id * SEG_N + f) * WAVE]; }
 
-DDH_FN void seg_elim(const Lane &L, int id, SegE &s) {
-    if (id == s.id) return;
-    s.id = id;
-    lcdouble *q = L.seg + (size_t)id * SEG_N * WAVE;
-    s.eps = q[SEG_EPS * WAVE]; s.offn = q[SEG_OFFN * WAVE]; s.offp = q[SEG_OFFP * WAVE];
-    s.dop = q[SEG_DOP * WAVE]; s.kd = q[SEG_KD * WAVE]; s.ni2 = q[SEG_NI2 * WAVE];
-    s.Nt = q[SEG_NT * WAVE]; s.Cn = q[SEG_CN * WAVE]; s.Cp = q[SEG_CP * WAVE];
-    s.nt0 = q[SEG_NT0 * WAVE]; s.pt0 = q[SEG_PT0 * WAVE]; s.am = q[SEG_AM * WAVE];
-    s.cm = q[SEG_CM * WAVE]; s.mun = q[SEG_MUN * WAVE]; s.mup = q[SEG_MUP * WAVE];
-    s.sP = q[SEG_SP * WAVE]; s.sCs = q[SEG_SCS * WAVE];
+// The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108); the interval's
+// mobilities are those of l's segment (the interval after the point).
+DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
+    Edge e;
+    const double lcn = Vt * segv(L, l.sid, SEG_MUN), lcp = Vt * segv(L, l.sid, SEG_MUP);
+    double xn = ((r.V + segv(L, r.sid, SEG_OFFN)) - (l.V + segv(L, l.sid, SEG_OFFN))) * iVt;
+    double xp = ((r.V + segv(L, r.sid, SEG_OFFP)) - (l.V + segv(L, l.sid, SEG_OFFP))) * iVt;
+    double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
+    bernoulli_pair(xn, bern, Bnf, Bnb, dBnf, dBnb);
+    bernoulli_pair(xp, bern, Bpf, Bpb, dBpf, dBpb);
+    e.an0 = -lcn * Bnb * l.n;
+    e.an1 = lcn * Bnf * r.n;
+    e.Fn = e.an1 + e.an0;
+    e.gn = lcn * (dBnf * r.n + dBnb * l.n) * iVt;
+    e.ap0 = lcp * Bpf * l.p;
+    e.ap1 = -lcp * Bpb * r.p;
+    e.Fp = e.ap0 + e.ap1;
+    e.gp = lcp * (dBpf * l.p + dBpb * r.p) * iVt;
+    return e;
 }
 
 // An interior point's loads. A sweep issues them one point ahead of their use, so that their
@@ -465,13 +456,11 @@ DDH_FN Contact contact_point(const Args &a, int lane, const Lane &L, double Vint
 // Point j's values: interior points from their loads (forming a solve's start where `fresh`),
 // the contacts from their boundary values.
 DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, const Raw &r,
-                       const SegE &s, const SweepCtl &c, double Vt, double iVt, Pt &q) {
+                       int sid, const SweepCtl &c, double Vt, double iVt, Pt &q) {
     const size_t n = L.n;
-    q.offn = s.offn; q.offp = s.offp; q.eps = s.eps; q.dop = s.dop;
-    q.cn = Vt * s.mun; q.cp = Vt * s.mup;
-    q.sP = s.sP; q.sCs = s.sCs;
-    q.am = c.mobile ? s.am : 0.0;
-    q.cm = c.mobile ? s.cm : 0.0;
+    q.sid = sid;
+    const double am = c.mobile ? segv(L, sid, SEG_AM) : 0.0;
+    const double cm = c.mobile ? segv(L, sid, SEG_CM) : 0.0;
     if (contact) {
         q.h = ct.h;
         q.V = ct.V;
@@ -486,8 +475,9 @@ DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, co
         if (c.fresh) {
             if (c.freeze_now) {
                 // the pre-bias's mobile ions as fixed ones (src/jaxdd/batch.py:859-872)
-                q.fix = L.pt[((size_t)j * PT_N + PT_FIX) * n] + s.am * exp(c.lstart[0] + y0 * iVt) -
-                        s.cm * exp(c.lstart[1] - y0 * iVt);
+                q.fix = L.pt[((size_t)j * PT_N + PT_FIX) * n] +
+                        segv(L, sid, SEG_AM) * exp(c.lstart[0] + y0 * iVt) -
+                        segv(L, sid, SEG_CM) * exp(c.lstart[1] - y0 * iVt);
                 L.fix[(size_t)(j - 1) * n] = q.fix;
             }
             if (c.ratio != 0.0) {
@@ -503,10 +493,12 @@ DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, co
         q.G = r.G;
         q.n = exp(y1);
         q.p = exp(y2);
-        q.R = recombination(q.n, q.p, s.kd, s.ni2, s.Nt, s.Cn, s.Cp, s.nt0, s.pt0, q.dRn, q.dRp);
+        q.R = recombination(q.n, q.p, segv(L, sid, SEG_KD), segv(L, sid, SEG_NI2),
+                            segv(L, sid, SEG_NT), segv(L, sid, SEG_CN), segv(L, sid, SEG_CP),
+                            segv(L, sid, SEG_NT0), segv(L, sid, SEG_PT0), q.dRn, q.dRp);
     }
-    q.an = q.am != 0.0 ? q.am * exp(c.logs[0] + q.V * iVt) : 0.0;
-    q.ca = q.cm != 0.0 ? q.cm * exp(c.logs[1] - q.V * iVt) : 0.0;
+    q.an = am != 0.0 ? am * exp(c.logs[0] + q.V * iVt) : 0.0;
+    q.ca = cm != 0.0 ? cm * exp(c.logs[1] - q.V * iVt) : 0.0;
 }
 
 template <typename T>
@@ -536,7 +528,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     // everything the loop reads of the arguments, in registers: the loop's stores to global
     // memory would otherwise keep these generic-pointer loads inside it
     const SweepCtl c = cref;
-    const int NP = a.NP, bern = a.bern;
+    const int NP = DDH_UNIFORM(a.NP), bern = DDH_UNIFORM(a.bern);
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, L = SC(SC_L), L2 = L * L;
     // the ions' totals (scaled in the start's stages); none once they are frozen (the slots
     // are then held, src/jaxdd/batch.py:859-872)
@@ -552,13 +544,15 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     const Contact left = contact_point(a, lane, Lv, c.Vint, false);
     const Contact right = contact_point(a, lane, Lv, c.Vint, true);
     const int sid0 = Lv.sid[0];
-    SegE sg; sg.id = -1;
+    // the ions' region masks, none once frozen
+    auto mam = [&](int id) { return c.mobile ? segv(Lv, id, SEG_AM) : 0.0; };
+    auto mcm = [&](int id) { return c.mobile ? segv(Lv, id, SEG_CM) : 0.0; };
     Pt mid, cur;
-    double rV = 0.0, ram = 0.0, rcm = 0.0;  // the row's right neighbour
-    Raw nxt, now;
-    seg_elim(Lv, Lv.sid[(size_t)(NP + 1) * n], sg);
+    double rV = 0.0;  // the row's right neighbour: its potential and segment
+    int rsid = 0;
+    Raw nxt;
     fetch_point(Lv, s1, s2, NP, nxt);
-    make_point(Lv, NP + 1, true, right, nxt, sg, c, Vt, iVt, mid);  // a contact: nxt unread
+    make_point(Lv, NP + 1, true, right, nxt, Lv.sid[(size_t)(NP + 1) * n], c, Vt, iVt, mid);  // nxt unread
     Edge eR = {0, 0, 0, 0, 0, 0, 0, 0};
     T cp[9], dp[3 * K];
 #pragma unroll
@@ -572,17 +566,14 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     // interior point. Every visit of the loop issues the same memory operations (the next
     // point's loads, then the row's factor stores), so that the wait for the loads never
     // has to cover the stores.
-    auto visit = [&](const int j, const bool has_row) {
-        now = nxt;
-        // the segment first: its rare reload is then waited for without the next point's loads
-        seg_elim(Lv, j > 0 ? now.sid : sid0, sg);
+    auto visit = [&](const int j, const bool has_row, const Raw &now, Raw &nxt) {
         fetch_point(Lv, s1, s2, j > 1 ? j - 1 : 1, nxt);
-        make_point(Lv, j, j == 0, left, now, sg, c, Vt, iVt, cur);
-        const Edge eL = edge_flux(cur, mid, iVt, bern);
+        make_point(Lv, j, j == 0, left, now, j > 0 ? now.sid : sid0, c, Vt, iVt, cur);
+        const Edge eL = edge_flux(Lv, cur, mid, Vt, iVt, bern);
         if constexpr (K > 1) {
             const double hLj = cur.h * L;
-            num0 += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
-            num1 += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
+            num0 += 0.5 * (mid.an + cur.an) * (mam(mid.sid) * mam(cur.sid)) * hLj;
+            num1 += 0.5 * (mid.ca + cur.ca) * (mcm(mid.sid) * mcm(cur.sid)) * hLj;
         }
         if (has_row) {
             // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
@@ -590,19 +581,20 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             const double hm = cur.h, hp = mid.h, hs = hp + hm;
             const double D = 1.0 / (hp * hm * hs * L2);
             const double fac_ = 0.5 * L2 * hp * hm * hs;
-            const double fu = 2 * mid.eps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
-            const double fl0 = 2 * mid.eps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
-            const double deps = mid.eps - cur.eps;     // the jump of eps between layers
+            const double meps = segv(Lv, mid.sid, SEG_EPS);
+            const double fu = 2 * meps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
+            const double fl0 = 2 * meps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
+            const double deps = meps - segv(Lv, cur.sid, SEG_EPS);  // the jump of eps between layers
             const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
             const double fac_m = jump - (fu + fl0);
             const double fac_l = fl0 - jump;
-            const double sP = mid.sP;
-            const double sC = mid.sCs * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
+            const double sP = segv(Lv, mid.sid, SEG_SP);
+            const double sC = segv(Lv, mid.sid, SEG_SCS) * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
             const double gen = mid.G * c.g_frac;
             const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
             const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
             const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
-                                Q_E * (mid.n - mid.p + mid.dop + c.share * mid.fix + mid.an - mid.ca);
+                                Q_E * (mid.n - mid.p + segv(Lv, mid.sid, SEG_DOP) + c.share * mid.fix + mid.an - mid.ca);
             double A[9], B[9], C[9], r[3 * K];
             A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
             A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
@@ -686,8 +678,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             if constexpr (K > 1) {
                 // gamma_i = W_i - c'_{i+1}^T gamma_{i+1}; W_i only in the potential's row
                 const double hLm = cur.h * L, hLp = mid.h * L;
-                const double w0 = 0.5 * (mid.am * cur.am * hLm + ram * mid.am * hLp) * mid.an * iVt * itot0;
-                const double w1 = -0.5 * (mid.cm * cur.cm * hLm + rcm * mid.cm * hLp) * mid.ca * iVt * itot1;
+                const double mida = mam(mid.sid), midc = mcm(mid.sid);
+                const double w0 = 0.5 * (mida * mam(cur.sid) * hLm + mam(rsid) * mida * hLp) * mid.an * iVt * itot0;
+                const double w1 = -0.5 * (midc * mcm(cur.sid) * hLm + mcm(rsid) * midc * hLp) * mid.ca * iVt * itot1;
                 double ng[3][2];
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {
@@ -713,12 +706,22 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
 #pragma unroll
             for (int q = 0; q < 3 * K; ++q) dp[q] = dn_[q];
         }
-        rV = mid.V; ram = mid.am; rcm = mid.cm;
+        rV = mid.V; rsid = mid.sid;
         mid = cur;
         eR = eL;
     };
-    visit(NP, false);
-    for (int j = NP - 1; j >= 0; --j) visit(j, true);
+    Raw now = nxt;
+    visit(NP, false, now, nxt);
+    {   // the stores row NP will overwrite: the loop is then entered, as it repeats, with
+        // the next point's loads followed by a row's stores
+        DDH_GLOBAL T *f = fac + (size_t)(NP - 1) * NFAC * n;
+#pragma unroll
+        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = T(0);
+    }
+    for (int j = NP - 1; j >= 0; --j) {
+        now = nxt;
+        visit(j, true, now, nxt);
+    }
     z[0] = z[1] = 0.0;
     if constexpr (K > 1) {
         // the constraints (solver.py:1255-1265) and their Schur complement; a slot without
@@ -738,25 +741,13 @@ template <typename T, int K>
 DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, const double *z,
                              double *out /* size, J, finite */) {
     const SweepCtl c = cref;  // in registers (as in the elimination)
-    const int NP = a.NP;
+    const int NP = DDH_UNIFORM(a.NP);
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, nfl = SC(SC_NFLOOR), lnfloor = log(nfl);
     const double capv = CAP_V * Vt;
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
     DDH_GLOBAL const T *fac = (DDH_GLOBAL const T *)(factors<T>(a) + lane);
     const double z0 = z[0], z1 = z[1];
-    int id = -1;
-    double offn = 0, offp = 0, kd = 0, ni2 = 0, Nt = 0, Cn = 0, Cp = 0, nt0 = 0, pt0 = 0;
-    double imun = 0, imup = 0;
-    auto seg = [&](int s) {
-        if (s == id) return;
-        id = s;
-        lcdouble *q = Lv.seg + (size_t)s * SEG_N * WAVE;
-        offn = q[SEG_OFFN * WAVE]; offp = q[SEG_OFFP * WAVE]; kd = q[SEG_KD * WAVE];
-        ni2 = q[SEG_NI2 * WAVE]; Nt = q[SEG_NT * WAVE]; Cn = q[SEG_CN * WAVE];
-        Cp = q[SEG_CP * WAVE]; nt0 = q[SEG_NT0 * WAVE]; pt0 = q[SEG_PT0 * WAVE];
-        imun = q[SEG_IMUN * WAVE]; imup = q[SEG_IMUP * WAVE];
-    };
     // point j's loads, issued one point ahead of their use (as in the elimination)
     struct Next {
         T f[9 + 3 * K];
@@ -773,14 +764,14 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
         r.x = Lv.pt[((size_t)j * PT_N + PT_X) * n];
         r.sid = Lv.sid[(size_t)j * n];
     };
-    Next nxt, now;
+    Next nxt;
     fetch(1, nxt);
-    seg(Lv.sid[0]);
     // the left contact
+    const int sid0 = Lv.sid[0];
     const double V0 = contact_V(a, lane, c.Vint, false);
-    const double Vg0n = V0 + offn, Vg0p = V0 + offp;
-    double wn_prev = exp(-Vg0n * iVt) * imun;
-    double wp_prev = exp(Vg0p * iVt) * imup;
+    const double Vg0n = V0 + segv(Lv, sid0, SEG_OFFN), Vg0p = V0 + segv(Lv, sid0, SEG_OFFP);
+    double wn_prev = exp(-Vg0n * iVt) * segv(Lv, sid0, SEG_IMUN);
+    double wp_prev = exp(Vg0p * iVt) * segv(Lv, sid0, SEG_IMUP);
     const double endn0 = exp(SC(SC_LNL) - Vg0n * iVt);
     const double endp0 = exp(SC(SC_LPL) + Vg0p * iVt);
     double U_prev = 0.0, iu_prev = 0.0, x_prev = Lv.pt[PT_X * n];
@@ -788,10 +779,9 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
     double xs0 = 0.0, xs1 = 0.0, xs2 = 0.0;
     double size = 0.0;
     int finite = 1;
-    for (int j = 1; j <= NP; ++j) {
-        now = nxt;
-        seg(now.sid);
+    auto visit = [&](const int j, const Next &now, Next &nxt) {
         fetch(j < NP ? j + 1 : NP, nxt);  // the same loads every visit (as in the elimination)
+        const int sid = now.sid;
         const T *f = now.f;
         double dF[3];
 #pragma unroll
@@ -824,10 +814,13 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
         size = nmax(size, fabs(st1) * (nn / (nn + nfl)));
         size = nmax(size, fabs(st2) * (pp / (pp + nfl)));
         // the integral current of the new state (src/jaxdd/batch.py:321-373)
-        const double R = recombination_value(nn, pp, kd, ni2, Nt, Cn, Cp, nt0, pt0);
+        const double R = recombination_value(nn, pp, segv(Lv, sid, SEG_KD), segv(Lv, sid, SEG_NI2),
+                                             segv(Lv, sid, SEG_NT), segv(Lv, sid, SEG_CN),
+                                             segv(Lv, sid, SEG_CP), segv(Lv, sid, SEG_NT0),
+                                             segv(Lv, sid, SEG_PT0));
         const double U = R - now.G * c.g_frac;
-        const double wn = exp(-(y0 + offn) * iVt) * imun;
-        const double wp = exp((y0 + offp) * iVt) * imup;
+        const double wn = exp(-(y0 + segv(Lv, sid, SEG_OFFN)) * iVt) * segv(Lv, sid, SEG_IMUN);
+        const double wp = exp((y0 + segv(Lv, sid, SEG_OFFP)) * iVt) * segv(Lv, sid, SEG_IMUP);
         const double xj = now.x;
         const double dx = xj - x_prev;
         const double iu = iu_prev + 0.5 * dx * (U_prev + U);
@@ -836,11 +829,21 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
         single_p += 0.5 * dx * (wp_prev + wp);
         double_p += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
+    };
+    // two load buffers taking turns, and the first visit before the loop, so that the loop
+    // is entered as it repeats: a buffer's loads, then a visit's stores (as in the elimination)
+    Next other;
+    visit(1, nxt, other);
+    int j = 2;
+    for (; j + 1 <= NP; j += 2) {
+        visit(j, other, nxt);
+        visit(j + 1, nxt, other);
     }
-    seg(Lv.sid[(size_t)(NP + 1) * n]);
+    if (j == NP) visit(NP, other, nxt);
+    const int sidN = Lv.sid[(size_t)(NP + 1) * n];
     const double VN = contact_V(a, lane, c.Vint, true);
-    const double endnN = exp(SC(SC_LNR) - (VN + offn) * iVt);
-    const double endpN = exp(SC(SC_LPR) + (VN + offp) * iVt);
+    const double endnN = exp(SC(SC_LNR) - (VN + segv(Lv, sidN, SEG_OFFN)) * iVt);
+    const double endpN = exp(SC(SC_LPR) + (VN + segv(Lv, sidN, SEG_OFFP)) * iVt);
     // K = (V_T (sn dens_N ev_N - sn dens_0 ev_0) - sn double) / single, sn = -1 electrons
     const double Kn = (Vt * (-endnN + endn0) + double_n) / single_n;
     const double Kp = (Vt * (endpN - endp0) - double_p) / single_p;
