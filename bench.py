@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
     p.add_argument("--workers", type=int, default=0, help="host processes building devices")
+    p.add_argument("--grad-n", type=int, default=4096,
+                   help="devices of the gradient workload (dJ/dtheta, 16 parameters); 0: skip")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device to device (HBM counter calibration)")
     return p.parse_args()
@@ -143,6 +145,48 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
 
 
+def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
+    """Algorithmic bytes of the adjoint per converged (device, voltage): the assembly at the
+    converged state, the transposed sweep's factors written and read, the parameter
+    contraction's reads of lambda."""
+    return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
+
+
+def gradient_bench(tangents_, V, warm: bool = True):
+    """dJ/dtheta of the gradient workload's curves: one FP64 solve keeping the converged states
+    and one adjoint launch, timed with HIP events on the launch stream."""
+    import torch
+
+    from jaxdd_amd.adjoint import jv_gradients
+
+    packed, tan = tangents_
+    n = packed["sc"].shape[1]
+    if warm:
+        sub = {k: (v[..., :64] if k != "nseg" else v) for k, v in packed.items()}
+        sub["R_series"] = packed["R_series"][:64]
+        jv_gradients(sub, {k: v[..., :64] for k, v in tan.items()}, V)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, ok, its, grad = jv_gradients(packed, tan, V, events=ev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    ok = ok.cpu().numpy()
+    unit = adjoint_b_unit()
+    achieved = float(ok.sum()) * unit / (adj_ms * 1e-3) / 1e9
+    return {
+        "value": n / (t1 - t0), "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters)",
+        "devices": n, "voltages": len(V), "ms": (t1 - t0) * 1e3,
+        "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms,
+        "converged_frac": float(ok.mean()),
+        "finite": bool(np.isfinite(grad.cpu().numpy()).all()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "ddh::adjoint_kernel",
+                     "bytes_per_converged_voltage": unit},
+    }
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per launch of the solve kernel from the committed rocprofv3 counter summary
     (profiles/hbm_traffic.json), when it was measured on this workload."""
@@ -170,6 +214,14 @@ def main() -> int:
     t = time.perf_counter()
     packed = S.packed(args.config, args.n, variant=args.variant, workers=workers)
     build_s = time.perf_counter() - t
+    grad_inputs, grad_host_s = None, 0.0
+    if args.grad_n:
+        from jaxdd_amd import adjoint as A
+
+        t = time.perf_counter()
+        theta = A.theta_from_table(S.draw(args.grad_n), range(args.grad_n))
+        grad_inputs = A.tangents(theta, workers=workers)
+        grad_host_s = time.perf_counter() - t
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.variant, args.cpu_curves)
@@ -251,6 +303,11 @@ def main() -> int:
         out["max_err_vs_fp64"] = float(np.max((np.abs(J - J64) / scale)[both]))
         out["flags_equal_calm"] = bool(np.array_equal(ok[calm], ok64[calm]))
         out["chaotic_points"] = int((~calm).sum())
+    if grad_inputs is not None:
+        out["gradient"] = gradient_bench(grad_inputs, V)
+        out["gradient"]["host_tangents_s"] = grad_host_s
+        out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
+                                       "voltages, FP64, 16 parameters")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -24,6 +24,8 @@ extern "C" {
 /* Segments (runs of points of one material) per device: the solve keeps each wavefront's
  * segment tables in the compute unit's 160 KB local data share. */
 #define DDH_MAX_SEGMENTS 14
+/* The adjoint keeps a second table (the tangents) beside it: at most 7 segments (3 layers). */
+#define DDH_ADJOINT_MAX_SEGMENTS 7
 
 enum ddh_status {
     DDH_OK = 0,
@@ -77,6 +79,29 @@ size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision)
 int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
                     double *J, uint8_t *ok, int32_t *its, int32_t *status, void *workspace,
                     size_t workspace_bytes, void *hip_stream);
+
+/* ddh_solve_batch that also keeps each voltage's final state: states [nV][NP][3][n] doubles
+ * (device memory), (V, ln n, ln p) of every interior point; voltages in the solve's order
+ * (ascending in batch mode). The linearisation points of ddh_adjoint. */
+int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
+                           double *J, uint8_t *ok, int32_t *its, int32_t *status, double *states,
+                           void *workspace, size_t workspace_bytes, void *hip_stream);
+
+/* Workspace bytes ddh_adjoint needs. */
+size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
+
+/* dJ(V)/dtheta at every converged voltage by the implicit function theorem: replaces
+ * src/jaxdd/batch.py:650-737 (`_converged_jvp`, `_tangent`: the transposed bordered solve of
+ * reverse mode) for devices without mobile ions. theta reaches the kernel as the tangents of
+ * the packed batch arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][22][n],
+ * dsc [ntheta][20][n] (d array / d theta_k). states and ok: from ddh_solve_batch_states on the
+ * same batch and ascending V. grad: [n][nV][ntheta]; 0 where a voltage did not converge (the
+ * reference's zero gradient there); NaN and status[d] = 1 where device d has mobile ions.
+ * nseg <= DDH_ADJOINT_MAX_SEGMENTS. bernoulli: as ddh_opts. */
+int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
+                const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
+                const double *dsc, int32_t bernoulli, double *grad, int32_t *status,
+                void *workspace, size_t workspace_bytes, void *hip_stream);
 
 /* The batched block-tridiagonal sweep: replaces src/jaxdd/sweep.py:313-330 (`block_sweep`,
  * the custom_vmap operator over src/jaxdd/solver.py:1534-1560 `_block_thomas`). Solves

@@ -28,6 +28,7 @@ class Opts(ctypes.Structure):
 
 #: The symbols include/ddhip.h declares.
 SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solve_batch",
+           "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
            "ddh_block_sweep")
 
 _lib = None
@@ -50,6 +51,14 @@ def load():
     lib.ddh_solve_batch.restype = ctypes.c_int
     lib.ddh_solve_batch.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts), vp, vp,
                                     vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_solve_batch_states.restype = ctypes.c_int
+    lib.ddh_solve_batch_states.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts),
+                                           vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_adjoint_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_adjoint_workspace_bytes.argtypes = [i32, i32, i32]
+    lib.ddh_adjoint.restype = ctypes.c_int
+    lib.ddh_adjoint.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, i32, vp, vp, vp, i32, vp,
+                                vp, vp, ctypes.c_size_t, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 1:

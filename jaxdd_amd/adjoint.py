@@ -1,0 +1,197 @@
+"""Gradients of J-V curves: dJ(V)/dtheta by the implicit function theorem (the reference's
+`_converged_jvp`, src/jaxdd/batch.py:650-737, taken through `jax.grad` over
+`device_tree(structure, params)`, src/jaxdd/batch.py:142-148).
+
+The parameters are those of the gradient workload, 16 per device (`THETA`): the three layer
+thicknesses, the electron and hole mobility of each layer, the ETL donor and the perovskite's
+and HTL's acceptor densities, the perovskite's bulk trap density, its SRH lifetime
+tau = 1 / (N_t C) (through the capture coefficient C_n = C_p = C), its generation rate and an
+offset of the left work function. They reach the GPU as the tangents of the packed batch
+arrays, d(array)/d(theta_k), taken here by central differences of the host device builder
+(which is smooth in theta at a fixed grid layout; a layout change inside the step is refused).
+The HIP adjoint kernel (`ddh_adjoint`) then linearises each converged voltage's residual and
+current in those directions and solves one transposed block-tridiagonal system per voltage.
+
+`curves(theta, V)` is the differentiable entry point: a `torch.autograd.Function` whose
+forward solves the curves and whose backward contracts the incoming dL/dJ with dJ/dtheta.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import synthetic as S
+from .batch import SC_FIELDS, from_packed, pack
+
+THETA = ("L_etl", "L_pvk", "L_htl",
+         "mu_n_etl", "mu_p_etl", "mu_n_pvk", "mu_p_pvk", "mu_n_htl", "mu_p_htl",
+         "N_D_etl", "N_A_pvk", "N_A_htl", "N_t", "tau", "G", "dW_L")
+#: The perovskite traps' capture coefficient at the drawn devices (BASE C_n_bulk, C_p_bulk).
+C_TRAP = 1e-14
+#: Relative step of the builder's central differences (an absolute step in eV for dW_L).
+REL_STEP = 1e-6
+
+
+def theta_from_table(t: dict, rows) -> np.ndarray:
+    """theta (len(rows), 16) of drawn devices (synthetic.draw): tau = 1 / (N_t C_TRAP), dW_L 0."""
+    rows = np.asarray(rows)
+    th = np.zeros((len(rows), len(THETA)))
+    th[:, 0], th[:, 1], th[:, 2] = t["L_etl"][rows], t["L_pvk"][rows], t["L_htl"][rows]
+    th[:, 3:9] = t["mu"][rows]
+    th[:, 9:12] = t["dop"][rows]
+    th[:, 12] = t["N_t"][rows]
+    th[:, 13] = 1.0 / (t["N_t"][rows] * C_TRAP)
+    th[:, 14] = t["G"][rows]
+    return th
+
+
+def device_from_theta(theta, NP: int = 400):
+    """The device of one parameter vector (the drawn workload's stack, no ions)."""
+    th = np.asarray(theta, dtype=float)
+    L = th[0:3].copy()
+    mu = th[3:9]
+    C = 1.0 / (th[12] * th[13])
+    p = dict(S.BASE, **S.SCALARS)
+    p.update(
+        L=L, mu_n=mu[0::2].copy(), mu_p=mu[1::2].copy(),
+        N_D=np.array([th[9], 0.0, 0.0]), N_A=np.array([0.0, th[10], th[11]]),
+        N_t_bulk=np.array([0.0, th[12], 0.0]), G_ehp=np.array([0.0, th[14], 0.0]),
+        C_n_bulk=np.array([C_TRAP, C, C_TRAP]), C_p_bulk=np.array([C_TRAP, C, C_TRAP]),
+        W_L=S.SCALARS["W_L"] + th[15],
+    )
+    return S.device_arrays(S._structure(L, NP), p)
+
+
+def _steps(th: np.ndarray) -> np.ndarray:
+    h = REL_STEP * np.abs(th)
+    h[..., 15] = REL_STEP
+    return h
+
+
+def _tangent_rows(args):
+    thetas, NP = args
+    base = pack([device_from_theta(t, NP) for t in thetas])
+    n, K = len(thetas), thetas.shape[1]
+    out = {"pt": np.zeros((K,) + base["pt"].shape), "seg": np.zeros((K,) + base["seg"].shape),
+           "sc": np.zeros((K,) + base["sc"].shape)}
+    h = _steps(thetas)
+    for k in range(K):
+        e = np.zeros(K)
+        e[k] = 1.0
+        hi = pack([device_from_theta(t + hk * e, NP) for t, hk in zip(thetas, h[:, k])])
+        lo = pack([device_from_theta(t - hk * e, NP) for t, hk in zip(thetas, h[:, k])])
+        for side in (hi, lo):
+            if side["nseg"] != base["nseg"] or not np.array_equal(side["sid"], base["sid"]):
+                raise ValueError(f"the grid layout changes within the step of {THETA[k]}")
+        for key in ("pt", "seg", "sc"):
+            out[key][k] = (hi[key] - lo[key]) / (2.0 * h[:, k])
+    return base, out
+
+
+def tangents(thetas, NP: int = 400, workers: int = 1, chunk: int = 64):
+    """(packed batch, tangents): the packed arrays of the devices of ``thetas`` (n, 16) and
+    their derivatives in each parameter, {"pt": (16, NP+2, 4, n), "seg": (16, nseg, 22, n),
+    "sc": (16, 20, n)}, by central differences of the builder. ``workers`` > 1 forks processes
+    (call it before the process touches the GPU)."""
+    from .batch import merge
+
+    thetas = np.atleast_2d(np.asarray(thetas, dtype=float))
+    jobs = [(thetas[s:s + chunk], NP) for s in range(0, len(thetas), chunk)]
+    if workers <= 1:
+        parts = [_tangent_rows(j) for j in jobs]
+    else:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(workers) as pool:
+            parts = pool.map(_tangent_rows, jobs)
+    nseg = max(p[0]["nseg"] for p in parts)
+    if any(p[0]["nseg"] != nseg for p in parts):
+        raise ValueError("devices with different segment counts in one gradient batch")
+    packed = merge([p[0] for p in parts])
+    tan = {k: np.concatenate([p[1][k] for p in parts], axis=-1) for k in ("pt", "seg", "sc")}
+    return packed, tan
+
+
+def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
+                 bernoulli_name="exact", device="cuda", return_states=False, events=None):
+    """J (n, nV), converged, iterations and dJ/dtheta (n, nV, 16) on the GPU for ascending
+    voltages: the FP64 solve keeping each voltage's final state, then the adjoint kernel.
+    Unconverged voltages get zero gradients. ``return_states``: also the linearisation points
+    (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream around the
+    solve kernel and around the adjoint kernel."""
+    lib = _lib.load()
+    V = np.asarray(voltages, dtype=float)
+    if np.any(np.diff(V) <= 0):
+        raise ValueError("jv_gradients takes ascending voltages")
+    batch = from_packed(packed, device)
+    n, NP, nV = batch["n"], batch["NP"], len(V)
+    opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
+                     0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0)
+    dev = batch["_pt"].device
+    Vd = torch.as_tensor(V, device=dev)
+    J = torch.empty((n, nV), dtype=torch.float64, device=dev)
+    ok = torch.zeros((n, nV), dtype=torch.uint8, device=dev)
+    its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
+    nbytes = lib.ddh_workspace_bytes(n, NP, nV, _lib.DDH_FP64)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
+                   batch["_seg"].data_ptr(), batch["_sc"].data_ptr())
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    K = tan["pt"].shape[0]
+    dpt, dseg, dsc = (torch.as_tensor(np.ascontiguousarray(tan[k]), device=dev)
+                      for k in ("pt", "seg", "sc"))
+    grad = torch.empty((n, nV, K), dtype=torch.float64, device=dev)
+    astatus = torch.zeros(n, dtype=torch.int32, device=dev)
+    abytes = lib.ddh_adjoint_workspace_bytes(n, NP, nV)
+    aws = torch.empty(abytes, dtype=torch.uint8, device=dev)
+    if events is not None:
+        events[0].record()
+    _lib.check(lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
+                                          J.data_ptr(), ok.data_ptr(), its.data_ptr(),
+                                          status.data_ptr(), states.data_ptr(), ws.data_ptr(),
+                                          nbytes, stream), "ddh_solve_batch_states")
+    if events is not None:
+        events[1].record()
+        events[2].record()
+    _lib.check(lib.ddh_adjoint(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(),
+                               ok.data_ptr(), K, dpt.data_ptr(), dseg.data_ptr(), dsc.data_ptr(),
+                               opts.bernoulli, grad.data_ptr(), astatus.data_ptr(), aws.data_ptr(),
+                               abytes, stream), "ddh_adjoint")
+    if events is not None:
+        events[3].record()
+    torch.cuda.current_stream(dev).synchronize()
+    if int(astatus.max()) != 0:
+        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+    G_shunt = batch["_sc"][SC_FIELDS.index("G_shunt")]
+    J = J + G_shunt[:, None] * Vd[None, :]
+    if return_states:
+        return J, ok.bool(), its.long(), grad, states
+    return J, ok.bool(), its.long(), grad
+
+
+class _Curves(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, voltages, tol, max_iter, workers):
+        packed, tan = tangents(theta.detach().cpu().numpy(), workers=workers)
+        J, ok, its, grad = jv_gradients(packed, tan, voltages, tol=tol, max_iter=max_iter)
+        ctx.save_for_backward(grad.to(theta.device))
+        ctx.mark_non_differentiable(ok)
+        return J.to(theta.device), ok.to(theta.device)
+
+    @staticmethod
+    def backward(ctx, J_bar, ok_bar):
+        (grad,) = ctx.saved_tensors
+        return torch.einsum("nv,nvk->nk", J_bar.to(grad.dtype), grad), None, None, None, None
+
+
+def curves(theta: torch.Tensor, voltages, tol=1e-10, max_iter=60, workers=1):
+    """J-V curves (n, nV) and converged flags of the devices of ``theta`` (n, 16) float64, at
+    ascending ``voltages``, differentiable in theta (backward: dJ/dtheta from the adjoint
+    kernel; zero at unconverged voltages, as the reference's)."""
+    return _Curves.apply(theta, voltages, tol, max_iter, workers)

@@ -83,6 +83,7 @@ struct Args {
     double *pfac;     // [NP][4][n] Poisson's scalar factors
     double *fix;      // [NP][n] net fixed ions in use (frozen after the pre-bias)
     LaneState *st;    // [n]
+    double *states;   // [nV][NP][3][n] each voltage's final state (NULL: not kept)
     double *J;        // [n][nV]
     uint8_t *ok;
     int32_t *its;
@@ -522,6 +523,59 @@ DDH_FN void to_t(const double *in, T *out, int m) {
 #define DDH_SWEEP __device__ __attribute__((noinline))
 #endif
 
+// Row i's Jacobian blocks of the scaled residual F (A: point i-1, B: point i, C: point i+1)
+// and the Newton right-hand side -F, with the ion border's two columns where K = 3
+// (src/jaxdd/solver.py:1021-1187, src/jaxdd/batch.py:518-549). cur, mid: points i-1, i; rV:
+// point i+1's potential; eL, eR: the edges (i-1, i) and (i, i+1); last: i = NP, whose
+// coupling to the contact is not an unknown.
+template <int K>
+DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
+                       const Edge &eR, bool last, double L2, double iVt, double share,
+                       double g_frac, double *A, double *B, double *C, double *r) {
+    const double hm = cur.h, hp = mid.h, hs = hp + hm;
+    const double D = 1.0 / (hp * hm * hs * L2);
+    const double fac_ = 0.5 * L2 * hp * hm * hs;
+    const double meps = segv(Lv, mid.sid, SEG_EPS);
+    const double fu = 2 * meps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
+    const double fl0 = 2 * meps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
+    const double deps = meps - segv(Lv, cur.sid, SEG_EPS);  // the jump of eps between layers
+    const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
+    const double fac_m = jump - (fu + fl0);
+    const double fac_l = fl0 - jump;
+    const double sP = segv(Lv, mid.sid, SEG_SP);
+    const double sC = segv(Lv, mid.sid, SEG_SCS) * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
+    const double gen = mid.G * g_frac;
+    const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
+    const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
+    const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
+                        Q_E * (mid.n - mid.p + segv(Lv, mid.sid, SEG_DOP) + share * mid.fix + mid.an - mid.ca);
+    A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
+    A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
+    A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
+    B[0] = (fac_m - Q_E * (mid.an + mid.ca) * iVt) * sP;
+    B[1] = -Q_E * mid.n * sP;
+    B[2] = Q_E * mid.p * sP;
+    B[3] = (-hp * eL.gn - hm * eR.gn) * sC;
+    B[4] = (-hp * eL.an1 + hm * eR.an0 - fac_ * mid.dRn) * sC;
+    B[5] = (-fac_ * mid.dRp) * sC;
+    B[6] = (hp * eL.gp + hm * eR.gp) * sC;
+    B[7] = (-fac_ * mid.dRn) * sC;
+    B[8] = (hp * eL.ap1 - hm * eR.ap0 - fac_ * mid.dRp) * sC;
+    if (last) {  // the coupling to the contact is not an unknown
+#pragma unroll
+        for (int q = 0; q < 9; ++q) C[q] = 0.0;
+    } else {
+        C[0] = fu * sP; C[1] = 0.0; C[2] = 0.0;
+        C[3] = hm * eR.gn * sC; C[4] = hm * eR.an1 * sC; C[5] = 0.0;
+        C[6] = -hm * eR.gp * sC; C[7] = 0.0; C[8] = -hm * eR.ap1 * sC;
+    }
+    r[0 * K] = -pois * sP; r[1 * K] = -elec * sC; r[2 * K] = -hole * sC;
+    if constexpr (K > 1) {
+        r[0 * K + 1] = -Q_E * mid.an * sP; r[0 * K + 2] = Q_E * mid.ca * sP;
+        r[1 * K + 1] = 0.0; r[1 * K + 2] = 0.0; r[2 * K + 1] = 0.0; r[2 * K + 2] = 0.0;
+    }
+}
+
 // Elimination sweep (right to left). Returns the ion step z.
 template <typename T, int K>
 DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double *z) {
@@ -578,49 +632,8 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
         if (has_row) {
             // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
             const int i = j + 1;
-            const double hm = cur.h, hp = mid.h, hs = hp + hm;
-            const double D = 1.0 / (hp * hm * hs * L2);
-            const double fac_ = 0.5 * L2 * hp * hm * hs;
-            const double meps = segv(Lv, mid.sid, SEG_EPS);
-            const double fu = 2 * meps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
-            const double fl0 = 2 * meps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
-            const double deps = meps - segv(Lv, cur.sid, SEG_EPS);  // the jump of eps between layers
-            const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
-            const double fac_m = jump - (fu + fl0);
-            const double fac_l = fl0 - jump;
-            const double sP = segv(Lv, mid.sid, SEG_SP);
-            const double sC = segv(Lv, mid.sid, SEG_SCS) * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
-            const double gen = mid.G * c.g_frac;
-            const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
-            const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
-            const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
-                                Q_E * (mid.n - mid.p + segv(Lv, mid.sid, SEG_DOP) + c.share * mid.fix + mid.an - mid.ca);
             double A[9], B[9], C[9], r[3 * K];
-            A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
-            A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
-            A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
-            B[0] = (fac_m - Q_E * (mid.an + mid.ca) * iVt) * sP;
-            B[1] = -Q_E * mid.n * sP;
-            B[2] = Q_E * mid.p * sP;
-            B[3] = (-hp * eL.gn - hm * eR.gn) * sC;
-            B[4] = (-hp * eL.an1 + hm * eR.an0 - fac_ * mid.dRn) * sC;
-            B[5] = (-fac_ * mid.dRp) * sC;
-            B[6] = (hp * eL.gp + hm * eR.gp) * sC;
-            B[7] = (-fac_ * mid.dRn) * sC;
-            B[8] = (hp * eL.ap1 - hm * eR.ap0 - fac_ * mid.dRp) * sC;
-            if (i == NP) {  // the coupling to the contact is not an unknown
-#pragma unroll
-                for (int q = 0; q < 9; ++q) C[q] = 0.0;
-            } else {
-                C[0] = fu * sP; C[1] = 0.0; C[2] = 0.0;
-                C[3] = hm * eR.gn * sC; C[4] = hm * eR.an1 * sC; C[5] = 0.0;
-                C[6] = -hm * eR.gp * sC; C[7] = 0.0; C[8] = -hm * eR.ap1 * sC;
-            }
-            r[0 * K] = -pois * sP; r[1 * K] = -elec * sC; r[2 * K] = -hole * sC;
-            if constexpr (K > 1) {
-                r[0 * K + 1] = -Q_E * mid.an * sP; r[0 * K + 2] = Q_E * mid.ca * sP;
-                r[1 * K + 1] = 0.0; r[1 * K + 2] = 0.0; r[2 * K + 1] = 0.0; r[2 * K + 2] = 0.0;
-            }
+            row_blocks<K>(Lv, cur, mid, rV, eL, eR, i == NP, L2, iVt, c.share, c.g_frac, A, B, C, r);
             if constexpr (sizeof(T) == 4) {
                 // row equilibration by powers of two (exact) before the rounding to FP32
 #pragma unroll
@@ -1040,6 +1053,12 @@ DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
         a.J[o] = J;
         a.ok[o] = (uint8_t)((ok && S.pre_ok) ? 1 : 0);
         a.its[o] = S.it;
+        if (a.states) {  // the state the voltage ended with (the adjoint's linearisation point)
+            const size_t n = (size_t)a.n, m = (size_t)a.NP * 3;
+            const double *src = a.Y + (size_t)S.b_work * m * n + lane;
+            double *dst = a.states + (size_t)idx * m * n + lane;
+            for (size_t q = 0; q < m; ++q) dst[q * n] = src[q * n];
+        }
         if (ok && fin) {  // only a converged, finite solution is carried on
             S.b_before = S.b_last; S.b_last = S.b_work;
             S.lbefore[0] = S.llast[0]; S.lbefore[1] = S.llast[1];

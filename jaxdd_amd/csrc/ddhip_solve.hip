@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "ddhip_core.h"
+#include "ddhip_adjoint.h"
 
 namespace ddh {
 
@@ -33,6 +34,17 @@ __global__ void __launch_bounds__(64) solve_kernel(Args a) {
             else newton_lane_step<float>(a, lane, ions, lseg);
         }
     }
+}
+
+// The gradients of every device's J-V curve: one lane per device; the wavefront's segment
+// tables and the current direction's tangent table in LDS.
+__global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
+    extern __shared__ double lds[];  // 2 * a.nseg * SEG_N * WAVE doubles
+    const int lane = blockIdx.x * 64 + threadIdx.x;
+    if (lane >= a.n) return;
+    ldouble *lval = (ldouble *)lds + threadIdx.x;
+    ldouble *ltan = lval + (size_t)a.nseg * SEG_N * WAVE;
+    adjoint_lane(a, lane, lval, ltan);
 }
 
 }  // namespace ddh
@@ -201,10 +213,9 @@ extern "C" size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t
     return ws_layout(n, NP, precision).total;
 }
 
-extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
-                               const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
-                               int32_t *status, void *workspace, size_t workspace_bytes,
-                               void *stream) {
+static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *o,
+                       double *J, uint8_t *ok, int32_t *its, int32_t *status, double *states,
+                       void *workspace, size_t workspace_bytes, void *stream) {
     if (!b || !V || !o || !J || !ok || !its || !status || !workspace)
         return fail(DDH_E_ARG, "ddh_solve_batch: NULL argument");
     if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
@@ -227,6 +238,7 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
     a.pfac = (double *)(base + w.pfac);
     a.fix = (double *)(base + w.fix);
     a.st = (ddh::LaneState *)(base + w.st);
+    a.states = states;
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = o->tol; a.max_iter = o->max_iter; a.start_iter = o->start_iter;
     a.strict = o->strict; a.bern = o->bernoulli; a.jv = o->jv_mode;
@@ -239,6 +251,75 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
     hipLaunchKernelGGL(ddh::solve_kernel, grid, block, lds, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "solve_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
+                               const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
+                               int32_t *status, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32_t nV,
+                                      const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
+                                      int32_t *status, double *states, void *workspace,
+                                      size_t workspace_bytes, void *stream) {
+    if (!states) return fail(DDH_E_ARG, "ddh_solve_batch_states: NULL states");
+    return solve_batch(b, V, nV, o, J, ok, its, status, states, workspace, workspace_bytes, stream);
+}
+
+struct AdjLayout {
+    size_t fac, lam, iu, total;
+};
+
+static AdjLayout adj_layout(int32_t n, int32_t NP, int32_t nV) {
+    AdjLayout w;
+    size_t off = 0;
+    w.fac = off; off = align256(off + (size_t)NP * ddh::ADJ_F * n * sizeof(double));
+    w.lam = off; off = align256(off + (size_t)nV * NP * 3 * n * sizeof(double));
+    w.iu = off; off = align256(off + (size_t)(NP + 1) * n * sizeof(double));
+    w.total = off;
+    return w;
+}
+
+extern "C" size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV) {
+    if (n <= 0 || NP < 2 || nV < 1) return 0;
+    return adj_layout(n, NP, nV).total;
+}
+
+extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
+                           const uint8_t *ok, int32_t ntheta, const double *dpt,
+                           const double *dseg, const double *dsc, int32_t bernoulli,
+                           double *grad, int32_t *status, void *workspace,
+                           size_t workspace_bytes, void *stream) {
+    if (!b || !V || !states || !ok || !dpt || !dseg || !dsc || !grad || !status || !workspace)
+        return fail(DDH_E_ARG, "ddh_adjoint: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || ntheta < 1 || b->nseg < 1 ||
+        b->nseg > DDH_ADJOINT_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_adjoint: n > 0, 2 <= NP <= 65536, nV >= 1, ntheta >= 1, "
+                                 "1 <= nseg <= 7");
+    if (bernoulli != 0 && bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_adjoint: bernoulli must be 0 (exact) or 1 (simss)");
+    AdjLayout w = adj_layout(b->n, b->NP, nV);
+    if (workspace_bytes < w.total) return fail(DDH_E_WORKSPACE, "ddh_adjoint: workspace too small");
+    char *base = (char *)workspace;
+    ddh::AdjArgs a;
+    a.n = b->n; a.NP = b->NP; a.nV = nV; a.nseg = b->nseg; a.ntheta = ntheta; a.bern = bernoulli;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
+    a.V = V; a.states = states; a.ok = ok; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
+    a.grad = grad; a.status = status;
+    a.fac = (double *)(base + w.fac);
+    a.lam = (double *)(base + w.lam);
+    a.iu = (double *)(base + w.iu);
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((b->n + 63) / 64), block(64);
+    const size_t lds = 2 * (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
+    (void)hipFuncSetAttribute((const void *)ddh::adjoint_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, lds, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "adjoint_kernel launch");
     return DDH_OK;
 }
 

@@ -6,6 +6,7 @@
 // compared with the oracle on a host without a GPU. Nothing in the product loads it.
 #define DDH_CPU_EMULATION 1
 #include "ddhip_core.h"
+#include "ddhip_adjoint.h"
 
 #include <cstdlib>
 #include <vector>
@@ -55,6 +56,9 @@ static void run(Args &a) {
 static int g_poison = 0;  // fill the workspace with this byte before a solve (0: zeros)
 extern "C" void emu_set_poison(int byte) { g_poison = byte; }
 
+static double *g_states = nullptr;  // where the next solve keeps each voltage's final state
+extern "C" void emu_keep_states(double *states) { g_states = states; }
+
 extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                                const double *seg, const double *sc, const double *V, int nV,
                                double tol, int max_iter, int start_iter, int strict, int bern,
@@ -77,10 +81,30 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V;
     a.Y = Y.data(); a.fac64 = fac64.data(); a.fac32 = fac32.data(); a.pfac = pfac.data();
-    a.fix = fix.data(); a.st = st.data();
+    a.fix = fix.data(); a.st = st.data(); a.states = g_states;
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = tol; a.max_iter = max_iter; a.start_iter = start_iter; a.strict = strict;
     a.bern = bern; a.jv = jv; a.mixed = precision ? 1 : 0;
     run(a);
+    return 0;
+}
+
+// The adjoint kernel's lanes in turn (each wavefront's LDS tables in host memory).
+extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
+                           const double *seg, const double *sc, const double *V, int nV,
+                           const double *states, const uint8_t *ok, int ntheta,
+                           const double *dpt, const double *dseg, const double *dsc, int bern,
+                           double *grad, int32_t *status) {
+    std::vector<double> fac((size_t)NP * ADJ_F * n), lam((size_t)nV * NP * 3 * n),
+        iu((size_t)(NP + 1) * n), lds((size_t)2 * nseg * SEG_N * WAVE);
+    AdjArgs a;
+    a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg; a.ntheta = ntheta; a.bern = bern;
+    a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
+    a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
+    a.fac = fac.data(); a.lam = lam.data(); a.iu = iu.data();
+    for (int lane = 0; lane < n; ++lane) {
+        double *lval = lds.data() + (lane % WAVE);
+        adjoint_lane(a, lane, lval, lval + (size_t)nseg * SEG_N * WAVE);
+    }
     return 0;
 }

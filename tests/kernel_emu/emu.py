@@ -30,6 +30,9 @@ def lib():
         vp, i = ctypes.c_void_p, ctypes.c_int
         _lib.emu_solve_batch.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, ctypes.c_double, i, i,
                                          i, i, i, i, vp, vp, vp, vp]
+        _lib.emu_keep_states.argtypes = [vp]
+        _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, i, vp, vp, vp, i,
+                                     vp, vp]
     return _lib
 
 
@@ -38,9 +41,10 @@ def _p(a):
 
 
 def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=False,
-                 bernoulli="exact", precision="fp64", jv=False):
+                 bernoulli="exact", precision="fp64", jv=False, states=None):
     """(J without the shunt, ok, its, status) for packed devices at voltages V (ascending in
-    batch mode, the given order in jv mode)."""
+    batch mode, the given order in jv mode). ``states``: an array (nV, NP, 3, n) that receives
+    each voltage's final state."""
     n = packed["sc"].shape[1]
     NP = packed["pt"].shape[0] - 2
     V = np.ascontiguousarray(V, dtype=np.float64)
@@ -50,8 +54,30 @@ def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=Fals
     its = np.zeros((n, nV), dtype=np.int32)
     status = np.zeros(n, dtype=np.int32)
     arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    if states is not None:
+        assert states.shape == (nV, NP, 3, n) and states.dtype == np.float64
+        assert states.flags.c_contiguous
+    lib().emu_keep_states(None if states is None else _p(states))
     lib().emu_solve_batch(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV, tol,
                           max_iter, start_iter or max_iter, int(strict),
                           0 if bernoulli == "exact" else 1, 0 if precision == "fp64" else 1,
                           int(jv), _p(J), _p(ok), _p(its), _p(status))
     return J, ok.astype(bool), its, status
+
+
+def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact"):
+    """dJ/dtheta (n, nV, ntheta) and status (n,) of the adjoint kernel's lanes, replayed."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    nV = len(V)
+    dpt, dseg, dsc = (np.ascontiguousarray(tangents[k]) for k in ("pt", "seg", "sc"))
+    K = dpt.shape[0]
+    grad = np.zeros((n, nV, K))
+    status = np.zeros(n, dtype=np.int32)
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    okb = np.ascontiguousarray(ok, dtype=np.uint8)
+    lib().emu_adjoint(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV,
+                      _p(np.ascontiguousarray(states)), _p(okb), K, _p(dpt), _p(dseg), _p(dsc),
+                      0 if bernoulli == "exact" else 1, _p(grad), _p(status))
+    return grad, status

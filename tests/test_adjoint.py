@@ -1,0 +1,134 @@
+"""The adjoint (dJ/dtheta by the implicit function theorem) on the CPU: the adjoint kernel's
+per-lane code replayed (tests/kernel_emu) against central finite differences of the oracle's
+curves, in the reference's own form (tests/test_solver.py:1784-1811: a random weighting of
+the curves, a random direction in the parameters, the directional derivative against central
+differences of the weighted sum)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+N_DEV = 4
+V_SUB = slice(None, None, 6)  # 11 of the 61 voltages
+
+
+@pytest.fixture(scope="module")
+def c4_case(emu):
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(N_DEV), range(N_DEV))
+    packed, tan = A.tangents(th)
+    V = S.VOLTAGES[V_SUB].copy()
+    states = np.zeros((len(V), 400, 3, N_DEV))
+    J, ok, its, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                     states=states)
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    return th, packed, tan, V, J, ok, grad, status
+
+
+def _theta_at(th, d, a):
+    out = th * (1.0 + a * d)
+    out[:, 15] = th[:, 15] + a * d[:, 15] * 0.01  # the work-function offset: 10 mV units
+    return out
+
+
+def test_tangents_are_the_builders_derivatives():
+    """The packed arrays' tangents by the builder's central differences agree with one-sided
+    differences of a ten times larger step (a smooth builder at a fixed grid layout)."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    th = A.theta_from_table(S.draw(1), [0])
+    base, tan = A.tangents(th)
+    for k in (1, 3, 10, 14, 15):
+        h = 10 * A._steps(th)[0, k]
+        e = np.zeros(16)
+        e[k] = h
+        hi = pack([A.device_from_theta(th[0] + e)])
+        for key in ("pt", "seg", "sc"):
+            one = (hi[key] - base[key]) / h
+            ref = np.max(np.abs(tan[key][k])) + 1e-300
+            assert np.max(np.abs(one - tan[key][k])) < 1e-3 * ref, (A.THETA[k], key)
+
+
+def test_adjoint_is_the_finite_difference_of_the_curves(c4_case, oracle):
+    from jaxdd_amd import adjoint as A
+
+    th, _, _, V, J, ok, grad, status = c4_case
+    assert not status.any() and ok.all()
+    rng = np.random.default_rng(0)
+    w = rng.normal(size=ok.shape)
+    d = rng.normal(size=th.shape)
+    scale = th.copy()
+    scale[:, 15] = 0.01
+    by_adjoint = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
+
+    def loss(a):
+        devs = [A.device_from_theta(x) for x in _theta_at(th, d, a)]
+        Jx, okx, _ = oracle.solve_batch(devs, V, tol=1e-12, strict=True, first_max_iter=None,
+                                        max_iter=200)
+        assert okx.all()
+        return np.sum(w * Jx)
+
+    h = 1e-5
+    fd = (loss(h) - loss(-h)) / (2 * h)
+    assert abs(by_adjoint - fd) < 1e-5 * abs(fd)
+
+
+def test_each_parameter_against_finite_differences(c4_case, oracle):
+    """Per device and parameter, dJ/dtheta_k of the whole curve against central differences
+    (relative step 1e-5), measured as the reference measures J: the error of dJ/dln(theta_k)
+    relative to the curve's current scale."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import j_scale
+
+    th, _, _, V, J, ok, grad, _ = c4_case
+    dev_idx = 1
+    dev = A.device_from_theta(th[dev_idx])
+    cscale = j_scale(dev) + np.max(np.abs(J[dev_idx]))
+    for k in range(16):
+        step = 1e-5 * abs(th[dev_idx, k]) if k != 15 else 1e-5
+        e = np.zeros(16)
+        e[k] = step
+        Jp = oracle.solve_batch([A.device_from_theta(th[dev_idx] + e)], V, tol=1e-12,
+                                strict=True, first_max_iter=None, max_iter=200)[0][0]
+        Jm = oracle.solve_batch([A.device_from_theta(th[dev_idx] - e)], V, tol=1e-12,
+                                strict=True, first_max_iter=None, max_iter=200)[0][0]
+        fd = (Jp - Jm) / (2 * step)
+        unit = abs(th[dev_idx, k]) if k != 15 else 1.0
+        err = np.max(np.abs(grad[dev_idx, :, k] - fd)) * unit / cscale
+        assert err < 1e-5, (A.THETA[k], err)
+
+
+def test_unconverged_voltages_have_zero_gradient(emu):
+    """As the reference's (tests/test_solver.py:1814-1829): a voltage that misses max_iter
+    contributes a zero, finite gradient."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(2), [0, 1])
+    packed, tan = A.tangents(th)
+    V = S.VOLTAGES[::10].copy()
+    states = np.zeros((len(V), 400, 3, 2))
+    J, ok, its, _ = emu.solve_packed(packed, V, max_iter=3, start_iter=60, states=states)
+    assert ok.any() and not ok.all()
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    assert np.isfinite(grad).all()
+    assert np.all(grad[~ok] == 0.0) and np.any(grad[ok] != 0.0)
+
+
+def test_devices_with_mobile_ions_are_refused(emu):
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    packed = pack(S.devices("c3", 1))
+    V = S.VOLTAGES[::20].copy()
+    states = np.zeros((len(V), 400, 3, 1))
+    J, ok, its, _ = emu.solve_packed(packed, V, states=states)
+    tan = {"pt": np.zeros((1,) + packed["pt"].shape), "seg": np.zeros((1,) + packed["seg"].shape),
+           "sc": np.zeros((1,) + packed["sc"].shape)}
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    assert status[0] == 1 and np.isnan(grad).all()

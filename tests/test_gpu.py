@@ -202,3 +202,77 @@ def test_full_batch_properties(torch_cuda):
     assert np.max((np.abs(Jm - J64) / scale)[both]) < 1e-6
     i0 = int(np.argmin(np.abs(S.VOLTAGES)))
     assert np.all(-J64[:, i0][ok64[:, i0]] <= _np(batch["J_scale"])[ok64[:, i0]] * (1 + 1e-9))
+
+
+# ---- the adjoint: dJ/dtheta on the GPU ----------------------------------------------------
+@pytest.fixture(scope="module")
+def c4_gpu(torch_cuda):
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(4), range(4))
+    packed, tan = A.tangents(th)
+    V = S.VOLTAGES[::6].copy()
+    J, ok, its, grad, states = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True,
+                                              max_iter=200, return_states=True)
+    return th, packed, tan, V, _np(J), _np(ok), _np(grad), _np(states)
+
+
+def test_adjoint_kernel_is_its_cpu_replay(c4_gpu, emu):
+    """The HIP adjoint through the C-ABI against the same per-lane code on the CPU, both
+    linearised at the GPU's converged states."""
+    th, packed, tan, V, J, ok, grad, states = c4_gpu
+    assert ok.all()
+    ge, status = emu.adjoint_packed(packed, V, np.ascontiguousarray(states), ok, tan)
+    assert not status.any()
+    rng = np.random.default_rng(0)
+    w, d = rng.normal(size=ok.shape), rng.normal(size=th.shape)
+    scale = th.copy()
+    scale[:, 15] = 0.01
+    a = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
+    b = np.sum(w[:, :, None] * ge * (scale * d)[:, None, :])
+    assert abs(a - b) < 1e-8 * abs(b)
+
+
+def test_adjoint_is_the_finite_difference_of_the_gpu_curves(c4_gpu):
+    """The reference's gradient test (tests/test_solver.py:1784-1811) on the GPU: a weighted sum
+    of the curves, a random direction in theta, the adjoint's directional derivative against
+    central differences of the GPU's own FP64 curves."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import prepare, solve_batch
+
+    th, packed, tan, V, J, ok, grad, _ = c4_gpu
+    rng = np.random.default_rng(1)
+    w, d = rng.normal(size=ok.shape), rng.normal(size=th.shape)
+    scale = th.copy()
+    scale[:, 15] = 0.01
+    by_adjoint = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
+
+    def loss(a):
+        x = th * (1.0 + a * d)
+        x[:, 15] = th[:, 15] + a * d[:, 15] * 0.01
+        batch = prepare([A.device_from_theta(t) for t in x])
+        Jx, okx, _ = solve_batch(batch, V, tol=1e-12, strict=True, max_iter=200)
+        assert _np(okx).all()
+        return float(np.sum(w * _np(Jx)))
+
+    h = 1e-5
+    fd = (loss(h) - loss(-h)) / (2 * h)
+    assert abs(by_adjoint - fd) < 1e-5 * abs(fd)
+
+
+def test_autograd_function_backward_is_the_adjoint(c4_gpu):
+    import torch
+
+    from jaxdd_amd import adjoint as A
+
+    th, packed, tan, V, J, ok, grad, _ = c4_gpu
+    theta = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    Jt, okt = A.curves(theta, V, tol=1e-12)
+    w = torch.as_tensor(np.random.default_rng(2).normal(size=Jt.shape), dtype=torch.float64)
+    (w * Jt).sum().backward()
+    # the same curves' adjoint at the default tolerance (its own linearisation points)
+    _, _, _, g = A.jv_gradients(packed, tan, V, tol=1e-12)
+    expect = np.einsum("nv,nvk->nk", w.numpy(), _np(g))
+    got = theta.grad.numpy()
+    assert np.allclose(got, expect, rtol=1e-9, atol=0.0)
