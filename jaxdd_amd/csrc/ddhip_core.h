@@ -576,8 +576,9 @@ DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, 
     }
 }
 
-// Elimination sweep (right to left). Returns the ion step z.
-template <typename T, int K>
+// Elimination sweep (right to left). Returns the ion step z. NOISE: the right-hand side is
+// the residual's round-off instead of -F (src/jaxdd/solver.py:1728-1751, `_noise_step`).
+template <typename T, int K, bool NOISE = false>
 DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double *z) {
     // everything the loop reads of the arguments, in registers: the loop's stores to global
     // memory would otherwise keep these generic-pointer loads inside it
@@ -604,6 +605,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     Pt mid, cur;
     double rV = 0.0;  // the row's right neighbour: its potential and segment
     int rsid = 0;
+    double rlnN = 0.0, rlnP = 0.0;  // and its log densities (the noise's right-hand side)
     Raw nxt;
     fetch_point(Lv, s1, s2, NP, nxt);
     make_point(Lv, NP + 1, true, right, nxt, Lv.sid[(size_t)(NP + 1) * n], c, Vt, iVt, mid);  // nxt unread
@@ -633,7 +635,26 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
             const int i = j + 1;
             double A[9], B[9], C[9], r[3 * K];
-            row_blocks<K>(Lv, cur, mid, rV, eL, eR, i == NP, L2, iVt, c.share, c.g_frac, A, B, C, r);
+            // (the noise's right-hand side takes the last row's coupling to the contact too)
+            row_blocks<K>(Lv, cur, mid, rV, eL, eR, !NOISE && i == NP, L2, iVt, c.share, c.g_frac,
+                          A, B, C, r);
+            if constexpr (NOISE) {
+                // eps * sum |dF/dy| |y| over the row's three points, y = (V, ln n, ln p)
+                const double yl[3] = {cur.V, log(cur.n), log(cur.p)};
+                const double ym[3] = {mid.V, log(mid.n), log(mid.p)};
+                const double yr[3] = {rV, rlnN, rlnP};
+#pragma unroll
+                for (int row = 0; row < 3; ++row) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        acc += fabs(A[3 * row + q]) * fabs(yl[q]);
+                        acc += fabs(B[3 * row + q]) * fabs(ym[q]);
+                        acc += fabs(C[3 * row + q]) * fabs(yr[q]);
+                    }
+                    r[row * K] = DBL_EPS * acc;
+                }
+            }
             if constexpr (sizeof(T) == 4) {
                 // row equilibration by powers of two (exact) before the rounding to FP32
 #pragma unroll
@@ -720,6 +741,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             for (int q = 0; q < 3 * K; ++q) dp[q] = dn_[q];
         }
         rV = mid.V; rsid = mid.sid;
+        if constexpr (NOISE) { rlnN = log(mid.n); rlnP = log(mid.p); }
         mid = cur;
         eR = eL;
     };
@@ -750,7 +772,8 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
 }
 
 // Back-substitution sweep (left to right): the step, the new state, its size and current.
-template <typename T, int K>
+// NOISE: only the step's size, measured as a Newton step's (the state is left as it is).
+template <typename T, int K, bool NOISE = false>
 DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, const double *z,
                              double *out /* size, J, finite */) {
     const SweepCtl c = cref;  // in registers (as in the elimination)
@@ -807,6 +830,13 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
         const double x1 = dF[1] - ((double)f[3] * xs0 + (double)f[4] * xs1 + (double)f[5] * xs2);
         const double x2 = dF[2] - ((double)f[6] * xs0 + (double)f[7] * xs1 + (double)f[8] * xs2);
         xs0 = x0; xs1 = x1; xs2 = x2;
+        if constexpr (NOISE) {
+            const double dn = exp(now.y1), dp = exp(now.y2);
+            size = nmax(size, fabs(x0) / Vt);
+            size = nmax(size, fabs(x1) * (dn / (dn + nfl)));
+            size = nmax(size, fabs(x2) * (dp / (dp + nfl)));
+            return;
+        }
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
         double y0 = now.y0, y1 = now.y1, y2 = now.y2;
         const double st0 = clipd(x0, -capv, capv);
@@ -864,7 +894,7 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
     // same U with opposite signs), so the device average is q (K_n + K_p), sign turned with
     // the anode on the left.
     out[0] = size;
-    out[1] = -SC(SC_ELEC) * Q_E * (Kn + Kp);
+    out[1] = NOISE ? 0.0 : -SC(SC_ELEC) * Q_E * (Kn + Kp);
     out[2] = finite;
 }
 
@@ -1077,7 +1107,7 @@ DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
 }
 
 // After a Newton step: the histories and the stop rules (src/jaxdd/batch.py:793-839; with
-// jv, src/jaxdd/solver.py:2150-2166 without the noise floor's extra solve).
+// jv, src/jaxdd/solver.py:2150-2166).
 DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, double J) {
     for (int k = 0; k < 3; ++k) { S.hist[k] = S.hist[k + 1]; S.Jh[k] = S.Jh[k + 1]; }
     S.hist[3] = size;
@@ -1103,6 +1133,17 @@ DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, doubl
         bool quadratic = S.it > 1 && size < QUADRATIC_TOL && size < QUADRATIC_RATIO * S.hist[2];
         done = size < a.tol || stalled || settled || quadratic;
     }
+    if (a.jv && !done && S.it > STALL_STEPS && !anynan && hmax < NOISE_FLOOR_CAP) {
+        // stalled above the round-off floor: stop below the problem's own noise floor
+        // (src/jaxdd/solver.py:2162-2166), computed once per solve, on the lane's next turn
+        if (S.noise_state == 2) {
+            done = hmax < S.noise_floor;
+        } else {
+            S.noise_state = 1;
+            S.fresh = 0;
+            return;
+        }
+    }
     if (done) { end_solve(a, lane, S, 1, J); return; }
     if (S.it >= S.maxit || size != size) { end_solve(a, lane, S, 0, J); return; }
     S.fresh = 0;
@@ -1122,6 +1163,19 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg)
     c.fresh = S.fresh; c.freeze_now = S.freeze_now && S.fresh; c.mobile = S.mobile_now;
     c.src1 = S.b_src1; c.src2 = S.b_src2; c.work = S.b_work;
     double z[2], out[3];
+    if (S.noise_state == 1) {
+        // the noise floor of the state the last step left (src/jaxdd/solver.py:1728-1751):
+        // one FP64 solve without the ions' border, then the stop the last step deferred
+        elim_sweep<double, 1, true>(a, lane, c, z);
+        backsub_sweep<double, 1, true>(a, lane, c, z, out);
+        S.noise_floor = out[0];
+        S.noise_state = 2;
+        double hmax = 0.0;
+        for (int k = 0; k < 4; ++k) hmax = fmax(hmax, S.hist[k]);
+        if (hmax < S.noise_floor) end_solve(a, lane, S, 1, S.Jh[3]);
+        else if (S.it >= S.maxit) end_solve(a, lane, S, 0, S.Jh[3]);
+        return;
+    }
     if (ions) elim_sweep<T, 3>(a, lane, c, z);
     else elim_sweep<T, 1>(a, lane, c, z);
     c.fresh = 0;

@@ -32,6 +32,7 @@ constexpr double CURRENT_STEP_CAP = 1e-3;
 constexpr double QUADRATIC_TOL = 1e-5;
 constexpr double QUADRATIC_RATIO = 1e-2;
 constexpr double POISSON_STOP = 1e-8;
+constexpr double DBL_EPS = 2.220446049250313e-16;  // 2^-52, the FP64 machine epsilon
 
 // jnp.clip and jnp.maximum keep NaN; fmin/fmax would drop it.
 DDH_MFN double clipd(double x, double lo, double hi) {

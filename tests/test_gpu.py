@@ -99,6 +99,22 @@ def test_solve_jv_matches_oracle(torch_cuda, oracle):
         assert np.max(np.abs(res.J - Jo)) < 1e-12 * curve_scale(dev, Jo)
 
 
+def test_solve_jv_noise_floor_stop(torch_cuda, oracle):
+    """At tol 1e-14 every solve_jv solve ends at the round-off floor or below its noise floor
+    (src/jaxdd/solver.py:1728-1751, 2162-2166): the GPU's flags are the oracle's and its
+    curve within 1e-10 of the scale."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.solver import solve_jv
+
+    V = S.VOLTAGES[::3].copy()
+    for dev in [fixture_device("thin_l1_light")] + S.devices("c2", 2):
+        Jo, oko, itso = oracle.solve_jv(dev, V, tol=1e-14, max_iter=200)
+        res = solve_jv(dev, V, tol=1e-14)
+        assert np.array_equal(res.converged, oko)
+        assert np.max(np.abs(res.J - Jo)) < 1e-10 * curve_scale(dev, Jo)
+        assert np.max(np.abs(res.iterations - itso)) <= 2
+
+
 def test_solve_jv_raises_on_a_failed_pre_bias(torch_cuda):
     from jaxdd_amd.solver import solve_jv
 

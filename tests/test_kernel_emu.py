@@ -90,3 +90,23 @@ def test_replay_nan_input_ends_unconverged(emu):
     J, ok, its, _ = emu.solve_packed(pack([bad, good]), V_FULL[::6].copy())
     assert not ok[0].any() and ok[1].all()
     assert its[0].max() <= 1
+
+
+def test_replay_noise_floor_stop_is_the_oracles(emu, oracle):
+    """solve_jv's noise-floor stop (src/jaxdd/solver.py:1728-1751, 2162-2166): at tol 1e-14
+    every solve ends at the round-off floor or below its own noise floor, and the replay takes
+    the oracle's steps exactly (without the noise floor it takes more on every device)."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    V = S.VOLTAGES[::3].copy()
+    devs = [fixture_device(c) for c in ("thin_l1_light", "steady_1e17_thin_light")]
+    devs += S.devices("c2", 3)
+    for dev in devs:
+        Jo, oko, itso = oracle.solve_jv(dev, V, tol=1e-14, max_iter=200)
+        J, ok, its, status = emu.solve_packed(pack([dev]), V, tol=1e-14, max_iter=200,
+                                              strict=True, jv=True)
+        assert not status.any()
+        assert np.array_equal(ok[0], oko) and np.array_equal(its[0], itso)
+        shunt = 1.0 / dev.R_shunt if dev.R_shunt > 0 else 0.0
+        assert np.max(np.abs(J[0] + shunt * V - Jo)) < 1e-10 * curve_scale(dev, Jo)
