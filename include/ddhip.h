@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define DDH_ABI_VERSION 1
+#define DDH_ABI_VERSION 2
 
 /* Segments (runs of points of one material) per device: the solve keeps each wavefront's
  * segment tables in the compute unit's 160 KB local data share. */
@@ -62,6 +62,9 @@ typedef struct {
     int32_t precision;   /* ddh_precision */
     int32_t jv_mode;     /* 0: solve_batch semantics; 1: solve_jv semantics (given order,
                             no extrapolation, tol/stall/noise-floor stops, pre-bias must hold) */
+    int32_t lanes;       /* lanes per device in batch mode: 2 (0, the default) splits each
+                            Newton solve at the grid's middle, 1 sweeps it from one end (the
+                            reference's order of elimination); jv mode always uses 1 */
 } ddh_opts;
 
 int ddh_abi_version(void);

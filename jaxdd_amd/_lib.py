@@ -23,7 +23,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
                 ("start_iter", ctypes.c_int32), ("strict", ctypes.c_int32),
                 ("bernoulli", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("jv_mode", ctypes.c_int32)]
+                ("jv_mode", ctypes.c_int32), ("lanes", ctypes.c_int32)]
 
 
 #: The symbols include/ddhip.h declares.
@@ -61,8 +61,8 @@ def load():
                                 vp, vp, ctypes.c_size_t, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
-    if lib.ddh_abi_version() != 1:
-        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 1")
+    if lib.ddh_abi_version() != 2:
+        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 2")
     _lib = lib
     return lib
 

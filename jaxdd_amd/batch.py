@@ -197,7 +197,7 @@ def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=Non
 
 def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
                 bernoulli_name="exact", strict=False, devices=None, precision="fp64",
-                events=None):
+                events=None, lanes=2):
     """J-V curves of a batch at ``voltages`` (reference: src/jaxdd/batch.py:1136-1162):
     ``(J, converged, iterations)``, each batch x voltages, in the order of ``voltages``, as
     torch tensors on the batch's GPU (float64, bool, int64).
@@ -209,7 +209,9 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     ``precision``: "fp64", or "mixed" (the Newton correction's block factorisation in FP32 on
     row-equilibrated blocks; residual, exponentials, current and convergence tests in FP64).
     ``devices`` is accepted for the reference's signature: one GPU solves the whole batch.
-    ``events``: optional (start, end) torch.cuda.Event pair timing the solve kernel."""
+    ``events``: optional (start, end) torch.cuda.Event pair timing the solve kernel.
+    ``lanes``: 2 (two GPU lanes per device, each Newton solve split at the grid's middle) or
+    1 (one lane sweeping the whole grid, the reference's order of elimination)."""
     del first_max_iter, devices
     if precision not in _lib.PRECISIONS:
         raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
@@ -217,8 +219,11 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
     V = np.asarray(voltages, dtype=float).ravel()
     order = np.argsort(V, kind="stable")
+    if lanes not in (1, 2):
+        raise ValueError(f"lanes {lanes!r}: 1 or 2")
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
-                     0 if bernoulli_name == "exact" else 1, _lib.PRECISIONS[precision], 0)
+                     0 if bernoulli_name == "exact" else 1, _lib.PRECISIONS[precision], 0,
+                     int(lanes))
     J, ok, its, _ = _run(batch, V[order], opts, _lib.PRECISIONS[precision], events)
     inv = torch.as_tensor(np.argsort(order, kind="stable"), device=J.device)
     J, ok, its = J[:, inv], ok[:, inv].bool(), its[:, inv].long()

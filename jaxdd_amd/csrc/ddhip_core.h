@@ -144,8 +144,7 @@ DDH_FN void refit_logs(const double num0[2], const double tot[2], double out[2])
     for (int k = 0; k < 2; ++k) out[k] = tot[k] > 0 ? log(tot[k] / num0[k]) : 0.0;
 }
 
-DDH_FN void poisson_lane(const Args &a, int lane) {
-    LaneState &S = a.st[lane];
+DDH_FN void poisson_lane(const Args &a, int lane, LaneState &S) {
     const int NP = a.NP;
     const double Vt = SC(SC_VT), L = SC(SC_L), L2 = L * L;
     const double tot[2] = {SC(SC_AT), SC(SC_CT)};
@@ -1152,9 +1151,8 @@ DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, doubl
 
 // One Newton step of one lane (both sweeps and the state machine). `ions`: some lane of
 // the wavefront has mobile ions (the sweeps then carry the border's right-hand sides).
-template <typename T>
-DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg) {
-    LaneState &S = a.st[lane];
+// The sweeps' view of the lane's state.
+DDH_FN SweepCtl sweep_ctl(const Args &a, int lane, const LaneState &S, lcdouble *lseg) {
     SweepCtl c;
     c.lseg = lseg;
     c.Vint = S.Vint; c.share = S.share; c.ratio = S.ratio; c.g_frac = SC(SC_GFRAC);
@@ -1162,6 +1160,23 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg)
     c.lstart[0] = S.lstart[0]; c.lstart[1] = S.lstart[1];
     c.fresh = S.fresh; c.freeze_now = S.freeze_now && S.fresh; c.mobile = S.mobile_now;
     c.src1 = S.b_src1; c.src2 = S.b_src2; c.work = S.b_work;
+    return c;
+}
+
+// After the sweeps: the ion step's cap, the step size with it, the stop rules.
+DDH_FN void finish_step(const Args &a, int lane, LaneState &S, const double z[2], double size,
+                        double J, int finite) {
+    const double dl0 = clipd(z[0], -ION_LOG_CAP, ION_LOG_CAP);
+    const double dl1 = clipd(z[1], -ION_LOG_CAP, ION_LOG_CAP);
+    S.lw[0] += dl0;
+    S.lw[1] += dl1;
+    S.fin_work = finite;
+    after_step(a, lane, S, nmax(size, nmax(fabs(dl0), fabs(dl1))), J);
+}
+
+template <typename T>
+DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg, LaneState &S) {
+    SweepCtl c = sweep_ctl(a, lane, S, lseg);
     double z[2], out[3];
     if (S.noise_state == 1) {
         // the noise floor of the state the last step left (src/jaxdd/solver.py:1728-1751):
@@ -1181,17 +1196,11 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg)
     c.fresh = 0;
     if (ions) backsub_sweep<T, 3>(a, lane, c, z, out);
     else backsub_sweep<T, 1>(a, lane, c, z, out);
-    double dl0 = clipd(z[0], -ION_LOG_CAP, ION_LOG_CAP);
-    double dl1 = clipd(z[1], -ION_LOG_CAP, ION_LOG_CAP);
-    S.lw[0] += dl0;
-    S.lw[1] += dl1;
-    const double size = nmax(out[0], nmax(fabs(dl0), fabs(dl1)));
-    S.fin_work = out[2] != 0.0;
-    after_step(a, lane, S, size, out[1]);
+    finish_step(a, lane, S, z, out[0], out[1], out[2] != 0.0);
 }
 
-DDH_FN bool lane_active(const Args &a, int lane) { return a.st[lane].phase != PH_DONE; }
-DDH_FN bool lane_mobile(const Args &a, int lane) { return a.st[lane].mobile_now != 0; }
-DDH_FN bool lane_needs64(const Args &a, int lane) { return a.st[lane].need64 != 0; }
+DDH_FN bool lane_active(const LaneState &S) { return S.phase != PH_DONE; }
+DDH_FN bool lane_mobile(const LaneState &S) { return S.mobile_now != 0; }
+DDH_FN bool lane_needs64(const LaneState &S) { return S.need64 != 0; }
 
 }  // namespace ddh

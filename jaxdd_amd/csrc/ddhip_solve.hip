@@ -9,6 +9,7 @@
 
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
+#include "ddhip_pair.h"
 
 namespace ddh {
 
@@ -19,19 +20,66 @@ __global__ void __launch_bounds__(64) solve_kernel(Args a) {
     const int lane = blockIdx.x * 64 + threadIdx.x;
     const bool valid = lane < a.n;
     ldouble *lseg = (ldouble *)lds + threadIdx.x;
+    LaneState &S = a.st[valid ? lane : 0];
     if (valid) {
         fill_segments(a, lane, lseg);
-        poisson_lane(a, lane);
-        start_lane(a, lane, a.st[lane]);
+        poisson_lane(a, lane, S);
+        start_lane(a, lane, S);
     }
     while (true) {
-        const bool act = valid && lane_active(a, lane);
+        const bool act = valid && lane_active(S);
         if (!__any(act)) break;
-        const bool ions = __any(act && lane_mobile(a, lane));
-        const bool f64 = !a.mixed || __any(act && lane_needs64(a, lane));
+        const bool ions = __any(act && lane_mobile(S));
+        const bool f64 = !a.mixed || __any(act && lane_needs64(S));
         if (act) {
-            if (f64) newton_lane_step<double>(a, lane, ions, lseg);
-            else newton_lane_step<float>(a, lane, ions, lseg);
+            if (f64) newton_lane_step<double>(a, lane, ions, lseg, S);
+            else newton_lane_step<float>(a, lane, ions, lseg, S);
+        }
+    }
+}
+
+// A register copy of the partner lane's record (lanes 2d and 2d + 1 hold device d).
+template <typename R>
+__device__ inline R partner(const R &r) {
+    static_assert(sizeof(R) % 4 == 0, "records of 32-bit words");
+    R o;
+    const int *src = reinterpret_cast<const int *>(&r);
+    int *dst = reinterpret_cast<int *>(&o);
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(R) / 4); ++q) dst[q] = __shfl_xor(src[q], 1);
+    return o;
+}
+
+// Two lanes per device (ddhip_pair.h): 32 devices per wavefront. The left lane of each pair
+// runs the Poisson start; both lanes then run the same state machine on their own state.
+__global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
+    extern __shared__ double lds[];  // a.nseg * SEG_N * WAVE doubles (32 device slots used)
+    const int t = threadIdx.x, half = t & 1;
+    const int lane = blockIdx.x * 32 + (t >> 1);  // the device
+    const bool valid = lane < a.n;
+    ldouble *lseg = (ldouble *)lds + (t >> 1);
+    LaneState &S = a.st[valid ? 2 * (size_t)lane + half : 0];
+    if (valid && half == 0) {
+        fill_segments(a, lane, lseg);
+        poisson_lane(a, lane, S);
+    }
+    __syncthreads();  // the segment table and the Poisson start, for the right lanes
+    if (valid && half == 1) S = a.st[2 * (size_t)lane];
+    if (valid) start_lane(a, lane, S);
+    while (true) {
+        const bool act = valid && lane_active(S);
+        if (!__any(act)) break;
+        const bool ions = __any(act && lane_mobile(S));
+        const bool f64 = !a.mixed || __any(act && lane_needs64(S));
+        if (act) {
+            PairStep st;
+            if (f64) pair_phase1<double>(a, lane, half, ions, lseg, S, st);
+            else pair_phase1<float>(a, lane, half, ions, lseg, S, st);
+            const PairElim oe = partner(st.e);
+            if (f64) pair_phase2<double>(a, lane, half, ions, st, oe);
+            else pair_phase2<float>(a, lane, half, ions, st, oe);
+            const PairBack ob = partner(st.b);
+            pair_phase3(a, lane, half, S, st, ob);
         }
     }
 }
@@ -193,7 +241,7 @@ struct WsLayout {
 
 // One region per array: the state buffers, the FP64 and FP32 Newton factors, Poisson's
 // factors, the fixed ions in use and the lanes' state machines.
-static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision) {
+static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision, int32_t lanes = 2) {
     WsLayout w;
     size_t off = 0;
     w.y = off; off = align256(off + (size_t)ddh::NBUF * NP * 3 * n * sizeof(double));
@@ -202,7 +250,7 @@ static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision) {
     if (precision == DDH_MIXED) off = align256(off + (size_t)NP * ddh::NFAC * n * sizeof(float));
     w.pfac = off; off = align256(off + (size_t)NP * 4 * n * sizeof(double));
     w.fix = off; off = align256(off + (size_t)NP * n * sizeof(double));
-    w.st = off; off = align256(off + (size_t)n * sizeof(ddh::LaneState));
+    w.st = off; off = align256(off + (size_t)lanes * n * sizeof(ddh::LaneState));
     w.total = off;
     return w;
 }
@@ -210,7 +258,7 @@ static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision) {
 extern "C" size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision) {
     (void)nV;
     if (n <= 0 || NP < 2) return 0;
-    return ws_layout(n, NP, precision).total;
+    return ws_layout(n, NP, precision, 2).total;  // enough for either kernel
 }
 
 static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *o,
@@ -226,7 +274,10 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
         return fail(DDH_E_ARG, "ddh_solve_batch: bernoulli must be 0 (exact) or 1 (simss)");
     if (o->max_iter < 1 || o->start_iter < 1)
         return fail(DDH_E_ARG, "ddh_solve_batch: max_iter and start_iter must be >= 1");
-    WsLayout w = ws_layout(b->n, b->NP, o->precision);
+    if (o->lanes != 0 && o->lanes != 1 && o->lanes != 2)
+        return fail(DDH_E_ARG, "ddh_solve_batch: lanes must be 1 or 2 (0: the default, 2)");
+    const int lanes = o->jv_mode ? 1 : (o->lanes == 0 ? 2 : o->lanes);
+    WsLayout w = ws_layout(b->n, b->NP, o->precision, lanes);
     if (workspace_bytes < w.total) return fail(DDH_E_WORKSPACE, "ddh_solve_batch: workspace too small");
     char *base = (char *)workspace;
     ddh::Args a;
@@ -244,11 +295,18 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
     a.strict = o->strict; a.bern = o->bernoulli; a.jv = o->jv_mode;
     a.mixed = o->precision == DDH_MIXED ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid((b->n + 63) / 64), block(64);
     const size_t lds = (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
-    (void)hipFuncSetAttribute((const void *)ddh::solve_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(ddh::solve_kernel, grid, block, lds, s, a);
+    if (lanes == 2) {
+        dim3 grid((b->n + 31) / 32), block(64);
+        (void)hipFuncSetAttribute((const void *)ddh::solve_pair_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(ddh::solve_pair_kernel, grid, block, lds, s, a);
+    } else {
+        dim3 grid((b->n + 63) / 64), block(64);
+        (void)hipFuncSetAttribute((const void *)ddh::solve_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(ddh::solve_kernel, grid, block, lds, s, a);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "solve_kernel launch");
     return DDH_OK;

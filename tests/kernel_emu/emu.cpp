@@ -7,6 +7,7 @@
 #define DDH_CPU_EMULATION 1
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
+#include "ddhip_pair.h"
 
 #include <cstdlib>
 #include <vector>
@@ -29,7 +30,7 @@ static void run(Args &a) {
             if (!started[w]) {
                 for (int lane = w0; lane < w1; ++lane) {
                     fill_segments(a, lane, &lds_all[(size_t)w * a.nseg * SEG_N * WAVE] + (lane - w0));
-                    poisson_lane(a, lane);
+                    poisson_lane(a, lane, a.st[lane]);
                     start_lane(a, lane, a.st[lane]);
                 }
                 started[w] = 1;
@@ -38,20 +39,77 @@ static void run(Args &a) {
             double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
             bool any = false, ions = false, f64 = !a.mixed;
             for (int lane = w0; lane < w1; ++lane) {
-                bool act = lane_active(a, lane);
+                bool act = lane_active(a.st[lane]);
                 any |= act;
-                ions |= act && lane_mobile(a, lane);
-                f64 |= act && lane_needs64(a, lane);
+                ions |= act && lane_mobile(a.st[lane]);
+                f64 |= act && lane_needs64(a.st[lane]);
             }
             if (!any) { done[w] = 1; --remaining; continue; }
             for (int lane = w0; lane < w1; ++lane) {
-                if (!lane_active(a, lane)) continue;
-                if (f64) newton_lane_step<double>(a, lane, ions, lds + (lane - w0));
-                else newton_lane_step<float>(a, lane, ions, lds + (lane - w0));
+                if (!lane_active(a.st[lane])) continue;
+                if (f64) newton_lane_step<double>(a, lane, ions, lds + (lane - w0), a.st[lane]);
+                else newton_lane_step<float>(a, lane, ions, lds + (lane - w0), a.st[lane]);
             }
         }
     }
 }
+
+// Two lanes per device (ddhip_pair.h): wavefronts of 32 devices; each pair's three phases
+// run lane by lane, the records exchanged between the phases as the GPU's shuffles do.
+static void run_pair(Args &a) {
+    const int nw = (a.n + 31) / 32;
+    std::vector<char> started(nw, 0), done(nw, 0);
+    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * WAVE);
+    int remaining = nw;
+    while (remaining > 0) {
+        for (int w = 0; w < nw; ++w) {
+            if (done[w]) continue;
+            const int d0 = w * 32, d1 = d0 + 32 < a.n ? d0 + 32 : a.n;
+            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
+            if (!started[w]) {
+                for (int d = d0; d < d1; ++d) {
+                    fill_segments(a, d, lds + (d - d0));
+                    poisson_lane(a, d, a.st[2 * d]);
+                    a.st[2 * d + 1] = a.st[2 * d];
+                    start_lane(a, d, a.st[2 * d]);
+                    start_lane(a, d, a.st[2 * d + 1]);
+                }
+                started[w] = 1;
+                continue;
+            }
+            bool any = false, ions = false, f64 = !a.mixed;
+            for (int q = 2 * d0; q < 2 * d1; ++q) {
+                bool act = lane_active(a.st[q]);
+                any |= act;
+                ions |= act && lane_mobile(a.st[q]);
+                f64 |= act && lane_needs64(a.st[q]);
+            }
+            if (!any) { done[w] = 1; --remaining; continue; }
+            for (int d = d0; d < d1; ++d) {
+                LaneState &S0 = a.st[2 * d], &S1 = a.st[2 * d + 1];
+                if (!lane_active(S0)) continue;
+                PairStep p0, p1;
+                const double *l = lds + (d - d0);
+                if (f64) {
+                    pair_phase1<double>(a, d, 0, ions, l, S0, p0);
+                    pair_phase1<double>(a, d, 1, ions, l, S1, p1);
+                    pair_phase2<double>(a, d, 0, ions, p0, p1.e);
+                    pair_phase2<double>(a, d, 1, ions, p1, p0.e);
+                } else {
+                    pair_phase1<float>(a, d, 0, ions, l, S0, p0);
+                    pair_phase1<float>(a, d, 1, ions, l, S1, p1);
+                    pair_phase2<float>(a, d, 0, ions, p0, p1.e);
+                    pair_phase2<float>(a, d, 1, ions, p1, p0.e);
+                }
+                pair_phase3(a, d, 0, S0, p0, p1.b);
+                pair_phase3(a, d, 1, S1, p1, p0.b);
+            }
+        }
+    }
+}
+
+static int g_lanes = 1;  // the next solve's lanes per device (jv mode: 1)
+extern "C" void emu_set_lanes(int lanes) { g_lanes = lanes; }
 
 static int g_poison = 0;  // fill the workspace with this byte before a solve (0: zeros)
 extern "C" void emu_set_poison(int byte) { g_poison = byte; }
@@ -68,7 +126,7 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     std::vector<double> fac64((size_t)NP * NFAC * n), pfac((size_t)NP * 4 * n);
     std::vector<float> fac32((size_t)NP * NFAC * n);
     std::vector<double> fix((size_t)NP * n);
-    std::vector<LaneState> st(n);
+    std::vector<LaneState> st(2 * (size_t)n);
     if (g_poison) {
         memset(Y.data(), g_poison, Y.size() * sizeof(double));
         memset(fac64.data(), g_poison, fac64.size() * sizeof(double));
@@ -85,7 +143,8 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = tol; a.max_iter = max_iter; a.start_iter = start_iter; a.strict = strict;
     a.bern = bern; a.jv = jv; a.mixed = precision ? 1 : 0;
-    run(a);
+    if (g_lanes == 2 && !jv) run_pair(a);
+    else run(a);
     return 0;
 }
 

@@ -31,6 +31,7 @@ def lib():
         _lib.emu_solve_batch.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, ctypes.c_double, i, i,
                                          i, i, i, i, vp, vp, vp, vp]
         _lib.emu_keep_states.argtypes = [vp]
+        _lib.emu_set_lanes.argtypes = [i]
         _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, i, vp, vp, vp, i,
                                      vp, vp]
     return _lib
@@ -41,7 +42,7 @@ def _p(a):
 
 
 def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=False,
-                 bernoulli="exact", precision="fp64", jv=False, states=None):
+                 bernoulli="exact", precision="fp64", jv=False, states=None, lanes=1):
     """(J without the shunt, ok, its, status) for packed devices at voltages V (ascending in
     batch mode, the given order in jv mode). ``states``: an array (nV, NP, 3, n) that receives
     each voltage's final state."""
@@ -58,6 +59,7 @@ def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=Fals
         assert states.shape == (nV, NP, 3, n) and states.dtype == np.float64
         assert states.flags.c_contiguous
     lib().emu_keep_states(None if states is None else _p(states))
+    lib().emu_set_lanes(int(lanes))
     lib().emu_solve_batch(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV, tol,
                           max_iter, start_iter or max_iter, int(strict),
                           0 if bernoulli == "exact" else 1, 0 if precision == "fp64" else 1,

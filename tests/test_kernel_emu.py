@@ -23,10 +23,14 @@ def fixture_runs(oracle):
     return cases, devs, pack(devs), (ref[0] - shunt[:, None] * V_FULL, ref[1], ref[2])
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("precision,jtol", [("fp64", 1e-12), ("mixed", 1e-9)])
-def test_replay_matches_oracle_on_fixtures(emu, fixture_runs, precision, jtol):
+def test_replay_matches_oracle_on_fixtures(emu, fixture_runs, precision, jtol, lanes):
+    """One lane per device (the reference's elimination order) and two (the solve split at the
+    grid's middle): the oracle's flags, in FP64 its iterations, and its curves."""
     cases, devs, packed, (Jo, oko, itso) = fixture_runs
-    J, ok, its, status = emu.solve_packed(packed, V_FULL, bernoulli="simss", precision=precision)
+    J, ok, its, status = emu.solve_packed(packed, V_FULL, bernoulli="simss", precision=precision,
+                                          lanes=lanes)
     assert not status.any()
     assert np.array_equal(ok, oko)
     if precision == "fp64":
@@ -35,15 +39,16 @@ def test_replay_matches_oracle_on_fixtures(emu, fixture_runs, precision, jtol):
         assert np.max(np.abs(J[k] - Jo[k])) < jtol * curve_scale(dev, Jo[k]), cases[k]
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("config,variant", [("c2", "free"), ("c3", "free"), ("c3", "frozen")])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_replay_matches_oracle_on_synthetic(emu, oracle, config, variant, precision):
+def test_replay_matches_oracle_on_synthetic(emu, oracle, config, variant, precision, lanes):
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import pack
 
     devs = S.devices(config, 12, variant=variant)
     Jo, oko, itso = oracle.solve_batch(devs, S.VOLTAGES, first_max_iter=None)
-    J, ok, its, _ = emu.solve_packed(pack(devs), S.VOLTAGES, precision=precision)
+    J, ok, its, _ = emu.solve_packed(pack(devs), S.VOLTAGES, precision=precision, lanes=lanes)
     agree, chaotic = flags_agree(ok, its, oko, itso)
     assert agree and chaotic <= 0.02 * ok.size
     both = ok & oko
