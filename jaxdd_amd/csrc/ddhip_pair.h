@@ -82,7 +82,11 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
     fetch_point(Lv, s1, s2, j0 + dir, nxt);
     make_point(Lv, j0, true, ct, nxt, Lv.sid[(size_t)j0 * n], c, Vt, iVt, mid);
     far = mid;
-    for (int v = 1; v < nvis; ++v) {
+    // visit v: the next point, the edge to it, the row of the point before (from v = 2). The
+    // first interior visit has no row and runs before the loop, followed by stores to the
+    // slot the first row then overwrites: every loop visit, and the loop's entry, then have a
+    // point's loads followed by a row's stores, and no wait for those loads covers the stores.
+    auto visit = [&](const int v, const bool has_row) {
         const int j = j0 + dir * v;
         now = nxt;
         fetch_point(Lv, s1, s2, v + 1 < nvis ? j + dir : j, nxt);
@@ -98,7 +102,7 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
                 num1 += 0.5 * (mid.ca + pn.ca) * (mcm(mid.sid) * mcm(pn.sid)) * hk;
             }
         }
-        if (v >= 2) {
+        if (has_row) {
             // the row of mid (point i), its neighbours far and the new point
             const int i = j - dir;
             const Pt lft = dir > 0 ? far : pn, rgt = dir > 0 ? pn : far;
@@ -200,7 +204,14 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
         far = mid;
         mid = pn;
         eold = en;
+    };
+    visit(1, false);
+    {
+        DDH_GLOBAL T *f = fac + (size_t)(half ? NP - 1 : 0) * NFAC * n;
+#pragma unroll
+        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = T(0);
     }
+    for (int v = 2; v < nvis; ++v) visit(v, true);
 #pragma unroll
     for (int q = 0; q < 9; ++q) out.cf[q] = (double)cp[q];
 #pragma unroll
@@ -322,7 +333,9 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
     double pwn = 0.0, pwp = 0.0, pU = 0.0, px = 0.0;  // the previous point's
     Next nxt, now;
     fetch(jf, nxt);
-    for (int v = 0; v < cnt; ++v) {
+    // visit v; the first (the point next to the middle, its x given) runs before the loop, so
+    // that the loop is entered, as it repeats, after a visit's stores
+    auto visit = [&](const int v) {
         const int j = jf + dir * v;
         now = nxt;
         fetch(v + 1 < cnt ? j + dir : j, nxt);
@@ -392,7 +405,9 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
             iu += u;
         }
         pwn = wn; pwp = wp; pU = U; px = xj;
-    }
+    };
+    visit(0);
+    for (int v = 1; v < cnt; ++v) visit(v);
     if (half) {
         const int sidN = Lv.sid[(size_t)(NP + 1) * n];
         const double VN = contact_V(a, lane, c.Vint, true);
