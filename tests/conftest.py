@@ -78,3 +78,16 @@ def flags_agree(ok, its, ok_ref, its_ref):
     (agree, number of chaotic entries)."""
     calm = (its < CHAOTIC_STEPS) & (its_ref < CHAOTIC_STEPS)
     return bool(np.array_equal(ok[calm], ok_ref[calm])), int((~calm).sum())
+
+
+def chaotic_devices(oracle, devs, V, J_ref, rel=1e-13, tol=1e-7):
+    """Devices whose oracle curve moves by more than ``tol`` of its scale when the generation
+    rate changes by ``rel``: there a start or pre-bias solve lands on one of several nearby
+    solutions by the last bits of the arithmetic (frozen ions keep whichever profile the
+    pre-bias found), so no implementation can be held to the oracle's curve."""
+    import dataclasses
+
+    moved = [dataclasses.replace(d, G=d.G * (1.0 + rel)) for d in devs]
+    J2, _, _ = oracle.solve_batch(moved, V, first_max_iter=None)
+    return np.array([np.max(np.abs(J2[k] - J_ref[k])) > tol * curve_scale(d, J_ref[k])
+                     for k, d in enumerate(devs)])

@@ -7,7 +7,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, curve_scale, fixture_device, fixture_jv, flags_agree
+from conftest import (FIXTURES, chaotic_devices, curve_scale, fixture_device, fixture_jv,
+                      flags_agree)
 
 pytestmark = pytest.mark.gpu
 
@@ -165,10 +166,14 @@ def test_synthetic_matches_oracle(torch_cuda, oracle, config, variant, n, precis
     Jo, oko, itso = oracle.solve_batch(devs, S.VOLTAGES, first_max_iter=None)
     J, ok, its = (_np(t) for t in solve_batch(prepare(devs), S.VOLTAGES, precision=precision,
                                               lanes=lanes))
-    agree, chaotic = flags_agree(ok, its, oko, itso)
-    assert agree and chaotic <= 0.02 * ok.size
+    wild = chaotic_devices(oracle, devs, S.VOLTAGES, Jo)
+    assert wild.mean() <= 0.05
+    agree, chaotic = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree and chaotic <= 0.02 * ok[~wild].size
     both = ok & oko
     for k, dev in enumerate(devs):
+        if wild[k]:
+            continue
         err = np.max(np.abs(J[k] - Jo[k])[both[k]], initial=0.0)
         assert err < 1e-6 * curve_scale(dev, Jo[k])  # the workload's accuracy bar
 

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, curve_scale, fixture_device, flags_agree
+from conftest import FIXTURES, chaotic_devices, curve_scale, fixture_device, flags_agree
 
 V_FULL = np.round(np.arange(-0.2, 1.3 + 1e-9, 0.025), 6)
 
@@ -49,10 +49,14 @@ def test_replay_matches_oracle_on_synthetic(emu, oracle, config, variant, precis
     devs = S.devices(config, 12, variant=variant)
     Jo, oko, itso = oracle.solve_batch(devs, S.VOLTAGES, first_max_iter=None)
     J, ok, its, _ = emu.solve_packed(pack(devs), S.VOLTAGES, precision=precision, lanes=lanes)
-    agree, chaotic = flags_agree(ok, its, oko, itso)
-    assert agree and chaotic <= 0.02 * ok.size
+    wild = chaotic_devices(oracle, devs, S.VOLTAGES, Jo)
+    assert wild.mean() <= 0.1
+    agree, chaotic = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree and chaotic <= 0.02 * ok[~wild].size
     both = ok & oko
     for k, dev in enumerate(devs):
+        if wild[k]:
+            continue
         err = np.max(np.abs(J[k] - Jo[k])[both[k]], initial=0.0)
         assert err < 1e-9 * curve_scale(dev, Jo[k])
 
