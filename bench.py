@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--config", default="c3", choices=["c2", "c3"])
     p.add_argument("--variant", default="free", choices=["free", "frozen"])
     p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    p.add_argument("--lanes", type=int, default=2, choices=[1, 2],
+                   help="GPU lanes per device (2: each Newton solve split at the grid's middle)")
     p.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
@@ -233,13 +235,14 @@ def main() -> int:
     batch = from_packed(packed, "cuda")
     V = S.VOLTAGES
     for _ in range(args.warmup):
-        solve_batch(batch, V, precision=args.precision)
+        solve_batch(batch, V, precision=args.precision, lanes=args.lanes)
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
     t0 = time.perf_counter()
     for k in range(args.steps):
-        J, ok, its = solve_batch(batch, V, precision=args.precision, events=evs[k])
+        J, ok, its = solve_batch(batch, V, precision=args.precision, events=evs[k],
+                                 lanes=args.lanes)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -251,7 +254,8 @@ def main() -> int:
     step_bytes = b_step(400, 3 if ions else 1, w)
     launch_bytes = float(its.sum()) * step_bytes
     achieved = launch_bytes / (kernel_ms * 1e-3) / 1e9
-    workload = f"{args.config}-{args.variant}-n{args.n}-{args.precision}"
+    workload = f"{args.config}-{args.variant}-n{args.n}-{args.precision}-lanes{args.lanes}"
+    kernel = "ddh::solve_pair_kernel" if args.lanes == 2 else "ddh::solve_kernel"
 
     out = {
         "metric": METRIC,
@@ -272,11 +276,12 @@ def main() -> int:
                          f"3x3 blocks{' + 2-slot ion border' if ions else ''}, "
                          f"{args.precision} precision, seed {S.SEED}"),
             "devices": args.n, "NP": 400, "voltages": len(V), "precision": args.precision,
+            "lanes_per_device": args.lanes,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(workload),
-            "kernel": "ddh::solve_kernel", "kernel_ms": kernel_ms,
+            "kernel": kernel, "kernel_ms": kernel_ms,
             "algorithmic_bytes_per_launch": launch_bytes, "B_step": step_bytes,
         },
         "cpu_baseline": cpu,
@@ -288,7 +293,7 @@ def main() -> int:
         e64 = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         torch.cuda.synchronize()
         a = time.perf_counter()
-        J64, ok64, its64 = solve_batch(batch, V, precision="fp64", events=e64)
+        J64, ok64, its64 = solve_batch(batch, V, precision="fp64", events=e64, lanes=args.lanes)
         torch.cuda.synchronize()
         b = time.perf_counter()
         J64, ok64, its64 = J64.cpu().numpy(), ok64.cpu().numpy(), its64.cpu().numpy()
@@ -298,6 +303,7 @@ def main() -> int:
         out["fp64"] = {
             "value": args.n / (b - a), "ms_per_step": (b - a) * 1e3,
             "kernel_ms": e64[0].elapsed_time(e64[1]),
+            "newton_steps_per_voltage": float(its64.mean()),
             "speedup_mixed_over_fp64": (b - a) / (elapsed / args.steps),
         }
         out["max_err_vs_fp64"] = float(np.max((np.abs(J - J64) / scale)[both]))
