@@ -90,20 +90,20 @@ struct Args {
     int32_t *status;
     double tol;
     int max_iter, start_iter, strict, bern, jv;
-    int mixed;  // FP32 factors, a solve switching to FP64 after MIXED_STEPS steps
+    int mixed;  // FP32 factors for a scan solve's first MIXED_STEPS steps
 };
 
-// Mixed precision: a Newton step uses FP32 factors only while the solve is far from its
-// solution; once a step falls below MIXED_SWITCH (V_T, relative densities), or after
-// MIXED_STEPS steps, the solve's remaining steps use FP64 factors. The steps that decide
+// Mixed precision: the first Newton step of each voltage of the scan (the correction of the
+// extrapolated start) uses FP32 factors on row-equilibrated blocks; the solve's later steps,
+// which decide convergence (src/jaxdd/batch.py:793-818: the quadratic and settled stops assume
+// true Newton), and the start solves use FP64 factors. More FP32 steps (measured: up to 4, or
+// until a step falls below 1e-2) cost more Newton steps than they save where the
+// Jacobian is ill-conditioned (high forward bias), and on this GPU an FP32 step costs what an
+// FP64 one does: the rounded factors save HBM bytes, not issue cycles. The steps that decide
 // convergence (src/jaxdd/batch.py:793-818: the quadratic and settled stops assume true Newton)
 // are then FP64 Newton steps, and an FP32 correction that no longer contracts (an
 // ill-conditioned Jacobian at high forward bias) cannot stall a solve.
-#ifndef DDH_MIXED_SWITCH
-#define DDH_MIXED_SWITCH 1e-2
-#endif
-constexpr double MIXED_SWITCH = DDH_MIXED_SWITCH;
-constexpr int MIXED_STEPS = 4;
+constexpr int MIXED_STEPS = 1;
 
 #define SC(f) a.sc[(size_t)(f) * a.n + lane]
 #define PT(j, f) a.pt[((size_t)(j) * PT_N + (f)) * a.n + lane]
@@ -538,7 +538,9 @@ DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, 
     const double fu = 2 * meps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
     const double fl0 = 2 * meps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
     const double deps = meps - segv(Lv, cur.sid, SEG_EPS);  // the jump of eps between layers
-    const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
+    // deps / (h_{i-1}^2 L^2) = deps (D h_i h_s)^2 L^2, D = 1 / (h_i h_{i-1} h_s L^2): no division
+    const double Dhh = D * hp * hs;
+    const double jump = deps != 0.0 ? deps * (Dhh * Dhh) * L2 : 0.0;
     const double fac_m = jump - (fu + fl0);
     const double fac_l = fl0 - jump;
     const double sP = segv(Lv, mid.sid, SEG_SP);
@@ -909,7 +911,9 @@ DDH_FN void begin_solve(const Args &a, LaneState &S, double Vint, double share,
     S.Vint = Vint; S.share = share; S.maxit = maxit;
     S.b_src1 = src1; S.b_src2 = src2; S.ratio = ratio; S.b_work = work;
     S.lw[0] = lg[0]; S.lw[1] = lg[1];
-    S.it = 0; S.fresh = 1; S.need64 = 0;
+    // FP32 factors only in the scan: the start solves (far from their solutions, with the ions'
+    // stages) converge in fewer FP64 steps, and their slowest devices set the kernel's time
+    S.it = 0; S.fresh = 1; S.need64 = S.phase != PH_SCAN;
     for (int k = 0; k < 4; ++k) { S.hist[k] = INFINITY; S.Jh[k] = INFINITY; }
     S.noise_state = 0;
 }
@@ -1146,7 +1150,7 @@ DDH_FN void after_step(const Args &a, int lane, LaneState &S, double size, doubl
     if (done) { end_solve(a, lane, S, 1, J); return; }
     if (S.it >= S.maxit || size != size) { end_solve(a, lane, S, 0, J); return; }
     S.fresh = 0;
-    if (a.mixed && (S.it >= MIXED_STEPS || size < MIXED_SWITCH)) S.need64 = 1;
+    if (a.mixed && S.it >= MIXED_STEPS) S.need64 = 1;
 }
 
 // One Newton step of one lane (both sweeps and the state machine). `ions`: some lane of

@@ -51,11 +51,14 @@ DDH_MFN void bernoulli_pair(double x, int kind, double &Bf, double &Bb, double &
                                       double &dBb) {
     double ax = fabs(x);
     if (kind == 0 && ax < SERIES_BELOW) {
+        // the reference's series, its divisions by constants as products with their
+        // reciprocals (|x| < 1e-2: the terms they touch are below 1e-5 of B)
+        constexpr double I12 = 1.0 / 12.0, I720 = 1.0 / 720.0, I6 = 1.0 / 6.0, I180 = 1.0 / 180.0;
         double x2 = x * x, x4 = x2 * x2;
-        Bf = 1.0 - x / 2.0 + x2 / 12.0 - x4 / 720.0;
-        Bb = 1.0 + x / 2.0 + x2 / 12.0 - x4 / 720.0;
-        dBf = -0.5 + x / 6.0 - x * x2 / 180.0;
-        dBb = -0.5 - x / 6.0 + x * x2 / 180.0;
+        Bf = 1.0 - x * 0.5 + x2 * I12 - x4 * I720;
+        Bb = 1.0 + x * 0.5 + x2 * I12 - x4 * I720;
+        dBf = -0.5 + x * I6 - x * x2 * I180;
+        dBb = -0.5 - x * I6 + x * x2 * I180;
         return;
     }
     if (kind == 1 && ax < 1.5) {

@@ -113,7 +113,7 @@ def test_unconverged_voltages_have_zero_gradient(emu):
     packed, tan = A.tangents(th)
     V = S.VOLTAGES[::10].copy()
     states = np.zeros((len(V), 400, 3, 2))
-    J, ok, its, _ = emu.solve_packed(packed, V, max_iter=3, start_iter=60, states=states)
+    J, ok, its, _ = emu.solve_packed(packed, V, max_iter=5, start_iter=60, states=states)
     assert ok.any() and not ok.all()
     grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
     assert np.isfinite(grad).all()
