@@ -212,6 +212,7 @@ DDH_FN void adjoint_lane(const AdjArgs &a, int lane, ldouble *lval, ldouble *lta
         // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
         auto point = [&](int j, Pt &q) {
             q.sid = a.sid[(size_t)j * n + lane];
+            seg_fields(Lv, false, q);
             q.h = APT(j, PT_H);
             q.an = q.ca = 0.0;
             if (j == 0 || j == NP + 1) {

@@ -332,15 +332,18 @@ DDH_FN void poisson_lane(const Args &a, int lane, LaneState &S) {
 // ---------------------------------------------------------------------------------------
 // The Newton sweeps.
 // ---------------------------------------------------------------------------------------
-// The values of one grid point a sweep carries from visit to visit. Its material parameters
-// stay in the lane's LDS segment table and are read, through the point's segment id, where
-// they are used: registers hold only what changes from point to point.
+// The values of one grid point a sweep carries from visit to visit, with the material
+// parameters its edges and the ion border read, copied from the lane's LDS segment table
+// when the point is made (one batch of LDS reads, one wait); a row reads the rest of its
+// point's fields in a second batch.
 struct Pt {
     double V, n, p;        // the state
     double h;
     double R, dRn, dRp;    // recombination and its derivatives in ln n, ln p
     double G, fix;
     double an, ca;         // mobile ion densities (0 where none or frozen)
+    double offn, offp, mun, mup;  // the segment's fields of the same names
+    double am, cm;         // the ions' region masks (0 where none or frozen)
     int sid;               // the point's segment
 };
 
@@ -399,13 +402,38 @@ DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
 // Field f of this lane's segment id (LDS).
 DDH_FN double segv(const Lane &L, int id, int f) { return L.seg[(id * SEG_N + f) * WAVE]; }
 
+// The segment fields a point carries (q.sid set); the ion masks only while ions move.
+DDH_FN void seg_fields(const Lane &L, bool mobile, Pt &q) {
+    lcdouble *s = L.seg + q.sid * SEG_N * WAVE;
+    q.offn = s[SEG_OFFN * WAVE]; q.offp = s[SEG_OFFP * WAVE];
+    q.mun = s[SEG_MUN * WAVE]; q.mup = s[SEG_MUP * WAVE];
+    const double am = s[SEG_AM * WAVE], cm = s[SEG_CM * WAVE];
+    q.am = mobile ? am : 0.0;
+    q.cm = mobile ? cm : 0.0;
+}
+
+// The segment fields the integral current reads at a point, in one batch of LDS reads.
+struct CurSeg {
+    double kd, ni2, nt, cn, cp, nt0, pt0, offn, offp, imun, imup;
+};
+
+DDH_FN CurSeg cur_seg(const Lane &L, int sid) {
+    lcdouble *s = L.seg + sid * SEG_N * WAVE;
+    CurSeg g;
+    g.kd = s[SEG_KD * WAVE]; g.ni2 = s[SEG_NI2 * WAVE]; g.nt = s[SEG_NT * WAVE];
+    g.cn = s[SEG_CN * WAVE]; g.cp = s[SEG_CP * WAVE]; g.nt0 = s[SEG_NT0 * WAVE];
+    g.pt0 = s[SEG_PT0 * WAVE]; g.offn = s[SEG_OFFN * WAVE]; g.offp = s[SEG_OFFP * WAVE];
+    g.imun = s[SEG_IMUN * WAVE]; g.imup = s[SEG_IMUP * WAVE];
+    return g;
+}
+
 // The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108); the interval's
 // mobilities are those of l's segment (the interval after the point).
 DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
     Edge e;
-    const double lcn = Vt * segv(L, l.sid, SEG_MUN), lcp = Vt * segv(L, l.sid, SEG_MUP);
-    double xn = ((r.V + segv(L, r.sid, SEG_OFFN)) - (l.V + segv(L, l.sid, SEG_OFFN))) * iVt;
-    double xp = ((r.V + segv(L, r.sid, SEG_OFFP)) - (l.V + segv(L, l.sid, SEG_OFFP))) * iVt;
+    const double lcn = Vt * l.mun, lcp = Vt * l.mup;
+    double xn = ((r.V + r.offn) - (l.V + l.offn)) * iVt;
+    double xp = ((r.V + r.offp) - (l.V + l.offp)) * iVt;
     double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
     bernoulli_pair(xn, bern, Bnf, Bnb, dBnf, dBnb);
     bernoulli_pair(xp, bern, Bpf, Bpb, dBpf, dBpb);
@@ -459,8 +487,12 @@ DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, co
                        int sid, const SweepCtl &c, double Vt, double iVt, Pt &q) {
     const size_t n = L.n;
     q.sid = sid;
-    const double am = c.mobile ? segv(L, sid, SEG_AM) : 0.0;
-    const double cm = c.mobile ? segv(L, sid, SEG_CM) : 0.0;
+    seg_fields(L, c.mobile, q);
+    lcdouble *s = L.seg + sid * SEG_N * WAVE;
+    const double kd = s[SEG_KD * WAVE], ni2 = s[SEG_NI2 * WAVE], nt = s[SEG_NT * WAVE];
+    const double cn = s[SEG_CN * WAVE], cpp = s[SEG_CP * WAVE], nt0 = s[SEG_NT0 * WAVE];
+    const double pt0 = s[SEG_PT0 * WAVE];
+    const double am = q.am, cm = q.cm;
     if (contact) {
         q.h = ct.h;
         q.V = ct.V;
@@ -493,9 +525,7 @@ DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, co
         q.G = r.G;
         q.n = exp(y1);
         q.p = exp(y2);
-        q.R = recombination(q.n, q.p, segv(L, sid, SEG_KD), segv(L, sid, SEG_NI2),
-                            segv(L, sid, SEG_NT), segv(L, sid, SEG_CN), segv(L, sid, SEG_CP),
-                            segv(L, sid, SEG_NT0), segv(L, sid, SEG_PT0), q.dRn, q.dRp);
+        q.R = recombination(q.n, q.p, kd, ni2, nt, cn, cpp, nt0, pt0, q.dRn, q.dRp);
     }
     q.an = am != 0.0 ? am * exp(c.logs[0] + q.V * iVt) : 0.0;
     q.ca = cm != 0.0 ? cm * exp(c.logs[1] - q.V * iVt) : 0.0;
@@ -531,25 +561,26 @@ template <int K>
 DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
                        const Edge &eR, bool last, double L2, double iVt, double share,
                        double g_frac, double *A, double *B, double *C, double *r) {
+    lcdouble *sm = Lv.seg + mid.sid * SEG_N * WAVE;
+    const double meps = sm[SEG_EPS * WAVE], ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * WAVE];
+    const double sP = sm[SEG_SP * WAVE], scs = sm[SEG_SCS * WAVE], dop = sm[SEG_DOP * WAVE];
     const double hm = cur.h, hp = mid.h, hs = hp + hm;
     const double D = 1.0 / (hp * hm * hs * L2);
     const double fac_ = 0.5 * L2 * hp * hm * hs;
-    const double meps = segv(Lv, mid.sid, SEG_EPS);
     const double fu = 2 * meps * hm * D;   // 2 eps / (h_i (h_i + h_{i-1}) L^2)
     const double fl0 = 2 * meps * hp * D;  // 2 eps / (h_{i-1} (h_i + h_{i-1}) L^2)
-    const double deps = meps - segv(Lv, cur.sid, SEG_EPS);  // the jump of eps between layers
+    const double deps = meps - ceps;  // the jump of eps between layers
     // deps / (h_{i-1}^2 L^2) = deps (D h_i h_s)^2 L^2, D = 1 / (h_i h_{i-1} h_s L^2): no division
     const double Dhh = D * hp * hs;
     const double jump = deps != 0.0 ? deps * (Dhh * Dhh) * L2 : 0.0;
     const double fac_m = jump - (fu + fl0);
     const double fac_l = fl0 - jump;
-    const double sP = segv(Lv, mid.sid, SEG_SP);
-    const double sC = segv(Lv, mid.sid, SEG_SCS) * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
+    const double sC = scs * (hm * hs * L2 * D);  // (1/h_i) = h_{i-1} h_s L^2 D
     const double gen = mid.G * g_frac;
     const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
     const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
     const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
-                        Q_E * (mid.n - mid.p + segv(Lv, mid.sid, SEG_DOP) + share * mid.fix + mid.an - mid.ca);
+                        Q_E * (mid.n - mid.p + dop + share * mid.fix + mid.an - mid.ca);
     A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
     A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
     A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
@@ -600,12 +631,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     const Contact left = contact_point(a, lane, Lv, c.Vint, false);
     const Contact right = contact_point(a, lane, Lv, c.Vint, true);
     const int sid0 = Lv.sid[0];
-    // the ions' region masks, none once frozen
-    auto mam = [&](int id) { return c.mobile ? segv(Lv, id, SEG_AM) : 0.0; };
-    auto mcm = [&](int id) { return c.mobile ? segv(Lv, id, SEG_CM) : 0.0; };
     Pt mid, cur;
-    double rV = 0.0;  // the row's right neighbour: its potential and segment
-    int rsid = 0;
+    double rV = 0.0;  // the row's right neighbour: its potential and ion masks
+    double ram = 0.0, rcm = 0.0;
     double rlnN = 0.0, rlnP = 0.0;  // and its log densities (the noise's right-hand side)
     Raw nxt;
     fetch_point(Lv, s1, s2, NP, nxt);
@@ -629,8 +657,8 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
         const Edge eL = edge_flux(Lv, cur, mid, Vt, iVt, bern);
         if constexpr (K > 1) {
             const double hLj = cur.h * L;
-            num0 += 0.5 * (mid.an + cur.an) * (mam(mid.sid) * mam(cur.sid)) * hLj;
-            num1 += 0.5 * (mid.ca + cur.ca) * (mcm(mid.sid) * mcm(cur.sid)) * hLj;
+            num0 += 0.5 * (mid.an + cur.an) * (mid.am * cur.am) * hLj;
+            num1 += 0.5 * (mid.ca + cur.ca) * (mid.cm * cur.cm) * hLj;
         }
         if (has_row) {
             // row i = j + 1: left = cur, mid, right; edges eL (i-1, i), eR (i, i+1)
@@ -713,9 +741,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             if constexpr (K > 1) {
                 // gamma_i = W_i - c'_{i+1}^T gamma_{i+1}; W_i only in the potential's row
                 const double hLm = cur.h * L, hLp = mid.h * L;
-                const double mida = mam(mid.sid), midc = mcm(mid.sid);
-                const double w0 = 0.5 * (mida * mam(cur.sid) * hLm + mam(rsid) * mida * hLp) * mid.an * iVt * itot0;
-                const double w1 = -0.5 * (midc * mcm(cur.sid) * hLm + mcm(rsid) * midc * hLp) * mid.ca * iVt * itot1;
+                const double mida = mid.am, midc = mid.cm;
+                const double w0 = 0.5 * (mida * cur.am * hLm + ram * mida * hLp) * mid.an * iVt * itot0;
+                const double w1 = -0.5 * (midc * cur.cm * hLm + rcm * midc * hLp) * mid.ca * iVt * itot1;
                 double ng[3][2];
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {
@@ -741,7 +769,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
 #pragma unroll
             for (int q = 0; q < 3 * K; ++q) dp[q] = dn_[q];
         }
-        rV = mid.V; rsid = mid.sid;
+        rV = mid.V; ram = mid.am; rcm = mid.cm;
         if constexpr (NOISE) { rlnN = log(mid.n); rlnP = log(mid.p); }
         mid = cur;
         eR = eL;
@@ -838,6 +866,7 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
             size = nmax(size, fabs(x2) * (dp / (dp + nfl)));
             return;
         }
+        const CurSeg g = cur_seg(Lv, sid);
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
         double y0 = now.y0, y1 = now.y1, y2 = now.y2;
         const double st0 = clipd(x0, -capv, capv);
@@ -858,13 +887,10 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
         size = nmax(size, fabs(st1) * (nn / (nn + nfl)));
         size = nmax(size, fabs(st2) * (pp / (pp + nfl)));
         // the integral current of the new state (src/jaxdd/batch.py:321-373)
-        const double R = recombination_value(nn, pp, segv(Lv, sid, SEG_KD), segv(Lv, sid, SEG_NI2),
-                                             segv(Lv, sid, SEG_NT), segv(Lv, sid, SEG_CN),
-                                             segv(Lv, sid, SEG_CP), segv(Lv, sid, SEG_NT0),
-                                             segv(Lv, sid, SEG_PT0));
+        const double R = recombination_value(nn, pp, g.kd, g.ni2, g.nt, g.cn, g.cp, g.nt0, g.pt0);
         const double U = R - now.G * c.g_frac;
-        const double wn = exp(-(y0 + segv(Lv, sid, SEG_OFFN)) * iVt) * segv(Lv, sid, SEG_IMUN);
-        const double wp = exp((y0 + segv(Lv, sid, SEG_OFFP)) * iVt) * segv(Lv, sid, SEG_IMUP);
+        const double wn = exp(-(y0 + g.offn) * iVt) * g.imun;
+        const double wp = exp((y0 + g.offp) * iVt) * g.imup;
         const double xj = now.x;
         const double dx = xj - x_prev;
         const double iu = iu_prev + 0.5 * dx * (U_prev + U);

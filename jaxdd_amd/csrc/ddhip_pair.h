@@ -64,8 +64,6 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
     gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
     gdouble *s2 = (c.fresh && c.ratio != 0.0) ? Lv.ys2 : s1;
     const Contact ct = contact_point(a, lane, Lv, c.Vint, half == 1);
-    auto mam = [&](int id) { return c.mobile ? segv(Lv, id, SEG_AM) : 0.0; };
-    auto mcm = [&](int id) { return c.mobile ? segv(Lv, id, SEG_CM) : 0.0; };
 
     T cp[9], dp[3 * K];
 #pragma unroll
@@ -98,8 +96,8 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
             // the ion numbers over the half's intervals: left 0..m-1, right m..NP
             if (dir > 0 ? j <= m : j >= m) {
                 const double hk = el.h * L;
-                num0 += 0.5 * (mid.an + pn.an) * (mam(mid.sid) * mam(pn.sid)) * hk;
-                num1 += 0.5 * (mid.ca + pn.ca) * (mcm(mid.sid) * mcm(pn.sid)) * hk;
+                num0 += 0.5 * (mid.an + pn.an) * (mid.am * pn.am) * hk;
+                num1 += 0.5 * (mid.ca + pn.ca) * (mid.cm * pn.cm) * hk;
             }
         }
         if (has_row) {
@@ -173,9 +171,9 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
             if constexpr (K > 1) {
                 // gamma_i = W_i - p'^T gamma_prev, p' the previous row's factor
                 const double hLm = lft.h * L, hLp = mid.h * L;
-                const double mida = mam(mid.sid), midc = mcm(mid.sid);
-                const double w0 = 0.5 * (mida * mam(lft.sid) * hLm + mam(rgt.sid) * mida * hLp) * mid.an * iVt * itot0;
-                const double w1 = -0.5 * (midc * mcm(lft.sid) * hLm + mcm(rgt.sid) * midc * hLp) * mid.ca * iVt * itot1;
+                const double mida = mid.am, midc = mid.cm;
+                const double w0 = 0.5 * (mida * lft.am * hLm + rgt.am * mida * hLp) * mid.an * iVt * itot0;
+                const double w1 = -0.5 * (midc * lft.cm * hLm + rgt.cm * midc * hLp) * mid.ca * iVt * itot1;
                 double ng[3][2];
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {
@@ -355,6 +353,7 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
             x2 = dF[2] - ((double)f[6] * xs0 + (double)f[7] * xs1 + (double)f[8] * xs2);
         }
         xs0 = x0; xs1 = x1; xs2 = x2;
+        const CurSeg g = cur_seg(Lv, sid);
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
         double y0 = now.y0, y1 = now.y1, y2 = now.y2;
         const double st0 = clipd(x0, -capv, capv);
@@ -374,13 +373,10 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
         size = nmax(size, fabs(st0) * iVt);
         size = nmax(size, fabs(st1) * (nn / (nn + nfl)));
         size = nmax(size, fabs(st2) * (pp / (pp + nfl)));
-        const double R = recombination_value(nn, pp, segv(Lv, sid, SEG_KD), segv(Lv, sid, SEG_NI2),
-                                             segv(Lv, sid, SEG_NT), segv(Lv, sid, SEG_CN),
-                                             segv(Lv, sid, SEG_CP), segv(Lv, sid, SEG_NT0),
-                                             segv(Lv, sid, SEG_PT0));
+        const double R = recombination_value(nn, pp, g.kd, g.ni2, g.nt, g.cn, g.cp, g.nt0, g.pt0);
         const double U = R - now.G * c.g_frac;
-        const double wn = exp(-(y0 + segv(Lv, sid, SEG_OFFN)) * iVt) * segv(Lv, sid, SEG_IMUN);
-        const double wp = exp((y0 + segv(Lv, sid, SEG_OFFP)) * iVt) * segv(Lv, sid, SEG_IMUP);
+        const double wn = exp(-(y0 + g.offn) * iVt) * g.imun;
+        const double wp = exp((y0 + g.offp) * iVt) * g.imup;
         const double xj = now.x;
         if (v == 0) {
             out.wn = wn; out.wp = wp; out.U = U; out.x = xj;
