@@ -24,8 +24,8 @@ extern "C" {
 /* Segments (runs of points of one material) per device: the solve keeps each wavefront's
  * segment tables in the compute unit's 160 KB local data share. */
 #define DDH_MAX_SEGMENTS 14
-/* The adjoint keeps a second table (the tangents) beside it: at most 7 segments (3 layers). */
-#define DDH_ADJOINT_MAX_SEGMENTS 7
+/* The adjoint reads the segment tables from global memory: the solve's limit. */
+#define DDH_ADJOINT_MAX_SEGMENTS DDH_MAX_SEGMENTS
 
 enum ddh_status {
     DDH_OK = 0,

@@ -4,7 +4,7 @@
 // At a converged state y*, F(y*, theta) = 0, so dy*/dtheta = -T^-1 dF/dtheta with T = dF/dy
 // (the Newton matrix, F the row-scaled residual), and
 //     dJ/dtheta = dJ/dtheta|_y - lambda^T dF/dtheta|_y,    T^T lambda = dJ/dy.
-// One lane per device; for each converged voltage:
+// One wavefront per device, one lane per voltage; at each converged voltage:
 //   forward pass (left to right)   the integral current's sums at y* (src/jaxdd/batch.py:302-403)
 //   elimination (right to left)    T's blocks at y* (the Newton sweep's own row assembly),
 //                                  T = U'L' eliminated as the Newton sweep does; dJ/dy by the
@@ -60,14 +60,17 @@ struct AdjArgs {
     const double *dsc;      // [ntheta][SC_N][n]
     double *grad;           // [n][nV][ntheta]
     int32_t *status;        // [n]
-    double *fac;            // [NP][ADJ_F][n] Mi, C, w per row
-    double *lam;            // [nV][NP][3][n]
-    double *iu;             // [NP + 1][n]
+    // Each launched lane's own rows, lane stride ls (the launch's lane count):
+    double *fac;            // [NP][ADJ_F][ls] Mi, C, w per row
+    double *st;             // [NP][3][ls] the state's V, n, p
+    double *lam;            // [NP][3][ls]
+    double *iu;             // [NP + 1][ls] the current's running integral of U
+    size_t ls;
 };
 constexpr int ADJ_F = 21;
-
-#define ASC(f) a.sc[(size_t)(f) * a.n + lane]
-#define APT(j, f) a.pt[((size_t)(j) * PT_N + (f)) * a.n + lane]
+// Workgroups of the adjoint launch (one wavefront each, looping over the devices): the
+// per-lane workspace is sized by them, not by the batch.
+constexpr int ADJ_SLOTS = 1024;
 
 // A point of a dual pass: its potential and densities (tangents only at the contacts, whose
 // values are parameters) and its spacing.
@@ -76,12 +79,13 @@ struct PD {
     int sid;
 };
 
-// The lane's segment field with its tangent (the tangent table of the current direction in
-// LDS, beside the value table).
+// The device's segment field with its tangent along the current direction (global memory;
+// the same device for the whole wavefront).
 struct SegD {
-    lcdouble *val, *tan;
+    const double *val, *tan;
+    size_t n;
     DDH_FN D1 operator()(int id, int f) const {
-        const int o = (id * SEG_N + f) * WAVE;
+        const size_t o = ((size_t)id * SEG_N + f) * n;
         return {val[o], tan[o]};
     }
 };
@@ -146,148 +150,156 @@ DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, d
 
 DDH_FN void inverse3d(const double *m, double *out) { inverse3<double>(m, out); }
 
-// One lane's gradients (every voltage, every direction).
-DDH_FN void adjoint_lane(const AdjArgs &a, int lane, ldouble *lval, ldouble *ltan) {
+// Device d's values (the same for every lane of the wavefront).
+#define ASC(f) a.sc[(size_t)(f) * a.n + d]
+#define APT(j, f) a.pt[((size_t)(j) * PT_N + (f)) * a.n + d]
+#define ASID(j) ((int)a.sid[(size_t)(j) * a.n + d])
+#define ASEG(id, f) a.seg[((size_t)(id) * SEG_N + (f)) * a.n + d]
+
+// The gradients of device d at voltage v (every direction), by the launched lane sl. A device
+// with mobile ions gets NaN, an unconverged voltage zeros.
+DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobile) {
     const int NP = a.NP, nV = a.nV, K = a.ntheta;
-    const size_t n = (size_t)a.n;
-    double *out = a.grad + (size_t)lane * nV * K;
-    if (ASC(SC_AT) > 0 || ASC(SC_CT) > 0) {
-        a.status[lane] = 1;
-        for (int q = 0; q < nV * K; ++q) out[q] = NAN;
+    const size_t n = (size_t)a.n, ls = a.ls;
+    double *out = a.grad + ((size_t)d * nV + v) * K;
+    if (mobile) {
+        for (int k = 0; k < K; ++k) out[k] = NAN;
+        return;
+    }
+    if (!a.ok[(size_t)d * nV + v]) {
+        for (int k = 0; k < K; ++k) out[k] = 0.0;
         return;
     }
     const double Vt = ASC(SC_VT), iVt = 1.0 / Vt, L = ASC(SC_L), L2 = L * L;
     const double gfr = ASC(SC_GFRAC), elec = ASC(SC_ELEC), cJ = -elec * Q_E;
-    for (int q = 0; q < a.nseg * SEG_N; ++q) lval[(size_t)q * WAVE] = a.seg[(size_t)q * n + lane];
-    Lane Lv;
-    Lv.pt = (gcdouble *)(a.pt + lane);
-    Lv.sid = (gcuint8 *)(a.sid + lane);
-    Lv.seg = lval;
-    Lv.n = n;
-    const int sid0 = a.sid[lane], sidN = a.sid[(size_t)(NP + 1) * n + lane];
-#define FAC(i, q) a.fac[((size_t)((i) - 1) * ADJ_F + (q)) * n + lane]
-#define LAM(v, i, f) a.lam[(((size_t)(v) * NP + ((i) - 1)) * 3 + (f)) * n + lane]
-#define IU(j) a.iu[(size_t)(j) * n + lane]
-#define YS(v, j, f) a.states[(((size_t)(v) * NP + ((j) - 1)) * 3 + (f)) * n + lane]
+    const int sid0 = ASID(0), sidN = ASID(NP + 1);
+#define FAC(i, q) a.fac[((size_t)((i) - 1) * ADJ_F + (q)) * ls + sl]
+#define STV(j, f) a.st[((size_t)((j) - 1) * 3 + (f)) * ls + sl]
+#define LAM(i, f) a.lam[((size_t)((i) - 1) * 3 + (f)) * ls + sl]
+#define IU(j) a.iu[(size_t)(j) * ls + sl]
+#define YS(j, f) a.states[(((size_t)v * NP + ((j) - 1)) * 3 + (f)) * n + d]
 
-    // ---- lambda at each converged voltage ----------------------------------------------
-    for (int v = 0; v < nV; ++v) {
-        if (!a.ok[(size_t)lane * nV + v]) continue;
-        const double Vint = a.V[v];
-        const double V0c = ASC(SC_V0) + 0.5 * elec * Vint, VLc = ASC(SC_VL) - 0.5 * elec * Vint;
-        // the current's sums (the Newton sweep's order)
-        const double Vg0n = V0c + segv(Lv, sid0, SEG_OFFN), Vg0p = V0c + segv(Lv, sid0, SEG_OFFP);
-        double wn_prev = exp(-Vg0n * iVt) * segv(Lv, sid0, SEG_IMUN);
-        double wp_prev = exp(Vg0p * iVt) * segv(Lv, sid0, SEG_IMUP);
-        const double endn0 = exp(ASC(SC_LNL) - Vg0n * iVt), endp0 = exp(ASC(SC_LPL) + Vg0p * iVt);
-        double U_prev = 0.0, iu_prev = 0.0, x_prev = APT(0, PT_X);
-        double Sn = 0.0, Dn = 0.0, Sp = 0.0, Dp = 0.0;
-        IU(0) = 0.0;
-        for (int j = 1; j <= NP; ++j) {
-            const int id = a.sid[(size_t)j * n + lane];
-            const double y0 = YS(v, j, 0), nn = exp(YS(v, j, 1)), pp = exp(YS(v, j, 2));
-            const double R = recombination_value(nn, pp, segv(Lv, id, SEG_KD), segv(Lv, id, SEG_NI2),
-                                                 segv(Lv, id, SEG_NT), segv(Lv, id, SEG_CN),
-                                                 segv(Lv, id, SEG_CP), segv(Lv, id, SEG_NT0),
-                                                 segv(Lv, id, SEG_PT0));
-            const double U = R - APT(j, PT_G) * gfr;
-            const double wn = exp(-(y0 + segv(Lv, id, SEG_OFFN)) * iVt) * segv(Lv, id, SEG_IMUN);
-            const double wp = exp((y0 + segv(Lv, id, SEG_OFFP)) * iVt) * segv(Lv, id, SEG_IMUP);
-            const double xj = APT(j, PT_X), dx = xj - x_prev;
-            const double iu = iu_prev + 0.5 * dx * (U_prev + U);
-            Sn += 0.5 * dx * (wn_prev + wn);
-            Dn += 0.5 * dx * (wn_prev * iu_prev + wn * iu);
-            Sp += 0.5 * dx * (wp_prev + wp);
-            Dp += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
-            IU(j) = iu;
-            wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
-        }
-        const double endnN = exp(ASC(SC_LNR) - (VLc + segv(Lv, sidN, SEG_OFFN)) * iVt);
-        const double endpN = exp(ASC(SC_LPR) + (VLc + segv(Lv, sidN, SEG_OFFP)) * iVt);
-        const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
-        const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
-        // J = cJ (Kn + Kp): its partials in the four sums
-        const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
+    // ---- lambda ------------------------------------------------------------------------
+    const double Vint = a.V[v];
+    const double V0c = ASC(SC_V0) + 0.5 * elec * Vint, VLc = ASC(SC_VL) - 0.5 * elec * Vint;
+    // the current's sums (the Newton sweep's order), the state's densities kept for the passes
+    const double Vg0n = V0c + ASEG(sid0, SEG_OFFN), Vg0p = V0c + ASEG(sid0, SEG_OFFP);
+    double wn_prev = exp(-Vg0n * iVt) * ASEG(sid0, SEG_IMUN);
+    double wp_prev = exp(Vg0p * iVt) * ASEG(sid0, SEG_IMUP);
+    const double endn0 = exp(ASC(SC_LNL) - Vg0n * iVt), endp0 = exp(ASC(SC_LPL) + Vg0p * iVt);
+    double U_prev = 0.0, iu_prev = 0.0, x_prev = APT(0, PT_X);
+    double Sn = 0.0, Dn = 0.0, Sp = 0.0, Dp = 0.0;
+    IU(0) = 0.0;
+    for (int j = 1; j <= NP; ++j) {
+        const int id = ASID(j);
+        const double y0 = YS(j, 0), nn = exp(YS(j, 1)), pp = exp(YS(j, 2));
+        STV(j, 0) = y0; STV(j, 1) = nn; STV(j, 2) = pp;
+        const double R = recombination_value(nn, pp, ASEG(id, SEG_KD), ASEG(id, SEG_NI2),
+                                             ASEG(id, SEG_NT), ASEG(id, SEG_CN), ASEG(id, SEG_CP),
+                                             ASEG(id, SEG_NT0), ASEG(id, SEG_PT0));
+        const double U = R - APT(j, PT_G) * gfr;
+        const double wn = exp(-(y0 + ASEG(id, SEG_OFFN)) * iVt) * ASEG(id, SEG_IMUN);
+        const double wp = exp((y0 + ASEG(id, SEG_OFFP)) * iVt) * ASEG(id, SEG_IMUP);
+        const double xj = APT(j, PT_X), dx = xj - x_prev;
+        const double iu = iu_prev + 0.5 * dx * (U_prev + U);
+        Sn += 0.5 * dx * (wn_prev + wn);
+        Dn += 0.5 * dx * (wn_prev * iu_prev + wn * iu);
+        Sp += 0.5 * dx * (wp_prev + wp);
+        Dp += 0.5 * dx * (wp_prev * iu_prev + wp * iu);
+        IU(j) = iu;
+        wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
+    }
+    const double endnN = exp(ASC(SC_LNR) - (VLc + ASEG(sidN, SEG_OFFN)) * iVt);
+    const double endpN = exp(ASC(SC_LPR) + (VLc + ASEG(sidN, SEG_OFFP)) * iVt);
+    const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
+    const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
+    // J = cJ (Kn + Kp): its partials in the four sums
+    const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
 
-        // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
-        auto point = [&](int j, Pt &q) {
-            q.sid = a.sid[(size_t)j * n + lane];
-            seg_fields(Lv, false, q);
-            q.h = APT(j, PT_H);
-            q.an = q.ca = 0.0;
-            if (j == 0 || j == NP + 1) {
-                q.V = j == 0 ? V0c : VLc;
-                q.n = exp(j == 0 ? ASC(SC_LNL) : ASC(SC_LNR));
-                q.p = exp(j == 0 ? ASC(SC_LPL) : ASC(SC_LPR));
-                q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
-                return;
-            }
-            q.V = YS(v, j, 0);
-            q.n = exp(YS(v, j, 1));
-            q.p = exp(YS(v, j, 2));
-            q.G = APT(j, PT_G);
-            q.fix = APT(j, PT_FIX);
-            q.R = recombination(q.n, q.p, segv(Lv, q.sid, SEG_KD), segv(Lv, q.sid, SEG_NI2),
-                                segv(Lv, q.sid, SEG_NT), segv(Lv, q.sid, SEG_CN),
-                                segv(Lv, q.sid, SEG_CP), segv(Lv, q.sid, SEG_NT0),
-                                segv(Lv, q.sid, SEG_PT0), q.dRn, q.dRp);
-        };
-        Pt right, mid, cur;
-        point(NP + 1, right);
-        point(NP, mid);
-        Edge eR = edge_flux(Lv, mid, right, Vt, iVt, a.bern);
-        double cpn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, wn_[3] = {0, 0, 0};
-        double Tn1 = 0.0, Tp1 = 0.0;  // the current's suffix sums from point i + 1
-        double x_next = APT(NP + 1, PT_X), dx_next = 0.0;  // no interval after NP in the sums
-        for (int i = NP; i >= 1; --i) {
-            point(i - 1, cur);
-            const Edge eL = edge_flux(Lv, cur, mid, Vt, iVt, a.bern);
-            double A[9], B[9], C[9], r[3];
-            row_blocks<1>(Lv, cur, mid, right.V, eL, eR, i == NP, L2, iVt, 1.0, gfr, A, B, C, r);
-            double M[9], Mi[9];
-#pragma unroll
-            for (int p = 0; p < 3; ++p)
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    M[3 * p + q] = B[3 * p + q] - (C[3 * p] * cpn[q] + C[3 * p + 1] * cpn[3 + q] +
-                                                   C[3 * p + 2] * cpn[6 + q]);
-            inverse3d(M, Mi);
-            // dJ/dy_i by the current's reverse mode
-            const double xi = APT(i, PT_X), dx = xi - APT(i - 1, PT_X);
-            const double ai = 0.5 * (dx + dx_next);
-            const double wn = exp(-(mid.V + segv(Lv, mid.sid, SEG_OFFN)) * iVt) * segv(Lv, mid.sid, SEG_IMUN);
-            const double wp = exp((mid.V + segv(Lv, mid.sid, SEG_OFFP)) * iVt) * segv(Lv, mid.sid, SEG_IMUP);
-            const double Tn = ai * wn + Tn1, Tp = ai * wp + Tp1;
-            const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
-                              bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
-            const double iu = IU(i);
-            double g[3];
-            g[0] = (bSn + bDn * iu) * ai * (-wn * iVt) + (bSp + bDp * iu) * ai * (wp * iVt);
-            g[1] = bU * mid.dRn;
-            g[2] = bU * mid.dRp;
-            // w_i = g_i - c'_{i+1}^T w_{i+1}
-            double w[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) w[q] = g[q] - (cpn[q] * wn_[0] + cpn[3 + q] * wn_[1] + cpn[6 + q] * wn_[2]);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) FAC(i, q) = Mi[q];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) FAC(i, 9 + q) = C[q];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) FAC(i, 18 + q) = w[q];
-            // c'_i = Mi A_i
-#pragma unroll
-            for (int p = 0; p < 3; ++p)
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    cpn[3 * p + q] = Mi[3 * p] * A[q] + Mi[3 * p + 1] * A[3 + q] + Mi[3 * p + 2] * A[6 + q];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) wn_[q] = w[q];
-            Tn1 = Tn; Tp1 = Tp; dx_next = dx; x_next = xi;
-            right = mid; mid = cur; eR = eL;
+    // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
+    auto point = [&](int j, Pt &q) {
+        q.sid = ASID(j);
+        q.offn = ASEG(q.sid, SEG_OFFN); q.offp = ASEG(q.sid, SEG_OFFP);
+        q.mun = ASEG(q.sid, SEG_MUN); q.mup = ASEG(q.sid, SEG_MUP);
+        q.am = q.cm = 0.0;
+        q.h = APT(j, PT_H);
+        q.an = q.ca = 0.0;
+        if (j == 0 || j == NP + 1) {
+            q.V = j == 0 ? V0c : VLc;
+            q.n = exp(j == 0 ? ASC(SC_LNL) : ASC(SC_LNR));
+            q.p = exp(j == 0 ? ASC(SC_LPL) : ASC(SC_LPR));
+            q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
+            return;
         }
-        (void)x_next;
-        // lambda_i = Mi_i^T (w_i - C_{i-1}^T lambda_{i-1}), left to right
+        q.V = STV(j, 0);
+        q.n = STV(j, 1);
+        q.p = STV(j, 2);
+        q.G = APT(j, PT_G);
+        q.fix = APT(j, PT_FIX);
+        q.R = recombination(q.n, q.p, ASEG(q.sid, SEG_KD), ASEG(q.sid, SEG_NI2),
+                            ASEG(q.sid, SEG_NT), ASEG(q.sid, SEG_CN), ASEG(q.sid, SEG_CP),
+                            ASEG(q.sid, SEG_NT0), ASEG(q.sid, SEG_PT0), q.dRn, q.dRp);
+    };
+    Pt right, mid, cur;
+    point(NP + 1, right);
+    point(NP, mid);
+    Edge eR = edge_flux_pt(mid, right, Vt, iVt, a.bern);
+    double cpn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, wn_[3] = {0, 0, 0};
+    double Tn1 = 0.0, Tp1 = 0.0;  // the current's suffix sums from point i + 1
+    double dx_next = 0.0;         // no interval after NP in the sums
+    for (int i = NP; i >= 1; --i) {
+        point(i - 1, cur);
+        const Edge eL = edge_flux_pt(cur, mid, Vt, iVt, a.bern);
+        double A[9], B[9], C[9], r[3];
+        RowSeg g;
+        g.meps = ASEG(mid.sid, SEG_EPS);
+        g.ceps = ASEG(cur.sid, SEG_EPS);
+        g.sP = ASEG(mid.sid, SEG_SP); g.scs = ASEG(mid.sid, SEG_SCS); g.dop = ASEG(mid.sid, SEG_DOP);
+        row_blocks_s<1>(g, cur, mid, right.V, eL, eR, i == NP, L2, iVt, 1.0, gfr, A, B, C, r);
+        double M[9], Mi[9];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                M[3 * p + q] = B[3 * p + q] - (C[3 * p] * cpn[q] + C[3 * p + 1] * cpn[3 + q] +
+                                               C[3 * p + 2] * cpn[6 + q]);
+        inverse3d(M, Mi);
+        // dJ/dy_i by the current's reverse mode
+        const double xi = APT(i, PT_X), dx = xi - APT(i - 1, PT_X);
+        const double ai = 0.5 * (dx + dx_next);
+        const double wn = exp(-(mid.V + mid.offn) * iVt) * ASEG(mid.sid, SEG_IMUN);
+        const double wp = exp((mid.V + mid.offp) * iVt) * ASEG(mid.sid, SEG_IMUP);
+        const double Tn = ai * wn + Tn1, Tp = ai * wp + Tp1;
+        const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
+                          bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
+        const double iu = IU(i);
+        double gv[3];
+        gv[0] = (bSn + bDn * iu) * ai * (-wn * iVt) + (bSp + bDp * iu) * ai * (wp * iVt);
+        gv[1] = bU * mid.dRn;
+        gv[2] = bU * mid.dRp;
+        // w_i = g_i - c'_{i+1}^T w_{i+1}
+        double w[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) w[q] = gv[q] - (cpn[q] * wn_[0] + cpn[3 + q] * wn_[1] + cpn[6 + q] * wn_[2]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) FAC(i, q) = Mi[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) FAC(i, 9 + q) = C[q];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) FAC(i, 18 + q) = w[q];
+        // c'_i = Mi A_i
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                cpn[3 * p + q] = Mi[3 * p] * A[q] + Mi[3 * p + 1] * A[3 + q] + Mi[3 * p + 2] * A[6 + q];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) wn_[q] = w[q];
+        Tn1 = Tn; Tp1 = Tp; dx_next = dx;
+        right = mid; mid = cur; eR = eL;
+    }
+    // lambda_i = Mi_i^T (w_i - C_{i-1}^T lambda_{i-1}), left to right
+    {
         double lp[3] = {0, 0, 0}, Cp_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int i = 1; i <= NP; ++i) {
             double t[3];
@@ -300,85 +312,80 @@ DDH_FN void adjoint_lane(const AdjArgs &a, int lane, ldouble *lval, ldouble *lta
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 lp[q] = Mi[q] * t[0] + Mi[3 + q] * t[1] + Mi[6 + q] * t[2];
-                LAM(v, i, q) = lp[q];
+                LAM(i, q) = lp[q];
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q) Cp_[q] = FAC(i, 9 + q);
         }
     }
 
-    // ---- per direction: dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y ---------------
+    // ---- per direction: dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y -------------
     for (int k = 0; k < K; ++k) {
-        const double *dpt = a.dpt + (size_t)k * (NP + 2) * PT_N * n + lane;
-        const double *dsc = a.dsc + (size_t)k * SC_N * n + lane;
-        const double *dsg = a.dseg + (size_t)k * a.nseg * SEG_N * n + lane;
-        for (int q = 0; q < a.nseg * SEG_N; ++q) ltan[(size_t)q * WAVE] = dsg[(size_t)q * n];
+        const double *dpt = a.dpt + (size_t)k * (NP + 2) * PT_N * n + d;
+        const double *dsc = a.dsc + (size_t)k * SC_N * n + d;
         SegD sd;
-        sd.val = lval;
-        sd.tan = ltan;
+        sd.val = a.seg + d;
+        sd.tan = a.dseg + (size_t)k * a.nseg * SEG_N * n + d;
+        sd.n = n;
         auto ptd = [&](int j, int f) -> D1 {
             return {APT(j, f), dpt[((size_t)j * PT_N + f) * n]};
         };
         auto scd = [&](int f) -> D1 { return {ASC(f), dsc[(size_t)f * n]}; };
         const D1 Ld = scd(SC_L);
-        for (int v = 0; v < nV; ++v) {
-            double *o = out + (size_t)v * K + k;
-            if (!a.ok[(size_t)lane * nV + v]) { *o = 0.0; continue; }
-            const double Vint = a.V[v];
-            const D1 V0c = scd(SC_V0) + 0.5 * elec * Vint, VLc = scd(SC_VL) - 0.5 * elec * Vint;
-            auto point = [&](int j, PD &q) {
-                q.sid = a.sid[(size_t)j * n + lane];
-                q.h = ptd(j, PT_H);
-                if (j == 0 || j == NP + 1) {
-                    q.V = j == 0 ? V0c : VLc;
-                    q.n = dexp(scd(j == 0 ? SC_LNL : SC_LNR));
-                    q.p = dexp(scd(j == 0 ? SC_LPL : SC_LPR));
-                    return;
-                }
-                q.V = dc(YS(v, j, 0));
-                q.n = dc(exp(YS(v, j, 1)));
-                q.p = dc(exp(YS(v, j, 2)));
-            };
-            // the current along the direction
-            const D1 Vg0n = V0c + sd(sid0, SEG_OFFN), Vg0p = V0c + sd(sid0, SEG_OFFP);
-            D1 wn_prev = dexp(-Vg0n * iVt) * sd(sid0, SEG_IMUN);
-            D1 wp_prev = dexp(Vg0p * iVt) * sd(sid0, SEG_IMUP);
-            const D1 endn0 = dexp(scd(SC_LNL) - Vg0n * iVt), endp0 = dexp(scd(SC_LPL) + Vg0p * iVt);
-            D1 U_prev = dc(0.0), iu_prev = dc(0.0), x_prev = ptd(0, PT_X);
-            D1 Sn = dc(0.0), Dn = dc(0.0), Sp = dc(0.0), Dp = dc(0.0);
-            double acc = 0.0;  // lambda . dF/dtheta_k
-            PD l, m, r;
-            point(0, l);
-            point(1, m);
-            for (int j = 1; j <= NP; ++j) {
-                point(j + 1, r);
-                // row j
-                D1 F[3];
-                row_d1(l, m, r, ptd(j, PT_G), ptd(j, PT_FIX), Ld, gfr, sd, Vt, iVt, a.bern, F);
-                acc += LAM(v, j, 0) * F[0].d + LAM(v, j, 1) * F[1].d + LAM(v, j, 2) * F[2].d;
-                // the current's terms at point j
-                const D1 R = recomb_d1(m.n, m.p, sd, m.sid);
-                const D1 U = R - ptd(j, PT_G) * gfr;
-                const D1 wn = dexp(-(m.V + sd(m.sid, SEG_OFFN)) * iVt) * sd(m.sid, SEG_IMUN);
-                const D1 wp = dexp((m.V + sd(m.sid, SEG_OFFP)) * iVt) * sd(m.sid, SEG_IMUP);
-                const D1 xj = ptd(j, PT_X), dx = xj - x_prev;
-                const D1 iu = iu_prev + 0.5 * dx * (U_prev + U);
-                Sn = Sn + 0.5 * dx * (wn_prev + wn);
-                Dn = Dn + 0.5 * dx * (wn_prev * iu_prev + wn * iu);
-                Sp = Sp + 0.5 * dx * (wp_prev + wp);
-                Dp = Dp + 0.5 * dx * (wp_prev * iu_prev + wp * iu);
-                wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
-                l = m; m = r;
+        const D1 V0d = scd(SC_V0) + 0.5 * elec * Vint, VLd = scd(SC_VL) - 0.5 * elec * Vint;
+        auto pointd = [&](int j, PD &q) {
+            q.sid = ASID(j);
+            q.h = ptd(j, PT_H);
+            if (j == 0 || j == NP + 1) {
+                q.V = j == 0 ? V0d : VLd;
+                q.n = dexp(scd(j == 0 ? SC_LNL : SC_LNR));
+                q.p = dexp(scd(j == 0 ? SC_LPL : SC_LPR));
+                return;
             }
-            const D1 endnN = dexp(scd(SC_LNR) - (VLc + sd(sidN, SEG_OFFN)) * iVt);
-            const D1 endpN = dexp(scd(SC_LPR) + (VLc + sd(sidN, SEG_OFFP)) * iVt);
-            const D1 Kn = (Vt * (endn0 - endnN) + Dn) / Sn;
-            const D1 Kp = (Vt * (endpN - endp0) - Dp) / Sp;
-            const D1 J = cJ * (Kn + Kp);
-            *o = J.d - acc;
+            q.V = dc(STV(j, 0));
+            q.n = dc(STV(j, 1));
+            q.p = dc(STV(j, 2));
+        };
+        // the current along the direction
+        const D1 Vg0nd = V0d + sd(sid0, SEG_OFFN), Vg0pd = V0d + sd(sid0, SEG_OFFP);
+        D1 wnp = dexp(-Vg0nd * iVt) * sd(sid0, SEG_IMUN);
+        D1 wpp = dexp(Vg0pd * iVt) * sd(sid0, SEG_IMUP);
+        const D1 endn0d = dexp(scd(SC_LNL) - Vg0nd * iVt), endp0d = dexp(scd(SC_LPL) + Vg0pd * iVt);
+        D1 Up = dc(0.0), iup = dc(0.0), xp = ptd(0, PT_X);
+        D1 Snd = dc(0.0), Dnd = dc(0.0), Spd = dc(0.0), Dpd = dc(0.0);
+        double acc = 0.0;  // lambda . dF/dtheta_k
+        PD l, m, r;
+        pointd(0, l);
+        pointd(1, m);
+        for (int j = 1; j <= NP; ++j) {
+            pointd(j + 1, r);
+            // row j
+            D1 F[3];
+            row_d1(l, m, r, ptd(j, PT_G), ptd(j, PT_FIX), Ld, gfr, sd, Vt, iVt, a.bern, F);
+            acc += LAM(j, 0) * F[0].d + LAM(j, 1) * F[1].d + LAM(j, 2) * F[2].d;
+            // the current's terms at point j
+            const D1 R = recomb_d1(m.n, m.p, sd, m.sid);
+            const D1 U = R - ptd(j, PT_G) * gfr;
+            const D1 wn = dexp(-(m.V + sd(m.sid, SEG_OFFN)) * iVt) * sd(m.sid, SEG_IMUN);
+            const D1 wp = dexp((m.V + sd(m.sid, SEG_OFFP)) * iVt) * sd(m.sid, SEG_IMUP);
+            const D1 xj = ptd(j, PT_X), dx = xj - xp;
+            const D1 iu = iup + 0.5 * dx * (Up + U);
+            Snd = Snd + 0.5 * dx * (wnp + wn);
+            Dnd = Dnd + 0.5 * dx * (wnp * iup + wn * iu);
+            Spd = Spd + 0.5 * dx * (wpp + wp);
+            Dpd = Dpd + 0.5 * dx * (wpp * iup + wp * iu);
+            wnp = wn; wpp = wp; Up = U; iup = iu; xp = xj;
+            l = m; m = r;
         }
+        const D1 endnNd = dexp(scd(SC_LNR) - (VLd + sd(sidN, SEG_OFFN)) * iVt);
+        const D1 endpNd = dexp(scd(SC_LPR) + (VLd + sd(sidN, SEG_OFFP)) * iVt);
+        const D1 Knd = (Vt * (endn0d - endnNd) + Dnd) / Snd;
+        const D1 Kpd = (Vt * (endpNd - endp0d) - Dpd) / Spd;
+        const D1 J = cJ * (Knd + Kpd);
+        out[k] = J.d - acc;
     }
 #undef FAC
+#undef STV
 #undef LAM
 #undef IU
 #undef YS
@@ -386,5 +393,7 @@ DDH_FN void adjoint_lane(const AdjArgs &a, int lane, ldouble *lval, ldouble *lta
 
 #undef ASC
 #undef APT
+#undef ASID
+#undef ASEG
 
 }  // namespace ddh

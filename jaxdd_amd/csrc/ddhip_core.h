@@ -429,7 +429,7 @@ DDH_FN CurSeg cur_seg(const Lane &L, int sid) {
 
 // The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108); the interval's
 // mobilities are those of l's segment (the interval after the point).
-DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
+DDH_FN Edge edge_flux_pt(const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
     Edge e;
     const double lcn = Vt * l.mun, lcp = Vt * l.mup;
     double xn = ((r.V + r.offn) - (l.V + l.offn)) * iVt;
@@ -446,6 +446,11 @@ DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double
     e.Fp = e.ap0 + e.ap1;
     e.gp = lcp * (dBpf * l.p + dBpb * r.p) * iVt;
     return e;
+}
+
+DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
+    (void)L;
+    return edge_flux_pt(l, r, Vt, iVt, bern);
 }
 
 // An interior point's loads. A sweep issues them one point ahead of their use, so that their
@@ -557,13 +562,18 @@ DDH_FN void to_t(const double *in, T *out, int m) {
 // (src/jaxdd/solver.py:1021-1187, src/jaxdd/batch.py:518-549). cur, mid: points i-1, i; rV:
 // point i+1's potential; eL, eR: the edges (i-1, i) and (i, i+1); last: i = NP, whose
 // coupling to the contact is not an unknown.
+// The segment fields a row reads: the permittivities of its point and of the point before,
+// and its point's Poisson row scale, continuity row scale and doping.
+struct RowSeg {
+    double meps, ceps, sP, scs, dop;
+};
+
 template <int K>
-DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
-                       const Edge &eR, bool last, double L2, double iVt, double share,
-                       double g_frac, double *A, double *B, double *C, double *r) {
-    lcdouble *sm = Lv.seg + mid.sid * SEG_N * WAVE;
-    const double meps = sm[SEG_EPS * WAVE], ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * WAVE];
-    const double sP = sm[SEG_SP * WAVE], scs = sm[SEG_SCS * WAVE], dop = sm[SEG_DOP * WAVE];
+DDH_FN void row_blocks_s(const RowSeg &g, const Pt &cur, const Pt &mid, double rV,
+                         const Edge &eL, const Edge &eR, bool last, double L2, double iVt,
+                         double share, double g_frac, double *A, double *B, double *C,
+                         double *r) {
+    const double meps = g.meps, ceps = g.ceps, sP = g.sP, scs = g.scs, dop = g.dop;
     const double hm = cur.h, hp = mid.h, hs = hp + hm;
     const double D = 1.0 / (hp * hm * hs * L2);
     const double fac_ = 0.5 * L2 * hp * hm * hs;
@@ -606,6 +616,18 @@ DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, 
         r[0 * K + 1] = -Q_E * mid.an * sP; r[0 * K + 2] = Q_E * mid.ca * sP;
         r[1 * K + 1] = 0.0; r[1 * K + 2] = 0.0; r[2 * K + 1] = 0.0; r[2 * K + 2] = 0.0;
     }
+}
+
+template <int K>
+DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
+                       const Edge &eR, bool last, double L2, double iVt, double share,
+                       double g_frac, double *A, double *B, double *C, double *r) {
+    lcdouble *sm = Lv.seg + mid.sid * SEG_N * WAVE;
+    RowSeg g;
+    g.meps = sm[SEG_EPS * WAVE];
+    g.ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * WAVE];
+    g.sP = sm[SEG_SP * WAVE]; g.scs = sm[SEG_SCS * WAVE]; g.dop = sm[SEG_DOP * WAVE];
+    row_blocks_s<K>(g, cur, mid, rV, eL, eR, last, L2, iVt, share, g_frac, A, B, C, r);
 }
 
 // Elimination sweep (right to left). Returns the ion step z. NOISE: the right-hand side is

@@ -84,15 +84,15 @@ __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
     }
 }
 
-// The gradients of every device's J-V curve: one lane per device; the wavefront's segment
-// tables and the current direction's tangent table in LDS.
+// The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
+// over the devices), one lane per voltage; the device's arrays are the same for all lanes.
 __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
-    extern __shared__ double lds[];  // 2 * a.nseg * SEG_N * WAVE doubles
-    const int lane = blockIdx.x * 64 + threadIdx.x;
-    if (lane >= a.n) return;
-    ldouble *lval = (ldouble *)lds + threadIdx.x;
-    ldouble *ltan = lval + (size_t)a.nseg * SEG_N * WAVE;
-    adjoint_lane(a, lane, lval, ltan);
+    const size_t sl = (size_t)blockIdx.x * 64 + threadIdx.x;
+    for (int d = blockIdx.x; d < a.n; d += gridDim.x) {
+        const bool mobile = a.sc[(size_t)SC_AT * a.n + d] > 0 || a.sc[(size_t)SC_CT * a.n + d] > 0;
+        if (mobile && threadIdx.x == 0) a.status[d] = 1;
+        for (int v = threadIdx.x; v < a.nV; v += 64) adjoint_voltage(a, d, v, sl, mobile);
+    }
 }
 
 }  // namespace ddh
@@ -328,15 +328,21 @@ extern "C" int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32
 }
 
 struct AdjLayout {
-    size_t fac, lam, iu, total;
+    size_t fac, st, lam, iu, total, slots, ls;
 };
 
+// Per launched lane (ADJ_SLOTS workgroups of 64 at most): the factors, the state's V, n, p,
+// lambda and the current's integral, each [rows][ls] lane-innermost.
 static AdjLayout adj_layout(int32_t n, int32_t NP, int32_t nV) {
+    (void)nV;
     AdjLayout w;
+    w.slots = (size_t)(n < ddh::ADJ_SLOTS ? n : ddh::ADJ_SLOTS);
+    w.ls = w.slots * 64;
     size_t off = 0;
-    w.fac = off; off = align256(off + (size_t)NP * ddh::ADJ_F * n * sizeof(double));
-    w.lam = off; off = align256(off + (size_t)nV * NP * 3 * n * sizeof(double));
-    w.iu = off; off = align256(off + (size_t)(NP + 1) * n * sizeof(double));
+    w.fac = off; off = align256(off + (size_t)NP * ddh::ADJ_F * w.ls * sizeof(double));
+    w.st = off; off = align256(off + (size_t)NP * 3 * w.ls * sizeof(double));
+    w.lam = off; off = align256(off + (size_t)NP * 3 * w.ls * sizeof(double));
+    w.iu = off; off = align256(off + (size_t)(NP + 1) * w.ls * sizeof(double));
     w.total = off;
     return w;
 }
@@ -356,7 +362,7 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nV < 1 || ntheta < 1 || b->nseg < 1 ||
         b->nseg > DDH_ADJOINT_MAX_SEGMENTS)
         return fail(DDH_E_SHAPE, "ddh_adjoint: n > 0, 2 <= NP <= 65536, nV >= 1, ntheta >= 1, "
-                                 "1 <= nseg <= 7");
+                                 "1 <= nseg <= 14");
     if (bernoulli != 0 && bernoulli != 1)
         return fail(DDH_E_ARG, "ddh_adjoint: bernoulli must be 0 (exact) or 1 (simss)");
     AdjLayout w = adj_layout(b->n, b->NP, nV);
@@ -368,14 +374,13 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     a.V = V; a.states = states; a.ok = ok; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
     a.grad = grad; a.status = status;
     a.fac = (double *)(base + w.fac);
+    a.st = (double *)(base + w.st);
     a.lam = (double *)(base + w.lam);
     a.iu = (double *)(base + w.iu);
+    a.ls = w.ls;
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid((b->n + 63) / 64), block(64);
-    const size_t lds = 2 * (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
-    (void)hipFuncSetAttribute((const void *)ddh::adjoint_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, lds, s, a);
+    dim3 grid((unsigned)w.slots), block(64);
+    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel launch");
     return DDH_OK;

@@ -148,22 +148,24 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     return 0;
 }
 
-// The adjoint kernel's lanes in turn (each wavefront's LDS tables in host memory).
+// The adjoint kernel's wavefronts in turn: one per device, a lane per voltage.
 extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                            const double *seg, const double *sc, const double *V, int nV,
                            const double *states, const uint8_t *ok, int ntheta,
                            const double *dpt, const double *dseg, const double *dsc, int bern,
                            double *grad, int32_t *status) {
-    std::vector<double> fac((size_t)NP * ADJ_F * n), lam((size_t)nV * NP * 3 * n),
-        iu((size_t)(NP + 1) * n), lds((size_t)2 * nseg * SEG_N * WAVE);
+    const size_t ls = 64;  // one workgroup: the devices in turn, a lane per voltage
+    std::vector<double> fac((size_t)NP * ADJ_F * ls), st((size_t)NP * 3 * ls),
+        lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls);
     AdjArgs a;
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg; a.ntheta = ntheta; a.bern = bern;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
     a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
-    a.fac = fac.data(); a.lam = lam.data(); a.iu = iu.data();
-    for (int lane = 0; lane < n; ++lane) {
-        double *lval = lds.data() + (lane % WAVE);
-        adjoint_lane(a, lane, lval, lval + (size_t)nseg * SEG_N * WAVE);
+    a.fac = fac.data(); a.st = st.data(); a.lam = lam.data(); a.iu = iu.data(); a.ls = ls;
+    for (int d = 0; d < n; ++d) {
+        const bool mobile = sc[(size_t)SC_AT * n + d] > 0 || sc[(size_t)SC_CT * n + d] > 0;
+        if (mobile) status[d] = 1;
+        for (int v = 0; v < nV; ++v) adjoint_voltage(a, d, v, (size_t)(v % 64), mobile);
     }
     return 0;
 }
