@@ -94,15 +94,15 @@ struct Args {
 };
 
 // Mixed precision: the first Newton step of each voltage of the scan (the correction of the
-// extrapolated start) uses FP32 factors on row-equilibrated blocks; the solve's later steps,
-// which decide convergence (src/jaxdd/batch.py:793-818: the quadratic and settled stops assume
-// true Newton), and the start solves use FP64 factors. More FP32 steps (measured: up to 4, or
-// until a step falls below 1e-2) cost more Newton steps than they save where the
-// Jacobian is ill-conditioned (high forward bias), and on this GPU an FP32 step costs what an
-// FP64 one does: the rounded factors save HBM bytes, not issue cycles. The steps that decide
-// convergence (src/jaxdd/batch.py:793-818: the quadratic and settled stops assume true Newton)
-// are then FP64 Newton steps, and an FP32 correction that no longer contracts (an
-// ill-conditioned Jacobian at high forward bias) cannot stall a solve.
+// extrapolated start) factorises in FP32 on row-equilibrated blocks; every other step (the
+// steps that decide convergence, src/jaxdd/batch.py:793-818: the quadratic and settled stops
+// assume true Newton; and the start solves) factorises in FP64. Every step stores its factors
+// c', d' in FP32, and every back-substitution runs in FP64 from them: an inexact Newton whose
+// FP64 residual still drives each solve to the tolerance. On this GPU an FP32 operation costs
+// what an FP64 one does (no packed math), so FP32 arithmetic saves no time; the stored
+// factors are half the sweeps' memory traffic, and the sweeps wait on memory. More FP32
+// factorisations (measured: up to 4 steps, or until a step falls below 1e-2) cost Newton steps
+// where the Jacobian is ill-conditioned (high forward bias).
 constexpr int MIXED_STEPS = 1;
 
 #define SC(f) a.sc[(size_t)(f) * a.n + lane]
@@ -632,7 +632,7 @@ DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, 
 
 // Elimination sweep (right to left). Returns the ion step z. NOISE: the right-hand side is
 // the residual's round-off instead of -F (src/jaxdd/solver.py:1728-1751, `_noise_step`).
-template <typename T, int K, bool NOISE = false>
+template <typename T, typename S, int K, bool NOISE = false>
 DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double *z) {
     // everything the loop reads of the arguments, in registers: the loop's stores to global
     // memory would otherwise keep these generic-pointer loads inside it
@@ -646,7 +646,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
-    DDH_GLOBAL T *fac = (DDH_GLOBAL T *)(factors<T>(a) + lane);
+    DDH_GLOBAL S *fac = (DDH_GLOBAL S *)(factors<S>(a) + lane);
     // a solve's first step reads its start from the source buffers, later steps the iterate
     gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
     gdouble *s2 = (c.fresh && c.ratio != 0.0) ? Lv.ys2 : s1;
@@ -755,11 +755,11 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
                 for (int q = 0; q < K; ++q)
                     dn_[p * K + q] = Mi[3 * p + 0] * rh[0 * K + q] + Mi[3 * p + 1] * rh[1 * K + q] + Mi[3 * p + 2] * rh[2 * K + q];
             }
-            DDH_GLOBAL T *f = fac + (size_t)(i - 1) * NFAC * n;
+            DDH_GLOBAL S *f = fac + (size_t)(i - 1) * NFAC * n;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[q * n] = cn_[q];
+            for (int q = 0; q < 9; ++q) f[q * n] = (S)cn_[q];
 #pragma unroll
-            for (int q = 0; q < 3 * K; ++q) f[(9 + q) * n] = dn_[q];
+            for (int q = 0; q < 3 * K; ++q) f[(9 + q) * n] = (S)dn_[q];
             if constexpr (K > 1) {
                 // gamma_i = W_i - c'_{i+1}^T gamma_{i+1}; W_i only in the potential's row
                 const double hLm = cur.h * L, hLp = mid.h * L;
@@ -800,9 +800,9 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
     visit(NP, false, now, nxt);
     {   // the stores row NP will overwrite: the loop is then entered, as it repeats, with
         // the next point's loads followed by a row's stores
-        DDH_GLOBAL T *f = fac + (size_t)(NP - 1) * NFAC * n;
+        DDH_GLOBAL S *f = fac + (size_t)(NP - 1) * NFAC * n;
 #pragma unroll
-        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = T(0);
+        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = S(0);
     }
     for (int j = NP - 1; j >= 0; --j) {
         now = nxt;
@@ -824,7 +824,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
 
 // Back-substitution sweep (left to right): the step, the new state, its size and current.
 // NOISE: only the step's size, measured as a Newton step's (the state is left as it is).
-template <typename T, int K, bool NOISE = false>
+template <typename T, typename S, int K, bool NOISE = false>
 DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, const double *z,
                              double *out /* size, J, finite */) {
     const SweepCtl c = cref;  // in registers (as in the elimination)
@@ -833,16 +833,16 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
     const double capv = CAP_V * Vt;
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
-    DDH_GLOBAL const T *fac = (DDH_GLOBAL const T *)(factors<T>(a) + lane);
+    DDH_GLOBAL const S *fac = (DDH_GLOBAL const S *)(factors<S>(a) + lane);
     const double z0 = z[0], z1 = z[1];
     // point j's loads, issued one point ahead of their use (as in the elimination)
     struct Next {
-        T f[9 + 3 * K];
+        S f[9 + 3 * K];
         double y0, y1, y2, G, x;
         int sid;
     };
     auto fetch = [&](int j, Next &r) {
-        DDH_GLOBAL const T *f = fac + (size_t)(j - 1) * NFAC * n;
+        DDH_GLOBAL const S *f = fac + (size_t)(j - 1) * NFAC * n;
 #pragma unroll
         for (int q = 0; q < 9 + 3 * K; ++q) r.f[q] = f[q * n];
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
@@ -869,7 +869,7 @@ DDH_SWEEP void backsub_sweep(const Args &a, int lane, const SweepCtl &cref, cons
     auto visit = [&](const int j, const Next &now, Next &nxt) {
         fetch(j < NP ? j + 1 : NP, nxt);  // the same loads every visit (as in the elimination)
         const int sid = now.sid;
-        const T *f = now.f;
+        const S *f = now.f;
         double dF[3];
 #pragma unroll
         for (int row = 0; row < 3; ++row) {
@@ -1226,6 +1226,16 @@ DDH_FN void finish_step(const Args &a, int lane, LaneState &S, const double z[2]
     after_step(a, lane, S, nmax(size, nmax(fabs(dl0), fabs(dl1))), J);
 }
 
+// One Newton step's two sweeps: T the factorisation's arithmetic, S the stored factors' type.
+template <typename T, typename S>
+DDH_FN void step_sweeps(const Args &a, int lane, bool ions, SweepCtl &c, double *z, double *out) {
+    if (ions) elim_sweep<T, S, 3>(a, lane, c, z);
+    else elim_sweep<T, S, 1>(a, lane, c, z);
+    c.fresh = 0;
+    if (ions) backsub_sweep<T, S, 3>(a, lane, c, z, out);
+    else backsub_sweep<T, S, 1>(a, lane, c, z, out);
+}
+
 template <typename T>
 DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg, LaneState &S) {
     SweepCtl c = sweep_ctl(a, lane, S, lseg);
@@ -1233,8 +1243,8 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg,
     if (S.noise_state == 1) {
         // the noise floor of the state the last step left (src/jaxdd/solver.py:1728-1751):
         // one FP64 solve without the ions' border, then the stop the last step deferred
-        elim_sweep<double, 1, true>(a, lane, c, z);
-        backsub_sweep<double, 1, true>(a, lane, c, z, out);
+        elim_sweep<double, double, 1, true>(a, lane, c, z);
+        backsub_sweep<double, double, 1, true>(a, lane, c, z, out);
         S.noise_floor = out[0];
         S.noise_state = 2;
         double hmax = 0.0;
@@ -1243,11 +1253,12 @@ DDH_FN void newton_lane_step(const Args &a, int lane, bool ions, lcdouble *lseg,
         else if (S.it >= S.maxit) end_solve(a, lane, S, 0, S.Jh[3]);
         return;
     }
-    if (ions) elim_sweep<T, 3>(a, lane, c, z);
-    else elim_sweep<T, 1>(a, lane, c, z);
-    c.fresh = 0;
-    if (ions) backsub_sweep<T, 3>(a, lane, c, z, out);
-    else backsub_sweep<T, 1>(a, lane, c, z, out);
+    if constexpr (sizeof(T) == 4) {
+        step_sweeps<float, float>(a, lane, ions, c, z, out);
+    } else {
+        if (a.mixed) step_sweeps<double, float>(a, lane, ions, c, z, out);
+        else step_sweeps<double, double>(a, lane, ions, c, z, out);
+    }
     finish_step(a, lane, S, z, out[0], out[1], out[2] != 0.0);
 }
 

@@ -46,7 +46,7 @@ struct PairBack {
 #endif
 
 // The elimination of one half (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination).
-template <typename T, int K>
+template <typename T, typename S, int K>
 DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cref, PairElim &out) {
     const SweepCtl c = cref;
     const int NP = DDH_UNIFORM(a.NP), bern = DDH_UNIFORM(a.bern);
@@ -60,7 +60,7 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
     const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
-    DDH_GLOBAL T *fac = (DDH_GLOBAL T *)(factors<T>(a) + lane);
+    DDH_GLOBAL S *fac = (DDH_GLOBAL S *)(factors<S>(a) + lane);
     gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
     gdouble *s2 = (c.fresh && c.ratio != 0.0) ? Lv.ys2 : s1;
     const Contact ct = contact_point(a, lane, Lv, c.Vint, half == 1);
@@ -163,11 +163,11 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
                 for (int q = 0; q < K; ++q)
                     dn_[p * K + q] = Mi[3 * p + 0] * rh[0 * K + q] + Mi[3 * p + 1] * rh[1 * K + q] + Mi[3 * p + 2] * rh[2 * K + q];
             }
-            DDH_GLOBAL T *f = fac + (size_t)(i - 1) * NFAC * n;
+            DDH_GLOBAL S *f = fac + (size_t)(i - 1) * NFAC * n;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[q * n] = cn_[q];
+            for (int q = 0; q < 9; ++q) f[q * n] = (S)cn_[q];
 #pragma unroll
-            for (int q = 0; q < 3 * K; ++q) f[(9 + q) * n] = dn_[q];
+            for (int q = 0; q < 3 * K; ++q) f[(9 + q) * n] = (S)dn_[q];
             if constexpr (K > 1) {
                 // gamma_i = W_i - p'^T gamma_prev, p' the previous row's factor
                 const double hLm = lft.h * L, hLp = mid.h * L;
@@ -205,9 +205,9 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
     };
     visit(1, false);
     {
-        DDH_GLOBAL T *f = fac + (size_t)(half ? NP - 1 : 0) * NFAC * n;
+        DDH_GLOBAL S *f = fac + (size_t)(half ? NP - 1 : 0) * NFAC * n;
 #pragma unroll
-        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = T(0);
+        for (int q = 0; q < 9 + 3 * K; ++q) f[q * n] = S(0);
     }
     for (int v = 2; v < nvis; ++v) visit(v, true);
 #pragma unroll
@@ -293,7 +293,7 @@ DDH_FN void pair_middle(const Args &a, int lane, const SweepCtl &c, int K, const
 
 // The back-substitution of one half, outwards from the middle: the step (x of the middle
 // point given), its caps, the new state, the step size, the current's parts.
-template <typename T, int K>
+template <typename T, typename S, int K>
 DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &cref,
                              const double *zin, const double *xfirst, PairBack &out) {
     const SweepCtl c = cref;
@@ -306,15 +306,15 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
     const double capv = CAP_V * Vt;
     const Lane Lv = lane_view(a, lane, c);
     const size_t n = Lv.n;
-    DDH_GLOBAL const T *fac = (DDH_GLOBAL const T *)(factors<T>(a) + lane);
+    DDH_GLOBAL const S *fac = (DDH_GLOBAL const S *)(factors<S>(a) + lane);
     const double z0 = zin[0], z1 = zin[1];
     struct Next {
-        T f[9 + 3 * K];
+        S f[9 + 3 * K];
         double y0, y1, y2, G, x;
         int sid;
     };
     auto fetch = [&](int j, Next &r) {
-        DDH_GLOBAL const T *f = fac + (size_t)(j - 1) * NFAC * n;
+        DDH_GLOBAL const S *f = fac + (size_t)(j - 1) * NFAC * n;
 #pragma unroll
         for (int q = 0; q < 9 + 3 * K; ++q) r.f[q] = f[q * n];
         gdouble *y = Lv.yw + (size_t)(j - 1) * 3 * n;
@@ -338,7 +338,7 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
         now = nxt;
         fetch(v + 1 < cnt ? j + dir : j, nxt);
         const int sid = now.sid;
-        const T *f = now.f;
+        const S *f = now.f;
         double x0 = xs0, x1 = xs1, x2 = xs2;
         if (v > 0) {
             double dF[3];
@@ -463,8 +463,16 @@ template <typename T>
 DDH_FN void pair_phase1(const Args &a, int lane, int half, bool ions, lcdouble *lseg,
                         const LaneState &S, PairStep &st) {
     st.c = sweep_ctl(a, lane, S, lseg);
-    if (ions) elim_half<T, 3>(a, lane, half, st.c, st.e);
-    else elim_half<T, 1>(a, lane, half, st.c, st.e);
+    if constexpr (sizeof(T) == 4) {
+        if (ions) elim_half<float, float, 3>(a, lane, half, st.c, st.e);
+        else elim_half<float, float, 1>(a, lane, half, st.c, st.e);
+    } else if (a.mixed) {
+        if (ions) elim_half<double, float, 3>(a, lane, half, st.c, st.e);
+        else elim_half<double, float, 1>(a, lane, half, st.c, st.e);
+    } else {
+        if (ions) elim_half<double, double, 3>(a, lane, half, st.c, st.e);
+        else elim_half<double, double, 1>(a, lane, half, st.c, st.e);
+    }
 }
 
 template <typename T>
@@ -474,8 +482,17 @@ DDH_FN void pair_phase2(const Args &a, int lane, int half, bool ions, PairStep &
     const PairElim &Lh = half ? other : st.e, &Rh = half ? st.e : other;
     pair_middle(a, lane, st.c, ions ? 3 : 1, Lh, Rh, st.z, xm, xm1);
     st.c.fresh = 0;
-    if (ions) backsub_half<T, 3>(a, lane, half, st.c, st.z, half ? xm1 : xm, st.b);
-    else backsub_half<T, 1>(a, lane, half, st.c, st.z, half ? xm1 : xm, st.b);
+    const double *x0 = half ? xm1 : xm;
+    if constexpr (sizeof(T) == 4) {
+        if (ions) backsub_half<float, float, 3>(a, lane, half, st.c, st.z, x0, st.b);
+        else backsub_half<float, float, 1>(a, lane, half, st.c, st.z, x0, st.b);
+    } else if (a.mixed) {
+        if (ions) backsub_half<double, float, 3>(a, lane, half, st.c, st.z, x0, st.b);
+        else backsub_half<double, float, 1>(a, lane, half, st.c, st.z, x0, st.b);
+    } else {
+        if (ions) backsub_half<double, double, 3>(a, lane, half, st.c, st.z, x0, st.b);
+        else backsub_half<double, double, 1>(a, lane, half, st.c, st.z, x0, st.b);
+    }
 }
 
 DDH_FN void pair_phase3(const Args &a, int lane, int half, LaneState &S, PairStep &st,
