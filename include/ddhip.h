@@ -116,6 +116,20 @@ int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, const void *A, 
                     const void *C, void *d_inout, int32_t precision, void *workspace,
                     void *hip_stream);
 
+/* The assembly operator at a given state: replaces src/jaxdd/solver.py:1021-1187
+ * (`_assemble`, Boltzmann, no optional modules) with the ion slots' columns of
+ * src/jaxdd/batch.py:518-549 (`_system`), and the integral current of src/jaxdd/batch.py:302-403
+ * (`_current_parts`), for every device of the batch. y: [NP][3][n] the interior points'
+ * (V, ln n, ln p); Vint: [n] the internal voltage (the contacts' potentials follow from it);
+ * logs: [2][n] the anion and cation normalisations, or NULL for no mobile ions. Outputs
+ * lane-innermost: F [NP][3][n] the row-scaled residual, A, B, C [NP][9][n] its Jacobian blocks
+ * (row-major 3x3; the first row's A and the last row's C couple to the contacts, as the
+ * reference returns them), U [NP][6][n] dF/d(logs) as [row][slot], J [n] the current without
+ * the shunt. bernoulli: as ddh_opts. */
+int ddh_assemble(const ddh_batch *b, const double *y, const double *Vint, const double *logs,
+                 int32_t bernoulli, double *F, double *A, double *B, double *C, double *U,
+                 double *J, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -231,6 +231,52 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     return J + batch["G_shunt"][:, None] * Vt[None, :], ok, its
 
 
+def assemble(batch, y, Vint, logs=None, bernoulli_name="exact"):
+    """The residual, Jacobian blocks, ion columns and current of every device at a state
+    (reference: src/jaxdd/solver.py:1021-1187 `_assemble` per device, with
+    src/jaxdd/batch.py:518-549 `_system`'s ion columns and src/jaxdd/batch.py:302-403's
+    current). ``y``: (n, NP, 3) interior (V, ln n, ln p); ``Vint``: (n,) internal voltages;
+    ``logs``: (n, 2) ion normalisations or None. Returns a dict of tensors on the batch's
+    GPU: F (n, NP, 3) row-scaled residual, A, B, C (n, NP, 3, 3), U (n, NP, 3, 2),
+    J (n,) current with the shunt term (``G_shunt * Vint``) added, as solve_batch's."""
+    if bernoulli_name not in ("exact", "simss"):
+        raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
+    lib = _lib.load()
+    dev = batch["_pt"].device
+    if dev.type != "cuda":
+        raise RuntimeError("assemble runs on the GPU: prepare(devices, device='cuda')")
+    n, NP = batch["n"], batch["NP"]
+    yt = torch.as_tensor(y, dtype=torch.float64, device=dev)
+    if tuple(yt.shape) != (n, NP, 3):
+        raise ValueError(f"y: shape (n, NP, 3) = {(n, NP, 3)}, got {tuple(yt.shape)}")
+    yl = yt.permute(1, 2, 0).contiguous()  # (NP, 3, n): lane-innermost
+    Vd = torch.as_tensor(Vint, dtype=torch.float64, device=dev).reshape(n).contiguous()
+    lg = None
+    if logs is not None:
+        lg = torch.as_tensor(logs, dtype=torch.float64, device=dev).reshape(n, 2).T.contiguous()
+    sc = _sc_now(batch)
+    F = torch.empty((NP, 3, n), dtype=torch.float64, device=dev)
+    A, B, C = (torch.empty((NP, 9, n), dtype=torch.float64, device=dev) for _ in range(3))
+    U = torch.empty((NP, 6, n), dtype=torch.float64, device=dev)
+    J = torch.empty(n, dtype=torch.float64, device=dev)
+    b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
+                   batch["_seg"].data_ptr(), sc.data_ptr())
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib.ddh_assemble(ctypes.byref(b), yl.data_ptr(), Vd.data_ptr(),
+                          lg.data_ptr() if lg is not None else None,
+                          0 if bernoulli_name == "exact" else 1, F.data_ptr(), A.data_ptr(),
+                          B.data_ptr(), C.data_ptr(), U.data_ptr(), J.data_ptr(), stream)
+    _lib.check(rc, "ddh_assemble")
+    torch.cuda.current_stream(dev).synchronize()
+    return {
+        "F": F.permute(2, 0, 1), "A": A.reshape(NP, 3, 3, n).permute(3, 0, 1, 2),
+        "B": B.reshape(NP, 3, 3, n).permute(3, 0, 1, 2),
+        "C": C.reshape(NP, 3, 3, n).permute(3, 0, 1, 2),
+        "U": U.reshape(NP, 3, 2, n).permute(3, 0, 1, 2),
+        "J": J + batch["G_shunt"] * Vd,
+    }
+
+
 def external_voltages(batch, voltages, J):
     """``V_ext = V_int + J * R_series`` (reference: src/jaxdd/batch.py:1104-1108)."""
     Vt = torch.as_tensor(np.asarray(voltages, dtype=float), device=J.device)

@@ -9,6 +9,7 @@
 
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
+#include "ddhip_assemble.h"
 #include "ddhip_pair.h"
 
 namespace ddh {
@@ -82,6 +83,12 @@ __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
             pair_phase3(a, lane, half, S, st, ob);
         }
     }
+}
+
+// The assembly operator: one lane per device.
+__global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
+    const int d = blockIdx.x * 64 + threadIdx.x;
+    if (d < a.n) assemble_lane(a, d);
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
@@ -383,6 +390,27 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" int ddh_assemble(const ddh_batch *b, const double *y, const double *Vint,
+                            const double *logs, int32_t bernoulli, double *F, double *A,
+                            double *B, double *C, double *U, double *J, void *stream) {
+    if (!b || !y || !Vint || !F || !A || !B || !C || !U || !J)
+        return fail(DDH_E_ARG, "ddh_assemble: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_assemble: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14");
+    if (bernoulli != 0 && bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_assemble: bernoulli must be 0 (exact) or 1 (simss)");
+    ddh::AsmArgs a;
+    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
+    a.y = y; a.Vint = Vint; a.logs = logs;
+    a.F = F; a.A = A; a.B = B; a.C = C; a.U = U; a.J = J;
+    dim3 grid((b->n + 63) / 64), block(64);
+    hipLaunchKernelGGL(ddh::assemble_kernel, grid, block, 0, (hipStream_t)stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "assemble_kernel launch");
     return DDH_OK;
 }
 

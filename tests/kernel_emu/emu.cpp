@@ -7,6 +7,7 @@
 #define DDH_CPU_EMULATION 1
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
+#include "ddhip_assemble.h"
 #include "ddhip_pair.h"
 
 #include <cstdlib>
@@ -167,5 +168,18 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
         if (mobile) status[d] = 1;
         for (int v = 0; v < nV; ++v) adjoint_voltage(a, d, v, (size_t)(v % 64), mobile);
     }
+    return 0;
+}
+
+// The assembly operator's lanes in turn.
+extern "C" int emu_assemble(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
+                            const double *seg, const double *sc, const double *y,
+                            const double *Vint, const double *logs, int bern, double *F,
+                            double *A, double *B, double *C, double *U, double *J) {
+    AsmArgs a;
+    a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern;
+    a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.logs = logs;
+    a.F = F; a.A = A; a.B = B; a.C = C; a.U = U; a.J = J;
+    for (int d = 0; d < n; ++d) assemble_lane(a, d);
     return 0;
 }

@@ -34,6 +34,8 @@ def lib():
         _lib.emu_set_lanes.argtypes = [i]
         _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, i, vp, vp, vp, i,
                                      vp, vp]
+        _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
+                                      vp, vp]
     return _lib
 
 
@@ -83,3 +85,23 @@ def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact"):
                       _p(np.ascontiguousarray(states)), _p(okb), K, _p(dpt), _p(dseg), _p(dsc),
                       0 if bernoulli == "exact" else 1, _p(grad), _p(status))
     return grad, status
+
+
+def assemble_packed(packed, y, Vint, logs=None, bernoulli="exact"):
+    """The assembly operator's CPU replay: y (n, NP, 3), Vint (n,), logs (n, 2) or None ->
+    F (n, NP, 3), A, B, C (n, NP, 3, 3), U (n, NP, 3, 2), J (n,) without the shunt."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    yl = np.ascontiguousarray(np.asarray(y, dtype=float).transpose(1, 2, 0))
+    Vd = np.ascontiguousarray(Vint, dtype=float)
+    lg = None if logs is None else np.ascontiguousarray(np.asarray(logs, dtype=float).T)
+    F = np.zeros((NP, 3, n)); A = np.zeros((NP, 9, n)); B = np.zeros((NP, 9, n))
+    C = np.zeros((NP, 9, n)); U = np.zeros((NP, 6, n)); J = np.zeros(n)
+    lib().emu_assemble(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(yl), _p(Vd),
+                       _p(lg) if lg is not None else None,
+                       0 if bernoulli == "exact" else 1, _p(F), _p(A), _p(B), _p(C), _p(U),
+                       _p(J))
+    return (F.transpose(2, 0, 1), A.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
+            B.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2), C.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
+            U.reshape(NP, 3, 2, n).transpose(3, 0, 1, 2), J)

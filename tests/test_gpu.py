@@ -30,6 +30,43 @@ def torch_cuda():
     return torch
 
 
+# ---- the assembly operator (reference test pattern test_solver.py:1578-1617) --------------
+@pytest.mark.parametrize("case", ["fixtures", "c3"])
+def test_assemble_matches_oracle(torch_cuda, oracle, emu, case):
+    """ddh_assemble against the oracle's `_assemble` and current at the same states, and
+    against its own CPU replay."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import assemble, pack, prepare
+    from test_kernel_emu import _asm_state
+
+    rng = np.random.default_rng(11)
+    if case == "fixtures":
+        devs = [fixture_device(c) for c in ("thin_l1_light", "frozen_1e17_x1_thin_light")]
+        logs, bern = None, "simss"
+    else:
+        devs = S.devices("c3", 5)
+        logs = np.array([[-30.0, -31.0], [-29.0, -32.5], [-33.0, -28.0], [-31.0, -30.0],
+                         [-30.5, -29.5]])
+        bern = "exact"
+    Vint = np.linspace(-0.2, 1.2, len(devs))
+    y, fulls = _asm_state(devs, Vint, rng)
+    out = {k: _np(v) for k, v in assemble(prepare(devs, "cuda"), y, Vint, logs, bern).items()}
+    rep = emu.assemble_packed(pack(devs), y, Vint, logs, bern)
+    shunt = np.array([1 / d.R_shunt if d.R_shunt > 0 else 0.0 for d in devs])
+    for k, dev in enumerate(devs):
+        la, lc = (0.0, 0.0) if logs is None else logs[k]
+        Fo, Ao, Bo, Co, Uo = oracle.assemble(dev, fulls[k], la=la, lc=lc, bernoulli_name=bern)
+        rs = np.max(np.abs(np.concatenate([Ao, Bo, Co], axis=2)), axis=2) + 1e-300
+        assert np.all(np.abs(out["F"][k] - Fo) <= 1e-11 * (rs + np.abs(Fo)))
+        for key, ref in (("A", Ao), ("B", Bo), ("C", Co)):
+            assert np.all(np.abs(out[key][k] - ref) <= 1e-12 * rs[:, :, None]), key
+        assert np.all(np.abs(out["U"][k] - Uo) <= 1e-12 * (np.abs(Uo) + rs[:, :, None]))
+        Jo = oracle.current(dev, fulls[k]) + shunt[k] * Vint[k]
+        assert abs(out["J"][k] - Jo) <= 1e-10 * (abs(Jo) + oracle.j_scale(dev))
+        # the GPU and its CPU replay: the same arithmetic up to FMA contraction
+        assert np.all(np.abs(out["A"][k] - rep[1][k]) <= 1e-13 * rs[:, :, None])
+
+
 # ---- the block sweep operator (reference tests test_solver.py:1620-1641, 1753-1773) ------
 @pytest.mark.parametrize("b,k", [(3, 1), (3, 3), (5, 4), (3, 8)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])

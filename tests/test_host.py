@@ -194,3 +194,7 @@ def test_library_rejects_bad_shapes_before_any_launch():
     o = _lib.Opts(1e-10, 60, 60, 0, 0, 1, 0)
     rc = lib.ddh_solve_batch(ctypes.byref(b), 1, 61, ctypes.byref(o), 1, 1, 1, 1, 1, 1 << 40, None)
     assert rc == -1 and b"nseg" in lib.ddh_last_error()
+    rc = lib.ddh_assemble(ctypes.byref(b), 1, 1, None, 0, 1, 1, 1, 1, 1, 1, None)
+    assert rc == -1 and b"nseg" in lib.ddh_last_error()
+    rc = lib.ddh_assemble(None, None, None, None, 0, None, None, None, None, None, None, None)
+    assert rc == -4

@@ -119,3 +119,51 @@ def test_replay_noise_floor_stop_is_the_oracles(emu, oracle):
         assert np.array_equal(ok[0], oko) and np.array_equal(its[0], itso)
         shunt = 1.0 / dev.R_shunt if dev.R_shunt > 0 else 0.0
         assert np.max(np.abs(J[0] + shunt * V - Jo)) < 1e-10 * curve_scale(dev, Jo)
+
+
+def _asm_state(devs, Vint, rng):
+    """Interior states (n, NP, 3): V and the log densities linear between the contacts plus
+    noise, and each device's full state with its contacts (the oracle's input)."""
+    ys, fulls = [], []
+    for dev, v in zip(devs, Vint):
+        NP = len(dev.x) - 2
+        lo = np.array([dev.V0 + 0.5 * dev.left_elec * v, np.log(dev.n_left), np.log(dev.p_left)])
+        hi = np.array([dev.VL - 0.5 * dev.left_elec * v, np.log(dev.n_right), np.log(dev.p_right)])
+        t = (dev.x / dev.x[-1])[:, None]
+        full = lo + (hi - lo) * t
+        full[1:-1] += rng.normal(scale=[0.02, 0.5, 0.5], size=(NP, 3))
+        ys.append(full[1:-1])
+        fulls.append(full)
+    return np.array(ys), fulls
+
+
+@pytest.mark.parametrize("case", ["fixtures", "c3"])
+def test_replay_assemble_matches_oracle(emu, oracle, case):
+    """The assembly operator (ddhip_assemble.h) against the oracle's `_assemble` restatement
+    and its current at the same states: residual, blocks, ion columns and J."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    rng = np.random.default_rng(7)
+    if case == "fixtures":
+        devs = [fixture_device(c) for c in ("thin_l1_light", "frozen_1e17_x1_thin_light")]
+        logs = None
+        bern = "simss"
+    else:
+        devs = S.devices("c3", 3)
+        logs = np.array([[-30.0, -31.0], [-29.0, -32.5], [-33.0, -28.0]])
+        bern = "exact"
+    Vint = np.array([0.3, 0.9, -0.1][: len(devs)])
+    y, fulls = _asm_state(devs, Vint, rng)
+    F, A, B, C, U, J = emu.assemble_packed(pack(devs), y, Vint, logs, bern)
+    for k, dev in enumerate(devs):
+        la, lc = (0.0, 0.0) if logs is None else logs[k]
+        Fo, Ao, Bo, Co, Uo = oracle.assemble(dev, fulls[k], la=la, lc=lc, bernoulli_name=bern)
+        # entries relative to their row's scale (the rows are scaled residuals)
+        rs = np.max(np.abs(np.concatenate([Ao, Bo, Co], axis=2)), axis=2) + 1e-300
+        assert np.all(np.abs(F[k] - Fo) <= 1e-11 * (rs + np.abs(Fo))), case
+        for got, ref in ((A[k], Ao), (B[k], Bo), (C[k], Co)):
+            assert np.all(np.abs(got - ref) <= 1e-12 * rs[:, :, None]), case
+        assert np.all(np.abs(U[k] - Uo) <= 1e-12 * (np.abs(Uo) + rs[:, :, None]))
+        Jo = oracle.current(dev, fulls[k])  # the ions do not enter the current
+        assert abs(J[k] - Jo) <= 1e-10 * (abs(Jo) + oracle.j_scale(dev)), (J[k], Jo)
