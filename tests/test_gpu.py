@@ -329,6 +329,49 @@ def test_adjoint_is_the_finite_difference_of_the_gpu_curves(c4_gpu):
     assert abs(by_adjoint - fd) < 1e-5 * abs(fd)
 
 
+def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
+    """The gradient workload's check at its own size (200 devices): per device, the adjoint's
+    derivative along a random direction in the 16 parameters (one tangent direction) against
+    central differences of the GPU's FP64 curves, every voltage where all three solves
+    converged, measured against the curve's current scale."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import j_scale, pack, prepare, solve_batch
+
+    n = 200
+    th = A.theta_from_table(S.draw(n), range(n))
+    d = np.random.default_rng(5).normal(size=th.shape)
+
+    def theta_at(a):
+        x = th * (1.0 + a * d)
+        x[:, 15] = th[:, 15] + a * d[:, 15] * 0.01  # the work-function offset: 10 mV units
+        return x
+
+    devs0 = [A.device_from_theta(t) for t in theta_at(0.0)]
+    base = pack(devs0)
+    e = A.REL_STEP
+    hi, lo = (pack([A.device_from_theta(t) for t in theta_at(s)]) for s in (e, -e))
+    for side in (hi, lo):
+        assert side["nseg"] == base["nseg"] and np.array_equal(side["sid"], base["sid"])
+    tan = {k: ((hi[k] - lo[k]) / (2 * e))[None] for k in ("pt", "seg", "sc")}
+    V = S.VOLTAGES[::3].copy()
+    J, ok, _, grad = A.jv_gradients(base, tan, V, tol=1e-12, strict=True, max_iter=200)
+    grad, ok = _np(grad)[:, :, 0], _np(ok)
+
+    h = 1e-5
+    curves = []
+    for s_ in (h, -h):
+        Jx, okx, _ = solve_batch(prepare([A.device_from_theta(t) for t in theta_at(s_)]), V,
+                                 tol=1e-12, strict=True, max_iter=200)
+        curves.append((_np(Jx), _np(okx)))
+    fd = (curves[0][0] - curves[1][0]) / (2 * h)
+    both = ok & curves[0][1] & curves[1][1]
+    assert both.mean() > 0.95
+    scale = np.array([j_scale(dv) for dv in devs0]) + np.max(np.abs(_np(J)), axis=1)
+    err = np.max(np.where(both, np.abs(grad - fd), 0.0), axis=1) / scale
+    assert np.max(err) < 1e-5, (np.argmax(err), np.max(err))
+
+
 def test_autograd_function_backward_is_the_adjoint(c4_gpu):
     import torch
 
