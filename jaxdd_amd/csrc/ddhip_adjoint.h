@@ -79,16 +79,33 @@ struct PD {
     int sid;
 };
 
-// The device's segment field with its tangent along the current direction (global memory;
-// the same device for the whole wavefront).
+// The device's segment field with its tangent along the current direction, from the
+// wavefront's table in LDS (one device per wavefront: every lane reads the same entry).
 struct SegD {
-    const double *val, *tan;
-    size_t n;
+    lcdouble *val, *tan;
     DDH_FN D1 operator()(int id, int f) const {
-        const size_t o = ((size_t)id * SEG_N + f) * n;
+        const int o = id * SEG_N + f;
         return {val[o], tan[o]};
     }
 };
+
+// The wavefront's LDS table: the device's segment fields and their tangents along the current
+// direction, its scalars and their tangents, its points' (h, x, G, fixed ions) and their
+// tangents, and its points' segment ids.
+struct AdjTab {
+    int S, sct, ptv, ptt, sid, size;
+    DDH_FN AdjTab(int nseg, int NP) {
+        S = nseg * SEG_N;
+        sct = 2 * S;
+        ptv = sct + 2 * SC_N;
+        ptt = ptv + (NP + 2) * PT_N;
+        sid = ptt + (NP + 2) * PT_N;
+        size = sid + NP + 2;
+    }
+};
+DDH_MFN int adj_table_size(int nseg, int NP) {
+    return 2 * nseg * SEG_N + 2 * SC_N + 2 * (NP + 2) * PT_N + NP + 2;
+}
 
 DDH_FN D1 recomb_d1(D1 n, D1 p, const SegD &sd, int id) {
     const D1 np_ = n * p;
@@ -150,57 +167,70 @@ DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, d
 
 DDH_FN void inverse3d(const double *m, double *out) { inverse3<double>(m, out); }
 
-// Device d's values (the same for every lane of the wavefront).
-#define ASC(f) a.sc[(size_t)(f) * a.n + d]
-#define APT(j, f) a.pt[((size_t)(j) * PT_N + (f)) * a.n + d]
-#define ASID(j) ((int)a.sid[(size_t)(j) * a.n + d])
-#define ASEG(id, f) a.seg[((size_t)(id) * SEG_N + (f)) * a.n + d]
 
-// The gradients of device d at voltage v (every direction), by the launched lane sl. A device
-// with mobile ions gets NaN, an unconverged voltage zeros.
-DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobile) {
-    const int NP = a.NP, nV = a.nV, K = a.ntheta;
-    const size_t n = (size_t)a.n, ls = a.ls;
-    double *out = a.grad + ((size_t)d * nV + v) * K;
-    if (mobile) {
-        for (int k = 0; k < K; ++k) out[k] = NAN;
-        return;
-    }
-    if (!a.ok[(size_t)d * nV + v]) {
-        for (int k = 0; k < K; ++k) out[k] = 0.0;
-        return;
-    }
-    const double Vt = ASC(SC_VT), iVt = 1.0 / Vt, L = ASC(SC_L), L2 = L * L;
-    const double gfr = ASC(SC_GFRAC), elec = ASC(SC_ELEC), cJ = -elec * Q_E;
-    const int sid0 = ASID(0), sidN = ASID(NP + 1);
+// Copies device d's values into the LDS table (lane `lane` of `nl`: the wavefront's lanes
+// together; the caller waits for all of them before the table is read).
+DDH_FN void adj_stage_values(const AdjArgs &a, int d, ldouble *tab, int lane, int nl) {
+    const size_t n = (size_t)a.n;
+    const AdjTab t(a.nseg, a.NP);
+    for (int q = lane; q < t.S; q += nl) tab[q] = a.seg[(size_t)q * n + d];
+    for (int q = lane; q < SC_N; q += nl) tab[t.sct + q] = a.sc[(size_t)q * n + d];
+    for (int q = lane; q < (a.NP + 2) * PT_N; q += nl) tab[t.ptv + q] = a.pt[(size_t)q * n + d];
+    for (int q = lane; q < a.NP + 2; q += nl) tab[t.sid + q] = (double)a.sid[(size_t)q * n + d];
+}
+
+// Copies direction k's tangents of device d into the LDS table (as adj_stage_values).
+DDH_FN void adj_stage_direction(const AdjArgs &a, int d, int k, ldouble *tab, int lane, int nl) {
+    const size_t n = (size_t)a.n;
+    const AdjTab t(a.nseg, a.NP);
+    const double *dsg = a.dseg + (size_t)k * t.S * n + d;
+    const double *dsc = a.dsc + (size_t)k * SC_N * n + d;
+    const double *dpt = a.dpt + (size_t)k * (a.NP + 2) * PT_N * n + d;
+    for (int q = lane; q < t.S; q += nl) tab[t.S + q] = dsg[(size_t)q * n];
+    for (int q = lane; q < SC_N; q += nl) tab[t.sct + SC_N + q] = dsc[(size_t)q * n];
+    for (int q = lane; q < (a.NP + 2) * PT_N; q += nl) tab[t.ptt + q] = dpt[(size_t)q * n];
+}
+
+#define TSC(f) tab[t.sct + (f)]
+#define TPT(j, f) tab[t.ptv + (j) * PT_N + (f)]
+#define TSID(j) ((int)tab[t.sid + (j)])
+#define TSEG(id, f) tab[(id) * SEG_N + (f)]
 #define FAC(i, q) a.fac[((size_t)((i) - 1) * ADJ_F + (q)) * ls + sl]
 #define STV(j, f) a.st[((size_t)((j) - 1) * 3 + (f)) * ls + sl]
 #define LAM(i, f) a.lam[((size_t)((i) - 1) * 3 + (f)) * ls + sl]
 #define IU(j) a.iu[(size_t)(j) * ls + sl]
 #define YS(j, f) a.states[(((size_t)v * NP + ((j) - 1)) * 3 + (f)) * n + d]
 
-    // ---- lambda ------------------------------------------------------------------------
+// Device d at converged voltage v, by the launched lane sl: lambda (T^T lambda = dJ/dy) and
+// the state's V, n, p, into the lane's workspace.
+DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
+    const AdjTab t(a.nseg, a.NP);
+    const int NP = a.NP;
+    const size_t n = (size_t)a.n, ls = a.ls;
+    const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), L2 = L * L;
+    const double gfr = TSC(SC_GFRAC), elec = TSC(SC_ELEC), cJ = -elec * Q_E;
+    const int sid0 = TSID(0), sidN = TSID(NP + 1);
     const double Vint = a.V[v];
-    const double V0c = ASC(SC_V0) + 0.5 * elec * Vint, VLc = ASC(SC_VL) - 0.5 * elec * Vint;
+    const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
     // the current's sums (the Newton sweep's order), the state's densities kept for the passes
-    const double Vg0n = V0c + ASEG(sid0, SEG_OFFN), Vg0p = V0c + ASEG(sid0, SEG_OFFP);
-    double wn_prev = exp(-Vg0n * iVt) * ASEG(sid0, SEG_IMUN);
-    double wp_prev = exp(Vg0p * iVt) * ASEG(sid0, SEG_IMUP);
-    const double endn0 = exp(ASC(SC_LNL) - Vg0n * iVt), endp0 = exp(ASC(SC_LPL) + Vg0p * iVt);
-    double U_prev = 0.0, iu_prev = 0.0, x_prev = APT(0, PT_X);
+    const double Vg0n = V0c + TSEG(sid0, SEG_OFFN), Vg0p = V0c + TSEG(sid0, SEG_OFFP);
+    double wn_prev = exp(-Vg0n * iVt) * TSEG(sid0, SEG_IMUN);
+    double wp_prev = exp(Vg0p * iVt) * TSEG(sid0, SEG_IMUP);
+    const double endn0 = exp(TSC(SC_LNL) - Vg0n * iVt), endp0 = exp(TSC(SC_LPL) + Vg0p * iVt);
+    double U_prev = 0.0, iu_prev = 0.0, x_prev = TPT(0, PT_X);
     double Sn = 0.0, Dn = 0.0, Sp = 0.0, Dp = 0.0;
     IU(0) = 0.0;
     for (int j = 1; j <= NP; ++j) {
-        const int id = ASID(j);
+        const int id = TSID(j);
         const double y0 = YS(j, 0), nn = exp(YS(j, 1)), pp = exp(YS(j, 2));
         STV(j, 0) = y0; STV(j, 1) = nn; STV(j, 2) = pp;
-        const double R = recombination_value(nn, pp, ASEG(id, SEG_KD), ASEG(id, SEG_NI2),
-                                             ASEG(id, SEG_NT), ASEG(id, SEG_CN), ASEG(id, SEG_CP),
-                                             ASEG(id, SEG_NT0), ASEG(id, SEG_PT0));
-        const double U = R - APT(j, PT_G) * gfr;
-        const double wn = exp(-(y0 + ASEG(id, SEG_OFFN)) * iVt) * ASEG(id, SEG_IMUN);
-        const double wp = exp((y0 + ASEG(id, SEG_OFFP)) * iVt) * ASEG(id, SEG_IMUP);
-        const double xj = APT(j, PT_X), dx = xj - x_prev;
+        const double R = recombination_value(nn, pp, TSEG(id, SEG_KD), TSEG(id, SEG_NI2),
+                                             TSEG(id, SEG_NT), TSEG(id, SEG_CN), TSEG(id, SEG_CP),
+                                             TSEG(id, SEG_NT0), TSEG(id, SEG_PT0));
+        const double U = R - TPT(j, PT_G) * gfr;
+        const double wn = exp(-(y0 + TSEG(id, SEG_OFFN)) * iVt) * TSEG(id, SEG_IMUN);
+        const double wp = exp((y0 + TSEG(id, SEG_OFFP)) * iVt) * TSEG(id, SEG_IMUP);
+        const double xj = TPT(j, PT_X), dx = xj - x_prev;
         const double iu = iu_prev + 0.5 * dx * (U_prev + U);
         Sn += 0.5 * dx * (wn_prev + wn);
         Dn += 0.5 * dx * (wn_prev * iu_prev + wn * iu);
@@ -209,8 +239,8 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
         IU(j) = iu;
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
     }
-    const double endnN = exp(ASC(SC_LNR) - (VLc + ASEG(sidN, SEG_OFFN)) * iVt);
-    const double endpN = exp(ASC(SC_LPR) + (VLc + ASEG(sidN, SEG_OFFP)) * iVt);
+    const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
+    const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
     const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
     const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
     // J = cJ (Kn + Kp): its partials in the four sums
@@ -218,27 +248,27 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
 
     // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
     auto point = [&](int j, Pt &q) {
-        q.sid = ASID(j);
-        q.offn = ASEG(q.sid, SEG_OFFN); q.offp = ASEG(q.sid, SEG_OFFP);
-        q.mun = ASEG(q.sid, SEG_MUN); q.mup = ASEG(q.sid, SEG_MUP);
+        q.sid = TSID(j);
+        q.offn = TSEG(q.sid, SEG_OFFN); q.offp = TSEG(q.sid, SEG_OFFP);
+        q.mun = TSEG(q.sid, SEG_MUN); q.mup = TSEG(q.sid, SEG_MUP);
         q.am = q.cm = 0.0;
-        q.h = APT(j, PT_H);
+        q.h = TPT(j, PT_H);
         q.an = q.ca = 0.0;
         if (j == 0 || j == NP + 1) {
             q.V = j == 0 ? V0c : VLc;
-            q.n = exp(j == 0 ? ASC(SC_LNL) : ASC(SC_LNR));
-            q.p = exp(j == 0 ? ASC(SC_LPL) : ASC(SC_LPR));
+            q.n = exp(j == 0 ? TSC(SC_LNL) : TSC(SC_LNR));
+            q.p = exp(j == 0 ? TSC(SC_LPL) : TSC(SC_LPR));
             q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
             return;
         }
         q.V = STV(j, 0);
         q.n = STV(j, 1);
         q.p = STV(j, 2);
-        q.G = APT(j, PT_G);
-        q.fix = APT(j, PT_FIX);
-        q.R = recombination(q.n, q.p, ASEG(q.sid, SEG_KD), ASEG(q.sid, SEG_NI2),
-                            ASEG(q.sid, SEG_NT), ASEG(q.sid, SEG_CN), ASEG(q.sid, SEG_CP),
-                            ASEG(q.sid, SEG_NT0), ASEG(q.sid, SEG_PT0), q.dRn, q.dRp);
+        q.G = TPT(j, PT_G);
+        q.fix = TPT(j, PT_FIX);
+        q.R = recombination(q.n, q.p, TSEG(q.sid, SEG_KD), TSEG(q.sid, SEG_NI2),
+                            TSEG(q.sid, SEG_NT), TSEG(q.sid, SEG_CN), TSEG(q.sid, SEG_CP),
+                            TSEG(q.sid, SEG_NT0), TSEG(q.sid, SEG_PT0), q.dRn, q.dRp);
     };
     Pt right, mid, cur;
     point(NP + 1, right);
@@ -252,9 +282,9 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
         const Edge eL = edge_flux_pt(cur, mid, Vt, iVt, a.bern);
         double A[9], B[9], C[9], r[3];
         RowSeg g;
-        g.meps = ASEG(mid.sid, SEG_EPS);
-        g.ceps = ASEG(cur.sid, SEG_EPS);
-        g.sP = ASEG(mid.sid, SEG_SP); g.scs = ASEG(mid.sid, SEG_SCS); g.dop = ASEG(mid.sid, SEG_DOP);
+        g.meps = TSEG(mid.sid, SEG_EPS);
+        g.ceps = TSEG(cur.sid, SEG_EPS);
+        g.sP = TSEG(mid.sid, SEG_SP); g.scs = TSEG(mid.sid, SEG_SCS); g.dop = TSEG(mid.sid, SEG_DOP);
         row_blocks_s<1>(g, cur, mid, right.V, eL, eR, i == NP, L2, iVt, 1.0, gfr, A, B, C, r);
         double M[9], Mi[9];
 #pragma unroll
@@ -265,10 +295,10 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
                                                C[3 * p + 2] * cpn[6 + q]);
         inverse3d(M, Mi);
         // dJ/dy_i by the current's reverse mode
-        const double xi = APT(i, PT_X), dx = xi - APT(i - 1, PT_X);
+        const double xi = TPT(i, PT_X), dx = xi - TPT(i - 1, PT_X);
         const double ai = 0.5 * (dx + dx_next);
-        const double wn = exp(-(mid.V + mid.offn) * iVt) * ASEG(mid.sid, SEG_IMUN);
-        const double wp = exp((mid.V + mid.offp) * iVt) * ASEG(mid.sid, SEG_IMUP);
+        const double wn = exp(-(mid.V + mid.offn) * iVt) * TSEG(mid.sid, SEG_IMUN);
+        const double wp = exp((mid.V + mid.offp) * iVt) * TSEG(mid.sid, SEG_IMUP);
         const double Tn = ai * wn + Tn1, Tp = ai * wp + Tp1;
         const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
                           bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
@@ -319,22 +349,31 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
         }
     }
 
-    // ---- per direction: dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y -------------
-    for (int k = 0; k < K; ++k) {
-        const double *dpt = a.dpt + (size_t)k * (NP + 2) * PT_N * n + d;
-        const double *dsc = a.dsc + (size_t)k * SC_N * n + d;
+}
+
+// dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y at device d's converged voltage v
+// (adj_lambda's workspace of lane sl), with direction k's tangents staged in the LDS table.
+DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
+    const int NP = a.NP;
+    const size_t n = (size_t)a.n, ls = a.ls;
+    const AdjTab t(a.nseg, a.NP);
+    const int S = t.S;
+    lcdouble *tsc = tab + t.sct;
+    const double Vt = tsc[SC_VT], iVt = 1.0 / Vt, gfr = tsc[SC_GFRAC], elec = tsc[SC_ELEC];
+    const double cJ = -elec * Q_E, Vint = a.V[v];
+    const int sid0 = TSID(0), sidN = TSID(NP + 1);
+    {
         SegD sd;
-        sd.val = a.seg + d;
-        sd.tan = a.dseg + (size_t)k * a.nseg * SEG_N * n + d;
-        sd.n = n;
+        sd.val = tab;
+        sd.tan = tab + S;
         auto ptd = [&](int j, int f) -> D1 {
-            return {APT(j, f), dpt[((size_t)j * PT_N + f) * n]};
+            return {tab[t.ptv + j * PT_N + f], tab[t.ptt + j * PT_N + f]};
         };
-        auto scd = [&](int f) -> D1 { return {ASC(f), dsc[(size_t)f * n]}; };
+        auto scd = [&](int f) -> D1 { return {tsc[f], tsc[SC_N + f]}; };
         const D1 Ld = scd(SC_L);
         const D1 V0d = scd(SC_V0) + 0.5 * elec * Vint, VLd = scd(SC_VL) - 0.5 * elec * Vint;
         auto pointd = [&](int j, PD &q) {
-            q.sid = ASID(j);
+            q.sid = TSID(j);
             q.h = ptd(j, PT_H);
             if (j == 0 || j == NP + 1) {
                 q.V = j == 0 ? V0d : VLd;
@@ -382,18 +421,18 @@ DDH_FN void adjoint_voltage(const AdjArgs &a, int d, int v, size_t sl, bool mobi
         const D1 Knd = (Vt * (endn0d - endnNd) + Dnd) / Snd;
         const D1 Kpd = (Vt * (endpNd - endp0d) - Dpd) / Spd;
         const D1 J = cJ * (Knd + Kpd);
-        out[k] = J.d - acc;
+        return J.d - acc;
     }
+}
+#undef TSC
+#undef TPT
+#undef TSID
+#undef TSEG
 #undef FAC
 #undef STV
 #undef LAM
 #undef IU
 #undef YS
-}
 
-#undef ASC
-#undef APT
-#undef ASID
-#undef ASEG
 
 }  // namespace ddh

@@ -92,13 +92,34 @@ __global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
-// over the devices), one lane per voltage; the device's arrays are the same for all lanes.
+// over the devices), one lane per voltage; each direction's segment and scalar tangents are
+// staged in LDS by the whole wavefront, read there by every lane.
 __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
-    const size_t sl = (size_t)blockIdx.x * 64 + threadIdx.x;
+    extern __shared__ double lds[];  // adj_table_size(a.nseg, a.NP) doubles
+    ldouble *tab = (ldouble *)lds;
+    const int lane = threadIdx.x;
+    const size_t sl = (size_t)blockIdx.x * 64 + lane;
+    const int nV = a.nV, K = a.ntheta;
     for (int d = blockIdx.x; d < a.n; d += gridDim.x) {
         const bool mobile = a.sc[(size_t)SC_AT * a.n + d] > 0 || a.sc[(size_t)SC_CT * a.n + d] > 0;
-        if (mobile && threadIdx.x == 0) a.status[d] = 1;
-        for (int v = threadIdx.x; v < a.nV; v += 64) adjoint_voltage(a, d, v, sl, mobile);
+        if (mobile && lane == 0) a.status[d] = 1;
+        __syncthreads();  // every lane is done with the previous device's table
+        adj_stage_values(a, d, tab, lane, 64);
+        __syncthreads();
+        for (int v0 = 0; v0 < nV; v0 += 64) {
+            const int v = v0 + lane;
+            const bool act = v < nV && !mobile && a.ok[(size_t)d * nV + v];
+            double *out = a.grad + ((size_t)d * nV + (v < nV ? v : 0)) * K;
+            if (v < nV && !act)
+                for (int k = 0; k < K; ++k) out[k] = mobile ? NAN : 0.0;
+            if (act) adj_lambda(a, d, v, sl, tab);
+            for (int k = 0; k < K; ++k) {
+                __syncthreads();  // every lane is done with the previous direction's tangents
+                adj_stage_direction(a, d, k, tab, lane, 64);
+                __syncthreads();
+                if (act) out[k] = adj_direction(a, d, v, sl, tab);
+            }
+        }
     }
 }
 
@@ -387,7 +408,11 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     a.ls = w.ls;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((unsigned)w.slots), block(64);
-    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, 0, s, a);
+    const size_t lds = (size_t)ddh::adj_table_size(b->nseg, b->NP) * sizeof(double);
+    if (lds > 160 * 1024) return fail(DDH_E_SHAPE, "ddh_adjoint: NP too large for the LDS table");
+    (void)hipFuncSetAttribute((const void *)ddh::adjoint_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, lds, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel launch");
     return DDH_OK;

@@ -163,10 +163,30 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
     a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
     a.fac = fac.data(); a.st = st.data(); a.lam = lam.data(); a.iu = iu.data(); a.ls = ls;
+    std::vector<double> tab(adj_table_size(nseg, NP));
     for (int d = 0; d < n; ++d) {
         const bool mobile = sc[(size_t)SC_AT * n + d] > 0 || sc[(size_t)SC_CT * n + d] > 0;
         if (mobile) status[d] = 1;
-        for (int v = 0; v < nV; ++v) adjoint_voltage(a, d, v, (size_t)(v % 64), mobile);
+        adj_stage_values(a, d, tab.data(), 0, 1);
+        for (int v0 = 0; v0 < nV; v0 += 64) {
+            const int v1 = v0 + 64 < nV ? v0 + 64 : nV;
+            std::vector<char> act(64, 0);
+            for (int v = v0; v < v1; ++v) {
+                act[v - v0] = !mobile && ok[(size_t)d * nV + v];
+                double *out = grad + ((size_t)d * nV + v) * ntheta;
+                if (!act[v - v0])
+                    for (int k = 0; k < ntheta; ++k) out[k] = mobile ? NAN : 0.0;
+                else
+                    adj_lambda(a, d, v, (size_t)(v - v0), tab.data());
+            }
+            for (int k = 0; k < ntheta; ++k) {
+                adj_stage_direction(a, d, k, tab.data(), 0, 1);
+                for (int v = v0; v < v1; ++v)
+                    if (act[v - v0])
+                        grad[((size_t)d * nV + v) * ntheta + k] =
+                            adj_direction(a, d, v, (size_t)(v - v0), tab.data());
+            }
+        }
     }
     return 0;
 }

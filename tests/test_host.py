@@ -198,3 +198,7 @@ def test_library_rejects_bad_shapes_before_any_launch():
     assert rc == -1 and b"nseg" in lib.ddh_last_error()
     rc = lib.ddh_assemble(None, None, None, None, 0, None, None, None, None, None, None, None)
     assert rc == -4
+    # the adjoint stages a device's arrays in LDS: a grid too fine for it is refused
+    b = _lib.Batch(64, 5000, 7, 0, 1, 1, 1, 1)
+    rc = lib.ddh_adjoint(ctypes.byref(b), 1, 61, 1, 1, 16, 1, 1, 1, 0, 1, 1, 1, 1 << 45, None)
+    assert rc == -1 and b"LDS" in lib.ddh_last_error()
