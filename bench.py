@@ -3,8 +3,9 @@
 
 A step is one `solve_batch` of the whole synthetic workload: 16384 devices (NP = 400) with
 mobile anions and cations in the perovskite, free at every voltage, 61 voltages from -0.2 to
-1.3 V, in mixed precision (FP32 block factorisation on row-equilibrated blocks while a solve is
-far from its solution, FP64 residual, exponentials, current and stop rules). The all-FP64
+1.3 V, in mixed precision (FP32 block factorisation of each voltage's first Newton step on
+row-equilibrated blocks, every step's factors stored in FP32; FP64 residual, exponentials,
+back-substitution, current and stop rules). The all-FP64
 form is timed beside it, and the mixed curves are checked against it (1e-6 of each curve's
 scale). One JSON line on stdout:
 
@@ -156,21 +157,22 @@ def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
 
 def gradient_bench(tangents_, V, warm: bool = True):
     """dJ/dtheta of the gradient workload's curves: one FP64 solve keeping the converged states
-    and one adjoint launch, timed with HIP events on the launch stream."""
+    and one adjoint launch, with the batch and its tangents resident in HBM before the timed
+    region (the warm-up call has allocated the workspaces), timed with HIP events on the
+    launch stream and by the host clock around the call."""
     import torch
 
-    from jaxdd_amd.adjoint import jv_gradients
+    from jaxdd_amd.adjoint import jv_gradients, resident
 
     packed, tan = tangents_
     n = packed["sc"].shape[1]
+    batch, dtan = resident(packed, tan)
     if warm:
-        sub = {k: (v[..., :64] if k != "nseg" else v) for k, v in packed.items()}
-        sub["R_series"] = packed["R_series"][:64]
-        jv_gradients(sub, {k: v[..., :64] for k, v in tan.items()}, V)
+        jv_gradients(batch, dtan, V)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, ok, its, grad = jv_gradients(packed, tan, V, events=ev)
+    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])

@@ -116,18 +116,28 @@ def tangents(thetas, NP: int = 400, workers: int = 1, chunk: int = 64):
     return packed, tan
 
 
+def resident(packed, tan, device="cuda"):
+    """The packed batch (`from_packed`) and the tangents moved to ``device`` once, for calls of
+    jv_gradients that should not copy them again."""
+    batch = from_packed(packed, device)
+    dtan = {k: torch.as_tensor(np.ascontiguousarray(tan[k]), device=batch["_pt"].device)
+            for k in ("pt", "seg", "sc")}
+    return batch, dtan
+
+
 def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
                  bernoulli_name="exact", device="cuda", return_states=False, events=None):
     """J (n, nV), converged, iterations and dJ/dtheta (n, nV, 16) on the GPU for ascending
     voltages: the FP64 solve keeping each voltage's final state, then the adjoint kernel.
-    Unconverged voltages get zero gradients. ``return_states``: also the linearisation points
-    (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream around the
-    solve kernel and around the adjoint kernel."""
+    Unconverged voltages get zero gradients. ``packed``, ``tan``: host arrays (`tangents`), or
+    the device-resident pair `resident` returns. ``return_states``: also the linearisation
+    points (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream
+    around the solve kernel and around the adjoint kernel."""
     lib = _lib.load()
     V = np.asarray(voltages, dtype=float)
     if np.any(np.diff(V) <= 0):
         raise ValueError("jv_gradients takes ascending voltages")
-    batch = from_packed(packed, device)
+    batch = packed if "_pt" in packed else from_packed(packed, device)
     n, NP, nV = batch["n"], batch["NP"], len(V)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
                      0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0)
@@ -144,7 +154,8 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
                    batch["_seg"].data_ptr(), batch["_sc"].data_ptr())
     stream = torch.cuda.current_stream(dev).cuda_stream
     K = tan["pt"].shape[0]
-    dpt, dseg, dsc = (torch.as_tensor(np.ascontiguousarray(tan[k]), device=dev)
+    dpt, dseg, dsc = (tan[k] if isinstance(tan[k], torch.Tensor)
+                      else torch.as_tensor(np.ascontiguousarray(tan[k]), device=dev)
                       for k in ("pt", "seg", "sc"))
     grad = torch.empty((n, nV, K), dtype=torch.float64, device=dev)
     astatus = torch.zeros(n, dtype=torch.int32, device=dev)
