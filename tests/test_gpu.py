@@ -215,6 +215,30 @@ def test_synthetic_matches_oracle(torch_cuda, oracle, config, variant, n, precis
         assert err < 1e-6 * curve_scale(dev, Jo[k])  # the workload's accuracy bar
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
+@pytest.mark.parametrize("ions", [False, True])
+def test_odd_grid_matches_oracle(torch_cuda, oracle, ions, lanes):
+    """NP = 301 interior points (the two-lane split's halves differ by a row), 37 devices (a
+    partial wavefront): the oracle's flags and iterations, and its curves."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import prepare, solve_batch
+
+    t = S.draw(37)
+    devs = [S.device_from_row(t, k, 301, ions, False) for k in range(37)]
+    V = S.VOLTAGES[::4].copy()
+    Jo, oko, itso = oracle.solve_batch(devs, V, first_max_iter=None)
+    J, ok, its = (_np(x) for x in solve_batch(prepare(devs), V, lanes=lanes))
+    wild = chaotic_devices(oracle, devs, V, Jo)
+    assert wild.mean() <= 0.05
+    agree, chaotic = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree and chaotic <= 0.02 * ok[~wild].size
+    for k, dev in enumerate(devs):
+        if wild[k]:
+            continue
+        both = ok[k] & oko[k]
+        assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
+
+
 def test_one_device_workload_c1(torch_cuda, oracle):
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.solver import solve_jv

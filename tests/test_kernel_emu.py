@@ -167,3 +167,25 @@ def test_replay_assemble_matches_oracle(emu, oracle, case):
         assert np.all(np.abs(U[k] - Uo) <= 1e-12 * (np.abs(Uo) + rs[:, :, None]))
         Jo = oracle.current(dev, fulls[k])  # the ions do not enter the current
         assert abs(J[k] - Jo) <= 1e-10 * (abs(Jo) + oracle.j_scale(dev)), (J[k], Jo)
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+@pytest.mark.parametrize("ions", [False, True])
+def test_replay_odd_grid_matches_oracle(emu, oracle, ions, lanes):
+    """An odd number of interior points (NP = 301): the two-lane split's halves differ by a
+    row; the oracle's flags and iterations, and its curves to the synthetic tests' bar (each
+    solve stops at a step below 1e-10, so the last bits of two FP64 orders differ there)."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    t = S.draw(4)
+    devs = [S.device_from_row(t, k, 301, ions, False) for k in range(4)]
+    V = S.VOLTAGES[::4].copy()
+    Jo, oko, itso = oracle.solve_batch(devs, V, first_max_iter=None)
+    J, ok, its, status = emu.solve_packed(pack(devs), V, precision="fp64", lanes=lanes)
+    assert not status.any()
+    agree, chaotic = flags_agree(ok, its, oko, itso)
+    assert agree and chaotic == 0
+    for k, dev in enumerate(devs):
+        both = ok[k] & oko[k]
+        assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
