@@ -55,8 +55,11 @@ def parse():
     p.add_argument("--config", default="c3", choices=["c2", "c3"])
     p.add_argument("--variant", default="free", choices=["free", "frozen"])
     p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    p.add_argument("--lanes", type=int, default=2, choices=[1, 2],
-                   help="GPU lanes per device (2: each Newton solve split at the grid's middle)")
+    p.add_argument("--lanes", type=int, default=2, choices=[1, 2, 4],
+                   help="GPU lanes per device (2: each Newton solve split at the grid's middle;"
+                        " 4, in quarters: c2 only, the devices without mobile ions)")
+    p.add_argument("--grad-lanes", type=int, default=4, choices=[1, 2, 4],
+                   help="GPU lanes per device of the gradient workload's solve")
     p.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
@@ -155,7 +158,7 @@ def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
     return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
 
 
-def gradient_bench(tangents_, V, warm: bool = True):
+def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4):
     """dJ/dtheta of the gradient workload's curves: one FP64 solve keeping the converged states
     and one adjoint launch, with the batch and its tangents resident in HBM before the timed
     region (the warm-up call has allocated the workspaces), timed with HIP events on the
@@ -168,11 +171,11 @@ def gradient_bench(tangents_, V, warm: bool = True):
     n = packed["sc"].shape[1]
     batch, dtan = resident(packed, tan)
     if warm:
-        jv_gradients(batch, dtan, V)
+        jv_gradients(batch, dtan, V, lanes=lanes)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev)
+    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev, lanes=lanes)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
@@ -182,7 +185,7 @@ def gradient_bench(tangents_, V, warm: bool = True):
     return {
         "value": n / (t1 - t0), "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters)",
         "devices": n, "voltages": len(V), "ms": (t1 - t0) * 1e3,
-        "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms,
+        "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms, "solve_lanes_per_device": lanes,
         "converged_frac": float(ok.mean()),
         "finite": bool(np.isfinite(grad.cpu().numpy()).all()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -257,7 +260,7 @@ def main() -> int:
     launch_bytes = float(its.sum()) * step_bytes
     achieved = launch_bytes / (kernel_ms * 1e-3) / 1e9
     workload = f"{args.config}-{args.variant}-n{args.n}-{args.precision}-lanes{args.lanes}"
-    kernel = "ddh::solve_pair_kernel" if args.lanes == 2 else "ddh::solve_kernel"
+    kernel = {1: "ddh::solve_kernel", 2: "ddh::solve_pair_kernel", 4: "ddh::solve_quad_kernel"}[args.lanes]
 
     out = {
         "metric": METRIC,
@@ -312,7 +315,7 @@ def main() -> int:
         out["flags_equal_calm"] = bool(np.array_equal(ok[calm], ok64[calm]))
         out["chaotic_points"] = int((~calm).sum())
     if grad_inputs is not None:
-        out["gradient"] = gradient_bench(grad_inputs, V)
+        out["gradient"] = gradient_bench(grad_inputs, V, lanes=args.grad_lanes)
         out["gradient"]["host_tangents_s"] = grad_host_s
         out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
                                        "voltages, FP64, 16 parameters")

@@ -64,21 +64,24 @@ typedef struct {
                             no extrapolation, tol/stall/noise-floor stops, pre-bias must hold) */
     int32_t lanes;       /* lanes per device in batch mode: 2 (0, the default) splits each
                             Newton solve at the grid's middle, 1 sweeps it from one end (the
-                            reference's order of elimination); jv mode always uses 1 */
+                            reference's order of elimination), 4 splits it in quarters (for
+                            batches too small to fill the GPU; devices without mobile ions
+                            and NP >= 16 only: a device with ions gets status 2, NaN currents
+                            and no converged voltage); jv mode always uses 1 */
 } ddh_opts;
 
 int ddh_abi_version(void);
 const char *ddh_last_error(void);
 
-/* Workspace bytes ddh_solve_batch needs for this shape and precision. */
+/* Workspace bytes ddh_solve_batch needs for this shape and precision (any lanes). */
 size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision);
 
 /* The J-V curves of a batch: replaces src/jaxdd/batch.py:1033-1047 (`_solve_batch`, the
  * jit+vmap of `_solve_one` :937-1006) and, with opts.jv_mode = 1, src/jaxdd/solver.py:2105-2245
  * (`_scan` of `solve_jv`) for each device. V: nV voltages (device memory), ascending in batch
  * mode, the caller's order in jv mode. Outputs [n][nV]: J (A/m^2, without the shunt),
- * converged, iterations. status: [n] int32, nonzero where a device's pre-bias failed in jv
- * mode (the reference raises RuntimeError there). */
+ * converged, iterations. status: [n] int32, 1 where a device's pre-bias failed in jv mode
+ * (the reference raises RuntimeError there), 2 where opts.lanes = 4 met mobile ions. */
 int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
                     double *J, uint8_t *ok, int32_t *its, int32_t *status, void *workspace,
                     size_t workspace_bytes, void *hip_stream);

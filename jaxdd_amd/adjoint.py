@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from . import synthetic as S
-from .batch import SC_FIELDS, from_packed, pack
+from .batch import SC_FIELDS, check_four_lanes, from_packed, pack
 
 THETA = ("L_etl", "L_pvk", "L_htl",
          "mu_n_etl", "mu_p_etl", "mu_n_pvk", "mu_p_pvk", "mu_n_htl", "mu_p_htl",
@@ -126,21 +126,31 @@ def resident(packed, tan, device="cuda"):
 
 
 def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
-                 bernoulli_name="exact", device="cuda", return_states=False, events=None):
+                 bernoulli_name="exact", device="cuda", return_states=False, events=None,
+                 lanes=4):
     """J (n, nV), converged, iterations and dJ/dtheta (n, nV, 16) on the GPU for ascending
     voltages: the FP64 solve keeping each voltage's final state, then the adjoint kernel.
     Unconverged voltages get zero gradients. ``packed``, ``tan``: host arrays (`tangents`), or
     the device-resident pair `resident` returns. ``return_states``: also the linearisation
     points (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream
-    around the solve kernel and around the adjoint kernel."""
+    around the solve kernel and around the adjoint kernel. ``lanes``: GPU lanes per device of
+    the solve (4: the grid in quarters, the fastest for the batches gradients are taken of;
+    2; 1), as `jaxdd_amd.batch.solve_batch`'s."""
     lib = _lib.load()
     V = np.asarray(voltages, dtype=float)
     if np.any(np.diff(V) <= 0):
         raise ValueError("jv_gradients takes ascending voltages")
     batch = packed if "_pt" in packed else from_packed(packed, device)
     n, NP, nV = batch["n"], batch["NP"], len(V)
+    if lanes not in (1, 2, 4):
+        raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
+    sc = batch["_sc"]
+    if bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any()):
+        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+    if lanes == 4:
+        check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
-                     0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0)
+                     0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0, int(lanes))
     dev = batch["_pt"].device
     Vd = torch.as_tensor(V, device=dev)
     J = torch.empty((n, nV), dtype=torch.float64, device=dev)

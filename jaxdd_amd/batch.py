@@ -154,6 +154,17 @@ def from_packed(packed: dict, device: str | torch.device = "cuda") -> dict:
     }
 
 
+def check_four_lanes(batch: dict) -> None:
+    """`ValueError` unless the four-lane kernel solves every device of the batch: no mobile
+    ions, NP >= 16 (the kernel marks a device with ions status 2 and leaves it unsolved)."""
+    sc = batch["_sc"]
+    ions = (sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)
+    if bool(ions.any()):
+        raise ValueError("lanes=4 solves devices without mobile ions only")
+    if batch["NP"] < 16:
+        raise ValueError(f"lanes=4 needs NP >= 16, got {batch['NP']}")
+
+
 def _sc_now(batch: dict) -> torch.Tensor:
     """The packed scalars with the per-device keys the caller may have replaced."""
     sc = batch["_sc"].clone()
@@ -210,8 +221,10 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     row-equilibrated blocks; residual, exponentials, current and convergence tests in FP64).
     ``devices`` is accepted for the reference's signature: one GPU solves the whole batch.
     ``events``: optional (start, end) torch.cuda.Event pair timing the solve kernel.
-    ``lanes``: 2 (two GPU lanes per device, each Newton solve split at the grid's middle) or
-    1 (one lane sweeping the whole grid, the reference's order of elimination)."""
+    ``lanes``: 2 (two GPU lanes per device, each Newton solve split at the grid's middle),
+    1 (one lane sweeping the whole grid, the reference's order of elimination) or 4 (four
+    lanes, the grid split in quarters: for batches too small to fill the GPU at two lanes;
+    devices without mobile ions only, `ValueError` otherwise)."""
     del first_max_iter, devices
     if precision not in _lib.PRECISIONS:
         raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
@@ -219,8 +232,10 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
     V = np.asarray(voltages, dtype=float).ravel()
     order = np.argsort(V, kind="stable")
-    if lanes not in (1, 2):
-        raise ValueError(f"lanes {lanes!r}: 1 or 2")
+    if lanes not in (1, 2, 4):
+        raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
+    if lanes == 4:
+        check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
                      0 if bernoulli_name == "exact" else 1, _lib.PRECISIONS[precision], 0,
                      int(lanes))

@@ -11,6 +11,7 @@
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
 #include "ddhip_pair.h"
+#include "ddhip_quad.h"
 
 namespace ddh {
 
@@ -39,15 +40,28 @@ __global__ void __launch_bounds__(64) solve_kernel(Args a) {
     }
 }
 
-// A register copy of the partner lane's record (lanes 2d and 2d + 1 hold device d).
-template <typename R>
+// A register copy of the record of lane (this lane ^ M): with M = 1 the partner lane of a pair
+// (lanes 2d and 2d + 1 hold device d).
+template <int M = 1, typename R>
 __device__ inline R partner(const R &r) {
     static_assert(sizeof(R) % 4 == 0, "records of 32-bit words");
     R o;
     const int *src = reinterpret_cast<const int *>(&r);
     int *dst = reinterpret_cast<int *>(&o);
 #pragma unroll
-    for (int q = 0; q < (int)(sizeof(R) / 4); ++q) dst[q] = __shfl_xor(src[q], 1);
+    for (int q = 0; q < (int)(sizeof(R) / 4); ++q) dst[q] = __shfl_xor(src[q], M);
+    return o;
+}
+
+// A register copy of lane `from`'s record.
+template <typename R>
+__device__ inline R lane_record(const R &r, int from) {
+    static_assert(sizeof(R) % 4 == 0, "records of 32-bit words");
+    R o;
+    const int *src = reinterpret_cast<const int *>(&r);
+    int *dst = reinterpret_cast<int *>(&o);
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(R) / 4); ++q) dst[q] = __shfl(src[q], from);
     return o;
 }
 
@@ -81,6 +95,61 @@ __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
             else pair_phase2<float>(a, lane, half, ions, st, oe);
             const PairBack ob = partner(st.b);
             pair_phase3(a, lane, half, S, st, ob);
+        }
+    }
+}
+
+// Four lanes per device (ddhip_quad.h): 16 devices per wavefront, devices without mobile ions
+// (a device with them is refused: status 2); the segment table's free columns hold lanes 1 and
+// 2's prefixes (the table has nseg * SEG_N >= 22 rows). Lane 0 of each device runs the Poisson start;
+// the four lanes then run the same state machine. A step's records move by shuffles: the
+// meeting records within lanes (0, 1) and (2, 3), the middle's two maps from lanes 1 and 2 to
+// all four, the chunks' current sums within the pairs and then the halves' across them.
+struct QuadMid {
+    double W[12];
+};
+
+__global__ void __launch_bounds__(64) solve_quad_kernel(Args a) {
+    extern __shared__ double lds[];  // a.nseg * SEG_N * WAVE doubles (16 device slots used)
+    const int t = threadIdx.x, q = t & 3, base = t & ~3;
+    const int lane = blockIdx.x * 16 + (t >> 2);  // the device
+    const bool inb = lane < a.n;
+    const bool refused = inb && quad_refused(a, lane);
+    const bool valid = inb && !refused;
+    ldouble *lseg = (ldouble *)lds + (t >> 2);
+    ldouble *pre = (ldouble *)lds + quad_pre_column(t >> 2, q);
+    LaneState &S = a.st[valid ? 4 * (size_t)lane + q : 0];
+    if (refused && q == 0) quad_refuse(a, lane);
+    if (valid && q == 0) {
+        fill_segments(a, lane, lseg);
+        poisson_lane(a, lane, S);
+    }
+    __syncthreads();  // the segment table and the Poisson start, for lanes 1..3
+    if (valid && q != 0) S = a.st[4 * (size_t)lane];
+    if (valid) start_lane(a, lane, S);
+    while (true) {
+        const bool act = valid && lane_active(S);
+        if (!__any(act)) break;
+        const bool f64 = !a.mixed || __any(act && lane_needs64(S));
+        if (act) {
+            QuadStep st;
+            if (f64) quad_phase1<double>(a, lane, q, lseg, pre, S, st);
+            else quad_phase1<float>(a, lane, q, lseg, pre, S, st);
+            const QuadEdge oe = partner<1>(st.e.m);
+            double Xa[12], Xb[12];
+            quad_meet((q & 1) ? oe : st.e.m, (q & 1) ? st.e.m : oe, Xa, Xb);
+            QuadMid w;
+            quad_middle_part(st.e, q, Xa, Xb, w.W);
+            const QuadMid P = lane_record(w, base + 1), Q = lane_record(w, base + 2);
+            double xs[3], sv[3];
+            quad_center(P.W, Q.W, q, Xa, Xb, xs, sv);
+            if (f64) quad_phase2<double>(a, lane, q, st, xs, sv);
+            else quad_phase2<float>(a, lane, q, st, xs, sv);
+            const QuadBack ob = partner<1>(st.b);
+            PairBack h;
+            quad_half((q & 1) ? ob : st.b, (q & 1) ? st.b : ob, q >> 1, h);
+            const PairBack oh = partner<2>(h);
+            quad_phase3(a, lane, S, (q >> 1) ? oh : h, (q >> 1) ? h : oh);
         }
     }
 }
@@ -269,13 +338,15 @@ struct WsLayout {
 
 // One region per array: the state buffers, the FP64 and FP32 Newton factors, Poisson's
 // factors, the fixed ions in use and the lanes' state machines.
-static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision, int32_t lanes = 2) {
+// Sized for every kernel: the four-lane solve's factors and lanes are the largest.
+static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision, int32_t lanes = 4) {
     WsLayout w;
+    const size_t nf = ddh::NFAC_Q > ddh::NFAC ? ddh::NFAC_Q : ddh::NFAC;
     size_t off = 0;
     w.y = off; off = align256(off + (size_t)ddh::NBUF * NP * 3 * n * sizeof(double));
-    w.fac64 = off; off = align256(off + (size_t)NP * ddh::NFAC * n * sizeof(double));
+    w.fac64 = off; off = align256(off + (size_t)NP * nf * n * sizeof(double));
     w.fac32 = off;
-    if (precision == DDH_MIXED) off = align256(off + (size_t)NP * ddh::NFAC * n * sizeof(float));
+    if (precision == DDH_MIXED) off = align256(off + (size_t)NP * nf * n * sizeof(float));
     w.pfac = off; off = align256(off + (size_t)NP * 4 * n * sizeof(double));
     w.fix = off; off = align256(off + (size_t)NP * n * sizeof(double));
     w.st = off; off = align256(off + (size_t)lanes * n * sizeof(ddh::LaneState));
@@ -286,7 +357,7 @@ static WsLayout ws_layout(int32_t n, int32_t NP, int32_t precision, int32_t lane
 extern "C" size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t precision) {
     (void)nV;
     if (n <= 0 || NP < 2) return 0;
-    return ws_layout(n, NP, precision, 2).total;  // enough for either kernel
+    return ws_layout(n, NP, precision, 4).total;  // enough for every kernel
 }
 
 static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *o,
@@ -302,9 +373,11 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
         return fail(DDH_E_ARG, "ddh_solve_batch: bernoulli must be 0 (exact) or 1 (simss)");
     if (o->max_iter < 1 || o->start_iter < 1)
         return fail(DDH_E_ARG, "ddh_solve_batch: max_iter and start_iter must be >= 1");
-    if (o->lanes != 0 && o->lanes != 1 && o->lanes != 2)
-        return fail(DDH_E_ARG, "ddh_solve_batch: lanes must be 1 or 2 (0: the default, 2)");
+    if (o->lanes != 0 && o->lanes != 1 && o->lanes != 2 && o->lanes != 4)
+        return fail(DDH_E_ARG, "ddh_solve_batch: lanes must be 1, 2 or 4 (0: the default, 2)");
     const int lanes = o->jv_mode ? 1 : (o->lanes == 0 ? 2 : o->lanes);
+    if (lanes == 4 && b->NP < ddh::QUAD_MIN_NP)
+        return fail(DDH_E_SHAPE, "ddh_solve_batch: four lanes per device need NP >= 16");
     WsLayout w = ws_layout(b->n, b->NP, o->precision, lanes);
     if (workspace_bytes < w.total) return fail(DDH_E_WORKSPACE, "ddh_solve_batch: workspace too small");
     char *base = (char *)workspace;
@@ -324,7 +397,12 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
     a.mixed = o->precision == DDH_MIXED ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     const size_t lds = (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
-    if (lanes == 2) {
+    if (lanes == 4) {
+        dim3 grid((b->n + 15) / 16), block(64);
+        (void)hipFuncSetAttribute((const void *)ddh::solve_quad_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(ddh::solve_quad_kernel, grid, block, lds, s, a);
+    } else if (lanes == 2) {
         dim3 grid((b->n + 31) / 32), block(64);
         (void)hipFuncSetAttribute((const void *)ddh::solve_pair_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -9,6 +9,7 @@
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
 #include "ddhip_pair.h"
+#include "ddhip_quad.h"
 
 #include <cstdlib>
 #include <vector>
@@ -109,6 +110,69 @@ static void run_pair(Args &a) {
     }
 }
 
+// Four lanes per device (ddhip_quad.h): wavefronts of 16 devices; each device's phases run
+// lane by lane, the records exchanged between them as the GPU's shuffles do.
+static void run_quad(Args &a) {
+    const int nw = (a.n + 15) / 16;
+    std::vector<char> started(nw, 0), done(nw, 0);
+    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * WAVE);
+    int remaining = nw;
+    while (remaining > 0) {
+        for (int w = 0; w < nw; ++w) {
+            if (done[w]) continue;
+            const int d0 = w * 16, d1 = d0 + 16 < a.n ? d0 + 16 : a.n;
+            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
+            if (!started[w]) {
+                for (int d = d0; d < d1; ++d) {
+                    if (quad_refused(a, d)) {
+                        quad_refuse(a, d);
+                        for (int q = 0; q < 4; ++q) a.st[4 * d + q].phase = PH_DONE;
+                        continue;
+                    }
+                    fill_segments(a, d, lds + (d - d0));
+                    poisson_lane(a, d, a.st[4 * d]);
+                    for (int q = 1; q < 4; ++q) a.st[4 * d + q] = a.st[4 * d];
+                    for (int q = 0; q < 4; ++q) start_lane(a, d, a.st[4 * d + q]);
+                }
+                started[w] = 1;
+                continue;
+            }
+            bool any = false, f64 = !a.mixed;
+            for (int k = 4 * d0; k < 4 * d1; ++k) {
+                bool act = lane_active(a.st[k]);
+                any |= act;
+                f64 |= act && lane_needs64(a.st[k]);
+            }
+            if (!any) { done[w] = 1; --remaining; continue; }
+            for (int d = d0; d < d1; ++d) {
+                LaneState *S = &a.st[4 * d];
+                if (!lane_active(S[0])) continue;
+                QuadStep st[4];
+                const double *l = lds + (d - d0);
+                for (int q = 0; q < 4; ++q) {
+                    double *pre = lds + quad_pre_column(d - d0, q);
+                    if (f64) quad_phase1<double>(a, d, q, l, pre, S[q], st[q]);
+                    else quad_phase1<float>(a, d, q, l, pre, S[q], st[q]);
+                }
+                double Xa[4][12], Xb[4][12], W[4][12];
+                for (int q = 0; q < 4; ++q) {
+                    quad_meet(st[q & ~1].e.m, st[q | 1].e.m, Xa[q], Xb[q]);
+                    quad_middle_part(st[q].e, q, Xa[q], Xb[q], W[q]);
+                }
+                for (int q = 0; q < 4; ++q) {
+                    double xs[3], sv[3];
+                    quad_center(W[1], W[2], q, Xa[q], Xb[q], xs, sv);
+                    if (f64) quad_phase2<double>(a, d, q, st[q], xs, sv);
+                    else quad_phase2<float>(a, d, q, st[q], xs, sv);
+                }
+                PairBack h[4];
+                for (int q = 0; q < 4; ++q) quad_half(st[q & ~1].b, st[q | 1].b, q >> 1, h[q]);
+                for (int q = 0; q < 4; ++q) quad_phase3(a, d, S[q], h[q & 1], h[2 | (q & 1)]);
+            }
+        }
+    }
+}
+
 static int g_lanes = 1;  // the next solve's lanes per device (jv mode: 1)
 extern "C" void emu_set_lanes(int lanes) { g_lanes = lanes; }
 
@@ -124,10 +188,11 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
                                int precision, int jv, double *J, uint8_t *ok, int32_t *its,
                                int32_t *status) {
     std::vector<double> Y((size_t)NBUF * NP * 3 * n);
-    std::vector<double> fac64((size_t)NP * NFAC * n), pfac((size_t)NP * 4 * n);
-    std::vector<float> fac32((size_t)NP * NFAC * n);
+    const size_t nf = NFAC_Q > NFAC ? NFAC_Q : NFAC;
+    std::vector<double> fac64((size_t)NP * nf * n), pfac((size_t)NP * 4 * n);
+    std::vector<float> fac32((size_t)NP * nf * n);
     std::vector<double> fix((size_t)NP * n);
-    std::vector<LaneState> st(2 * (size_t)n);
+    std::vector<LaneState> st(4 * (size_t)n);
     if (g_poison) {
         memset(Y.data(), g_poison, Y.size() * sizeof(double));
         memset(fac64.data(), g_poison, fac64.size() * sizeof(double));
@@ -144,7 +209,8 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = tol; a.max_iter = max_iter; a.start_iter = start_iter; a.strict = strict;
     a.bern = bern; a.jv = jv; a.mixed = precision ? 1 : 0;
-    if (g_lanes == 2 && !jv) run_pair(a);
+    if (g_lanes == 4 && !jv) run_quad(a);
+    else if (g_lanes == 2 && !jv) run_pair(a);
     else run(a);
     return 0;
 }

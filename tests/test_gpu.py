@@ -239,6 +239,59 @@ def test_odd_grid_matches_oracle(torch_cuda, oracle, ions, lanes):
         assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
 
 
+# ---- four lanes per device: the grid in quarters (devices without mobile ions) -------------
+@pytest.mark.parametrize("precision,jtol", [("fp64", 1e-12), ("mixed", 1e-9)])
+def test_four_lanes_match_oracle_on_fixtures(fixtures_on_gpu, precision, jtol):
+    """ddh_solve_batch with opts.lanes = 4 on the whole fixture batch: the devices without
+    mobile ions get the oracle's flags, in FP64 its iterations, and its curves; the others are
+    refused by the kernel (status 2, no converged voltage, NaN currents)."""
+    from jaxdd_amd import _lib
+    from jaxdd_amd.batch import SC_FIELDS, _run
+
+    cases, devs, batch, (Jo, oko, itso) = fixtures_on_gpu
+    sc = _np(batch["_sc"])
+    ions = (sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)
+    assert ions.any() and not ions.all()
+    opts = _lib.Opts(1e-10, 60, 60, 0, 1, _lib.PRECISIONS[precision], 0, 4)
+    J, ok, its, status = (_np(t) for t in _run(batch, V_FULL, opts, _lib.PRECISIONS[precision]))
+    shunt = _np(batch["G_shunt"])
+    J = J + shunt[:, None] * V_FULL[None, :]
+    assert np.array_equal(status, np.where(ions, 2, 0))
+    assert not ok[ions].any() and np.isnan(J[ions]).all() and not its[ions].any()
+    free = np.flatnonzero(~ions)
+    assert np.array_equal(ok[free].astype(bool), oko[free])
+    if precision == "fp64":
+        assert np.array_equal(its[free], itso[free])
+    for k in free:
+        assert np.max(np.abs(J[k] - Jo[k])) < jtol * curve_scale(devs[k], Jo[k]), cases[k]
+
+
+@pytest.mark.parametrize("np_,n", [(400, 48), (301, 37), (16, 21)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_four_lanes_match_oracle_on_synthetic(torch_cuda, oracle, np_, n, precision):
+    """The gradient workload's devices (no ions) on its grid (NP = 400), an odd grid (301:
+    quarters of 75, 75, 75 and 76 rows) and the smallest the split takes (16), in partial
+    wavefronts (16 devices a wavefront): the oracle's flags and iterations, and its curves."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import prepare, solve_batch
+
+    t = S.draw(n)
+    devs = [S.device_from_row(t, k, np_, False, False) for k in range(n)]
+    V = S.VOLTAGES[::2].copy()
+    Jo, oko, itso = oracle.solve_batch(devs, V, first_max_iter=None)
+    J, ok, its = (_np(x) for x in solve_batch(prepare(devs), V, precision=precision, lanes=4))
+    wild = chaotic_devices(oracle, devs, V, Jo)
+    assert wild.mean() <= 0.05
+    agree, chaotic = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree and chaotic <= 0.02 * ok[~wild].size
+    bar = 1e-9 if precision == "fp64" else 1e-6  # the workload's accuracy bar in mixed
+    for k, dev in enumerate(devs):
+        if wild[k]:
+            continue
+        both = ok[k] & oko[k]
+        assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < bar * curve_scale(dev, Jo[k])
+
+
 def test_one_device_workload_c1(torch_cuda, oracle):
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.solver import solve_jv

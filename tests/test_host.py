@@ -198,7 +198,37 @@ def test_library_rejects_bad_shapes_before_any_launch():
     assert rc == -1 and b"nseg" in lib.ddh_last_error()
     rc = lib.ddh_assemble(None, None, None, None, 0, None, None, None, None, None, None, None)
     assert rc == -4
+    # lanes: 1, 2 or 4; four lanes need NP >= 16
+    b = _lib.Batch(64, 400, 7, 0, 1, 1, 1, 1)
+    o = _lib.Opts(1e-10, 60, 60, 0, 0, 1, 0, 3)
+    rc = lib.ddh_solve_batch(ctypes.byref(b), 1, 61, ctypes.byref(o), 1, 1, 1, 1, 1, 1 << 40, None)
+    assert rc == -4 and b"lanes" in lib.ddh_last_error()
+    b = _lib.Batch(64, 12, 7, 0, 1, 1, 1, 1)
+    o = _lib.Opts(1e-10, 60, 60, 0, 0, 1, 0, 4)
+    rc = lib.ddh_solve_batch(ctypes.byref(b), 1, 61, ctypes.byref(o), 1, 1, 1, 1, 1, 1 << 40, None)
+    assert rc == -1 and b"NP >= 16" in lib.ddh_last_error()
     # the adjoint stages a device's arrays in LDS: a grid too fine for it is refused
     b = _lib.Batch(64, 5000, 7, 0, 1, 1, 1, 1)
     rc = lib.ddh_adjoint(ctypes.byref(b), 1, 61, 1, 1, 16, 1, 1, 1, 0, 1, 1, 1, 1 << 45, None)
     assert rc == -1 and b"LDS" in lib.ddh_last_error()
+
+
+def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
+    """solve_batch(lanes=4) and jv_gradients check the batch before any launch: mobile ions
+    (ValueError for the solve, NotImplementedError for gradients) and NP < 16."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import from_packed, pack, solve_batch
+
+    t = S.draw(3)
+    ions = from_packed(pack([S.device_from_row(t, k, 40, True, False) for k in range(3)]), "cpu")
+    with pytest.raises(ValueError, match="mobile ions"):
+        solve_batch(ions, [0.0, 0.5], lanes=4)
+    coarse = from_packed(pack([S.device_from_row(t, k, 12, False, False) for k in range(3)]), "cpu")
+    with pytest.raises(ValueError, match="NP >= 16"):
+        solve_batch(coarse, [0.0, 0.5], lanes=4)
+    with pytest.raises(ValueError, match="lanes"):
+        solve_batch(coarse, [0.0, 0.5], lanes=3)
+    tan = {"pt": np.zeros((1,) + tuple(ions["_pt"].shape)), "seg": None, "sc": None}
+    with pytest.raises(NotImplementedError, match="mobile ions"):
+        A.jv_gradients(ions, tan, [0.0, 0.5])

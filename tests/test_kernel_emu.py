@@ -189,3 +189,58 @@ def test_replay_odd_grid_matches_oracle(emu, oracle, ions, lanes):
     for k, dev in enumerate(devs):
         both = ok[k] & oko[k]
         assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
+
+
+# ---- four lanes per device (ddhip_quad.h): devices without mobile ions -------------------
+def _ion_free(packed):
+    from jaxdd_amd.batch import SC_FIELDS
+
+    sc = packed["sc"]
+    ions = (sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)
+    return np.flatnonzero(~ions)
+
+
+@pytest.mark.parametrize("precision,jtol", [("fp64", 1e-12), ("mixed", 1e-9)])
+def test_replay_four_lanes_match_oracle_on_fixtures(emu, fixture_runs, precision, jtol):
+    """The grid in quarters, the unknowns at the middle kept as extra columns: the oracle's
+    flags, in FP64 its iterations, and its curves; devices with mobile ions refused (status 2,
+    no converged voltage, NaN currents)."""
+    cases, devs, packed, (Jo, oko, itso) = fixture_runs
+    J, ok, its, status = emu.solve_packed(packed, V_FULL, bernoulli="simss", precision=precision,
+                                          lanes=4)
+    free = _ion_free(packed)
+    assert 0 < len(free) < len(devs)
+    refused = np.setdiff1d(np.arange(len(devs)), free)
+    assert np.all(status[refused] == 2) and not status[free].any()
+    assert not ok[refused].any() and np.isnan(J[refused]).all() and not its[refused].any()
+    assert np.array_equal(ok[free], oko[free])
+    if precision == "fp64":
+        assert np.array_equal(its[free], itso[free])
+    for k in free:
+        assert np.max(np.abs(J[k] - Jo[k])) < jtol * curve_scale(devs[k], Jo[k]), cases[k]
+
+
+@pytest.mark.parametrize("np_", [301, 16, 19])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_replay_four_lanes_match_oracle_on_synthetic(emu, oracle, np_, precision):
+    """The gradient workload's devices (no ions) on an odd grid (NP = 301: quarters of 75, 75,
+    75, 76 rows) and on the smallest grids the split takes (16: four rows a lane; 19): the
+    oracle's flags and iterations, and its curves."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    t = S.draw(12)
+    devs = [S.device_from_row(t, k, np_, False, False) for k in range(12)]
+    V = S.VOLTAGES[::2].copy()
+    Jo, oko, itso = oracle.solve_batch(devs, V, first_max_iter=None)
+    J, ok, its, status = emu.solve_packed(pack(devs), V, precision=precision, lanes=4)
+    assert not status.any()
+    wild = chaotic_devices(oracle, devs, V, Jo)
+    assert wild.mean() <= 0.1
+    agree, chaotic = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree and chaotic <= 0.02 * ok[~wild].size
+    for k, dev in enumerate(devs):
+        if wild[k]:
+            continue
+        both = ok[k] & oko[k]
+        assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
