@@ -61,6 +61,7 @@ struct LaneState {
     int fresh, freeze_now, mobile_now, mobile0, ok_f, fin_f, ok_c, pre_ok, fin_work;
     int noise_state;  // jv mode: 0 no noise floor yet, 1 pending, 2 known
     int need64;       // mixed precision: this solve has switched to FP64 factors
+    int part, parts;  // this lane's share of its device's state copies (lanes per device)
     double Vint, share, ratio, first, V_last, V_before, before_share, noise_floor;
     double lw[2], lpois[2], lf[2], lc[2], llast[2], lbefore[2], lstart[2];
     double hist[4], Jh[4];
@@ -1135,10 +1136,20 @@ DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
         a.ok[o] = (uint8_t)((ok && S.pre_ok) ? 1 : 0);
         a.its[o] = S.it;
         if (a.states) {  // the state the voltage ended with (the adjoint's linearisation point)
+            // each of the device's lanes copies its share, eight loads in flight at a time
             const size_t n = (size_t)a.n, m = (size_t)a.NP * 3;
             const double *src = a.Y + (size_t)S.b_work * m * n + lane;
             double *dst = a.states + (size_t)idx * m * n + lane;
-            for (size_t q = 0; q < m; ++q) dst[q * n] = src[q * n];
+            const size_t hi = m * (size_t)(S.part + 1) / (size_t)S.parts;
+            size_t q = m * (size_t)S.part / (size_t)S.parts;
+            for (; q + 8 <= hi; q += 8) {
+                double t[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = src[(q + k) * n];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dst[(q + k) * n] = t[k];
+            }
+            for (; q < hi; ++q) dst[q * n] = src[q * n];
         }
         if (ok && fin) {  // only a converged, finite solution is carried on
             S.b_before = S.b_last; S.b_last = S.b_work;

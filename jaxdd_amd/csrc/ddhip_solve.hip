@@ -26,6 +26,7 @@ __global__ void __launch_bounds__(64) solve_kernel(Args a) {
     if (valid) {
         fill_segments(a, lane, lseg);
         poisson_lane(a, lane, S);
+        S.part = 0; S.parts = 1;
         start_lane(a, lane, S);
     }
     while (true) {
@@ -80,7 +81,10 @@ __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
     }
     __syncthreads();  // the segment table and the Poisson start, for the right lanes
     if (valid && half == 1) S = a.st[2 * (size_t)lane];
-    if (valid) start_lane(a, lane, S);
+    if (valid) {
+        S.part = half; S.parts = 2;
+        start_lane(a, lane, S);
+    }
     while (true) {
         const bool act = valid && lane_active(S);
         if (!__any(act)) break;
@@ -126,7 +130,10 @@ __global__ void __launch_bounds__(64) solve_quad_kernel(Args a) {
     }
     __syncthreads();  // the segment table and the Poisson start, for lanes 1..3
     if (valid && q != 0) S = a.st[4 * (size_t)lane];
-    if (valid) start_lane(a, lane, S);
+    if (valid) {
+        S.part = q; S.parts = 4;
+        start_lane(a, lane, S);
+    }
     while (true) {
         const bool act = valid && lane_active(S);
         if (!__any(act)) break;

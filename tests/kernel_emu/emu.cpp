@@ -33,6 +33,7 @@ static void run(Args &a) {
                 for (int lane = w0; lane < w1; ++lane) {
                     fill_segments(a, lane, &lds_all[(size_t)w * a.nseg * SEG_N * WAVE] + (lane - w0));
                     poisson_lane(a, lane, a.st[lane]);
+                    a.st[lane].part = 0; a.st[lane].parts = 1;
                     start_lane(a, lane, a.st[lane]);
                 }
                 started[w] = 1;
@@ -73,8 +74,10 @@ static void run_pair(Args &a) {
                     fill_segments(a, d, lds + (d - d0));
                     poisson_lane(a, d, a.st[2 * d]);
                     a.st[2 * d + 1] = a.st[2 * d];
-                    start_lane(a, d, a.st[2 * d]);
-                    start_lane(a, d, a.st[2 * d + 1]);
+                    for (int h = 0; h < 2; ++h) {
+                        a.st[2 * d + h].part = h; a.st[2 * d + h].parts = 2;
+                        start_lane(a, d, a.st[2 * d + h]);
+                    }
                 }
                 started[w] = 1;
                 continue;
@@ -132,7 +135,10 @@ static void run_quad(Args &a) {
                     fill_segments(a, d, lds + (d - d0));
                     poisson_lane(a, d, a.st[4 * d]);
                     for (int q = 1; q < 4; ++q) a.st[4 * d + q] = a.st[4 * d];
-                    for (int q = 0; q < 4; ++q) start_lane(a, d, a.st[4 * d + q]);
+                    for (int q = 0; q < 4; ++q) {
+                        a.st[4 * d + q].part = q; a.st[4 * d + q].parts = 4;
+                        start_lane(a, d, a.st[4 * d + q]);
+                    }
                 }
                 started[w] = 1;
                 continue;

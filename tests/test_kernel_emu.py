@@ -244,3 +244,27 @@ def test_replay_four_lanes_match_oracle_on_synthetic(emu, oracle, np_, precision
             continue
         both = ok[k] & oko[k]
         assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])
+
+
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_replay_kept_states_are_the_one_lane_states(emu, lanes):
+    """ddh_solve_batch_states with a device's state copy split over its lanes: every voltage's
+    kept state (NP = 301: shares of 903 values, not a multiple of the copy's batches of eight)
+    is the one-lane solve's state to the step tolerance, and the curves agree."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    t = S.draw(5)
+    devs = [S.device_from_row(t, k, 301, False, False) for k in range(5)]
+    packed = pack(devs)
+    V = S.VOLTAGES[::5].copy()
+    shape = (len(V), 301, 3, len(devs))
+    ref, got = np.full(shape, np.nan), np.full(shape, np.nan)
+    J1, ok1, _, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200, states=ref)
+    J, ok, _, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200, states=got,
+                                   lanes=lanes)
+    assert ok1.all() and ok.all()
+    assert not np.isnan(got).any()
+    assert np.max(np.abs(got - ref)) < 1e-9
+    for k, dev in enumerate(devs):
+        assert np.max(np.abs(J[k] - J1[k])) < 1e-9 * curve_scale(dev, J1[k])
