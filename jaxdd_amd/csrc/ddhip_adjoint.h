@@ -355,7 +355,7 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
 // (adj_lambda's workspace of lane sl), with direction k's tangents staged in the LDS table.
 DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
     const int NP = a.NP;
-    const size_t n = (size_t)a.n, ls = a.ls;
+    const size_t ls = a.ls;
     const AdjTab t(a.nseg, a.NP);
     const int S = t.S;
     lcdouble *tsc = tab + t.sct;
