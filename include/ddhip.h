@@ -134,6 +134,18 @@ int ddh_assemble(const ddh_batch *b, const double *y, const double *Vint, const 
                  int32_t bernoulli, double *F, double *A, double *B, double *C, double *U,
                  double *J, void *hip_stream);
 
+/* The 5x5 system with the mobile ions as unknowns, at a steady state: replaces
+ * src/jaxdd/solver.py:1364-1473 (`_assemble_ions` with dti = 0, no sites, no gauge). y: the
+ * interior state [NP][5][n] (V, ln n, ln p, ln a, ln c); Vint [n]; logs [2][n] the slots'
+ * normalisations (one anion slot, one cation slot). Outputs (device memory): F [NP][5][n] the
+ * scaled residual (the carriers' rows `_assemble`'s at each point's ion logarithms; each ion's
+ * row ln a - la - V/V_T, ln c - lc + V/V_T in its region, its unknown outside), A, B, C
+ * [NP][25][n] the 5x5 blocks row-major, U [NP][10][n] dF/d(logs) as [row][slot] (only the ions'
+ * rows). bernoulli: as ddh_opts. */
+int ddh_assemble_ions(const ddh_batch *b, const double *y, const double *Vint,
+                      const double *logs, int32_t bernoulli, double *F, double *A, double *B,
+                      double *C, double *U, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ class Opts(ctypes.Structure):
 #: The symbols include/ddhip.h declares.
 SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solve_batch",
            "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
-           "ddh_block_sweep", "ddh_assemble")
+           "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions")
 
 _lib = None
 
@@ -62,6 +62,9 @@ def load():
     lib.ddh_assemble.restype = ctypes.c_int
     lib.ddh_assemble.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, i32, vp, vp, vp, vp, vp, vp,
                                  vp]
+    lib.ddh_assemble_ions.restype = ctypes.c_int
+    lib.ddh_assemble_ions.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, i32, vp, vp, vp, vp, vp,
+                                      vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 2:

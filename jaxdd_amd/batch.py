@@ -292,6 +292,45 @@ def assemble(batch, y, Vint, logs=None, bernoulli_name="exact"):
     }
 
 
+def assemble_ions(batch, y, Vint, logs, bernoulli_name="exact"):
+    """The steady 5x5 system with the mobile ions as unknowns of every device (reference:
+    src/jaxdd/solver.py:1364-1473, `_assemble_ions` with dti = 0). ``y``: (n, NP, 5) interior
+    (V, ln n, ln p, ln a, ln c); ``Vint``: (n,); ``logs``: (n, 2) the anion and cation slots'
+    normalisations. Returns tensors on the batch's GPU: F (n, NP, 5), A, B, C (n, NP, 5, 5),
+    U (n, NP, 5, 2)."""
+    if bernoulli_name not in ("exact", "simss"):
+        raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
+    lib = _lib.load()
+    dev = batch["_pt"].device
+    if dev.type != "cuda":
+        raise RuntimeError("assemble_ions runs on the GPU: prepare(devices, device='cuda')")
+    n, NP = batch["n"], batch["NP"]
+    yt = torch.as_tensor(y, dtype=torch.float64, device=dev)
+    if tuple(yt.shape) != (n, NP, 5):
+        raise ValueError(f"y: shape (n, NP, 5) = {(n, NP, 5)}, got {tuple(yt.shape)}")
+    yl = yt.permute(1, 2, 0).contiguous()  # (NP, 5, n): lane-innermost
+    Vd = torch.as_tensor(Vint, dtype=torch.float64, device=dev).reshape(n).contiguous()
+    lg = torch.as_tensor(logs, dtype=torch.float64, device=dev).reshape(n, 2).T.contiguous()
+    sc = _sc_now(batch)
+    F = torch.empty((NP, 5, n), dtype=torch.float64, device=dev)
+    A, B, C = (torch.empty((NP, 25, n), dtype=torch.float64, device=dev) for _ in range(3))
+    U = torch.empty((NP, 10, n), dtype=torch.float64, device=dev)
+    b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
+                   batch["_seg"].data_ptr(), sc.data_ptr())
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib.ddh_assemble_ions(ctypes.byref(b), yl.data_ptr(), Vd.data_ptr(), lg.data_ptr(),
+                               0 if bernoulli_name == "exact" else 1, F.data_ptr(), A.data_ptr(),
+                               B.data_ptr(), C.data_ptr(), U.data_ptr(), stream)
+    _lib.check(rc, "ddh_assemble_ions")
+    torch.cuda.current_stream(dev).synchronize()
+    return {
+        "F": F.permute(2, 0, 1), "A": A.reshape(NP, 5, 5, n).permute(3, 0, 1, 2),
+        "B": B.reshape(NP, 5, 5, n).permute(3, 0, 1, 2),
+        "C": C.reshape(NP, 5, 5, n).permute(3, 0, 1, 2),
+        "U": U.reshape(NP, 5, 2, n).permute(3, 0, 1, 2),
+    }
+
+
 def external_voltages(batch, voltages, J):
     """``V_ext = V_int + J * R_series`` (reference: src/jaxdd/batch.py:1104-1108)."""
     Vt = torch.as_tensor(np.asarray(voltages, dtype=float), device=J.device)

@@ -161,10 +161,11 @@ __global__ void __launch_bounds__(64) solve_quad_kernel(Args a) {
     }
 }
 
-// The assembly operator: one lane per device.
+// The assembly operators: one lane per device; FIVE the 5x5 form with the ions as unknowns.
+template <bool FIVE>
 __global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
     const int d = blockIdx.x * 64 + threadIdx.x;
-    if (d < a.n) assemble_lane(a, d);
+    if (d < a.n) assemble_lane<FIVE>(a, d);
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
@@ -512,13 +513,34 @@ extern "C" int ddh_assemble(const ddh_batch *b, const double *y, const double *V
         return fail(DDH_E_SHAPE, "ddh_assemble: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14");
     if (bernoulli != 0 && bernoulli != 1)
         return fail(DDH_E_ARG, "ddh_assemble: bernoulli must be 0 (exact) or 1 (simss)");
-    ddh::AsmArgs a;
+    ddh::AsmArgs a = {};
     a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli;
     a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
     a.y = y; a.Vint = Vint; a.logs = logs;
     a.F = F; a.A = A; a.B = B; a.C = C; a.U = U; a.J = J;
     dim3 grid((b->n + 63) / 64), block(64);
-    hipLaunchKernelGGL(ddh::assemble_kernel, grid, block, 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(ddh::assemble_kernel<false>, grid, block, 0, (hipStream_t)stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "assemble_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" int ddh_assemble_ions(const ddh_batch *b, const double *y, const double *Vint,
+                                 const double *logs, int32_t bernoulli, double *F, double *A,
+                                 double *B, double *C, double *U, void *stream) {
+    if (!b || !y || !Vint || !logs || !F || !A || !B || !C || !U)
+        return fail(DDH_E_ARG, "ddh_assemble_ions: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_assemble_ions: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14");
+    if (bernoulli != 0 && bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_assemble_ions: bernoulli must be 0 (exact) or 1 (simss)");
+    ddh::AsmArgs a = {};
+    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
+    a.y5 = y; a.Vint = Vint; a.logs = logs;
+    a.F5 = F; a.A5 = A; a.B5 = B; a.C5 = C; a.U5 = U;
+    dim3 grid((b->n + 63) / 64), block(64);
+    hipLaunchKernelGGL(ddh::assemble_kernel<true>, grid, block, 0, (hipStream_t)stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "assemble_kernel launch");
     return DDH_OK;

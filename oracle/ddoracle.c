@@ -182,8 +182,11 @@ static double recomb(double n, double p, double kd, double ni2, double Nt, doubl
 
 /* ---- assembly: solver.py:1021-1187 (Boltzmann, no optional modules) --------------------- */
 /* full: (NP+2) x 3. F: NP x 3. A, B, C: NP x 9. U: NP x 6 (3 rows x 2 slots). */
-static void assemble(const problem *P, const double *full, double g_frac, double la, double lc,
-                     double share, double *F, double *A, double *B, double *C, double *U) {
+/* la_pt, lc_pt: each interior point's ion logarithms (the 5x5 form, _assemble_ions passing
+   `la, lc` per point, solver.py:1393-1395), or NULL for the slots' la, lc at every point. */
+static void assemble_pt(const problem *P, const double *full, double g_frac, double la,
+                        double lc, const double *la_pt, const double *lc_pt, double share,
+                        double *F, double *A, double *B, double *C, double *U) {
     const ddo_device *d = P->dev;
     int NP = P->NP;
     double Vt = d->Vt;
@@ -219,8 +222,8 @@ static void assemble(const problem *P, const double *full, double g_frac, double
         double hm = P->hm[r], hp = P->hp[r], fac = P->fac[r], sP = P->sP[r], sC = P->sC[r];
         double elec = hm * Fn[i] - hp * Fn[i - 1] + fac * (gen - R);
         double hole = hp * Fp[i - 1] - hm * Fp[i] + fac * (gen - R);
-        double anion = P->am[r] * exp(la + Vm / Vt);
-        double cation = P->cm[r] * exp(lc - Vm / Vt);
+        double anion = P->am[r] * exp((la_pt ? la_pt[r] : la) + Vm / Vt);
+        double cation = P->cm[r] * exp((lc_pt ? lc_pt[r] : lc) - Vm / Vt);
         double afix = P->afix[r] * share, cfix = P->cfix[r] * share;
         double poisson = P->fac_l[r] * full[3 * (i - 1)] + P->fac_m[r] * Vm +
                          P->fac_u[r] * full[3 * (i + 1)] -
@@ -255,11 +258,73 @@ static void assemble(const problem *P, const double *full, double g_frac, double
     }
 }
 
+static void assemble(const problem *P, const double *full, double g_frac, double la, double lc,
+                     double share, double *F, double *A, double *B, double *C, double *U) {
+    assemble_pt(P, full, g_frac, la, lc, NULL, NULL, share, F, A, B, C, U);
+}
+
 int ddo_assemble(const ddo_device *dev, const double *full, double g_frac, double la, double lc,
                  int bernoulli, double *F, double *A, double *B, double *C, double *U) {
     problem P;
     problem_init(&P, dev, bernoulli);
     assemble(&P, full, g_frac, la, lc, 1.0, F, A, B, C, U);
+    problem_free(&P);
+    return 0;
+}
+
+/* ---- the 5x5 steady system: solver.py:1364-1473 (`_assemble_ions`, dti = 0) ---------------- */
+/* full5: (NP+2) x 5 (V, ln n, ln p, ln a, ln c); logs: the anion and cation slots' logarithms.
+   F: NP x 5. A, B, C: NP x 25. U: NP x 10 (5 rows x 2 slots). The carriers' rows are the 3x3
+   assembly's at each point's ion logarithms ln a - V/V_T, ln c + V/V_T (solver.py:1393-1395);
+   Poisson's loses the ions' V dependence and gains their columns (:1405-1409); each ion's row is
+   Boltzmann's in its region, its unknown outside (`steady` of `species`, :1448-1451). */
+int ddo_assemble_ions(const ddo_device *dev, const double *full5, double g_frac,
+                      const double *logs, int bernoulli, double *F, double *A, double *B,
+                      double *C, double *U) {
+    problem P;
+    problem_init(&P, dev, bernoulli);
+    const int NP = P.NP;
+    const double Vt = dev->Vt;
+    double *full3 = alloc_d((size_t)(NP + 2) * 3);
+    double *la_pt = alloc_d((size_t)NP), *lc_pt = alloc_d((size_t)NP);
+    double *F3 = alloc_d((size_t)NP * 3), *U3 = alloc_d((size_t)NP * 6);
+    double *A3 = alloc_d((size_t)NP * 9), *B3 = alloc_d((size_t)NP * 9), *C3 = alloc_d((size_t)NP * 9);
+    for (int i = 0; i < NP + 2; ++i)
+        for (int k = 0; k < 3; ++k) full3[3 * i + k] = full5[5 * i + k];
+    for (int r = 0; r < NP; ++r) {
+        const double *y = full5 + 5 * (r + 1);
+        la_pt[r] = y[3] - y[0] / Vt;
+        lc_pt[r] = y[4] + y[0] / Vt;
+    }
+    assemble_pt(&P, full3, g_frac, 0.0, 0.0, la_pt, lc_pt, 1.0, F3, A3, B3, C3, U3);
+    for (int r = 0; r < NP; ++r) {
+        const double *y = full5 + 5 * (r + 1);
+        double *f = F + 5 * r, *a = A + 25 * r, *b = B + 25 * r, *c = C + 25 * r, *u = U + 10 * r;
+        for (int k = 0; k < 25; ++k) a[k] = b[k] = c[k] = 0.0;
+        for (int k = 0; k < 10; ++k) u[k] = 0.0;
+        for (int row = 0; row < 3; ++row) {
+            for (int col = 0; col < 3; ++col) {
+                a[5 * row + col] = A3[9 * r + 3 * row + col];
+                b[5 * row + col] = B3[9 * r + 3 * row + col];
+                c[5 * row + col] = C3[9 * r + 3 * row + col];
+            }
+            f[row] = F3[3 * r + row];
+        }
+        const double a_m = P.am[r] * exp(y[3]), c_m = P.cm[r] * exp(y[4]);
+        b[0] += Q_E * (a_m + c_m) / Vt * P.sP[r];
+        b[3] = U3[6 * r + 0];
+        b[4] = U3[6 * r + 1];
+        const int in_a = P.am[r] > 0, in_c = P.cm[r] > 0;
+        f[3] = in_a ? y[3] - logs[0] + (-1.0 * y[0]) / Vt : y[3];
+        f[4] = in_c ? y[4] - logs[1] + (1.0 * y[0]) / Vt : y[4];
+        b[15] = in_a ? -1.0 / Vt : 0.0;
+        b[18] = 1.0;
+        b[20] = in_c ? 1.0 / Vt : 0.0;
+        b[24] = 1.0;
+        u[6] = in_a ? -1.0 : 0.0;
+        u[9] = in_c ? -1.0 : 0.0;
+    }
+    free(full3); free(la_pt); free(lc_pt); free(F3); free(U3); free(A3); free(B3); free(C3);
     problem_free(&P);
     return 0;
 }

@@ -66,6 +66,9 @@ int ddo_solve_jv(const ddo_device *dev, const double *V, int nV, double tol, int
  * U (NP x 3 x 2). la/lc are the ions' normalisations (the point logs of one region each). */
 int ddo_assemble(const ddo_device *dev, const double *full, double g_frac, double la, double lc,
                  int bernoulli, double *F, double *A, double *B, double *C, double *U);
+int ddo_assemble_ions(const ddo_device *dev, const double *full5, double g_frac,
+                      const double *logs, int bernoulli, double *F, double *A, double *B,
+                      double *C, double *U);
 
 /* _block_thomas: n systems in one call are not needed; one system of NP blocks b x b with k
  * right-hand sides, row-major (NP x b x b, NP x b x k). */

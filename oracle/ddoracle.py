@@ -68,6 +68,8 @@ def _load():
                                  ctypes.POINTER(ctypes.c_int32)]
     lib.ddo_assemble.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_int, _P, _P, _P, _P, _P]
+    lib.ddo_assemble_ions.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_double, _P,
+                                      ctypes.c_int, _P, _P, _P, _P, _P]
     lib.ddo_block_thomas.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]
     return lib
 
@@ -228,6 +230,23 @@ def assemble(dev, full, g_frac=None, la=0.0, lc=0.0, bernoulli_name="exact"):
     return F, A, B, C, U
 
 
+def assemble_ions(dev, full5, logs, g_frac=None, bernoulli_name="exact"):
+    """`_assemble_ions` at dti = 0 (src/jaxdd/solver.py:1364-1473) at a full state (NP + 2 x 5:
+    V, ln n, ln p, ln a, ln c) with the slots' logarithms ``logs`` (anion, cation): F (NP x 5),
+    A, B, C (NP x 5 x 5), U (NP x 5 x 2)."""
+    NP = len(dev.x) - 2
+    full5 = np.ascontiguousarray(full5, dtype=np.float64)
+    lg = np.ascontiguousarray(logs, dtype=np.float64)
+    F = np.zeros((NP, 5))
+    A, B, C = (np.zeros((NP, 5, 5)) for _ in range(3))
+    U = np.zeros((NP, 5, 2))
+    pin = _Pinned(dev, g_frac)
+    lib().ddo_assemble_ions(ctypes.byref(pin.struct), _ptr(full5), pin.struct.g_frac, _ptr(lg),
+                            BERNOULLI[bernoulli_name], _ptr(F), _ptr(A), _ptr(B), _ptr(C),
+                            _ptr(U))
+    return F, A, B, C, U
+
+
 def block_thomas(A, B, C, d):
     """`_block_thomas` for one system: blocks (NP x b x b), right-hand sides (NP x b x k)."""
     NP, b, _ = A.shape
@@ -246,6 +265,6 @@ def current(dev, full, g_frac=None):
     return lib().ddo_current(ctypes.byref(pin.struct), _ptr(full), pin.struct.g_frac)
 
 
-__all__ = ["assemble", "bernoulli", "block_thomas", "build", "current", "j_scale", "lib",
+__all__ = ["assemble", "assemble_ions", "bernoulli", "block_thomas", "build", "current", "j_scale", "lib",
            "solve_batch", "solve_jv", "solve_one"]
 

@@ -268,10 +268,23 @@ extern "C" int emu_assemble(int n, int NP, int nseg, const double *pt, const uin
                             const double *seg, const double *sc, const double *y,
                             const double *Vint, const double *logs, int bern, double *F,
                             double *A, double *B, double *C, double *U, double *J) {
-    AsmArgs a;
+    AsmArgs a = {};
     a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.logs = logs;
     a.F = F; a.A = A; a.B = B; a.C = C; a.U = U; a.J = J;
-    for (int d = 0; d < n; ++d) assemble_lane(a, d);
+    for (int d = 0; d < n; ++d) assemble_lane<false>(a, d);
+    return 0;
+}
+
+// The 5x5 assembly operator's lanes in turn.
+extern "C" int emu_assemble_ions(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
+                                 const double *seg, const double *sc, const double *y,
+                                 const double *Vint, const double *logs, int bern, double *F,
+                                 double *A, double *B, double *C, double *U) {
+    AsmArgs a = {};
+    a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern;
+    a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y5 = y; a.Vint = Vint; a.logs = logs;
+    a.F5 = F; a.A5 = A; a.B5 = B; a.C5 = C; a.U5 = U;
+    for (int d = 0; d < n; ++d) assemble_lane<true>(a, d);
     return 0;
 }

@@ -36,6 +36,8 @@ def lib():
                                      vp, vp]
         _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
                                       vp, vp]
+        _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
+                                           vp, vp]
     return _lib
 
 
@@ -105,3 +107,22 @@ def assemble_packed(packed, y, Vint, logs=None, bernoulli="exact"):
     return (F.transpose(2, 0, 1), A.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
             B.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2), C.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
             U.reshape(NP, 3, 2, n).transpose(3, 0, 1, 2), J)
+
+
+def assemble_ions_packed(packed, y, Vint, logs, bernoulli="exact"):
+    """The 5x5 assembly operator's CPU replay: y (n, NP, 5), Vint (n,), logs (n, 2) ->
+    F (n, NP, 5), A, B, C (n, NP, 5, 5), U (n, NP, 5, 2)."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    yl = np.ascontiguousarray(np.asarray(y, dtype=float).transpose(1, 2, 0))
+    Vd = np.ascontiguousarray(Vint, dtype=float)
+    lg = np.ascontiguousarray(np.asarray(logs, dtype=float).T)
+    F = np.zeros((NP, 5, n)); A = np.zeros((NP, 25, n)); B = np.zeros((NP, 25, n))
+    C = np.zeros((NP, 25, n)); U = np.zeros((NP, 10, n))
+    lib().emu_assemble_ions(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(yl), _p(Vd),
+                            _p(lg), 0 if bernoulli == "exact" else 1, _p(F), _p(A), _p(B),
+                            _p(C), _p(U))
+    return (F.transpose(2, 0, 1), A.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2),
+            B.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2), C.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2),
+            U.reshape(NP, 5, 2, n).transpose(3, 0, 1, 2))

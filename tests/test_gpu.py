@@ -67,6 +67,31 @@ def test_assemble_matches_oracle(torch_cuda, oracle, emu, case):
         assert np.all(np.abs(out["A"][k] - rep[1][k]) <= 1e-13 * rs[:, :, None])
 
 
+@pytest.mark.parametrize("bern", ["exact", "simss"])
+def test_assemble_ions_matches_oracle(torch_cuda, oracle, emu, bern):
+    """ddh_assemble_ions (the steady 5x5 system, ions as unknowns) against the oracle's
+    `_assemble_ions` restatement and against its own CPU replay, at ions off their Boltzmann
+    profiles."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import assemble_ions, pack, prepare
+    from test_kernel_emu import _asm5_state, check_assemble_ions
+
+    rng = np.random.default_rng(12)
+    devs = S.devices("c3", 5) + [fixture_device("steady_1e17_thin_light")]
+    devs = devs if bern == "exact" else devs[5:]
+    logs = np.array([[-30.0, -31.0], [-29.0, -32.5], [-33.0, -28.0], [-31.0, -30.0],
+                     [-30.5, -29.5], [40.0, 41.0]])[-len(devs):]
+    Vint = np.linspace(-0.2, 1.2, 6)[-len(devs):]
+    y, fulls = _asm5_state(devs, Vint, logs, rng)
+    out = assemble_ions(prepare(devs, "cuda"), y, Vint, logs, bern)
+    got = tuple(_np(out[k]) for k in ("F", "A", "B", "C", "U"))
+    check_assemble_ions(got, devs, fulls, logs, oracle, bern)
+    rep = emu.assemble_ions_packed(pack(devs), y, Vint, logs, bern)
+    rs = np.max(np.abs(np.concatenate(rep[1:4], axis=3)), axis=3) + 1e-300
+    for g, r in zip(got[1:4], rep[1:4]):  # the same arithmetic up to FMA contraction
+        assert np.all(np.abs(g - r) <= 1e-13 * rs[..., None])
+
+
 # ---- the block sweep operator (reference tests test_solver.py:1620-1641, 1753-1773) ------
 @pytest.mark.parametrize("b,k", [(3, 1), (3, 3), (5, 4), (3, 8)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])

@@ -198,6 +198,10 @@ def test_library_rejects_bad_shapes_before_any_launch():
     assert rc == -1 and b"nseg" in lib.ddh_last_error()
     rc = lib.ddh_assemble(None, None, None, None, 0, None, None, None, None, None, None, None)
     assert rc == -4
+    rc = lib.ddh_assemble_ions(ctypes.byref(b), 1, 1, 1, 0, 1, 1, 1, 1, 1, None)
+    assert rc == -1 and b"nseg" in lib.ddh_last_error()
+    rc = lib.ddh_assemble_ions(ctypes.byref(b), 1, 1, None, 0, 1, 1, 1, 1, 1, None)
+    assert rc == -4 and b"NULL" in lib.ddh_last_error()
     # lanes: 1, 2 or 4; four lanes need NP >= 16
     b = _lib.Batch(64, 400, 7, 0, 1, 1, 1, 1)
     o = _lib.Opts(1e-10, 60, 60, 0, 0, 1, 0, 3)

@@ -169,6 +169,53 @@ def test_replay_assemble_matches_oracle(emu, oracle, case):
         assert abs(J[k] - Jo) <= 1e-10 * (abs(Jo) + oracle.j_scale(dev)), (J[k], Jo)
 
 
+def _asm5_state(devs, Vint, logs, rng):
+    """5x5 interior states (n, NP, 5): `_asm_state`'s carriers and each ion's log density off
+    its Boltzmann profile of the slot's normalisation in its region (0 outside), and each
+    device's full state (NP + 2 x 5, the oracle's input)."""
+    y3, fulls3 = _asm_state(devs, Vint, rng)
+    ys, fulls = [], []
+    for dev, full, (la, lc) in zip(devs, fulls3, logs):
+        V = full[:, 0]
+        am, cm = np.asarray(dev.anion_mask) > 0, np.asarray(dev.cation_mask) > 0
+        lna = np.where(am, la + V / dev.Vt + rng.normal(0, 0.3, V.shape), 0.0)
+        lnc = np.where(cm, lc - V / dev.Vt + rng.normal(0, 0.3, V.shape), 0.0)
+        f5 = np.column_stack([full, lna, lnc])
+        ys.append(f5[1:-1])
+        fulls.append(f5)
+    return np.array(ys), fulls
+
+
+def check_assemble_ions(out, devs, fulls, logs, oracle, bern):
+    """The 5x5 operator's outputs (F, A, B, C, U as arrays (n, ...)) against the oracle's
+    `_assemble_ions` restatement at the same states, relative to each row's scale."""
+    F, A, B, C, U = out
+    for k, dev in enumerate(devs):
+        Fo, Ao, Bo, Co, Uo = oracle.assemble_ions(dev, fulls[k], logs[k], bernoulli_name=bern)
+        rs = np.max(np.abs(np.concatenate([Ao, Bo, Co], axis=2)), axis=2) + 1e-300
+        assert np.all(np.abs(F[k] - Fo) <= 1e-11 * (rs + np.abs(Fo)))
+        for got, ref in ((A[k], Ao), (B[k], Bo), (C[k], Co)):
+            assert np.all(np.abs(got - ref) <= 1e-12 * rs[:, :, None])
+        assert np.array_equal(U[k], Uo)
+
+
+@pytest.mark.parametrize("bern", ["exact", "simss"])
+def test_replay_assemble_ions_matches_oracle(emu, oracle, bern):
+    """The 5x5 assembly operator (ions as unknowns, steady) against the oracle's
+    `_assemble_ions` restatement: residual, blocks and the slots' columns."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    rng = np.random.default_rng(9)
+    devs = S.devices("c3", 3) + [fixture_device("steady_1e17_thin_light")]
+    devs = devs if bern == "exact" else devs[3:]
+    logs = np.array([[-30.0, -31.0], [-29.0, -32.5], [-33.0, -28.0], [40.0, 41.0]])[-len(devs):]
+    Vint = np.array([0.3, 0.9, -0.1, 0.5])[-len(devs):]
+    y, fulls = _asm5_state(devs, Vint, logs, rng)
+    out = emu.assemble_ions_packed(pack(devs), y, Vint, logs, bern)
+    check_assemble_ions(out, devs, fulls, logs, oracle, bern)
+
+
 @pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("ions", [False, True])
 def test_replay_odd_grid_matches_oracle(emu, oracle, ions, lanes):

@@ -221,3 +221,79 @@ def test_voltage_order_does_not_matter(oracle):
     down = oracle.solve_batch(devs, V_COARSE[::-1])
     for a, b in zip(up, down):
         assert np.array_equal(a, b[:, ::-1])
+
+
+# ---- the 5x5 steady system with the ions as unknowns (solver.py:1364-1473) ---------------
+def _ion_state(dev, full3, la, lc, rng, noise):
+    """(NP + 2) x 5: the carriers' state and each ion's log density, Boltzmann's of the slot's
+    normalisation in its region (plus noise), 0 outside it."""
+    V = full3[:, 0]
+    am, cm = np.asarray(dev.anion_mask) > 0, np.asarray(dev.cation_mask) > 0
+    lna = np.where(am, la + V / dev.Vt + rng.normal(0, noise, V.shape), 0.0)
+    lnc = np.where(cm, lc - V / dev.Vt + rng.normal(0, noise, V.shape), 0.0)
+    return np.column_stack([full3, lna, lnc])
+
+
+@pytest.mark.parametrize("case", ["steady_1e17_thin_light", "frozen_1e17_x1_thin_light"])
+def test_ion_system_reduces_to_the_3x3_at_boltzmann_ions(oracle, case):
+    """Ions in Boltzmann's profiles of the slots' normalisations: the 5x5 carriers' rows are the
+    3x3 assembly's with those normalisations, the ions' rows vanish in their regions and hold
+    the unknown outside them, and Poisson's V column with its ion columns through
+    d ln a/dV = 1/V_T, d ln c/dV = -1/V_T is the 3x3's V column (the chain rule)."""
+    dev = fixture_device(case)
+    full = _perturbed_state(dev)
+    la, lc = 40.0, 41.0
+    full5 = _ion_state(dev, full, la, lc, np.random.default_rng(3), 0.0)
+    F5, A5, B5, C5, U5 = oracle.assemble_ions(dev, full5, [la, lc])
+    F, A, B, C, U = oracle.assemble(dev, full, la=la, lc=lc)
+    rs = np.max(np.abs(np.concatenate([A, B, C], axis=2)), axis=2)  # each row's scale
+    assert np.all(np.abs(F5[:, :3] - F) <= 1e-12 * (rs + np.abs(F)))
+    for got, ref in ((A5, A), (C5, C)):
+        assert np.all(np.abs(got[:, :3, :3] - ref) <= 1e-12 * rs[:, :, None])
+        assert not got[:, 3:].any() and not got[:, :, 3:].any()
+    assert np.all(np.abs(B5[:, 1:3, :3] - B[:, 1:3]) <= 1e-12 * rs[:, 1:3, None])
+    chain = B5[:, 0, 0] + (B5[:, 0, 3] - B5[:, 0, 4]) / dev.Vt
+    assert np.all(np.abs(chain - B[:, 0, 0]) <= 1e-12 * rs[:, 0])
+    assert np.all(np.abs(B5[:, 0, 1:3] - B[:, 0, 1:3]) <= 1e-12 * rs[:, 0, None])
+    am, cm = np.asarray(dev.anion_mask)[1:-1] > 0, np.asarray(dev.cation_mask)[1:-1] > 0
+    assert am.any() and cm.any() and not am.all()
+    assert np.all(np.abs(F5[am, 3]) < 1e-12 * (la + np.abs(full5[1:-1][am, 3])))
+    assert np.all(np.abs(F5[cm, 4]) < 1e-12 * (lc + np.abs(full5[1:-1][cm, 4])))
+    assert np.array_equal(F5[~am, 3], full5[1:-1][~am, 3])
+    assert np.array_equal(F5[~cm, 4], full5[1:-1][~cm, 4])
+
+
+@pytest.mark.parametrize("bern", ["exact", "simss"])
+def test_ion_system_jacobian_is_the_residuals_derivative(oracle, bern):
+    """The 5x5 blocks and the slots' columns against central differences of the 5x5 residual,
+    at ions off their Boltzmann profiles (test_solver.py:2077-2112 checks the reference's
+    blocks against its autodiff)."""
+    dev = fixture_device("steady_1e17_thin_light")
+    la, lc = 40.0, 41.0
+    full5 = _ion_state(dev, _perturbed_state(dev), la, lc, np.random.default_rng(4), 0.4)
+    NP = len(dev.x) - 2
+    F5, A5, B5, C5, U5 = oracle.assemble_ions(dev, full5, [la, lc], bernoulli_name=bern)
+    rng = np.random.default_rng(8)
+    for i in rng.choice(np.arange(1, NP + 1), 10, replace=False):
+        for col in range(5):
+            h = 1e-7 * max(1.0, abs(full5[i, col]))
+            fp, fm = full5.copy(), full5.copy()
+            fp[i, col] += h
+            fm[i, col] -= h
+            dF = (oracle.assemble_ions(dev, fp, [la, lc], bernoulli_name=bern)[0]
+                  - oracle.assemble_ions(dev, fm, [la, lc], bernoulli_name=bern)[0]) / (2 * h)
+            r = i - 1
+            for row_pt, blk in ((r, B5), (r + 1, A5), (r - 1, C5)):
+                if 0 <= row_pt < NP:
+                    got, want = blk[row_pt, :, col], dF[row_pt]
+                    scale = np.max(np.abs(blk[row_pt])) + 1e-300  # as the 3x3 test's
+                    tol = 1e-5 if bern == "exact" else 2e-3
+                    assert np.max(np.abs(got - want)) < tol * scale, (i, col)
+    for k in range(2):
+        h = 1e-6
+        lp, lm = [la, lc], [la, lc]
+        lp[k] += h
+        lm[k] -= h
+        dF = (oracle.assemble_ions(dev, full5, lp, bernoulli_name=bern)[0]
+              - oracle.assemble_ions(dev, full5, lm, bernoulli_name=bern)[0]) / (2 * h)
+        assert np.max(np.abs(U5[:, :, k] - dF)) < 1e-8
