@@ -65,9 +65,17 @@ struct AdjArgs {
     double *st;             // [NP][3][ls] the state's V, n, p
     double *lam;            // [NP][3][ls]
     double *iu;             // [NP + 1][ls] the current's running integral of U
+    double *ts;             // [NP + 2][2][ls] the current's suffix sums of the point weights
     size_t ls;
+    int passes;             // 1: one pass per direction even where a reverse pass would do
 };
 constexpr int ADJ_F = 21;
+// Directions contracted in one reverse pass (more go through one pass per direction), and the
+// points whose tangents the wavefront stages in LDS at a time for the contraction.
+constexpr int ADJ_MAX_K = 16;
+constexpr int ADJ_CHUNK = 4;
+// The points' segment ids in the table: bytes, (NP + 2 + 7) / 8 doubles' room.
+DDH_MFN int adj_sid_words(int NP) { return (NP + 2 + 7) / 8; }
 // Workgroups of the adjoint launch (one wavefront each, looping over the devices): the
 // per-lane workspace is sized by them, not by the batch.
 constexpr int ADJ_SLOTS = 1024;
@@ -89,23 +97,40 @@ struct SegD {
     }
 };
 
-// The wavefront's LDS table: the device's segment fields and their tangents along the current
-// direction, its scalars and their tangents, its points' (h, x, G, fixed ions) and their
-// tangents, and its points' segment ids.
+// The wavefront's LDS table. K = 0 (one pass per direction): the device's segment fields and
+// their tangents along the current direction, its scalars and their tangents, its points'
+// (h, x, G, fixed ions) and their tangents, and its points' segment ids. K > 0 (the reverse
+// pass): the values, the segment ids, then the segment fields' and the scalars' tangents of all
+// K directions ([k][field]; the points' tangents are read from global memory).
 struct AdjTab {
-    int S, sct, ptv, ptt, sid, size;
-    DDH_FN AdjTab(int nseg, int NP) {
+    int S, sct, ptv, ptt, sid, segT, scT, size;
+    DDH_FN AdjTab(int nseg, int NP, int K = 0) {
         S = nseg * SEG_N;
-        sct = 2 * S;
-        ptv = sct + 2 * SC_N;
-        ptt = ptv + (NP + 2) * PT_N;
-        sid = ptt + (NP + 2) * PT_N;
-        size = sid + NP + 2;
+        if (K == 0) {
+            sct = 2 * S;
+            ptv = sct + 2 * SC_N;
+            ptt = ptv + (NP + 2) * PT_N;
+            sid = ptt + (NP + 2) * PT_N;
+            segT = scT = size = sid + adj_sid_words(NP);
+        } else {
+            sct = S;
+            ptv = sct + SC_N;
+            ptt = ptv;
+            sid = ptv + (NP + 2) * PT_N;
+            segT = sid + adj_sid_words(NP);
+            scT = segT + K * S;
+            ptt = scT + K * SC_N;  // a chunk of ADJ_CHUNK points' tangents, [k][point][field]
+            size = ptt + K * ADJ_CHUNK * PT_N;
+        }
     }
 };
-DDH_MFN int adj_table_size(int nseg, int NP) {
-    return 2 * nseg * SEG_N + 2 * SC_N + 2 * (NP + 2) * PT_N + NP + 2;
+DDH_MFN int adj_table_size(int nseg, int NP, int K = 0) {
+    return K == 0 ? 2 * nseg * SEG_N + 2 * SC_N + 2 * (NP + 2) * PT_N + adj_sid_words(NP)
+                  : nseg * SEG_N + SC_N + (NP + 2) * PT_N + adj_sid_words(NP) +
+                        K * (nseg * SEG_N + SC_N + ADJ_CHUNK * PT_N);
 }
+// The directions of one reverse pass: all of them up to ADJ_MAX_K, else none (one pass each).
+DDH_MFN int adj_k(int ntheta, int passes) { return (passes || ntheta > ADJ_MAX_K) ? 0 : ntheta; }
 
 DDH_FN D1 recomb_d1(D1 n, D1 p, const SegD &sd, int id) {
     const D1 np_ = n * p;
@@ -172,11 +197,23 @@ DDH_FN void inverse3d(const double *m, double *out) { inverse3<double>(m, out); 
 // together; the caller waits for all of them before the table is read).
 DDH_FN void adj_stage_values(const AdjArgs &a, int d, ldouble *tab, int lane, int nl) {
     const size_t n = (size_t)a.n;
-    const AdjTab t(a.nseg, a.NP);
+    const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
     for (int q = lane; q < t.S; q += nl) tab[q] = a.seg[(size_t)q * n + d];
     for (int q = lane; q < SC_N; q += nl) tab[t.sct + q] = a.sc[(size_t)q * n + d];
     for (int q = lane; q < (a.NP + 2) * PT_N; q += nl) tab[t.ptv + q] = a.pt[(size_t)q * n + d];
-    for (int q = lane; q < a.NP + 2; q += nl) tab[t.sid + q] = (double)a.sid[(size_t)q * n + d];
+    DDH_LDS uint8_t *sb = (DDH_LDS uint8_t *)(tab + t.sid);
+    for (int q = lane; q < a.NP + 2; q += nl) sb[q] = a.sid[(size_t)q * n + d];
+    if (t.segT < t.size) {  // the reverse pass: every direction's segment and scalar tangents
+        const int K = adj_k(a.ntheta, a.passes);
+        for (int q = lane; q < K * t.S; q += nl) {
+            const int k = q / t.S, f = q - k * t.S;
+            tab[t.segT + q] = a.dseg[((size_t)k * t.S + f) * n + d];
+        }
+        for (int q = lane; q < K * SC_N; q += nl) {
+            const int k = q / SC_N, f = q - k * SC_N;
+            tab[t.scT + q] = a.dsc[((size_t)k * SC_N + f) * n + d];
+        }
+    }
 }
 
 // Copies direction k's tangents of device d into the LDS table (as adj_stage_values).
@@ -193,18 +230,19 @@ DDH_FN void adj_stage_direction(const AdjArgs &a, int d, int k, ldouble *tab, in
 
 #define TSC(f) tab[t.sct + (f)]
 #define TPT(j, f) tab[t.ptv + (j) * PT_N + (f)]
-#define TSID(j) ((int)tab[t.sid + (j)])
+#define TSID(j) ((int)((DDH_LDS const uint8_t *)(tab + t.sid))[j])
 #define TSEG(id, f) tab[(id) * SEG_N + (f)]
 #define FAC(i, q) a.fac[((size_t)((i) - 1) * ADJ_F + (q)) * ls + sl]
 #define STV(j, f) a.st[((size_t)((j) - 1) * 3 + (f)) * ls + sl]
 #define LAM(i, f) a.lam[((size_t)((i) - 1) * 3 + (f)) * ls + sl]
 #define IU(j) a.iu[(size_t)(j) * ls + sl]
+#define TS(j, f) a.ts[((size_t)(j) * 2 + (f)) * ls + sl]
 #define YS(j, f) a.states[(((size_t)v * NP + ((j) - 1)) * 3 + (f)) * n + d]
 
 // Device d at converged voltage v, by the launched lane sl: lambda (T^T lambda = dJ/dy) and
 // the state's V, n, p, into the lane's workspace.
 DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
-    const AdjTab t(a.nseg, a.NP);
+    const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
     const int NP = a.NP;
     const size_t n = (size_t)a.n, ls = a.ls;
     const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), L2 = L * L;
@@ -243,8 +281,10 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
     const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
     const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
     const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
-    // J = cJ (Kn + Kp): its partials in the four sums
+    // J = cJ (Kn + Kp): its partials in the four sums (the sums kept for the reverse pass in
+    // the suffix sums' unused ends)
     const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
+    TS(0, 0) = Sn; TS(0, 1) = Sp; TS(NP + 1, 0) = Dn; TS(NP + 1, 1) = Dp;
 
     // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
     auto point = [&](int j, Pt &q) {
@@ -300,6 +340,8 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         const double wn = exp(-(mid.V + mid.offn) * iVt) * TSEG(mid.sid, SEG_IMUN);
         const double wp = exp((mid.V + mid.offp) * iVt) * TSEG(mid.sid, SEG_IMUP);
         const double Tn = ai * wn + Tn1, Tp = ai * wp + Tp1;
+        TS(i, 0) = Tn;
+        TS(i, 1) = Tp;
         const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
                           bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
         const double iu = IU(i);
@@ -424,6 +466,279 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
         return J.d - acc;
     }
 }
+// dJ/dtheta_k for every direction k < K at device d's converged voltage v, in one reverse pass
+// (lambda, the state and the current's sums and suffix sums from adj_lambda): the partials of
+// J - lambda . F in every segment field, scalar and point field that the rows and the current
+// read, the state held. The segment fields' and scalars' are contracted at once with the K
+// directions' tangents (from the LDS table) into gk; the points' are kept in the lane's
+// workspace for adj_contract_points. The same derivative as adj_direction's K passes: the rows
+// of row_d1 and the current of adj_lambda, reversed.
+DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab, double *gk) {
+    (void)d;
+    const int NP = a.NP, K = adj_k(a.ntheta, a.passes);
+    const size_t ls = a.ls;
+    const AdjTab t(a.nseg, NP, K);
+    const int S = t.S;
+    const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), L2 = L * L;
+    const double gfr = TSC(SC_GFRAC), elec = TSC(SC_ELEC), cJ = -elec * Q_E;
+    const int sid0 = TSID(0), sidN = TSID(NP + 1);
+    const double Vint = a.V[v];
+    const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
+    auto seg_now = [&](int id, int f, double b) {
+        lcdouble *tg = tab + t.segT + id * SEG_N + f;
+#pragma unroll
+        for (int k = 0; k < ADJ_MAX_K; ++k)
+            if (k < K) gk[k] += b * tg[k * S];
+    };
+    // a segment's partials are summed over its run of points (a layer's interior) and
+    // contracted once the run ends
+    double acc[SEG_N];
+#pragma unroll
+    for (int f = 0; f < SEG_N; ++f) acc[f] = 0.0;
+    int run = -1;
+    auto flush = [&]() {
+        if (run >= 0) {
+#pragma unroll
+            for (int f = 0; f < SEG_N; ++f) {
+                seg_now(run, f, acc[f]);
+                acc[f] = 0.0;
+            }
+        }
+    };
+    auto seg_add = [&](int id, int f, double b) {
+        if (id == run) acc[f] += b;
+        else seg_now(id, f, b);
+    };
+    auto sc_add = [&](int f, double b) {
+        lcdouble *tg = tab + t.scT + f;
+#pragma unroll
+        for (int k = 0; k < ADJ_MAX_K; ++k)
+            if (k < K) gk[k] += b * tg[k * SC_N];
+    };
+    // the points' partials: into the lane's workspace (the elimination's factors, free once
+    // lambda is known), [point 0..NP][field], contracted by adj_contract_points
+    auto bp = [&](int j, int f) -> double & { return a.fac[((size_t)j * PT_N + f) * ls + sl]; };
+    double pendH = 0.0, pendX = 0.0;  // point j - 1's h and x partials, from rows j - 1, j
+    bp(0, PT_G) = 0.0;
+    bp(0, PT_FIX) = 0.0;
+    // J = cJ (Kn + Kp) and its partials in the sums and the contacts' terms
+    const double Sn = TS(0, 0), Sp = TS(0, 1), Dn = TS(NP + 1, 0), Dp = TS(NP + 1, 1);
+    const double Vg0n = V0c + TSEG(sid0, SEG_OFFN), Vg0p = V0c + TSEG(sid0, SEG_OFFP);
+    const double endn0 = exp(TSC(SC_LNL) - Vg0n * iVt), endp0 = exp(TSC(SC_LPL) + Vg0p * iVt);
+    const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
+    const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
+    const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
+    const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
+    const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
+    {
+        double e = cJ * Vt / Sn * endn0;  // endn0 = exp(LNL - (V0 + offn) / V_T)
+        sc_add(SC_LNL, e); sc_add(SC_V0, -e * iVt); seg_add(sid0, SEG_OFFN, -e * iVt);
+        e = -cJ * Vt / Sp * endp0;        // endp0 = exp(LPL + (V0 + offp) / V_T)
+        sc_add(SC_LPL, e); sc_add(SC_V0, e * iVt); seg_add(sid0, SEG_OFFP, e * iVt);
+        e = -cJ * Vt / Sn * endnN;
+        sc_add(SC_LNR, e); sc_add(SC_VL, -e * iVt); seg_add(sidN, SEG_OFFN, -e * iVt);
+        e = cJ * Vt / Sp * endpN;
+        sc_add(SC_LPR, e); sc_add(SC_VL, e * iVt); seg_add(sidN, SEG_OFFP, e * iVt);
+    }
+    // the edge fluxes reversed: partials of bFn Fn + bFp Fp over (l, r) (edge_d1)
+    auto edge_rev = [&](int sl_, int sr_, double Vl, double nl, double pl, double Vr, double nr,
+                        double pr, bool lc, bool rc, double bFn, double bFp) {
+        const double mun = TSEG(sl_, SEG_MUN), mup = TSEG(sl_, SEG_MUP);
+        const double lcn = Vt * mun, lcp = Vt * mup;
+        const double xn = ((Vr + TSEG(sr_, SEG_OFFN)) - (Vl + TSEG(sl_, SEG_OFFN))) * iVt;
+        const double xp = ((Vr + TSEG(sr_, SEG_OFFP)) - (Vl + TSEG(sl_, SEG_OFFP))) * iVt;
+        double Bnf, Bnb, dBnf, dBnb, Bpf, Bpb, dBpf, dBpb;
+        bernoulli_pair(xn, a.bern, Bnf, Bnb, dBnf, dBnb);
+        bernoulli_pair(xp, a.bern, Bpf, Bpb, dBpf, dBpb);
+        seg_add(sl_, SEG_MUN, bFn * (Bnf * nr - Bnb * nl) * Vt);
+        seg_add(sl_, SEG_MUP, bFp * (Bpf * pl - Bpb * pr) * Vt);
+        const double bxn = bFn * lcn * nr * dBnf + bFn * lcn * nl * dBnb;
+        const double bxp = bFp * lcp * pl * dBpf + bFp * lcp * pr * dBpb;
+        seg_add(sr_, SEG_OFFN, bxn * iVt); seg_add(sl_, SEG_OFFN, -bxn * iVt);
+        seg_add(sr_, SEG_OFFP, bxp * iVt); seg_add(sl_, SEG_OFFP, -bxp * iVt);
+        if (lc) {  // the left contact: V0 and its densities
+            sc_add(SC_V0, -(bxn + bxp) * iVt);
+            sc_add(SC_LNL, -bFn * lcn * Bnb * nl);
+            sc_add(SC_LPL, bFp * lcp * Bpf * pl);
+        }
+        if (rc) {
+            sc_add(SC_VL, (bxn + bxp) * iVt);
+            sc_add(SC_LNR, bFn * lcn * Bnf * nr);
+            sc_add(SC_LPR, -bFp * lcp * Bpb * pr);
+        }
+    };
+    // the current's weights at the left contact (iu = 0 there)
+    const double x0 = TPT(0, PT_X), x1 = TPT(1, PT_X);
+    {
+        const double en = exp(-Vg0n * iVt), ep = exp(Vg0p * iVt);
+        const double bwn = bSn * 0.5 * (x1 - x0), bwp = bSp * 0.5 * (x1 - x0);
+        const double wn = en * TSEG(sid0, SEG_IMUN), wp = ep * TSEG(sid0, SEG_IMUP);
+        seg_add(sid0, SEG_IMUN, bwn * en); seg_add(sid0, SEG_IMUP, bwp * ep);
+        sc_add(SC_V0, -bwn * wn * iVt + bwp * wp * iVt);
+        seg_add(sid0, SEG_OFFN, -bwn * wn * iVt); seg_add(sid0, SEG_OFFP, bwp * wp * iVt);
+    }
+    double wn_prev = exp(-Vg0n * iVt) * TSEG(sid0, SEG_IMUN);
+    double wp_prev = exp(Vg0p * iVt) * TSEG(sid0, SEG_IMUP);
+    double U_prev = 0.0, iu_prev = 0.0, x_prev = x0;
+    double bL = 0.0;
+    for (int j = 1; j <= NP; ++j) {
+        const bool lc = j == 1, rc = j == NP;
+        const int sl_ = TSID(j - 1), sm = TSID(j), sr_ = TSID(j + 1);
+        if (sm != run) {
+            flush();
+            run = sm;
+        }
+        const double Vl = lc ? V0c : STV(j - 1, 0), nl = lc ? exp(TSC(SC_LNL)) : STV(j - 1, 1);
+        const double pl = lc ? exp(TSC(SC_LPL)) : STV(j - 1, 2);
+        const double Vm = STV(j, 0), nm = STV(j, 1), pm = STV(j, 2);
+        const double Vr = rc ? VLc : STV(j + 1, 0), nr = rc ? exp(TSC(SC_LNR)) : STV(j + 1, 1);
+        const double pr = rc ? exp(TSC(SC_LPR)) : STV(j + 1, 2);
+        const double hm = TPT(j - 1, PT_H), hp = TPT(j, PT_H), hs = hp + hm;
+        // the row's values (row_d1)
+        const double fac_ = 0.5 * L2 * hp * hm * hs;
+        const double meps = TSEG(sm, SEG_EPS), deps = meps - TSEG(sl_, SEG_EPS);
+        const double fu = 2.0 * meps / (hp * hs * L2), fl0 = 2.0 * meps / (hm * hs * L2);
+        const double jump = deps != 0.0 ? deps / (hm * hm * L2) : 0.0;
+        const double fac_m = jump - (fu + fl0), fac_l = fl0 - jump;
+        const double sP = TSEG(sm, SEG_SP), scs = TSEG(sm, SEG_SCS), sC = scs / hp;
+        const double G = TPT(j, PT_G), gen = G * gfr;
+        const double kd = TSEG(sm, SEG_KD), ni2 = TSEG(sm, SEG_NI2), Nt = TSEG(sm, SEG_NT);
+        const double Cn = TSEG(sm, SEG_CN), Cp = TSEG(sm, SEG_CP);
+        const double nt0 = TSEG(sm, SEG_NT0), pt0 = TSEG(sm, SEG_PT0);
+        const double R = recombination_value(nm, pm, kd, ni2, Nt, Cn, Cp, nt0, pt0);
+        double FnL, FpL, FnR, FpR;
+        {
+            double Bf, Bb, dBf, dBb;
+            const double xnL = ((Vm + TSEG(sm, SEG_OFFN)) - (Vl + TSEG(sl_, SEG_OFFN))) * iVt;
+            bernoulli_pair(xnL, a.bern, Bf, Bb, dBf, dBb);
+            FnL = Vt * TSEG(sl_, SEG_MUN) * (Bf * nm - Bb * nl);
+            const double xpL = ((Vm + TSEG(sm, SEG_OFFP)) - (Vl + TSEG(sl_, SEG_OFFP))) * iVt;
+            bernoulli_pair(xpL, a.bern, Bf, Bb, dBf, dBb);
+            FpL = Vt * TSEG(sl_, SEG_MUP) * (Bf * pl - Bb * pm);
+            const double xnR = ((Vr + TSEG(sr_, SEG_OFFN)) - (Vm + TSEG(sm, SEG_OFFN))) * iVt;
+            bernoulli_pair(xnR, a.bern, Bf, Bb, dBf, dBb);
+            FnR = Vt * TSEG(sm, SEG_MUN) * (Bf * nr - Bb * nm);
+            const double xpR = ((Vr + TSEG(sr_, SEG_OFFP)) - (Vm + TSEG(sm, SEG_OFFP))) * iVt;
+            bernoulli_pair(xpR, a.bern, Bf, Bb, dBf, dBb);
+            FpR = Vt * TSEG(sm, SEG_MUP) * (Bf * pm - Bb * pr);
+        }
+        const double elec_ = hm * FnR - hp * FnL + fac_ * (gen - R);
+        const double hole = hp * FpL - hm * FpR + fac_ * (gen - R);
+        const double pois = fac_l * Vl + fac_m * Vm + fu * Vr -
+                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + TPT(j, PT_FIX));
+        // the row reversed, seeded with -lambda_j
+        const double b0 = -LAM(j, 0), b1 = -LAM(j, 1), b2 = -LAM(j, 2);
+        const double bpois = b0 * sP;
+        seg_add(sm, SEG_SP, b0 * pois);
+        const double belec = b1 * sC, bhole = b2 * sC, bsC = b1 * elec_ + b2 * hole;
+        seg_add(sm, SEG_SCS, bsC / hp);
+        double bhp = -bsC * scs / (hp * hp), bhm = 0.0;
+        const double bfac_l = bpois * Vl, bfac_m = bpois * Vm;
+        double bfu = bpois * Vr;
+        if (lc) sc_add(SC_V0, bpois * fac_l);
+        if (rc) sc_add(SC_VL, bpois * fu);
+        seg_add(sm, SEG_DOP, -Q_E * bpois);
+        bp(j, PT_FIX) = -Q_E * bpois;
+        const double bjump = bfac_m - bfac_l;
+        bfu -= bfac_m;
+        const double bfl0 = bfac_l - bfac_m;
+        if (deps != 0.0) {
+            const double bdeps = bjump / (hm * hm * L2);
+            seg_add(sm, SEG_EPS, bdeps);
+            seg_add(sl_, SEG_EPS, -bdeps);
+            bhm += -2.0 * bjump * jump / hm;
+            bL += -2.0 * bjump * jump / L;
+        }
+        seg_add(sm, SEG_EPS, bfu * 2.0 / (hp * hs * L2) + bfl0 * 2.0 / (hm * hs * L2));
+        bhp += -bfu * fu * (1.0 / hp + 1.0 / hs) - bfl0 * fl0 / hs;
+        bhm += -bfu * fu / hs - bfl0 * fl0 * (1.0 / hm + 1.0 / hs);
+        bL += -2.0 * (bfu * fu + bfl0 * fl0) / L;
+        bhm += belec * FnR - bhole * FpR;
+        bhp += -belec * FnL + bhole * FpL;
+        const double bfac = (belec + bhole) * (gen - R);
+        bhp += bfac * 0.5 * L2 * hm * (hs + hp);
+        bhm += bfac * 0.5 * L2 * hp * (hs + hm);
+        bL += bfac * L * hp * hm * hs;
+        double bR = -(belec + bhole) * fac_;
+        double bG = (belec + bhole) * fac_ * gfr;
+        edge_rev(sl_, sm, Vl, nl, pl, Vm, nm, pm, lc, false, -belec * hp, bhole * hp);
+        edge_rev(sm, sr_, Vm, nm, pm, Vr, nr, pr, false, rc, belec * hm, -bhole * hm);
+        // the current at point j (adj_lambda's sums)
+        const double xj = TPT(j, PT_X), dx = xj - x_prev;
+        const double dxn = j < NP ? TPT(j + 1, PT_X) - xj : 0.0;
+        const double aj = 0.5 * (dx + dxn);
+        const double en = exp(-(Vm + TSEG(sm, SEG_OFFN)) * iVt), ep = exp((Vm + TSEG(sm, SEG_OFFP)) * iVt);
+        const double wn = en * TSEG(sm, SEG_IMUN), wp = ep * TSEG(sm, SEG_IMUP);
+        const double U = R - G * gfr, iu = IU(j);
+        const double Tn = TS(j, 0), Tp = TS(j, 1);
+        const double Tn1 = j < NP ? TS(j + 1, 0) : 0.0, Tp1 = j < NP ? TS(j + 1, 1) : 0.0;
+        const double bwn = (bSn + bDn * iu) * aj, bwp = (bSp + bDp * iu) * aj;
+        seg_add(sm, SEG_IMUN, bwn * en); seg_add(sm, SEG_IMUP, bwp * ep);
+        seg_add(sm, SEG_OFFN, -bwn * wn * iVt); seg_add(sm, SEG_OFFP, bwp * wp * iVt);
+        const double bU = bDn * (0.5 * dx * Tn + 0.5 * dxn * Tn1) + bDp * (0.5 * dx * Tp + 0.5 * dxn * Tp1);
+        bR += bU;
+        bG -= bU * gfr;
+        // interval j (x_{j-1}, x_j)
+        const double bdx = bSn * 0.5 * (wn_prev + wn) + bDn * (0.5 * (wn_prev * iu_prev + wn * iu) + 0.5 * (U_prev + U) * Tn) +
+                           bSp * 0.5 * (wp_prev + wp) + bDp * (0.5 * (wp_prev * iu_prev + wp * iu) + 0.5 * (U_prev + U) * Tp);
+        bp(j, PT_G) = bG;
+        bp(j - 1, PT_H) = pendH + bhm;
+        bp(j - 1, PT_X) = pendX - bdx;
+        pendH = bhp;
+        pendX = bdx;
+        // the recombination's parameters (recomb_d1), the state held
+        const double np_ = nm * pm;
+        seg_add(sm, SEG_KD, bR * (np_ - ni2));
+        seg_add(sm, SEG_NI2, -bR * kd);
+        if (Nt > 0) {
+            const double X = np_ - nt0 * pt0, den = Cn * (nm + nt0) + Cp * (pm + pt0);
+            const double T = Nt * Cn * Cp * X / den;
+            seg_add(sm, SEG_NT, bR * Cn * Cp * X / den);
+            seg_add(sm, SEG_CN, bR * (Nt * Cp * X / den - T * (nm + nt0) / den));
+            seg_add(sm, SEG_CP, bR * (Nt * Cn * X / den - T * (pm + pt0) / den));
+            seg_add(sm, SEG_NT0, bR * (-Nt * Cn * Cp * pt0 / den - T * Cn / den));
+            seg_add(sm, SEG_PT0, bR * (-Nt * Cn * Cp * nt0 / den - T * Cp / den));
+        }
+        wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
+    }
+    bp(NP, PT_H) = pendH;
+    bp(NP, PT_X) = pendX;
+    flush();
+    sc_add(SC_L, bL);
+}
+
+// The points' part of the contraction, for points c0 .. c0 + ADJ_CHUNK - 1 (at most NP): their
+// partials from the lane's workspace against the chunk's tangents the wavefront staged in LDS.
+DDH_FN void adj_contract_points(const AdjArgs &a, int c0, size_t sl, lcdouble *tab, double *gk) {
+    const int NP = a.NP, K = adj_k(a.ntheta, a.passes);
+    const size_t ls = a.ls;
+    const AdjTab t(a.nseg, NP, K);
+    for (int jj = 0; jj < ADJ_CHUNK && c0 + jj <= NP; ++jj) {
+#pragma unroll
+        for (int f = 0; f < PT_N; ++f) {
+            const double b = a.fac[((size_t)(c0 + jj) * PT_N + f) * ls + sl];
+            lcdouble *tg = tab + t.ptt + jj * PT_N + f;
+#pragma unroll
+            for (int k = 0; k < ADJ_MAX_K; ++k)
+                if (k < K) gk[k] += b * tg[k * ADJ_CHUNK * PT_N];
+        }
+    }
+}
+
+// Stages the tangents of points c0 .. c0 + ADJ_CHUNK - 1 of every direction (as
+// adj_stage_values: lane `lane` of `nl`).
+DDH_FN void adj_stage_points(const AdjArgs &a, int d, int c0, ldouble *tab, int lane, int nl) {
+    const int NP = a.NP, K = adj_k(a.ntheta, a.passes);
+    const size_t n = (size_t)a.n;
+    const AdjTab t(a.nseg, NP, K);
+    for (int q = lane; q < K * ADJ_CHUNK * PT_N; q += nl) {
+        const int k = q / (ADJ_CHUNK * PT_N), r = q - k * ADJ_CHUNK * PT_N;
+        const int j = c0 + r / PT_N, f = r % PT_N;
+        tab[t.ptt + q] = j <= NP + 1 ? a.dpt[(((size_t)k * (NP + 2) + j) * PT_N + f) * n + d] : 0.0;
+    }
+}
+#undef TS
+
 #undef TSC
 #undef TPT
 #undef TSID

@@ -190,6 +190,24 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
             if (v < nV && !act)
                 for (int k = 0; k < K; ++k) out[k] = mobile ? NAN : 0.0;
             if (act) adj_lambda(a, d, v, sl, tab);
+            if (adj_k(K, a.passes) > 0) {  // every direction in one reverse pass
+                double g[ADJ_MAX_K];
+#pragma unroll
+                for (int k = 0; k < ADJ_MAX_K; ++k) g[k] = 0.0;
+                if (act) adj_contract(a, d, v, sl, tab, g);
+                for (int c0 = 0; c0 <= a.NP; c0 += ADJ_CHUNK) {
+                    __syncthreads();  // every lane is done with the previous chunk
+                    adj_stage_points(a, d, c0, tab, lane, 64);
+                    __syncthreads();
+                    if (act) adj_contract_points(a, c0, sl, tab, g);
+                }
+                if (act) {
+#pragma unroll
+                    for (int k = 0; k < ADJ_MAX_K; ++k)
+                        if (k < K) out[k] = g[k];
+                }
+                continue;
+            }
             for (int k = 0; k < K; ++k) {
                 __syncthreads();  // every lane is done with the previous direction's tangents
                 adj_stage_direction(a, d, k, tab, lane, 64);
@@ -442,7 +460,7 @@ extern "C" int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32
 }
 
 struct AdjLayout {
-    size_t fac, st, lam, iu, total, slots, ls;
+    size_t fac, st, lam, iu, ts, total, slots, ls;
 };
 
 // Per launched lane (ADJ_SLOTS workgroups of 64 at most): the factors, the state's V, n, p,
@@ -457,6 +475,7 @@ static AdjLayout adj_layout(int32_t n, int32_t NP, int32_t nV) {
     w.st = off; off = align256(off + (size_t)NP * 3 * w.ls * sizeof(double));
     w.lam = off; off = align256(off + (size_t)NP * 3 * w.ls * sizeof(double));
     w.iu = off; off = align256(off + (size_t)(NP + 1) * w.ls * sizeof(double));
+    w.ts = off; off = align256(off + (size_t)(NP + 2) * 2 * w.ls * sizeof(double));
     w.total = off;
     return w;
 }
@@ -491,10 +510,13 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     a.st = (double *)(base + w.st);
     a.lam = (double *)(base + w.lam);
     a.iu = (double *)(base + w.iu);
+    a.ts = (double *)(base + w.ts);
     a.ls = w.ls;
+    a.passes = 0;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((unsigned)w.slots), block(64);
-    const size_t lds = (size_t)ddh::adj_table_size(b->nseg, b->NP) * sizeof(double);
+    const size_t lds =
+        (size_t)ddh::adj_table_size(b->nseg, b->NP, ddh::adj_k(ntheta, 0)) * sizeof(double);
     if (lds > 160 * 1024) return fail(DDH_E_SHAPE, "ddh_adjoint: NP too large for the LDS table");
     (void)hipFuncSetAttribute((const void *)ddh::adjoint_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

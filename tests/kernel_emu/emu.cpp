@@ -222,6 +222,9 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
 }
 
 // The adjoint kernel's wavefronts in turn: one per device, a lane per voltage.
+static int g_adj_passes = 0;  // 1: the adjoint's directions one pass each
+extern "C" void emu_set_adjoint_passes(int passes) { g_adj_passes = passes; }
+
 extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                            const double *seg, const double *sc, const double *V, int nV,
                            const double *states, const uint8_t *ok, int ntheta,
@@ -229,13 +232,16 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
                            double *grad, int32_t *status) {
     const size_t ls = 64;  // one workgroup: the devices in turn, a lane per voltage
     std::vector<double> fac((size_t)NP * ADJ_F * ls), st((size_t)NP * 3 * ls),
-        lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls);
+        lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls), ts((size_t)(NP + 2) * 2 * ls);
     AdjArgs a;
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg; a.ntheta = ntheta; a.bern = bern;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
     a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
     a.fac = fac.data(); a.st = st.data(); a.lam = lam.data(); a.iu = iu.data(); a.ls = ls;
-    std::vector<double> tab(adj_table_size(nseg, NP));
+    a.ts = ts.data();
+    a.passes = g_adj_passes;
+    const int Kc = adj_k(ntheta, a.passes);
+    std::vector<double> tab(adj_table_size(nseg, NP, Kc));
     for (int d = 0; d < n; ++d) {
         const bool mobile = sc[(size_t)SC_AT * n + d] > 0 || sc[(size_t)SC_CT * n + d] > 0;
         if (mobile) status[d] = 1;
@@ -250,6 +256,22 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
                     for (int k = 0; k < ntheta; ++k) out[k] = mobile ? NAN : 0.0;
                 else
                     adj_lambda(a, d, v, (size_t)(v - v0), tab.data());
+            }
+            if (Kc > 0) {  // every direction in one reverse pass
+                std::vector<double> g((size_t)64 * ADJ_MAX_K, 0.0);
+                for (int v = v0; v < v1; ++v)
+                    if (act[v - v0]) adj_contract(a, d, v, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K]);
+                for (int c0 = 0; c0 <= NP; c0 += ADJ_CHUNK) {
+                    adj_stage_points(a, d, c0, tab.data(), 0, 1);
+                    for (int v = v0; v < v1; ++v)
+                        if (act[v - v0])
+                            adj_contract_points(a, c0, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K]);
+                }
+                for (int v = v0; v < v1; ++v)
+                    if (act[v - v0])
+                        for (int k = 0; k < ntheta; ++k)
+                            grad[((size_t)d * nV + v) * ntheta + k] = g[(size_t)(v - v0) * ADJ_MAX_K + k];
+                continue;
             }
             for (int k = 0; k < ntheta; ++k) {
                 adj_stage_direction(a, d, k, tab.data(), 0, 1);

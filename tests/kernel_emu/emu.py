@@ -32,6 +32,7 @@ def lib():
                                          i, i, i, i, vp, vp, vp, vp]
         _lib.emu_keep_states.argtypes = [vp]
         _lib.emu_set_lanes.argtypes = [i]
+        _lib.emu_set_adjoint_passes.argtypes = [i]
         _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, i, vp, vp, vp, i,
                                      vp, vp]
         _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
@@ -71,8 +72,9 @@ def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=Fals
     return J, ok.astype(bool), its, status
 
 
-def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact"):
-    """dJ/dtheta (n, nV, ntheta) and status (n,) of the adjoint kernel's lanes, replayed."""
+def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact", passes=False):
+    """dJ/dtheta (n, nV, ntheta) and status (n,) of the adjoint kernel's lanes, replayed;
+    ``passes``: one pass per direction instead of the reverse pass over the parameters."""
     n = packed["sc"].shape[1]
     NP = packed["pt"].shape[0] - 2
     V = np.ascontiguousarray(V, dtype=np.float64)
@@ -83,6 +85,7 @@ def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact"):
     status = np.zeros(n, dtype=np.int32)
     arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
     okb = np.ascontiguousarray(ok, dtype=np.uint8)
+    lib().emu_set_adjoint_passes(int(bool(passes)))
     lib().emu_adjoint(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV,
                       _p(np.ascontiguousarray(states)), _p(okb), K, _p(dpt), _p(dseg), _p(dsc),
                       0 if bernoulli == "exact" else 1, _p(grad), _p(status))

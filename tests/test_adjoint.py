@@ -132,3 +132,26 @@ def test_devices_with_mobile_ions_are_refused(emu):
            "sc": np.zeros((1,) + packed["sc"].shape)}
     grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
     assert status[0] == 1 and np.isnan(grad).all()
+
+
+def test_reverse_pass_is_the_directions_passes(c4_case, emu):
+    """The one reverse pass over the parameters (every direction contracted on the way) and one
+    forward dual-number pass per direction: the same derivative by two independent
+    differentiations of the same rows and current, per device, voltage and parameter."""
+    th, packed, tan, V, J, ok, grad, status = c4_case
+    states = np.zeros((len(V), 400, 3, N_DEV))
+    emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200, states=states)
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import j_scale
+
+    by_pass, st2 = emu.adjoint_packed(packed, V, states, ok, tan, passes=True)
+    assert not st2.any()
+    # dJ/dln(theta_k) against each curve's current scale, as the finite-difference tests
+    # measure (a parameter J hardly depends on, such as a trap density far from where carriers
+    # recombine, has its two terms cancel to round-off of the larger)
+    unit = np.abs(th)
+    unit[:, 15] = 1.0
+    cscale = np.array([j_scale(A.device_from_theta(x)) for x in th]) + np.max(np.abs(J), axis=1)
+    err = np.abs(grad - by_pass) * unit[:, None, :] / cscale[:, None, None]
+    assert np.max(err) < 1e-11, np.unravel_index(np.argmax(err), err.shape)
+    assert np.max(np.abs(by_pass) * unit[:, None, :] / cscale[:, None, None]) > 0.1
