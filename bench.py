@@ -153,8 +153,9 @@ def copy_calibration(mib: int) -> None:
 
 def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
     """Algorithmic bytes of the adjoint per converged (device, voltage): the assembly at the
-    converged state, the transposed sweep's factors written and read, the parameter
-    contraction's reads of lambda."""
+    converged state, the transposed sweep's factors written and read, the reverse pass's reads
+    of lambda (its contraction with the 16 directions' tangents reads each device's tangents
+    once for all its voltages)."""
     return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
 
 

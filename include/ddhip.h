@@ -103,8 +103,9 @@ size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
  * dsc [ntheta][20][n] (d array / d theta_k). states and ok: from ddh_solve_batch_states on the
  * same batch and ascending V. grad: [n][nV][ntheta]; 0 where a voltage did not converge (the
  * reference's zero gradient there); NaN and status[d] = 1 where device d has mobile ions.
- * nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each device's arrays and one direction's
- * tangents are staged in the local data share). bernoulli: as ddh_opts. */
+ * nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each device's arrays and tangents are staged
+ * in the local data share). Up to 16 directions take one reverse pass over the parameters,
+ * more one forward pass each. bernoulli: as ddh_opts. */
 int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
                 const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
                 const double *dsc, int32_t bernoulli, double *grad, int32_t *status,
