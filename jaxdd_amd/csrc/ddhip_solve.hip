@@ -169,10 +169,12 @@ __global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
-// over the devices), one lane per voltage; each direction's segment and scalar tangents are
-// staged in LDS by the whole wavefront, read there by every lane.
+// over the devices), one lane per voltage; the device's tables and tangents are staged in LDS
+// by the whole wavefront, read there by every lane. REV: every direction in one reverse pass
+// over the parameters (up to ADJ_MAX_K directions), else one dual-number pass per direction.
+template <bool REV>
 __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
-    extern __shared__ double lds[];  // adj_table_size(a.nseg, a.NP) doubles
+    extern __shared__ double lds[];  // adj_table_size(a.nseg, a.NP, K or 0) doubles
     ldouble *tab = (ldouble *)lds;
     const int lane = threadIdx.x;
     const size_t sl = (size_t)blockIdx.x * 64 + lane;
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
             if (v < nV && !act)
                 for (int k = 0; k < K; ++k) out[k] = mobile ? NAN : 0.0;
             if (act) adj_lambda(a, d, v, sl, tab);
-            if (adj_k(K, a.passes) > 0) {  // every direction in one reverse pass
+            if (REV) {  // every direction in one reverse pass
                 double g[ADJ_MAX_K];
 #pragma unroll
                 for (int k = 0; k < ADJ_MAX_K; ++k) g[k] = 0.0;
@@ -518,9 +520,12 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     const size_t lds =
         (size_t)ddh::adj_table_size(b->nseg, b->NP, ddh::adj_k(ntheta, 0)) * sizeof(double);
     if (lds > 160 * 1024) return fail(DDH_E_SHAPE, "ddh_adjoint: NP too large for the LDS table");
-    (void)hipFuncSetAttribute((const void *)ddh::adjoint_kernel,
+    const bool rev = ddh::adj_k(ntheta, 0) > 0;
+    (void)hipFuncSetAttribute(rev ? (const void *)ddh::adjoint_kernel<true>
+                                  : (const void *)ddh::adjoint_kernel<false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(ddh::adjoint_kernel, grid, block, lds, s, a);
+    if (rev) hipLaunchKernelGGL(ddh::adjoint_kernel<true>, grid, block, lds, s, a);
+    else hipLaunchKernelGGL(ddh::adjoint_kernel<false>, grid, block, lds, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel launch");
     return DDH_OK;
