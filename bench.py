@@ -208,9 +208,9 @@ def traffic_from_profiles(workload: str):
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the devices of a batch are independent, a batch "
-              "splits across GPUs with no communication (DESIGN.md, 'Sharding'), and this "
-              f"build measures one GPU's share; --gpus {args.gpus} is not supported.",
+        print("bench.py runs on one GPU only: the devices of a batch are independent, so a "
+              "batch splits across GPUs by devices with no communication, and the throughput "
+              f"measured is one GPU's share; --gpus {args.gpus} is not supported.",
               file=sys.stderr)
         return 2
     if args.copy_calibration:

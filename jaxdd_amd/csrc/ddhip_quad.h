@@ -9,9 +9,10 @@
 //
 // An unknown kept is three more right-hand-side columns: x_i = d'_i [1, -s] - c'_i x_next.
 // Lanes 0 and 1 meet at (m1, m1+1) and lanes 2 and 3 at (m3, m3+1) as the two lanes of a pair
-// do. Lanes 1 and 2 also carry, during their elimination, the prefix product of their factors
-// (x of their first row as an affine function of the point beyond their last row), so that xi
-// and eta follow from one 3x3 solve at the middle without another pass over the rows. Each
+// do. Lanes 1 and 2 also accumulate, during their elimination, the prefix product of their
+// factors (x of their first row as an affine function of the point beyond their last row; in
+// the wavefront's local data share, not in registers), so that xi and eta follow from one 3x3
+// solve at the middle without another pass over the rows. Each
 // lane then back-substitutes its chunk outwards from its meeting point; the current's sums of
 // the four chunks join over the three intervals between them, pairwise and then as the two
 // lanes of a pair join theirs. Every joint value is computed from the same records in the same
