@@ -60,6 +60,8 @@ def parse():
                         " 4, in quarters: c2 only, the devices without mobile ions)")
     p.add_argument("--grad-lanes", type=int, default=4, choices=[1, 2, 4],
                    help="GPU lanes per device of the gradient workload's solve")
+    p.add_argument("--grad-precision", default="fp64", choices=["mixed", "fp64"],
+                   help="precision of the gradient workload's solve (the adjoint is FP64)")
     p.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
@@ -159,8 +161,8 @@ def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
     return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
 
 
-def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4):
-    """dJ/dtheta of the gradient workload's curves: one FP64 solve keeping the converged states
+def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: str = "fp64"):
+    """dJ/dtheta of the gradient workload's curves: one solve keeping the converged states
     and one adjoint launch, with the batch and its tangents resident in HBM before the timed
     region (the warm-up call has allocated the workspaces), timed with HIP events on the
     launch stream and by the host clock around the call."""
@@ -172,11 +174,11 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4):
     n = packed["sc"].shape[1]
     batch, dtan = resident(packed, tan)
     if warm:
-        jv_gradients(batch, dtan, V, lanes=lanes)
+        jv_gradients(batch, dtan, V, lanes=lanes, precision=precision)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev, lanes=lanes)
+    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev, lanes=lanes, precision=precision)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
@@ -187,6 +189,7 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4):
         "value": n / (t1 - t0), "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters)",
         "devices": n, "voltages": len(V), "ms": (t1 - t0) * 1e3,
         "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms, "solve_lanes_per_device": lanes,
+        "solve_precision": precision,
         "converged_frac": float(ok.mean()),
         "finite": bool(np.isfinite(grad.cpu().numpy()).all()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -316,7 +319,8 @@ def main() -> int:
         out["flags_equal_calm"] = bool(np.array_equal(ok[calm], ok64[calm]))
         out["chaotic_points"] = int((~calm).sum())
     if grad_inputs is not None:
-        out["gradient"] = gradient_bench(grad_inputs, V, lanes=args.grad_lanes)
+        out["gradient"] = gradient_bench(grad_inputs, V, lanes=args.grad_lanes,
+                                         precision=args.grad_precision)
         out["gradient"]["host_tangents_s"] = grad_host_s
         out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
                                        "voltages, FP64, 16 parameters")

@@ -127,15 +127,17 @@ def resident(packed, tan, device="cuda"):
 
 def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
                  bernoulli_name="exact", device="cuda", return_states=False, events=None,
-                 lanes=4):
+                 lanes=4, precision="fp64"):
     """J (n, nV), converged, iterations and dJ/dtheta (n, nV, 16) on the GPU for ascending
-    voltages: the FP64 solve keeping each voltage's final state, then the adjoint kernel.
+    voltages: the solve keeping each voltage's final state, then the adjoint kernel.
     Unconverged voltages get zero gradients. ``packed``, ``tan``: host arrays (`tangents`), or
     the device-resident pair `resident` returns. ``return_states``: also the linearisation
     points (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream
     around the solve kernel and around the adjoint kernel. ``lanes``: GPU lanes per device of
     the solve (4: the grid in quarters, the fastest for the batches gradients are taken of;
-    2; 1), as `jaxdd_amd.batch.solve_batch`'s."""
+    2; 1), as `jaxdd_amd.batch.solve_batch`'s. ``precision``: the solve's, as solve_batch's
+    ("mixed": the Newton correction's factors in FP32; the residual, the states and the
+    adjoint stay FP64, and the states converge to the same ``tol``)."""
     lib = _lib.load()
     V = np.asarray(voltages, dtype=float)
     if np.any(np.diff(V) <= 0):
@@ -144,13 +146,16 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     n, NP, nV = batch["n"], batch["NP"], len(V)
     if lanes not in (1, 2, 4):
         raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
+    if precision not in _lib.PRECISIONS:
+        raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
+    prec = _lib.PRECISIONS[precision]
     sc = batch["_sc"]
     if bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any()):
         raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
     if lanes == 4:
         check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
-                     0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0, int(lanes))
+                     0 if bernoulli_name == "exact" else 1, prec, 0, int(lanes))
     dev = batch["_pt"].device
     Vd = torch.as_tensor(V, device=dev)
     J = torch.empty((n, nV), dtype=torch.float64, device=dev)
@@ -158,7 +163,7 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
     status = torch.zeros(n, dtype=torch.int32, device=dev)
     states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
-    nbytes = lib.ddh_workspace_bytes(n, NP, nV, _lib.DDH_FP64)
+    nbytes = lib.ddh_workspace_bytes(n, NP, nV, prec)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
                    batch["_seg"].data_ptr(), batch["_sc"].data_ptr())

@@ -474,6 +474,27 @@ def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
     assert np.max(err) < 1e-5, (np.argmax(err), np.max(err))
 
 
+def test_gradients_of_the_mixed_precision_solve(c4_gpu):
+    """The solve with its Newton factors in FP32 converges to the FP64 solve's states (its
+    residual is FP64), so the adjoint linearised there gives the same gradients: the same
+    directional metric as the replay test, against the FP64 solve's gradients."""
+    from jaxdd_amd import adjoint as A
+
+    th, packed, tan, V, J, ok, grad, states = c4_gpu
+    Jm, okm, _, gm, sm = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True, max_iter=200,
+                                        precision="mixed", return_states=True)
+    assert _np(okm).all()
+    assert np.max(np.abs(_np(sm) - states)) < 1e-9
+    assert np.max(np.abs(_np(Jm) - J)) <= 1e-9 * np.max(np.abs(J))
+    rng = np.random.default_rng(3)
+    w, d = rng.normal(size=ok.shape), rng.normal(size=th.shape)
+    scale = th.copy()
+    scale[:, 15] = 0.01
+    a = np.sum(w[:, :, None] * _np(gm) * (scale * d)[:, None, :])
+    b = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
+    assert abs(a - b) < 1e-6 * abs(b)
+
+
 def test_autograd_function_backward_is_the_adjoint(c4_gpu):
     import torch
 

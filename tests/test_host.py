@@ -236,3 +236,5 @@ def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
     tan = {"pt": np.zeros((1,) + tuple(ions["_pt"].shape)), "seg": None, "sc": None}
     with pytest.raises(NotImplementedError, match="mobile ions"):
         A.jv_gradients(ions, tan, [0.0, 0.5])
+    with pytest.raises(ValueError, match="precision"):
+        A.jv_gradients(ions, tan, [0.0, 0.5], precision="fp32")
