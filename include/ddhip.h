@@ -147,6 +147,25 @@ int ddh_assemble_ions(const ddh_batch *b, const double *y, const double *Vint,
                       const double *logs, int32_t bernoulli, double *F, double *A, double *B,
                       double *C, double *U, void *hip_stream);
 
+/* Device bytes of ddh_impedance's workspace (the per-device tables) for n devices of NP
+ * interior points. */
+size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP);
+
+/* The admittance dJ/dV_int of every device at a steady state, per frequency: replaces
+ * src/jaxdd/impedance.py:230-360 (`_admittance`) as `_batch_admittance` (:363-368) calls it for
+ * devices without mobile ions (bulk traps one level, their occupations at their own rates; no
+ * interface traps). y: [NP][3][n] the steady interior state (ddh_solve_batch_states at one
+ * voltage); Vint [n] its internal voltage; iface [NP + 1][n] 1 where the interval after a point
+ * crosses from one layer to the next; laps [nf][2] (re, im) of iw, or of implicit Euler's
+ * (1 - e^(-iw dt))/dt; dt: implicit Euler's step for the traps' occupations (0: continuous, as
+ * the reference's dt = None). Output Y [n][nf][2] (re, im) in SIMsalabim's sign;
+ * status [n]: 0, or 1 for a device with mobile ions (Y NaN; the reference solves those with the
+ * ions as unknowns). The external impedance is 1/(Y + G_shunt) + R_series (impedance.py:405).
+ * workspace: ddh_impedance_workspace_bytes(n, NP) bytes of device memory. */
+int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint, const uint8_t *iface,
+                  const double *laps, int32_t nf, double dt, int32_t bernoulli, double *Y,
+                  int32_t *status, void *workspace, size_t workspace_bytes, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

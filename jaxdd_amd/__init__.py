@@ -2,8 +2,8 @@
 
 The reference's API for this path (src/jaxdd/__init__.py:14-66): SIMsalabim input files in
 (`read_setup`, `setup_from_texts`), a device built from them (`build_device`), its J-V curve
-(`solve_jv`) or a batch's (`prepare`, `solve_batch`), and the batched block sweep operator
-(`block_sweep`). The solve runs in hand-written HIP kernels for gfx950 (``libddhip.so``),
+(`solve_jv`) or a batch's (`prepare`, `solve_batch`), the batched block sweep operator
+(`block_sweep`) and a batch's impedance at a steady state (`solve_impedance`). The solve runs in hand-written HIP kernels for gfx950 (``libddhip.so``),
 called through a C-ABI; torch tensors hold the buffers.
 """
 
@@ -23,6 +23,7 @@ _LAZY = {
     "external_voltages": "batch", "prepare": "batch", "solve_batch": "batch",
     "JVResult": "solver", "bernoulli": "solver", "bernoulli_simss": "solver", "solve_jv": "solver",
     "block_sweep": "sweep",
+    "ImpedanceResult": "impedance", "solve_impedance": "impedance",
 }
 
 
@@ -37,6 +38,7 @@ __version__ = "0.1.0"
 
 __all__ = [
     "Device",
+    "ImpedanceResult",
     "JVResult",
     "Layer",
     "Setup",
@@ -56,5 +58,6 @@ __all__ = [
     "setup_from_texts",
     "simss_voltages",
     "solve_batch",
+    "solve_impedance",
     "solve_jv",
 ]

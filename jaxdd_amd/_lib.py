@@ -11,6 +11,8 @@ LIB = HERE / "libddhip.so"
 
 DDH_FP64, DDH_MIXED = 0, 1
 PRECISIONS = {"fp64": DDH_FP64, "mixed": DDH_MIXED}
+#: The impedance operator's per-device table: fields per grid point, scalars per device.
+IMP_TABLE_FIELDS, IMP_SCALARS = 47, 8
 
 
 class Batch(ctypes.Structure):
@@ -29,7 +31,8 @@ class Opts(ctypes.Structure):
 #: The symbols include/ddhip.h declares.
 SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solve_batch",
            "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
-           "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions")
+           "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions",
+           "ddh_impedance_workspace_bytes", "ddh_impedance")
 
 _lib = None
 
@@ -65,6 +68,11 @@ def load():
     lib.ddh_assemble_ions.restype = ctypes.c_int
     lib.ddh_assemble_ions.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, i32, vp, vp, vp, vp, vp,
                                       vp]
+    lib.ddh_impedance_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_impedance_workspace_bytes.argtypes = [i32, i32]
+    lib.ddh_impedance.restype = ctypes.c_int
+    lib.ddh_impedance.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, i32, ctypes.c_double,
+                                  i32, vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 2:

@@ -102,8 +102,14 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     ], axis=-1)  # (4, P, n)
     pt = np.ascontiguousarray(pt.transpose(1, 0, 2))  # (P, 4, n)
     sc = np.ascontiguousarray(np.array([_scalars(d) for d in devices]).T)  # (20, n)
+    # the intervals that cross from one layer to the next (the current's interfaces,
+    # src/jaxdd/batch.py:375-376)
+    iface = np.zeros((P - 1, n), dtype=np.uint8)
+    for k, d in enumerate(devices):
+        iface[list(d.i1[:-1]), k] = 1
     return {"pt": pt, "sid": np.ascontiguousarray(sid.T.astype(np.uint8)), "seg": seg,
-            "sc": sc, "nseg": nseg, "R_series": np.array([d.R_series for d in devices])}
+            "sc": sc, "nseg": nseg, "R_series": np.array([d.R_series for d in devices]),
+            "iface": iface}
 
 
 def merge(parts: list[dict]) -> dict:
@@ -117,7 +123,7 @@ def merge(parts: list[dict]) -> dict:
             seg = np.concatenate([seg, np.repeat(seg[-1:], nseg - seg.shape[0], axis=0)])
         segs.append(seg)
     out = {"nseg": nseg, "seg": np.concatenate(segs, axis=2)}
-    for k in ("pt", "sid", "sc"):
+    for k in ("pt", "sid", "sc", "iface"):
         out[k] = np.ascontiguousarray(np.concatenate([p[k] for p in parts], axis=-1))
     out["R_series"] = np.concatenate([p["R_series"] for p in parts])
     return out
@@ -144,7 +150,7 @@ def from_packed(packed: dict, device: str | torch.device = "cuda") -> dict:
     return {
         "n": n, "NP": NP, "nseg": packed["nseg"],
         "_pt": t(packed["pt"]), "_sid": t(packed["sid"], torch.uint8), "_seg": t(packed["seg"]),
-        "_sc": t(sc),
+        "_sc": t(sc), "_iface": t(packed["iface"], torch.uint8),
         # the reference's per-device keys (read by solve_batch at each call)
         "J_scale": t(col["J_scale"]), "g_frac": t(col["g_frac"]), "Vt": t(col["Vt"]),
         "L": t(col["L"]), "V_pre": t(col["V_pre"]), "pre_cond": t(col["pre_cond"]),

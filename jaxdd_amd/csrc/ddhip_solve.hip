@@ -10,6 +10,7 @@
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
+#include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
 
@@ -166,6 +167,19 @@ template <bool FIVE>
 __global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
     const int d = blockIdx.x * 64 + threadIdx.x;
     if (d < a.n) assemble_lane<FIVE>(a, d);
+}
+
+// The impedance operator: the per-device tables (one lane per device), then one lane per
+// (device, frequency), the frequencies of a device on neighbouring lanes so that their reads of
+// its table coincide.
+__global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
+    const int d = blockIdx.x * 64 + threadIdx.x;
+    if (d < a.n) impedance_prep_lane(a, d);
+}
+
+__global__ void __launch_bounds__(256) impedance_kernel(ImpArgs a) {
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid < (long)a.n * a.nf) impedance_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
@@ -570,6 +584,45 @@ extern "C" int ddh_assemble_ions(const ddh_batch *b, const double *y, const doub
     hipLaunchKernelGGL(ddh::assemble_kernel<true>, grid, block, 0, (hipStream_t)stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "assemble_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP) {
+    if (n <= 0 || NP < 1) return 0;
+    return ((size_t)n * NP * ddh::IM_N + (size_t)n * ddh::IS_N) * sizeof(double);
+}
+
+extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint,
+                             const uint8_t *iface, const double *laps, int32_t nf, double dt,
+                             int32_t bernoulli, double *Y, int32_t *status, void *workspace,
+                             size_t workspace_bytes, void *stream) {
+    if (!b || !y || !Vint || !iface || !laps || !Y || !status || !workspace)
+        return fail(DDH_E_ARG, "ddh_impedance: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_impedance: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14");
+    if (nf < 1 || (long)b->n * nf > (1L << 30))
+        return fail(DDH_E_SHAPE, "ddh_impedance: nf >= 1 and n * nf <= 2^30");
+    if (bernoulli != 0 && bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_impedance: bernoulli must be 0 (exact) or 1 (simss)");
+    if (!(dt >= 0.0) || !isfinite(dt))
+        return fail(DDH_E_ARG, "ddh_impedance: dt must be finite and >= 0 (0: continuous)");
+    if (workspace_bytes < ddh_impedance_workspace_bytes(b->n, b->NP))
+        return fail(DDH_E_WORKSPACE, "ddh_impedance: workspace smaller than ddh_impedance_workspace_bytes");
+    ddh::ImpArgs a = {};
+    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli; a.nf = nf;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
+    a.y = y; a.Vint = Vint; a.iface = iface; a.laps = laps; a.dt = dt;
+    a.tab = (double *)workspace;
+    a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_N;
+    a.Y = Y; a.status = status;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3((b->n + 63) / 64), dim3(64), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "impedance_prep_kernel launch");
+    const long lanes = (long)b->n * nf;
+    hipLaunchKernelGGL(ddh::impedance_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "impedance_kernel launch");
     return DDH_OK;
 }
 

@@ -796,8 +796,8 @@ static void freeze(problem *P, const double *y, const double *logs, ionscale *sc
 }
 
 /* ---- _solve_one (batch.py:937-1006) with _start (:875-934) ------------------------------ */
-int ddo_solve_one(const ddo_device *d, const double *Vs, int nV, const ddo_opts *o, double *Jout,
-                  uint8_t *okout, int32_t *itsout) {
+int ddo_solve_one_states(const ddo_device *d, const double *Vs, int nV, const ddo_opts *o,
+                         double *Jout, uint8_t *okout, int32_t *itsout, double *states) {
     int NP = d->NP;
     if (NP < 2 || NP > 4096 || nV < 1) return -1;
     problem P;
@@ -900,12 +900,18 @@ int ddo_solve_one(const ddo_device *d, const double *Vs, int nV, const ddo_opts 
         Jout[idx] = J;
         okout[idx] = (uint8_t)(ok && pre_ok);
         itsout[idx] = it;
+        if (states) memcpy(states + (size_t)idx * ny, guess, ny * sizeof(double));
     }
     free(ys); free(yprev); free(ycur); free(guess);
     free(y); free(yf); free(yc); free(yk); free(Vf);
     scratch_free(&s);
     problem_free(&P);
     return 0;
+}
+
+int ddo_solve_one(const ddo_device *d, const double *Vs, int nV, const ddo_opts *o, double *Jout,
+                  uint8_t *okout, int32_t *itsout) {
+    return ddo_solve_one_states(d, Vs, nV, o, Jout, okout, itsout, NULL);
 }
 
 /* ---- solve_jv (solver.py:2041-2245) --------------------------------------------------- */

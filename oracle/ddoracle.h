@@ -56,6 +56,10 @@ double ddo_bernoulli(double x, int kind, double *dB);
 /* _solve_one for ascending voltages: J (without the shunt), converged, iterations. */
 int ddo_solve_one(const ddo_device *dev, const double *V_sorted, int nV, const ddo_opts *o,
                   double *J, uint8_t *ok, int32_t *its);
+/* The same, also each voltage's final iterate (nV x NP x 3) into states. */
+int ddo_solve_one_states(const ddo_device *dev, const double *V_sorted, int nV,
+                         const ddo_opts *o, double *J, uint8_t *ok, int32_t *its,
+                         double *states);
 
 /* solve_jv: voltages in the given order, J with the shunt. Returns -2 when the pre-bias does
  * not converge (the reference raises RuntimeError). */

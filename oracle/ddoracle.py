@@ -63,6 +63,7 @@ def _load():
     lib.ddo_solve_one.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_int,
                                   ctypes.POINTER(_Opts), _P, ctypes.POINTER(ctypes.c_uint8),
                                   ctypes.POINTER(ctypes.c_int32)]
+    lib.ddo_solve_one_states.argtypes = lib.ddo_solve_one.argtypes + [_P]
     lib.ddo_solve_jv.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_uint8),
                                  ctypes.POINTER(ctypes.c_int32)]
@@ -136,8 +137,9 @@ class _Pinned:
 
 
 def solve_one(dev, V_sorted, tol=1e-10, max_iter=60, start_iter=60, strict=False,
-              bernoulli="exact", g_frac=None):
-    """One device's curve at ascending voltages: (J without the shunt, converged, iterations)."""
+              bernoulli="exact", g_frac=None, return_states=False):
+    """One device's curve at ascending voltages: (J without the shunt, converged, iterations);
+    ``return_states``: also each voltage's final iterate (nV x NP x 3)."""
     V = np.ascontiguousarray(V_sorted, dtype=np.float64)
     n = len(V)
     J = np.zeros(n)
@@ -145,11 +147,15 @@ def solve_one(dev, V_sorted, tol=1e-10, max_iter=60, start_iter=60, strict=False
     its = np.zeros(n, dtype=np.int32)
     pin = _Pinned(dev, g_frac)
     o = _Opts(tol, max_iter, start_iter, int(strict), BERNOULLI[bernoulli])
-    rc = lib().ddo_solve_one(ctypes.byref(pin.struct), _ptr(V), n, ctypes.byref(o), _ptr(J),
-                             ok.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                             its.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    states = np.zeros((n, len(dev.x) - 2, 3))
+    rc = lib().ddo_solve_one_states(ctypes.byref(pin.struct), _ptr(V), n, ctypes.byref(o),
+                                    _ptr(J), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                    its.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    _ptr(states))
     if rc != 0:
         raise ValueError(f"ddo_solve_one: error {rc}")
+    if return_states:
+        return J, ok.astype(bool), its, states
     return J, ok.astype(bool), its
 
 

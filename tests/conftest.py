@@ -52,6 +52,17 @@ def oracle():
     return ddoracle
 
 
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available(), "the GPU tests need the MI355X"
+    from jaxdd_amd import _lib
+
+    _lib.load()  # the HIP library must be there: no fallback
+    return torch
+
+
 @pytest.fixture(scope="session")
 def emu():
     import emu as _emu

@@ -8,6 +8,7 @@
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
+#include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
 
@@ -295,6 +296,22 @@ extern "C" int emu_assemble(int n, int NP, int nseg, const double *pt, const uin
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.logs = logs;
     a.F = F; a.A = A; a.B = B; a.C = C; a.U = U; a.J = J;
     for (int d = 0; d < n; ++d) assemble_lane<false>(a, d);
+    return 0;
+}
+
+// The impedance operator: the per-device table, then every (device, frequency) lane.
+extern "C" int emu_impedance(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
+                             const double *seg, const double *sc, const double *y,
+                             const double *Vint, const uint8_t *iface, const double *laps, int nf,
+                             double dt, int bern, double *tab, double *dsc, double *Y,
+                             int32_t *status) {
+    ImpArgs a = {};
+    a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern; a.nf = nf;
+    a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.iface = iface;
+    a.laps = laps; a.dt = dt; a.tab = tab; a.dsc = dsc; a.Y = Y; a.status = status;
+    for (int d = 0; d < n; ++d) impedance_prep_lane(a, d);
+    for (int d = 0; d < n; ++d)
+        for (int f = 0; f < nf; ++f) impedance_lane(a, d, f);
     return 0;
 }
 

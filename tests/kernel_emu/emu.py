@@ -37,6 +37,8 @@ def lib():
                                      vp, vp]
         _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
                                       vp, vp]
+        _lib.emu_impedance.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, i,
+                                       ctypes.c_double, i, vp, vp, vp, vp]
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
     return _lib
@@ -110,6 +112,30 @@ def assemble_packed(packed, y, Vint, logs=None, bernoulli="exact"):
     return (F.transpose(2, 0, 1), A.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
             B.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2), C.reshape(NP, 3, 3, n).transpose(3, 0, 1, 2),
             U.reshape(NP, 3, 2, n).transpose(3, 0, 1, 2), J)
+
+
+def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=False):
+    """The impedance operator's CPU replay: y (n, NP, 3) steady states, Vint (n,), laps
+    (complex, nf) -> Y (n, nf) complex, status (n,); ``table``: also the per-device tables
+    (n, NP, fields) and scalars (n, 8)."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
+    yl = np.ascontiguousarray(np.asarray(y, dtype=float).transpose(1, 2, 0))
+    Vd = np.ascontiguousarray(Vint, dtype=float)
+    iface = np.ascontiguousarray(packed["iface"], dtype=np.uint8)
+    lp = np.ascontiguousarray(np.stack([np.real(laps), np.imag(laps)], -1), dtype=float)
+    nf = lp.shape[0]
+    from jaxdd_amd._lib import IMP_TABLE_FIELDS, IMP_SCALARS
+
+    tab = np.zeros((n, NP, IMP_TABLE_FIELDS)); dsc = np.zeros((n, IMP_SCALARS))
+    Y = np.zeros((n, nf, 2)); status = np.zeros(n, dtype=np.int32)
+    lib().emu_impedance(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(yl), _p(Vd),
+                        _p(iface), _p(lp), nf, 0.0 if dt is None else float(dt),
+                        0 if bernoulli == "exact" else 1, _p(tab), _p(dsc), _p(Y), _p(status))
+    if table:
+        return Y[..., 0] + 1j * Y[..., 1], status, tab, dsc
+    return Y[..., 0] + 1j * Y[..., 1], status
 
 
 def assemble_ions_packed(packed, y, Vint, logs, bernoulli="exact"):

@@ -19,17 +19,6 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.fixture(scope="module")
-def torch_cuda():
-    import torch
-
-    assert torch.cuda.is_available(), "the GPU tests need the MI355X"
-    from jaxdd_amd import _lib
-
-    _lib.load()  # the HIP library must be there: no fallback
-    return torch
-
-
 # ---- the assembly operator (reference test pattern test_solver.py:1578-1617) --------------
 @pytest.mark.parametrize("case", ["fixtures", "c3"])
 def test_assemble_matches_oracle(torch_cuda, oracle, emu, case):
