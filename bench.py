@@ -68,6 +68,10 @@ def parse():
     p.add_argument("--workers", type=int, default=0, help="host processes building devices")
     p.add_argument("--grad-n", type=int, default=4096,
                    help="devices of the gradient workload (dJ/dtheta, 16 parameters); 0: skip")
+    p.add_argument("--imp-n", type=int, default=2048,
+                   help="devices of the impedance workload (NP=300, no mobile ions); 0: skip")
+    p.add_argument("--imp-freqs", type=int, default=50,
+                   help="frequencies of the impedance workload (log-spaced, 1e-2..1e7 Hz)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device to device (HBM counter calibration)")
     return p.parse_args()
@@ -198,6 +202,51 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: s
     }
 
 
+#: FP64 operations per (device, frequency, grid point) of ddh::impedance_kernel, counted from
+#: its code (complex multiply 6, complex add 2, complex divide 11, real x complex 2): the
+#: occupation's lag 12, the diagonal's time and trap terms 28, the elimination's A C' and A d'
+#: 138, the block inverse by its adjugate 213, d' 66, C' 90, the current's coefficients 19, the
+#: contraction's g and its sum 96.
+IMP_FLOPS_PER_POINT = 662
+FP64_PEAK_TFLOPS = 78.6
+
+
+def impedance_bench(packed, freqs, voltage=0.0):
+    """The impedance workload: the steady solve at ``voltage`` keeping its states, then
+    ddh_impedance over every (device, frequency), the batch resident in HBM before the timed
+    call (a warm-up call has run), timed with HIP events on the launch stream and by the host
+    clock around the call."""
+    import torch
+
+    from jaxdd_amd.batch import from_packed
+    from jaxdd_amd.impedance import solve_impedance
+
+    batch = from_packed(packed, "cuda")
+    n, NP = batch["n"], batch["NP"]
+    solve_impedance(batch, freqs, voltage=voltage)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = solve_impedance(batch, freqs, voltage=voltage, events=ev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    steady_ms, imp_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    systems = n * len(freqs)
+    achieved = systems * NP * IMP_FLOPS_PER_POINT / (imp_ms * 1e-3) / 1e12
+    return {
+        "value": n / (t1 - t0),
+        "unit": f"impedance spectra/s (steady state + {len(freqs)} frequencies)",
+        "devices": n, "frequencies": len(freqs), "ms": (t1 - t0) * 1e3,
+        "steady_kernel_ms": steady_ms, "impedance_kernels_ms": imp_ms,
+        "systems_per_s": systems / (imp_ms * 1e-3),
+        "converged_frac": float(r.converged.mean()), "finite": bool(np.isfinite(r.Y).all()),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "kernel": "ddh::impedance_prep_kernel + ddh::impedance_kernel",
+                     "flops_per_point": IMP_FLOPS_PER_POINT},
+    }
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per launch of the solve kernel from the committed rocprofv3 counter summary
     (profiles/hbm_traffic.json), when it was measured on this workload."""
@@ -233,6 +282,9 @@ def main() -> int:
         theta = A.theta_from_table(S.draw(args.grad_n), range(args.grad_n))
         grad_inputs = A.tangents(theta, workers=workers)
         grad_host_s = time.perf_counter() - t
+    imp_packed = None
+    if args.imp_n:
+        imp_packed = S.packed("c2", args.imp_n, workers=workers, NP=300)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.variant, args.cpu_curves)
@@ -324,6 +376,12 @@ def main() -> int:
         out["gradient"]["host_tangents_s"] = grad_host_s
         out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
                                        "voltages, FP64, 16 parameters")
+    if imp_packed is not None:
+        freqs = np.logspace(-2, 7, args.imp_freqs)
+        out["impedance"] = impedance_bench(imp_packed, freqs)
+        out["impedance"]["workload"] = (f"c5 without ions: {args.imp_n} devices, NP=300, "
+                                        f"{args.imp_freqs} frequencies 1e-2..1e7 Hz at 0 V, "
+                                        "complex128, 3x3 blocks")
     print(json.dumps(out), flush=True)
     return 0
 

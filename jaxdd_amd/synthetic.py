@@ -156,12 +156,13 @@ def _pack_rows(args):
 
 
 def packed(config: str, n: int, seed: int = SEED, variant: str = "free", workers: int = 1,
-           chunk: int = 256) -> dict:
-    """The packed kernel arrays (`batch.pack`) of ``n`` devices of a workload, built in
-    ``workers`` processes. Call it before the process touches the GPU: the workers are forked."""
+           chunk: int = 256, NP: int = 400) -> dict:
+    """The packed kernel arrays (`batch.pack`) of ``n`` devices of a workload (``NP`` interior
+    points), built in ``workers`` processes. Call it before the process touches the GPU: the
+    workers are forked."""
     from .batch import merge
 
-    jobs = [(config, s, min(s + chunk, n), seed, variant, 400) for s in range(0, n, chunk)]
+    jobs = [(config, s, min(s + chunk, n), seed, variant, NP) for s in range(0, n, chunk)]
     if workers <= 1:
         parts = [_pack_rows(j) for j in jobs]
     else:
