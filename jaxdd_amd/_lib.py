@@ -12,7 +12,7 @@ LIB = HERE / "libddhip.so"
 DDH_FP64, DDH_MIXED = 0, 1
 PRECISIONS = {"fp64": DDH_FP64, "mixed": DDH_MIXED}
 #: The impedance operator's per-device table: fields per grid point, scalars per device.
-IMP_TABLE_FIELDS, IMP_SCALARS = 47, 8
+IMP_TABLE_FIELDS, IMP_SCALARS = 48, 8
 
 
 class Batch(ctypes.Structure):

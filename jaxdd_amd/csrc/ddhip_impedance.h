@@ -3,11 +3,11 @@
 // (J + lap M) dy = -(dF/dV_int) dV, one complex block-tridiagonal system per frequency, and the
 // admittance is the transient current's linearisation dJ = c . dy + J_V (impedance.py:316-358).
 //
-// Two launches. `impedance_prep_lane`, one lane per device, assembles the steady Jacobian's
-// blocks at the state (the Newton sweeps' row arithmetic, row_blocks_s), the time terms M, the
-// bulk traps' occupation terms and the coefficients of the linearised current, which do not
-// depend on the frequency, into a per-device table. `impedance_lane`, one lane per (device,
-// frequency), eliminates its complex system from the first row to the last and contracts the
+// Two launches. The first, one wavefront per device, assembles the steady Jacobian's blocks at
+// the state (the Newton sweeps' row arithmetic, row_blocks_s), the time terms M, the bulk traps'
+// occupation terms and the coefficients of the linearised current, which do not depend on the
+// frequency, into a per-device table. `impedance_lane`, one lane per (device, frequency),
+// eliminates its complex system from the first row to the last and contracts the
 // current's coefficients on the way: with x_i = d'_i - C'_i x_(i+1), sum_i c_i x_i =
 // sum_i g_i d'_i for g_0 = c_0, g_i = c_i - g_(i-1) C'_(i-1), so the solve needs no
 // back-substitution and stores no factors: the lanes of one device read its table together
@@ -49,9 +49,9 @@ DDH_MFN cplx operator/(cplx a, cplx b) {
     return cplx((a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r);
 }
 
-// Per point of the per-device table [n][NP][IM_N]: the steady blocks (row-major 3x3, the last
-// row's C zero: its coupling is the contact's), the time terms, the traps' occupation terms
-// and the current's coefficients.
+// The per-device table [n][NP][IM_N] (a point's fields together, 384 bytes: three cache lines
+// that the lanes of its device read at once): the steady blocks (row-major 3x3, the last row's C zero: its coupling is the
+// contact's), the time terms, the traps' occupation terms and the current's coefficients.
 enum ImpField {
     IM_A = 0, IM_B = 9, IM_C = 18,
     IM_MN = 27, IM_MP,     // -fac sC n, -fac sC p: lap times these on the continuity diagonals
@@ -64,6 +64,7 @@ enum ImpField {
     IM_GLN, IM_GLP, IM_GT,
     IM_DRN, IM_DRP,        // dR_n/df, dR_p/df
     IM_NN, IM_PP,          // n, p
+    IM_TMP,                // the assembly's scratch (sum of h before the point)
     IM_N
 };
 // Per device [n][IS_N]: the drive (-dF/dV_int) of the first and the last row, the current's
@@ -92,175 +93,221 @@ struct ImpArgs {
 #define ISID(j) ((int)a.sid[(size_t)(j) * a.n + d])
 #define ISEG(id, f) a.seg[((size_t)(id) * SEG_N + (f)) * a.n + d]
 #define IY(j, f) a.y[(((size_t)(j) - 1) * 3 + (f)) * a.n + d]
+#define ITAB(f, r) a.tab[((size_t)d * a.NP + (r)) * IM_N + (f)]
 
-DDH_FN void impedance_prep_lane(const ImpArgs &a, int d) {
-    const int NP = a.NP;
-    double *tab = a.tab + (size_t)d * NP * IM_N;
-    double *ds = a.dsc + (size_t)d * IS_N;
-    if (ISC(SC_AT) > 0 || ISC(SC_CT) > 0) {
-        a.status[d] = 1;
-        return;
-    }
-    a.status[d] = 0;
-    const double Vt = ISC(SC_VT), iVt = 1.0 / Vt, L = ISC(SC_L), L2 = L * L;
-    const double gfr = ISC(SC_GFRAC), elec = ISC(SC_ELEC), Vint = a.Vint[d];
-    const double half = 0.5 * elec;
-    const double V0 = ISC(SC_V0) + half * Vint, VN = ISC(SC_VL) - half * Vint;
-    auto point = [&](int j, Pt &q) {
-        q.sid = ISID(j);
-        q.offn = ISEG(q.sid, SEG_OFFN); q.offp = ISEG(q.sid, SEG_OFFP);
-        q.mun = ISEG(q.sid, SEG_MUN); q.mup = ISEG(q.sid, SEG_MUP);
-        q.am = q.cm = q.an = q.ca = 0.0;
-        q.h = IPT(j, PT_H);
-        if (j == 0 || j == NP + 1) {
-            q.V = j == 0 ? V0 : VN;
-            q.n = exp(j == 0 ? ISC(SC_LNL) : ISC(SC_LNR));
-            q.p = exp(j == 0 ? ISC(SC_LPL) : ISC(SC_LPR));
-            q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
-        } else {
-            q.V = IY(j, 0);
-            q.n = exp(IY(j, 1));
-            q.p = exp(IY(j, 2));
-            q.G = IPT(j, PT_G);
-            q.fix = IPT(j, PT_FIX);
-            q.R = recombination(q.n, q.p, ISEG(q.sid, SEG_KD), ISEG(q.sid, SEG_NI2),
-                                ISEG(q.sid, SEG_NT), ISEG(q.sid, SEG_CN), ISEG(q.sid, SEG_CP),
-                                ISEG(q.sid, SEG_NT0), ISEG(q.sid, SEG_PT0), q.dRn, q.dRp);
-        }
-    };
-    // pass 1: the blocks, the time and trap terms; the current's sums (S, D per carrier), each
-    // point's trapezoid weights c_j, iu_j and the prefixes sum_(j<i) c_j in the table's
-    // coefficient slots until pass 2
-    Pt cur, mid, rgt;
-    point(0, cur);
-    point(1, mid);
-    Edge eL = edge_flux_pt(cur, mid, Vt, iVt, a.bern);
-    double H = 0.0;
-    for (int j = 0; j <= NP; ++j) H += IPT(j, PT_H);
+// The table is assembled by one wavefront per device in three phases: (A) each point's blocks,
+// time and trap terms and its parts of the current's integrals, lanes over the points; (B) the
+// integrals' running sums, each lane over a contiguous segment of points, the segments' totals
+// scanned across the wavefront; (C) the current's coefficients, lanes over the points again.
+struct ImpDev {
+    double Vt, iVt, L, L2, gfr, elec, half, V0, VN;
+    double cn0, cp0, h0;   // the left contact's trapezoid weights and spacing
+};
+
+DDH_FN bool imp_mobile_ions(const ImpArgs &a, int d) { return ISC(SC_AT) > 0 || ISC(SC_CT) > 0; }
+
+DDH_FN ImpDev imp_device(const ImpArgs &a, int d) {
+    ImpDev k;
+    k.Vt = ISC(SC_VT); k.iVt = 1.0 / k.Vt; k.L = ISC(SC_L); k.L2 = k.L * k.L;
+    k.gfr = ISC(SC_GFRAC); k.elec = ISC(SC_ELEC); k.half = 0.5 * k.elec;
+    const double Vint = a.Vint[d];
+    k.V0 = ISC(SC_V0) + k.half * Vint;
+    k.VN = ISC(SC_VL) - k.half * Vint;
+    const int s0 = ISID(0);
     const double dx0 = IPT(1, PT_X) - IPT(0, PT_X);
-    const double gn0 = exp(-(V0 + cur.offn) * iVt) * ISEG(cur.sid, SEG_IMUN);
-    const double gp0 = exp((V0 + cur.offp) * iVt) * ISEG(cur.sid, SEG_IMUP);
-    double cpn = 0.5 * dx0 * gn0, cpp = 0.5 * dx0 * gp0;  // c_0
-    double Sn = cpn, Sp = cpp, Dn = 0.0, Dp = 0.0;        // iu_0 = 0
-    double U_prev = 0.0, iu = 0.0, x_prev = IPT(0, PT_X);
-    for (int i = 1; i <= NP; ++i) {
-        point(i + 1, rgt);
-        const Edge eR = edge_flux_pt(mid, rgt, Vt, iVt, a.bern);
-        RowSeg g;
-        g.meps = ISEG(mid.sid, SEG_EPS);
-        g.ceps = ISEG(cur.sid, SEG_EPS);
-        g.sP = ISEG(mid.sid, SEG_SP); g.scs = ISEG(mid.sid, SEG_SCS); g.dop = ISEG(mid.sid, SEG_DOP);
-        double Ab[9], Bb[9], Cb[9], r[3];
-        row_blocks_s<1>(g, cur, mid, rgt.V, eL, eR, false, L2, iVt, 1.0, gfr, Ab, Bb, Cb, r);
-        double *t = tab + (size_t)(i - 1) * IM_N;
-        if (i == 1) {
-            for (int row = 0; row < 3; ++row) ds[IS_D0 + row] = -Ab[3 * row] * half;
-        }
-        if (i == NP) {
-            for (int row = 0; row < 3; ++row) ds[IS_DN + row] = Cb[3 * row] * half;
-            for (int q = 0; q < 9; ++q) Cb[q] = 0.0;
-        }
-        for (int q = 0; q < 9; ++q) {
-            t[IM_A + q] = Ab[q];
-            t[IM_B + q] = Bb[q];
-            t[IM_C + q] = Cb[q];
-        }
-        const double hm = cur.h, hs = mid.h + cur.h;
-        const double fs = -0.5 * L2 * hm * hs * g.scs;  // -fac sC
-        t[IM_MN] = fs * mid.n;
-        t[IM_MP] = fs * mid.p;
-        // the bulk traps' occupation (impedance.py:63-87), one level
-        const double Nt = ISEG(mid.sid, SEG_NT), Cn = ISEG(mid.sid, SEG_CN), Cp = ISEG(mid.sid, SEG_CP);
-        const double nt0 = ISEG(mid.sid, SEG_NT0), pt0 = ISEG(mid.sid, SEG_PT0);
-        double rate = 1.0, un = 0.0, up = 0.0, drn = 0.0, drp = 0.0;
-        if (Nt > 0) {
-            rate = Cn * (mid.n + nt0) + Cp * (mid.p + pt0);
-            const double f = (Cn * mid.n + Cp * pt0) / rate;
-            un = Cn * mid.n * (1.0 - f) / rate;
-            up = -f * Cp * mid.p / rate;
-            drn = -Nt * Cn * (mid.n + nt0);
-            drp = Nt * Cp * (mid.p + pt0);
-        }
-        t[IM_RATE] = rate;
-        if (a.dt > 0) {
-            const double x = rate * a.dt;
-            t[IM_E] = exp(-x);
-            t[IM_M] = -expm1(-x) / x;
-        } else {
-            t[IM_E] = t[IM_M] = 0.0;
-        }
-        t[IM_UN] = un; t[IM_UP] = up;
-        t[IM_TN] = fs * drn; t[IM_TP] = fs * drp;
-        t[IM_DRN] = drn; t[IM_DRP] = drp;
-        t[IM_NN] = mid.n; t[IM_PP] = mid.p;
-        // the current's sums at point i (the steady rates: U = R - G g_frac, 0 at the contacts)
-        const double Uu = mid.R - mid.G * gfr;
-        const double xj = IPT(i, PT_X), dxl = xj - x_prev;
-        iu += 0.5 * dxl * (U_prev + Uu);
-        const double dxr = i < NP ? IPT(i + 1, PT_X) - xj : 0.0;
-        const double wgt = 0.5 * (dxl + dxr);
-        const double gn = exp(-(mid.V + mid.offn) * iVt) * ISEG(mid.sid, SEG_IMUN);
-        const double gp = exp((mid.V + mid.offp) * iVt) * ISEG(mid.sid, SEG_IMUP);
-        const double cn = wgt * gn, cp = wgt * gp;
-        t[IM_KV] = cn; t[IM_KD] = cp; t[IM_G0N] = iu;
-        t[IM_G0P] = Sn; t[IM_GLN] = Sp;  // prefixes sum_(j<i) c_j
-        t[IM_GLP] = mid.dRn; t[IM_GT] = mid.dRp;
-        Sn += cn; Sp += cp;
-        Dn += cn * iu; Dp += cp * iu;
-        U_prev = Uu; x_prev = xj;
-        cur = mid; mid = rgt; eL = eR;
+    k.cn0 = 0.5 * dx0 * exp(-(k.V0 + ISEG(s0, SEG_OFFN)) * k.iVt) * ISEG(s0, SEG_IMUN);
+    k.cp0 = 0.5 * dx0 * exp((k.V0 + ISEG(s0, SEG_OFFP)) * k.iVt) * ISEG(s0, SEG_IMUP);
+    k.h0 = IPT(0, PT_H);
+    return k;
+}
+
+DDH_FN void imp_point_at(const ImpArgs &a, int d, const ImpDev &k, int j, Pt &q) {
+    const int NP = a.NP;
+    q.sid = ISID(j);
+    q.offn = ISEG(q.sid, SEG_OFFN); q.offp = ISEG(q.sid, SEG_OFFP);
+    q.mun = ISEG(q.sid, SEG_MUN); q.mup = ISEG(q.sid, SEG_MUP);
+    q.am = q.cm = q.an = q.ca = 0.0;
+    q.h = IPT(j, PT_H);
+    if (j == 0 || j == NP + 1) {
+        q.V = j == 0 ? k.V0 : k.VN;
+        q.n = exp(j == 0 ? ISC(SC_LNL) : ISC(SC_LNR));
+        q.p = exp(j == 0 ? ISC(SC_LPL) : ISC(SC_LPR));
+        q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
+    } else {
+        q.V = IY(j, 0);
+        q.n = exp(IY(j, 1));
+        q.p = exp(IY(j, 2));
+        q.G = IPT(j, PT_G);
+        q.fix = IPT(j, PT_FIX);
+        q.R = recombination(q.n, q.p, ISEG(q.sid, SEG_KD), ISEG(q.sid, SEG_NI2),
+                            ISEG(q.sid, SEG_NT), ISEG(q.sid, SEG_CN), ISEG(q.sid, SEG_CP),
+                            ISEG(q.sid, SEG_NT0), ISEG(q.sid, SEG_PT0), q.dRn, q.dRp);
     }
-    // K per carrier (sn = -1 electrons, +1 holes)
+}
+
+// (A) row r (point i = r + 1): blocks, time and trap terms, and in the coefficients' slots the
+// point's trapezoid weights c_n, c_p, its step of the integral of U, 0.5 dx_(i-1) (U_(i-1) +
+// U_i), and dR/dln n, dR/dln p.
+DDH_FN void imp_assemble_row(const ImpArgs &a, int d, const ImpDev &k, int r) {
+    const int NP = a.NP, i = r + 1;
+    Pt cur, mid, rgt;
+    imp_point_at(a, d, k, i - 1, cur);
+    imp_point_at(a, d, k, i, mid);
+    imp_point_at(a, d, k, i + 1, rgt);
+    const Edge eL = edge_flux_pt(cur, mid, k.Vt, k.iVt, a.bern);
+    const Edge eR = edge_flux_pt(mid, rgt, k.Vt, k.iVt, a.bern);
+    RowSeg g;
+    g.meps = ISEG(mid.sid, SEG_EPS);
+    g.ceps = ISEG(cur.sid, SEG_EPS);
+    g.sP = ISEG(mid.sid, SEG_SP); g.scs = ISEG(mid.sid, SEG_SCS); g.dop = ISEG(mid.sid, SEG_DOP);
+    double Ab[9], Bb[9], Cb[9], rr[3];
+    row_blocks_s<1>(g, cur, mid, rgt.V, eL, eR, false, k.L2, k.iVt, 1.0, k.gfr, Ab, Bb, Cb, rr);
+    double *ds = a.dsc + (size_t)d * IS_N;
+    if (r == 0)
+        for (int row = 0; row < 3; ++row) ds[IS_D0 + row] = -Ab[3 * row] * k.half;
+    if (r == NP - 1) {
+        for (int row = 0; row < 3; ++row) ds[IS_DN + row] = Cb[3 * row] * k.half;
+        for (int q = 0; q < 9; ++q) Cb[q] = 0.0;
+    }
+    for (int q = 0; q < 9; ++q) {
+        ITAB(IM_A + q, r) = Ab[q];
+        ITAB(IM_B + q, r) = Bb[q];
+        ITAB(IM_C + q, r) = Cb[q];
+    }
+    const double fs = -0.5 * k.L2 * cur.h * (mid.h + cur.h) * g.scs;  // -fac sC
+    ITAB(IM_MN, r) = fs * mid.n;
+    ITAB(IM_MP, r) = fs * mid.p;
+    // the bulk traps' occupation (impedance.py:63-87), one level
+    const double Nt = ISEG(mid.sid, SEG_NT), Cn = ISEG(mid.sid, SEG_CN), Cp = ISEG(mid.sid, SEG_CP);
+    const double nt0 = ISEG(mid.sid, SEG_NT0), pt0 = ISEG(mid.sid, SEG_PT0);
+    double rate = 1.0, un = 0.0, up = 0.0, drn = 0.0, drp = 0.0;
+    if (Nt > 0) {
+        rate = Cn * (mid.n + nt0) + Cp * (mid.p + pt0);
+        const double f = (Cn * mid.n + Cp * pt0) / rate;
+        un = Cn * mid.n * (1.0 - f) / rate;
+        up = -f * Cp * mid.p / rate;
+        drn = -Nt * Cn * (mid.n + nt0);
+        drp = Nt * Cp * (mid.p + pt0);
+    }
+    ITAB(IM_RATE, r) = rate;
+    if (a.dt > 0) {
+        const double x = rate * a.dt;
+        ITAB(IM_E, r) = exp(-x);
+        ITAB(IM_M, r) = -expm1(-x) / x;
+    } else {
+        ITAB(IM_E, r) = 0.0;
+        ITAB(IM_M, r) = 0.0;
+    }
+    ITAB(IM_UN, r) = un; ITAB(IM_UP, r) = up;
+    ITAB(IM_TN, r) = fs * drn; ITAB(IM_TP, r) = fs * drp;
+    ITAB(IM_DRN, r) = drn; ITAB(IM_DRP, r) = drp;
+    ITAB(IM_NN, r) = mid.n; ITAB(IM_PP, r) = mid.p;
+    // the current's parts (the steady rates: U = R - G g_frac, 0 at the contacts)
+    const double U_i = mid.R - mid.G * k.gfr;
+    const double U_l = i > 1 ? cur.R - cur.G * k.gfr : 0.0;
+    const double xj = IPT(i, PT_X), dxl = xj - IPT(i - 1, PT_X);
+    const double dxr = i < NP ? IPT(i + 1, PT_X) - xj : 0.0;
+    const double wgt = 0.5 * (dxl + dxr);
+    ITAB(IM_KV, r) = wgt * exp(-(mid.V + mid.offn) * k.iVt) * ISEG(mid.sid, SEG_IMUN);
+    ITAB(IM_KD, r) = wgt * exp((mid.V + mid.offp) * k.iVt) * ISEG(mid.sid, SEG_IMUP);
+    ITAB(IM_G0N, r) = 0.5 * dxl * (U_l + U_i);
+    ITAB(IM_GLP, r) = mid.dRn;
+    ITAB(IM_GT, r) = mid.dRp;
+}
+
+// (B) the rows of one segment [r0, r1): their totals of the integral's steps, c_n, c_p and h
+DDH_FN void imp_segment_totals(const ImpArgs &a, int d, int r0, int r1, double tot[4]) {
+    tot[0] = tot[1] = tot[2] = tot[3] = 0.0;
+    for (int r = r0; r < r1; ++r) {
+        tot[0] += ITAB(IM_G0N, r);
+        tot[1] += ITAB(IM_KV, r);
+        tot[2] += ITAB(IM_KD, r);
+        tot[3] += IPT(r + 1, PT_H);
+    }
+}
+
+// (B) the segment's running sums from the totals of the segments before it (`off`, with the
+// left contact's parts): iu_i, sum_(j<i) c_n,j, sum_(j<i) c_p,j, sum_(j<i) h_j; returns the
+// segment's parts of D_n, D_p
+DDH_FN void imp_segment_prefix(const ImpArgs &a, int d, int r0, int r1, const double off[4],
+                               double &Dn, double &Dp) {
+    double iu = off[0], pn = off[1], pp = off[2], hs = off[3];
+    Dn = Dp = 0.0;
+    for (int r = r0; r < r1; ++r) {
+        const double cn = ITAB(IM_KV, r), cp = ITAB(IM_KD, r);
+        iu += ITAB(IM_G0N, r);
+        ITAB(IM_G0N, r) = iu;
+        ITAB(IM_G0P, r) = pn;
+        ITAB(IM_GLN, r) = pp;
+        ITAB(IM_TMP, r) = hs;
+        Dn += cn * iu;
+        Dp += cp * iu;
+        pn += cn;
+        pp += cp;
+        hs += IPT(r + 1, PT_H);
+    }
+}
+
+// The integrals' totals of a device, after (B): S_n, S_p (with the left contact's weight),
+// D_n, D_p, H; K per carrier; the contacts' and the displacement current's direct terms.
+struct ImpTot {
+    double Sn, Sp, Dn, Dp, H, Kn, Kp;
+};
+
+DDH_FN ImpTot imp_totals(const ImpArgs &a, int d, const ImpDev &k, double Sn, double Sp,
+                         double Dn, double Dp, double H, bool write) {
+    const int NP = a.NP;
+    ImpTot t;
+    t.Sn = Sn; t.Sp = Sp; t.Dn = Dn; t.Dp = Dp; t.H = H;
     const int sid0 = ISID(0), sidN = ISID(NP + 1);
-    const double En0 = exp(ISC(SC_LNL) - (V0 + ISEG(sid0, SEG_OFFN)) * iVt);   // n_0 E_0
-    const double Ep0 = exp(ISC(SC_LPL) + (V0 + ISEG(sid0, SEG_OFFP)) * iVt);
-    const double EnN = exp(ISC(SC_LNR) - (VN + ISEG(sidN, SEG_OFFN)) * iVt);
-    const double EpN = exp(ISC(SC_LPR) + (VN + ISEG(sidN, SEG_OFFP)) * iVt);
-    const double Kn = (-Vt * (EnN - En0) + Dn) / Sn;
-    const double Kp = (Vt * (EpN - Ep0) - Dp) / Sp;
-    const double sign = -elec, qs = sign * Q_E;
-    // the contacts: dK/dV_0 (through e^(sn Vg_0/V_T) and g_0), dK/dV_(N+1)
-    const double dKn0 = (-En0 - Kn * (-1.0) * cpn * iVt) / Sn, dKnN = EnN / Sn;
-    const double dKp0 = (-Ep0 - Kp * cpp * iVt) / Sp, dKpN = EpN / Sp;
-    ds[IS_JV] = qs * ((dKn0 + dKp0) * half - (dKnN + dKpN) * half);
-    const double iLH = 1.0 / (L * H);
-    const double eps0 = ISEG(sid0, SEG_EPS), epsN = ISEG(ISID(NP), SEG_EPS);
-    ds[IS_JD] = sign * (-half) * (epsN + eps0) * iLH;
-    // pass 2: the coefficients
-    double hsum = 0.0;  // sum_(j<i) h_j
-    double eps_prev = eps0;
-    x_prev = IPT(0, PT_X);
-    hsum = IPT(0, PT_H);
-    for (int i = 1; i <= NP; ++i) {
-        double *t = tab + (size_t)(i - 1) * IM_N;
-        const double cn = t[IM_KV], cp = t[IM_KD], iu_i = t[IM_G0N];
-        const double Pn = Sn - t[IM_G0P], Pp = Sp - t[IM_GLN];  // sum_(j>=i) c_j
-        const double dRln = t[IM_GLP], dRlp = t[IM_GT];
-        const double xj = IPT(i, PT_X), dxl = xj - x_prev;
-        const double dxr = i < NP ? IPT(i + 1, PT_X) - xj : 0.0;
-        const double dKnV = -cn * (iu_i - Kn) * iVt / Sn;
-        const double dKpV = -cp * (iu_i + Kp) * iVt / Sp;
-        // dD/dU_i = dx_i P_(i+1)/2 + dx_(i-1) P_i/2
-        const double dDn = 0.5 * dxr * (Pn - cn) + 0.5 * dxl * Pn;
-        const double dDp = 0.5 * dxr * (Pp - cp) + 0.5 * dxl * Pp;
-        const double hi = IPT(i, PT_H), hl = IPT(i - 1, PT_H);
-        const double width = 0.5 * (hl + hi);
-        const double T = (H - hsum - (a.iface[(size_t)i * a.n + d] ? hi : 0.0)) / H;
-        const double direct = Q_E * L * width * T;
-        const double wn = qs * (dDn / Sn) - sign * direct;   // dK_n/dU = +dD_n/S_n (sn = -1)
-        const double wp = qs * (-dDp / Sp) + sign * direct;
-        const double eps_i = ISEG(ISID(i), SEG_EPS);
-        t[IM_KV] = qs * (dKnV + dKpV);
-        t[IM_KD] = sign * (eps_prev - eps_i) * iLH;
-        t[IM_G0N] = (wn + wp) * dRln;
-        t[IM_G0P] = (wn + wp) * dRlp;
-        t[IM_GLN] = wn * t[IM_NN];
-        t[IM_GLP] = wp * t[IM_PP];
-        t[IM_GT] = wn * t[IM_DRN] + wp * t[IM_DRP];
-        eps_prev = eps_i;
-        hsum += hi;
-        x_prev = xj;
+    const double En0 = exp(ISC(SC_LNL) - (k.V0 + ISEG(sid0, SEG_OFFN)) * k.iVt);  // n_0 E_0
+    const double Ep0 = exp(ISC(SC_LPL) + (k.V0 + ISEG(sid0, SEG_OFFP)) * k.iVt);
+    const double EnN = exp(ISC(SC_LNR) - (k.VN + ISEG(sidN, SEG_OFFN)) * k.iVt);
+    const double EpN = exp(ISC(SC_LPR) + (k.VN + ISEG(sidN, SEG_OFFP)) * k.iVt);
+    t.Kn = (-k.Vt * (EnN - En0) + Dn) / Sn;
+    t.Kp = (k.Vt * (EpN - Ep0) - Dp) / Sp;
+    if (write) {
+        double *ds = a.dsc + (size_t)d * IS_N;
+        const double qs = -k.elec * Q_E;
+        // dK/dV_0 through e^(sn Vg_0/V_T) and the left contact's weight; dK/dV_(N+1)
+        const double dKn0 = (-En0 + t.Kn * k.cn0 * k.iVt) / Sn, dKnN = EnN / Sn;
+        const double dKp0 = (-Ep0 - t.Kp * k.cp0 * k.iVt) / Sp, dKpN = EpN / Sp;
+        ds[IS_JV] = qs * ((dKn0 + dKp0) * k.half - (dKnN + dKpN) * k.half);
+        const double eps0 = ISEG(sid0, SEG_EPS), epsN = ISEG(ISID(NP), SEG_EPS);
+        ds[IS_JD] = -k.elec * (-k.half) * (epsN + eps0) / (k.L * H);
     }
+    return t;
+}
+
+// (C) row r's coefficients of the linearised current
+DDH_FN void imp_coefficients(const ImpArgs &a, int d, const ImpDev &k, const ImpTot &t, int r) {
+    const int NP = a.NP, i = r + 1;
+    const double sign = -k.elec, qs = sign * Q_E;
+    const double cn = ITAB(IM_KV, r), cp = ITAB(IM_KD, r), iu = ITAB(IM_G0N, r);
+    const double Pn = t.Sn - ITAB(IM_G0P, r), Pp = t.Sp - ITAB(IM_GLN, r);  // sum_(j>=i) c_j
+    const double hsum = ITAB(IM_TMP, r);
+    const double dRln = ITAB(IM_GLP, r), dRlp = ITAB(IM_GT, r);
+    const double xj = IPT(i, PT_X), dxl = xj - IPT(i - 1, PT_X);
+    const double dxr = i < NP ? IPT(i + 1, PT_X) - xj : 0.0;
+    const double dKnV = -cn * (iu - t.Kn) * k.iVt / t.Sn;
+    const double dKpV = -cp * (iu + t.Kp) * k.iVt / t.Sp;
+    // dD/dU_i = dx_i P_(i+1)/2 + dx_(i-1) P_i/2
+    const double dDn = 0.5 * dxr * (Pn - cn) + 0.5 * dxl * Pn;
+    const double dDp = 0.5 * dxr * (Pp - cp) + 0.5 * dxl * Pp;
+    const double hi = IPT(i, PT_H), hl = IPT(i - 1, PT_H);
+    const double width = 0.5 * (hl + hi);
+    const double T = (t.H - hsum - (a.iface[(size_t)i * a.n + d] ? hi : 0.0)) / t.H;
+    const double direct = Q_E * k.L * width * T;
+    const double wn = qs * (dDn / t.Sn) - sign * direct;  // dK_n/dU = +dD_n/S_n (sn = -1)
+    const double wp = qs * (-dDp / t.Sp) + sign * direct;
+    const double eps_l = ISEG(ISID(i - 1), SEG_EPS), eps_i = ISEG(ISID(i), SEG_EPS);
+    ITAB(IM_KV, r) = qs * (dKnV + dKpV);
+    ITAB(IM_KD, r) = sign * (eps_l - eps_i) / (k.L * t.H);
+    ITAB(IM_G0N, r) = (wn + wp) * dRln;
+    ITAB(IM_G0P, r) = (wn + wp) * dRlp;
+    ITAB(IM_GLN, r) = wn * ITAB(IM_NN, r);
+    ITAB(IM_GLP, r) = wp * ITAB(IM_PP, r);
+    ITAB(IM_GT, r) = wn * ITAB(IM_DRN, r) + wp * ITAB(IM_DRP, r);
 }
 
 // One (device, frequency): the complex system eliminated row by row, the current's
@@ -281,26 +328,27 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
     cplx acc(0.0, 0.0);
     for (int r = 0; r < NP; ++r) {
         const double *t = tab + (size_t)r * IM_N;
+        auto T = [&](int f) { return t[f]; };
         double A[9], C[9];
         cplx B[9];
         for (int q = 0; q < 9; ++q) {
-            A[q] = t[IM_A + q];
-            B[q] = cplx(t[IM_B + q]);
-            C[q] = t[IM_C + q];
+            A[q] = T(IM_A + q);
+            B[q] = cplx(T(IM_B + q));
+            C[q] = T(IM_C + q);
         }
         // the occupation beyond its steady response, per unit of it (impedance.py:104-117)
         cplx h;
         if (steps) {
-            const double E = t[IM_E], m = t[IM_M];
+            const double E = T(IM_E), m = T(IM_M);
             h = -(m * ld) / (cplx(1.0 - E) + E * ld);
         } else {
-            h = -lap / (lap + cplx(t[IM_RATE]));
+            h = -lap / (lap + cplx(T(IM_RATE)));
         }
-        const double un = t[IM_UN], up = t[IM_UP], tn = t[IM_TN], tp = t[IM_TP];
-        B[4] = B[4] + lap * t[IM_MN] + h * (tn * un);
+        const double un = T(IM_UN), up = T(IM_UP), tn = T(IM_TN), tp = T(IM_TP);
+        B[4] = B[4] + lap * T(IM_MN) + h * (tn * un);
         B[5] = B[5] + h * (tn * up);
         B[7] = B[7] + h * (tp * un);
-        B[8] = B[8] + lap * t[IM_MP] + h * (tp * up);
+        B[8] = B[8] + lap * T(IM_MP) + h * (tp * up);
         cplx rhs[3];
         if (r > 0) {
             for (int i = 0; i < 3; ++i) {
@@ -320,11 +368,11 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
         for (int i = 0; i < 3; ++i)
             dp[i] = inv[3 * i] * rhs[0] + inv[3 * i + 1] * rhs[1] + inv[3 * i + 2] * rhs[2];
         // the current's coefficients of this point's dy
-        const double gt = t[IM_GT];
+        const double gt = T(IM_GT);
         cplx c[3];
-        c[0] = cplx(t[IM_KV]) + lap * t[IM_KD];
-        c[1] = cplx(t[IM_G0N]) + lap * t[IM_GLN] + h * (un * gt);
-        c[2] = cplx(t[IM_G0P]) + lap * t[IM_GLP] + h * (up * gt);
+        c[0] = cplx(T(IM_KV)) + lap * T(IM_KD);
+        c[1] = cplx(T(IM_G0N)) + lap * T(IM_GLN) + h * (un * gt);
+        c[2] = cplx(T(IM_G0P)) + lap * T(IM_GLP) + h * (up * gt);
         if (r > 0) {
             cplx gn[3];
             for (int k = 0; k < 3; ++k)

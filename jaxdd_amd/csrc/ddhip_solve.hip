@@ -169,12 +169,52 @@ __global__ void __launch_bounds__(64) assemble_kernel(AsmArgs a) {
     if (d < a.n) assemble_lane<FIVE>(a, d);
 }
 
-// The impedance operator: the per-device tables (one lane per device), then one lane per
+// The impedance operator: the per-device tables (one wavefront per device), then one lane per
 // (device, frequency), the frequencies of a device on neighbouring lanes so that their reads of
 // its table coincide.
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+    return x;
+}
+
+// exclusive prefix sum over the wavefront's lanes
+__device__ inline double wave_exclusive_scan(double x, int lane) {
+    double s = x;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        const double y = __shfl_up(s, k);
+        if (lane >= k) s += y;
+    }
+    return s - x;
+}
+
 __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
-    const int d = blockIdx.x * 64 + threadIdx.x;
-    if (d < a.n) impedance_prep_lane(a, d);
+    const int d = blockIdx.x, lane = threadIdx.x, NP = a.NP;
+    if (imp_mobile_ions(a, d)) {
+        if (lane == 0) a.status[d] = 1;
+        return;
+    }
+    if (lane == 0) a.status[d] = 0;
+    const ImpDev k = imp_device(a, d);
+    for (int r = lane; r < NP; r += 64) imp_assemble_row(a, d, k, r);
+    __syncthreads();
+    const int seg = (NP + 63) / 64;
+    const int r0 = lane * seg < NP ? lane * seg : NP;
+    const int r1 = r0 + seg < NP ? r0 + seg : NP;
+    double tot[4], off[4];
+    imp_segment_totals(a, d, r0, r1, tot);
+    const double base[4] = {0.0, k.cn0, k.cp0, k.h0};
+    for (int q = 0; q < 4; ++q) off[q] = base[q] + wave_exclusive_scan(tot[q], lane);
+    double Dn, Dp;
+    imp_segment_prefix(a, d, r0, r1, off, Dn, Dp);
+    Dn = wave_sum(Dn);
+    Dp = wave_sum(Dp);
+    const double Sn = k.cn0 + wave_sum(tot[1]), Sp = k.cp0 + wave_sum(tot[2]);
+    const double H = k.h0 + wave_sum(tot[3]);
+    const ImpTot t = imp_totals(a, d, k, Sn, Sp, Dn, Dp, H, lane == 0);
+    __syncthreads();
+    for (int r = lane; r < NP; r += 64) imp_coefficients(a, d, k, t, r);
 }
 
 __global__ void __launch_bounds__(256) impedance_kernel(ImpArgs a) {
@@ -616,7 +656,7 @@ extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *
     a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_N;
     a.Y = Y; a.status = status;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3((b->n + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3(b->n), dim3(64), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "impedance_prep_kernel launch");
     const long lanes = (long)b->n * nf;

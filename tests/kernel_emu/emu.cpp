@@ -309,7 +309,34 @@ extern "C" int emu_impedance(int n, int NP, int nseg, const double *pt, const ui
     a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern; a.nf = nf;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.iface = iface;
     a.laps = laps; a.dt = dt; a.tab = tab; a.dsc = dsc; a.Y = Y; a.status = status;
-    for (int d = 0; d < n; ++d) impedance_prep_lane(a, d);
+    // the table's wavefront per device: its phases in turn, the scans across its lanes serial
+    for (int d = 0; d < n; ++d) {
+        if (imp_mobile_ions(a, d)) {
+            status[d] = 1;
+            continue;
+        }
+        status[d] = 0;
+        const ImpDev k = imp_device(a, d);
+        for (int r = 0; r < NP; ++r) imp_assemble_row(a, d, k, r);
+        const int seg = (NP + 63) / 64;
+        double tot[64][4], run[4] = {0.0, k.cn0, k.cp0, k.h0}, Dn = 0.0, Dp = 0.0;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r0 = lane * seg < NP ? lane * seg : NP;
+            const int r1 = r0 + seg < NP ? r0 + seg : NP;
+            imp_segment_totals(a, d, r0, r1, tot[lane]);
+        }
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r0 = lane * seg < NP ? lane * seg : NP;
+            const int r1 = r0 + seg < NP ? r0 + seg : NP;
+            double off[4] = {run[0], run[1], run[2], run[3]}, dn, dp;
+            imp_segment_prefix(a, d, r0, r1, off, dn, dp);
+            Dn += dn;
+            Dp += dp;
+            for (int q = 0; q < 4; ++q) run[q] += tot[lane][q];
+        }
+        const ImpTot t = imp_totals(a, d, k, run[1], run[2], Dn, Dp, run[3], true);
+        for (int r = 0; r < NP; ++r) imp_coefficients(a, d, k, t, r);
+    }
     for (int d = 0; d < n; ++d)
         for (int f = 0; f < nf; ++f) impedance_lane(a, d, f);
     return 0;

@@ -133,7 +133,7 @@ def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=Fa
     lib().emu_impedance(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(yl), _p(Vd),
                         _p(iface), _p(lp), nf, 0.0 if dt is None else float(dt),
                         0 if bernoulli == "exact" else 1, _p(tab), _p(dsc), _p(Y), _p(status))
-    if table:
+    if table:  # the table per point: (n, NP, fields)
         return Y[..., 0] + 1j * Y[..., 1], status, tab, dsc
     return Y[..., 0] + 1j * Y[..., 1], status
 
