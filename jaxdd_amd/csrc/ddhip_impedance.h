@@ -49,12 +49,15 @@ DDH_MFN cplx operator/(cplx a, cplx b) {
     return cplx((a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r);
 }
 
-// The per-device table [n][NP][IM_N] (a point's fields together, 384 bytes: three cache lines
+// The per-device table [n][NP][IM_ROW] (a point's fields together, 384 bytes: three cache lines
 // that the lanes of its device read at once): the steady blocks (row-major 3x3, the last row's C zero: its coupling is the
 // contact's), the time terms, the traps' occupation terms and the current's coefficients.
 enum ImpField {
-    IM_A = 0, IM_B = 9, IM_C = 18,
-    IM_MN = 27, IM_MP,     // -fac sC n, -fac sC p: lap times these on the continuity diagonals
+    // the blocks: A and C keep their five entries that are not structural zeros (each
+    // continuity row couples to V and to its own carrier only: [[0,0,0],[3,4,0],[6,0,8]]),
+    // B all nine
+    IM_A = 0, IM_B = 5, IM_C = 14,
+    IM_MN = 19, IM_MP,     // -fac sC n, -fac sC p: lap times these on the continuity diagonals
     IM_RATE,               // the trap level's rate a (1 without traps)
     IM_E, IM_M,            // e^(-a dt), (1 - e^(-a dt))/(a dt) (implicit Euler's steps)
     IM_UN, IM_UP,          // the steady occupation's drive per unit of ln n, ln p, over a
@@ -62,10 +65,10 @@ enum ImpField {
     IM_KV, IM_KD,          // the current's coefficient of dV: KV + lap KD
     IM_G0N, IM_G0P,        // ... of dln n: G0N + lap GLN + h UN GT; of dln p likewise
     IM_GLN, IM_GLP, IM_GT,
-    IM_DRN, IM_DRP,        // dR_n/df, dR_p/df
-    IM_NN, IM_PP,          // n, p
-    IM_TMP,                // the assembly's scratch (sum of h before the point)
-    IM_N
+    // the assembly's own: dR_n/df, dR_p/df, n, p, the sum of h before the point
+    IM_DRN, IM_DRP, IM_NN, IM_PP, IM_TMP,
+    IM_N,
+    IM_ROW = 48            // doubles per point: three 128-byte lines
 };
 // Per device [n][IS_N]: the drive (-dF/dV_int) of the first and the last row, the current's
 // direct dependence on V_int (IS_JV + lap IS_JD).
@@ -82,7 +85,7 @@ struct ImpArgs {
     const uint8_t *iface;   // [NP + 1][n] 1 where the interval after the point crosses layers
     const double *laps;     // [nf][2] lap (re, im)
     double dt;              // implicit Euler's step (0: the continuous response)
-    double *tab;            // [n][NP][IM_N]
+    double *tab;            // [n][NP][IM_ROW]
     double *dsc;            // [n][IS_N]
     double *Y;              // [n][nf][2] dJ/dV_int
     int32_t *status;        // [n] 0, or 1: mobile ions (not handled; Y = NaN)
@@ -93,7 +96,7 @@ struct ImpArgs {
 #define ISID(j) ((int)a.sid[(size_t)(j) * a.n + d])
 #define ISEG(id, f) a.seg[((size_t)(id) * SEG_N + (f)) * a.n + d]
 #define IY(j, f) a.y[(((size_t)(j) - 1) * 3 + (f)) * a.n + d]
-#define ITAB(f, r) a.tab[((size_t)d * a.NP + (r)) * IM_N + (f)]
+#define ITAB(f, r) a.tab[((size_t)d * a.NP + (r)) * IM_ROW + (f)]
 
 // The table is assembled by one wavefront per device in three phases: (A) each point's blocks,
 // time and trap terms and its parts of the current's integrals, lanes over the points; (B) the
@@ -169,11 +172,11 @@ DDH_FN void imp_assemble_row(const ImpArgs &a, int d, const ImpDev &k, int r) {
         for (int row = 0; row < 3; ++row) ds[IS_DN + row] = Cb[3 * row] * k.half;
         for (int q = 0; q < 9; ++q) Cb[q] = 0.0;
     }
-    for (int q = 0; q < 9; ++q) {
-        ITAB(IM_A + q, r) = Ab[q];
-        ITAB(IM_B + q, r) = Bb[q];
-        ITAB(IM_C + q, r) = Cb[q];
-    }
+    for (int q = 0; q < 9; ++q) ITAB(IM_B + q, r) = Bb[q];
+    ITAB(IM_A, r) = Ab[0]; ITAB(IM_A + 1, r) = Ab[3]; ITAB(IM_A + 2, r) = Ab[4];
+    ITAB(IM_A + 3, r) = Ab[6]; ITAB(IM_A + 4, r) = Ab[8];
+    ITAB(IM_C, r) = Cb[0]; ITAB(IM_C + 1, r) = Cb[3]; ITAB(IM_C + 2, r) = Cb[4];
+    ITAB(IM_C + 3, r) = Cb[6]; ITAB(IM_C + 4, r) = Cb[8];
     const double fs = -0.5 * k.L2 * cur.h * (mid.h + cur.h) * g.scs;  // -fac sC
     ITAB(IM_MN, r) = fs * mid.n;
     ITAB(IM_MP, r) = fs * mid.p;
@@ -319,23 +322,31 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
         return;
     }
     const int NP = a.NP;
-    const double *tab = a.tab + (size_t)d * NP * IM_N;
+    const double *tab = a.tab + (size_t)d * NP * IM_ROW;
     const double *ds = a.dsc + (size_t)d * IS_N;
     const cplx lap(a.laps[2 * f], a.laps[2 * f + 1]);
     const bool steps = a.dt > 0;
     const cplx ld = lap * a.dt;
     cplx Cp[9], dp[3], g[3];
     cplx acc(0.0, 0.0);
+    // the row's fields in registers, the next row's loads in flight during this row's work
+    double cur[IM_DRN], nxt[IM_DRN];
+#pragma unroll
+    for (int q = 0; q < IM_DRN; ++q) cur[q] = tab[q];
     for (int r = 0; r < NP; ++r) {
-        const double *t = tab + (size_t)r * IM_N;
-        auto T = [&](int f) { return t[f]; };
-        double A[9], C[9];
-        cplx B[9];
-        for (int q = 0; q < 9; ++q) {
-            A[q] = T(IM_A + q);
-            B[q] = cplx(T(IM_B + q));
-            C[q] = T(IM_C + q);
+        if (r + 1 < NP) {
+            const double *tn_ = tab + (size_t)(r + 1) * IM_ROW;
+#pragma unroll
+            for (int q = 0; q < IM_DRN; ++q) nxt[q] = tn_[q];
         }
+        auto T = [&](int f) { return cur[f]; };
+        // A = [[a0,0,0],[a3,a4,0],[a6,0,a8]], C likewise
+        const double a0 = T(IM_A), a3 = T(IM_A + 1), a4 = T(IM_A + 2), a6 = T(IM_A + 3),
+                     a8 = T(IM_A + 4);
+        const double c0 = T(IM_C), c3 = T(IM_C + 1), c4 = T(IM_C + 2), c6 = T(IM_C + 3),
+                     c8 = T(IM_C + 4);
+        cplx B[9];
+        for (int q = 0; q < 9; ++q) B[q] = cplx(T(IM_B + q));
         // the occupation beyond its steady response, per unit of it (impedance.py:104-117)
         cplx h;
         if (steps) {
@@ -351,13 +362,14 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
         B[8] = B[8] + lap * T(IM_MP) + h * (tp * up);
         cplx rhs[3];
         if (r > 0) {
-            for (int i = 0; i < 3; ++i) {
-                for (int k = 0; k < 3; ++k) {
-                    B[3 * i + k] = B[3 * i + k] - (A[3 * i] * Cp[k] + A[3 * i + 1] * Cp[3 + k] +
-                                                   A[3 * i + 2] * Cp[6 + k]);
-                }
-                rhs[i] = -(A[3 * i] * dp[0] + A[3 * i + 1] * dp[1] + A[3 * i + 2] * dp[2]);
+            for (int k = 0; k < 3; ++k) {
+                B[k] = B[k] - a0 * Cp[k];
+                B[3 + k] = B[3 + k] - (a3 * Cp[k] + a4 * Cp[3 + k]);
+                B[6 + k] = B[6 + k] - (a6 * Cp[k] + a8 * Cp[6 + k]);
             }
+            rhs[0] = -(a0 * dp[0]);
+            rhs[1] = -(a3 * dp[0] + a4 * dp[1]);
+            rhs[2] = -(a6 * dp[0] + a8 * dp[2]);
         }
         if (r == 0)
             for (int i = 0; i < 3; ++i) rhs[i] = cplx(ds[IS_D0 + i]);
@@ -381,10 +393,14 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
         } else {
             for (int k = 0; k < 3; ++k) g[k] = c[k];
         }
-        for (int i = 0; i < 3; ++i)
-            for (int k = 0; k < 3; ++k)
-                Cp[3 * i + k] = inv[3 * i] * C[k] + inv[3 * i + 1] * C[3 + k] + inv[3 * i + 2] * C[6 + k];
+        for (int i = 0; i < 3; ++i) {
+            Cp[3 * i] = inv[3 * i] * c0 + inv[3 * i + 1] * c3 + inv[3 * i + 2] * c6;
+            Cp[3 * i + 1] = inv[3 * i + 1] * c4;
+            Cp[3 * i + 2] = inv[3 * i + 2] * c8;
+        }
         acc = acc + g[0] * dp[0] + g[1] * dp[1] + g[2] * dp[2];
+#pragma unroll
+        for (int q = 0; q < IM_DRN; ++q) cur[q] = nxt[q];
     }
     const cplx Y = cplx(ds[IS_JV]) + lap * ds[IS_JD] + acc;
     Yo[0] = Y.re;

@@ -629,7 +629,7 @@ extern "C" int ddh_assemble_ions(const ddh_batch *b, const double *y, const doub
 
 extern "C" size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP) {
     if (n <= 0 || NP < 1) return 0;
-    return ((size_t)n * NP * ddh::IM_N + (size_t)n * ddh::IS_N) * sizeof(double);
+    return ((size_t)n * NP * ddh::IM_ROW + (size_t)n * ddh::IS_N) * sizeof(double);
 }
 
 extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint,
@@ -653,7 +653,7 @@ extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *
     a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
     a.y = y; a.Vint = Vint; a.iface = iface; a.laps = laps; a.dt = dt;
     a.tab = (double *)workspace;
-    a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_N;
+    a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_ROW;
     a.Y = Y; a.status = status;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3(b->n), dim3(64), 0, s, a);

@@ -114,6 +114,13 @@ def assemble_packed(packed, y, Vint, logs=None, bernoulli="exact"):
             U.reshape(NP, 3, 2, n).transpose(3, 0, 1, 2), J)
 
 
+#: The impedance table's fields (jaxdd_amd/csrc/ddhip_impedance.h `ImpField`).
+IMP_FIELDS = {name: k for k, name in enumerate(
+    ["A0", "A3", "A4", "A6", "A8"] + [f"B{q}" for q in range(9)] + ["C0", "C3", "C4", "C6", "C8"]
+    + ["MN", "MP", "RATE", "E", "M", "UN", "UP", "TN", "TP", "KV", "KD", "G0N", "G0P", "GLN",
+       "GLP", "GT", "DRN", "DRP", "NN", "PP", "TMP"])}
+
+
 def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=False):
     """The impedance operator's CPU replay: y (n, NP, 3) steady states, Vint (n,), laps
     (complex, nf) -> Y (n, nf) complex, status (n,); ``table``: also the per-device tables

@@ -202,14 +202,16 @@ def test_replay_current_coefficients_are_the_complex_step_derivatives(oracle, em
     d0 = np.array([[dJ(unit(i + 1, c), 0.0) for c in range(3)] for i in range(NP)])
     d1 = np.array([[dJ(unit(i + 1, c), 1.0) for c in range(3)] for i in range(NP)])
     lap_part = d1 - d0
-    # columns: GLN (dln n), GLP (dln p) per unit of lap; G0N, G0P at dti = 0
-    for col, ref in ((40, lap_part[:, 1]), (41, lap_part[:, 2]), (38, d0[:, 1]), (39, d0[:, 2])):
+    F = emu.IMP_FIELDS
+    # GLN (dln n), GLP (dln p) per unit of lap; G0N, G0P at dti = 0
+    for col, ref in ((F["GLN"], lap_part[:, 1]), (F["GLP"], lap_part[:, 2]),
+                     (F["G0N"], d0[:, 1]), (F["G0P"], d0[:, 2])):
         err = np.abs(tab[:, col] - ref)
         assert np.all(err <= 1e-6 * np.abs(ref) + 1e-12 * np.max(np.abs(ref))), col
     # KD (dV per unit of lap): eps's jumps over L H, 0 inside a layer (the complex step's
     # rounding there: 1e-15 of eps/(L H))
     scale = np.max(dev.eps) / (dev.L * np.sum(np.asarray(dev.h)[: NP + 1]))
-    assert np.all(np.abs(tab[:, 37] - lap_part[:, 0]) <= 1e-12 * scale)
+    assert np.all(np.abs(tab[:, F["KD"]] - lap_part[:, 0]) <= 1e-12 * scale)
     contacts = np.zeros_like(full)
     contacts[0, 0], contacts[-1, 0] = 0.5 * dev.left_elec, -0.5 * dev.left_elec
     jv0, jv1 = dJ(contacts, 0.0), dJ(contacts, 1.0)
