@@ -6,7 +6,7 @@ At the steady state at the internal voltage `voltage` (the solve kernel, keeping
 state) a small voltage `dV e^(iwt)` drives `(J + iw M) dy = -(dF/dV) dV`: `J` the steady
 residual's Jacobian, `M` implicit Euler's time terms, the bulk traps' occupations lagging at
 their own rates. `ddh_impedance` builds a per-device table of the frequency-independent terms
-(one GPU lane per device), then solves every (device, frequency) system on a lane of its own,
+(one wavefront per device), then solves every (device, frequency) system on a lane of its own,
 contracting the linearised transient current on the way; `Y = dJ/dV` in SIMsalabim's sign and
 the external `Z = 1/(Y + G_shunt) + R_series`.
 """
@@ -59,7 +59,7 @@ def external_impedance(Y, G_shunt, R_series) -> np.ndarray:
 
 
 def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
-                    bernoulli_name="exact", ion_flux="electrochemical", dt=None, lanes=2,
+                    bernoulli_name="exact", ion_flux="electrochemical", dt=None, lanes=None,
                     events=None, return_states=False) -> ImpedanceResult:
     """The impedance of a batch (`jaxdd_amd.batch.prepare`) at the internal voltage ``voltage``
     (V) at each of ``frequencies`` (Hz), from the steady state there (solved with ``tol`` and
@@ -67,8 +67,10 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
     bulk traps' occupations follow at their own rates. ``dt`` (s): instead of the continuous
     response, what implicit Euler's steps of ``dt`` see (the time terms at
     `(1 - e^(-iw dt))/dt`, the traps' occupations over each step). ``lanes``: the steady solve's
-    (`solve_batch`'s). ``events``: four torch.cuda.Event recorded on the launch stream around
-    the steady solve and around the impedance kernels. ``return_states``: also the steady
+    (`solve_batch`'s; default four, the grid in quarters, for NP >= 16: these devices have no
+    mobile ions, and the batches impedance is taken of are small). ``events``: four
+    torch.cuda.Event recorded on the launch stream around the steady solve and around the
+    impedance kernels. ``return_states``: also the steady
     states (n, NP, 3) the admittance was taken at.
 
     Raises `ValueError` for an unknown ``ion_flux`` or a ``dt`` that is not positive, and
@@ -80,6 +82,8 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
     if dt is not None and not (np.isfinite(dt) and dt > 0):
         raise ValueError(f"dt {dt!r}: a positive step (s), or None for the continuous response")
+    if lanes is None:
+        lanes = 4 if batch["NP"] >= 16 else 2
     if lanes not in (1, 2, 4):
         raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
     frequencies = np.atleast_1d(np.asarray(frequencies, dtype=float))
