@@ -208,180 +208,6 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: s
 #: with A's five entries 74, the block inverse by its adjugate 213, d' 66, C' from C's five
 #: entries 42, the current's coefficients 19, the contraction's g and its sum 96.
 IMP_FLOPS_PER_POINT = 550
-FP64_PEAK_TFLOPS = 78.6
-SEGMENT_COEFFS = 22  # s: per-point coefficients of the reference's stencils
-CHAOTIC_STEPS = 20
-
-
-def b_step(NP: int, k: int, w: int, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
-    """Algorithmic bytes per Newton step per device: state read+write, coefficients once,
-    factors written by the elimination and read by the back-substitution."""
-    return NP * (16 * b + 8 * s + 2 * w * (b * b + b * k))
-
-
-def parse():
-    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--n", type=int, default=16384)
-    p.add_argument("--config", default="c3", choices=["c2", "c3"])
-    p.add_argument("--variant", default="free", choices=["free", "frozen"])
-    p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    p.add_argument("--lanes", type=int, default=2, choices=[1, 2, 4],
-                   help="GPU lanes per device (2: each Newton solve split at the grid's middle;"
-                        " 4, in quarters: c2 only, the devices without mobile ions)")
-    p.add_argument("--grad-lanes", type=int, default=4, choices=[1, 2, 4],
-                   help="GPU lanes per device of the gradient workload's solve")
-    p.add_argument("--grad-precision", default="fp64", choices=["mixed", "fp64"],
-                   help="precision of the gradient workload's solve (the adjoint is FP64)")
-    p.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-curves", type=int, default=24, help="curves per CPU process")
-    p.add_argument("--workers", type=int, default=0, help="host processes building devices")
-    p.add_argument("--grad-n", type=int, default=4096,
-                   help="devices of the gradient workload (dJ/dtheta, 16 parameters); 0: skip")
-    p.add_argument("--imp-n", type=int, default=2048,
-                   help="devices of the impedance workload (NP=300, no mobile ions); 0: skip")
-    p.add_argument("--imp-freqs", type=int, default=50,
-                   help="frequencies of the impedance workload (log-spaced, 1e-2..1e7 Hz)")
-    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                   help="only copy MIB MiB device to device (HBM counter calibration)")
-    return p.parse_args()
-
-
-def host_cores() -> int:
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
-
-
-# ---- CPU baseline: the oracle, one pinned single-threaded process per core -----------------
-def _cpu_worker(core, rows, config, variant, start_evt, ready, out):
-    os.environ["OMP_NUM_THREADS"] = "1"
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
-    try:
-        os.sched_setaffinity(0, {core})
-    except (AttributeError, OSError):
-        pass
-    from jaxdd_amd import synthetic as S
-    from oracle import ddoracle as O
-
-    O.lib()
-    t = S.draw(rows[1], S.SEED)
-    ions = config == "c3"
-    devs = [S.device_from_row(t, k, 400, ions, ions and variant == "frozen")
-            for k in range(*rows)]
-    O.solve_batch(devs[:1], S.VOLTAGES)  # warm-up
-    ready.put(core)
-    start_evt.wait()
-    t0 = time.perf_counter()
-    J, ok, _ = O.solve_batch(devs, S.VOLTAGES)
-    t1 = time.perf_counter()
-    out.put((t0, t1, len(devs), float(ok.mean())))
-
-
-def cpu_baseline(config, variant, per_proc):
-    import multiprocessing as mp
-
-    ctx = mp.get_context("fork")
-    cores = host_cores()
-    try:
-        core_ids = sorted(os.sched_getaffinity(0))[:cores]
-    except AttributeError:
-        core_ids = list(range(cores))
-    start, ready, out = ctx.Event(), ctx.Queue(), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_worker,
-                         args=(c, (i * per_proc, (i + 1) * per_proc), config, variant, start,
-                               ready, out))
-             for i, c in enumerate(core_ids)]
-    for p in procs:
-        p.start()
-    for _ in procs:  # every worker has built its devices and warmed up
-        ready.get(timeout=600)
-    start.set()
-    res = [out.get(timeout=600) for _ in procs]
-    for p in procs:
-        p.join()
-    t0 = min(r[0] for r in res)
-    t1 = max(r[1] for r in res)
-    curves = sum(r[2] for r in res)
-    return {"value": curves / (t1 - t0), "unit": "J-V curves/s", "cores": len(procs),
-            "kind": "port",
-            "sample": f"{curves} curves ({per_proc} devices per process) of the same workload, "
-                      f"FP64 C oracle with the reference's two passes, {len(procs)} pinned "
-                      "single-threaded processes",
-            "per_core": curves / (t1 - t0) / len(procs), "converged": float(np.mean([r[3] for r in res]))}
-
-
-def copy_calibration(mib: int) -> None:
-    """A plain device-to-device copy of known size, for scaling rocprofv3's FETCH_SIZE and
-    WRITE_SIZE counters to bytes."""
-    import torch
-
-    n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
-
-
-def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
-    """Algorithmic bytes of the adjoint per converged (device, voltage): the assembly at the
-    converged state, the transposed sweep's factors written and read, the reverse pass's reads
-    of lambda (its contraction with the 16 directions' tangents reads each device's tangents
-    once for all its voltages)."""
-    return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
-
-
-def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: str = "fp64"):
-    """dJ/dtheta of the gradient workload's curves: one solve keeping the converged states
-    and one adjoint launch, with the batch and its tangents resident in HBM before the timed
-    region (the warm-up call has allocated the workspaces), timed with HIP events on the
-    launch stream and by the host clock around the call."""
-    import torch
-
-    from jaxdd_amd.adjoint import jv_gradients, resident
-
-    packed, tan = tangents_
-    n = packed["sc"].shape[1]
-    batch, dtan = resident(packed, tan)
-    if warm:
-        jv_gradients(batch, dtan, V, lanes=lanes, precision=precision)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev, lanes=lanes, precision=precision)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
-    ok = ok.cpu().numpy()
-    unit = adjoint_b_unit()
-    achieved = float(ok.sum()) * unit / (adj_ms * 1e-3) / 1e9
-    return {
-        "value": n / (t1 - t0), "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters)",
-        "devices": n, "voltages": len(V), "ms": (t1 - t0) * 1e3,
-        "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms, "solve_lanes_per_device": lanes,
-        "solve_precision": precision,
-        "converged_frac": float(ok.mean()),
-        "finite": bool(np.isfinite(grad.cpu().numpy()).all()),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "ddh::adjoint_kernel",
-                     "bytes_per_converged_voltage": unit},
-    }
-
-
-#: FP64 operations per (device, frequency, grid point) of ddh::impedance_kernel, counted from
-#: its code (complex multiply 6, complex add 2, complex divide 11, real x complex 2): the
-#: occupation's lag 12, the diagonal's time and trap terms 28, the elimination's A C' and A d'
-#: 138, the block inverse by its adjugate 213, d' 66, C' 90, the current's coefficients 19, the
-#: contraction's g and its sum 96.
-IMP_FLOPS_PER_POINT = 662
-FP64_PEAK_TFLOPS = 78.6
 
 
 def impedance_bench(packed, freqs, voltage=0.0):
