@@ -28,7 +28,7 @@ def _steady(oracle, dev, V, tol=1e-12):
     return st[0]
 
 
-def _close_to_oracle(Y, dev, y, V, laps, dt):
+def _close_to_oracle(Y, dev, y, V, laps, dt, bern="exact"):
     """Y (one device's, per lap) against the oracle at the same state; the bar is 20 times the
     spread of two eliminations that are equal in exact arithmetic (the reference's adjugate
     inverses and pivoted solves), relative to each |Y|: what the conditioning of the device's
@@ -37,8 +37,8 @@ def _close_to_oracle(Y, dev, y, V, laps, dt):
     from oracle import ddimpedance as I
 
     full = I.full_state(dev, y, V)
-    Yo = I.admittance(dev, full, laps, dt=dt)
-    Yp = I.admittance(dev, full, laps, dt=dt, pivoted=True)
+    Yo = I.admittance(dev, full, laps, dt=dt, bernoulli_name=bern)
+    Yp = I.admittance(dev, full, laps, dt=dt, bernoulli_name=bern, pivoted=True)
     spread = float(np.max(np.abs(Yo - Yp) / np.abs(Yo)))
     err = np.abs(np.asarray(Y) - Yo) / np.abs(Yo)
     assert np.max(err) <= 20 * spread + 1e-10, (np.max(err), spread)
@@ -149,25 +149,43 @@ def test_external_impedance_of_the_series_resistance_and_the_shunt():
 
 
 # ---- the kernels' CPU replay ----------------------------------------------------------------
+def _cases():
+    """(devices, voltage, Bernoulli): synthetic devices lit at three voltages and dark at one,
+    the SimSS fixture devices without ions with SimSS's Bernoulli (their interfaces move the
+    current's weights)."""
+    import dataclasses
+
+    lit = _synthetic(4, 150)
+    dark = [dataclasses.replace(d, G_frac=0.0) for d in lit]
+    fix = [fixture_device(c) for c in NO_ION_FIXTURES]
+    return ([(lit, V, "exact") for V in (0.0, V0, 1.0)] + [(dark, V0, "exact")]
+            + [(fix, V0, "simss")])
+
+
+def _steady_b(oracle, dev, V, bern):
+    J, ok, its, st = oracle.solve_one(dev, [V], tol=1e-12, max_iter=200, bernoulli=bern,
+                                      return_states=True)
+    assert ok.all()
+    return st[0]
+
+
 @pytest.mark.parametrize("dt", [None, 1e-7])
 def test_replay_matches_the_oracle(oracle, emu, dt):
     """The impedance kernels' code (the per-device table, the per-frequency elimination with the
     current contracted on the way) on the CPU against the oracle, which solves by the
     reference's block elimination and back-substitution and takes the current's derivatives by
-    the complex step: synthetic devices at three voltages and the SimSS fixture devices without
-    ions (their interfaces move the current's weights)."""
+    the complex step."""
     from jaxdd_amd.batch import pack
+
     from oracle import ddimpedance as I
 
     laps = I.laps_of(FREQS, dt)
-    cases = [(_synthetic(4, 150), V) for V in (0.0, V0, 1.0)]
-    cases.append(([fixture_device(c) for c in NO_ION_FIXTURES], V0))
-    for devs, V in cases:
-        ys = np.stack([_steady(oracle, d, V) for d in devs])
-        Y, status = emu.impedance_packed(pack(devs), ys, np.full(len(devs), V), laps, dt)
+    for devs, V, bern in _cases():
+        ys = np.stack([_steady_b(oracle, d, V, bern) for d in devs])
+        Y, status = emu.impedance_packed(pack(devs), ys, np.full(len(devs), V), laps, dt, bern)
         assert not status.any()
         for k, dev in enumerate(devs):
-            _close_to_oracle(Y[k], dev, ys[k], V, laps, dt)
+            _close_to_oracle(Y[k], dev, ys[k], V, laps, dt, bern)
 
 
 def test_replay_current_coefficients_are_the_complex_step_derivatives(oracle, emu):
@@ -266,11 +284,17 @@ def test_the_c_abi_rejects_bad_arguments():
 # ---- the GPU --------------------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def gpu_batches(torch_cuda):
+    """(devices, batch, Bernoulli): synthetic devices lit and dark, the SimSS fixture devices
+    without ions with SimSS's Bernoulli."""
+    import dataclasses
+
     from jaxdd_amd.batch import prepare
 
     syn = _synthetic(6, 150)
+    dark = [dataclasses.replace(d, G_frac=0.0) for d in syn]
     fix = [fixture_device(c) for c in NO_ION_FIXTURES]
-    return [(syn, prepare(syn)), (fix, prepare(fix))]
+    return [(syn, prepare(syn), "exact"), (dark, prepare(dark), "exact"),
+            (fix, prepare(fix), "simss")]
 
 
 @pytest.mark.gpu
@@ -279,13 +303,13 @@ def test_gpu_impedance_matches_the_oracle(gpu_batches, dt):
     """ddh_impedance through the C-ABI against the oracle at the GPU's own steady states."""
     from jaxdd_amd.impedance import laps_of, solve_impedance
 
-    for devs, batch in gpu_batches:
+    for devs, batch, bern in gpu_batches:
         r, states = solve_impedance(batch, FREQS, voltage=V0, tol=1e-12, max_iter=200, dt=dt,
-                                    return_states=True)
+                                    bernoulli_name=bern, return_states=True)
         assert r.converged.all()
         ys = states.cpu().numpy()
         for k, dev in enumerate(devs):
-            _close_to_oracle(r.Y[k], dev, ys[k], V0, laps_of(FREQS, dt), dt)
+            _close_to_oracle(r.Y[k], dev, ys[k], V0, laps_of(FREQS, dt), dt, bern)
         Z = 1.0 / (r.Y + batch["G_shunt"].cpu().numpy()[:, None]) + batch["R_series"].cpu().numpy()[:, None]
         assert np.allclose(r.Z, Z, rtol=1e-14)
 
@@ -298,10 +322,12 @@ def test_gpu_limits(gpu_batches):
     from jaxdd_amd.impedance import solve_impedance
     from oracle import ddimpedance as I
 
-    for devs, batch in gpu_batches:
-        r = solve_impedance(batch, [1e-7, 1e14], voltage=V0, tol=1e-12, max_iter=200)
+    for devs, batch, bern in gpu_batches:
+        r = solve_impedance(batch, [1e-7, 1e14], voltage=V0, tol=1e-12, max_iter=200,
+                            bernoulli_name=bern)
         dV = 1e-5
-        J, ok, _ = solve_batch(batch, [V0 - dV, V0 + dV], tol=1e-12, max_iter=200)
+        J, ok, _ = solve_batch(batch, [V0 - dV, V0 + dV], tol=1e-12, max_iter=200,
+                               bernoulli_name=bern)
         J = J.cpu().numpy()
         G_shunt = batch["G_shunt"].cpu().numpy()
         slope = (J[:, 1] - J[:, 0]) / (2 * dV) - G_shunt  # Y is the internal admittance
@@ -316,7 +342,7 @@ def test_gpu_series_resistance_and_shunt_leave_the_admittance(gpu_batches):
     shunt change Z = 1/(Y + G_shunt) + R_series, not Y."""
     from jaxdd_amd.impedance import solve_impedance
 
-    devs, batch = gpu_batches[0]
+    devs, batch, _ = gpu_batches[0]
     plain = solve_impedance(batch, [1e3, 1e6], voltage=V0)
     b2 = dict(batch, R_series=batch["R_series"] + 1e-4, G_shunt=batch["G_shunt"] + 10.0)
     r = solve_impedance(b2, [1e3, 1e6], voltage=V0)
