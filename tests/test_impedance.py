@@ -158,8 +158,9 @@ def _cases():
     lit = _synthetic(4, 150)
     dark = [dataclasses.replace(d, G_frac=0.0) for d in lit]
     fix = [fixture_device(c) for c in NO_ION_FIXTURES]
+    small = _synthetic(3, 20)  # fewer points than lanes: most of the table's segments empty
     return ([(lit, V, "exact") for V in (0.0, V0, 1.0)] + [(dark, V0, "exact")]
-            + [(fix, V0, "simss")])
+            + [(fix, V0, "simss"), (small, V0, "exact")])
 
 
 def _steady_b(oracle, dev, V, bern):
@@ -285,7 +286,8 @@ def test_the_c_abi_rejects_bad_arguments():
 @pytest.fixture(scope="module")
 def gpu_batches(torch_cuda):
     """(devices, batch, Bernoulli): synthetic devices lit and dark, the SimSS fixture devices
-    without ions with SimSS's Bernoulli."""
+    without ions with SimSS's Bernoulli, and devices of 20 points (fewer than a wavefront's
+    lanes)."""
     import dataclasses
 
     from jaxdd_amd.batch import prepare
@@ -293,8 +295,9 @@ def gpu_batches(torch_cuda):
     syn = _synthetic(6, 150)
     dark = [dataclasses.replace(d, G_frac=0.0) for d in syn]
     fix = [fixture_device(c) for c in NO_ION_FIXTURES]
+    small = _synthetic(3, 20)
     return [(syn, prepare(syn), "exact"), (dark, prepare(dark), "exact"),
-            (fix, prepare(fix), "simss")]
+            (fix, prepare(fix), "simss"), (small, prepare(small), "exact")]
 
 
 @pytest.mark.gpu
@@ -317,7 +320,9 @@ def test_gpu_impedance_matches_the_oracle(gpu_batches, dt):
 @pytest.mark.gpu
 def test_gpu_limits(gpu_batches):
     """On the GPU alone: at 1e-7 Hz the slope of the GPU's own steady J-V curve (+-10 uV, to
-    1e-5), at 1e14 Hz the geometric capacitance (to 1e-5)."""
+    1e-5), at 1e14 Hz the geometric capacitance (to 1e-5 on the grids of 150 and 400 points;
+    the discrete displacement current and Poisson stencil reach the continuum's 1/int dx/eps
+    only as the grid is refined)."""
     from jaxdd_amd.batch import solve_batch
     from jaxdd_amd.impedance import solve_impedance
     from oracle import ddimpedance as I
@@ -332,6 +337,8 @@ def test_gpu_limits(gpu_batches):
         G_shunt = batch["G_shunt"].cpu().numpy()
         slope = (J[:, 1] - J[:, 0]) / (2 * dV) - G_shunt  # Y is the internal admittance
         assert np.allclose(r.Y[:, 0].real, slope, rtol=1e-5, atol=0)
+        if batch["NP"] < 100:  # the continuum's limit: 1e-3 apart on 20 points, 1e-5 by 150
+            continue
         geo = np.array([I.geometric_capacitance(d) for d in devs])
         assert np.allclose(r.capacitance[:, 1], geo, rtol=1e-5, atol=0)
 
