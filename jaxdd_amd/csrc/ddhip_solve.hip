@@ -189,8 +189,17 @@ __device__ inline double wave_exclusive_scan(double x, int lane) {
     return s - x;
 }
 
+// Workgroups go to the 8 XCDs in turn, each with its own L2; a 128-byte line of the
+// lane-innermost batch arrays holds 16 neighbouring devices' values. Each XCD therefore takes a
+// contiguous run of devices (workgroup b: XCD b % 8, its (b / 8)-th device), so that a line it
+// fetches serves the next devices it runs.
+constexpr int XCDS = 8;
+
 __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
-    const int d = blockIdx.x, lane = threadIdx.x, NP = a.NP;
+    const int per = (a.n + XCDS - 1) / XCDS;
+    const int d = (int)(blockIdx.x % XCDS) * per + (int)(blockIdx.x / XCDS);
+    if (d >= a.n) return;
+    const int lane = threadIdx.x, NP = a.NP;
     if (imp_mobile_ions(a, d)) {
         if (lane == 0) a.status[d] = 1;
         return;
@@ -656,7 +665,8 @@ extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *
     a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_ROW;
     a.Y = Y; a.status = status;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3(b->n), dim3(64), 0, s, a);
+    const unsigned prep_groups = (unsigned)(((b->n + ddh::XCDS - 1) / ddh::XCDS) * ddh::XCDS);
+    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3(prep_groups), dim3(64), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "impedance_prep_kernel launch");
     const long lanes = (long)b->n * nf;
