@@ -242,7 +242,13 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
     const int lane = threadIdx.x;
     const size_t sl = (size_t)blockIdx.x * 64 + lane;
     const int nV = a.nV, K = a.ntheta;
-    for (int d = blockIdx.x; d < a.n; d += gridDim.x) {
+    // the XCDs in turn (as the impedance tables): each runs a contiguous run of devices, so
+    // that the lines of the lane-innermost arrays one wavefront fetches serve its neighbours
+    const int nb = (int)gridDim.x, G = nb < XCDS ? nb : XCDS;
+    const int x = (int)blockIdx.x % G, j = (int)blockIdx.x / G;
+    const int cx = (nb - x + G - 1) / G, per = (a.n + G - 1) / G;
+    for (int k = j; k < per && x * per + k < a.n; k += cx) {
+        const int d = x * per + k;
         const bool mobile = a.sc[(size_t)SC_AT * a.n + d] > 0 || a.sc[(size_t)SC_CT * a.n + d] > 0;
         if (mobile && lane == 0) a.status[d] = 1;
         __syncthreads();  // every lane is done with the previous device's table
