@@ -93,6 +93,26 @@ int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32_t nV, cons
                            double *J, uint8_t *ok, int32_t *its, int32_t *status, double *states,
                            void *workspace, size_t workspace_bytes, void *hip_stream);
 
+/* The device memory the library holds for ddh_solve_batch_noalloc, per GPU it runs on: 256 MiB,
+ * ddh_solve_batch's workspace for about 1,500 devices of NP = 400 (mixed precision). */
+#define DDH_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* DDH_NOALLOC_SCRATCH_BYTES, for callers that bind the library without its header. */
+size_t ddh_noalloc_scratch_bytes(void);
+
+/* ddh_solve_batch without allocating: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The workspace is the library's own static device array of
+ * DDH_NOALLOC_SCRATCH_BYTES on the current GPU (its address taken at this entry point's first
+ * call on that GPU); the call allocates and frees nothing. Arguments and results as
+ * ddh_solve_batch. A batch whose workspace (ddh_workspace_bytes for its lanes) exceeds the
+ * scratch is refused with DDH_E_WORKSPACE. Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams, other host threads) race and must be
+ * serialised by the caller. */
+int ddh_solve_batch_noalloc(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
+                            double *J, uint8_t *ok, int32_t *its, int32_t *status,
+                            void *hip_stream);
+
 /* Workspace bytes ddh_adjoint needs. */
 size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
 

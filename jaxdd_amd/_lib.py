@@ -32,7 +32,8 @@ class Opts(ctypes.Structure):
 SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solve_batch",
            "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
            "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions",
-           "ddh_impedance_workspace_bytes", "ddh_impedance")
+           "ddh_impedance_workspace_bytes", "ddh_impedance", "ddh_noalloc_scratch_bytes",
+           "ddh_solve_batch_noalloc")
 
 _lib = None
 
@@ -73,6 +74,11 @@ def load():
     lib.ddh_impedance.restype = ctypes.c_int
     lib.ddh_impedance.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, i32, ctypes.c_double,
                                   i32, vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_noalloc_scratch_bytes.restype = ctypes.c_size_t
+    lib.ddh_noalloc_scratch_bytes.argtypes = []
+    lib.ddh_solve_batch_noalloc.restype = ctypes.c_int
+    lib.ddh_solve_batch_noalloc.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts),
+                                            vp, vp, vp, vp, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 2:

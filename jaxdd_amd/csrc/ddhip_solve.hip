@@ -7,6 +7,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
@@ -528,6 +530,64 @@ extern "C" int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32
                                       size_t workspace_bytes, void *stream) {
     if (!states) return fail(DDH_E_ARG, "ddh_solve_batch_states: NULL states");
     return solve_batch(b, V, nV, o, J, ok, its, status, states, workspace, workspace_bytes, stream);
+}
+
+// ddh_solve_batch without an allocation: the workspace is this library's own device array of
+// DDH_NOALLOC_SCRATCH_BYTES, one copy per GPU, held until the process ends. In host code the
+// array's name is only a stand-in for its symbol; its device address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and only that address
+// is handed to the kernels. Every call on one GPU shares its copy (the solve initialises every
+// part of the workspace it reads, so nothing needs clearing between calls); calls that overlap
+// on one GPU race.
+static __device__ __align__(256) unsigned char scratch[DDH_NOALLOC_SCRATCH_BYTES];
+
+// the scratch's device address on each GPU, by device ordinal
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+static int scratch_address(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "ddh_solve_batch_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(DDH_E_ARG, "ddh_solve_batch_noalloc: device ordinal beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "ddh_solve_batch_noalloc: hipGetSymbolAddress");
+        if (!p) return fail(DDH_E_HIP, "ddh_solve_batch_noalloc: no device address for the scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DDH_OK;
+}
+
+extern "C" size_t ddh_noalloc_scratch_bytes(void) { return (size_t)DDH_NOALLOC_SCRATCH_BYTES; }
+
+extern "C" int ddh_solve_batch_noalloc(const ddh_batch *b, const double *V, int32_t nV,
+                                       const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
+                                       int32_t *status, void *stream) {
+    if (!b || !o) return fail(DDH_E_ARG, "ddh_solve_batch_noalloc: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536)
+        return fail(DDH_E_SHAPE, "ddh_solve_batch_noalloc: n > 0, 2 <= NP <= 65536");
+    if (o->precision != DDH_FP64 && o->precision != DDH_MIXED)
+        return fail(DDH_E_ARG, "ddh_solve_batch_noalloc: precision must be DDH_FP64 or DDH_MIXED");
+    const int lanes = o->jv_mode ? 1 : (o->lanes == 0 ? 2 : o->lanes);
+    const size_t need = ws_layout(b->n, b->NP, o->precision, lanes).total;
+    if (need > (size_t)DDH_NOALLOC_SCRATCH_BYTES) {
+        snprintf(g_err, sizeof(g_err),
+                 "ddh_solve_batch_noalloc: %d devices of NP = %d need %zu workspace bytes, more "
+                 "than the library's scratch of %zu (use ddh_solve_batch with a workspace)",
+                 b->n, b->NP, need, (size_t)DDH_NOALLOC_SCRATCH_BYTES);
+        return DDH_E_WORKSPACE;
+    }
+    void *ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != DDH_OK) return rc;
+    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, ws,
+                       (size_t)DDH_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 struct AdjLayout {
