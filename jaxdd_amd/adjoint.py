@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from . import synthetic as S
-from .batch import SC_FIELDS, check_four_lanes, from_packed, pack
+from .batch import SC_FIELDS, _sc_now, check_four_lanes, from_packed, pack
 
 THETA = ("L_etl", "L_pvk", "L_htl",
          "mu_n_etl", "mu_p_etl", "mu_n_pvk", "mu_p_pvk", "mu_n_htl", "mu_p_htl",
@@ -106,8 +106,12 @@ def tangents(thetas, NP: int = 400, workers: int = 1, chunk: int = 64):
     else:
         import multiprocessing as mp
 
-        with mp.get_context("fork").Pool(workers) as pool:
+        pool = mp.get_context("fork").Pool(workers)
+        try:
             parts = pool.map(_tangent_rows, jobs)
+        finally:
+            pool.close()  # let the workers exit on their own (no SIGTERM from terminate())
+            pool.join()
     nseg = max(p[0]["nseg"] for p in parts)
     if any(p[0]["nseg"] != nseg for p in parts):
         raise ValueError("devices with different segment counts in one gradient batch")
@@ -165,8 +169,10 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
     nbytes = lib.ddh_workspace_bytes(n, NP, nV, prec)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    # the per-device keys the caller may have replaced (g_frac, J_scale, V_pre), as solve_batch
+    sc = _sc_now(batch)
     b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
-                   batch["_seg"].data_ptr(), batch["_sc"].data_ptr())
+                   batch["_seg"].data_ptr(), sc.data_ptr())
     stream = torch.cuda.current_stream(dev).cuda_stream
     K = tan["pt"].shape[0]
     dpt, dseg, dsc = (tan[k] if isinstance(tan[k], torch.Tensor)
@@ -194,8 +200,8 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     torch.cuda.current_stream(dev).synchronize()
     if int(astatus.max()) != 0:
         raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
-    G_shunt = batch["_sc"][SC_FIELDS.index("G_shunt")]
-    J = J + G_shunt[:, None] * Vd[None, :]
+    del sc, ws, aws  # the launches that read them are done
+    J = J + batch["G_shunt"][:, None] * Vd[None, :]
     if return_states:
         return J, ok.bool(), its.long(), grad, states
     return J, ok.bool(), its.long(), grad
@@ -204,7 +210,15 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
 class _Curves(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, voltages, tol, max_iter, workers):
-        packed, tan = tangents(theta.detach().cpu().numpy(), workers=workers)
+        th = theta.detach().cpu().numpy()
+        if not ctx.needs_input_grad[0]:  # no backward will follow: the curves only
+            from .batch import solve_batch
+
+            batch = from_packed(pack([device_from_theta(t) for t in np.atleast_2d(th)]))
+            J, ok, _ = solve_batch(batch, voltages, tol=tol, max_iter=max_iter)
+            ctx.mark_non_differentiable(ok)
+            return J.to(theta.device), ok.to(theta.device)
+        packed, tan = tangents(th, workers=workers)
         J, ok, its, grad = jv_gradients(packed, tan, voltages, tol=tol, max_iter=max_iter)
         ctx.save_for_backward(grad.to(theta.device))
         ctx.mark_non_differentiable(ok)

@@ -21,7 +21,8 @@ from . import _lib
 from .constants import Q
 from .device import Device
 
-__all__ = ["external_voltages", "from_packed", "merge", "pack", "prepare", "solve_batch"]
+__all__ = ["external_voltages", "from_packed", "merge", "pack", "prepare", "solve_batch",
+           "solve_states"]
 
 # Field orders of jaxdd_amd/csrc/ddhip_layout.h.
 PT_FIELDS = ("h", "x", "G", "fix")
@@ -107,9 +108,12 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     iface = np.zeros((P - 1, n), dtype=np.uint8)
     for k, d in enumerate(devices):
         iface[list(d.i1[:-1]), k] = 1
+    # each species' fixed density apart (the kernels read their difference, `pt`'s "fix"): the
+    # states' anion and cation profiles (`solve_states`)
+    fixed = np.stack([np.stack([d.anion_fixed, d.cation_fixed]) for d in devices], axis=-1)
     return {"pt": pt, "sid": np.ascontiguousarray(sid.T.astype(np.uint8)), "seg": seg,
             "sc": sc, "nseg": nseg, "R_series": np.array([d.R_series for d in devices]),
-            "iface": iface}
+            "iface": iface, "fixed": np.ascontiguousarray(fixed)}
 
 
 def merge(parts: list[dict]) -> dict:
@@ -123,7 +127,7 @@ def merge(parts: list[dict]) -> dict:
             seg = np.concatenate([seg, np.repeat(seg[-1:], nseg - seg.shape[0], axis=0)])
         segs.append(seg)
     out = {"nseg": nseg, "seg": np.concatenate(segs, axis=2)}
-    for k in ("pt", "sid", "sc", "iface"):
+    for k in ("pt", "sid", "sc", "iface", "fixed"):
         out[k] = np.ascontiguousarray(np.concatenate([p[k] for p in parts], axis=-1))
     out["R_series"] = np.concatenate([p["R_series"] for p in parts])
     return out
@@ -157,6 +161,8 @@ def from_packed(packed: dict, device: str | torch.device = "cuda") -> dict:
         "fix_ions": t(col["fix_ions"]), "left_elec": t(col["left_elec"]),
         "G_shunt": t(col["G_shunt"]), "R_series": t(packed["R_series"]),
         "x": t(packed["pt"][:, 1, :].T), "h": t(packed["pt"][:, 0, :].T),
+        # host only: (2, NP + 2, n) fixed anions and cations, read by solve_states
+        "_fixed": packed.get("fixed"),
     }
 
 
@@ -179,10 +185,13 @@ def _sc_now(batch: dict) -> torch.Tensor:
     return sc
 
 
-def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=None):
+def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=None,
+         states: torch.Tensor | None = None):
     """One call of ddh_solve_batch: (J, ok, its, status) as (n, nV) tensors on the batch's
     device, V as given (sorted by the caller in batch mode). ``events``: a pair of
-    torch.cuda.Event recorded on the launch stream just before and after the solve kernel."""
+    torch.cuda.Event recorded on the launch stream just before and after the solve kernel.
+    ``states``: a (nV, NP, 3, n) float64 tensor on the batch's device to receive each
+    voltage's final state (ddh_solve_batch_states)."""
     lib = _lib.load()
     dev = batch["_pt"].device
     if dev.type != "cuda":
@@ -201,10 +210,19 @@ def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=Non
     stream = torch.cuda.current_stream(dev).cuda_stream
     if events is not None:
         events[0].record()
-    rc = lib.ddh_solve_batch(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
-                             J.data_ptr(), ok.data_ptr(), its.data_ptr(), status.data_ptr(),
-                             ws.data_ptr(), nbytes, stream)
-    _lib.check(rc, "ddh_solve_batch")
+    if states is None:
+        rc = lib.ddh_solve_batch(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
+                                 J.data_ptr(), ok.data_ptr(), its.data_ptr(), status.data_ptr(),
+                                 ws.data_ptr(), nbytes, stream)
+        _lib.check(rc, "ddh_solve_batch")
+    else:
+        if tuple(states.shape) != (nV, NP, 3, n) or states.dtype != torch.float64:
+            raise ValueError(f"states: float64 of shape {(nV, NP, 3, n)}")
+        rc = lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
+                                        J.data_ptr(), ok.data_ptr(), its.data_ptr(),
+                                        status.data_ptr(), states.data_ptr(), ws.data_ptr(),
+                                        nbytes, stream)
+        _lib.check(rc, "ddh_solve_batch_states")
     if events is not None:
         events[1].record()
     # the workspace and the inputs live until the kernels are done
@@ -250,6 +268,142 @@ def solve_batch(batch, voltages, tol=1e-10, max_iter=60, first_max_iter=6,
     J, ok, its = J[:, inv], ok[:, inv].bool(), its[:, inv].long()
     Vt = torch.as_tensor(V, device=J.device)
     return J + batch["G_shunt"][:, None] * Vt[None, :], ok, its
+
+
+def _point_field(batch: dict, name: str) -> torch.Tensor:
+    """(n, NP + 2) one material field (`SEG_FIELDS`) on every point of every device, read
+    through the points' segment ids."""
+    seg = batch["_seg"][:, SEG_FIELDS.index(name), :]  # (nseg, n)
+    return torch.gather(seg, 0, batch["_sid"].long()).T  # (n, NP + 2)
+
+
+def solve_states(batch, voltages=0.0, tol=1e-10, max_iter=60, first_max_iter=6,
+                 bernoulli_name="exact", strict=False, devices=None, precision="fp64", lanes=2):
+    """The steady states of a batch at ``voltages`` (one or several, internal) and what
+    photoluminescence measures of them (reference: src/jaxdd/batch.py:1165-1191
+    `solve_states`, with `_observed` :1009-1030 and src/jaxdd/solver.py:2096-2102
+    `_ions_on_grid`). A dict of tensors on the batch's GPU: ``V``, ``n``, ``p``, ``qfls``,
+    ``anion``, ``cation`` (batch x voltages x NP + 2: the contacts' points their boundary
+    values, the ions' their neighbours'; ``V`` in its gauge, densities in m^-3, ``qfls`` in V),
+    ``implied_voc`` (V) and ``plqy`` of direct recombination, ``J``, ``converged``,
+    ``iterations`` (batch x voltages). Solved as `solve_batch` (same arguments and results):
+    the solve kernel keeps each voltage's final state (ddh_solve_batch_states). Mobile ions
+    take Boltzmann's normalisation of their region's total at the solved potential (closed
+    form, src/jaxdd/solver.py:1695-1712); ions frozen after a pre-bias take the profile of the
+    free-ion solve at the pre-bias, as the reference's freeze does (batch.py:859-872)."""
+    del first_max_iter, devices
+    if precision not in _lib.PRECISIONS:
+        raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
+    if bernoulli_name not in ("exact", "simss"):
+        raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
+    if lanes not in (1, 2, 4):
+        raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
+    if lanes == 4:
+        check_four_lanes(batch)
+    V = np.atleast_1d(np.asarray(voltages, dtype=float)).ravel()
+    order = np.argsort(V, kind="stable")
+    n, NP, nV = batch["n"], batch["NP"], len(V)
+    dev = batch["_pt"].device
+    prec = _lib.PRECISIONS[precision]
+    opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
+                     0 if bernoulli_name == "exact" else 1, prec, 0, int(lanes))
+    st = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
+    J, ok, its, _ = _run(batch, V[order], opts, prec, states=st)
+    inv = torch.as_tensor(np.argsort(order, kind="stable"), device=dev)
+    J, ok, its = J[:, inv], ok[:, inv].bool(), its[:, inv].long()
+    Vt_in = torch.as_tensor(V, device=dev)
+    J = J + batch["G_shunt"][:, None] * Vt_in[None, :]
+    y = st[inv.cpu()].permute(3, 0, 1, 2)  # (n, nV, NP, 3) in the caller's order
+    full = _full_states(batch, y, Vt_in)
+    an, ca = _ions_on_grid(batch, full[..., 0])
+    frozen = _sc_now(batch)[SC_FIELDS.index("pre_cond")] * batch["fix_ions"] > 0
+    if bool(frozen.any()):  # the pre-bias's free-ion profile, held at every voltage
+        an_f, ca_f = _frozen_ions(batch, frozen, tol, max_iter, bernoulli_name, strict,
+                                  precision, lanes)
+        an = torch.where(frozen[:, None, None], an_f[:, None, :], an)
+        ca = torch.where(frozen[:, None, None], ca_f[:, None, :], ca)
+    lnn, lnp = full[..., 1], full[..., 2]
+    nn, pp = torch.exp(lnn), torch.exp(lnp)
+    ni2 = _point_field(batch, "ni2")[:, None, :]
+    Vt = batch["Vt"][:, None]
+    qfls = Vt[..., None] * torch.log(nn * pp / ni2)
+    # control volumes of the interior points, direct recombination and generation
+    h, L = batch["h"], batch["L"][:, None]
+    vol = 0.5 * L * (h[:, :-2] + h[:, 1:-1])  # (n, NP)
+    kd = _point_field(batch, "kd")[:, 1:-1]
+    G = batch["_pt"][:, PT_FIELDS.index("G"), :].T[:, 1:-1]
+    rad = torch.sum((vol * kd)[:, None, :] * nn[..., 1:-1] * pp[..., 1:-1], dim=-1)
+    rad0 = torch.sum(vol * kd * ni2[:, 0, 1:-1], dim=-1)[:, None]
+    gen = (batch["g_frac"] * torch.sum(vol * G, dim=-1))[:, None]
+    nan = torch.full_like(rad, float("nan"))
+    implied = torch.where(rad0 > 0, Vt * torch.log(rad / torch.where(rad0 > 0, rad0, 1.0)), nan)
+    plqy = torch.where(gen > 0, (rad - rad0) / torch.where(gen > 0, gen, 1.0), nan)
+    return {"V": full[..., 0], "n": nn, "p": pp, "qfls": qfls, "anion": an, "cation": ca,
+            "implied_voc": implied, "plqy": plqy, "J": J, "converged": ok, "iterations": its}
+
+
+def _full_states(batch: dict, y: torch.Tensor, Vint: torch.Tensor) -> torch.Tensor:
+    """(n, nV, NP + 2, 3) states on every point: the interior's ``y`` (n, nV, NP, 3) between
+    the contacts' boundary values at internal voltages ``Vint`` (nV,) (src/jaxdd/batch.py:255-267
+    `_bounds`, `_full`)."""
+    sc = batch["_sc"]
+
+    def col(name):
+        return sc[SC_FIELDS.index(name)][:, None]
+
+    elec = col("left_elec")
+    n, nV = y.shape[0], y.shape[1]
+    left = torch.stack([col("V0") + 0.5 * elec * Vint[None, :],
+                        col("ln_n_left").expand(n, nV), col("ln_p_left").expand(n, nV)], -1)
+    right = torch.stack([col("VL") - 0.5 * elec * Vint[None, :],
+                         col("ln_n_right").expand(n, nV), col("ln_p_right").expand(n, nV)], -1)
+    return torch.cat([left[:, :, None], y, right[:, :, None]], dim=2)
+
+
+def _ions_on_grid(batch: dict, Vfull: torch.Tensor):
+    """Anions and cations (n, nV, NP + 2) at potentials ``Vfull`` (n, nV, NP + 2): fixed,
+    plus Boltzmann's profile where mobile with each region's closed-form normalisation of
+    its total, trapezoidal over the region (src/jaxdd/solver.py:951-961, :1241-1252,
+    :1695-1712); the contacts' points their neighbours' (:2096-2102)."""
+    sc = _sc_now(batch)
+    Vt = sc[SC_FIELDS.index("Vt")][:, None, None]
+    hL = (batch["h"] * batch["L"][:, None])[:, None, :-1]  # (n, 1, NP + 1)
+    fixed = batch.get("_fixed")
+    out = []
+    for sign, mask_name, total_name, k in ((1.0, "am", "a_total", 0), (-1.0, "cm", "c_total", 1)):
+        m = _point_field(batch, mask_name)[:, None, :]  # (n, 1, NP + 2)
+        total = sc[SC_FIELDS.index(total_name)][:, None]
+        prof = m * torch.exp(sign * Vfull / Vt)
+        pair = (m[..., 1:] * m[..., :-1])
+        number = torch.sum(0.5 * (prof[..., 1:] + prof[..., :-1]) * pair * hL, dim=-1)
+        has = total > 0
+        scale = torch.where(has, total / torch.where(has & (number > 0), number, 1.0), 0.0)
+        dens = prof * scale[..., None]
+        if fixed is not None:
+            dens = dens + torch.as_tensor(fixed[k].T, device=dens.device)[:, None, :]
+        dens = torch.cat([dens[..., 1:2], dens[..., 1:-1], dens[..., -2:-1]], dim=-1)
+        out.append(dens)
+    return out[0], out[1]
+
+
+def _frozen_ions(batch, frozen, tol, max_iter, bernoulli_name, strict, precision, lanes):
+    """(n, NP + 2) anions and cations of the free-ion steady state at each device's pre-bias:
+    the profiles the reference's freeze holds (src/jaxdd/batch.py:859-872, 955-960)."""
+    sc = batch["_sc"].clone()
+    sc[SC_FIELDS.index("pre_cond")] = 0.0
+    sc[SC_FIELDS.index("fix_ions")] = 0.0
+    free = dict(batch, _sc=sc, pre_cond=torch.zeros_like(batch["pre_cond"]),
+                fix_ions=torch.zeros_like(batch["fix_ions"]))
+    # one solve per distinct pre-bias (the reference's V_pre is a per-device scalar)
+    V_pre = batch["V_pre"]
+    an = torch.zeros((batch["n"], batch["NP"] + 2), dtype=torch.float64, device=V_pre.device)
+    ca = torch.zeros_like(an)
+    for v in torch.unique(V_pre[frozen]).tolist():
+        out = solve_states(free, [v], tol, max_iter, None, bernoulli_name, strict, None,
+                           precision, lanes)
+        sel = frozen & (V_pre == v)
+        an[sel], ca[sel] = out["anion"][sel, 0], out["cation"][sel, 0]
+    return an, ca
 
 
 def assemble(batch, y, Vint, logs=None, bernoulli_name="exact"):

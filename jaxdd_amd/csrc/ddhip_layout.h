@@ -18,6 +18,13 @@
 
 #include <stdint.h>
 
+#ifdef DDH_CPU_EMULATION
+#define DDH_MAP_FN inline
+#else
+#include <hip/hip_runtime.h>
+#define DDH_MAP_FN __host__ __device__ inline
+#endif
+
 namespace ddh {
 
 enum PtField { PT_H = 0, PT_X, PT_G, PT_FIX, PT_N };
@@ -62,5 +69,37 @@ enum ScalarField {
 constexpr int NBUF = 6;
 // Factors per grid point: c' (3x3) and d' (3 x 3 right-hand sides).
 constexpr int NFAC = 18;
+
+// Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8), each XCD with its own L2,
+// and a 128-byte line of the lane-innermost batch arrays holds 16 neighbouring devices' values.
+// A launch that gives each workgroup one device at a time maps workgroup b's k-th device so
+// that every XCD takes a contiguous run of the n devices: XCD x the devices
+// [x * per, (x + 1) * per), per = ceil(n / G) over G = min(8, workgroups) XCDs, its c
+// workgroups (the x-th of each round of G) the run's devices j, j + c, j + 2c, ... Each device
+// goes to exactly one (workgroup, k); -1 past the run's end. With as many workgroups as
+// devices rounded up to a multiple of 8 every workgroup takes at most one device.
+constexpr int XCDS = 8;
+
+struct XcdMap {
+    int x, j, c, per;
+};
+
+DDH_MAP_FN XcdMap xcd_map(int block, int nblocks, int n) {
+    const int G = nblocks < XCDS ? nblocks : XCDS;
+    XcdMap m;
+    m.x = block % G;
+    m.j = block / G;
+    m.c = (nblocks - m.x + G - 1) / G;  // workgroups of XCD x
+    m.per = (n + G - 1) / G;
+    return m;
+}
+
+// Workgroup's k-th device (k = 0, 1, ...), or -1 when it has no k-th device.
+DDH_MAP_FN int xcd_device(const XcdMap &m, int k, int n) {
+    const int i = m.j + k * m.c;
+    if (i >= m.per) return -1;
+    const int d = m.x * m.per + i;
+    return d < n ? d : -1;
+}
 
 }  // namespace ddh

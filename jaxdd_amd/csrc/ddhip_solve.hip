@@ -191,16 +191,11 @@ __device__ inline double wave_exclusive_scan(double x, int lane) {
     return s - x;
 }
 
-// Workgroups go to the 8 XCDs in turn, each with its own L2; a 128-byte line of the
-// lane-innermost batch arrays holds 16 neighbouring devices' values. Each XCD therefore takes a
-// contiguous run of devices (workgroup b: XCD b % 8, its (b / 8)-th device), so that a line it
-// fetches serves the next devices it runs.
-constexpr int XCDS = 8;
-
+// One device per workgroup, each XCD a contiguous run of devices (xcd_map, ddhip_layout.h), so
+// that a line of the lane-innermost arrays it fetches serves the next devices it runs.
 __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
-    const int per = (a.n + XCDS - 1) / XCDS;
-    const int d = (int)(blockIdx.x % XCDS) * per + (int)(blockIdx.x / XCDS);
-    if (d >= a.n) return;
+    const int d = xcd_device(xcd_map((int)blockIdx.x, (int)gridDim.x, a.n), 0, a.n);
+    if (d < 0) return;
     const int lane = threadIdx.x, NP = a.NP;
     if (imp_mobile_ions(a, d)) {
         if (lane == 0) a.status[d] = 1;
@@ -246,13 +241,12 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
     const int nV = a.nV, K = a.ntheta;
     // the XCDs in turn (as the impedance tables): each runs a contiguous run of devices, so
     // that the lines of the lane-innermost arrays one wavefront fetches serve its neighbours
-    const int nb = (int)gridDim.x, G = nb < XCDS ? nb : XCDS;
-    const int x = (int)blockIdx.x % G, j = (int)blockIdx.x / G;
-    const int cx = (nb - x + G - 1) / G, per = (a.n + G - 1) / G;
-    for (int k = j; k < per && x * per + k < a.n; k += cx) {
-        const int d = x * per + k;
+    const XcdMap map = xcd_map((int)blockIdx.x, (int)gridDim.x, a.n);
+    for (int k = 0;; ++k) {
+        const int d = xcd_device(map, k, a.n);
+        if (d < 0) break;
         const bool mobile = a.sc[(size_t)SC_AT * a.n + d] > 0 || a.sc[(size_t)SC_CT * a.n + d] > 0;
-        if (mobile && lane == 0) a.status[d] = 1;
+        if (lane == 0) a.status[d] = mobile ? 1 : 0;
         __syncthreads();  // every lane is done with the previous device's table
         adj_stage_values(a, d, tab, lane, 64);
         __syncthreads();

@@ -168,6 +168,10 @@ def packed(config: str, n: int, seed: int = SEED, variant: str = "free", workers
     else:
         import multiprocessing as mp
 
-        with mp.get_context("fork").Pool(workers) as pool:
+        pool = mp.get_context("fork").Pool(workers)
+        try:
             parts = pool.map(_pack_rows, jobs)
+        finally:
+            pool.close()  # let the workers exit on their own (no SIGTERM from terminate())
+            pool.join()
     return merge(parts)

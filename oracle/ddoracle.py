@@ -271,6 +271,84 @@ def current(dev, full, g_frac=None):
     return lib().ddo_current(ctypes.byref(pin.struct), _ptr(full), pin.struct.g_frac)
 
 
+def solve_states(dev, voltages, tol=1e-10, max_iter=60, bernoulli_name="exact", g_frac=None):
+    """`solve_states` for one device (src/jaxdd/batch.py:1165-1191 with `_observed`
+    :1009-1030): the states at ``voltages`` on every point and what photoluminescence measures
+    of them, per voltage (the caller's order), as a dict of NumPy arrays: V, n, p, qfls, anion,
+    cation (nV x NP + 2), implied_voc, plqy, J (with the shunt), converged, iterations (nV).
+    One pass with ``max_iter`` (the two passes' result)."""
+    import dataclasses
+
+    V = np.asarray(voltages, dtype=float).ravel()
+    order = np.argsort(V, kind="stable")
+    J, ok, its, st = solve_one(dev, V[order], tol, max_iter, max_iter, False, bernoulli_name,
+                               g_frac, return_states=True)
+    inv = np.argsort(order, kind="stable")
+    J, ok, its, st = J[inv], ok[inv], its[inv], st[inv]
+    nV, NP = len(V), len(dev.x) - 2
+    full = np.zeros((nV, NP + 2, 3))
+    full[:, 1:-1] = st
+    # the contacts (batch.py:255-259 `_bounds`)
+    full[:, 0] = np.stack([dev.V0 + 0.5 * dev.left_elec * V,
+                           np.full(nV, np.log(dev.n_left)), np.full(nV, np.log(dev.p_left))], -1)
+    full[:, -1] = np.stack([dev.VL - 0.5 * dev.left_elec * V,
+                            np.full(nV, np.log(dev.n_right)), np.full(nV, np.log(dev.p_right))],
+                           -1)
+    gf = dev.G_frac if g_frac is None else g_frac
+    if dev.pre_cond and dev.fix_ions:
+        # the freeze (batch.py:859-872): the free-ion profile of the pre-bias's steady state
+        free = dataclasses.replace(dev, pre_cond=False, fix_ions=False)
+        pre = solve_states(free, [dev.V_pre], tol, max_iter, bernoulli_name, g_frac)
+        anion = np.repeat(pre["anion"], nV, axis=0)
+        cation = np.repeat(pre["cation"], nV, axis=0)
+    else:
+        anion, cation = _ions_on_grid(dev, full[:, :, 0])
+    qfls, implied, plqy = observed(dev, full, gf)
+    shunt = 1.0 / dev.R_shunt if dev.R_shunt > 0 else 0.0
+    return {"V": full[..., 0], "n": np.exp(full[..., 1]), "p": np.exp(full[..., 2]),
+            "qfls": qfls, "anion": anion, "cation": cation, "implied_voc": implied,
+            "plqy": plqy, "J": J + shunt * V, "converged": ok, "iterations": its}
+
+
+def observed(dev, full, g_frac):
+    """What photoluminescence measures of states ``full`` (nV x NP + 2 x 3): the quasi-Fermi
+    level splitting on every point (batch.py:1183-1189), the implied V_oc and the PLQY of
+    direct recombination (`_observed`, batch.py:1021-1029: control volumes
+    solver.py:1952-1954, the photons against their equilibrium rate and the generation)."""
+    n, p = np.exp(full[..., 1]), np.exp(full[..., 2])
+    ni2 = dev.ni ** 2
+    qfls = dev.Vt * np.log(n * p / ni2)
+    vol = 0.5 * dev.L * (dev.h[:-2] + dev.h[1:-1])
+    k = dev.k_direct[1:-1]
+    rad = np.sum(vol * k * n[:, 1:-1] * p[:, 1:-1], axis=1)
+    rad0 = np.sum(vol * k * ni2[1:-1])
+    gen = g_frac * np.sum(vol * dev.G[1:-1])
+    nV = full.shape[0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        implied = dev.Vt * np.log(rad / rad0) if rad0 > 0 else np.full(nV, np.nan)
+        plqy = (rad - rad0) / gen if gen > 0 else np.full(nV, np.nan)
+    return qfls, implied, plqy
+
+
+def _ions_on_grid(dev, Vfull):
+    """Anions and cations on every point at potentials ``Vfull`` (nV x NP + 2): fixed plus
+    Boltzmann where mobile (solver.py:951-961) with each region's normalisation in closed
+    form (`_ion_logs` :1695-1712, trapezoidal number :1241-1252), the contacts' points their
+    neighbours' (`_ions_on_grid` :2096-2102)."""
+    hL = dev.h[:-1] * dev.L
+    out = []
+    for sign, mask, total, fixed in ((1.0, dev.anion_mask, dev.anion_total, dev.anion_fixed),
+                                     (-1.0, dev.cation_mask, dev.cation_total, dev.cation_fixed)):
+        m = np.asarray(mask, dtype=float)
+        prof = m * np.exp(sign * Vfull / dev.Vt)
+        number = np.sum(0.5 * (prof[:, 1:] + prof[:, :-1]) * (m[1:] * m[:-1]) * hL, axis=1)
+        logs = np.log(total / number) if total > 0 else np.zeros(len(Vfull))
+        dens = fixed + m * np.exp(logs[:, None] + sign * Vfull / dev.Vt)
+        dens[:, 0], dens[:, -1] = dens[:, 1], dens[:, -2]
+        out.append(dens)
+    return out[0], out[1]
+
+
 __all__ = ["assemble", "assemble_ions", "bernoulli", "block_thomas", "build", "current", "j_scale", "lib",
-           "solve_batch", "solve_jv", "solve_one"]
+           "observed", "solve_batch", "solve_jv", "solve_one", "solve_states"]
 

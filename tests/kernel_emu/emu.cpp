@@ -354,3 +354,21 @@ extern "C" int emu_assemble_ions(int n, int NP, int nseg, const double *pt, cons
     for (int d = 0; d < n; ++d) assemble_lane<true>(a, d);
     return 0;
 }
+
+// How many (workgroup, k) pairs of a launch of nblocks workgroups the XCD map gives each of n
+// devices (hits[n]); returns the number of pairs that fell outside [0, n).
+extern "C" int emu_xcd_hits(int n, int nblocks, int32_t *hits) {
+    int outside = 0;
+    for (int d = 0; d < n; ++d) hits[d] = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const XcdMap m = xcd_map(b, nblocks, n);
+        for (int k = 0;; ++k) {
+            const int d = xcd_device(m, k, n);
+            if (d < 0) break;
+            if (d >= n) { ++outside; break; }
+            ++hits[d];
+            if (k > n) { ++outside; break; }  // a map that never ends
+        }
+    }
+    return outside;
+}

@@ -41,6 +41,7 @@ def lib():
                                        ctypes.c_double, i, vp, vp, vp, vp]
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
+        _lib.emu_xcd_hits.argtypes = [i, i, vp]
     return _lib
 
 
@@ -162,3 +163,10 @@ def assemble_ions_packed(packed, y, Vint, logs, bernoulli="exact"):
     return (F.transpose(2, 0, 1), A.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2),
             B.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2), C.reshape(NP, 5, 5, n).transpose(3, 0, 1, 2),
             U.reshape(NP, 5, 2, n).transpose(3, 0, 1, 2))
+
+
+def xcd_hits(n: int, nblocks: int):
+    """(hits per device, pairs outside [0, n)) of the kernels' workgroup-to-device XCD map."""
+    hits = np.zeros(n, dtype=np.int32)
+    outside = lib().emu_xcd_hits(n, nblocks, _p(hits))
+    return hits, outside

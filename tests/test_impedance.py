@@ -286,8 +286,8 @@ def test_the_c_abi_rejects_bad_arguments():
 @pytest.fixture(scope="module")
 def gpu_batches(torch_cuda):
     """(devices, batch, Bernoulli): synthetic devices lit and dark, the SimSS fixture devices
-    without ions with SimSS's Bernoulli, and devices of 20 points (fewer than a wavefront's
-    lanes)."""
+    without ions with SimSS's Bernoulli, devices of 20 points (fewer than a wavefront's
+    lanes) and a batch of 37 devices (the tables' XCD map giving each XCD a run of five)."""
     import dataclasses
 
     from jaxdd_amd.batch import prepare
@@ -296,8 +296,11 @@ def gpu_batches(torch_cuda):
     dark = [dataclasses.replace(d, G_frac=0.0) for d in syn]
     fix = [fixture_device(c) for c in NO_ION_FIXTURES]
     small = _synthetic(3, 20)
+    # more devices than XCDs: each XCD's workgroups take a run of several devices
+    many = _synthetic(37, 60)
     return [(syn, prepare(syn), "exact"), (dark, prepare(dark), "exact"),
-            (fix, prepare(fix), "simss"), (small, prepare(small), "exact")]
+            (fix, prepare(fix), "simss"), (small, prepare(small), "exact"),
+            (many, prepare(many), "exact")]
 
 
 @pytest.mark.gpu

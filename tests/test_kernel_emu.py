@@ -315,3 +315,13 @@ def test_replay_kept_states_are_the_one_lane_states(emu, lanes):
     assert np.max(np.abs(got - ref)) < 1e-9
     for k, dev in enumerate(devs):
         assert np.max(np.abs(J[k] - J1[k])) < 1e-9 * curve_scale(dev, J1[k])
+
+
+@pytest.mark.parametrize("n", [1, 3, 7, 8, 9, 37, 64, 1023, 1024, 1025, 4096, 5001])
+def test_xcd_map_gives_every_device_to_one_workgroup(emu, n):
+    """The impedance tables' launch (as many workgroups as devices, rounded up to whole XCD
+    rounds) and the adjoint's (at most 1024 workgroups, each looping over its devices): every
+    device to exactly one (workgroup, turn), none outside the batch."""
+    for nblocks in sorted({((n + 7) // 8) * 8, min(n, 1024), 1, 5, 8, 13, 1024}):
+        hits, outside = emu.xcd_hits(n, nblocks)
+        assert outside == 0 and np.all(hits == 1), (n, nblocks)
