@@ -13,6 +13,7 @@ from .simss_input import (
     Setup,
     check_supported,
     read_setup,
+    read_tvg,
     setup_from_texts,
     simss_voltages,
 )
@@ -21,6 +22,7 @@ from .simss_input import (
 # modules (the reader, the device builder, the synthetic workloads) import without it.
 _LAZY = {
     "external_voltages": "batch", "prepare": "batch", "solve_batch": "batch",
+    "solve_states": "batch",
     "JVResult": "solver", "bernoulli": "solver", "bernoulli_simss": "solver", "solve_jv": "solver",
     "block_sweep": "sweep",
     "ImpedanceResult": "impedance", "solve_impedance": "impedance",
@@ -55,9 +57,11 @@ __all__ = [
     "make_grid",
     "prepare",
     "read_setup",
+    "read_tvg",
     "setup_from_texts",
     "simss_voltages",
     "solve_batch",
     "solve_impedance",
     "solve_jv",
+    "solve_states",
 ]

@@ -23,7 +23,8 @@ import hashlib
 
 import numpy as np
 
-from .device import Structure, _graded_layout, device_arrays
+from .device import Structure, device_arrays
+from .grid import graded_layout
 
 SEED = 20261107
 VOLTAGES = np.linspace(-0.2, 1.3, 61)
@@ -52,11 +53,10 @@ ION_MOBILITY = 1e-12
 
 
 def _structure(L, NP, ions=False, pre_cond=False, fix_ions=False) -> Structure:
-    i0, i1, lid, raw, norm = _graded_layout([repr(float(x)) for x in L], NP)
     traps = np.array([False, True, False])
     mobile = np.array([False, ions, False])
-    return Structure(
-        NP=NP, i0=tuple(i0), i1=tuple(i1), lid=lid, raw=raw, norm=norm,
+    return Structure.on_layout(
+        graded_layout([repr(float(x)) for x in L], NP),
         layer_gen=np.array([False, True, False]), traps=traps,
         anion_mobile=mobile, anion_fixed=np.zeros(3, bool),
         cation_mobile=mobile, cation_fixed=np.zeros(3, bool),

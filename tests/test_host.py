@@ -43,13 +43,14 @@ def test_grid_split_on_a_whole_share(stack, simss, exact):
 
 
 def test_graded_grid_gives_every_layer_its_ends():
-    from jaxdd_amd.device import GRADED_SMALLEST, _graded_layout, _spacing
+    from jaxdd_amd.grid import GRADED_SMALLEST, graded_layout
 
     stack = ["1e-09", "5e-07", "3e-08"]
     L = np.array([float(s) for s in stack])
     for NP in (400, 1600):
-        i0, i1, lid, raw, norm = _graded_layout(stack, NP)
-        x, h = _spacing(L, L.sum(), i0, lid, raw, norm)
+        lay = graded_layout(stack, NP)
+        i0, i1, raw = lay.i0, lay.i1, lay.unit
+        x, h = lay.place(L, L.sum())
         assert len(x) == NP + 2 and np.all(np.diff(x) > 0)
         assert i0[0] == 0 and i1[-1] == NP + 1
         assert all(b + 1 == a for a, b in zip(i0[1:], i1[:-1]))

@@ -312,7 +312,9 @@ def test_replay_kept_states_are_the_one_lane_states(emu, lanes):
                                    lanes=lanes)
     assert ok1.all() and ok.all()
     assert not np.isnan(got).any()
-    assert np.max(np.abs(got - ref)) < 1e-9
+    # the two eliminations round differently: the states agree to the round-off floor of the
+    # log-densities (~1e-11 of |ln p| ~ 50 deep in a doped layer)
+    assert np.all(np.abs(got - ref) <= 1e-9 + 5e-11 * np.abs(ref))
     for k, dev in enumerate(devs):
         assert np.max(np.abs(J[k] - J1[k])) < 1e-9 * curve_scale(dev, J1[k])
 
