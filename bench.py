@@ -15,8 +15,13 @@ scale). One JSON line on stdout:
 B_step = NP*(16b + 8s + 2w(b^2 + bk)), b = 3, s = 22, k = 3 with the ion border, w the
 factor's bytes) over its duration from HIP events on its launch stream, against 8 TB/s.
 ``cpu_baseline``: the FP64 CPU oracle (a C restatement of the reference's algorithm) timed
-on this host, one pinned single-threaded process per core, each on its own devices of the
-same workload, as the reference times its CPU path.
+on this host, one pinned single-threaded process per core of the GPU's CPU share, each on its
+own devices of the same workload, as the reference times its CPU path; ``vs_oracle`` holds
+the GPU curves of those devices against the oracle's (mixed, and FP64 under ``fp64``).
+``lockstep``: Newton steps needed against steps paid (each wavefront runs until its slowest
+device is done). ``frozen``: the same workload with the ions frozen after a 1.2 V pre-bias,
+timed and checked against the oracle. ``gradient`` (dJ/dtheta of 4096 devices without ions)
+and ``impedance`` (2048 devices, NP = 300, 50 frequencies) are legs of their own.
 """
 
 from __future__ import annotations
@@ -68,6 +73,9 @@ def parse():
     p.add_argument("--workers", type=int, default=0, help="host processes building devices")
     p.add_argument("--grad-n", type=int, default=4096,
                    help="devices of the gradient workload (dJ/dtheta, 16 parameters); 0: skip")
+    p.add_argument("--frozen-n", type=int, default=16384,
+                   help="devices of the c3 frozen-after-pre-bias leg (timed beside the free "
+                        "one); 0: skip")
     p.add_argument("--imp-n", type=int, default=2048,
                    help="devices of the impedance workload (NP=300, no mobile ions); 0: skip")
     p.add_argument("--imp-freqs", type=int, default=50,
@@ -77,12 +85,20 @@ def parse():
     return p.parse_args()
 
 
-def host_cores() -> int:
+#: The host cores one GPU's share of the box gives a job (the CPU leg's processes and the
+#: host builders' pools stay within it whatever the affinity mask shows).
+CPU_SHARE = 16
+
+
+def affinity_cores() -> int:
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        return os.cpu_count() or 1
+
+
+def host_cores() -> int:
+    return max(1, min(CPU_SHARE, affinity_cores()))
 
 
 # ---- CPU baseline: the oracle, one pinned single-threaded process per core -----------------
@@ -105,9 +121,9 @@ def _cpu_worker(core, rows, config, variant, start_evt, ready, out):
     ready.put(core)
     start_evt.wait()
     t0 = time.perf_counter()
-    J, ok, _ = O.solve_batch(devs, S.VOLTAGES)
+    J, ok, its = O.solve_batch(devs, S.VOLTAGES)
     t1 = time.perf_counter()
-    out.put((t0, t1, len(devs), float(ok.mean())))
+    out.put((t0, t1, len(devs), float(ok.mean()), rows, J, ok, its))
 
 
 def cpu_baseline(config, variant, per_proc):
@@ -135,12 +151,52 @@ def cpu_baseline(config, variant, per_proc):
     t0 = min(r[0] for r in res)
     t1 = max(r[1] for r in res)
     curves = sum(r[2] for r in res)
+    res.sort(key=lambda r: r[4][0])
+    oracle_curves = tuple(np.concatenate([r[k] for r in res]) for k in (5, 6, 7))
     return {"value": curves / (t1 - t0), "unit": "J-V curves/s", "cores": len(procs),
-            "kind": "port",
-            "sample": f"{curves} curves ({per_proc} devices per process) of the same workload, "
-                      f"FP64 C oracle with the reference's two passes, {len(procs)} pinned "
-                      "single-threaded processes",
-            "per_core": curves / (t1 - t0) / len(procs), "converged": float(np.mean([r[3] for r in res]))}
+            "affinity_cores": affinity_cores(), "kind": "port",
+            "sample": f"{curves} curves ({per_proc} devices per process, devices 0-{curves - 1}) "
+                      f"of the same workload, FP64 C oracle with the reference's two passes, "
+                      f"{len(procs)} pinned single-threaded processes (the box's CPU share for "
+                      "one GPU)",
+            "per_core": curves / (t1 - t0) / len(procs),
+            "converged": float(np.mean([r[3] for r in res]))}, oracle_curves
+
+
+def versus_oracle(J, ok, its, oracle_curves, J_scale, wild=None):
+    """The GPU's curves of the devices the CPU leg solved against the oracle's: the largest
+    |J - J_oracle| over each curve's scale (J_scale + max |J_oracle|) where both converged,
+    converged flags equal where neither side took CHAOTIC_STEPS steps or more, and the points
+    whose Newton steps differ."""
+    Jo, oko, itso = oracle_curves
+    m = len(Jo)
+    J, ok, its = J[:m], ok[:m], its[:m]
+    keep = np.ones(m, bool) if wild is None else ~wild
+    excluded = np.flatnonzero(~keep).tolist()
+    J, ok, its, Jo, oko, itso = (a[keep] for a in (J, ok, its, Jo, oko, itso))
+    scale = J_scale[:m][keep, None] + np.max(np.abs(Jo), axis=1, keepdims=True)
+    both = ok & oko
+    calm = (its < CHAOTIC_STEPS) & (itso < CHAOTIC_STEPS)
+    return {"devices": m, "excluded_chaotic_pre_bias": excluded,
+            "max_err": float(np.max((np.abs(J - Jo) / scale)[both], initial=0.0)),
+            "flags_equal_calm": bool(np.array_equal(ok[calm], oko[calm])),
+            "chaotic_points": int((~calm).sum()),
+            "steps_differ": int(np.sum(its[calm] != itso[calm]))}
+
+
+def lockstep(its, lanes: int):
+    """Newton steps needed (the sum of every device's iterations) against steps paid: a
+    wavefront (64 / lanes devices) runs until its slowest device has finished its scan, so
+    it pays its devices' count times their largest total (the reference's lockstep,
+    benchmarks/run.py:108-113, whose vmap pays the batch's slowest device at every voltage)."""
+    per = its.sum(axis=1)
+    w = 64 // lanes
+    paid = sum(len(per[i:i + w]) * int(per[i:i + w].max()) for i in range(0, len(per), w))
+    needed = int(per.sum())
+    nV = its.shape[1]
+    return {"steps_needed_per_voltage": needed / its.size,
+            "steps_paid_per_voltage": paid / its.size, "lockstep": paid / needed,
+            "voltages": nV}
 
 
 def copy_calibration(mib: int) -> None:
@@ -284,9 +340,28 @@ def main() -> int:
     imp_packed = None
     if args.imp_n:
         imp_packed = S.packed("c2", args.imp_n, workers=workers, NP=300)
-    cpu = None
+    frozen_packed = None
+    if args.frozen_n and args.config == "c3" and args.variant == "free":
+        frozen_packed = S.packed("c3", args.frozen_n, variant="frozen", workers=workers)
+    cpu, oracle_curves = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.config, args.variant, args.cpu_curves)
+        cpu, oracle_curves = cpu_baseline(args.config, args.variant, args.cpu_curves)
+    frozen_oracle = frozen_wild = None
+    if frozen_packed is not None:
+        from oracle import ddoracle as O
+
+        import dataclasses
+
+        fdevs = S.devices("c3", 32, variant="frozen")
+        frozen_oracle = O.solve_batch(fdevs, S.VOLTAGES)
+        # devices whose pre-bias lands on one of several nearby profiles by the last bits of
+        # the arithmetic: the oracle's own curve moves by more than 1e-7 of its scale when the
+        # generation rate changes by 1e-13
+        moved = O.solve_batch([dataclasses.replace(d, G=d.G * (1 + 1e-13)) for d in fdevs],
+                              S.VOLTAGES)[0]
+        fscale = (np.array([O.j_scale(d) for d in fdevs])[:, None]
+                  + np.max(np.abs(frozen_oracle[0]), axis=1, keepdims=True))
+        frozen_wild = np.max(np.abs(moved - frozen_oracle[0]) / fscale, axis=1) > 1e-7
 
     import torch
 
@@ -347,8 +422,12 @@ def main() -> int:
         "cpu_baseline": cpu,
         "converged_frac": float(ok.mean()),
         "newton_steps_per_voltage": float(its.mean()),
+        "lockstep": lockstep(its, args.lanes),
         "host_build_s": build_s,
     }
+    J_scale = batch["J_scale"].cpu().numpy()
+    if oracle_curves is not None:
+        out["vs_oracle"] = versus_oracle(J, ok, its, oracle_curves, J_scale)
     if not args.no_fp64 and args.precision == "mixed":
         e64 = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         torch.cuda.synchronize()
@@ -369,10 +448,43 @@ def main() -> int:
         out["max_err_vs_fp64"] = float(np.max((np.abs(J - J64) / scale)[both]))
         out["flags_equal_calm"] = bool(np.array_equal(ok[calm], ok64[calm]))
         out["chaotic_points"] = int((~calm).sum())
+        if oracle_curves is not None:
+            out["fp64"]["vs_oracle"] = versus_oracle(J64, ok64, its64, oracle_curves, J_scale)
+    if frozen_packed is not None:
+        fb = from_packed(frozen_packed, "cuda")
+        solve_batch(fb, V, precision=args.precision, lanes=args.lanes)  # warm-up
+        ef = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        Jf, okf, itsf = solve_batch(fb, V, precision=args.precision, events=ef, lanes=args.lanes)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        Jf, okf, itsf = Jf.cpu().numpy(), okf.cpu().numpy(), itsf.cpu().numpy()
+        fkernel = ef[0].elapsed_time(ef[1])
+        fbytes = float(itsf.sum()) * step_bytes
+        out["frozen"] = {
+            "value": args.frozen_n / (b - a), "unit": "J-V curves/s", "ms_per_step": (b - a) * 1e3,
+            "kernel_ms": fkernel, "workload": (f"c3 frozen after a 1.2 V pre-bias: "
+                                               f"{args.frozen_n} devices, NP=400, 61 voltages, "
+                                               f"{args.precision} precision"),
+            "converged_frac": float(okf.mean()), "newton_steps_per_voltage": float(itsf.mean()),
+            "lockstep": lockstep(itsf, args.lanes),
+            "roofline": {"bound": "hbm", "achieved": fbytes / (fkernel * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": fbytes / (fkernel * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "kernel": kernel, "B_step": step_bytes},
+            "vs_oracle": versus_oracle(Jf, okf, itsf, frozen_oracle,
+                                       fb["J_scale"].cpu().numpy(), frozen_wild),
+        }
+        del fb
     if grad_inputs is not None:
         out["gradient"] = gradient_bench(grad_inputs, V, lanes=args.grad_lanes,
                                          precision=args.grad_precision)
-        out["gradient"]["host_tangents_s"] = grad_host_s
+        g = out["gradient"]
+        g["host_tangents_s"] = grad_host_s
+        g["unit"] += ", the parameter tangents resident in HBM"
+        # the whole gradient, the host's central-difference tangents of the builder included
+        g["end_to_end_value"] = args.grad_n / (grad_host_s + g["ms"] * 1e-3)
         out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
                                        "voltages, FP64, 16 parameters")
     if imp_packed is not None:

@@ -916,8 +916,17 @@ int ddo_solve_one(const ddo_device *d, const double *Vs, int nV, const ddo_opts 
 
 /* ---- solve_jv (solver.py:2041-2245) --------------------------------------------------- */
 /* _noise_step (solver.py:1728-1751): the step the residual's round-off alone gives. */
+static double noise_step_eps(const problem *P, scratch *s, const double *y, const double *logs,
+                             double Vint, double g_frac, const ionscale *sc, double eps);
+
 static double noise_step(const problem *P, scratch *s, const double *y, const double *logs,
                          double Vint, double g_frac, const ionscale *sc) {
+    return noise_step_eps(P, s, y, logs, Vint, g_frac, sc, DBL_EPSILON);
+}
+
+/* The same with the round-off unit eps in place of the double's (the step is linear in it). */
+static double noise_step_eps(const problem *P, scratch *s, const double *y, const double *logs,
+                             double Vint, double g_frac, const ionscale *sc, double eps) {
     int NP = P->NP;
     double bnd[6], Vt = P->dev->Vt;
     bounds_of(P->dev, Vint, bnd);
@@ -931,7 +940,7 @@ static double noise_step(const problem *P, scratch *s, const double *y, const do
                 acc += fabs(s->B[9 * r + 3 * row + c]) * fabs(s->full[3 * (r + 1) + c]);
                 acc += fabs(s->C[9 * r + 3 * row + c]) * fabs(s->full[3 * (r + 2) + c]);
             }
-            s->rhs[3 * r + row] = DBL_EPSILON * acc;
+            s->rhs[3 * r + row] = eps * acc;
         }
     block_thomas(NP, 3, 1, s->A, s->B, s->C, s->rhs, s->X, s->cp, s->dp);
     double m = 0.0;
@@ -943,6 +952,19 @@ static double noise_step(const problem *P, scratch *s, const double *y, const do
         }
     }
     return m;
+}
+
+double ddo_noise_step(const ddo_device *d, const double *y, const double *logs, double Vint,
+                      double g_frac, int bernoulli, double eps) {
+    problem P;
+    problem_init(&P, d, bernoulli);
+    scratch s;
+    scratch_init(&s, d->NP);
+    ionscale sc = {1.0, d->a_total, d->c_total};
+    double out = noise_step_eps(&P, &s, y, logs, Vint, g_frac, &sc, eps);
+    scratch_free(&s);
+    problem_free(&P);
+    return out;
 }
 
 /* settle (solver.py:2132-2175, no contacts with finite surface recombination). */

@@ -79,6 +79,12 @@ int ddo_assemble_ions(const ddo_device *dev, const double *full5, double g_frac,
 int ddo_block_thomas(int NP, int b, int k, const double *A, const double *B, const double *C,
                      const double *d, double *x);
 
+/* _noise_step (solver.py:1728-1751) at the interior state y (NP x 3) with the ions' point
+ * normalisations logs[2] and internal voltage Vint, the round-off unit eps (DBL_EPSILON in
+ * solve_jv's stop). */
+double ddo_noise_step(const ddo_device *dev, const double *y, const double *logs, double Vint,
+                      double g_frac, int bernoulli, double eps);
+
 /* The integral current of a full state (batch._current, no shunt). */
 double ddo_current(const ddo_device *dev, const double *full, double g_frac);
 

@@ -60,6 +60,9 @@ def _load():
     lib.ddo_bernoulli.argtypes = [ctypes.c_double, ctypes.c_int, _P]
     lib.ddo_current.restype = ctypes.c_double
     lib.ddo_current.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_double]
+    lib.ddo_noise_step.restype = ctypes.c_double
+    lib.ddo_noise_step.argtypes = [ctypes.POINTER(_Device), _P, _P, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_double]
     lib.ddo_solve_one.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_int,
                                   ctypes.POINTER(_Opts), _P, ctypes.POINTER(ctypes.c_uint8),
                                   ctypes.POINTER(ctypes.c_int32)]
@@ -265,6 +268,17 @@ def block_thomas(A, B, C, d):
     return x
 
 
+def noise_step(dev, y, Vint, logs=(0.0, 0.0), g_frac=None, bernoulli_name="exact",
+               eps=np.finfo(float).eps):
+    """`_noise_step` (solver.py:1728-1751) at the interior state ``y`` (NP x 3): the Newton
+    step the residual's round-off alone gives, with round-off unit ``eps``."""
+    pin = _Pinned(dev, g_frac)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lg = np.ascontiguousarray(logs, dtype=np.float64)
+    return lib().ddo_noise_step(ctypes.byref(pin.struct), _ptr(y), _ptr(lg), float(Vint),
+                                pin.struct.g_frac, BERNOULLI[bernoulli_name], float(eps))
+
+
 def current(dev, full, g_frac=None):
     pin = _Pinned(dev, g_frac)
     full = np.ascontiguousarray(full, dtype=np.float64)
@@ -350,5 +364,5 @@ def _ions_on_grid(dev, Vfull):
 
 
 __all__ = ["assemble", "assemble_ions", "bernoulli", "block_thomas", "build", "current", "j_scale", "lib",
-           "observed", "solve_batch", "solve_jv", "solve_one", "solve_states"]
+           "noise_step", "observed", "solve_batch", "solve_jv", "solve_one", "solve_states"]
 

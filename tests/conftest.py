@@ -102,3 +102,22 @@ def chaotic_devices(oracle, devs, V, J_ref, rel=1e-13, tol=1e-7):
     J2, _, _ = oracle.solve_batch(moved, V, first_max_iter=None)
     return np.array([np.max(np.abs(J2[k] - J_ref[k])) > tol * curve_scale(d, J_ref[k])
                      for k, d in enumerate(devs)])
+
+
+#: Newton solves the oracle finishes in at most this many steps follow one path: their step
+#: counts are compared exactly (longer solves wander, and last-bit differences of the
+#: arithmetic change their count by a step or two).
+SHORT_STEPS = 8
+
+
+def short_points(devs, V, its, its_ref, wild):
+    """(devices x voltages) mask of the points whose Newton step counts are compared exactly:
+    the oracle took at most SHORT_STEPS, neither side CHAOTIC_STEPS, the device is not among
+    the ``wild`` ones, and it is not the scan's first voltage, whose solve starts from the
+    converged start state so that every step is round-off and its count a matter of the last
+    bits (the lowest voltage, or the pre-bias's with a pre-bias)."""
+    V = np.asarray(V)
+    first = [int(np.searchsorted(V, d.V_pre - 1e-9)) if d.pre_cond else 0 for d in devs]
+    mask = (its_ref <= SHORT_STEPS) & (its < CHAOTIC_STEPS) & ~np.asarray(wild)[:, None]
+    mask[np.arange(len(devs)), np.minimum(first, len(V) - 1)] = False
+    return mask

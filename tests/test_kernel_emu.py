@@ -327,3 +327,30 @@ def test_xcd_map_gives_every_device_to_one_workgroup(emu, n):
     for nblocks in sorted({((n + 7) // 8) * 8, min(n, 1024), 1, 5, 8, 13, 1024}):
         hits, outside = emu.xcd_hits(n, nblocks)
         assert outside == 0 and np.all(hits == 1), (n, nblocks)
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+@pytest.mark.parametrize("config,variant", [("c2", "free"), ("c3", "free"), ("c3", "frozen")])
+def test_replay_takes_the_oracles_newton_steps(emu, oracle, config, variant, lanes):
+    """The kernels' per-lane code in FP64 on 16 devices of each synthetic workload: the
+    oracle's Newton step count on every short solve (conftest.short_points), its flags, and
+    its curves to 1e-9 of each curve's scale."""
+    from conftest import short_points
+
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    devs = S.devices(config, 16, variant=variant)
+    V = S.VOLTAGES[::2].copy()
+    Jo, oko, itso = oracle.solve_batch(devs, V, first_max_iter=None)
+    J, ok, its, _ = emu.solve_packed(pack(devs), V, lanes=lanes)
+    wild = chaotic_devices(oracle, devs, V, Jo)
+    short = short_points(devs, V, its, itso, wild)
+    assert short.sum() > 0.8 * its.size * (1 - wild.mean())
+    assert np.array_equal(its[short], itso[short])
+    agree, _ = flags_agree(ok[~wild], its[~wild], oko[~wild], itso[~wild])
+    assert agree
+    for k, dev in enumerate(devs):
+        if not wild[k]:
+            both = ok[k] & oko[k]
+            assert np.max(np.abs(J[k] - Jo[k])[both], initial=0.0) < 1e-9 * curve_scale(dev, Jo[k])

@@ -297,3 +297,77 @@ def test_ion_system_jacobian_is_the_residuals_derivative(oracle, bern):
         dF = (oracle.assemble_ions(dev, full5, lp, bernoulli_name=bern)[0]
               - oracle.assemble_ions(dev, full5, lm, bernoulli_name=bern)[0]) / (2 * h)
         assert np.max(np.abs(U5[:, :, k] - dF)) < 1e-8
+
+
+# ---- the noise floor of solve_jv's stop (solver.py:1728-1751) -------------------------------
+def _fd_blocks(oracle, dev, full, bern, rel=1e-6):
+    """A, B, C (NP x 3 x 3) of the residual by central differences of the oracle's F in every
+    value of the full state (the contacts' included)."""
+    NP = len(dev.x) - 2
+    A, B, C = (np.zeros((NP, 3, 3)) for _ in range(3))
+    for j in range(NP + 2):
+        for c in range(3):
+            h = rel * max(1.0, abs(full[j, c]))
+            up, dn = full.copy(), full.copy()
+            up[j, c] += h
+            dn[j, c] -= h
+            dF = (oracle.assemble(dev, up, bernoulli_name=bern)[0]
+                  - oracle.assemble(dev, dn, bernoulli_name=bern)[0]) / (2 * h)
+            for r in range(NP):
+                off = j - (r + 1)  # point j relative to row r's own point r + 1
+                if off == -1:
+                    A[r, :, c] = dF[r]
+                elif off == 0:
+                    B[r, :, c] = dF[r]
+                elif off == 1:
+                    C[r, :, c] = dF[r]
+    return A, B, C
+
+
+@pytest.mark.parametrize("bern", ["exact", "simss"])
+def test_noise_step_is_the_round_off_step_of_the_dense_system(oracle, bern):
+    """The oracle's `_noise_step`, rebuilt independently: the round-off right-hand side
+    eps * sum |dF/dy| |y| from a central-difference Jacobian of the residual, the system solved
+    densely (numpy), the step weighted as the Newton step is (V over V_T; the densities'
+    logarithms by n / (n + n_floor)). Also exactly linear in eps."""
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(8)
+    dev = S.device_from_row(t, 5, 20, False, False)
+    NP = 20
+    V = 0.8
+    _, ok, _, st = oracle.solve_one(dev, [V], bernoulli=bern, return_states=True)
+    assert ok.all()
+    rng = np.random.default_rng(2)
+    for y in (st[0], st[0] + 1e-3 * rng.standard_normal(st[0].shape)):
+        full = np.zeros((NP + 2, 3))
+        full[1:-1] = y
+        full[0] = [dev.V0 + 0.5 * dev.left_elec * V, np.log(dev.n_left), np.log(dev.p_left)]
+        full[-1] = [dev.VL - 0.5 * dev.left_elec * V, np.log(dev.n_right), np.log(dev.p_right)]
+        Af, Bf, Cf = _fd_blocks(oracle, dev, full, bern)
+        _, A, B, C, _ = oracle.assemble(dev, full, bernoulli_name=bern)
+        scale = np.max(np.abs(np.concatenate([A, B, C], axis=2)), axis=2)[:, :, None]
+        for fd, an in ((Af, A), (Bf, B), (Cf, C)):
+            assert np.all(np.abs(fd - an) <= 1e-6 * scale)
+        a = np.abs(full)
+        eps = np.finfo(float).eps
+        rhs = eps * (np.einsum("irc,ic->ir", np.abs(Af), a[:-2])
+                     + np.einsum("irc,ic->ir", np.abs(Bf), a[1:-1])
+                     + np.einsum("irc,ic->ir", np.abs(Cf), a[2:]))
+        T = np.zeros((3 * NP, 3 * NP))
+        for i in range(NP):
+            T[3 * i:3 * i + 3, 3 * i:3 * i + 3] = B[i]
+            if i > 0:
+                T[3 * i:3 * i + 3, 3 * i - 3:3 * i] = A[i]
+            if i < NP - 1:
+                T[3 * i:3 * i + 3, 3 * i + 3:3 * i + 6] = C[i]
+        X = np.linalg.solve(T, rhs.ravel()).reshape(NP, 3)
+        dens = np.exp(y[:, 1:])
+        n_floor = 1e-12 * np.max(dev.N_c)
+        want = max(np.max(np.abs(X[:, 0])) / dev.Vt,
+                   np.max(np.abs(X[:, 1:]) * dens / (dens + n_floor)))
+        got = oracle.noise_step(dev, y, V, bernoulli_name=bern)
+        assert got > 0 and abs(got / want - 1) < 1e-6, (got, want)
+        assert oracle.noise_step(dev, y, V, bernoulli_name=bern, eps=2 * eps) == 2 * got
+        assert abs(oracle.noise_step(dev, y, V, bernoulli_name=bern, eps=3 * eps) / (3 * got)
+                   - 1) < 1e-13
