@@ -795,6 +795,56 @@ static void freeze(problem *P, const double *y, const double *logs, ionscale *sc
     sc->a_total = sc->c_total = 0.0;
 }
 
+/* _start (batch.py:875-934): Poisson's start at `first`, then Boltzmann's ions at once, else
+ * through the ion density stages. y (NP x 3) and logs receive the start; returns whether it
+ * is a solution (the solve that follows may then be skipped). */
+static int start_state(problem *P, scratch *s, const ddo_device *d, double first, double g_frac,
+                       const ddo_opts *o, double *y, double *logs) {
+    int NP = d->NP;
+    size_t ny = (size_t)NP * 3;
+    double *yf = alloc_d(ny), *yc = alloc_d(ny), *yk = alloc_d(ny), *Vf = alloc_d(NP + 2);
+    double Jdummy;
+    int itdummy;
+    ionscale full_sc = {1.0, d->a_total, d->c_total};
+    /* Poisson's start, then Boltzmann's ions at once, else through the stages */
+    initial(d, first, y);
+    poisson_solve(P, s, y, first, &full_sc, o->start_iter);
+    full_V(d, first, y, Vf);
+    ion_logs(P, Vf, &full_sc, logs);
+    memcpy(yf, y, ny * sizeof(double));
+    double logs_f[2] = {logs[0], logs[1]};
+    int ok_f = newton_loop(P, s, yf, logs_f, first, g_frac, &full_sc, o->tol, o->start_iter,
+                           o->strict, 0, &Jdummy, &itdummy);
+    int mobile = d->a_total > 0 || d->c_total > 0;
+    int done = ok_f || !mobile;
+    memcpy(yc, y, ny * sizeof(double));
+    double logs_c[2] = {logs[0], logs[1]}, before = 1.0;
+    int ok_c = 1;
+    for (int k = 0; k < 4; ++k) {
+        double share = ION_STAGES[k];
+        ionscale sk = {share, d->a_total * share, d->c_total * share};
+        logs_c[0] += log(share / before);
+        logs_c[1] += log(share / before);
+        memcpy(yk, yc, ny * sizeof(double));
+        double logs_k[2] = {logs_c[0], logs_c[1]};
+        ok_c = newton_loop(P, s, yk, logs_k, first, g_frac, &sk, o->tol, o->start_iter, o->strict,
+                           done, &Jdummy, &itdummy);
+        if (all_finite(yk, (int)ny) && isfinite(logs_k[0]) && isfinite(logs_k[1])) {
+            memcpy(yc, yk, ny * sizeof(double));
+            logs_c[0] = logs_k[0];
+            logs_c[1] = logs_k[1];
+        }
+        before = share;
+    }
+    int staged = !done && ok_c;
+    int kept = all_finite(yf, (int)ny) && isfinite(logs_f[0]) && isfinite(logs_f[1]);
+    if (staged) { memcpy(y, yc, ny * sizeof(double)); logs[0] = logs_c[0]; logs[1] = logs_c[1]; }
+    else if (kept) { memcpy(y, yf, ny * sizeof(double)); logs[0] = logs_f[0]; logs[1] = logs_f[1]; }
+    int solved = ok_f || staged;
+    free(yf); free(yc); free(yk); free(Vf);
+    return solved;
+}
+
 /* ---- _solve_one (batch.py:937-1006) with _start (:875-934) ------------------------------ */
 int ddo_solve_one_states(const ddo_device *d, const double *Vs, int nV, const ddo_opts *o,
                          double *Jout, uint8_t *okout, int32_t *itsout, double *states) {
@@ -812,42 +862,8 @@ int ddo_solve_one_states(const ddo_device *d, const double *Vs, int nV, const dd
     ionscale full_sc = {1.0, d->a_total, d->c_total};
     double first = d->pre_cond ? d->V_pre : Vs[0];
 
-    /* Poisson's start, then Boltzmann's ions at once, else through the stages */
-    initial(d, first, y);
-    poisson_solve(&P, &s, y, first, &full_sc, o->start_iter);
     double logs[2];
-    full_V(d, first, y, Vf);
-    ion_logs(&P, Vf, &full_sc, logs);
-    memcpy(yf, y, ny * sizeof(double));
-    double logs_f[2] = {logs[0], logs[1]};
-    int ok_f = newton_loop(&P, &s, yf, logs_f, first, g_frac, &full_sc, o->tol, o->start_iter,
-                           o->strict, 0, &Jdummy, &itdummy);
-    int mobile = d->a_total > 0 || d->c_total > 0;
-    int done = ok_f || !mobile;
-    memcpy(yc, y, ny * sizeof(double));
-    double logs_c[2] = {logs[0], logs[1]}, before = 1.0;
-    int ok_c = 1;
-    for (int k = 0; k < 4; ++k) {
-        double share = ION_STAGES[k];
-        ionscale sk = {share, d->a_total * share, d->c_total * share};
-        logs_c[0] += log(share / before);
-        logs_c[1] += log(share / before);
-        memcpy(yk, yc, ny * sizeof(double));
-        double logs_k[2] = {logs_c[0], logs_c[1]};
-        ok_c = newton_loop(&P, &s, yk, logs_k, first, g_frac, &sk, o->tol, o->start_iter, o->strict,
-                           done, &Jdummy, &itdummy);
-        if (all_finite(yk, (int)ny) && isfinite(logs_k[0]) && isfinite(logs_k[1])) {
-            memcpy(yc, yk, ny * sizeof(double));
-            logs_c[0] = logs_k[0];
-            logs_c[1] = logs_k[1];
-        }
-        before = share;
-    }
-    int staged = !done && ok_c;
-    int kept = all_finite(yf, (int)ny) && isfinite(logs_f[0]) && isfinite(logs_f[1]);
-    if (staged) { memcpy(y, yc, ny * sizeof(double)); logs[0] = logs_c[0]; logs[1] = logs_c[1]; }
-    else if (kept) { memcpy(y, yf, ny * sizeof(double)); logs[0] = logs_f[0]; logs[1] = logs_f[1]; }
-    int solved = ok_f || staged;
+    int solved = start_state(&P, &s, d, first, g_frac, o, y, logs);
     /* the pre-bias (or first voltage) solve, skipped where the start is the solution */
     int pre_ok = newton_loop(&P, &s, y, logs, first, g_frac, &full_sc, o->tol, o->start_iter,
                              o->strict, solved, &Jdummy, &itdummy);
@@ -1062,4 +1078,681 @@ int ddo_solve_jv(const ddo_device *d, const double *Vs, int nV, double tol, int 
     scratch_free(&s);
     problem_free(&P);
     return rc;
+}
+
+/* ==== transients: src/jaxdd/transient.py:270-464 (`_solve_one`), the subset without an open
+ * circuit, a series resistance, interface traps, free ends, finite surface recombination or
+ * ZimT's automatic stop (Track 0): a steady start, then one implicit-Euler Newton solve per
+ * step (`solver._time_terms` :573-591), mobile ions as unknowns of their own (5 x 5 blocks,
+ * `solver._assemble_ions` :1364-1473, `_ion_constraints_time` :1349-1361), the bulk traps'
+ * occupations carried from step to step (`_trap_step` :594-620), the transient current
+ * (`batch._current_parts` with a time step :316-418, `_ion_currents` :433-483). ========== */
+
+/* Forward-mode numbers with NDUAL partial derivatives: the rows' derivatives follow their
+ * values exactly, as the reference's jvp does. */
+#define NDUAL 7
+typedef struct {
+    double v, d[NDUAL];
+} dual;
+
+static dual dc(double v) { dual r; r.v = v; for (int k = 0; k < NDUAL; ++k) r.d[k] = 0.0; return r; }
+static dual dv(double v, int k) { dual r = dc(v); r.d[k] = 1.0; return r; }
+static dual dadd(dual a, dual b) { for (int k = 0; k < NDUAL; ++k) a.d[k] += b.d[k]; a.v += b.v; return a; }
+static dual dsub(dual a, dual b) { for (int k = 0; k < NDUAL; ++k) a.d[k] -= b.d[k]; a.v -= b.v; return a; }
+static dual dsc(dual a, double c) { for (int k = 0; k < NDUAL; ++k) a.d[k] *= c; a.v *= c; return a; }
+static dual dadds(dual a, double c) { a.v += c; return a; }
+static dual dmul(dual a, dual b) {
+    dual r; r.v = a.v * b.v;
+    for (int k = 0; k < NDUAL; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+static dual ddiv(dual a, dual b) {
+    dual r; r.v = a.v / b.v;
+    for (int k = 0; k < NDUAL; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+static dual dfun(dual a, double f, double df) {
+    dual r; r.v = f;
+    for (int k = 0; k < NDUAL; ++k) r.d[k] = df * a.d[k];
+    return r;
+}
+static dual dexp_(dual a) { double e = exp(a.v); return dfun(a, e, e); }
+static dual dbern(dual x, int kind) { double dB, B = ddo_bernoulli(x.v, kind, &dB); return dfun(x, B, dB); }
+/* -expm1(-x)/x, the trap occupation's mean factor over a step of a x Delta t = x */
+static dual dmean(dual x) {
+    double e = expm1(-x.v), m = -e / x.v;
+    double dm = (exp(-x.v) * x.v + e) / (x.v * x.v); /* d/dx of -expm1(-x)/x */
+    return dfun(x, m, dm);
+}
+
+/* What a time step holds fixed (`_time_terms` :573-591 and the current's `tr`). */
+typedef struct {
+    double dti;
+    double *n_prev, *p_prev;   /* NP + 2: the densities before the step */
+    double *a_prev, *c_prev;   /* NP + 2: the ions before the step (0 outside their regions) */
+    double *V_prev;            /* NP + 2 */
+    double *fb;                /* NP: the bulk traps' occupation before the step */
+    double *width;             /* NP + 1: each point's width in the integral current's sums */
+    int zimt;                  /* ZimT's sums (the left contact's change counted too) */
+} tstep;
+
+/* The carriers' rates over the step (`_bulk_rates` :680-700 with `_trap_step`'s mean and
+ * vacant share): direct recombination, and each carrier's capture by the bulk traps at the
+ * occupation's mean over the step (the steady SRH rate when dti = 0). */
+static void step_rates(double kd, double ni2, double Nt, double Cn, double Cp, double nt0,
+                       double pt0, dual n, dual p, double f_old, double dti, dual *Rn, dual *Rp) {
+    dual direct = dsc(dadds(dmul(n, p), -ni2), kd);
+    *Rn = *Rp = direct;
+    if (!(Nt > 0)) return;
+    dual a = dadd(dsc(dadds(n, nt0), Cn), dsc(dadds(p, pt0), Cp));
+    dual steady = ddiv(dadds(dsc(n, Cn), Cp * pt0), a);
+    dual vac_s = ddiv(dadds(dsc(p, Cp), Cn * nt0), a);
+    dual fbar = steady, vbar = vac_s;
+    if (dti > 0) {
+        dual M = dmean(dsc(a, 1.0 / dti));
+        fbar = dadd(steady, dmul(dsub(dc(f_old), steady), M));
+        vbar = dadd(vac_s, dmul(dsub(dc(1.0 - f_old), vac_s), M));
+    }
+    *Rn = dadd(direct, dsc(dsub(dmul(n, vbar), dsc(fbar, nt0)), Nt * Cn));
+    *Rp = dadd(direct, dsc(dsub(dmul(p, fbar), dsc(vbar, pt0)), Nt * Cp));
+}
+
+/* The traps' occupation at the step's end (`_bulk_occupation` :711-716). */
+static double trap_end(double Nt, double Cn, double Cp, double nt0, double pt0, double n,
+                       double p, double f_old, double dti) {
+    if (!(Nt > 0)) return 0.0;
+    double a = Cn * (n + nt0) + Cp * (p + pt0);
+    double steady = (Cn * n + Cp * pt0) / a;
+    if (!(dti > 0)) return steady;
+    return steady + (f_old - steady) * exp(-a / dti);
+}
+
+/* `_ion_full` (:1315-1346): each contact's ion densities where a region reaches it, from its
+ * neighbour (no flux through the contact's interval); 0 elsewhere. full: (NP + 2) x 5. */
+static void ion_full(const problem *P, double *full, double dti) {
+    int NP = P->NP, nc = 5;
+    double Vt = P->dev->Vt;
+    double x0 = (full[nc * 1] - full[0]) / Vt, xN = (full[nc * (NP + 1)] - full[nc * NP]) / Vt;
+    double r0 = dti > 0 ? log(ddo_bernoulli(x0, P->bern, NULL)) - log(ddo_bernoulli(-x0, P->bern, NULL)) : -x0;
+    double rN = dti > 0 ? log(ddo_bernoulli(xN, P->bern, NULL)) - log(ddo_bernoulli(-xN, P->bern, NULL)) : -xN;
+    double *L0 = full, *R0 = full + nc * (NP + 1);
+    L0[3] = P->amask_full[0] > 0 ? full[nc + 3] + r0 : 0.0;
+    L0[4] = P->cmask_full[0] > 0 ? full[nc + 4] - r0 : 0.0;
+    R0[3] = P->amask_full[NP + 1] > 0 ? full[nc * NP + 3] - rN : 0.0;
+    R0[4] = P->cmask_full[NP + 1] > 0 ? full[nc * NP + 4] + rN : 0.0;
+}
+
+/* One time step's residual F (NP x nc, rows scaled), blocks A, B, C (NP x nc x nc) and the
+ * slots' columns U (NP x nc x 2) at full ((NP + 2) x nc; nc = 5 with the ions' logarithms as
+ * unknowns, its contacts' ions from ion_full), the slots' logarithms logs. The carriers' rows
+ * are `_assemble`'s (:1021-1187) with each carrier's rate over the step and the implicit
+ * Euler term -fac dti (n - n_prev); each ion's row (nc = 5) is `_assemble_ions`' `species`
+ * (:1414-1451): Scharfetter-Gummel fluxes inside its region with the density before the step
+ * scaled by exp(its slot's log), Boltzmann's at dti = 0, its unknown held outside. */
+static void assemble_step(const problem *P, const double *full, int nc, double g_frac,
+                          const tstep *ts, const double *logs, double *F, double *A, double *B,
+                          double *C, double *U) {
+    const ddo_device *d = P->dev;
+    int NP = P->NP, bb = nc * nc;
+    double Vt = d->Vt, dti = ts->dti;
+    dual *Fn = (dual *)calloc((size_t)(NP + 1), sizeof(dual));
+    dual *Fp = (dual *)calloc((size_t)(NP + 1), sizeof(dual));
+    /* per edge e (points e, e + 1): partials 0 V_e, 1 V_e+1, 2 ln n_e, 3 ln n_e+1, 4 ln p_e,
+       5 ln p_e+1 */
+    for (int e = 0; e <= NP; ++e) {
+        dual V0 = dv(full[nc * e], 0), V1 = dv(full[nc * (e + 1)], 1);
+        dual n0 = dexp_(dv(full[nc * e + 1], 2)), n1 = dexp_(dv(full[nc * (e + 1) + 1], 3));
+        dual p0 = dexp_(dv(full[nc * e + 2], 4)), p1 = dexp_(dv(full[nc * (e + 1) + 2], 5));
+        dual xn = dsc(dadds(dsub(V1, V0), d->offset_n[e + 1] - d->offset_n[e]), 1.0 / Vt);
+        dual xp = dsc(dadds(dsub(V1, V0), d->offset_p[e + 1] - d->offset_p[e]), 1.0 / Vt);
+        double cn = Vt * d->mu_n[e], cp = Vt * d->mu_p[e];
+        Fn[e] = dsc(dsub(dmul(dbern(xn, P->bern), n1), dmul(dbern(dsc(xn, -1.0), P->bern), n0)), cn);
+        Fp[e] = dsc(dsub(dmul(dbern(xp, P->bern), p0), dmul(dbern(dsc(xp, -1.0), P->bern), p1)), cp);
+    }
+    for (int r = 0; r < NP; ++r) {
+        int i = r + 1;
+        double *a = A + (size_t)bb * r, *b = B + (size_t)bb * r, *c = C + (size_t)bb * r;
+        double *f = F + (size_t)nc * r, *u = U + (size_t)nc * 2 * r;
+        for (int k = 0; k < bb; ++k) a[k] = b[k] = c[k] = 0.0;
+        for (int k = 0; k < 2 * nc; ++k) u[k] = 0.0;
+        double hm = P->hm[r], hp = P->hp[r], fac = P->fac[r], sP = P->sP[r], sC = P->sC[r];
+        /* the rates at the point: partials 0 ln n, 1 ln p */
+        dual n = dexp_(dv(full[nc * i + 1], 0)), p = dexp_(dv(full[nc * i + 2], 1)), Rn, Rp;
+        step_rates(P->kd[r], P->ni2[r], P->Nt[r], P->Cn[r], P->Cp[r], P->nt0[r], P->pt0[r], n,
+                   p, ts->fb[r], dti, &Rn, &Rp);
+        double gen = P->Gs[r] * g_frac;
+        /* electrons: hm Fn[i] - hp Fn[i-1] + fac (gen - Rn) - fac dti (n - n_prev) */
+        double elec = hm * Fn[i].v - hp * Fn[i - 1].v + fac * (gen - Rn.v) - fac * dti * (n.v - ts->n_prev[i]);
+        double hole = hp * Fp[i - 1].v - hm * Fp[i].v + fac * (gen - Rp.v) - fac * dti * (p.v - ts->p_prev[i]);
+        /* columns: left point (edge i-1's partials 0, 2, 4), own (edge i-1's 1, 3, 5 and edge
+           i's 0, 2, 4), right (edge i's 1, 3, 5) */
+        for (int q = 0; q < 3; ++q) {
+            a[nc * 1 + q] = -hp * Fn[i - 1].d[2 * q] * sC;
+            a[nc * 2 + q] = hp * Fp[i - 1].d[2 * q] * sC;
+            b[nc * 1 + q] = (hm * Fn[i].d[2 * q] - hp * Fn[i - 1].d[2 * q + 1]) * sC;
+            b[nc * 2 + q] = (hp * Fp[i - 1].d[2 * q + 1] - hm * Fp[i].d[2 * q]) * sC;
+            c[nc * 1 + q] = hm * Fn[i].d[2 * q + 1] * sC;
+            c[nc * 2 + q] = -hm * Fp[i].d[2 * q + 1] * sC;
+        }
+        b[nc * 1 + 1] += (-fac * Rn.d[0] - fac * dti * n.v) * sC;
+        b[nc * 1 + 2] += -fac * Rn.d[1] * sC;
+        b[nc * 2 + 1] += -fac * Rp.d[0] * sC;
+        b[nc * 2 + 2] += (-fac * Rp.d[1] - fac * dti * p.v) * sC;
+        /* Poisson's: the mobile ions at their own densities (nc = 5) */
+        double an = nc == 5 ? P->am[r] * exp(full[nc * i + 3]) : 0.0;
+        double ca = nc == 5 ? P->cm[r] * exp(full[nc * i + 4]) : 0.0;
+        double poisson = P->fac_l[r] * full[nc * (i - 1)] + P->fac_m[r] * full[nc * i] +
+                         P->fac_u[r] * full[nc * (i + 1)] -
+                         Q_E * (n.v - p.v + P->dop[r] + P->afix[r] + an - P->cfix[r] - ca);
+        a[0] = P->fac_l[r] * sP;
+        c[0] = P->fac_u[r] * sP;
+        b[0] = P->fac_m[r] * sP;
+        b[1] = -Q_E * n.v * sP;
+        b[2] = Q_E * p.v * sP;
+        f[0] = poisson * sP;
+        f[1] = elec * sC;
+        f[2] = hole * sC;
+        if (nc == 3) continue;
+        b[3] = -Q_E * an * sP;
+        b[4] = Q_E * ca * sP;
+        /* the ions' rows: partials 0 V_l, 1 V_c, 2 V_r, 3 ln_l, 4 ln_c, 5 ln_r, 6 the slot's log */
+        for (int sp = 0; sp < 2; ++sp) {
+            int col = 3 + sp;
+            double sn = sp == 0 ? -1.0 : 1.0;
+            const double *mfull = sp == 0 ? P->amask_full : P->cmask_full;
+            const double *mu = sp == 0 ? d->mu_a : d->mu_c;
+            const double *prevs = sp == 0 ? ts->a_prev : ts->c_prev;
+            double m_i = mfull[i];
+            double el = r == 0 ? 0.0 : mfull[i - 1] * m_i, er = r == NP - 1 ? 0.0 : m_i * mfull[i + 1];
+            double cl = Vt * mu[i - 1], cr = Vt * mu[i];
+            dual Vl = dv(full[nc * (i - 1)], 0), Vc = dv(full[nc * i], 1), Vr = dv(full[nc * (i + 1)], 2);
+            dual ll = dv(full[nc * (i - 1) + col], 3), lc = dv(full[nc * i + col], 4);
+            dual lr = dv(full[nc * (i + 1) + col], 5), ls = dv(logs[sp], 6);
+            dual val;
+            if (!(m_i > 0)) {
+                val = lc;
+            } else if (!(dti > 0)) {
+                val = dadd(dsub(lc, ls), dsc(Vc, sn / Vt));
+            } else {
+                dual xl = dsc(dsub(Vc, Vl), 1.0 / Vt), xr = dsc(dsub(Vr, Vc), 1.0 / Vt);
+                dual nl = dexp_(ll), nm_ = dexp_(lc), nr = dexp_(lr);
+                dual Fl, Fr, flux;
+                if (sn < 0) { /* anions, as electrons */
+                    Fl = dsc(dsub(dmul(dbern(xl, P->bern), nm_), dmul(dbern(dsc(xl, -1.0), P->bern), nl)), cl);
+                    Fr = dsc(dsub(dmul(dbern(xr, P->bern), nr), dmul(dbern(dsc(xr, -1.0), P->bern), nm_)), cr);
+                    flux = dsub(dsc(Fr, er * hm), dsc(Fl, el * hp));
+                } else { /* cations, as holes */
+                    Fl = dsc(dsub(dmul(dbern(xl, P->bern), nl), dmul(dbern(dsc(xl, -1.0), P->bern), nm_)), cl);
+                    Fr = dsc(dsub(dmul(dbern(xr, P->bern), nm_), dmul(dbern(dsc(xr, -1.0), P->bern), nr)), cr);
+                    flux = dsub(dsc(Fl, el * hp), dsc(Fr, er * hm));
+                }
+                double before = prevs[i], prev = before > 0 ? before : 1.0;
+                double scale = 1.0 / (prev * (fac * dti + hm * cr + hp * cl));
+                dual stored = dsc(dsub(dsc(dexp_(ls), before), nm_), fac * dti);
+                val = dsc(dadd(flux, stored), scale);
+            }
+            f[col] = val.v;
+            a[nc * col + 0] = val.d[0];
+            b[nc * col + 0] = val.d[1];
+            c[nc * col + 0] = val.d[2];
+            a[nc * col + col] = val.d[3];
+            b[nc * col + col] = val.d[4];
+            c[nc * col + col] = val.d[5];
+            u[2 * col + sp] = val.d[6] * m_i;
+        }
+    }
+    free(Fn);
+    free(Fp);
+}
+
+/* `_ion_constraints_time` (:1349-1361): each slot's number of ions from their own densities
+ * (trapezoidal over its region, the contacts' from ion_full) over its total, less 1; W (NP x 2)
+ * its derivative in each interior point's own ion logarithm. full: (NP + 2) x 5. */
+static void ion_constraints_time(const problem *P, const double *full, double *g, double *W) {
+    int NP = P->NP;
+    const double *masks[2] = {P->amask_full, P->cmask_full};
+    double totals[2] = {P->a_total, P->c_total};
+    for (int k = 0; k < 2; ++k) {
+        double number = 0.0;
+        for (int e = 0; e <= NP; ++e) {
+            double pair = masks[k][e] * masks[k][e + 1];
+            double d0 = masks[k][e] * exp(full[5 * e + 3 + k]);
+            double d1 = masks[k][e + 1] * exp(full[5 * (e + 1) + 3 + k]);
+            number += 0.5 * (d1 + d0) * pair * P->hL[e];
+        }
+        int has = totals[k] > 0;
+        g[k] = has ? number / totals[k] - 1.0 : 0.0;
+        for (int r = 0; r < NP; ++r) {
+            int i = r + 1;
+            double w = masks[k][i - 1] * masks[k][i] * P->hL[i - 1] + masks[k][i] * masks[k][i + 1] * P->hL[i];
+            W[2 * r + k] = has ? 0.5 * w * masks[k][i] * exp(full[5 * i + 3 + k]) / totals[k] : 0.0;
+        }
+    }
+}
+
+typedef struct {
+    double *full, *F, *A, *B, *C, *U, *rhs, *X, *cp, *dp, *W;
+} tscratch;
+
+static void tscratch_init(tscratch *s, int NP) {
+    size_t P2 = (size_t)(NP + 2);
+    s->full = alloc_d(P2 * 5); s->F = alloc_d((size_t)NP * 5);
+    s->A = alloc_d((size_t)NP * 25); s->B = alloc_d((size_t)NP * 25); s->C = alloc_d((size_t)NP * 25);
+    s->U = alloc_d((size_t)NP * 10); s->rhs = alloc_d((size_t)NP * 15); s->X = alloc_d((size_t)NP * 15);
+    s->cp = alloc_d((size_t)NP * 25); s->dp = alloc_d((size_t)NP * 15); s->W = alloc_d((size_t)NP * 2);
+}
+static void tscratch_free(tscratch *s) {
+    double *f[] = {s->full, s->F, s->A, s->B, s->C, s->U, s->rhs, s->X, s->cp, s->dp, s->W};
+    for (int i = 0; i < 11; ++i) free(f[i]);
+}
+
+/* `_full` (batch.py:262-267) with the contacts at Vint, the ions' contacts by ion_full. */
+static void full_of(const problem *P, const double *y, int nc, double Vint, double dti, double *full) {
+    int NP = P->NP;
+    double bnd[6];
+    bounds_of(P->dev, Vint, bnd);
+    for (int k = 0; k < nc; ++k) {
+        full[k] = k < 3 ? bnd[k] : 0.0;
+        full[nc * (NP + 1) + k] = k < 3 ? bnd[3 + k] : 0.0;
+    }
+    memcpy(full + nc, y, (size_t)NP * nc * sizeof(double));
+    if (nc == 5) ion_full(P, full, dti);
+}
+
+/* The transient current (`batch._current` with `tr`, :302-418): each carrier's integral form
+ * with the densities' change in the integrand, the mobile ions' currents of their regions
+ * (`_ion_currents` :433-483) and the displacement current, averaged over the intervals. */
+static double current_step(const problem *P, const double *full, int nc, double g_frac,
+                           const tstep *ts) {
+    const ddo_device *d = P->dev;
+    int NP = P->NP;
+    double Vt = d->Vt, L = d->L, dti = ts->dti;
+    double *w = alloc_d((size_t)(NP + 2) * 8);
+    double *U = w, *ev = U + (NP + 2), *iu = ev + (NP + 2), *J = iu + (NP + 2);
+    double *n = J + (NP + 2), *p = n + (NP + 2), *Jsum = p + (NP + 2), *dens = Jsum + (NP + 2);
+    double sign = -d->left_elec;
+    for (int i = 0; i < NP + 2; ++i) {
+        n[i] = exp(full[nc * i + 1]);
+        p[i] = exp(full[nc * i + 2]);
+        Jsum[i] = 0.0;
+    }
+    for (int c = 0; c < 2; ++c) {
+        double sn = c == 0 ? -1.0 : 1.0;
+        const double *off = c == 0 ? d->offset_n : d->offset_p;
+        const double *mu = c == 0 ? d->mu_n : d->mu_p;
+        const double *dn = c == 0 ? n : p;
+        const double *before = c == 0 ? ts->n_prev : ts->p_prev;
+        for (int i = 0; i < NP + 2; ++i) {
+            double Ri = 0.0;
+            if (i > 0 && i <= NP) {
+                dual Rn, Rp;
+                step_rates(d->k_direct[i], P->ni2_full[i], d->Nt[i], d->Cn[i], d->Cp[i], d->nt0[i],
+                           d->pt0[i], dc(n[i]), dc(p[i]), ts->fb[i - 1], dti, &Rn, &Rp);
+                Ri = c == 0 ? Rn.v : Rp.v;
+            }
+            U[i] = (i == 0 || i == NP + 1) ? 0.0 : Ri - d->G[i] * g_frac;
+            double change = dti * (dn[i] - before[i]);
+            if (i > 0 || ts->zimt) U[i] += change;
+        }
+        for (int i = 0; i <= NP; ++i) ev[i] = exp(sn * (full[nc * i] + off[i]) / Vt);
+        iu[0] = 0.0;
+        for (int j = 0; j < NP; ++j) iu[j + 1] = iu[j] + 0.5 * (d->x[j + 1] - d->x[j]) * (U[j] + U[j + 1]);
+        double single = 0.0, dbl = 0.0;
+        for (int j = 0; j < NP; ++j) {
+            double dx = d->x[j + 1] - d->x[j];
+            single += 0.5 * dx * (ev[j] / mu[j] + ev[j + 1] / mu[j + 1]);
+            dbl += 0.5 * dx * (ev[j] * iu[j] / mu[j] + ev[j + 1] * iu[j + 1] / mu[j + 1]);
+        }
+        double VgN = full[nc * (NP + 1)] + off[NP + 1], Vg0 = full[0] + off[0];
+        double K = (Vt * (sn * dn[NP + 1] * exp(sn * VgN / Vt) - sn * dn[0] * exp(sn * Vg0 / Vt)) - sn * dbl) / single;
+        J[0] = Q_E * K;
+        double acc = 0.0;
+        for (int e = 1; e <= NP; ++e) {
+            acc += sn * Q_E * L * ts->width[e] * U[e];
+            J[e] = Q_E * K + acc;
+        }
+        for (int e = NP; e >= 1; --e)
+            if (d->iface[e]) J[e] = J[e - 1];
+        for (int e = 0; e <= NP; ++e) Jsum[e] += sign * J[e];
+    }
+    if (nc == 5) { /* the mobile ions, one region per species */
+        const double *masks[2] = {P->amask_full, P->cmask_full};
+        double totals[2] = {P->a_total, P->c_total};
+        for (int k = 0; k < 2; ++k) {
+            if (!(totals[k] > 0)) continue;
+            double sn = k == 0 ? -1.0 : 1.0;
+            const double *m = masks[k], *mu = k == 0 ? d->mu_a : d->mu_c;
+            const double *before = k == 0 ? ts->a_prev : ts->c_prev;
+            int first = -1, last = -1;
+            for (int i = 0; i < NP + 2; ++i)
+                if (m[i] > 0) { if (first < 0) first = i; last = i; }
+            if (first < 0) continue;
+            for (int i = 0; i < NP + 2; ++i) {
+                dens[i] = exp(full[nc * i + 3 + k]);
+                U[i] = dti * (dens[i] - before[i]);
+                ev[i] = exp(sn * full[nc * i] / Vt);
+            }
+            iu[0] = 0.0;
+            double single = 0.0, dbl = 0.0;
+            for (int e = 0; e <= NP; ++e) {
+                double pair = m[e] * m[e + 1], dx = d->x[e + 1] - d->x[e];
+                iu[e + 1] = iu[e] + 0.5 * dx * (U[e] + U[e + 1]) * pair;
+            }
+            for (int e = 0; e <= NP; ++e) {
+                double pair = m[e] * m[e + 1], dx = d->x[e + 1] - d->x[e];
+                double m0 = mu[e] > 0 ? mu[e] : 1.0, m1 = mu[e + 1] > 0 ? mu[e + 1] : 1.0;
+                single += 0.5 * dx * (ev[e] / m0 + ev[e + 1] / m1) * pair;
+                dbl += 0.5 * dx * (ev[e] * iu[e] / m0 + ev[e + 1] * iu[e + 1] / m1) * pair;
+            }
+            if (!(single > 0)) single = 1.0;
+            double ends = sn * (dens[last] * ev[last] - dens[first] * ev[first]);
+            double K = (Vt * ends - sn * dbl) / single;
+            double acc = 0.0;
+            for (int e = 0; e <= NP; ++e) {
+                double pair = m[e] * m[e + 1];
+                if (e > first) acc += sn * Q_E * L * ts->width[e] * U[e] * pair;
+                J[e] = Q_E * K + acc;
+            }
+            for (int e = NP; e >= 1; --e)
+                if (d->iface[e]) J[e] = J[e - 1];
+            for (int e = 0; e <= NP; ++e)
+                if (m[e] * m[e + 1] > 0) Jsum[e] += sign * J[e];
+        }
+    }
+    /* the displacement current on each interval */
+    for (int e = 0; e <= NP; ++e) {
+        double dV = full[nc * (e + 1)] - full[nc * e], dV0 = ts->V_prev[e + 1] - ts->V_prev[e];
+        Jsum[e] += sign * d->eps[e] * (dV - dV0) * dti / (L * d->h[e]);
+    }
+    double num = 0.0, den = 0.0;
+    for (int e = 0; e <= NP; ++e) { num += d->h[e] * Jsum[e]; den += d->h[e]; }
+    free(w);
+    return num / den;
+}
+
+/* One Newton step of a time step in place on (y, logs) (batch.py:759-785 with `_system`'s 5 x 5
+ * branch :518-536 and `_bordered_solve` :595-603); returns the step's size. nc = 3: no border
+ * (a device without mobile ions holds both slots). */
+static double newton_step_time(const problem *P, tscratch *s, double *y, double *logs, int nc,
+                               double Vint, double g_frac, const tstep *ts) {
+    int NP = P->NP;
+    double Vt = P->dev->Vt;
+    full_of(P, y, nc, Vint, ts->dti, s->full);
+    assemble_step(P, s->full, nc, g_frac, ts, logs, s->F, s->A, s->B, s->C, s->U);
+    int kr = nc == 5 ? 3 : 1; /* right-hand sides: -F, and the slots' columns */
+    for (int r = 0; r < NP; ++r)
+        for (int row = 0; row < nc; ++row) {
+            s->rhs[(size_t)kr * (nc * r + row)] = -s->F[nc * r + row];
+            if (nc == 5) {
+                s->rhs[(size_t)kr * (nc * r + row) + 1] = s->U[2 * (nc * r + row)];
+                s->rhs[(size_t)kr * (nc * r + row) + 2] = s->U[2 * (nc * r + row) + 1];
+            }
+        }
+    block_thomas(NP, nc, kr, s->A, s->B, s->C, s->rhs, s->X, s->cp, s->dp);
+    double z[2] = {0.0, 0.0};
+    if (nc == 5) {
+        double g[2];
+        ion_constraints_time(P, s->full, g, s->W);
+        double held[2] = {P->a_total > 0 ? 0.0 : 1.0, P->c_total > 0 ? 0.0 : 1.0};
+        double S[4] = {held[0], 0.0, 0.0, held[1]};
+        z[0] = -g[0] - held[0] * logs[0];
+        z[1] = -g[1] - held[1] * logs[1];
+        for (int r = 0; r < NP; ++r)
+            for (int k = 0; k < 2; ++k) { /* W lives in the ions' own columns */
+                double w = s->W[2 * r + k];
+                const double *x = s->X + (size_t)kr * (nc * r + 3 + k);
+                S[2 * k + 0] -= w * x[1];
+                S[2 * k + 1] -= w * x[2];
+                z[k] -= w * x[0];
+            }
+        solve_small(2, S, z);
+    }
+    double capv = CAP_V * Vt, lnfloor = log(P->n_floor), size = 0.0;
+    for (int r = 0; r < NP; ++r) {
+        double step[5];
+        for (int c = 0; c < nc; ++c) {
+            const double *x = s->X + (size_t)kr * (nc * r + c);
+            double delta = nc == 5 ? x[0] - (x[1] * z[0] + x[2] * z[1]) : x[0];
+            if (c == 0) {
+                step[c] = clipd(delta, -capv, capv);
+            } else {
+                step[c] = clipd(delta, -CAP_N, CAP_N);
+                if (c < 3 && y[nc * r + c] < lnfloor) { /* density_steps (solver.py:74-88) */
+                    double in_n = delta > -1 ? log1p(delta) : -CAP_N;
+                    step[c] = clipd(in_n, -FLOOR_DROP_CAP, CAP_N);
+                }
+            }
+        }
+        for (int c = 0; c < nc; ++c) y[nc * r + c] += step[c];
+        size = nmax(size, fabs(step[0]) / Vt);
+        for (int c = 1; c < nc; ++c) {
+            double dens = exp(y[nc * r + c]);
+            size = nmax(size, fabs(step[c]) * (dens / (dens + P->n_floor)));
+        }
+    }
+    if (nc == 5) {
+        double dl0 = clipd(z[0], -ION_LOG_CAP, ION_LOG_CAP), dl1 = clipd(z[1], -ION_LOG_CAP, ION_LOG_CAP);
+        logs[0] += dl0;
+        logs[1] += dl1;
+        size = nmax(size, nmax(fabs(dl0), fabs(dl1)));
+    }
+    return size;
+}
+
+/* `_newton_loop` (batch.py:740-848) of a time step: the four stop rules, the current of each
+ * iterate the transient's. Returns converged; *J the last iterate's current, *it the steps. */
+static int newton_loop_time(const problem *P, tscratch *s, double *y, double *logs, int nc,
+                            double Vint, double g_frac, const tstep *ts, double tol, int max_iter,
+                            double *J_out, int *it_out) {
+    double hist[STALL_STEPS + 1], Jh[STALL_STEPS + 1];
+    for (int k = 0; k <= STALL_STEPS; ++k) hist[k] = Jh[k] = INFINITY;
+    int it = 0, done = 0;
+    double J_scale = P->dev->J_scale;
+    while (!done && it < max_iter && !isnan(hist[STALL_STEPS])) {
+        double size = newton_step_time(P, s, y, logs, nc, Vint, g_frac, ts);
+        for (int k = 0; k < STALL_STEPS; ++k) { hist[k] = hist[k + 1]; Jh[k] = Jh[k + 1]; }
+        hist[STALL_STEPS] = size;
+        full_of(P, y, nc, Vint, ts->dti, s->full);
+        Jh[STALL_STEPS] = current_step(P, s->full, nc, g_frac, ts);
+        ++it;
+        double hmax = -INFINITY, hmin_tail = INFINITY;
+        int anynan = 0;
+        for (int k = 0; k <= STALL_STEPS; ++k) { hmax = fmax(hmax, hist[k]); anynan |= isnan(hist[k]); }
+        for (int k = 1; k <= STALL_STEPS; ++k) hmin_tail = fmin(hmin_tail, hist[k]);
+        int stalled = it > STALL_STEPS && !anynan && hmax < ROUND_OFF_FLOOR && hmin_tail >= 0.1 * hist[0];
+        double Jn = Jh[STALL_STEPS], scale = J_scale + fabs(Jn), jdev = 0.0;
+        int jnan = 0;
+        for (int k = 0; k <= STALL_STEPS; ++k) {
+            double v = fabs(Jh[k] - Jn);
+            if (isnan(v)) jnan = 1;
+            jdev = fmax(jdev, v);
+        }
+        int settled = it > STALL_STEPS && size < CURRENT_STEP_CAP && !jnan && jdev < CURRENT_TOL * scale;
+        double prev = hist[STALL_STEPS - 1];
+        int quadratic = it > 1 && size < QUADRATIC_TOL && size < QUADRATIC_RATIO * prev;
+        done = size < tol || stalled || settled || quadratic;
+    }
+    *J_out = Jh[STALL_STEPS];
+    *it_out = it;
+    return done;
+}
+
+/* `solve_transient` for one device (transient.py:270-464 `_solve_one`, init None): the steady
+ * state at the first step's voltage from the start (`batch._start`), then every step from the
+ * one before, the first guess extrapolated linearly in time from the last two kept states
+ * (held to one Newton step's caps; none from or into a steady state), each step's dt from the
+ * last step kept; failureMode 0 stops at a failed step, 1 keeps it, 2 skips it (Track 0
+ * throughout). Outputs per step: J (internal), J_ext (with the shunt), converged, iterations,
+ * kept (a row of ZimT's tJFile). Returns 0, -1 for a bad shape, -3 for a device outside this
+ * subset (a pre-bias; an ion region reaching a contact). */
+int ddo_transient(const ddo_device *d, const double *t, const double *V, const double *G,
+                  int nt, const ddo_topts *o, double *Jout, double *Jext, uint8_t *okout,
+                  int32_t *itsout, uint8_t *keptout) {
+    int NP = d->NP;
+    if (NP < 2 || NP > 4096 || nt < 1) return -1;
+    if (d->pre_cond) return -3;
+    problem P;
+    problem_init(&P, d, o->bernoulli);
+    int ions = d->a_total > 0 || d->c_total > 0;
+    if (ions && (P.amask_full[0] > 0 || P.cmask_full[0] > 0 || P.amask_full[NP + 1] > 0 ||
+                 P.cmask_full[NP + 1] > 0)) {
+        problem_free(&P);
+        return -3;
+    }
+    int nc = ions ? 5 : 3;
+    size_t ny = (size_t)NP * nc;
+    scratch s3;
+    scratch_init(&s3, NP);
+    tscratch s;
+    tscratch_init(&s, NP);
+    double Vt = d->Vt;
+    /* the steady start at the first voltage (transient.py:307-313) */
+    ddo_opts so = {o->tol, o->max_iter, o->max_iter, 0, o->bernoulli};
+    double *y3 = alloc_d((size_t)NP * 3), logs[2];
+    double V0 = V[0];
+    start_state(&P, &s3, d, V0, G[0], &so, y3, logs);
+    {
+        double *yn = alloc_d((size_t)NP * 3), ln[2] = {logs[0], logs[1]}, Jd;
+        int itd;
+        ionscale sc = {1.0, d->a_total, d->c_total};
+        memcpy(yn, y3, (size_t)NP * 3 * sizeof(double));
+        int ok_s = newton_loop(&P, &s3, yn, ln, V0, G[0], &sc, o->tol, o->max_iter, 0, 0, &Jd, &itd);
+        if (ok_s) { memcpy(y3, yn, (size_t)NP * 3 * sizeof(double)); logs[0] = ln[0]; logs[1] = ln[1]; }
+        free(yn);
+    }
+    double *y = alloc_d(ny), *y_prev = alloc_d(ny), *guess = alloc_d(ny), *after = alloc_d((size_t)(NP + 2) * nc);
+    for (int r = 0; r < NP; ++r) {
+        for (int c = 0; c < 3; ++c) y[nc * r + c] = y3[3 * r + c];
+        if (ions) { /* the ions' densities as unknowns: Boltzmann's profiles, 0 outside */
+            double Vm = y3[3 * r];
+            y[nc * r + 3] = P.am[r] > 0 ? logs[0] + Vm / Vt : 0.0;
+            y[nc * r + 4] = P.cm[r] > 0 ? logs[1] - Vm / Vt : 0.0;
+        }
+    }
+    memcpy(y_prev, y, ny * sizeof(double));
+    double logs_prev[2] = {logs[0], logs[1]};
+    double *fb = alloc_d(NP), *fb_new = alloc_d(NP);
+    tstep ts;
+    ts.n_prev = alloc_d(NP + 2); ts.p_prev = alloc_d(NP + 2); ts.a_prev = alloc_d(NP + 2);
+    ts.c_prev = alloc_d(NP + 2); ts.V_prev = alloc_d(NP + 2); ts.fb = fb;
+    ts.width = alloc_d(NP + 1);
+    ts.zimt = o->zimt_current;
+    for (int e = 0; e <= NP; ++e) /* `_sum_widths` (batch.py:405-418) */
+        ts.width[e] = o->zimt_current || e == 0 ? d->h[e] : 0.5 * (d->h[e - 1] + d->h[e]);
+    double V_now = V0, V_before = V0, dti_now = 0.0, t_last = 0.0;
+    int first = 1, stopped = 0;
+    double caps[5] = {CAP_V * Vt, CAP_N, CAP_N, CAP_N, CAP_N};
+    for (int k = 0; k < nt; ++k) {
+        double dti = t[k] == 0 ? 0.0 : 1.0 / (t[k] > t_last ? t[k] - t_last : 1.0);
+        /* the state before the step and its time terms (`_time_terms`) */
+        full_of(&P, y, nc, V_now, dti_now, after);
+        ts.dti = dti;
+        for (int i = 0; i < NP + 2; ++i) {
+            ts.n_prev[i] = exp(after[nc * i + 1]);
+            ts.p_prev[i] = exp(after[nc * i + 2]);
+            ts.V_prev[i] = after[nc * i];
+            ts.a_prev[i] = ions ? exp(after[nc * i + 3]) * P.amask_full[i] : 0.0;
+            ts.c_prev[i] = ions ? exp(after[nc * i + 4]) * P.cmask_full[i] : 0.0;
+        }
+        int moving = dti > 0 && dti_now > 0 && !first;
+        double rt = moving ? dti_now / dti : 0.0;
+        for (size_t q = 0; q < ny; ++q) {
+            double cap = caps[q % nc];
+            guess[q] = y[q] + clipd(rt * (y[q] - y_prev[q]), -cap, cap);
+        }
+        double lg[2];
+        for (int j = 0; j < 2; ++j) lg[j] = logs[j] + clipd(rt * (logs[j] - logs_prev[j]), -ION_LOG_CAP, ION_LOG_CAP);
+        double Vk = V[k];
+        if (ions) { /* in a steady state Boltzmann's normalisations, over a step the scales */
+            if (dti > 0) {
+                if (!moving) lg[0] = lg[1] = 0.0;
+            } else {
+                double *Vf = alloc_d(NP + 2), bnd[6];
+                ionscale sc = {1.0, d->a_total, d->c_total};
+                bounds_of(d, Vk, bnd);
+                Vf[0] = bnd[0]; Vf[NP + 1] = bnd[3];
+                for (int r = 0; r < NP; ++r) Vf[r + 1] = guess[nc * r];
+                ion_logs(&P, Vf, &sc, lg);
+                free(Vf);
+            }
+        }
+        double J;
+        int it;
+        int ok = newton_loop_time(&P, &s, guess, lg, nc, Vk, G[k], &ts, o->tol, o->max_iter, &J, &it);
+        int finite = all_finite(guess, (int)ny) && isfinite(lg[0]) && isfinite(lg[1]);
+        int running = !stopped;
+        int failed = running && !ok && o->failure_mode == 0;
+        int keep = running && !failed && finite &&
+                   (ok || o->failure_mode == 1 || (o->failure_mode == 2 && dti == 0));
+        Jout[k] = J;
+        Jext[k] = J + d->G_shunt * Vk;
+        okout[k] = (uint8_t)ok;
+        itsout[k] = it;
+        keptout[k] = (uint8_t)keep;
+        if (keep) {
+            full_of(&P, guess, nc, Vk, dti, after);
+            for (int r = 0; r < NP; ++r) {
+                int i = r + 1;
+                fb_new[r] = trap_end(P.Nt[r], P.Cn[r], P.Cp[r], P.nt0[r], P.pt0[r],
+                                     exp(after[nc * i + 1]), exp(after[nc * i + 2]), fb[r], dti);
+            }
+            memcpy(fb, fb_new, (size_t)NP * sizeof(double));
+            memcpy(y_prev, y, ny * sizeof(double));
+            memcpy(y, guess, ny * sizeof(double));
+            logs_prev[0] = logs[0]; logs_prev[1] = logs[1];
+            logs[0] = lg[0]; logs[1] = lg[1];
+            V_before = V_now;
+            V_now = Vk;
+            dti_now = dti;
+            t_last = t[k];
+        }
+        if (failed) stopped = 1;
+        first = 0;
+    }
+    (void)V_before;
+    free(y3); free(y); free(y_prev); free(guess); free(after); free(fb); free(fb_new);
+    free(ts.n_prev); free(ts.p_prev); free(ts.a_prev); free(ts.c_prev); free(ts.V_prev); free(ts.width);
+    tscratch_free(&s);
+    scratch_free(&s3);
+    problem_free(&P);
+    return 0;
+}
+
+/* One time step's system at a state, for the tests: F, A, B, C, U as assemble_step writes them
+ * (nc = 3 or 5), the current, and the constraints g (nc = 5). The step's fixed part: dti, the
+ * state before it (full_prev, (NP + 2) x nc, its ions' contacts set by the caller's ion_full
+ * rule) and the traps' occupation fb (NP). */
+int ddo_assemble_step(const ddo_device *d, const double *full, int nc, double g_frac, double dti,
+                      const double *full_prev, const double *fb, const double *logs, int bernoulli,
+                      int zimt, double *F, double *A, double *B, double *C, double *U, double *J,
+                      double *g) {
+    int NP = d->NP;
+    if (nc != 3 && nc != 5) return -1;
+    problem P;
+    problem_init(&P, d, bernoulli);
+    tstep ts;
+    ts.dti = dti;
+    ts.n_prev = alloc_d(NP + 2); ts.p_prev = alloc_d(NP + 2); ts.a_prev = alloc_d(NP + 2);
+    ts.c_prev = alloc_d(NP + 2); ts.V_prev = alloc_d(NP + 2); ts.width = alloc_d(NP + 1);
+    ts.fb = (double *)fb;
+    ts.zimt = zimt;
+    for (int e = 0; e <= NP; ++e)
+        ts.width[e] = zimt || e == 0 ? d->h[e] : 0.5 * (d->h[e - 1] + d->h[e]);
+    for (int i = 0; i < NP + 2; ++i) {
+        ts.n_prev[i] = exp(full_prev[nc * i + 1]);
+        ts.p_prev[i] = exp(full_prev[nc * i + 2]);
+        ts.V_prev[i] = full_prev[nc * i];
+        ts.a_prev[i] = nc == 5 ? exp(full_prev[nc * i + 3]) * P.amask_full[i] : 0.0;
+        ts.c_prev[i] = nc == 5 ? exp(full_prev[nc * i + 4]) * P.cmask_full[i] : 0.0;
+    }
+    assemble_step(&P, full, nc, g_frac, &ts, logs, F, A, B, C, U);
+    if (J) *J = current_step(&P, full, nc, g_frac, &ts);
+    if (g && nc == 5) {
+        double *W = alloc_d((size_t)NP * 2);
+        ion_constraints_time(&P, full, g, W);
+        free(W);
+    }
+    free(ts.n_prev); free(ts.p_prev); free(ts.a_prev); free(ts.c_prev); free(ts.V_prev); free(ts.width);
+    problem_free(&P);
+    return 0;
 }

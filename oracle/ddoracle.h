@@ -41,6 +41,7 @@ typedef struct {
     double ln_n_left, ln_n_right, ln_p_left, ln_p_right;
     double left_elec, a_total, c_total, V_pre, J_scale, G_shunt;
     int32_t pre_cond, fix_ions;
+    const double *mu_a, *mu_c; /* the mobile ions' mobility per point (transients) */
 } ddo_device;
 
 typedef struct {
@@ -84,6 +85,27 @@ int ddo_block_thomas(int NP, int b, int k, const double *A, const double *B, con
  * solve_jv's stop). */
 double ddo_noise_step(const ddo_device *dev, const double *y, const double *logs, double Vint,
                       double g_frac, int bernoulli, double eps);
+
+/* Options of a transient (transient.py:537-637 `solve_transient`). */
+typedef struct {
+    double tol;
+    int32_t max_iter, bernoulli, zimt_current, failure_mode;
+} ddo_topts;
+
+/* solve_transient for one device through nt steps (t, V internal = external, G the generation
+ * fraction): per step J (internal current), J_ext (with the shunt), converged, iterations,
+ * kept. Returns 0, -1 for a bad shape, -3 for a device outside the subset (a pre-bias, an ion
+ * region reaching a contact). */
+int ddo_transient(const ddo_device *dev, const double *t, const double *V, const double *G,
+                  int nt, const ddo_topts *o, double *J, double *J_ext, uint8_t *ok,
+                  int32_t *its, uint8_t *kept);
+
+/* One time step's residual, blocks (nc = 3, or 5 with the ions as unknowns), slots' columns,
+ * transient current and the ions' constraints at the full state `full` ((NP + 2) x nc). */
+int ddo_assemble_step(const ddo_device *dev, const double *full, int nc, double g_frac,
+                      double dti, const double *full_prev, const double *fb,
+                      const double *logs, int bernoulli, int zimt, double *F, double *A,
+                      double *B, double *C, double *U, double *J, double *g);
 
 /* The integral current of a full state (batch._current, no shunt). */
 double ddo_current(const ddo_device *dev, const double *full, double g_frac);

@@ -34,7 +34,7 @@ class _Device(ctypes.Structure):
             "L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_right",
             "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
             "G_shunt")
-    ] + [("pre_cond", ctypes.c_int32), ("fix_ions", ctypes.c_int32)]
+    ] + [("pre_cond", ctypes.c_int32), ("fix_ions", ctypes.c_int32), ("mu_a", _P), ("mu_c", _P)]
 
 
 class _Opts(ctypes.Structure):
@@ -44,6 +44,12 @@ class _Opts(ctypes.Structure):
 
 
 BERNOULLI = {"exact": 0, "simss": 1}
+
+
+class _TOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+                ("bernoulli", ctypes.c_int32), ("zimt_current", ctypes.c_int32),
+                ("failure_mode", ctypes.c_int32)]
 
 
 def build() -> Path:
@@ -60,6 +66,14 @@ def _load():
     lib.ddo_bernoulli.argtypes = [ctypes.c_double, ctypes.c_int, _P]
     lib.ddo_current.restype = ctypes.c_double
     lib.ddo_current.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_double]
+    lib.ddo_transient.restype = ctypes.c_int
+    lib.ddo_transient.argtypes = [ctypes.POINTER(_Device), _P, _P, _P, ctypes.c_int,
+                                  ctypes.POINTER(_TOpts), _P, _P, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+    lib.ddo_assemble_step.restype = ctypes.c_int
+    lib.ddo_assemble_step.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, _P, _P, _P, ctypes.c_int, ctypes.c_int,
+                                      _P, _P, _P, _P, _P, _P, _P]
     lib.ddo_noise_step.restype = ctypes.c_double
     lib.ddo_noise_step.argtypes = [ctypes.POINTER(_Device), _P, _P, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_double]
@@ -112,6 +126,8 @@ class _Pinned:
             "a_mask": np.asarray(dev.anion_mask, dtype=float),
             "c_mask": np.asarray(dev.cation_mask, dtype=float),
             "a_fixed": dev.anion_fixed, "c_fixed": dev.cation_fixed,
+            "mu_a": np.zeros(NP + 2) if dev.mu_anion is None else dev.mu_anion,
+            "mu_c": np.zeros(NP + 2) if dev.mu_cation is None else dev.mu_cation,
         }
         self.keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in arrays.items()}
         iface = np.zeros(NP + 1, dtype=np.uint8)
@@ -279,6 +295,55 @@ def noise_step(dev, y, Vint, logs=(0.0, 0.0), g_frac=None, bernoulli_name="exact
                                 pin.struct.g_frac, BERNOULLI[bernoulli_name], float(eps))
 
 
+def transient(dev, t, V, G_frac, tol=1e-10, max_iter=200, bernoulli_name="exact",
+              zimt_current=False, failure_mode=1):
+    """`solve_transient` for one device (transient.py:270-464) through the steps ``t`` (s),
+    internal voltages ``V`` and generation fractions ``G_frac``: a dict of per-step ``J``
+    (internal current), ``J_ext`` (with the shunt), ``converged``, ``iterations``, ``kept``."""
+    t, V, G = (np.ascontiguousarray(a, dtype=np.float64) for a in (t, V, G_frac))
+    n = len(t)
+    out = {"J": np.zeros(n), "J_ext": np.zeros(n), "converged": np.zeros(n, np.uint8),
+           "iterations": np.zeros(n, np.int32), "kept": np.zeros(n, np.uint8)}
+    pin = _Pinned(dev)
+    o = _TOpts(tol, max_iter, BERNOULLI[bernoulli_name], int(zimt_current), int(failure_mode))
+    rc = lib().ddo_transient(ctypes.byref(pin.struct), _ptr(t), _ptr(V), _ptr(G), n,
+                             ctypes.byref(o), _ptr(out["J"]), _ptr(out["J_ext"]),
+                             out["converged"].ctypes.data, out["iterations"].ctypes.data,
+                             out["kept"].ctypes.data)
+    if rc != 0:
+        raise ValueError(f"ddo_transient: error {rc}")
+    out["converged"] = out["converged"].astype(bool)
+    out["kept"] = out["kept"].astype(bool)
+    return out
+
+
+def assemble_step(dev, full, g_frac, dti, full_prev, fb, logs=(0.0, 0.0),
+                  bernoulli_name="exact", zimt=False):
+    """One time step's system at ``full`` ((NP + 2) x nc, nc = 3 or 5 with the ions'
+    logarithms): F (NP x nc), A, B, C (NP x nc x nc), U (NP x nc x 2), the transient current J
+    and the ions' constraints g (2), the step fixed by ``dti``, the state before it
+    ``full_prev`` and the traps' occupation ``fb`` (NP)."""
+    full = np.ascontiguousarray(full, dtype=np.float64)
+    nc = full.shape[1]
+    NP = full.shape[0] - 2
+    F = np.zeros((NP, nc))
+    A, B, C = (np.zeros((NP, nc, nc)) for _ in range(3))
+    U = np.zeros((NP, nc, 2))
+    J = np.zeros(1)
+    g = np.zeros(2)
+    pin = _Pinned(dev)
+    prev = np.ascontiguousarray(full_prev, dtype=np.float64)
+    f = np.ascontiguousarray(fb, dtype=np.float64)
+    lg = np.ascontiguousarray(logs, dtype=np.float64)
+    rc = lib().ddo_assemble_step(ctypes.byref(pin.struct), _ptr(full), nc, float(g_frac),
+                                 float(dti), _ptr(prev), _ptr(f), _ptr(lg),
+                                 BERNOULLI[bernoulli_name], int(zimt), _ptr(F), _ptr(A),
+                                 _ptr(B), _ptr(C), _ptr(U), _ptr(J), _ptr(g))
+    if rc != 0:
+        raise ValueError(f"ddo_assemble_step: error {rc}")
+    return F, A, B, C, U, float(J[0]), g
+
+
 def current(dev, full, g_frac=None):
     pin = _Pinned(dev, g_frac)
     full = np.ascontiguousarray(full, dtype=np.float64)
@@ -364,5 +429,5 @@ def _ions_on_grid(dev, Vfull):
 
 
 __all__ = ["assemble", "assemble_ions", "bernoulli", "block_thomas", "build", "current", "j_scale", "lib",
-           "noise_step", "observed", "solve_batch", "solve_jv", "solve_one", "solve_states"]
+           "assemble_step", "noise_step", "observed", "transient", "solve_batch", "solve_jv", "solve_one", "solve_states"]
 
