@@ -186,6 +186,38 @@ int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint, const
                   const double *laps, int32_t nf, double dt, int32_t bernoulli, double *Y,
                   int32_t *status, void *workspace, size_t workspace_bytes, void *hip_stream);
 
+/* Options of ddh_transient: src/jaxdd/transient.py:537-637 (`solve_transient`). */
+typedef struct {
+    double tol;            /* Newton step tolerance of each time step (1e-10) */
+    int32_t max_iter;      /* Newton steps per time step (200) */
+    int32_t bernoulli;     /* 0 exact B(x), 1 SIMsalabim's */
+    int32_t zimt_current;  /* 1: ZimT's sums in the integral current (the interval after each
+                              point, the left contact's change counted); 0: control volumes */
+    int32_t failure_mode;  /* ZimT's failureMode: 0 stop at a failed step, 1 keep it, 2 skip it */
+} ddh_topts;
+
+/* Workspace bytes ddh_transient needs. */
+size_t ddh_transient_workspace_bytes(int32_t n, int32_t NP);
+
+/* Transients of a batch: replaces src/jaxdd/transient.py:467-480 (`_solve_batch`, the vmap of
+ * `_solve_one` :270-464) for devices without an open circuit, a series resistance, interface
+ * traps or ZimT's automatic stop (Track 0), each ion region inside the device. Every device
+ * runs its own steps: t, V (internal voltage), G (generation fraction), [n][nt]; t[.][0] = 0
+ * (a steady state) and a time 0 later is a steady state again. y0: [NP][3][n] each device's
+ * steady state at its first voltage and generation fraction (ddh_solve_batch_states), from
+ * which the first step starts. ion_mu: [nseg][2][n] the mobile anions' and cations' mobility of
+ * each segment; iface: [NP + 1][n] 1 where the interval after a point crosses a layer
+ * interface. ions: 1 (any device with mobile ions) solves 5 x 5 systems with the ions'
+ * densities as unknowns, 0 the 3 x 3 ones. Outputs [n][nt]: J the internal current (A/m^2),
+ * J_ext with the shunt, converged, Newton steps, kept (the steps ZimT keeps). status [n]: 0, or
+ * 3 for a device outside this path (a pre-bias; ions reaching a contact), whose outputs are NaN
+ * and 0. workspace: ddh_transient_workspace_bytes(n, NP) bytes of device memory. */
+int ddh_transient(const ddh_batch *b, const double *ion_mu, const uint8_t *iface, const double *t,
+                  const double *V, const double *G, int32_t nt, const double *y0,
+                  const ddh_topts *opts, int32_t ions, double *J, double *J_ext, uint8_t *ok,
+                  int32_t *its, uint8_t *kept, int32_t *status, void *workspace,
+                  size_t workspace_bytes, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ _LAZY = {
     "JVResult": "solver", "bernoulli": "solver", "bernoulli_simss": "solver", "solve_jv": "solver",
     "block_sweep": "sweep",
     "ImpedanceResult": "impedance", "solve_impedance": "impedance",
+    "TransientResult": "transient", "solve_transient": "transient",
 }
 
 
@@ -42,6 +43,7 @@ __all__ = [
     "Device",
     "ImpedanceResult",
     "JVResult",
+    "TransientResult",
     "Layer",
     "Setup",
     "Structure",
@@ -64,4 +66,5 @@ __all__ = [
     "solve_impedance",
     "solve_jv",
     "solve_states",
+    "solve_transient",
 ]

@@ -21,6 +21,12 @@ class Batch(ctypes.Structure):
                 ("seg", ctypes.c_void_p), ("sc", ctypes.c_void_p)]
 
 
+class TOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+                ("bernoulli", ctypes.c_int32), ("zimt_current", ctypes.c_int32),
+                ("failure_mode", ctypes.c_int32)]
+
+
 class Opts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
                 ("start_iter", ctypes.c_int32), ("strict", ctypes.c_int32),
@@ -33,7 +39,7 @@ SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solv
            "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
            "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions",
            "ddh_impedance_workspace_bytes", "ddh_impedance", "ddh_noalloc_scratch_bytes",
-           "ddh_solve_batch_noalloc")
+           "ddh_solve_batch_noalloc", "ddh_transient_workspace_bytes", "ddh_transient")
 
 _lib = None
 
@@ -79,6 +85,12 @@ def load():
     lib.ddh_solve_batch_noalloc.restype = ctypes.c_int
     lib.ddh_solve_batch_noalloc.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts),
                                             vp, vp, vp, vp, vp]
+    lib.ddh_transient_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_transient_workspace_bytes.argtypes = [i32, i32]
+    lib.ddh_transient.restype = ctypes.c_int
+    lib.ddh_transient.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, vp, i32, vp,
+                                  ctypes.POINTER(TOpts), i32, vp, vp, vp, vp, vp, vp, vp,
+                                  ctypes.c_size_t, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 2:

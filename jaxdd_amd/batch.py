@@ -82,7 +82,12 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     n = len(devices)
     P = next(iter(sizes))
     fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 22)
-    change = np.any(fields[:, 1:] != fields[:, :-1], axis=2)
+    # the mobile ions' mobilities (read by the transient only) also split segments
+    ion_mu = np.stack([np.stack([np.zeros(P) if d.mu_anion is None else d.mu_anion,
+                                 np.zeros(P) if d.mu_cation is None else d.mu_cation], -1)
+                       for d in devices])  # (n, P, 2)
+    change = np.any(np.concatenate([fields, ion_mu], axis=2)[:, 1:]
+                    != np.concatenate([fields, ion_mu], axis=2)[:, :-1], axis=2)
     sid = np.concatenate([np.zeros((n, 1), dtype=np.int64), np.cumsum(change, axis=1)], axis=1)
     nseg = int(sid.max()) + 1
     if nseg > MAX_SEGMENTS:
@@ -92,12 +97,15 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     first = np.concatenate([np.ones((n, 1), dtype=bool), change], axis=1)  # (n, P)
     dev_idx, pt_idx = np.nonzero(first)
     seg[sid[dev_idx, pt_idx], :, dev_idx] = fields[dev_idx, pt_idx]
+    imu = np.zeros((nseg, 2, n))
+    imu[sid[dev_idx, pt_idx], :, dev_idx] = ion_mu[dev_idx, pt_idx]
     # segments beyond a device's own count repeat its last (never read)
     last = sid[:, -1]
     for s in range(nseg):
         pad = np.flatnonzero(last < s)
         if pad.size:
             seg[s, :, pad] = seg[last[pad], :, pad]
+            imu[s, :, pad] = imu[last[pad], :, pad]
     pt = np.stack([
         np.stack([d.h, d.x, d.G, d.anion_fixed - d.cation_fixed], axis=0) for d in devices
     ], axis=-1)  # (4, P, n)
@@ -113,20 +121,24 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     fixed = np.stack([np.stack([d.anion_fixed, d.cation_fixed]) for d in devices], axis=-1)
     return {"pt": pt, "sid": np.ascontiguousarray(sid.T.astype(np.uint8)), "seg": seg,
             "sc": sc, "nseg": nseg, "R_series": np.array([d.R_series for d in devices]),
-            "iface": iface, "fixed": np.ascontiguousarray(fixed)}
+            "iface": iface, "fixed": np.ascontiguousarray(fixed), "imu": imu}
 
 
 def merge(parts: list[dict]) -> dict:
     """Packed arrays of several device lists joined along the lane axis (the segment tables
     padded to the largest count, the extra segments never read)."""
     nseg = max(p["nseg"] for p in parts)
-    segs = []
-    for p in parts:
-        seg = p["seg"]
-        if seg.shape[0] < nseg:
-            seg = np.concatenate([seg, np.repeat(seg[-1:], nseg - seg.shape[0], axis=0)])
-        segs.append(seg)
-    out = {"nseg": nseg, "seg": np.concatenate(segs, axis=2)}
+
+    def padded(key):
+        out = []
+        for p in parts:
+            a = p[key]
+            if a.shape[0] < nseg:
+                a = np.concatenate([a, np.repeat(a[-1:], nseg - a.shape[0], axis=0)])
+            out.append(a)
+        return np.concatenate(out, axis=2)
+
+    out = {"nseg": nseg, "seg": padded("seg"), "imu": padded("imu")}
     for k in ("pt", "sid", "sc", "iface", "fixed"):
         out[k] = np.ascontiguousarray(np.concatenate([p[k] for p in parts], axis=-1))
     out["R_series"] = np.concatenate([p["R_series"] for p in parts])
@@ -154,7 +166,7 @@ def from_packed(packed: dict, device: str | torch.device = "cuda") -> dict:
     return {
         "n": n, "NP": NP, "nseg": packed["nseg"],
         "_pt": t(packed["pt"]), "_sid": t(packed["sid"], torch.uint8), "_seg": t(packed["seg"]),
-        "_sc": t(sc), "_iface": t(packed["iface"], torch.uint8),
+        "_sc": t(sc), "_iface": t(packed["iface"], torch.uint8), "_imu": t(packed["imu"]),
         # the reference's per-device keys (read by solve_batch at each call)
         "J_scale": t(col["J_scale"]), "g_frac": t(col["g_frac"]), "Vt": t(col["Vt"]),
         "L": t(col["L"]), "V_pre": t(col["V_pre"]), "pre_cond": t(col["pre_cond"]),

@@ -15,6 +15,7 @@
 #include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
+#include "ddhip_transient.h"
 
 namespace ddh {
 
@@ -226,6 +227,23 @@ __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
 __global__ void __launch_bounds__(256) impedance_kernel(ImpArgs a) {
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     if (gid < (long)a.n * a.nf) impedance_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
+}
+
+// The transient (ddhip_transient.h): one lane per device through all of its time steps, one
+// Newton step per turn of the loop; a lane whose step's solve ends begins its next step at
+// once. NC = 5 with mobile ions (their densities as unknowns), 3 without.
+template <int NC>
+__global__ void __launch_bounds__(64) transient_kernel(TrArgs a) {
+    const int lane = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = lane < a.n;
+    TrLane S;
+    S.done = 1;
+    if (valid) tr_init<NC>(a, lane, S);
+    while (true) {
+        const bool act = valid && !S.done;
+        if (!__any(act)) break;
+        if (act) tr_newton<NC>(a, lane, S);
+    }
 }
 
 // The gradients of every device's J-V curve: one wavefront per device (the workgroups loop
@@ -763,5 +781,63 @@ extern "C" int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, cons
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "block_sweep_kernel launch");
+    return DDH_OK;
+}
+
+
+// ---- the transient (ddhip_transient.h) -----------------------------------------------------
+struct TrLayout {
+    size_t Y, fb, fac, total;
+};
+
+static TrLayout tr_layout(int32_t n, int32_t NP) {
+    TrLayout w;
+    size_t off = 0;
+    w.Y = off; off = align256(off + (size_t)3 * NP * ddh::TR_NC * n * sizeof(double));
+    w.fb = off; off = align256(off + (size_t)NP * n * sizeof(double));
+    w.fac = off; off = align256(off + (size_t)NP * ddh::TR_FAC * n * sizeof(double));
+    w.total = off;
+    return w;
+}
+
+extern "C" size_t ddh_transient_workspace_bytes(int32_t n, int32_t NP) {
+    if (n <= 0 || NP < 2) return 0;
+    return tr_layout(n, NP).total;
+}
+
+extern "C" int ddh_transient(const ddh_batch *b, const double *ion_mu, const uint8_t *iface,
+                             const double *t, const double *V, const double *G, int32_t nt,
+                             const double *y0, const ddh_topts *o, int32_t ions, double *J,
+                             double *J_ext, uint8_t *ok, int32_t *its, uint8_t *kept,
+                             int32_t *status, void *workspace, size_t workspace_bytes,
+                             void *stream) {
+    if (!b || !ion_mu || !iface || !t || !V || !G || !y0 || !o || !J || !J_ext || !ok || !its ||
+        !kept || !status || !workspace)
+        return fail(DDH_E_ARG, "ddh_transient: NULL argument");
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || nt < 1 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
+        return fail(DDH_E_SHAPE, "ddh_transient: n > 0, 2 <= NP <= 65536, nt >= 1, 1 <= nseg <= 14");
+    if (o->bernoulli != 0 && o->bernoulli != 1)
+        return fail(DDH_E_ARG, "ddh_transient: bernoulli must be 0 (exact) or 1 (simss)");
+    if (o->max_iter < 1) return fail(DDH_E_ARG, "ddh_transient: max_iter must be >= 1");
+    if (o->failure_mode < 0 || o->failure_mode > 2)
+        return fail(DDH_E_ARG, "ddh_transient: failure_mode must be 0, 1 or 2");
+    const TrLayout w = tr_layout(b->n, b->NP);
+    if (workspace_bytes < w.total) return fail(DDH_E_WORKSPACE, "ddh_transient: workspace too small");
+    char *base = (char *)workspace;
+    ddh::TrArgs a = {};
+    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.nt = nt;
+    a.pt = b->pt; a.sid = b->sid; a.iface = iface; a.seg = b->seg; a.sc = b->sc; a.imu = ion_mu;
+    a.t = t; a.V = V; a.G = G; a.y0 = y0;
+    a.Y = (double *)(base + w.Y);
+    a.fb = (double *)(base + w.fb);
+    a.fac = (double *)(base + w.fac);
+    a.J = J; a.Jext = J_ext; a.ok = ok; a.kept = kept; a.its = its; a.status = status;
+    a.tol = o->tol; a.max_iter = o->max_iter; a.bern = o->bernoulli; a.zimt = o->zimt_current;
+    a.failure_mode = o->failure_mode;
+    dim3 grid((b->n + 63) / 64), block(64);
+    if (ions) hipLaunchKernelGGL(ddh::transient_kernel<5>, grid, block, 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL(ddh::transient_kernel<3>, grid, block, 0, (hipStream_t)stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "transient_kernel launch");
     return DDH_OK;
 }

@@ -11,6 +11,7 @@
 #include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
+#include "ddhip_transient.h"
 
 #include <cstdlib>
 #include <vector>
@@ -371,4 +372,33 @@ extern "C" int emu_xcd_hits(int n, int nblocks, int32_t *hits) {
         }
     }
     return outside;
+}
+
+// The transient (ddhip_transient.h): each lane's whole sequence of steps in turn (the kernel's
+// lanes share nothing).
+extern "C" int emu_transient(int n, int NP, int nseg, int nt, const double *pt, const uint8_t *sid,
+                             const uint8_t *iface, const double *seg, const double *sc,
+                             const double *imu, const double *t, const double *V, const double *G,
+                             const double *y0, double tol, int max_iter, int bern, int zimt,
+                             int failure_mode, int ions, double *J, double *Jext, uint8_t *ok,
+                             int32_t *its, uint8_t *kept, int32_t *status) {
+    TrArgs a = {};
+    a.n = n; a.NP = NP; a.nseg = nseg; a.nt = nt;
+    a.pt = pt; a.sid = sid; a.iface = iface; a.seg = seg; a.sc = sc; a.imu = imu;
+    a.t = t; a.V = V; a.G = G; a.y0 = y0;
+    std::vector<double> Y((size_t)3 * NP * TR_NC * n), fb((size_t)NP * n), fac((size_t)NP * TR_FAC * n);
+    a.Y = Y.data(); a.fb = fb.data(); a.fac = fac.data();
+    a.J = J; a.Jext = Jext; a.ok = ok; a.kept = kept; a.its = its; a.status = status;
+    a.tol = tol; a.max_iter = max_iter; a.bern = bern; a.zimt = zimt; a.failure_mode = failure_mode;
+    for (int lane = 0; lane < n; ++lane) {
+        TrLane S;
+        if (ions) {
+            tr_init<5>(a, lane, S);
+            while (!S.done) tr_newton<5>(a, lane, S);
+        } else {
+            tr_init<3>(a, lane, S);
+            while (!S.done) tr_newton<3>(a, lane, S);
+        }
+    }
+    return 0;
 }

@@ -42,6 +42,8 @@ def lib():
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
         _lib.emu_xcd_hits.argtypes = [i, i, vp]
+        _lib.emu_transient.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       ctypes.c_double, i, i, i, i, i, vp, vp, vp, vp, vp, vp]
     return _lib
 
 
@@ -170,3 +172,26 @@ def xcd_hits(n: int, nblocks: int):
     hits = np.zeros(n, dtype=np.int32)
     outside = lib().emu_xcd_hits(n, nblocks, _p(hits))
     return hits, outside
+
+
+def transient_packed(packed, t, V, G, y0, tol=1e-10, max_iter=200, bern="exact", zimt=False,
+                     failure_mode=1, ions=None):
+    """The transient kernel's per-lane code on the CPU: ``t``, ``V``, ``G`` (n, nt) per device,
+    ``y0`` the steady start (NP, 3, n) lane-innermost. Returns J, J_ext, ok, its, kept (n, nt)
+    and status (n)."""
+    n = packed["sc"].shape[1]
+    NP = packed["pt"].shape[0] - 2
+    nt = np.asarray(t).shape[-1]
+    t, V, G = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, float), (n, nt))) for a in (t, V, G))
+    if ions is None:
+        ions = bool(np.any(packed["sc"][12] > 0) or np.any(packed["sc"][13] > 0))
+    J, Jext = np.zeros((n, nt)), np.zeros((n, nt))
+    ok, kept = np.zeros((n, nt), np.uint8), np.zeros((n, nt), np.uint8)
+    its, status = np.zeros((n, nt), np.int32), np.zeros(n, np.int32)
+    arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "iface", "seg", "sc", "imu")]
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    lib().emu_transient(n, NP, packed["nseg"], nt, *(_p(a) for a in arrs), _p(t), _p(V), _p(G),
+                        _p(y0), tol, max_iter, {"exact": 0, "simss": 1}[bern], int(zimt),
+                        failure_mode, int(ions), _p(J), _p(Jext), _p(ok), _p(its), _p(kept),
+                        _p(status))
+    return J, Jext, ok.astype(bool), its, kept.astype(bool), status

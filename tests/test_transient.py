@@ -158,3 +158,133 @@ def test_one_implicit_euler_step_is_the_impedance_at_its_rate(oracle):
             assert abs(slope / Y.real - 1) < 1e-6, (k, dt, slope, Y)
             if dt <= 1e-5:  # the mid-band: far from the steady slope
                 assert Y.real > 4 * Y_dc
+
+
+# ---- the kernel's per-lane code on the CPU, and the GPU -------------------------------------
+def c5_steps(nsteps, V_step=0.9):
+    """The transient workload's steps: the steady state at 0 V, then a step to ``V_step`` held
+    through ``nsteps`` log-spaced implicit-Euler steps from 1 ns to 10 s, lit."""
+    t = np.concatenate([[0.0], np.logspace(-9, 1, nsteps)])
+    V = np.concatenate([[0.0], np.full(nsteps, V_step)])
+    return t, V, np.ones(nsteps + 1)
+
+
+def _replay(emu, devs, t, V, G, bern="exact", zimt=False, failure_mode=1):
+    from jaxdd_amd.batch import pack
+
+    packed = pack(devs)
+    start = dict(packed, sc=packed["sc"].copy())
+    start["sc"][2] = G[0]
+    NP = len(devs[0].x) - 2
+    st = np.zeros((1, NP, 3, len(devs)))
+    emu.solve_packed(start, np.array([V[0]]), max_iter=200, bernoulli=bern, states=st)
+    return emu.transient_packed(packed, t, V, G, st[0], bern=bern, zimt=zimt,
+                                failure_mode=failure_mode)
+
+
+@pytest.mark.parametrize("case", sorted(ZIMT))
+def test_replay_transient_is_the_oracles_on_zimt_runs(oracle, emu, case):
+    """The transient kernel's code on the CPU against the oracle on ZimT's runs: the same
+    Newton steps, the same currents to 1e-10 of the scale, and ZimT's to its tolerance."""
+    dev, tvg, ref = zimt_case(case)
+    V = np.where(tvg.oc, 0.0, tvg.Vext)
+    J, Jx, ok, its, kept, status = _replay(emu, [dev], tvg.t, V, tvg.G_frac, "simss", True,
+                                           tvg.failure_mode)
+    r = oracle.transient(dev, tvg.t, V, tvg.G_frac, bernoulli_name="simss", zimt_current=True,
+                         failure_mode=tvg.failure_mode)
+    scale = np.max(np.abs(ref["Jext"]))
+    assert not status.any() and ok.all() and kept.all()
+    assert np.array_equal(its[0], r["iterations"])
+    assert np.max(np.abs(Jx[0] - r["J_ext"])) < 1e-10 * scale
+    assert np.max(np.abs(Jx[0] - ref["Jext"])) < ZIMT[case] * scale
+
+
+@pytest.mark.parametrize("ions", [True, False])
+def test_replay_transient_is_the_oracles_on_the_workload(oracle, emu, ions):
+    """The transient workload's devices (ions and traps in the perovskite, NP = 300) through a
+    voltage step: the kernel's code against the oracle, curve by curve (1e-9 of each curve's
+    scale), flags equal."""
+    from jaxdd_amd import synthetic as S
+
+    tab = S.draw(4)
+    devs = [S.device_from_row(tab, k, 300, ions, False) for k in range(4)]
+    t, V, G = c5_steps(80)
+    J, Jx, ok, its, kept, status = _replay(emu, devs, t, V, G)
+    assert not status.any()
+    for k, dev in enumerate(devs):
+        r = oracle.transient(dev, t, V, G)
+        assert np.array_equal(ok[k], r["converged"]) and np.array_equal(kept[k], r["kept"])
+        scale = oracle.j_scale(dev) + np.max(np.abs(r["J_ext"]))
+        assert np.max(np.abs(Jx[k] - r["J_ext"])) < 1e-9 * scale
+
+
+def test_what_the_transient_refuses():
+    from jaxdd_amd.simss_input import TVG
+    from jaxdd_amd.transient import _steps
+
+    base = dict(t=np.array([0.0, 1e-6]), Vext=np.array([0.0, 0.5]), oc=np.array([False, False]),
+                G_frac=np.ones(2), track=np.zeros(2, int), dti=np.array([0.0, 1e6]))
+    _steps(TVG(**base), 2)
+    with pytest.raises(NotImplementedError, match="open circuit"):
+        _steps(TVG(**dict(base, oc=np.array([False, True]))), 2)
+    with pytest.raises(NotImplementedError, match="Track"):
+        _steps(TVG(**dict(base, track=np.array([0, 1]))), 2)
+    with pytest.raises(ValueError, match="one per device"):
+        _steps([TVG(**base)] * 3, 2)
+
+
+GPU_OFF = ("the GPU transient kernel is switched off: its first run faulted and the cause is not "
+           "yet shown (the CPU replay of its code passes bounds, ASan and UBSan checks)")
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason=GPU_OFF)
+@pytest.mark.parametrize("case", sorted(ZIMT))
+def test_gpu_transient_reproduces_zimt(torch_cuda, oracle, case):
+    """solve_transient on the GPU (ddh_transient through the C-ABI): ZimT's Jext to its
+    tolerance and the oracle's to 1e-9 of the scale, every step converged and kept."""
+    from jaxdd_amd.batch import prepare
+    from jaxdd_amd.transient import solve_transient
+
+    dev, tvg, ref = zimt_case(case)
+    print(f"transient {case} on the GPU", flush=True)
+    r = solve_transient(prepare([dev]), tvg, bernoulli_name="simss", zimt_current=True)
+    o = oracle.transient(dev, tvg.t, np.where(tvg.oc, 0.0, tvg.Vext), tvg.G_frac,
+                         bernoulli_name="simss", zimt_current=True, failure_mode=tvg.failure_mode)
+    scale = np.max(np.abs(ref["Jext"]))
+    assert r.converged.all() and r.accepted.all()
+    assert np.max(np.abs(r.J_ext[0] - o["J_ext"])) < 1e-9 * scale
+    assert np.max(np.abs(r.J_ext[0] - ref["Jext"])) < ZIMT[case] * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason=GPU_OFF)
+@pytest.mark.parametrize("ions", [True, False])
+def test_gpu_transient_matches_the_oracle_on_the_workload(torch_cuda, oracle, ions):
+    """The transient workload's devices (NP = 300, 24 devices: a partial wavefront) through a
+    voltage step on the GPU against the oracle: flags equal, each curve to 1e-9 of its scale."""
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import prepare
+    from jaxdd_amd.simss_input import TVG
+    from jaxdd_amd.transient import solve_transient
+
+    tab = S.draw(24)
+    devs = [S.device_from_row(tab, k, 300, ions, False) for k in range(24)]
+    t, V, G = c5_steps(120)
+    tvg = TVG(t=t, Vext=V, oc=np.zeros(len(t), bool), G_frac=G, track=np.zeros(len(t), int),
+              dti=np.zeros(len(t)))
+    print(f"transient workload ions={ions}: 24 devices x {len(t)} steps", flush=True)
+    r = solve_transient(prepare(devs), tvg)
+    for k, dev in enumerate(devs):
+        o = oracle.transient(dev, t, V, G)
+        assert np.array_equal(r.converged[k], o["converged"])
+        assert np.array_equal(r.accepted[k], o["kept"])
+        scale = oracle.j_scale(dev) + np.max(np.abs(o["J_ext"]))
+        assert np.max(np.abs(r.J_ext[k] - o["J_ext"])) < 1e-9 * scale
+
+
+def test_solve_transient_refuses_while_the_gpu_kernel_is_off():
+    from jaxdd_amd.transient import solve_transient
+
+    with pytest.raises(NotImplementedError, match="switched off"):
+        solve_transient({}, None)
