@@ -1219,7 +1219,7 @@ static void assemble_step(const problem *P, const double *full, int nc, double g
         /* the rates at the point: partials 0 ln n, 1 ln p */
         dual n = dexp_(dv(full[nc * i + 1], 0)), p = dexp_(dv(full[nc * i + 2], 1)), Rn, Rp;
         step_rates(P->kd[r], P->ni2[r], P->Nt[r], P->Cn[r], P->Cp[r], P->nt0[r], P->pt0[r], n,
-                   p, ts->fb[r], dti, &Rn, &Rp);
+                   p, ts->fb ? ts->fb[r] : 0.0, ts->fb ? dti : 0.0, &Rn, &Rp);
         double gen = P->Gs[r] * g_frac;
         /* electrons: hm Fn[i] - hp Fn[i-1] + fac (gen - Rn) - fac dti (n - n_prev) */
         double elec = hm * Fn[i].v - hp * Fn[i - 1].v + fac * (gen - Rn.v) - fac * dti * (n.v - ts->n_prev[i]);
@@ -1387,7 +1387,8 @@ static double current_step(const problem *P, const double *full, int nc, double 
             if (i > 0 && i <= NP) {
                 dual Rn, Rp;
                 step_rates(d->k_direct[i], P->ni2_full[i], d->Nt[i], d->Cn[i], d->Cp[i], d->nt0[i],
-                           d->pt0[i], dc(n[i]), dc(p[i]), ts->fb[i - 1], dti, &Rn, &Rp);
+                           d->pt0[i], dc(n[i]), dc(p[i]), ts->fb ? ts->fb[i - 1] : 0.0,
+                           ts->fb ? dti : 0.0, &Rn, &Rp);
                 Ri = c == 0 ? Rn.v : Rp.v;
             }
             U[i] = (i == 0 || i == NP + 1) ? 0.0 : Ri - d->G[i] * g_frac;
@@ -1725,7 +1726,7 @@ int ddo_transient(const ddo_device *d, const double *t, const double *V, const d
 int ddo_assemble_step(const ddo_device *d, const double *full, int nc, double g_frac, double dti,
                       const double *full_prev, const double *fb, const double *logs, int bernoulli,
                       int zimt, double *F, double *A, double *B, double *C, double *U, double *J,
-                      double *g) {
+                      double *g, double *Wout) {
     int NP = d->NP;
     if (nc != 3 && nc != 5) return -1;
     problem P;
@@ -1747,9 +1748,11 @@ int ddo_assemble_step(const ddo_device *d, const double *full, int nc, double g_
     }
     assemble_step(&P, full, nc, g_frac, &ts, logs, F, A, B, C, U);
     if (J) *J = current_step(&P, full, nc, g_frac, &ts);
-    if (g && nc == 5) {
-        double *W = alloc_d((size_t)NP * 2);
-        ion_constraints_time(&P, full, g, W);
+    if ((g || Wout) && nc == 5) {
+        double *W = alloc_d((size_t)NP * 2), gl[2];
+        ion_constraints_time(&P, full, gl, W);
+        if (g) { g[0] = gl[0]; g[1] = gl[1]; }
+        if (Wout) for (int k = 0; k < 2 * NP; ++k) Wout[k] = W[k];
         free(W);
     }
     free(ts.n_prev); free(ts.p_prev); free(ts.a_prev); free(ts.c_prev); free(ts.V_prev); free(ts.width);

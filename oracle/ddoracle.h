@@ -101,11 +101,14 @@ int ddo_transient(const ddo_device *dev, const double *t, const double *V, const
                   int32_t *its, uint8_t *kept);
 
 /* One time step's residual, blocks (nc = 3, or 5 with the ions as unknowns), slots' columns,
- * transient current and the ions' constraints at the full state `full` ((NP + 2) x nc). */
+ * transient current and the ions' constraints g (2) with their rows W (NP x 2, the derivative in
+ * each interior point's own ion logarithm) at the full state `full` ((NP + 2) x nc). fb NULL:
+ * the bulk traps at their steady occupation (instantaneous) instead of over the step. J, g and
+ * W may be NULL. */
 int ddo_assemble_step(const ddo_device *dev, const double *full, int nc, double g_frac,
                       double dti, const double *full_prev, const double *fb,
                       const double *logs, int bernoulli, int zimt, double *F, double *A,
-                      double *B, double *C, double *U, double *J, double *g);
+                      double *B, double *C, double *U, double *J, double *g, double *W);
 
 /* The integral current of a full state (batch._current, no shunt). */
 double ddo_current(const ddo_device *dev, const double *full, double g_frac);

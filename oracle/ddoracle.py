@@ -73,7 +73,7 @@ def _load():
     lib.ddo_assemble_step.restype = ctypes.c_int
     lib.ddo_assemble_step.argtypes = [ctypes.POINTER(_Device), _P, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, _P, _P, _P, ctypes.c_int, ctypes.c_int,
-                                      _P, _P, _P, _P, _P, _P, _P]
+                                      _P, _P, _P, _P, _P, _P, _P, _P]
     lib.ddo_noise_step.restype = ctypes.c_double
     lib.ddo_noise_step.argtypes = [ctypes.POINTER(_Device), _P, _P, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_double]
@@ -318,11 +318,13 @@ def transient(dev, t, V, G_frac, tol=1e-10, max_iter=200, bernoulli_name="exact"
 
 
 def assemble_step(dev, full, g_frac, dti, full_prev, fb, logs=(0.0, 0.0),
-                  bernoulli_name="exact", zimt=False):
+                  bernoulli_name="exact", zimt=False, rows=False):
     """One time step's system at ``full`` ((NP + 2) x nc, nc = 3 or 5 with the ions'
     logarithms): F (NP x nc), A, B, C (NP x nc x nc), U (NP x nc x 2), the transient current J
     and the ions' constraints g (2), the step fixed by ``dti``, the state before it
-    ``full_prev`` and the traps' occupation ``fb`` (NP)."""
+    ``full_prev`` and the traps' occupation ``fb`` (NP; None: the traps at their steady
+    occupation, instantaneous). ``rows``: also the constraints' rows W (NP x 2), each the
+    derivative in the point's own ion logarithm."""
     full = np.ascontiguousarray(full, dtype=np.float64)
     nc = full.shape[1]
     NP = full.shape[0] - 2
@@ -331,17 +333,19 @@ def assemble_step(dev, full, g_frac, dti, full_prev, fb, logs=(0.0, 0.0),
     U = np.zeros((NP, nc, 2))
     J = np.zeros(1)
     g = np.zeros(2)
+    W = np.zeros((NP, 2))
     pin = _Pinned(dev)
     prev = np.ascontiguousarray(full_prev, dtype=np.float64)
-    f = np.ascontiguousarray(fb, dtype=np.float64)
+    f = None if fb is None else _ptr(np.ascontiguousarray(fb, dtype=np.float64))
     lg = np.ascontiguousarray(logs, dtype=np.float64)
     rc = lib().ddo_assemble_step(ctypes.byref(pin.struct), _ptr(full), nc, float(g_frac),
-                                 float(dti), _ptr(prev), _ptr(f), _ptr(lg),
+                                 float(dti), _ptr(prev), f, _ptr(lg),
                                  BERNOULLI[bernoulli_name], int(zimt), _ptr(F), _ptr(A),
-                                 _ptr(B), _ptr(C), _ptr(U), _ptr(J), _ptr(g))
+                                 _ptr(B), _ptr(C), _ptr(U), _ptr(J), _ptr(g), _ptr(W))
     if rc != 0:
         raise ValueError(f"ddo_assemble_step: error {rc}")
-    return F, A, B, C, U, float(J[0]), g
+    out = (F, A, B, C, U, float(J[0]), g)
+    return out + (W,) if rows else out
 
 
 def current(dev, full, g_frac=None):

@@ -148,6 +148,135 @@ def test_external_impedance_of_the_series_resistance_and_the_shunt():
                        rtol=1e-15)
 
 
+# ---- the oracle's branch with mobile ions (impedance.py:242-262) ----------------------------
+def _ion_devices(n, NP):
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(max(n, 4) + 1)
+    return [S.device_from_row(t, k, NP, True, False) for k in range(1, n + 1)]
+
+
+def _ion_state(oracle, dev, V):
+    """The steady state at V with the ions' logarithms (`ddimpedance.ion_state`), the ions'
+    densities on the grid from the oracle's `solve_states`."""
+    from oracle import ddimpedance as I
+
+    st = oracle.solve_states(dev, [V], tol=1e-12, max_iter=200)
+    assert st["converged"].all()
+    full = np.stack([st["V"][0], np.log(st["n"][0]), np.log(st["p"][0])], -1)
+    return I.ion_state(dev, full, st["anion"][0], st["cation"][0])
+
+
+def test_ion_branch_of_a_device_without_ions_is_the_3x3_branch(oracle):
+    """A device without mobile ions through the 5 x 5 branch (its ion rows held, its slots
+    identities, as in a batch where other devices have ions) gives the 3 x 3 branch's admittance
+    to 20 times the spread of the 3 x 3 branch's two eliminations (adjugate and pivoted): what
+    the low frequencies' conditioning lets rounding move (up to 1e-6 of Y on a dark device)."""
+    from oracle import ddimpedance as I
+
+    for dev in _synthetic(2, 100):
+        full = I.full_state(dev, _steady(oracle, dev, V0), V0)
+        full5 = np.concatenate([full, np.zeros((full.shape[0], 2))], 1)
+        for dt in (None, 1e-6):
+            laps = I.laps_of(np.logspace(-3, 12, 6), dt)
+            Y3 = I.admittance(dev, full, laps, dt=dt)
+            Yp = I.admittance(dev, full, laps, dt=dt, pivoted=True)
+            Y5 = I.admittance_ions(dev, full5, laps, dt=dt)
+            spread = np.abs(Yp / Y3 - 1)
+            assert np.all(np.abs(Y5 / Y3 - 1) <= 20 * spread + 1e-10)
+
+
+def test_ion_masses_are_the_slots_columns(oracle):
+    """Each ion row's time term per unit of dti (`ion_masses`, restated from the row's form) is
+    minus its slot's column (the C oracle's U: the row's derivative in the slot's log, the
+    density before the step scaled by exp(log)), at the steady state and dti = 1; 0 outside the
+    region; the carriers' the 3 x 3 branch's."""
+    from oracle import ddimpedance as I
+
+    for dev in _ion_devices(2, 90):
+        f5 = _ion_state(oracle, dev, V0)
+        _, _, _, _, U, _, _ = oracle.assemble_step(dev, f5, dev.G_frac, 1.0, f5, None)
+        M = I.ion_masses(dev, f5)
+        for k in (0, 1):
+            col = M[:, 3 + k]
+            assert np.all(np.abs(col + U[:, 3 + k, k]) <= 1e-14 * np.max(np.abs(col)))
+            assert np.count_nonzero(col) == np.count_nonzero(np.asarray(
+                (dev.anion_mask, dev.cation_mask)[k])[1:-1])
+
+
+def test_ion_low_frequency_limit_is_the_steady_slope(oracle):
+    """The reference's tests/test_impedance.py:20-33 with ions: at 1e-9 Hz the ions follow the
+    voltage to their steady state, and the admittance is the slope of the steady current with
+    free ions (the oracle's solves at +-10 uV) to 1e-5, its imaginary part below 1e-4 of it."""
+    from oracle import ddimpedance as I
+
+    for dev in _ion_devices(2, 120):
+        dV = 1e-5
+        J, ok, _ = oracle.solve_one(dev, [V0 - dV, V0 + dV], tol=1e-12, max_iter=200)
+        assert ok.all()
+        slope = (J[1] - J[0]) / (2 * dV)
+        Y = I.admittance_ions(dev, _ion_state(oracle, dev, V0), I.laps_of([1e-9]))[0]
+        assert Y.real == pytest.approx(slope, rel=1e-5)
+        assert abs(Y.imag) < 1e-4 * abs(slope)
+
+
+def test_ion_high_frequency_limit_is_the_geometric_capacitance(oracle):
+    """With ions too: at 1e14 Hz nothing but the dielectrics responds (1e-5)."""
+    from oracle import ddimpedance as I
+
+    for dev in _ion_devices(2, 120):
+        Y = I.admittance_ions(dev, _ion_state(oracle, dev, 0.0), I.laps_of([1e14]))[0]
+        assert Y.imag / (2 * np.pi * 1e14) == pytest.approx(I.geometric_capacitance(dev), rel=1e-5)
+
+
+def test_ion_branch_is_one_implicit_euler_step_of_the_transient(oracle):
+    """The mid-band with ions: one implicit-Euler step of dt from the steady state with a
+    voltage step dV changes the transient oracle's current (its ions as unknowns, each species'
+    number kept by its slot; pinned to ZimT's runs with ions) by Y(lap = 1/dt) dV, central in
+    +-dV, to 1e-6, from 1 ns to 1 s."""
+    from oracle import ddimpedance as I
+
+    for dev in _ion_devices(2, 120):
+        f5 = _ion_state(oracle, dev, V0)
+        dV = 1e-4
+        for dt in (1e-9, 1e-6, 1e-3, 1.0):
+            Y = I.admittance_ions(dev, f5, np.array([1.0 / dt + 0j]), dt=dt)[0]
+            J = []
+            for s in (1, -1):
+                r = oracle.transient(dev, [0.0, dt], [V0, V0 + s * dV], [1.0, 1.0], tol=1e-14)
+                assert r["converged"].all()
+                J.append(r["J"][1])
+            assert abs((J[0] - J[1]) / (2 * dV) / Y.real - 1) < 1e-6, dt
+
+
+def test_ion_branch_is_the_transients_response_to_a_sine(oracle):
+    """The reference's tests/test_impedance.py:50-84 with ions: through implicit Euler at steps
+    of dt (40 a period, 24 periods: the ions' slow settling leaves 1.4e-6 after 12) the
+    transient oracle's response to a 0.1 mV sine at 0.6 V, fitted over the last 4 periods, is
+    the admittance with dt to 1e-7; the continuous response differs from it by implicit
+    Euler's error (> 1e-2)."""
+    from oracle import ddimpedance as I
+
+    dev = _ion_devices(1, 80)[0]
+    f, amp, per, periods, fitted = 1e2, 1e-4, 40, 24, 4
+    dt = 1.0 / (f * per)
+    k = np.arange(1, per * periods + 1)
+    t = np.concatenate([[0.0], k * dt])
+    V = np.concatenate([[V0], V0 + amp * np.sin(2 * np.pi * f * k * dt)])
+    r = oracle.transient(dev, t, V, np.ones(len(t)), tol=1e-13, max_iter=60)
+    assert r["converged"].all()
+    m = k > per * (periods - fitted)
+    ph = 2 * np.pi * f * k[m] * dt
+    X = np.stack([np.ones(m.sum()), np.sin(ph), np.cos(ph)], -1)
+    c = np.linalg.lstsq(X, r["J"][1:][m], rcond=None)[0]
+    Y_t = (c[1] + 1j * c[2]) / amp
+    f5 = _ion_state(oracle, dev, V0)
+    discrete = I.admittance_ions(dev, f5, I.laps_of([f], dt), dt=dt)[0]
+    continuous = I.admittance_ions(dev, f5, I.laps_of([f]))[0]
+    assert abs(discrete / Y_t - 1) < 1e-7
+    assert abs(continuous / Y_t - 1) > 1e-2
+
+
 # ---- the kernels' CPU replay ----------------------------------------------------------------
 def _cases():
     """(devices, voltage, Bernoulli): synthetic devices lit at three voltages and dark at one,
