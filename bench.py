@@ -77,7 +77,7 @@ def parse():
                    help="devices of the c3 frozen-after-pre-bias leg (timed beside the free "
                         "one); 0: skip")
     p.add_argument("--imp-n", type=int, default=2048,
-                   help="devices of the impedance workload (NP=300, no mobile ions); 0: skip")
+                   help="devices of the impedance workload (NP=300, mobile ions); 0: skip")
     p.add_argument("--imp-freqs", type=int, default=50,
                    help="frequencies of the impedance workload (log-spaced, 1e-2..1e7 Hz)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -258,12 +258,21 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: s
     }
 
 
-#: FP64 operations per (device, frequency, grid point) of ddh::impedance_kernel, counted from
-#: its code (complex multiply 6, complex add 2, complex divide 11, real x complex 2): the
-#: occupation's lag 12, the diagonal's time and trap terms 28, the elimination's A C' and A d'
-#: with A's five entries 74, the block inverse by its adjugate 213, d' 66, C' from C's five
-#: entries 42, the current's coefficients 19, the contraction's g and its sum 96.
+#: FP64 operations per (device, frequency, grid point) of ddh::impedance_kernel<false> (3 x 3,
+#: no mobile ions), counted from its code (complex multiply 6, complex add 2, complex divide 11,
+#: real x complex 2): the occupation's lag 12, the diagonal's time and trap terms 28, the
+#: elimination's A C' and A d' with A's five entries 74, the block inverse by its adjugate 213,
+#: d' 66, C' from C's five entries 42, the current's coefficients 19, the contraction's g and
+#: its sum 96.
 IMP_FLOPS_PER_POINT = 550
+#: ... of ddh::impedance_kernel<true> (5 x 5 with the ions as unknowns, three right-hand sides,
+#: three functionals), counted the same way: the occupation's lag 13, the diagonal's time and
+#: trap terms 28, the current's coefficients 25, the elimination's A C' (A's nine entries) 190
+#: and A d' 114, the functionals' g (3 x 5 x 5) 600, the partitioned inverse 934 (the 3 x 3
+#: adjugate 213, P^-1 Q and R P^-1 264, the 2 x 2 Schur complement and its inverse 145, the
+#: blocks of the inverse 312), d' = inv rhs 600, C' (column 0 dense, four diagonal entries)
+#: 140, the functionals' sums (3 x 3 x 5) 360.
+IMP_ION_FLOPS_PER_POINT = 3004
 
 
 def impedance_bench(packed, freqs, voltage=0.0):
@@ -273,11 +282,14 @@ def impedance_bench(packed, freqs, voltage=0.0):
     clock around the call."""
     import torch
 
-    from jaxdd_amd.batch import from_packed
+    from jaxdd_amd.batch import SC_FIELDS, from_packed
     from jaxdd_amd.impedance import solve_impedance
 
     batch = from_packed(packed, "cuda")
     n, NP = batch["n"], batch["NP"]
+    sc = packed["sc"]
+    ions = bool(np.any(sc[SC_FIELDS.index("a_total")] > 0) or np.any(sc[SC_FIELDS.index("c_total")] > 0))
+    flops = IMP_ION_FLOPS_PER_POINT if ions else IMP_FLOPS_PER_POINT
     solve_impedance(batch, freqs, voltage=voltage)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     torch.cuda.synchronize()
@@ -287,7 +299,7 @@ def impedance_bench(packed, freqs, voltage=0.0):
     t1 = time.perf_counter()
     steady_ms, imp_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
     systems = n * len(freqs)
-    achieved = systems * NP * IMP_FLOPS_PER_POINT / (imp_ms * 1e-3) / 1e12
+    achieved = systems * NP * flops / (imp_ms * 1e-3) / 1e12
     return {
         "value": n / (t1 - t0),
         "unit": f"impedance spectra/s (steady state + {len(freqs)} frequencies)",
@@ -297,8 +309,9 @@ def impedance_bench(packed, freqs, voltage=0.0):
         "converged_frac": float(r.converged.mean()), "finite": bool(np.isfinite(r.Y).all()),
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "kernel": "ddh::impedance_prep_kernel + ddh::impedance_kernel",
-                     "flops_per_point": IMP_FLOPS_PER_POINT},
+                     "kernel": f"ddh::impedance_prep_kernel<{str(ions).lower()}> + "
+                               f"ddh::impedance_kernel<{str(ions).lower()}>",
+                     "flops_per_point": flops},
     }
 
 
@@ -339,7 +352,8 @@ def main() -> int:
         grad_host_s = time.perf_counter() - t
     imp_packed = None
     if args.imp_n:
-        imp_packed = S.packed("c2", args.imp_n, workers=workers, NP=300)
+        # the c5 devices: the c3 stack with its mobile ions free, on 300 points
+        imp_packed = S.packed("c3", args.imp_n, workers=workers, NP=300)
     frozen_packed = None
     if args.frozen_n and args.config == "c3" and args.variant == "free":
         frozen_packed = S.packed("c3", args.frozen_n, variant="frozen", workers=workers)
@@ -490,9 +504,9 @@ def main() -> int:
     if imp_packed is not None:
         freqs = np.logspace(-2, 7, args.imp_freqs)
         out["impedance"] = impedance_bench(imp_packed, freqs)
-        out["impedance"]["workload"] = (f"c5 without ions: {args.imp_n} devices, NP=300, "
+        out["impedance"]["workload"] = (f"c5: {args.imp_n} devices with mobile ions, NP=300, "
                                         f"{args.imp_freqs} frequencies 1e-2..1e7 Hz at 0 V, "
-                                        "complex128, 3x3 blocks")
+                                        "complex128, 5x5 blocks with the slots' border")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -186,6 +186,25 @@ int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint, const
                   const double *laps, int32_t nf, double dt, int32_t bernoulli, double *Y,
                   int32_t *status, void *workspace, size_t workspace_bytes, void *hip_stream);
 
+/* Device bytes of ddh_impedance_ions' workspace for n devices of NP interior points. */
+size_t ddh_impedance_ions_workspace_bytes(int32_t n, int32_t NP);
+
+/* ddh_impedance with the mobile ions' logarithms as unknowns of their own: replaces the branch
+ * of src/jaxdd/impedance.py:242-262 (`_admittance` with `ions`: the 5x5 system of
+ * `solver._assemble_ions` at a step of dti = 1 from the steady state with that step's time terms
+ * taken back out, each species' number kept by its slot and its trapezoidal number as a bordered
+ * constraint, the ions' currents in the linearised transient current), which the reference takes
+ * for a whole batch when any device has mobile ions; a device without them gives the 3x3
+ * result. The ions' densities are the steady state's closed-form Boltzmann profiles of the
+ * potential (`solve_states`' anion and cation). ion_mu: [nseg][2][n] each segment's anion and
+ * cation mobility. Arguments otherwise as ddh_impedance; status [n]: 0, or 2 for a device
+ * outside this path (a species in several regions, or a region reaching a contact; Y NaN).
+ * workspace: ddh_impedance_ions_workspace_bytes(n, NP) bytes of device memory. */
+int ddh_impedance_ions(const ddh_batch *b, const double *ion_mu, const double *y,
+                       const double *Vint, const uint8_t *iface, const double *laps, int32_t nf,
+                       double dt, int32_t bernoulli, double *Y, int32_t *status, void *workspace,
+                       size_t workspace_bytes, void *hip_stream);
+
 /* Options of ddh_transient: src/jaxdd/transient.py:537-637 (`solve_transient`). */
 typedef struct {
     double tol;            /* Newton step tolerance of each time step (1e-10) */

@@ -12,7 +12,7 @@ LIB = HERE / "libddhip.so"
 DDH_FP64, DDH_MIXED = 0, 1
 PRECISIONS = {"fp64": DDH_FP64, "mixed": DDH_MIXED}
 #: The impedance operator's per-device table: fields per grid point, scalars per device.
-IMP_TABLE_FIELDS, IMP_SCALARS = 48, 8
+IMP_TABLE_FIELDS, IMP_ION_TABLE_FIELDS, IMP_SCALARS = 48, 64, 8
 
 
 class Batch(ctypes.Structure):
@@ -39,7 +39,8 @@ SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solv
            "ddh_solve_batch_states", "ddh_adjoint_workspace_bytes", "ddh_adjoint",
            "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions",
            "ddh_impedance_workspace_bytes", "ddh_impedance", "ddh_noalloc_scratch_bytes",
-           "ddh_solve_batch_noalloc", "ddh_transient_workspace_bytes", "ddh_transient")
+           "ddh_solve_batch_noalloc", "ddh_transient_workspace_bytes", "ddh_transient",
+           "ddh_impedance_ions_workspace_bytes", "ddh_impedance_ions")
 
 _lib = None
 
@@ -80,6 +81,11 @@ def load():
     lib.ddh_impedance.restype = ctypes.c_int
     lib.ddh_impedance.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, i32, ctypes.c_double,
                                   i32, vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_impedance_ions_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_impedance_ions_workspace_bytes.argtypes = [i32, i32]
+    lib.ddh_impedance_ions.restype = ctypes.c_int
+    lib.ddh_impedance_ions.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, vp, i32,
+                                       ctypes.c_double, i32, vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_noalloc_scratch_bytes.restype = ctypes.c_size_t
     lib.ddh_noalloc_scratch_bytes.argtypes = []
     lib.ddh_solve_batch_noalloc.restype = ctypes.c_int

@@ -70,6 +70,23 @@ enum ImpField {
     IM_N,
     IM_ROW = 48            // doubles per point: three 128-byte lines
 };
+// With mobile ions (impedance.py:242-262) each point's row of the table carries the ions'
+// fields after these (IMI_ROW doubles, four 128-byte lines): their rows of the 5 x 5 blocks (each
+// couples to the potential and to its own species only), Poisson's derivatives in their
+// logarithms, their time terms, the slots' constraint rows and the current's coefficients.
+enum ImpIonField {
+    IM_B03 = IM_N, IM_B04,            // Poisson's derivatives in ln a, ln c
+    IM_IA,                            // the anions' row: A30, A33, B30, B33 (no time term), C30, C33
+    IM_IC = IM_IA + 6,                // the cations' row likewise (column 4)
+    IM_MA = IM_IC + 6, IM_MC,         // their time terms per unit of lap; the slots' columns are -M
+    IM_WA, IM_WC,                     // the slots' constraint rows (d number/total / d ln)
+    IM_CA0, IM_CAL, IM_CC0, IM_CCL,   // the current's coefficients of dln a: CA0 + lap CAL; dln c
+    IM_LA, IM_LC,                     // ln a, ln c at the point (the assembly's own)
+    IMI_N,
+    IMI_ROW = 64
+};
+static_assert(IMI_N <= IMI_ROW, "the ion table's row holds its fields");
+
 // Per device [n][IS_N]: the drive (-dF/dV_int) of the first and the last row, the current's
 // direct dependence on V_int (IS_JV + lap IS_JD).
 enum ImpScalar { IS_D0 = 0, IS_DN = 3, IS_JV = 6, IS_JD, IS_N };
@@ -88,7 +105,10 @@ struct ImpArgs {
     double *tab;            // [n][NP][IM_ROW]
     double *dsc;            // [n][IS_N]
     double *Y;              // [n][nf][2] dJ/dV_int
-    int32_t *status;        // [n] 0, or 1: mobile ions (not handled; Y = NaN)
+    int32_t *status;        // [n] 0; 1: mobile ions on the 3 x 3 path; 2: ions outside the
+                            // 5 x 5 path (a region at a contact, several of a species); Y = NaN
+    const double *imu;      // [nseg][2][n] the ions' mobilities (the 5 x 5 path)
+    int row;                // doubles per point of the table: IM_ROW, or IMI_ROW with ions
 };
 
 #define ISC(f) a.sc[(size_t)(f) * a.n + d]
@@ -96,7 +116,8 @@ struct ImpArgs {
 #define ISID(j) ((int)a.sid[(size_t)(j) * a.n + d])
 #define ISEG(id, f) a.seg[((size_t)(id) * SEG_N + (f)) * a.n + d]
 #define IY(j, f) a.y[(((size_t)(j) - 1) * 3 + (f)) * a.n + d]
-#define ITAB(f, r) a.tab[((size_t)d * a.NP + (r)) * IM_ROW + (f)]
+#define ITAB(f, r) a.tab[((size_t)d * a.NP + (r)) * a.row + (f)]
+#define IMU(id, q) a.imu[((size_t)(id) * 2 + (q)) * a.n + d]
 
 // The table is assembled by one wavefront per device in three phases: (A) each point's blocks,
 // time and trap terms and its parts of the current's integrals, lanes over the points; (B) the
@@ -322,7 +343,7 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
         return;
     }
     const int NP = a.NP;
-    const double *tab = a.tab + (size_t)d * NP * IM_ROW;
+    const double *tab = a.tab + (size_t)d * NP * a.row;
     const double *ds = a.dsc + (size_t)d * IS_N;
     const cplx lap(a.laps[2 * f], a.laps[2 * f + 1]);
     const bool steps = a.dt > 0;
@@ -335,7 +356,7 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
     for (int q = 0; q < IM_DRN; ++q) cur[q] = tab[q];
     for (int r = 0; r < NP; ++r) {
         if (r + 1 < NP) {
-            const double *tn_ = tab + (size_t)(r + 1) * IM_ROW;
+            const double *tn_ = tab + (size_t)(r + 1) * a.row;
 #pragma unroll
             for (int q = 0; q < IM_DRN; ++q) nxt[q] = tn_[q];
         }
@@ -407,6 +428,399 @@ DDH_FN void impedance_lane(const ImpArgs &a, int d, int f) {
     Yo[1] = Y.im;
 }
 
+// ---- mobile ions: the ions' logarithms as unknowns of their own (impedance.py:242-262) ------
+//
+// The 5 x 5 system of `_assemble_ions` (solver.py:1364-1473) at a step of dti = 1 from the
+// steady state itself, with dti's time terms taken back out (`shift`, impedance.py:261): the
+// carriers' 3 x 3 rows as above, Poisson's row with the ions' densities as unknowns, and each
+// ion's row inside its region Scharfetter-Gummel's flux with implicit Euler's time term, scaled
+// by 1/(a_prev (fac + h_(i-1) V_T mu_i + h_i V_T mu_(i-1))). Each species' number is kept by a
+// slot: the column U = -M of its rows and its trapezoidal number as a constraint W, closed by
+// the 2 x 2 Schur complement of the bordered system (impedance.py:301-308).
+//
+// The ions' current (batch.py:421-485 `_ion_currents`, one region [f, l] per species): J_e =
+// q K + sum_(f<k<=e) sn q L w_k U_k on the region's intervals, K = (V_T sn (G_l - G_f) - sn D)/S
+// with G = dens e^(sn V/V_T), S = sum c_j, D = sum c_j iu_j, U = dti (dens - dens_prev). At the
+// steady state U = 0 and G is constant over the region, so its linearisation is
+// dK = (V_T sn (dG_l - dG_f) - sn dD - K dS)/S, dG = G (dln + sn dV/V_T), dS = sum c_j sn dV_j/V_T,
+// dD = sum_k (dx_k P_(k+1) + dx_(k-1) P_k)/2 dU_k (P_k = sum_(j>=k) c_j over the region),
+// dU = dti dens dln, and the device average takes K over the region's share of the intervals.
+
+// Each species' normalisation in closed form from the steady potential (`_ion_logs`
+// solver.py:1695-1712, trapezoidal number :1241-1252) and its region's first and last point.
+struct ImpIon {
+    double la[2];
+    int first[2], last[2];
+    bool ok;  // one region per species, reaching no contact
+};
+
+DDH_FN double imp_V(const ImpArgs &a, int d, const ImpDev &k, int j) {
+    return j == 0 ? k.V0 : (j == a.NP + 1 ? k.VN : IY(j, 0));
+}
+
+DDH_FN double imp_mask(const ImpArgs &a, int d, int sid, int q) {
+    return ISEG(sid, q == 0 ? SEG_AM : SEG_CM);
+}
+
+DDH_FN ImpIon imp_ion_setup(const ImpArgs &a, int d, const ImpDev &k) {
+    ImpIon s;
+    const double tot[2] = {ISC(SC_AT), ISC(SC_CT)};
+    double num[2] = {0.0, 0.0}, dprev[2] = {0.0, 0.0}, mprev[2] = {0.0, 0.0};
+    int count[2] = {0, 0};
+    s.first[0] = s.first[1] = s.last[0] = s.last[1] = -1;
+    double hprev = 0.0;
+    for (int j = 0; j <= a.NP + 1; ++j) {
+        const double V = imp_V(a, d, k, j);
+        const int sid = ISID(j);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const double m = imp_mask(a, d, sid, q);
+            const double dq = m * exp((q == 0 ? 1.0 : -1.0) * V * k.iVt);
+            if (j >= 1) num[q] += 0.5 * (dq + dprev[q]) * (m * mprev[q]) * hprev * k.L;
+            dprev[q] = dq;
+            mprev[q] = m;
+            if (m > 0) {
+                if (s.first[q] < 0) s.first[q] = j;
+                s.last[q] = j;
+                ++count[q];
+            }
+        }
+        hprev = IPT(j, PT_H);
+    }
+    s.ok = true;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (tot[q] > 0) {
+            s.la[q] = log(tot[q] / num[q]);
+            if (s.first[q] < 1 || s.last[q] > a.NP || count[q] != s.last[q] - s.first[q] + 1)
+                s.ok = false;
+        } else {
+            s.la[q] = 0.0;
+            s.first[q] = s.last[q] = -1;
+        }
+    }
+    return s;
+}
+
+// ln of species q's density at point j (0 outside its region)
+DDH_FN double imp_ion_log(const ImpArgs &a, int d, const ImpDev &k, const ImpIon &s, int j, int q) {
+    if (s.first[q] < 0 || j < s.first[q] || j > s.last[q]) return 0.0;
+    return s.la[q] + (q == 0 ? 1.0 : -1.0) * imp_V(a, d, k, j) * k.iVt;
+}
+
+// Row r's ion fields (after imp_assemble_row): each species' row, its time term and constraint
+// row, Poisson's derivatives in the ions' logarithms; the current's ion coefficients zeroed.
+DDH_FN void imp_ion_row(const ImpArgs &a, int d, const ImpDev &k, const ImpIon &s, int r) {
+    const int NP = a.NP, i = r + 1;
+    const int sl = ISID(i - 1), sm = ISID(i), sr = ISID(i + 1);
+    const double Vl = imp_V(a, d, k, i - 1), Vc = IY(i, 0), Vr = imp_V(a, d, k, i + 1);
+    const double hl = IPT(i - 1, PT_H), hi = IPT(i, PT_H);
+    const double fac = 0.5 * k.L2 * hi * hl * (hi + hl);
+    const double sP = ISEG(sm, SEG_SP);
+    const double tot[2] = {ISC(SC_AT), ISC(SC_CT)};
+    const double xl = (Vc - Vl) * k.iVt, xr = (Vr - Vc) * k.iVt;
+    double Blf, Blb, dBlf, dBlb, Brf, Brb, dBrf, dBrb;
+    bernoulli_pair(xl, a.bern, Blf, Blb, dBlf, dBlb);
+    bernoulli_pair(xr, a.bern, Brf, Brb, dBrf, dBrb);
+    double dens[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int base = q == 0 ? IM_IA : IM_IC;
+        const double mc = imp_mask(a, d, sm, q);
+        const double lnc = imp_ion_log(a, d, k, s, i, q);
+        dens[q] = mc > 0 ? exp(lnc) : 0.0;
+        if (!(mc > 0) || s.first[q] < 0) {  // held: the row is the unknown itself
+            ITAB(base + 0, r) = 0.0; ITAB(base + 1, r) = 0.0; ITAB(base + 2, r) = 0.0;
+            ITAB(base + 3, r) = 1.0; ITAB(base + 4, r) = 0.0; ITAB(base + 5, r) = 0.0;
+            ITAB(q == 0 ? IM_MA : IM_MC, r) = 0.0;
+            ITAB(q == 0 ? IM_WA : IM_WC, r) = 0.0;
+            dens[q] = 0.0;
+            continue;
+        }
+        const double ml = imp_mask(a, d, sl, q), mr = imp_mask(a, d, sr, q);
+        const double el = r == 0 ? 0.0 : ml * mc, er = r == NP - 1 ? 0.0 : mc * mr;
+        const double cl = k.Vt * IMU(sl, q), cr = k.Vt * IMU(sm, q);
+        const double nl = exp(imp_ion_log(a, d, k, s, i - 1, q)), nc = dens[q];
+        const double nr = exp(imp_ion_log(a, d, k, s, i + 1, q));
+        const double scale = 1.0 / (nc * (fac + hl * cr + hi * cl));
+        double dVl, dVc, dVr, dll, dlc, dlr;
+        if (q == 0) {  // anions as electrons: er h_(i-1) F_r - el h_i F_l
+            const double dFl = cl * (dBlf * nc + dBlb * nl), dFr = cr * (dBrf * nr + dBrb * nc);
+            dVl = el * hi * dFl * k.iVt;
+            dVc = -er * hl * dFr * k.iVt - el * hi * dFl * k.iVt;
+            dVr = er * hl * dFr * k.iVt;
+            dll = el * hi * cl * Blb * nl;
+            dlc = -er * hl * cr * Brb * nc - el * hi * cl * Blf * nc;
+            dlr = er * hl * cr * Brf * nr;
+        } else {  // cations as holes: el h_i F_l - er h_(i-1) F_r
+            const double dFl = cl * (dBlf * nl + dBlb * nc), dFr = cr * (dBrf * nc + dBrb * nr);
+            dVl = -el * hi * dFl * k.iVt;
+            dVc = el * hi * dFl * k.iVt + er * hl * dFr * k.iVt;
+            dVr = -er * hl * dFr * k.iVt;
+            dll = el * hi * cl * Blf * nl;
+            dlc = -el * hi * cl * Blb * nc - er * hl * cr * Brf * nc;
+            dlr = er * hl * cr * Brb * nr;
+        }
+        ITAB(base + 0, r) = dVl * scale; ITAB(base + 1, r) = dll * scale;
+        ITAB(base + 2, r) = dVc * scale; ITAB(base + 3, r) = dlc * scale;
+        ITAB(base + 4, r) = dVr * scale; ITAB(base + 5, r) = dlr * scale;
+        ITAB(q == 0 ? IM_MA : IM_MC, r) = -fac * nc * scale;
+        const double w = ml * mc * hl * k.L + mc * mr * hi * k.L;
+        ITAB(q == 0 ? IM_WA : IM_WC, r) = tot[q] > 0 ? 0.5 * w * mc * nc / tot[q] : 0.0;
+    }
+    ITAB(IM_B03, r) = -Q_E * dens[0] * sP;
+    ITAB(IM_B04, r) = Q_E * dens[1] * sP;
+    ITAB(IM_LA, r) = imp_ion_log(a, d, k, s, i, 0);
+    ITAB(IM_LC, r) = imp_ion_log(a, d, k, s, i, 1);
+    ITAB(IM_CA0, r) = 0.0; ITAB(IM_CAL, r) = 0.0;
+    ITAB(IM_CC0, r) = 0.0; ITAB(IM_CCL, r) = 0.0;
+}
+
+// The ions' current coefficients (after imp_coefficients; one lane, the regions in turn): each
+// point's dln coefficient (CA0/CC0 + lap CAL/CCL) and its dV coefficient added to IM_KV.
+DDH_FN void imp_ion_current(const ImpArgs &a, int d, const ImpDev &k, const ImpIon &s, double H) {
+    const double sign = -k.elec, qs = sign * Q_E;
+    for (int q = 0; q < 2; ++q) {
+        const int f = s.first[q], l = s.last[q];
+        if (f < 0) continue;
+        const double sn = q == 0 ? -1.0 : 1.0;
+        const int c0 = q == 0 ? IM_CA0 : IM_CC0, cl_ = q == 0 ? IM_CAL : IM_CCL;
+        // S, the region's share of the intervals, G at the ends
+        double S = 0.0, Hreg = 0.0;
+        for (int j = f; j <= l; ++j) {
+            const double dxl = j > f ? IPT(j, PT_X) - IPT(j - 1, PT_X) : 0.0;
+            const double dxr = j < l ? IPT(j + 1, PT_X) - IPT(j, PT_X) : 0.0;
+            const double mu = IMU(ISID(j), q);
+            S += 0.5 * (dxl + dxr) * exp(sn * IY(j, 0) * k.iVt) / (mu > 0 ? mu : 1.0);
+            if (j < l) Hreg += IPT(j, PT_H);
+        }
+        const double Gf = exp(imp_ion_log(a, d, k, s, f, q) + sn * IY(f, 0) * k.iVt);
+        const double Gl = exp(imp_ion_log(a, d, k, s, l, q) + sn * IY(l, 0) * k.iVt);
+        const double K = k.Vt * sn * (Gl - Gf) / S;
+        const double base = qs * (Hreg / H) / S;
+        // from the last point back: P_j = sum_(i>=j) c_i, the suffix of h over the region
+        double P = 0.0, hsuf = 0.0;
+        for (int j = l; j >= f; --j) {
+            const double xj = IPT(j, PT_X);
+            const double dxl = j > f ? xj - IPT(j - 1, PT_X) : 0.0;
+            const double dxr = j < l ? IPT(j + 1, PT_X) - xj : 0.0;
+            const double mu = IMU(ISID(j), q);
+            const double ev = exp(sn * IY(j, 0) * k.iVt);
+            const double c = 0.5 * (dxl + dxr) * ev / (mu > 0 ? mu : 1.0);
+            const double Pnext = P;  // sum over i > j
+            P += c;
+            const double hj = IPT(j, PT_H);
+            if (j < l) hsuf += hj;
+            const double ends = (j == l ? 1.0 : 0.0) - (j == f ? 1.0 : 0.0);
+            const double dens = exp(imp_ion_log(a, d, k, s, j, q));
+            const double G = dens * ev;
+            const double dDdU = 0.5 * dxr * Pnext + 0.5 * dxl * P;
+            double lapc = -base * sn * dDdU * dens;
+            if (j > f && j < l) {
+                const double w = 0.5 * (IPT(j - 1, PT_H) + hj);
+                const double T = (hsuf - (a.iface[(size_t)j * a.n + d] ? hj : 0.0)) / H;
+                lapc += qs * sn * k.L * w * T * dens;
+            }
+            const int r = j - 1;
+            ITAB(c0, r) = base * k.Vt * sn * G * ends;
+            ITAB(cl_, r) = lapc;
+            ITAB(IM_KV, r) += base * (G * ends - K * c * sn * k.iVt);
+        }
+    }
+}
+
+// 5 x 5 inverses by their partition (solver.py:1505-1517 `_inverse5`): the carriers' 3 x 3
+// block by its adjugate, the ions' 2 x 2 Schur complement by its own.
+DDH_FN void inverse5c(const cplx *m, cplx *out) {
+    cplx P[9], Pi[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[3 * i + j] = m[5 * i + j];
+    inverse3<cplx>(P, Pi);
+    cplx PiQ[6], RPi[6];  // 3 x 2, 2 x 3
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            PiQ[2 * i + j] = Pi[3 * i] * m[3 + j] + Pi[3 * i + 1] * m[8 + j] + Pi[3 * i + 2] * m[13 + j];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            RPi[3 * i + j] = m[5 * (3 + i)] * Pi[j] + m[5 * (3 + i) + 1] * Pi[3 + j] +
+                             m[5 * (3 + i) + 2] * Pi[6 + j];
+    cplx Sc[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            Sc[2 * i + j] = m[5 * (3 + i) + 3 + j] -
+                            (m[5 * (3 + i)] * PiQ[j] + m[5 * (3 + i) + 1] * PiQ[2 + j] +
+                             m[5 * (3 + i) + 2] * PiQ[4 + j]);
+    const cplx rd = cplx(1.0) / (Sc[0] * Sc[3] - Sc[1] * Sc[2]);
+    const cplx Si[4] = {Sc[3] * rd, -Sc[1] * rd, -Sc[2] * rd, Sc[0] * rd};
+    cplx SiR[6];  // Si RPi, 2 x 3
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) SiR[3 * i + j] = Si[2 * i] * RPi[j] + Si[2 * i + 1] * RPi[3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            out[5 * i + j] = Pi[3 * i + j] + PiQ[2 * i] * SiR[j] + PiQ[2 * i + 1] * SiR[3 + j];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            out[5 * i + 3 + j] = -(PiQ[2 * i] * Si[j] + PiQ[2 * i + 1] * Si[2 + j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[5 * (3 + i) + j] = -SiR[3 * i + j];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) out[5 * (3 + i) + 3 + j] = Si[2 * i + j];
+    }
+}
+
+// One (device, frequency) with the ions as unknowns: the complex 5 x 5 system with three
+// right-hand sides (the drive and the two slots' columns) eliminated row by row, three
+// functionals (the current, the two constraints) contracted on the way (g_i = c_i -
+// g_(i-1) C'_(i-1)), the border closed at the end by the slots' 2 x 2 Schur complement.
+DDH_FN void impedance_ion_lane(const ImpArgs &a, int d, int f) {
+    double *Yo = a.Y + ((size_t)d * a.nf + f) * 2;
+    if (a.status[d] != 0) {
+        Yo[0] = Yo[1] = NAN;
+        return;
+    }
+    const int NP = a.NP;
+    const double *tab = a.tab + (size_t)d * NP * a.row;
+    const double *ds = a.dsc + (size_t)d * IS_N;
+    const cplx lap(a.laps[2 * f], a.laps[2 * f + 1]);
+    const bool steps = a.dt > 0;
+    const cplx ld = lap * a.dt;
+    const double held[2] = {a.sc[(size_t)SC_AT * a.n + d] > 0 ? 0.0 : 1.0,
+                            a.sc[(size_t)SC_CT * a.n + d] > 0 ? 0.0 : 1.0};
+    cplx Cp[25], dp[15], g[15], acc[9];  // dp [5][3]; g [3][5]; acc [3 functionals][3 columns]
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = cplx(0.0);
+    for (int r = 0; r < NP; ++r) {
+        const double *t = tab + (size_t)r * a.row;
+        // the blocks' sparse A and C: column 0 of every row, the diagonal of rows 1-4
+        const double A0[5] = {t[IM_A], t[IM_A + 1], t[IM_A + 3], t[IM_IA + 0], t[IM_IC + 0]};
+        const double Ad[5] = {0.0, t[IM_A + 2], t[IM_A + 4], t[IM_IA + 1], t[IM_IC + 1]};
+        const double C0[5] = {t[IM_C], t[IM_C + 1], t[IM_C + 3], t[IM_IA + 4], t[IM_IC + 4]};
+        const double Cd[5] = {0.0, t[IM_C + 2], t[IM_C + 4], t[IM_IA + 5], t[IM_IC + 5]};
+        cplx h;
+        if (steps) {
+            const double E = t[IM_E], m = t[IM_M];
+            h = -(m * ld) / (cplx(1.0 - E) + E * ld);
+        } else {
+            h = -lap / (lap + cplx(t[IM_RATE]));
+        }
+        const double un = t[IM_UN], up = t[IM_UP], tn = t[IM_TN], tp = t[IM_TP];
+        cplx B[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) B[q] = cplx(0.0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) B[5 * i + j] = cplx(t[IM_B + 3 * i + j]);
+        B[6] = B[6] + lap * t[IM_MN] + h * (tn * un);
+        B[7] = B[7] + h * (tn * up);
+        B[11] = B[11] + h * (tp * un);
+        B[12] = B[12] + lap * t[IM_MP] + h * (tp * up);
+        B[3] = cplx(t[IM_B03]);
+        B[4] = cplx(t[IM_B04]);
+        B[15] = cplx(t[IM_IA + 2]);
+        B[18] = cplx(t[IM_IA + 3]) + lap * t[IM_MA];
+        B[20] = cplx(t[IM_IC + 2]);
+        B[24] = cplx(t[IM_IC + 3]) + lap * t[IM_MC];
+        cplx rhs[15];
+#pragma unroll
+        for (int q = 0; q < 15; ++q) rhs[q] = cplx(0.0);
+        if (r == 0)
+            for (int i = 0; i < 3; ++i) rhs[3 * i] = cplx(ds[IS_D0 + i]);
+        if (r == NP - 1)
+            for (int i = 0; i < 3; ++i) rhs[3 * i] = rhs[3 * i] + cplx(ds[IS_DN + i]);
+        rhs[3 * 3 + 1] = cplx(-t[IM_MA]);  // the slots' columns U = -M
+        rhs[3 * 4 + 2] = cplx(-t[IM_MC]);
+        // the current's and the constraints' coefficients of this point's dy
+        const double gt = t[IM_GT];
+        const cplx cc[5] = {cplx(t[IM_KV]) + lap * t[IM_KD],
+                            cplx(t[IM_G0N]) + lap * t[IM_GLN] + h * (un * gt),
+                            cplx(t[IM_G0P]) + lap * t[IM_GLP] + h * (up * gt),
+                            cplx(t[IM_CA0]) + lap * t[IM_CAL], cplx(t[IM_CC0]) + lap * t[IM_CCL]};
+        if (r > 0) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    B[5 * i + j] = B[5 * i + j] - (A0[i] * Cp[j] + (i > 0 ? Ad[i] * Cp[5 * i + j] : cplx(0.0)));
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    rhs[3 * i + c] = rhs[3 * i + c] - (A0[i] * dp[c] + (i > 0 ? Ad[i] * dp[3 * i + c] : cplx(0.0)));
+            }
+            cplx gn[15];
+#pragma unroll
+            for (int fn = 0; fn < 3; ++fn)
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    cplx v = fn == 0 ? cc[j] : cplx((fn == 1 && j == 3) ? t[IM_WA] : ((fn == 2 && j == 4) ? t[IM_WC] : 0.0));
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) v = v - g[5 * fn + i] * Cp[5 * i + j];
+                    gn[5 * fn + j] = v;
+                }
+#pragma unroll
+            for (int q = 0; q < 15; ++q) g[q] = gn[q];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                g[j] = cc[j];
+                g[5 + j] = cplx(j == 3 ? t[IM_WA] : 0.0);
+                g[10 + j] = cplx(j == 4 ? t[IM_WC] : 0.0);
+            }
+        }
+        cplx inv[25];
+        inverse5c(B, inv);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cplx v(0.0);
+#pragma unroll
+                for (int j = 0; j < 5; ++j) v = v + inv[5 * i + j] * rhs[3 * j + c];
+                dp[3 * i + c] = v;
+            }
+            cplx v(0.0);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) v = v + inv[5 * i + j] * C0[j];
+            Cp[5 * i] = v;
+#pragma unroll
+            for (int j = 1; j < 5; ++j) Cp[5 * i + j] = inv[5 * i + j] * Cd[j];
+        }
+#pragma unroll
+        for (int fn = 0; fn < 3; ++fn)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cplx v = acc[3 * fn + c];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) v = v + g[5 * fn + i] * dp[3 * i + c];
+                acc[3 * fn + c] = v;
+            }
+    }
+    // the border: S z = -W X_F with S = diag(held) - W X_U, then dy = X_F - X_U z
+    const cplx s00 = cplx(held[0]) - acc[3 + 1], s01 = -acc[3 + 2];
+    const cplx s10 = -acc[6 + 1], s11 = cplx(held[1]) - acc[6 + 2];
+    const cplx b0 = -acc[3], b1 = -acc[6];
+    const cplx det = s00 * s11 - s01 * s10;
+    const cplx z0 = (b0 * s11 - s01 * b1) / det, z1 = (s00 * b1 - s10 * b0) / det;
+    const cplx Y = cplx(ds[IS_JV]) + lap * ds[IS_JD] + acc[0] - (acc[1] * z0 + acc[2] * z1);
+    Yo[0] = Y.re;
+    Yo[1] = Y.im;
+}
+
+#undef IMU
 #undef ISC
 #undef IPT
 #undef ISID

@@ -194,17 +194,31 @@ __device__ inline double wave_exclusive_scan(double x, int lane) {
 
 // One device per workgroup, each XCD a contiguous run of devices (xcd_map, ddhip_layout.h), so
 // that a line of the lane-innermost arrays it fetches serves the next devices it runs.
+// IONS: the 5 x 5 path (the ions' logarithms as unknowns; their rows, the slots' constraint rows
+// and the ions' current coefficients after the carriers'), else the 3 x 3 path, which marks a
+// device with mobile ions (status 1). A device outside the 5 x 5 path gets status 2.
+template <bool IONS>
 __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
     const int d = xcd_device(xcd_map((int)blockIdx.x, (int)gridDim.x, a.n), 0, a.n);
     if (d < 0) return;
     const int lane = threadIdx.x, NP = a.NP;
-    if (imp_mobile_ions(a, d)) {
+    const ImpDev k = imp_device(a, d);
+    ImpIon ions;
+    if (IONS) {
+        ions = imp_ion_setup(a, d, k);
+        if (!ions.ok) {
+            if (lane == 0) a.status[d] = 2;
+            return;
+        }
+    } else if (imp_mobile_ions(a, d)) {
         if (lane == 0) a.status[d] = 1;
         return;
     }
     if (lane == 0) a.status[d] = 0;
-    const ImpDev k = imp_device(a, d);
-    for (int r = lane; r < NP; r += 64) imp_assemble_row(a, d, k, r);
+    for (int r = lane; r < NP; r += 64) {
+        imp_assemble_row(a, d, k, r);
+        if (IONS) imp_ion_row(a, d, k, ions, r);
+    }
     __syncthreads();
     const int seg = (NP + 63) / 64;
     const int r0 = lane * seg < NP ? lane * seg : NP;
@@ -222,11 +236,18 @@ __global__ void __launch_bounds__(64) impedance_prep_kernel(ImpArgs a) {
     const ImpTot t = imp_totals(a, d, k, Sn, Sp, Dn, Dp, H, lane == 0);
     __syncthreads();
     for (int r = lane; r < NP; r += 64) imp_coefficients(a, d, k, t, r);
+    if (IONS) {
+        __syncthreads();
+        if (lane == 0) imp_ion_current(a, d, k, ions, t.H);
+    }
 }
 
+template <bool IONS>
 __global__ void __launch_bounds__(256) impedance_kernel(ImpArgs a) {
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-    if (gid < (long)a.n * a.nf) impedance_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
+    if (gid >= (long)a.n * a.nf) return;
+    if (IONS) impedance_ion_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
+    else impedance_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
 }
 
 // The transient (ddhip_transient.h): one lane per device through all of its time steps, one
@@ -714,44 +735,88 @@ extern "C" int ddh_assemble_ions(const ddh_batch *b, const double *y, const doub
     return DDH_OK;
 }
 
-extern "C" size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP) {
+static size_t imp_bytes(int32_t n, int32_t NP, int row) {
     if (n <= 0 || NP < 1) return 0;
-    return ((size_t)n * NP * ddh::IM_ROW + (size_t)n * ddh::IS_N) * sizeof(double);
+    return ((size_t)n * NP * row + (size_t)n * ddh::IS_N) * sizeof(double);
+}
+
+extern "C" size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP) {
+    return imp_bytes(n, NP, ddh::IM_ROW);
+}
+
+extern "C" size_t ddh_impedance_ions_workspace_bytes(int32_t n, int32_t NP) {
+    return imp_bytes(n, NP, ddh::IMI_ROW);
+}
+
+static int impedance_launch(const char *name, bool ions, const ddh_batch *b, const double *ion_mu,
+                            const double *y, const double *Vint, const uint8_t *iface,
+                            const double *laps, int32_t nf, double dt, int32_t bernoulli,
+                            double *Y, int32_t *status, void *workspace, size_t workspace_bytes,
+                            void *stream) {
+    char msg[160];
+    if (!b || !y || !Vint || !iface || !laps || !Y || !status || !workspace || (ions && !ion_mu)) {
+        snprintf(msg, sizeof msg, "%s: NULL argument", name);
+        return fail(DDH_E_ARG, msg);
+    }
+    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS) {
+        snprintf(msg, sizeof msg, "%s: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14", name);
+        return fail(DDH_E_SHAPE, msg);
+    }
+    if (nf < 1 || (long)b->n * nf > (1L << 30)) {
+        snprintf(msg, sizeof msg, "%s: nf >= 1 and n * nf <= 2^30", name);
+        return fail(DDH_E_SHAPE, msg);
+    }
+    if (bernoulli != 0 && bernoulli != 1) {
+        snprintf(msg, sizeof msg, "%s: bernoulli must be 0 (exact) or 1 (simss)", name);
+        return fail(DDH_E_ARG, msg);
+    }
+    if (!(dt >= 0.0) || !isfinite(dt)) {
+        snprintf(msg, sizeof msg, "%s: dt must be finite and >= 0 (0: continuous)", name);
+        return fail(DDH_E_ARG, msg);
+    }
+    const int row = ions ? (int)ddh::IMI_ROW : (int)ddh::IM_ROW;
+    if (workspace_bytes < imp_bytes(b->n, b->NP, row)) {
+        snprintf(msg, sizeof msg, "%s: workspace smaller than its workspace_bytes", name);
+        return fail(DDH_E_WORKSPACE, msg);
+    }
+    ddh::ImpArgs a = {};
+    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli; a.nf = nf;
+    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
+    a.y = y; a.Vint = Vint; a.iface = iface; a.laps = laps; a.dt = dt;
+    a.tab = (double *)workspace;
+    a.dsc = a.tab + (size_t)b->n * b->NP * row;
+    a.Y = Y; a.status = status;
+    a.imu = ion_mu; a.row = row;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned prep_groups = (unsigned)(((b->n + ddh::XCDS - 1) / ddh::XCDS) * ddh::XCDS);
+    if (ions) hipLaunchKernelGGL(ddh::impedance_prep_kernel<true>, dim3(prep_groups), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(ddh::impedance_prep_kernel<false>, dim3(prep_groups), dim3(64), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "impedance_prep_kernel launch");
+    const long lanes = (long)b->n * nf;
+    const dim3 grid((unsigned)((lanes + 255) / 256));
+    if (ions) hipLaunchKernelGGL(ddh::impedance_kernel<true>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(ddh::impedance_kernel<false>, grid, dim3(256), 0, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "impedance_kernel launch");
+    return DDH_OK;
 }
 
 extern "C" int ddh_impedance(const ddh_batch *b, const double *y, const double *Vint,
                              const uint8_t *iface, const double *laps, int32_t nf, double dt,
                              int32_t bernoulli, double *Y, int32_t *status, void *workspace,
                              size_t workspace_bytes, void *stream) {
-    if (!b || !y || !Vint || !iface || !laps || !Y || !status || !workspace)
-        return fail(DDH_E_ARG, "ddh_impedance: NULL argument");
-    if (b->n <= 0 || b->NP < 2 || b->NP > 65536 || b->nseg < 1 || b->nseg > DDH_MAX_SEGMENTS)
-        return fail(DDH_E_SHAPE, "ddh_impedance: n > 0, 2 <= NP <= 65536, 1 <= nseg <= 14");
-    if (nf < 1 || (long)b->n * nf > (1L << 30))
-        return fail(DDH_E_SHAPE, "ddh_impedance: nf >= 1 and n * nf <= 2^30");
-    if (bernoulli != 0 && bernoulli != 1)
-        return fail(DDH_E_ARG, "ddh_impedance: bernoulli must be 0 (exact) or 1 (simss)");
-    if (!(dt >= 0.0) || !isfinite(dt))
-        return fail(DDH_E_ARG, "ddh_impedance: dt must be finite and >= 0 (0: continuous)");
-    if (workspace_bytes < ddh_impedance_workspace_bytes(b->n, b->NP))
-        return fail(DDH_E_WORKSPACE, "ddh_impedance: workspace smaller than ddh_impedance_workspace_bytes");
-    ddh::ImpArgs a = {};
-    a.n = b->n; a.NP = b->NP; a.nseg = b->nseg; a.bern = bernoulli; a.nf = nf;
-    a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
-    a.y = y; a.Vint = Vint; a.iface = iface; a.laps = laps; a.dt = dt;
-    a.tab = (double *)workspace;
-    a.dsc = a.tab + (size_t)b->n * b->NP * ddh::IM_ROW;
-    a.Y = Y; a.status = status;
-    hipStream_t s = (hipStream_t)stream;
-    const unsigned prep_groups = (unsigned)(((b->n + ddh::XCDS - 1) / ddh::XCDS) * ddh::XCDS);
-    hipLaunchKernelGGL(ddh::impedance_prep_kernel, dim3(prep_groups), dim3(64), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "impedance_prep_kernel launch");
-    const long lanes = (long)b->n * nf;
-    hipLaunchKernelGGL(ddh::impedance_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "impedance_kernel launch");
-    return DDH_OK;
+    return impedance_launch("ddh_impedance", false, b, nullptr, y, Vint, iface, laps, nf, dt,
+                            bernoulli, Y, status, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ddh_impedance_ions(const ddh_batch *b, const double *ion_mu, const double *y,
+                                  const double *Vint, const uint8_t *iface, const double *laps,
+                                  int32_t nf, double dt, int32_t bernoulli, double *Y,
+                                  int32_t *status, void *workspace, size_t workspace_bytes,
+                                  void *stream) {
+    return impedance_launch("ddh_impedance_ions", true, b, ion_mu, y, Vint, iface, laps, nf, dt,
+                            bernoulli, Y, status, workspace, workspace_bytes, stream);
 }
 
 extern "C" int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, const void *A,

@@ -1,6 +1,5 @@
 """Impedance in the frequency domain from the steady state's Jacobian (reference:
-src/jaxdd/impedance.py:47-61 `ImpedanceResult`, :371-409 `solve_impedance`), for devices without
-mobile ions.
+src/jaxdd/impedance.py:47-61 `ImpedanceResult`, :371-409 `solve_impedance`).
 
 At the steady state at the internal voltage `voltage` (the solve kernel, keeping its final
 state) a small voltage `dV e^(iwt)` drives `(J + iw M) dy = -(dF/dV) dV`: `J` the steady
@@ -8,7 +7,10 @@ residual's Jacobian, `M` implicit Euler's time terms, the bulk traps' occupation
 their own rates. `ddh_impedance` builds a per-device table of the frequency-independent terms
 (one wavefront per device), then solves every (device, frequency) system on a lane of its own,
 contracting the linearised transient current on the way; `Y = dJ/dV` in SIMsalabim's sign and
-the external `Z = 1/(Y + G_shunt) + R_series`.
+the external `Z = 1/(Y + G_shunt) + R_series`. When any device of the batch has mobile ions,
+the whole batch goes through `ddh_impedance_ions`, as the reference's: the ions' logarithms are
+unknowns of their own (5 x 5 blocks), each species' number kept by a slot, and the ions'
+currents join the linearised current.
 """
 
 from __future__ import annotations
@@ -64,32 +66,38 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
     """The impedance of a batch (`jaxdd_amd.batch.prepare`) at the internal voltage ``voltage``
     (V) at each of ``frequencies`` (Hz), from the steady state there (solved with ``tol`` and
     ``max_iter``, as `solve_batch` at that one voltage) by one linear solve per frequency. The
-    bulk traps' occupations follow at their own rates. ``dt`` (s): instead of the continuous
-    response, what implicit Euler's steps of ``dt`` see (the time terms at
+    bulk traps' occupations follow at their own rates; mobile ions move with the voltage, their
+    steady state the free ions' (Boltzmann's in each region). ``dt`` (s): instead of the
+    continuous response, what implicit Euler's steps of ``dt`` see (the time terms at
     `(1 - e^(-iw dt))/dt`, the traps' occupations over each step). ``lanes``: the steady solve's
-    (`solve_batch`'s; default four, the grid in quarters, for NP >= 16: these devices have no
-    mobile ions, and the batches impedance is taken of are small). ``events``: four
-    torch.cuda.Event recorded on the launch stream around the steady solve and around the
-    impedance kernels. ``return_states``: also the steady
-    states (n, NP, 3) the admittance was taken at.
+    (`solve_batch`'s; default four, the grid in quarters, for NP >= 16 and no mobile ions, else
+    two). ``events``: four torch.cuda.Event recorded on the launch stream around the steady
+    solve and around the impedance kernels. ``return_states``: also the steady states
+    (n, NP, 3) the admittance was taken at.
 
     Raises `ValueError` for an unknown ``ion_flux`` or a ``dt`` that is not positive, and
-    `NotImplementedError` for devices with mobile ions (the reference solves those with the
-    ions as unknowns of their own)."""
+    `NotImplementedError` for ions frozen at a pre-bias (as the reference), the modified-drift
+    flux (ions bounded by their sites are outside this build's device model, and without sites
+    both fluxes are Scharfetter-Gummel's), and devices whose ions reach a contact or fill
+    several regions of one species."""
     if ion_flux not in ION_FLUXES:
         raise ValueError(f"ion_flux {ion_flux!r}: one of {ION_FLUXES}")
     if bernoulli_name not in ("exact", "simss"):
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
     if dt is not None and not (np.isfinite(dt) and dt > 0):
         raise ValueError(f"dt {dt!r}: a positive step (s), or None for the continuous response")
+    sc = batch["_sc"]
+    ions = bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any())
+    if ions and bool(((batch["pre_cond"] > 0) & (batch["fix_ions"] > 0)).any()):
+        raise NotImplementedError("impedance with ions frozen at a pre-bias")
+    if ions and ion_flux != "electrochemical":
+        raise NotImplementedError("the modified drift differs from the electrochemical flux "
+                                  "only for ions bounded by their sites, outside this path")
     if lanes is None:
-        lanes = 4 if batch["NP"] >= 16 else 2
+        lanes = 4 if batch["NP"] >= 16 and not ions else 2
     if lanes not in (1, 2, 4):
         raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
     frequencies = np.atleast_1d(np.asarray(frequencies, dtype=float))
-    sc = batch["_sc"]
-    if bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any()):
-        raise NotImplementedError("impedance of devices with mobile ions is outside this path")
     if lanes == 4:
         from .batch import check_four_lanes
 
@@ -120,7 +128,8 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
     Vint = torch.full((n,), float(voltage), dtype=torch.float64, device=dev)
     Y = torch.empty((n, nf, 2), dtype=torch.float64, device=dev)
     istatus = torch.zeros(n, dtype=torch.int32, device=dev)
-    ibytes = lib.ddh_impedance_workspace_bytes(n, NP)
+    ibytes = (lib.ddh_impedance_ions_workspace_bytes if ions
+              else lib.ddh_impedance_workspace_bytes)(n, NP)
     iws = torch.empty(ibytes, dtype=torch.uint8, device=dev)
     if events is not None:
         events[0].record()
@@ -131,16 +140,24 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
     if events is not None:
         events[1].record()
         events[2].record()
-    _lib.check(lib.ddh_impedance(ctypes.byref(b), states.data_ptr(), Vint.data_ptr(),
-                                 batch["_iface"].data_ptr(), lap_d.data_ptr(), nf,
-                                 0.0 if dt is None else float(dt), bern, Y.data_ptr(),
-                                 istatus.data_ptr(), iws.data_ptr(), ibytes, stream),
-               "ddh_impedance")
+    dt_ = 0.0 if dt is None else float(dt)
+    if ions:
+        _lib.check(lib.ddh_impedance_ions(ctypes.byref(b), batch["_imu"].data_ptr(),
+                                          states.data_ptr(), Vint.data_ptr(),
+                                          batch["_iface"].data_ptr(), lap_d.data_ptr(), nf, dt_,
+                                          bern, Y.data_ptr(), istatus.data_ptr(), iws.data_ptr(),
+                                          ibytes, stream), "ddh_impedance_ions")
+    else:
+        _lib.check(lib.ddh_impedance(ctypes.byref(b), states.data_ptr(), Vint.data_ptr(),
+                                     batch["_iface"].data_ptr(), lap_d.data_ptr(), nf, dt_, bern,
+                                     Y.data_ptr(), istatus.data_ptr(), iws.data_ptr(), ibytes,
+                                     stream), "ddh_impedance")
     if events is not None:
         events[3].record()
     torch.cuda.current_stream(dev).synchronize()
     if int(istatus.max()) != 0:
-        raise NotImplementedError("impedance of devices with mobile ions is outside this path")
+        raise NotImplementedError("a device whose ions reach a contact or fill several regions "
+                                  "of one species is outside this path")
     Yc = Y.cpu().numpy()
     Yc = Yc[..., 0] + 1j * Yc[..., 1]
     G_shunt = batch["G_shunt"].cpu().numpy()

@@ -305,20 +305,31 @@ extern "C" int emu_impedance(int n, int NP, int nseg, const double *pt, const ui
                              const double *seg, const double *sc, const double *y,
                              const double *Vint, const uint8_t *iface, const double *laps, int nf,
                              double dt, int bern, double *tab, double *dsc, double *Y,
-                             int32_t *status) {
+                             int32_t *status, int ions, const double *imu) {
     ImpArgs a = {};
     a.n = n; a.NP = NP; a.nseg = nseg; a.bern = bern; a.nf = nf;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.y = y; a.Vint = Vint; a.iface = iface;
     a.laps = laps; a.dt = dt; a.tab = tab; a.dsc = dsc; a.Y = Y; a.status = status;
+    a.imu = imu; a.row = ions ? (int)IMI_ROW : (int)IM_ROW;
     // the table's wavefront per device: its phases in turn, the scans across its lanes serial
     for (int d = 0; d < n; ++d) {
-        if (imp_mobile_ions(a, d)) {
+        const ImpDev k = imp_device(a, d);
+        ImpIon is;
+        if (ions) {
+            is = imp_ion_setup(a, d, k);
+            if (!is.ok) {
+                status[d] = 2;
+                continue;
+            }
+        } else if (imp_mobile_ions(a, d)) {
             status[d] = 1;
             continue;
         }
         status[d] = 0;
-        const ImpDev k = imp_device(a, d);
-        for (int r = 0; r < NP; ++r) imp_assemble_row(a, d, k, r);
+        for (int r = 0; r < NP; ++r) {
+            imp_assemble_row(a, d, k, r);
+            if (ions) imp_ion_row(a, d, k, is, r);
+        }
         const int seg = (NP + 63) / 64;
         double tot[64][4], run[4] = {0.0, k.cn0, k.cp0, k.h0}, Dn = 0.0, Dp = 0.0;
         for (int lane = 0; lane < 64; ++lane) {
@@ -337,9 +348,13 @@ extern "C" int emu_impedance(int n, int NP, int nseg, const double *pt, const ui
         }
         const ImpTot t = imp_totals(a, d, k, run[1], run[2], Dn, Dp, run[3], true);
         for (int r = 0; r < NP; ++r) imp_coefficients(a, d, k, t, r);
+        if (ions) imp_ion_current(a, d, k, is, t.H);
     }
     for (int d = 0; d < n; ++d)
-        for (int f = 0; f < nf; ++f) impedance_lane(a, d, f);
+        for (int f = 0; f < nf; ++f) {
+            if (ions) impedance_ion_lane(a, d, f);
+            else impedance_lane(a, d, f);
+        }
     return 0;
 }
 

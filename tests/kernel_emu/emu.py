@@ -38,7 +38,7 @@ def lib():
         _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
                                       vp, vp]
         _lib.emu_impedance.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, i,
-                                       ctypes.c_double, i, vp, vp, vp, vp]
+                                       ctypes.c_double, i, vp, vp, vp, vp, i, vp]
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
         _lib.emu_xcd_hits.argtypes = [i, i, vp]
@@ -121,13 +121,17 @@ def assemble_packed(packed, y, Vint, logs=None, bernoulli="exact"):
 IMP_FIELDS = {name: k for k, name in enumerate(
     ["A0", "A3", "A4", "A6", "A8"] + [f"B{q}" for q in range(9)] + ["C0", "C3", "C4", "C6", "C8"]
     + ["MN", "MP", "RATE", "E", "M", "UN", "UP", "TN", "TP", "KV", "KD", "G0N", "G0P", "GLN",
-       "GLP", "GT", "DRN", "DRP", "NN", "PP", "TMP"])}
+       "GLP", "GT", "DRN", "DRP", "NN", "PP", "TMP"]
+    # the 5 x 5 path's fields after the 40 above (the table's 64-double rows)
+    + ["B03", "B04"] + [f"IA{q}" for q in range(6)] + [f"IC{q}" for q in range(6)]
+    + ["MA", "MC", "WA", "WC", "CA0", "CAL", "CC0", "CCL", "LA", "LC"])}
 
 
-def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=False):
+def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=False,
+                     ions=False):
     """The impedance operator's CPU replay: y (n, NP, 3) steady states, Vint (n,), laps
     (complex, nf) -> Y (n, nf) complex, status (n,); ``table``: also the per-device tables
-    (n, NP, fields) and scalars (n, 8)."""
+    (n, NP, fields) and scalars (n, 8); ``ions``: the 5 x 5 path (the ions as unknowns)."""
     n = packed["sc"].shape[1]
     NP = packed["pt"].shape[0] - 2
     arrs = [np.ascontiguousarray(packed[k]) for k in ("pt", "sid", "seg", "sc")]
@@ -136,13 +140,16 @@ def impedance_packed(packed, y, Vint, laps, dt=None, bernoulli="exact", table=Fa
     iface = np.ascontiguousarray(packed["iface"], dtype=np.uint8)
     lp = np.ascontiguousarray(np.stack([np.real(laps), np.imag(laps)], -1), dtype=float)
     nf = lp.shape[0]
-    from jaxdd_amd._lib import IMP_TABLE_FIELDS, IMP_SCALARS
+    imu = np.ascontiguousarray(packed["imu"], dtype=float)
+    from jaxdd_amd._lib import IMP_TABLE_FIELDS, IMP_ION_TABLE_FIELDS, IMP_SCALARS
 
-    tab = np.zeros((n, NP, IMP_TABLE_FIELDS)); dsc = np.zeros((n, IMP_SCALARS))
+    fields = IMP_ION_TABLE_FIELDS if ions else IMP_TABLE_FIELDS
+    tab = np.zeros((n, NP, fields)); dsc = np.zeros((n, IMP_SCALARS))
     Y = np.zeros((n, nf, 2)); status = np.zeros(n, dtype=np.int32)
     lib().emu_impedance(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(yl), _p(Vd),
                         _p(iface), _p(lp), nf, 0.0 if dt is None else float(dt),
-                        0 if bernoulli == "exact" else 1, _p(tab), _p(dsc), _p(Y), _p(status))
+                        0 if bernoulli == "exact" else 1, _p(tab), _p(dsc), _p(Y), _p(status),
+                        int(bool(ions)), _p(imu))
     if table:  # the table per point: (n, NP, fields)
         return Y[..., 0] + 1j * Y[..., 1], status, tab, dsc
     return Y[..., 0] + 1j * Y[..., 1], status

@@ -318,6 +318,118 @@ def test_replay_matches_the_oracle(oracle, emu, dt):
             _close_to_oracle(Y[k], dev, ys[k], V, laps, dt, bern)
 
 
+def _close_to_oracle_ions(Y, dev, full5, laps, dt, bern="exact"):
+    """`_close_to_oracle` for the 5 x 5 path: Y against the oracle's ion branch at the same
+    state, to 20 times the spread of its two eliminations (adjugate and pivoted)."""
+    from oracle import ddimpedance as I
+
+    Yo = I.admittance_ions(dev, full5, laps, dt=dt, bernoulli_name=bern)
+    Yp = I.admittance_ions(dev, full5, laps, dt=dt, bernoulli_name=bern, pivoted=True)
+    spread = float(np.max(np.abs(Yo - Yp) / np.abs(Yo)))
+    err = np.abs(np.asarray(Y) - Yo) / np.abs(Yo)
+    assert np.max(err) <= 20 * spread + 1e-10, (np.max(err), spread)
+    return float(np.max(err))
+
+
+def _ion_cases():
+    """(devices, voltage, Bernoulli) for the 5 x 5 path: synthetic devices with ions in their
+    absorber at three voltages (one batch also holding a device without ions, which the 5 x 5
+    path must give the 3 x 3 result), and ZimT's device with ions (SimSS's Bernoulli)."""
+    from test_transient import zimt_case
+
+    ion = _ion_devices(3, 90)
+    mixed = ion[:2] + _synthetic(1, 90)
+    zimt = [zimt_case("zimt_ions_fast_light")[0]]
+    return ([(ion, V, "exact") for V in (0.0, V0, 1.0)] + [(mixed, V0, "exact")]
+            + [(zimt, 0.5, "simss")])
+
+
+def _ion_full_state(oracle, dev, V, bern):
+    """The oracle's steady state at V as the 5 x 5 path's full state (`ion_state`; zero ion
+    columns for a device without ions)."""
+    from oracle import ddimpedance as I
+
+    st = oracle.solve_states(dev, [V], tol=1e-12, max_iter=200, bernoulli_name=bern)
+    assert st["converged"].all()
+    full = np.stack([st["V"][0], np.log(st["n"][0]), np.log(st["p"][0])], -1)
+    return I.ion_state(dev, full, st["anion"][0], st["cation"][0])
+
+
+@pytest.mark.parametrize("dt", [None, 1e-7])
+def test_replay_with_ions_matches_the_oracle(oracle, emu, dt):
+    """The 5 x 5 path's code (the ions' rows, constraint rows and current coefficients in the
+    table; the bordered elimination with three right-hand sides and three functionals) on the
+    CPU against the oracle's ion branch at the same states."""
+    from jaxdd_amd.batch import pack
+
+    from oracle import ddimpedance as I
+
+    laps = I.laps_of(FREQS, dt)
+    for devs, V, bern in _ion_cases():
+        fulls = [_ion_full_state(oracle, d, V, bern) for d in devs]
+        ys = np.stack([f[1:-1, :3] for f in fulls])
+        Y, status = emu.impedance_packed(pack(devs), ys, np.full(len(devs), V), laps, dt, bern,
+                                         ions=True)
+        assert not status.any()
+        for k, dev in enumerate(devs):
+            _close_to_oracle_ions(Y[k], dev, fulls[k], laps, dt, bern)
+
+
+def test_replay_ion_current_coefficients_are_the_complex_step_derivatives(oracle, emu):
+    """The table's coefficients of the ions' logarithms in the linearised current against the
+    complex-step derivatives of the oracle's transient current with ions, point by point: at
+    dti = 0 (only the region's two ends move K) and per unit of lap (the densities' change
+    through the double integral and the direct sum)."""
+    from jaxdd_amd.batch import pack
+    from oracle import ddimpedance as I
+
+    dev = _ion_devices(1, 70)[0]
+    f5 = _ion_full_state(oracle, dev, V0, "exact")
+    _, _, tab, _ = emu.impedance_packed(pack([dev]), f5[None, 1:-1, :3], np.full(1, V0),
+                                        I.laps_of([1e3]), table=True, ions=True)
+    tab = tab[0]
+    regions = I.ion_regions(dev)
+    before = (np.exp(f5[:, 1]), np.exp(f5[:, 2]), f5[:, 0].copy())
+    ions = (regions, tuple(np.exp(f5[:, 3 + k]) * r["mask"] for k, r in enumerate(regions)))
+    NP = len(dev.x) - 2
+
+    def dJ(i, c, dti):
+        v = np.zeros_like(f5)
+        v[i, c] = 1.0
+        return I._cstep(lambda s: I.transient_current(dev, f5 + s * v, dev.G_frac, dti, before,
+                                                      ions=ions), 1.0)
+
+    F = emu.IMP_FIELDS
+    for q, (c0, cl) in enumerate(((F["CA0"], F["CAL"]), (F["CC0"], F["CCL"]))):
+        d0 = np.array([dJ(i + 1, 3 + q, 0.0) for i in range(NP)])
+        d1 = np.array([dJ(i + 1, 3 + q, 1.0) for i in range(NP)])
+        scale = np.max(np.abs(d1 - d0))
+        assert scale > 0
+        assert np.max(np.abs(tab[:, cl] - (d1 - d0))) < 1e-9 * scale
+        assert np.max(np.abs(tab[:, c0] - d0)) < 1e-9 * scale
+
+
+def test_replay_refuses_an_ion_region_at_a_contact(oracle, emu):
+    """A device whose anions reach a contact is outside the 5 x 5 path: status 2, Y NaN; the
+    other devices of the batch are solved."""
+    import dataclasses
+
+    from jaxdd_amd.batch import pack
+    from oracle import ddimpedance as I
+
+    devs = _ion_devices(2, 60)
+    m = np.asarray(devs[1].anion_mask, float).copy()
+    m[: int(np.argmax(m > 0)) + 1] = 1.0  # the region widened to the left contact
+    devs[1] = dataclasses.replace(devs[1], anion_mask=m)
+    with pytest.raises(ValueError, match="contact"):
+        I.ion_regions(devs[1])
+    f0 = _ion_full_state(oracle, devs[0], V0, "exact")
+    ys = np.stack([f0[1:-1, :3], f0[1:-1, :3]])
+    Y, status = emu.impedance_packed(pack(devs), ys, np.full(2, V0), I.laps_of([1e3]), ions=True)
+    assert list(status) == [0, 2]
+    assert np.isfinite(Y[0]).all() and np.isnan(Y[1]).all()
+
+
 def test_replay_current_coefficients_are_the_complex_step_derivatives(oracle, emu):
     """The per-device table's coefficients of the linearised current against the complex-step
     derivatives of the oracle's transient current, point by point, on a fixture device with
@@ -369,7 +481,8 @@ def test_replay_current_coefficients_are_the_complex_step_derivatives(oracle, em
 
 # ---- the host -------------------------------------------------------------------------------
 def test_what_the_host_refuses():
-    """An unknown ion flux, a step that is not positive, mobile ions: before any launch."""
+    """An unknown ion flux, a step that is not positive; with mobile ions, ions frozen at a
+    pre-bias (as the reference) and the modified drift: before any launch."""
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import from_packed, pack
     from jaxdd_amd.impedance import solve_impedance
@@ -380,8 +493,13 @@ def test_what_the_host_refuses():
         solve_impedance(plain, [1e3], ion_flux="drift")
     with pytest.raises(ValueError, match="dt"):
         solve_impedance(plain, [1e3], dt=0.0)
+    frozen = from_packed(pack([S.device_from_row(t, k, 40, True, True) for k in range(3)]), "cpu")
+    with pytest.raises(NotImplementedError, match="frozen"):
+        solve_impedance(frozen, [1e3])
     ions = from_packed(pack([S.device_from_row(t, k, 40, True, False) for k in range(3)]), "cpu")
-    with pytest.raises(NotImplementedError, match="mobile ions"):
+    with pytest.raises(NotImplementedError, match="modified drift"):
+        solve_impedance(ions, [1e3], ion_flux="modified-drift")
+    with pytest.raises(RuntimeError, match="GPU"):
         solve_impedance(ions, [1e3])
 
 
@@ -489,3 +607,68 @@ def test_gpu_series_resistance_and_shunt_leave_the_admittance(gpu_batches):
     np.testing.assert_allclose(
         r.Z, 1.0 / (plain.Y + b2["G_shunt"].cpu().numpy()[:, None])
         + b2["R_series"].cpu().numpy()[:, None], rtol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def gpu_ion_batches(torch_cuda):
+    """(devices, batch, Bernoulli, voltage) for the 5 x 5 path: synthetic devices with ions
+    (one batch of 37, the tables' XCD map giving each XCD a run of several, with two devices
+    without ions among them), ZimT's device with ions (SimSS's Bernoulli)."""
+    from jaxdd_amd.batch import prepare
+    from test_transient import zimt_case
+
+    ion = _ion_devices(6, 120)
+    many = _ion_devices(35, 60) + _synthetic(2, 60)
+    zimt = [zimt_case("zimt_ions_fast_light")[0]]
+    return [(ion, prepare(ion), "exact", V0), (many, prepare(many), "exact", V0),
+            (zimt, prepare(zimt), "simss", 0.5)]
+
+
+def _gpu_ion_state(oracle, dev, y, V):
+    """The 5 x 5 path's full state at the GPU's own steady state y (NP x 3): the ions' closed-form
+    profiles of its potential, as `solve_states` gives them."""
+    from oracle import ddimpedance as I
+    from oracle.ddoracle import _ions_on_grid
+
+    full = I.full_state(dev, y, V)
+    anion, cation = _ions_on_grid(dev, full[None, :, 0])
+    return I.ion_state(dev, full, anion[0], cation[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt", [None, 1e-7])
+def test_gpu_impedance_with_ions_matches_the_oracle(gpu_ion_batches, oracle, dt):
+    """ddh_impedance_ions through the C-ABI against the oracle's ion branch at the GPU's own
+    steady states (devices without ions in the batch: the 3 x 3 result)."""
+    from jaxdd_amd.impedance import laps_of, solve_impedance
+
+    for devs, batch, bern, V in gpu_ion_batches:
+        print(f"impedance with ions: {len(devs)} devices, NP = {batch['NP']}, dt = {dt}",
+              flush=True)
+        r, states = solve_impedance(batch, FREQS, voltage=V, tol=1e-12, max_iter=200, dt=dt,
+                                    bernoulli_name=bern, return_states=True)
+        assert r.converged.all()
+        ys = states.cpu().numpy()
+        for k, dev in enumerate(devs):
+            _close_to_oracle_ions(r.Y[k], dev, _gpu_ion_state(oracle, dev, ys[k], V),
+                                  laps_of(FREQS, dt), dt, bern)
+
+
+@pytest.mark.gpu
+def test_gpu_impedance_with_ions_limits(gpu_ion_batches):
+    """With ions on the GPU alone: at 1e-9 Hz the slope of the GPU's own steady curve with free
+    ions (+-10 uV, to 1e-5), at 1e14 Hz the geometric capacitance (1e-5, 120 points)."""
+    from jaxdd_amd.batch import solve_batch
+    from jaxdd_amd.impedance import solve_impedance
+    from oracle import ddimpedance as I
+
+    devs, batch, bern, V = gpu_ion_batches[0]
+    r = solve_impedance(batch, [1e-9, 1e14], voltage=V, tol=1e-12, max_iter=200)
+    dV = 1e-5
+    J, ok, _ = solve_batch(batch, [V - dV, V + dV], tol=1e-12, max_iter=200)
+    assert ok.cpu().numpy().all()
+    J = J.cpu().numpy()
+    slope = (J[:, 1] - J[:, 0]) / (2 * dV) - batch["G_shunt"].cpu().numpy()
+    assert np.allclose(r.Y[:, 0].real, slope, rtol=1e-5, atol=0)
+    geo = np.array([I.geometric_capacitance(d) for d in devs])
+    assert np.allclose(r.capacitance[:, 1], geo, rtol=1e-5, atol=0)
