@@ -7,8 +7,11 @@ thicknesses, the electron and hole mobility of each layer, the ETL donor and the
 and HTL's acceptor densities, the perovskite's bulk trap density, its SRH lifetime
 tau = 1 / (N_t C) (through the capture coefficient C_n = C_p = C), its generation rate and an
 offset of the left work function. They reach the GPU as the tangents of the packed batch
-arrays, d(array)/d(theta_k), taken here by central differences of the host device builder
-(which is smooth in theta at a fixed grid layout; a layout change inside the step is refused).
+arrays, d(array)/d(theta_k), taken here by central differences of the host device builder on
+each device's own structure: as the reference's `device_tree(structure, params)`, the grid's
+layout (points per layer, their unscaled spacing) is the structure's, fixed at the device's
+parameters, and the thicknesses move the points only by scaling each layer's spacing
+(src/jaxdd/device.py:225-233 `_spacing`).
 The HIP adjoint kernel (`ddh_adjoint`) then linearises each converged voltage's residual and
 current in those directions and solves one transposed block-tridiagonal system per voltage.
 
@@ -49,8 +52,15 @@ def theta_from_table(t: dict, rows) -> np.ndarray:
     return th
 
 
-def device_from_theta(theta, NP: int = 400):
-    """The device of one parameter vector (the drawn workload's stack, no ions)."""
+def structure_of(theta, NP: int = 400):
+    """The structure of the device of ``theta``: the drawn workload's stack on the graded
+    layout of its own thicknesses (reference: src/jaxdd/device.py:385-438 `Structure`)."""
+    return S._structure(np.asarray(theta, dtype=float)[0:3].copy(), NP)
+
+
+def device_from_theta(theta, NP: int = 400, structure=None):
+    """The device of one parameter vector (the drawn workload's stack, no ions) on
+    ``structure`` (default: `structure_of(theta)`, the layout of its own thicknesses)."""
     th = np.asarray(theta, dtype=float)
     L = th[0:3].copy()
     mu = th[3:9]
@@ -63,7 +73,7 @@ def device_from_theta(theta, NP: int = 400):
         C_n_bulk=np.array([C_TRAP, C, C_TRAP]), C_p_bulk=np.array([C_TRAP, C, C_TRAP]),
         W_L=S.SCALARS["W_L"] + th[15],
     )
-    return S.device_arrays(S._structure(L, NP), p)
+    return S.device_arrays(structure_of(th, NP) if structure is None else structure, p)
 
 
 def _steps(th: np.ndarray) -> np.ndarray:
@@ -74,7 +84,8 @@ def _steps(th: np.ndarray) -> np.ndarray:
 
 def _tangent_rows(args):
     thetas, NP = args
-    base = pack([device_from_theta(t, NP) for t in thetas])
+    structs = [structure_of(t, NP) for t in thetas]
+    base = pack([device_from_theta(t, NP, st) for t, st in zip(thetas, structs)])
     n, K = len(thetas), thetas.shape[1]
     out = {"pt": np.zeros((K,) + base["pt"].shape), "seg": np.zeros((K,) + base["seg"].shape),
            "sc": np.zeros((K,) + base["sc"].shape)}
@@ -82,11 +93,13 @@ def _tangent_rows(args):
     for k in range(K):
         e = np.zeros(K)
         e[k] = 1.0
-        hi = pack([device_from_theta(t + hk * e, NP) for t, hk in zip(thetas, h[:, k])])
-        lo = pack([device_from_theta(t - hk * e, NP) for t, hk in zip(thetas, h[:, k])])
+        hi = pack([device_from_theta(t + hk * e, NP, st)
+                   for t, hk, st in zip(thetas, h[:, k], structs)])
+        lo = pack([device_from_theta(t - hk * e, NP, st)
+                   for t, hk, st in zip(thetas, h[:, k], structs)])
         for side in (hi, lo):
             if side["nseg"] != base["nseg"] or not np.array_equal(side["sid"], base["sid"]):
-                raise ValueError(f"the grid layout changes within the step of {THETA[k]}")
+                raise ValueError(f"the segments change within the step of {THETA[k]}")
         for key in ("pt", "seg", "sc"):
             out[key][k] = (hi[key] - lo[key]) / (2.0 * h[:, k])
     return base, out
@@ -95,8 +108,9 @@ def _tangent_rows(args):
 def tangents(thetas, NP: int = 400, workers: int = 1, chunk: int = 64):
     """(packed batch, tangents): the packed arrays of the devices of ``thetas`` (n, 16) and
     their derivatives in each parameter, {"pt": (16, NP+2, 4, n), "seg": (16, nseg, 22, n),
-    "sc": (16, 20, n)}, by central differences of the builder. ``workers`` > 1 forks processes
-    (call it before the process touches the GPU)."""
+    "sc": (16, 20, n)}, by central differences of the builder on each device's structure (its
+    layout fixed). ``workers`` > 1 forks processes (call it before the process touches the
+    GPU)."""
     from .batch import merge
 
     thetas = np.atleast_2d(np.asarray(thetas, dtype=float))

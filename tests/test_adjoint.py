@@ -42,12 +42,13 @@ def test_tangents_are_the_builders_derivatives():
     from jaxdd_amd.batch import pack
 
     th = A.theta_from_table(S.draw(1), [0])
+    st = A.structure_of(th[0])
     base, tan = A.tangents(th)
     for k in (1, 3, 10, 14, 15):
         h = 10 * A._steps(th)[0, k]
         e = np.zeros(16)
         e[k] = h
-        hi = pack([A.device_from_theta(th[0] + e)])
+        hi = pack([A.device_from_theta(th[0] + e, structure=st)])
         for key in ("pt", "seg", "sc"):
             one = (hi[key] - base[key]) / h
             ref = np.max(np.abs(tan[key][k])) + 1e-300
@@ -66,8 +67,10 @@ def test_adjoint_is_the_finite_difference_of_the_curves(c4_case, oracle):
     scale[:, 15] = 0.01
     by_adjoint = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
 
+    structs = [A.structure_of(t) for t in th]  # each device's layout, held as theta moves
+
     def loss(a):
-        devs = [A.device_from_theta(x) for x in _theta_at(th, d, a)]
+        devs = [A.device_from_theta(x, structure=st) for x, st in zip(_theta_at(th, d, a), structs)]
         Jx, okx, _ = oracle.solve_batch(devs, V, tol=1e-12, strict=True, first_max_iter=None,
                                         max_iter=200)
         assert okx.all()
@@ -87,20 +90,42 @@ def test_each_parameter_against_finite_differences(c4_case, oracle):
 
     th, _, _, V, J, ok, grad, _ = c4_case
     dev_idx = 1
-    dev = A.device_from_theta(th[dev_idx])
+    st = A.structure_of(th[dev_idx])  # the device's layout, held as theta moves
+    dev = A.device_from_theta(th[dev_idx], structure=st)
     cscale = j_scale(dev) + np.max(np.abs(J[dev_idx]))
     for k in range(16):
         step = 1e-5 * abs(th[dev_idx, k]) if k != 15 else 1e-5
         e = np.zeros(16)
         e[k] = step
-        Jp = oracle.solve_batch([A.device_from_theta(th[dev_idx] + e)], V, tol=1e-12,
-                                strict=True, first_max_iter=None, max_iter=200)[0][0]
-        Jm = oracle.solve_batch([A.device_from_theta(th[dev_idx] - e)], V, tol=1e-12,
-                                strict=True, first_max_iter=None, max_iter=200)[0][0]
+        Jp = oracle.solve_batch([A.device_from_theta(th[dev_idx] + e, structure=st)], V,
+                                tol=1e-12, strict=True, first_max_iter=None, max_iter=200)[0][0]
+        Jm = oracle.solve_batch([A.device_from_theta(th[dev_idx] - e, structure=st)], V,
+                                tol=1e-12, strict=True, first_max_iter=None, max_iter=200)[0][0]
         fd = (Jp - Jm) / (2 * step)
         unit = abs(th[dev_idx, k]) if k != 15 else 1.0
         err = np.max(np.abs(grad[dev_idx, :, k] - fd)) * unit / cscale
         assert err < 1e-5, (A.THETA[k], err)
+
+
+def test_thickness_moves_the_points_by_scaling_only():
+    """The reference's semantics of a thickness's derivative (src/jaxdd/batch.py:142-148
+    `device_tree(structure, params)`, src/jaxdd/device.py:225-233 `_spacing`): the layout is the
+    structure's, so the points of the layer whose thickness grows move by scaling its spacing
+    (with the interface means); here the tangents of x against the closed form of `place`."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(2), [1])
+    _, tan = A.tangents(th)
+    lay = A.structure_of(th[0]).layout
+    for k in range(3):
+        h = 1e-6 * th[0, k]
+        Lp, Lm = th[0, 0:3].copy(), th[0, 0:3].copy()
+        Lp[k] += h
+        Lm[k] -= h
+        ref = (lay.place(Lp)[0] - lay.place(Lm)[0]) / (2 * h)
+        got = tan["pt"][k, :, 1, 0]  # d x / d L_k
+        assert np.max(np.abs(got - ref)) < 1e-7 * np.max(np.abs(ref))
 
 
 def test_unconverged_voltages_have_zero_gradient(emu):
@@ -148,10 +173,11 @@ def test_reverse_pass_is_the_directions_passes(c4_case, emu):
     assert not st2.any()
     # dJ/dln(theta_k) against each curve's current scale, as the finite-difference tests
     # measure (a parameter J hardly depends on, such as a trap density far from where carriers
-    # recombine, has its two terms cancel to round-off of the larger)
+    # recombine, has its two terms cancel to round-off of the larger; a thickness's terms,
+    # through every point's position, cancel to 1.2e-11 of the scale at worst)
     unit = np.abs(th)
     unit[:, 15] = 1.0
     cscale = np.array([j_scale(A.device_from_theta(x)) for x in th]) + np.max(np.abs(J), axis=1)
     err = np.abs(grad - by_pass) * unit[:, None, :] / cscale[:, None, None]
-    assert np.max(err) < 1e-11, np.unravel_index(np.argmax(err), err.shape)
+    assert np.max(err) < 5e-11, np.unravel_index(np.argmax(err), err.shape)
     assert np.max(np.abs(by_pass) * unit[:, None, :] / cscale[:, None, None]) > 0.1

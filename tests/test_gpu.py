@@ -427,10 +427,12 @@ def test_adjoint_is_the_finite_difference_of_the_gpu_curves(c4_gpu):
     scale[:, 15] = 0.01
     by_adjoint = np.sum(w[:, :, None] * grad * (scale * d)[:, None, :])
 
+    structs = [A.structure_of(t) for t in th]  # each device's layout, held as theta moves
+
     def loss(a):
         x = th * (1.0 + a * d)
         x[:, 15] = th[:, 15] + a * d[:, 15] * 0.01
-        batch = prepare([A.device_from_theta(t) for t in x])
+        batch = prepare([A.device_from_theta(t, structure=st) for t, st in zip(x, structs)])
         Jx, okx, _ = solve_batch(batch, V, tol=1e-12, strict=True, max_iter=200)
         assert _np(okx).all()
         return float(np.sum(w * _np(Jx)))
@@ -458,10 +460,15 @@ def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
         x[:, 15] = th[:, 15] + a * d[:, 15] * 0.01  # the work-function offset: 10 mV units
         return x
 
-    devs0 = [A.device_from_theta(t) for t in theta_at(0.0)]
+    structs = [A.structure_of(t) for t in th]  # each device's layout, held as theta moves
+
+    def devices_at(a):
+        return [A.device_from_theta(t, structure=st) for t, st in zip(theta_at(a), structs)]
+
+    devs0 = devices_at(0.0)
     base = pack(devs0)
     e = A.REL_STEP
-    hi, lo = (pack([A.device_from_theta(t) for t in theta_at(s)]) for s in (e, -e))
+    hi, lo = (pack(devices_at(s)) for s in (e, -e))
     for side in (hi, lo):
         assert side["nseg"] == base["nseg"] and np.array_equal(side["sid"], base["sid"])
     tan = {k: ((hi[k] - lo[k]) / (2 * e))[None] for k in ("pt", "seg", "sc")}
@@ -472,8 +479,8 @@ def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
     h = 1e-5
     curves = []
     for s_ in (h, -h):
-        Jx, okx, _ = solve_batch(prepare([A.device_from_theta(t) for t in theta_at(s_)]), V,
-                                 tol=1e-12, strict=True, max_iter=200)
+        Jx, okx, _ = solve_batch(prepare(devices_at(s_)), V, tol=1e-12, strict=True,
+                                 max_iter=200)
         curves.append((_np(Jx), _np(okx)))
     fd = (curves[0][0] - curves[1][0]) / (2 * h)
     both = ok & curves[0][1] & curves[1][1]
