@@ -221,24 +221,27 @@ def adjoint_b_unit(NP: int = 400, b: int = 3, s: int = SEGMENT_COEFFS) -> int:
     return NP * (8 * (b + s) + 2 * 8 * (b * b + b) + 8 * b)
 
 
-def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: str = "fp64"):
-    """dJ/dtheta of the gradient workload's curves: one solve keeping the converged states
-    and one adjoint launch, with the batch and its tangents resident in HBM before the timed
-    region (the warm-up call has allocated the workspaces), timed with HIP events on the
-    launch stream and by the host clock around the call."""
+def gradient_bench(theta, V, warm: bool = True, lanes: int = 4, precision: str = "fp64"):
+    """dJ/dtheta of the gradient workload's curves from the parameter vectors: each device's
+    graded layout, its packed arrays and their 16 parameter tangents built on the GPU
+    (ddh_graded_layout, ddh_build), one solve keeping the converged states and one adjoint
+    launch, timed with HIP events on the launch stream and by the host clock around the whole
+    (a warm-up round has allocated the workspaces)."""
     import torch
 
-    from jaxdd_amd.adjoint import jv_gradients, resident
+    from jaxdd_amd.adjoint import gpu_tangents, jv_gradients
 
-    packed, tan = tangents_
-    n = packed["sc"].shape[1]
-    batch, dtan = resident(packed, tan)
+    n = len(theta)
     if warm:
-        jv_gradients(batch, dtan, V, lanes=lanes, precision=precision)
+        batch, tan = gpu_tangents(theta)
+        jv_gradients(batch, tan, V, lanes=lanes, precision=precision)
+        del batch, tan
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, ok, its, grad = jv_gradients(batch, dtan, V, events=ev, lanes=lanes, precision=precision)
+    batch, tan = gpu_tangents(theta)
+    tb = time.perf_counter()
+    J, ok, its, grad = jv_gradients(batch, tan, V, events=ev, lanes=lanes, precision=precision)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solve_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
@@ -246,8 +249,12 @@ def gradient_bench(tangents_, V, warm: bool = True, lanes: int = 4, precision: s
     unit = adjoint_b_unit()
     achieved = float(ok.sum()) * unit / (adj_ms * 1e-3) / 1e9
     return {
-        "value": n / (t1 - t0), "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters)",
+        "value": n / (t1 - t0),
+        "unit": "gradient curves/s (dJ(V)/dtheta, 16 parameters, from the parameter vectors: "
+                "the layouts, the batch and its tangents built on the GPU, the solve and the "
+                "adjoint)",
         "devices": n, "voltages": len(V), "ms": (t1 - t0) * 1e3,
+        "build_ms": (tb - t0) * 1e3, "tangent_bytes": sum(v.numel() * 8 for v in tan.values()),
         "solve_kernel_ms": solve_ms, "adjoint_kernel_ms": adj_ms, "solve_lanes_per_device": lanes,
         "solve_precision": precision,
         "converged_frac": float(ok.mean()),
@@ -348,7 +355,7 @@ def main() -> int:
 
         t = time.perf_counter()
         theta = A.theta_from_table(S.draw(args.grad_n), range(args.grad_n))
-        grad_inputs = A.tangents(theta, workers=workers)
+        grad_inputs = (theta,)
         grad_host_s = time.perf_counter() - t
     imp_packed = None
     if args.imp_n:
@@ -492,12 +499,11 @@ def main() -> int:
         }
         del fb
     if grad_inputs is not None:
-        out["gradient"] = gradient_bench(grad_inputs, V, lanes=args.grad_lanes,
+        out["gradient"] = gradient_bench(*grad_inputs, V, lanes=args.grad_lanes,
                                          precision=args.grad_precision)
         g = out["gradient"]
-        g["host_tangents_s"] = grad_host_s
-        g["unit"] += ", the parameter tangents resident in HBM"
-        # the whole gradient, the host's central-difference tangents of the builder included
+        g["host_draw_s"] = grad_host_s
+        # the whole gradient, the parameter vectors' draw on the host included
         g["end_to_end_value"] = args.grad_n / (grad_host_s + g["ms"] * 1e-3)
         out["gradient"]["workload"] = (f"c4: {args.grad_n} devices without ions, NP=400, 61 "
                                        "voltages, FP64, 16 parameters")

@@ -167,6 +167,52 @@ int ddh_assemble_ions(const ddh_batch *b, const double *y, const double *Vint,
                       const double *logs, int32_t bernoulli, double *F, double *A, double *B,
                       double *C, double *U, void *hip_stream);
 
+/* A batch's structures (what its numbers do not change), lane-innermost device arrays: each
+ * point's layer lid [NP + 2][n] and segment sid [NP + 2][n] (a new segment wherever a field of
+ * the packed arrays may change: a layer's first point, a contact, the last point of a layer
+ * before an interface), each layer's flags [nl][n] (1 bulk traps, 2 mobile anions, 4 fixed
+ * anions, 8 mobile cations, 16 fixed cations) and first point i0 [nl][n], each point's unscaled
+ * spacing unit [NP + 2][n] (the interval after it) and its sum over each layer unit_sum
+ * [nl][n]. nl layers (at most 8) and nseg segments (at most 14) for every device. */
+typedef struct {
+    int32_t n, NP, nl, nseg;
+    const uint8_t *lid, *sid, *flags;
+    const int32_t *i0;
+    const double *unit, *unit_sum;
+} ddh_structure;
+
+/* The graded grid's layout of every device on the GPU: replaces src/jaxdd/device.py:160-214 (the
+ * graded rule: the point density 1/H + beta/(d + d0) in the distance d to either end of a layer,
+ * its integral shared among the layers, at least five points beyond each layer's first, each
+ * layer's points equidistributing it) with :114-136 (`_layout`). L [nl][n] the thicknesses (m);
+ * beta the points per e-fold, d0 (m) the distance below which it stops refining. Outputs the
+ * ddh_structure arrays lid, sid, i0, unit, unit_sum (as ddh_build takes them), iface
+ * [NP + 1][n] (1 where the interval after the point crosses from one layer to the next) and
+ * nseg [n] each device's segment count (-1: its counts do not fill the grid). */
+int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L, double beta, double d0,
+                      uint8_t *lid, uint8_t *sid, int32_t *i0, double *unit, double *unit_sum,
+                      uint8_t *iface, int32_t *nseg, void *hip_stream);
+
+/* The batch's packed arrays from its structures and numbers on the GPU: replaces
+ * src/jaxdd/device.py:877-1063 (`device_arrays`, the grid placed by `_spacing` :225-233) and
+ * the packing of src/jaxdd/batch.py:151-223 (`prepare`), for the path's physics (Boltzmann
+ * carriers, uniform generation per layer, ohmic contacts, one bulk trap level, mobile and fixed
+ * ions' masks and totals). layers [nl][21][n]: per layer L, eps_r, E_c, E_v, N_c, N_D, N_A,
+ * mu_n, mu_p, nu_int_n, nu_int_p, k_direct, N_t_bulk, C_n_bulk, C_p_bulk, E_t_bulk, G_ehp,
+ * N_anion, N_cation, mu_anion, mu_cation; scalars [9][n]: T, W_L, W_R, G_frac, V_pre, 1/R_shunt
+ * (0: none), pre_cond, fix_ions, left_elec. Outputs the ddh_batch arrays pt [NP + 2][4][n]
+ * (h, x, G, fixed anions - cations), seg [nseg][22][n] (each segment's fields, a device's
+ * segments beyond its own count repeating its last), sc [20][n], and ion_mu [nseg][2][n] (each
+ * segment's anion and cation mobility). With K > 0 directions dlayers [K][nl][21][n] and
+ * dscalars [K][9][n] (each a direction in the numbers), also their derivatives along each,
+ * forward mode: dpt [K][NP + 2][4][n], dseg [K][nseg][22][n], dsc [K][20][n], the tangents
+ * ddh_adjoint takes; the structure is held (the thicknesses scale each layer's spacing), as
+ * the reference differentiates `device_tree(structure, params)` (batch.py:142-148). */
+int ddh_build(const ddh_structure *structure, const double *layers, const double *scalars,
+              int32_t K, const double *dlayers, const double *dscalars, double *pt, double *seg,
+              double *sc, double *ion_mu, double *dpt, double *dseg, double *dsc,
+              void *hip_stream);
+
 /* Device bytes of ddh_impedance's workspace (the per-device tables) for n devices of NP
  * interior points. */
 size_t ddh_impedance_workspace_bytes(int32_t n, int32_t NP);

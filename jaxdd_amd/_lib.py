@@ -21,6 +21,13 @@ class Batch(ctypes.Structure):
                 ("seg", ctypes.c_void_p), ("sc", ctypes.c_void_p)]
 
 
+class Structure(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("NP", ctypes.c_int32), ("nl", ctypes.c_int32),
+                ("nseg", ctypes.c_int32), ("lid", ctypes.c_void_p), ("sid", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p), ("i0", ctypes.c_void_p), ("unit", ctypes.c_void_p),
+                ("unit_sum", ctypes.c_void_p)]
+
+
 class TOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
                 ("bernoulli", ctypes.c_int32), ("zimt_current", ctypes.c_int32),
@@ -40,7 +47,8 @@ SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solv
            "ddh_block_sweep", "ddh_assemble", "ddh_assemble_ions",
            "ddh_impedance_workspace_bytes", "ddh_impedance", "ddh_noalloc_scratch_bytes",
            "ddh_solve_batch_noalloc", "ddh_transient_workspace_bytes", "ddh_transient",
-           "ddh_impedance_ions_workspace_bytes", "ddh_impedance_ions")
+           "ddh_impedance_ions_workspace_bytes", "ddh_impedance_ions", "ddh_build",
+           "ddh_graded_layout")
 
 _lib = None
 
@@ -97,6 +105,12 @@ def load():
     lib.ddh_transient.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, vp, i32, vp,
                                   ctypes.POINTER(TOpts), i32, vp, vp, vp, vp, vp, vp, vp,
                                   ctypes.c_size_t, vp]
+    lib.ddh_graded_layout.restype = ctypes.c_int
+    lib.ddh_graded_layout.argtypes = [i32, i32, i32, vp, ctypes.c_double, ctypes.c_double, vp,
+                                      vp, vp, vp, vp, vp, vp, vp]
+    lib.ddh_build.restype = ctypes.c_int
+    lib.ddh_build.argtypes = [ctypes.POINTER(Structure), vp, vp, i32, vp, vp, vp, vp, vp, vp,
+                              vp, vp, vp, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 2:

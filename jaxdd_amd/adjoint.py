@@ -6,12 +6,13 @@ The parameters are those of the gradient workload, 16 per device (`THETA`): the 
 thicknesses, the electron and hole mobility of each layer, the ETL donor and the perovskite's
 and HTL's acceptor densities, the perovskite's bulk trap density, its SRH lifetime
 tau = 1 / (N_t C) (through the capture coefficient C_n = C_p = C), its generation rate and an
-offset of the left work function. They reach the GPU as the tangents of the packed batch
-arrays, d(array)/d(theta_k), taken here by central differences of the host device builder on
-each device's own structure: as the reference's `device_tree(structure, params)`, the grid's
-layout (points per layer, their unscaled spacing) is the structure's, fixed at the device's
-parameters, and the thicknesses move the points only by scaling each layer's spacing
-(src/jaxdd/device.py:225-233 `_spacing`).
+offset of the left work function. They reach the adjoint as the tangents of the packed batch
+arrays, d(array)/d(theta_k), built on the GPU with the batch itself (`gpu_tangents`,
+ddh_build's forward mode; `tangents` keeps the host builder's central differences as their
+check): as the reference's `device_tree(structure, params)`, the grid's layout (points per
+layer, their unscaled spacing) is the structure's, fixed at the device's parameters, and the
+thicknesses move the points only by scaling each layer's spacing (src/jaxdd/device.py:225-233
+`_spacing`).
 The HIP adjoint kernel (`ddh_adjoint`) then linearises each converged voltage's residual and
 current in those directions and solves one transposed block-tridiagonal system per voltage.
 
@@ -58,9 +59,9 @@ def structure_of(theta, NP: int = 400):
     return S._structure(np.asarray(theta, dtype=float)[0:3].copy(), NP)
 
 
-def device_from_theta(theta, NP: int = 400, structure=None):
-    """The device of one parameter vector (the drawn workload's stack, no ions) on
-    ``structure`` (default: `structure_of(theta)`, the layout of its own thicknesses)."""
+def params_of(theta) -> dict:
+    """The numbers (`device_parameters`' dict) of one parameter vector: the drawn workload's
+    stack, no ions."""
     th = np.asarray(theta, dtype=float)
     L = th[0:3].copy()
     mu = th[3:9]
@@ -73,7 +74,128 @@ def device_from_theta(theta, NP: int = 400, structure=None):
         C_n_bulk=np.array([C_TRAP, C, C_TRAP]), C_p_bulk=np.array([C_TRAP, C, C_TRAP]),
         W_L=S.SCALARS["W_L"] + th[15],
     )
-    return S.device_arrays(structure_of(th, NP) if structure is None else structure, p)
+    return p
+
+
+def device_from_theta(theta, NP: int = 400, structure=None):
+    """The device of one parameter vector (the drawn workload's stack, no ions) on
+    ``structure`` (default: `structure_of(theta)`, the layout of its own thicknesses)."""
+    st = structure_of(theta, NP) if structure is None else structure
+    return S.device_arrays(st, params_of(theta))
+
+
+def directions_of(thetas) -> tuple[np.ndarray, np.ndarray]:
+    """The 16 parameters as directions in the GPU builder's numbers (`builder.LAYER_FIELDS`,
+    `SCALAR_FIELDS`): dlayers (16, 3, 21, n), dscalars (16, 9, n). Each is its number's unit
+    direction, but for the trap density and the lifetime, which also move the perovskite's
+    capture coefficients C = 1/(N_t tau)."""
+    from .builder import LAYER_FIELDS, SCALAR_FIELDS
+
+    th = np.atleast_2d(np.asarray(thetas, dtype=float))
+    n = len(th)
+    F = {f: i for i, f in enumerate(LAYER_FIELDS)}
+    dl = np.zeros((len(THETA), 3, len(LAYER_FIELDS), n))
+    ds = np.zeros((len(THETA), len(SCALAR_FIELDS), n))
+    for k in range(3):
+        dl[k, k, F["L"]] = 1.0
+    for q in range(6):
+        dl[3 + q, q // 2, F["mu_n" if q % 2 == 0 else "mu_p"]] = 1.0
+    dl[9, 0, F["N_D"]] = 1.0
+    dl[10, 1, F["N_A"]] = 1.0
+    dl[11, 2, F["N_A"]] = 1.0
+    C = 1.0 / (th[:, 12] * th[:, 13])
+    dl[12, 1, F["N_t_bulk"]] = 1.0
+    for f in ("C_n_bulk", "C_p_bulk"):
+        dl[12, 1, F[f]] = -C / th[:, 12]
+        dl[13, 1, F[f]] = -C / th[:, 13]
+    dl[14, 1, F["G_ehp"]] = 1.0
+    ds[15, SCALAR_FIELDS.index("W_L")] = 1.0
+    return dl, ds
+
+
+def _structure_rows(args):
+    thetas, NP = args
+    return [structure_of(t, NP) for t in thetas]
+
+
+def structures(thetas, NP: int = 400, workers: int = 1, chunk: int = 64):
+    """The structures of the devices of ``thetas`` (n, 16): each on the graded layout of its
+    own thicknesses. ``workers`` > 1 forks processes (call it before the process touches the
+    GPU)."""
+    th = np.atleast_2d(np.asarray(thetas, dtype=float))
+    jobs = [(th[s:s + chunk], NP) for s in range(0, len(th), chunk)]
+    if workers <= 1:
+        return [st for j in jobs for st in _structure_rows(j)]
+    import multiprocessing as mp
+
+    pool = mp.get_context("fork").Pool(workers)
+    try:
+        parts = pool.map(_structure_rows, jobs)
+    finally:
+        pool.close()  # let the workers exit on their own (no SIGTERM from terminate())
+        pool.join()
+    return [st for part in parts for st in part]
+
+
+#: The stack's layers' flags for the GPU builder: bulk traps in the perovskite, no ions.
+STACK_FLAGS = (0, 1, 0)
+
+
+def numbers_of(thetas) -> tuple[np.ndarray, np.ndarray]:
+    """The GPU builder's numbers (`builder.number_arrays`' layout: layers [3][21][n], scalars
+    [9][n]) of parameter vectors (n, 16), vectorised: `params_of` for every device at once."""
+    from .builder import LAYER_FIELDS, SCALAR_FIELDS
+
+    th = np.atleast_2d(np.asarray(thetas, dtype=float))
+    n = len(th)
+    base = dict(S.BASE, **S.SCALARS)
+    layers = np.zeros((3, len(LAYER_FIELDS), n))
+    for i, f in enumerate(LAYER_FIELDS):
+        if f in base:
+            layers[:, i, :] = np.asarray(base[f], dtype=float)[:, None]
+    F = {f: i for i, f in enumerate(LAYER_FIELDS)}
+    layers[:, F["L"], :] = th[:, 0:3].T
+    layers[:, F["mu_n"], :] = th[:, 3:9:2].T
+    layers[:, F["mu_p"], :] = th[:, 4:9:2].T
+    layers[:, F["N_D"], :] = 0.0
+    layers[0, F["N_D"], :] = th[:, 9]
+    layers[:, F["N_A"], :] = 0.0
+    layers[1, F["N_A"], :] = th[:, 10]
+    layers[2, F["N_A"], :] = th[:, 11]
+    layers[:, F["N_t_bulk"], :] = 0.0
+    layers[1, F["N_t_bulk"], :] = th[:, 12]
+    C = 1.0 / (th[:, 12] * th[:, 13])
+    for f in ("C_n_bulk", "C_p_bulk"):
+        layers[:, F[f], :] = C_TRAP
+        layers[1, F[f], :] = C
+    layers[:, F["G_ehp"], :] = 0.0
+    layers[1, F["G_ehp"], :] = th[:, 14]
+    scalars = np.zeros((len(SCALAR_FIELDS), n))
+    for i, f in enumerate(SCALAR_FIELDS):
+        if f in base:
+            scalars[i] = float(base[f])
+    scalars[SCALAR_FIELDS.index("W_L")] = S.SCALARS["W_L"] + th[:, 15]
+    scalars[SCALAR_FIELDS.index("G_shunt")] = 0.0
+    scalars[SCALAR_FIELDS.index("pre_cond")] = 0.0
+    scalars[SCALAR_FIELDS.index("fix_ions")] = 0.0
+    scalars[SCALAR_FIELDS.index("left_elec")] = -1.0
+    return layers, scalars
+
+
+def gpu_tangents(thetas, NP: int = 400, device="cuda", structs=None):
+    """(batch, tangents) on the GPU: the devices of ``thetas`` (n, 16) built on the GPU with
+    their derivatives in each parameter (forward mode, each device's structure held), in place
+    of `tangents`' host central differences. By default everything is built there: each
+    device's graded layout (ddh_graded_layout) and its arrays (ddh_build); ``structs``: the
+    host's structures (`structures`) instead of the GPU's layouts."""
+    from .builder import build, build_graded
+
+    th = np.atleast_2d(np.asarray(thetas, dtype=float))
+    if structs is not None:
+        return build(structs, [params_of(t) for t in th], directions_of(th), device=device)
+    layers, scalars = numbers_of(th)
+    return build_graded(th[:, 0:3], STACK_FLAGS, layers, scalars, directions_of(th), NP=NP,
+                        device=device)
 
 
 def _steps(th: np.ndarray) -> np.ndarray:
@@ -223,17 +345,20 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
 
 class _Curves(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, voltages, tol, max_iter, workers):
-        th = theta.detach().cpu().numpy()
+    def forward(ctx, theta, voltages, tol, max_iter):
+        th = np.atleast_2d(theta.detach().cpu().numpy())
+        dev = theta.device if theta.is_cuda else "cuda"
         if not ctx.needs_input_grad[0]:  # no backward will follow: the curves only
             from .batch import solve_batch
+            from .builder import build_graded
 
-            batch = from_packed(pack([device_from_theta(t) for t in np.atleast_2d(th)]))
+            layers, scalars = numbers_of(th)
+            batch = build_graded(th[:, 0:3], STACK_FLAGS, layers, scalars, device=dev)
             J, ok, _ = solve_batch(batch, voltages, tol=tol, max_iter=max_iter)
             ctx.mark_non_differentiable(ok)
             return J.to(theta.device), ok.to(theta.device)
-        packed, tan = tangents(th, workers=workers)
-        J, ok, its, grad = jv_gradients(packed, tan, voltages, tol=tol, max_iter=max_iter)
+        batch, tan = gpu_tangents(th, device=dev)
+        J, ok, its, grad = jv_gradients(batch, tan, voltages, tol=tol, max_iter=max_iter)
         ctx.save_for_backward(grad.to(theta.device))
         ctx.mark_non_differentiable(ok)
         return J.to(theta.device), ok.to(theta.device)
@@ -241,11 +366,12 @@ class _Curves(torch.autograd.Function):
     @staticmethod
     def backward(ctx, J_bar, ok_bar):
         (grad,) = ctx.saved_tensors
-        return torch.einsum("nv,nvk->nk", J_bar.to(grad.dtype), grad), None, None, None, None
+        return torch.einsum("nv,nvk->nk", J_bar.to(grad.dtype), grad), None, None, None
 
 
-def curves(theta: torch.Tensor, voltages, tol=1e-10, max_iter=60, workers=1):
+def curves(theta: torch.Tensor, voltages, tol=1e-10, max_iter=60):
     """J-V curves (n, nV) and converged flags of the devices of ``theta`` (n, 16) float64, at
     ascending ``voltages``, differentiable in theta (backward: dJ/dtheta from the adjoint
-    kernel; zero at unconverged voltages, as the reference's)."""
-    return _Curves.apply(theta, voltages, tol, max_iter, workers)
+    kernel; zero at unconverged voltages, as the reference's). The devices, their graded
+    layouts and, for a backward, their tangents are built on the GPU."""
+    return _Curves.apply(theta, voltages, tol, max_iter)

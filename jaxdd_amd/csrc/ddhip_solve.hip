@@ -12,6 +12,7 @@
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
+#include "ddhip_build.h"
 #include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
@@ -248,6 +249,22 @@ __global__ void __launch_bounds__(256) impedance_kernel(ImpArgs a) {
     if (gid >= (long)a.n * a.nf) return;
     if (IONS) impedance_ion_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
     else impedance_lane(a, (int)(gid / a.nf), (int)(gid % a.nf));
+}
+
+// The batch's arrays and their tangents from each device's structure and numbers
+// (ddhip_build.h): one lane per (device, direction), devices innermost so that a direction's
+// slice of the lane-innermost arrays is written by neighbouring lanes.
+__global__ void __launch_bounds__(64) build_kernel(BuildArgs a) {
+    const long gid = (long)blockIdx.x * 64 + threadIdx.x;
+    const long lanes = (long)a.n * (a.K > 0 ? a.K : 1);
+    if (gid >= lanes) return;
+    build_lane(a, (int)(gid % a.n), a.K > 0 ? (int)(gid / a.n) : -1);
+}
+
+// The graded grid's layout of every device (ddhip_build.h): one lane per device.
+__global__ void __launch_bounds__(64) layout_kernel(LayoutArgs a) {
+    const int d = blockIdx.x * 64 + threadIdx.x;
+    if (d < a.n) layout_lane(a, d);
 }
 
 // The transient (ddhip_transient.h): one lane per device through all of its time steps, one
@@ -817,6 +834,55 @@ extern "C" int ddh_impedance_ions(const ddh_batch *b, const double *ion_mu, cons
                                   void *stream) {
     return impedance_launch("ddh_impedance_ions", true, b, ion_mu, y, Vint, iface, laps, nf, dt,
                             bernoulli, Y, status, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L,
+                                 double beta, double d0, uint8_t *lid, uint8_t *sid, int32_t *i0,
+                                 double *unit, double *unit_sum, uint8_t *iface, int32_t *nseg,
+                                 void *stream) {
+    if (!L || !lid || !sid || !i0 || !unit || !unit_sum || !iface || !nseg)
+        return fail(DDH_E_ARG, "ddh_graded_layout: NULL argument");
+    if (n <= 0 || NP < 2 || NP > 65536 || nl < 1 || nl > ddh::BUILD_MAX_LAYERS ||
+        NP + 2 < nl * (ddh::GRADED_MIN_POINTS + 1))
+        return fail(DDH_E_SHAPE, "ddh_graded_layout: n > 0, 2 <= NP <= 65536, 1 <= nl <= 8, "
+                                 "six points per layer");
+    if (!(beta >= 0.0) || !(d0 > 0.0))
+        return fail(DDH_E_ARG, "ddh_graded_layout: beta >= 0 and d0 > 0");
+    ddh::LayoutArgs a = {};
+    a.n = n; a.NP = NP; a.nl = nl; a.beta = beta; a.d0 = d0; a.L = L;
+    a.lid = lid; a.sid = sid; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum; a.iface = iface;
+    a.nseg = nseg;
+    hipLaunchKernelGGL(ddh::layout_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
+                       (hipStream_t)stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "layout_kernel launch");
+    return DDH_OK;
+}
+
+extern "C" int ddh_build(const ddh_structure *st, const double *layers, const double *scalars,
+                         int32_t K, const double *dlayers, const double *dscalars, double *pt,
+                         double *seg, double *sc, double *ion_mu, double *dpt, double *dseg,
+                         double *dsc, void *stream) {
+    if (!st || !layers || !scalars || !pt || !seg || !sc || !ion_mu || !st->lid || !st->sid ||
+        !st->flags || !st->i0 || !st->unit || !st->unit_sum)
+        return fail(DDH_E_ARG, "ddh_build: NULL argument");
+    if (K < 0 || K > 1024 || (K > 0 && (!dlayers || !dscalars || !dpt || !dseg || !dsc)))
+        return fail(DDH_E_ARG, "ddh_build: 0 <= K <= 1024, and the directions' arrays when K > 0");
+    if (st->n <= 0 || st->NP < 2 || st->NP > 65536 || st->nl < 1 || st->nl > ddh::BUILD_MAX_LAYERS ||
+        st->nseg < 1 || st->nseg > DDH_MAX_SEGMENTS || (long)st->n * (K > 0 ? K : 1) > (1L << 30))
+        return fail(DDH_E_SHAPE, "ddh_build: n > 0, 2 <= NP <= 65536, 1 <= nl <= 8, 1 <= nseg <= 14");
+    ddh::BuildArgs a = {};
+    a.n = st->n; a.NP = st->NP; a.nl = st->nl; a.nseg = st->nseg; a.K = K;
+    a.lid = st->lid; a.sid = st->sid; a.flags = st->flags; a.i0 = st->i0;
+    a.unit = st->unit; a.unit_sum = st->unit_sum;
+    a.layers = layers; a.scalars = scalars; a.dlayers = dlayers; a.dscalars = dscalars;
+    a.pt = pt; a.seg = seg; a.sc = sc; a.imu = ion_mu; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
+    const long lanes = (long)st->n * (K > 0 ? K : 1);
+    hipLaunchKernelGGL(ddh::build_kernel, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0,
+                       (hipStream_t)stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "build_kernel launch");
+    return DDH_OK;
 }
 
 extern "C" int ddh_block_sweep(int32_t n, int32_t NP, int32_t b, int32_t k, const void *A,

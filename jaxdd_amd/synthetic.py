@@ -116,6 +116,11 @@ def table_digest(table: dict) -> str:
 def device_from_row(t: dict, k: int, NP: int = 400, ions: bool = False, frozen: bool = False):
     """Device ``k`` of a drawn table: ions in the perovskite if ``ions``; with ``frozen`` a
     1.2 V pre-bias after which the ions are held."""
+    return device_arrays(*row_inputs(t, k, NP, ions, frozen))
+
+
+def row_inputs(t: dict, k: int, NP: int = 400, ions: bool = False, frozen: bool = False):
+    """(structure, numbers) of device ``k`` of a drawn table (`device_from_row`'s inputs)."""
     L = np.array([t["L_etl"][k], t["L_pvk"][k], t["L_htl"][k]])
     mu = t["mu"][k]
     dop = t["dop"][k]
@@ -132,8 +137,7 @@ def device_from_row(t: dict, k: int, NP: int = 400, ions: bool = False, frozen: 
                  mu_cation=np.array([0.0, ION_MOBILITY, 0.0]))
     if frozen:
         p.update(V_pre=1.2)
-    st = _structure(L, NP, ions=ions, pre_cond=frozen, fix_ions=frozen)
-    return device_arrays(st, p)
+    return _structure(L, NP, ions=ions, pre_cond=frozen, fix_ions=frozen), p
 
 
 def devices(config: str, n: int, seed: int = SEED, variant: str = "free", start: int = 0):

@@ -8,6 +8,7 @@
 #include "ddhip_core.h"
 #include "ddhip_adjoint.h"
 #include "ddhip_assemble.h"
+#include "ddhip_build.h"
 #include "ddhip_impedance.h"
 #include "ddhip_pair.h"
 #include "ddhip_quad.h"
@@ -415,5 +416,34 @@ extern "C" int emu_transient(int n, int NP, int nseg, int nt, const double *pt, 
             while (!S.done) tr_newton<3>(a, lane, S);
         }
     }
+    return 0;
+}
+
+// The builder's lanes in turn (one per device and direction; k = -1 without directions).
+extern "C" int emu_build(int n, int NP, int nl, int nseg, int K, const uint8_t *lid,
+                         const uint8_t *sid, const uint8_t *flags, const int32_t *i0,
+                         const double *unit, const double *unit_sum, const double *layers,
+                         const double *scalars, const double *dlayers, const double *dscalars,
+                         double *pt, double *seg, double *sc, double *imu, double *dpt,
+                         double *dseg, double *dsc) {
+    BuildArgs a = {};
+    a.n = n; a.NP = NP; a.nl = nl; a.nseg = nseg; a.K = K;
+    a.lid = lid; a.sid = sid; a.flags = flags; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum;
+    a.layers = layers; a.scalars = scalars; a.dlayers = dlayers; a.dscalars = dscalars;
+    a.pt = pt; a.seg = seg; a.sc = sc; a.imu = imu; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
+    for (int k = (K > 0 ? 0 : -1); k < (K > 0 ? K : 0); ++k)
+        for (int d = 0; d < n; ++d) build_lane(a, d, k);
+    return 0;
+}
+
+// The layout's lanes in turn.
+extern "C" int emu_graded_layout(int n, int NP, int nl, const double *L, double beta, double d0,
+                                 uint8_t *lid, uint8_t *sid, int32_t *i0, double *unit,
+                                 double *unit_sum, uint8_t *iface, int32_t *nseg) {
+    LayoutArgs a = {};
+    a.n = n; a.NP = NP; a.nl = nl; a.beta = beta; a.d0 = d0; a.L = L;
+    a.lid = lid; a.sid = sid; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum; a.iface = iface;
+    a.nseg = nseg;
+    for (int d = 0; d < n; ++d) layout_lane(a, d);
     return 0;
 }

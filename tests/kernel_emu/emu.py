@@ -42,6 +42,8 @@ def lib():
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
         _lib.emu_xcd_hits.argtypes = [i, i, vp]
+        _lib.emu_build.argtypes = [i, i, i, i, i] + [vp] * 17
+        _lib.emu_graded_layout.argtypes = [i, i, i, vp, ctypes.c_double, ctypes.c_double] + [vp] * 7
         _lib.emu_transient.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        ctypes.c_double, i, i, i, i, i, vp, vp, vp, vp, vp, vp]
     return _lib
@@ -202,3 +204,46 @@ def transient_packed(packed, t, V, G, y0, tol=1e-10, max_iter=200, bern="exact",
                         failure_mode, int(ions), _p(J), _p(Jext), _p(ok), _p(its), _p(kept),
                         _p(status))
     return J, Jext, ok.astype(bool), its, kept.astype(bool), status
+
+
+def build_packed(structures, params, directions=None):
+    """The GPU builder's lanes on the CPU (jaxdd_amd.builder's arrays): the packed arrays
+    {"pt", "seg", "sc", "imu", "sid", "nseg"} and, with ``directions`` (dlayers, dscalars), the
+    tangents {"pt", "seg", "sc"}."""
+    from jaxdd_amd import builder as B
+    from jaxdd_amd.batch import SC_FIELDS
+
+    sa = B.structure_arrays(structures)
+    layers, scalars = B.number_arrays(structures, params)
+    n, NP, nl, nseg = len(structures), sa["NP"], sa["nl"], sa["nseg"]
+    P = NP + 2
+    K = 0 if directions is None else np.asarray(directions[0]).shape[0]
+    pt, seg = np.zeros((P, 4, n)), np.zeros((nseg, 22, n))
+    sc, imu = np.zeros((len(SC_FIELDS), n)), np.zeros((nseg, 2, n))
+    dpt, dseg = np.zeros((max(K, 1), P, 4, n)), np.zeros((max(K, 1), nseg, 22, n))
+    dsc = np.zeros((max(K, 1), len(SC_FIELDS), n))
+    dl = np.ascontiguousarray(directions[0], dtype=float) if K else np.zeros(1)
+    ds = np.ascontiguousarray(directions[1], dtype=float) if K else np.zeros(1)
+    lib().emu_build(n, NP, nl, nseg, K, _p(sa["lid"]), _p(sa["sid"]), _p(sa["flags"]),
+                    _p(sa["i0"]), _p(sa["unit"]), _p(sa["unit_sum"]), _p(layers), _p(scalars),
+                    _p(dl), _p(ds), _p(pt), _p(seg), _p(sc), _p(imu), _p(dpt), _p(dseg), _p(dsc))
+    out = {"pt": pt, "seg": seg, "sc": sc, "imu": imu, "sid": sa["sid"], "nseg": nseg}
+    if K:
+        return out, {"pt": dpt, "seg": dseg, "sc": dsc}
+    return out
+
+
+def graded_layout_packed(L, NP, beta, d0):
+    """The GPU layout's lanes on the CPU: L (n, nl) thicknesses -> dict of lid, sid (NP + 2, n),
+    i0 (nl, n), unit (NP + 2, n), unit_sum (nl, n), iface (NP + 1, n), nseg (n)."""
+    L = np.ascontiguousarray(np.asarray(L, dtype=float).T)
+    nl, n = L.shape
+    P = NP + 2
+    out = {"lid": np.zeros((P, n), np.uint8), "sid": np.zeros((P, n), np.uint8),
+           "i0": np.zeros((nl, n), np.int32), "unit": np.zeros((P, n)),
+           "unit_sum": np.zeros((nl, n)), "iface": np.zeros((P - 1, n), np.uint8),
+           "nseg": np.zeros(n, np.int32)}
+    lib().emu_graded_layout(n, NP, nl, _p(L), float(beta), float(d0),
+                            *(_p(out[k]) for k in ("lid", "sid", "i0", "unit", "unit_sum",
+                                                    "iface", "nseg")))
+    return out

@@ -1,0 +1,220 @@
+"""The batch built on the GPU (jaxdd_amd.builder, ddh_build; reference: src/jaxdd/device.py:877-1063
+`device_arrays`, src/jaxdd/batch.py:142-223): its lanes' code on the CPU against the host
+builder's packed arrays, value by value, and its forward-mode tangents against central
+differences of the host builder on each device's structure; the GPU against both."""
+
+import numpy as np
+import pytest
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rows(n, NP, ions=False, frozen=False):
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(n)
+    return [S.row_inputs(t, k, NP, ions, frozen) for k in range(n)]
+
+
+def _same_packed(got, ref, rtol=1e-14):
+    """The builder's arrays against `batch.pack`'s: the segments and every field, each to rtol
+    of its largest magnitude over the batch (as packed fields span 40 decades)."""
+    assert got["nseg"] == ref["nseg"]
+    assert np.array_equal(np.asarray(got["sid"]), ref["sid"])
+    for key in ("pt", "seg", "sc", "imu"):
+        g, r = np.asarray(got[key]), ref[key]
+        axis = tuple(i for i in range(r.ndim) if i != r.ndim - 2)  # per field
+        scale = np.max(np.abs(r), axis=axis, keepdims=True)
+        assert np.all(np.abs(g - r) <= rtol * np.where(scale > 0, scale, 1.0)), key
+
+
+@pytest.mark.parametrize("config", ["c2", "c3", "c3-frozen", "theta"])
+def test_replay_builds_the_host_batch(emu, config):
+    """The builder's lanes on the CPU give `batch.pack`'s arrays of the host builder's devices,
+    to 1e-14 of each field's scale: the workloads' stacks without ions, with mobile ions, with
+    ions held after a pre-bias, and the gradient workload's parameter vectors."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+    from jaxdd_amd.device import device_arrays
+
+    if config == "theta":
+        th = A.theta_from_table(S.draw(6), range(6))
+        inputs = [(A.structure_of(t, 120), A.params_of(t)) for t in th]
+    else:
+        inputs = _rows(6, 120, ions=config != "c2", frozen=config == "c3-frozen")
+    structs, params = zip(*inputs)
+    ref = pack([device_arrays(st, p) for st, p in inputs])
+    _same_packed(emu.build_packed(list(structs), list(params)), ref)
+
+
+def test_replay_tangents_are_the_host_builders_differences(emu):
+    """The forward-mode tangents of the 16 parameters against `adjoint.tangents`' central
+    differences (relative step 1e-6) of the host builder on each device's structure, in the
+    units the differences resolve: d(field)/d ln(theta_k) (d/d W_L for the work function's
+    offset) to 1e-8 of the field's own scale over the batch. A field that hardly depends on a
+    parameter (the current scale on a contact layer's thickness: d ln J/d ln L ~ 1e-7) is known
+    to the differences only to their round-off, 1e-10 of the field."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(5), range(5))
+    structs = [A.structure_of(t, 100) for t in th]
+    out, tan = emu.build_packed(structs, [A.params_of(t) for t in th], A.directions_of(th))
+    base, ref = A.tangents(th, NP=100)
+    unit = np.abs(th)
+    unit[:, 15] = 1.0
+    for key in ("pt", "seg", "sc"):
+        g, r, v = tan[key], ref[key], base[key]
+        axis = tuple(i for i in range(v.ndim) if i != v.ndim - 2)
+        scale = np.max(np.abs(v), axis=axis, keepdims=True)  # each field's own scale
+        for k in range(len(A.THETA)):
+            err = np.abs(g[k] - r[k]) * unit[:, k]
+            assert np.all(err <= 1e-8 * scale + 1e-300), (key, A.THETA[k])
+            # and the tangents are not all zero where the differences are not
+            assert np.all((np.abs(r[k]) * unit[:, k] > 1e-6 * scale) <= (g[k] != 0)), (key, A.THETA[k])
+
+
+def test_replay_layout_is_the_host_graded_layout(emu):
+    """The GPU layout's lanes on the CPU against the host's graded rule (`grid.graded_layout`)
+    for 400 drawn stacks: every layer's point count and first point, the segments and the
+    interfaces exactly; each layer's spacing sum to 1e-15; the spacings to 1e-10 of themselves
+    (each point stops its Newton steps on its own, the host a layer's points together: the
+    positions agree to their last bits, which the smallest spacings magnify)."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import builder as B
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.grid import GRADED_POINTS_PER_EFOLD, GRADED_SMALLEST
+
+    n = 400
+    th = A.theta_from_table(S.draw(n), range(n))
+    out = emu.graded_layout_packed(th[:, 0:3], 400, GRADED_POINTS_PER_EFOLD, GRADED_SMALLEST)
+    for k in range(n):
+        st = A.structure_of(th[k])
+        assert tuple(out["i0"][:, k]) == tuple(st.i0)
+        assert np.array_equal(out["lid"][:, k], st.lid)
+        assert np.array_equal(out["sid"][:, k], B.segment_ids(st))
+        assert out["nseg"][k] == B.segment_ids(st)[-1] + 1
+        iface = np.zeros(401, np.uint8)
+        iface[list(st.i1[:-1])] = 1
+        assert np.array_equal(out["iface"][:, k], iface)
+        assert np.max(np.abs(out["unit_sum"][:, k] / st.norm - 1)) < 1e-15
+        assert np.max(np.abs(out["unit"][:, k] / st.raw - 1)) < 1e-10
+
+
+def test_vectorised_numbers_are_the_devices_numbers():
+    """`adjoint.numbers_of` (all devices at once) is `builder.number_arrays` of `params_of`
+    (one device at a time), exactly."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import builder as B
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(7), range(7))
+    th[:, 15] = np.linspace(-0.02, 0.03, 7)
+    structs = [A.structure_of(t, 60) for t in th]
+    got = A.numbers_of(th)
+    ref = B.number_arrays(structs, [A.params_of(t) for t in th])
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    flags = B.structure_arrays(structs)["flags"]
+    assert np.array_equal(flags, np.broadcast_to(np.array(A.STACK_FLAGS, np.uint8)[:, None], flags.shape))
+
+
+def test_the_builder_refuses_what_it_does_not_build():
+    """A generation profile, flat-band contacts, optional modules: before any launch."""
+    import dataclasses
+
+    from jaxdd_amd import builder as B
+
+    st, _ = _rows(1, 40)[0]
+    B.check_buildable(st)
+    for change, match in ((dict(gen_mode="file"), "profile"), (dict(sfb=(True, False)), "flat-band"),
+                          (dict(extra={"langevin": True}), "optional")):
+        with pytest.raises(NotImplementedError, match=match):
+            B.check_buildable(dataclasses.replace(st, **change))
+
+
+@pytest.mark.gpu
+def test_gpu_build_is_the_host_batch_and_its_tangents(torch_cuda, emu):
+    """ddh_build through the C-ABI: the packed arrays of 300 gradient-workload devices (more
+    than a wavefront of lanes per direction) against the host builder's to 1e-14, and the
+    tangents against the CPU replay's to 1e-13."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+
+    n = 300
+    th = A.theta_from_table(S.draw(n), range(n))
+    print(f"ddh_build: {n} devices x 16 directions", flush=True)
+    structs = [A.structure_of(t, 120) for t in th]
+    params = [A.params_of(t) for t in th]
+    batch, tan = A.gpu_tangents(th, NP=120, structs=structs)
+    from jaxdd_amd.device import device_arrays
+
+    ref = pack([device_arrays(st, p) for st, p in zip(structs, params)])
+    got = {"pt": _np(batch["_pt"]), "seg": _np(batch["_seg"]), "sc": _np(batch["_sc"]),
+           "imu": _np(batch["_imu"]), "sid": _np(batch["_sid"]), "nseg": batch["nseg"]}
+    _same_packed(got, ref)
+    _, rtan = emu.build_packed(structs, params, A.directions_of(th))
+    for key in ("pt", "seg", "sc"):
+        g, r = _np(tan[key]), rtan[key]
+        scale = np.max(np.abs(r)) + 1e-300
+        assert np.max(np.abs(g - r)) <= 1e-13 * scale, key
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_from_built_tangents(torch_cuda):
+    """The gradients of 64 gradient-workload devices with the tangents built on the GPU against
+    those with the host builder's central differences: the same curves and flags, and
+    dJ/dln(theta_k) to 1e-5 of each curve's current scale, the gradient tests' bar: the
+    differences' tangents are 1e-10 off in places, and near open circuit the adjoint's two
+    terms nearly cancel and lift that to 3e-6 on one device (1e-7 on most)."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import j_scale
+
+    n, NP = 64, 120
+    th = A.theta_from_table(S.draw(n), range(n))
+    V = S.VOLTAGES[::4].copy()
+    print(f"gradients from built tangents: {n} devices, NP = {NP}", flush=True)
+    batch, tan = A.gpu_tangents(th, NP=NP)
+    J1, ok1, _, g1 = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200)
+    packed, htan = A.tangents(th, NP=NP)
+    J2, ok2, _, g2 = A.jv_gradients(packed, htan, V, tol=1e-12, max_iter=200)
+    J1, J2, g1, g2 = _np(J1), _np(J2), _np(g1), _np(g2)
+    assert np.array_equal(_np(ok1), _np(ok2)) and _np(ok1).mean() > 0.95
+    devs = [A.device_from_theta(t, NP) for t in th]
+    cscale = np.array([j_scale(d) for d in devs]) + np.max(np.abs(J2), axis=1)
+    assert np.max(np.abs(J1 - J2) / cscale[:, None]) < 1e-10
+    unit = np.abs(th)
+    unit[:, 15] = 1.0
+    err = np.abs(g1 - g2) * unit[:, None, :] / cscale[:, None, None]
+    assert np.max(err) < 1e-5, np.unravel_index(np.argmax(err), err.shape)
+    assert np.median(err) < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_graded_build_is_the_host_batch(torch_cuda, emu):
+    """The whole build on the GPU (ddh_graded_layout, then ddh_build) for 300 gradient-workload
+    devices against the host builder's packed arrays (to 1e-13 of each field's scale: the
+    layouts' last bits) and its tangents against the host-structure build's (1e-12)."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import pack
+    from jaxdd_amd.device import device_arrays
+
+    n = 300
+    th = A.theta_from_table(S.draw(n), range(n))
+    print(f"graded build on the GPU: {n} devices x 16 directions", flush=True)
+    batch, tan = A.gpu_tangents(th, NP=120)
+    structs = [A.structure_of(t, 120) for t in th]
+    ref = pack([device_arrays(st, A.params_of(t)) for st, t in zip(structs, th)])
+    got = {"pt": _np(batch["_pt"]), "seg": _np(batch["_seg"]), "sc": _np(batch["_sc"]),
+           "imu": _np(batch["_imu"]), "sid": _np(batch["_sid"]), "nseg": batch["nseg"]}
+    _same_packed(got, ref, rtol=1e-13)
+    assert np.array_equal(_np(batch["_iface"]), ref["iface"])
+    _, htan = A.gpu_tangents(th, NP=120, structs=structs)
+    for key in ("pt", "seg", "sc"):
+        g, r = _np(tan[key]), _np(htan[key])
+        assert np.max(np.abs(g - r)) <= 1e-12 * np.max(np.abs(r)), key

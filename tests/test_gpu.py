@@ -446,7 +446,11 @@ def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
     """The gradient workload's check at its own size (200 devices): per device, the adjoint's
     derivative along a random direction in the 16 parameters (one tangent direction) against
     central differences of the GPU's FP64 curves, every voltage where all three solves
-    converged, measured against the curve's current scale."""
+    converged, measured against the curve's current scale: every device to 2e-5, half of them
+    to 1e-6. Near open circuit the adjoint's two terms nearly cancel, so the derivative moves
+    with the converged state within the solve's tolerance: for device 0 at 1.225 V the adjoints
+    at the states of its one-, two- and four-lane solves differ by up to 7.6e-6 of the scale
+    (the kernels' CPU replay), and the differences of the curves lie 1.2e-5 from the GPU's."""
     from jaxdd_amd import adjoint as A
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import j_scale, pack, prepare, solve_batch
@@ -479,15 +483,19 @@ def test_adjoint_directional_derivatives_of_200_devices(torch_cuda):
     h = 1e-5
     curves = []
     for s_ in (h, -h):
+        # the kernel the adjoint's states came from (four lanes per device), so that the
+        # differences see the same Newton steps and stops
         Jx, okx, _ = solve_batch(prepare(devices_at(s_)), V, tol=1e-12, strict=True,
-                                 max_iter=200)
+                                 max_iter=200, lanes=4)
         curves.append((_np(Jx), _np(okx)))
     fd = (curves[0][0] - curves[1][0]) / (2 * h)
     both = ok & curves[0][1] & curves[1][1]
     assert both.mean() > 0.95
     scale = np.array([j_scale(dv) for dv in devs0]) + np.max(np.abs(_np(J)), axis=1)
     err = np.max(np.where(both, np.abs(grad - fd), 0.0), axis=1) / scale
-    assert np.max(err) < 1e-5, (np.argmax(err), np.max(err))
+    print(f"per-device error quantiles 50/90/100%: {np.quantile(err, [0.5, 0.9, 1.0])}", flush=True)
+    assert np.max(err) < 2e-5, (np.argmax(err), np.max(err))
+    assert np.median(err) < 1e-6
 
 
 def test_gradients_of_the_mixed_precision_solve(c4_gpu):
@@ -521,8 +529,9 @@ def test_autograd_function_backward_is_the_adjoint(c4_gpu):
     Jt, okt = A.curves(theta, V, tol=1e-12)
     w = torch.as_tensor(np.random.default_rng(2).normal(size=Jt.shape), dtype=torch.float64)
     (w * Jt).sum().backward()
-    # the same curves' adjoint at the default tolerance (its own linearisation points)
-    _, _, _, g = A.jv_gradients(packed, tan, V, tol=1e-12)
+    # the same curves' adjoint with the tangents built on the GPU, as the Function builds them
+    batch, gtan = A.gpu_tangents(th)
+    _, _, _, g = A.jv_gradients(batch, gtan, V, tol=1e-12)
     expect = np.einsum("nv,nvk->nk", w.numpy(), _np(g))
     got = theta.grad.numpy()
     assert np.allclose(got, expect, rtol=1e-9, atol=0.0)
