@@ -131,6 +131,22 @@ int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *s
                 const double *dsc, int32_t bernoulli, double *grad, int32_t *status,
                 void *workspace, size_t workspace_bytes, void *hip_stream);
 
+/* Workspace bytes ddh_adjoint_vjp needs. */
+size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t ntheta);
+
+/* The vector-Jacobian product of the curves in the parameters, as the reference's `jax.grad`
+ * of a loss of `solve_batch(device_tree(structure, params), V)` takes it (batch.py:142-148,
+ * 650-737): theta_bar [n][ntheta], theta_bar[d][k] = sum_v dJbar[d][v] dJ(V_v)/dtheta_k, with
+ * dJbar [n][nV] the loss's derivative in each curve's current. Arguments otherwise as
+ * ddh_adjoint (its gradient kept in the workspace and contracted on the device); status and
+ * the zero contribution of unconverged voltages as ddh_adjoint's (NaN for a device with
+ * mobile ions). workspace: ddh_adjoint_vjp_workspace_bytes(n, NP, nV, ntheta) bytes. */
+int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV, const double *states,
+                    const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
+                    const double *dsc, int32_t bernoulli, const double *dJbar,
+                    double *theta_bar, int32_t *status, void *workspace,
+                    size_t workspace_bytes, void *hip_stream);
+
 /* The batched block-tridiagonal sweep: replaces src/jaxdd/sweep.py:313-330 (`block_sweep`,
  * the custom_vmap operator over src/jaxdd/solver.py:1534-1560 `_block_thomas`). Solves
  * A[i] x[i-1] + B[i] x[i] + C[i] x[i+1] = d[i] for n systems of NP blocks b x b (b = 3 or 5)

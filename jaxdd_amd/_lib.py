@@ -48,7 +48,7 @@ SYMBOLS = ("ddh_abi_version", "ddh_last_error", "ddh_workspace_bytes", "ddh_solv
            "ddh_impedance_workspace_bytes", "ddh_impedance", "ddh_noalloc_scratch_bytes",
            "ddh_solve_batch_noalloc", "ddh_transient_workspace_bytes", "ddh_transient",
            "ddh_impedance_ions_workspace_bytes", "ddh_impedance_ions", "ddh_build",
-           "ddh_graded_layout")
+           "ddh_graded_layout", "ddh_adjoint_vjp_workspace_bytes", "ddh_adjoint_vjp")
 
 _lib = None
 
@@ -105,6 +105,11 @@ def load():
     lib.ddh_transient.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, vp, vp, i32, vp,
                                   ctypes.POINTER(TOpts), i32, vp, vp, vp, vp, vp, vp, vp,
                                   ctypes.c_size_t, vp]
+    lib.ddh_adjoint_vjp_workspace_bytes.restype = ctypes.c_size_t
+    lib.ddh_adjoint_vjp_workspace_bytes.argtypes = [i32, i32, i32, i32]
+    lib.ddh_adjoint_vjp.restype = ctypes.c_int
+    lib.ddh_adjoint_vjp.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, i32, vp, vp, vp, i32,
+                                    vp, vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_graded_layout.restype = ctypes.c_int
     lib.ddh_graded_layout.argtypes = [i32, i32, i32, vp, ctypes.c_double, ctypes.c_double, vp,
                                       vp, vp, vp, vp, vp, vp, vp]

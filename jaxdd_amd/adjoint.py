@@ -343,6 +343,96 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     return J, ok.bool(), its.long(), grad
 
 
+def directions_for(wrt, n: int, nl: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit directions in the builder's numbers: ``wrt`` names per-device numbers
+    (`builder.SCALAR_FIELDS`, e.g. "W_L") or (name, layer) pairs of per-layer numbers
+    (`builder.LAYER_FIELDS`, e.g. ("mu_n", 1)); returns (dlayers (K, nl, 21, n), dscalars
+    (K, 9, n))."""
+    from .builder import LAYER_FIELDS, SCALAR_FIELDS
+
+    K = len(wrt)
+    dl = np.zeros((K, nl, len(LAYER_FIELDS), n))
+    ds = np.zeros((K, len(SCALAR_FIELDS), n))
+    for k, w in enumerate(wrt):
+        if isinstance(w, str):
+            if w not in SCALAR_FIELDS:
+                raise ValueError(f"{w!r}: not one of the per-device numbers {SCALAR_FIELDS}")
+            ds[k, SCALAR_FIELDS.index(w)] = 1.0
+            continue
+        name, layer = w
+        if name not in LAYER_FIELDS or not 0 <= int(layer) < nl:
+            raise ValueError(f"{w!r}: not a per-layer number {LAYER_FIELDS} of a layer < {nl}")
+        dl[k, int(layer), LAYER_FIELDS.index(name)] = 1.0
+    return dl, ds
+
+
+def param_gradients(structures, params, wrt, voltages, tol=1e-10, max_iter=60, lanes=4,
+                    precision="fp64", device="cuda"):
+    """dJ(V)/d(number) of each device's curve in the numbers ``wrt`` (`directions_for`): the
+    counterpart of `jax.grad` over `device_tree(structure, params)` (reference:
+    src/jaxdd/batch.py:142-148, 650-737) for any of the numbers the GPU builder takes, each
+    device's structure held. ``structures``, ``params``: the devices' (`device_parameters`);
+    returns J (n, nV), converged, iterations and the gradient (n, nV, len(wrt)) as
+    `jv_gradients`."""
+    from .builder import build
+
+    dl, ds = directions_for(wrt, len(structures), len(structures[0].i0))
+    batch, tan = build(list(structures), list(params), (dl, ds), device=device)
+    return jv_gradients(batch, tan, voltages, tol=tol, max_iter=max_iter, lanes=lanes,
+                        precision=precision)
+
+
+def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="exact",
+           lanes=4):
+    """The loss's gradient in the parameters from its derivative in the curves: theta_bar
+    (n, K) = sum over voltages of dJbar (n, nV) times dJ/dtheta, contracted on the device
+    (ddh_adjoint_vjp); also J (n, nV) and converged. ``batch``, ``tan``: device-resident
+    (`gpu_tangents`, `builder.build`)."""
+    lib = _lib.load()
+    V = np.asarray(voltages, dtype=float)
+    if np.any(np.diff(V) <= 0):
+        raise ValueError("jv_vjp takes ascending voltages")
+    n, NP, nV = batch["n"], batch["NP"], len(V)
+    K = tan["pt"].shape[0]
+    dev = batch["_pt"].device
+    Jb = torch.as_tensor(np.asarray(dJbar, dtype=float), device=dev).contiguous()
+    if tuple(Jb.shape) != (n, nV):
+        raise ValueError(f"dJbar: ({n}, {nV})")
+    if lanes == 4:
+        check_four_lanes(batch)
+    opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), 0,
+                     0 if bernoulli_name == "exact" else 1, _lib.DDH_FP64, 0, int(lanes))
+    Vd = torch.as_tensor(V, device=dev)
+    J = torch.empty((n, nV), dtype=torch.float64, device=dev)
+    ok = torch.zeros((n, nV), dtype=torch.uint8, device=dev)
+    its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
+    nbytes = lib.ddh_workspace_bytes(n, NP, nV, _lib.DDH_FP64)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    sc = _sc_now(batch)
+    b = _lib.Batch(n, NP, batch["nseg"], 0, batch["_pt"].data_ptr(), batch["_sid"].data_ptr(),
+                   batch["_seg"].data_ptr(), sc.data_ptr())
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
+                                          J.data_ptr(), ok.data_ptr(), its.data_ptr(),
+                                          status.data_ptr(), states.data_ptr(), ws.data_ptr(),
+                                          nbytes, stream), "ddh_solve_batch_states")
+    tbar = torch.empty((n, K), dtype=torch.float64, device=dev)
+    astatus = torch.zeros(n, dtype=torch.int32, device=dev)
+    vbytes = lib.ddh_adjoint_vjp_workspace_bytes(n, NP, nV, K)
+    vws = torch.empty(vbytes, dtype=torch.uint8, device=dev)
+    _lib.check(lib.ddh_adjoint_vjp(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(),
+                                   ok.data_ptr(), K, tan["pt"].data_ptr(), tan["seg"].data_ptr(),
+                                   tan["sc"].data_ptr(), opts.bernoulli, Jb.data_ptr(),
+                                   tbar.data_ptr(), astatus.data_ptr(), vws.data_ptr(), vbytes,
+                                   stream), "ddh_adjoint_vjp")
+    torch.cuda.current_stream(dev).synchronize()
+    if int(astatus.max()) != 0:
+        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+    return tbar, J + batch["G_shunt"][:, None] * Vd[None, :], ok.bool()
+
+
 class _Curves(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, voltages, tol, max_iter):

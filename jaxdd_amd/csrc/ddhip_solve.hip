@@ -710,6 +710,53 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     return DDH_OK;
 }
 
+namespace ddh {
+// theta_bar[d][k] = sum_v dJbar[d][v] grad[d][v][k]: one thread per (device, direction)
+__global__ void __launch_bounds__(256) vjp_kernel(int n, int nV, int K, const double *dJbar,
+                                                  const double *grad, double *theta_bar) {
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (long)n * K) return;
+    const int d = (int)(gid / K), k = (int)(gid % K);
+    double s = 0.0;
+    for (int v = 0; v < nV; ++v)
+        s += dJbar[(size_t)d * nV + v] * grad[((size_t)d * nV + v) * K + k];
+    theta_bar[gid] = s;
+}
+}  // namespace ddh
+
+extern "C" size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t nV,
+                                                  int32_t ntheta) {
+    if (n <= 0 || NP < 2 || nV < 1 || ntheta < 1) return 0;
+    return align256(ddh_adjoint_workspace_bytes(n, NP, nV)) +
+           (size_t)n * nV * ntheta * sizeof(double);
+}
+
+extern "C" int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV,
+                               const double *states, const uint8_t *ok, int32_t ntheta,
+                               const double *dpt, const double *dseg, const double *dsc,
+                               int32_t bernoulli, const double *dJbar, double *theta_bar,
+                               int32_t *status, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    if (!b || !dJbar || !theta_bar || !workspace)
+        return fail(DDH_E_ARG, "ddh_adjoint_vjp: NULL argument");
+    if (b->n <= 0 || nV < 1 || ntheta < 1)
+        return fail(DDH_E_SHAPE, "ddh_adjoint_vjp: n > 0, nV >= 1, ntheta >= 1");
+    const size_t need = ddh_adjoint_vjp_workspace_bytes(b->n, b->NP, nV, ntheta);
+    if (need == 0 || workspace_bytes < need)
+        return fail(DDH_E_WORKSPACE, "ddh_adjoint_vjp: workspace smaller than ddh_adjoint_vjp_workspace_bytes");
+    const size_t abytes = align256(ddh_adjoint_workspace_bytes(b->n, b->NP, nV));
+    double *grad = (double *)((char *)workspace + abytes);
+    const int rc = ddh_adjoint(b, V, nV, states, ok, ntheta, dpt, dseg, dsc, bernoulli, grad,
+                               status, workspace, abytes, stream);
+    if (rc != DDH_OK) return rc;
+    const long threads = (long)b->n * ntheta;
+    hipLaunchKernelGGL(ddh::vjp_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, b->n, nV, ntheta, dJbar, grad, theta_bar);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "vjp_kernel launch");
+    return DDH_OK;
+}
+
 extern "C" int ddh_assemble(const ddh_batch *b, const double *y, const double *Vint,
                             const double *logs, int32_t bernoulli, double *F, double *A,
                             double *B, double *C, double *U, double *J, void *stream) {

@@ -218,3 +218,48 @@ def test_gpu_graded_build_is_the_host_batch(torch_cuda, emu):
     for key in ("pt", "seg", "sc"):
         g, r = _np(tan[key]), _np(htan[key])
         assert np.max(np.abs(g - r)) <= 1e-12 * np.max(np.abs(r)), key
+
+
+def test_directions_in_named_numbers_are_the_parameters_directions():
+    """`adjoint.directions_for` names the same directions as the 16 parameters where they are
+    single numbers: the perovskite's electron mobility, the ETL's thickness, the left work
+    function, the HTL's acceptors."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    th = A.theta_from_table(S.draw(3), range(3))
+    dl, ds = A.directions_for([("mu_n", 1), ("L", 0), "W_L", ("N_A", 2)], 3, 3)
+    tl, ts = A.directions_of(th)
+    for k, j in enumerate((5, 0, 15, 11)):
+        assert np.array_equal(dl[k], tl[j]) and np.array_equal(ds[k], ts[j])
+    with pytest.raises(ValueError, match="per-layer"):
+        A.directions_for([("mu_x", 0)], 3, 3)
+    with pytest.raises(ValueError, match="per-device"):
+        A.directions_for(["nope"], 3, 3)
+
+
+@pytest.mark.gpu
+def test_gpu_named_gradients_and_the_vjp(torch_cuda):
+    """`param_gradients` in named numbers equals the parameters' gradient in the same
+    directions (the same build, solve and adjoint), and ddh_adjoint_vjp's contraction on the
+    device equals the dense gradient contracted on the host (1e-12)."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    n, NP = 40, 100
+    th = A.theta_from_table(S.draw(n), range(n))
+    V = S.VOLTAGES[::5].copy()
+    structs = [A.structure_of(t, NP) for t in th]
+    print(f"named gradients and the VJP: {n} devices", flush=True)
+    J, ok, _, g = A.param_gradients(structs, [A.params_of(t) for t in th],
+                                    [("mu_n", 1), "W_L"], V, tol=1e-12, max_iter=200)
+    batch, tan = A.gpu_tangents(th, NP=NP, structs=structs)
+    J2, ok2, _, g2 = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200)
+    g, g2 = _np(g), _np(g2)
+    assert np.array_equal(_np(ok), _np(ok2))
+    np.testing.assert_allclose(g[..., 0], g2[..., 5], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g[..., 1], g2[..., 15], rtol=1e-12, atol=0)
+    w = np.random.default_rng(3).normal(size=(n, len(V)))
+    tbar, J3, ok3 = A.jv_vjp(batch, tan, V, w, tol=1e-12, max_iter=200)
+    expect = np.einsum("nv,nvk->nk", w, g2)
+    np.testing.assert_allclose(_np(tbar), expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max())
