@@ -334,7 +334,8 @@ DDH_BFN void build_lane(const BuildArgs &a, int d, int k) {
 // F over the stack (at least five points beyond its first, the largest layer giving back what
 // the minimum added), and its points equidistribute F: u_i with F(u_i) = F(L_k) i/(m + 1),
 // found by Newton steps kept inside a shrinking bracket (a bisection step where Newton would
-// leave it), each point until its step no longer moves it.
+// leave it), each point until its step no longer moves it. Three passes: the structure (one
+// lane per device), every spacing (one lane per point), each layer's sum (one lane per device).
 
 constexpr int GRADED_MIN_POINTS = 5;
 constexpr int GRADED_NEWTON_STEPS = 80;
@@ -375,26 +376,33 @@ DDH_BFN double graded_invert(double t, double t_end, double Lk, double H, double
     return u;
 }
 
-DDH_BFN void layout_lane(const LayoutArgs &a, int d) {
+// A device's layer weights and counts: its thicknesses, H, each layer's share w_k of the
+// density's integral and its first and last point.
+struct LayoutDev {
+    double L[BUILD_MAX_LAYERS], w[BUILD_MAX_LAYERS], H;
+    int starts[BUILD_MAX_LAYERS], ends[BUILD_MAX_LAYERS];
+    bool fits;
+};
+
+DDH_BFN void layout_device(const LayoutArgs &a, int d, LayoutDev &D) {
     const size_t n = (size_t)a.n;
     const int NP = a.NP, nl = a.nl, P = NP + 2;
-    double L[BUILD_MAX_LAYERS] = {}, w[BUILD_MAX_LAYERS] = {};
-    int counts[BUILD_MAX_LAYERS] = {}, starts[BUILD_MAX_LAYERS] = {}, ends[BUILD_MAX_LAYERS] = {};
+    int counts[BUILD_MAX_LAYERS] = {};
     double Ls = 0.0;
     for (int k = 0; k < nl; ++k) {
-        L[k] = a.L[(size_t)k * n + d];
-        Ls += L[k];
+        D.L[k] = a.L[(size_t)k * n + d];
+        Ls += D.L[k];
     }
-    const double H = Ls / NP;
+    D.H = Ls / NP;
     double wsum = 0.0;
     for (int k = 0; k < nl; ++k) {
         double dF;
-        graded_integral(L[k], L[k], H, a.beta, a.d0, w[k], dF);
-        wsum += w[k];
+        graded_integral(D.L[k], D.L[k], D.H, a.beta, a.d0, D.w[k], dF);
+        wsum += D.w[k];
     }
     int total = 0, big = 0;
     for (int k = 0; k < nl; ++k) {
-        const int c = (int)floor((NP + 2 - nl) * w[k] / wsum);
+        const int c = (int)floor((NP + 2 - nl) * D.w[k] / wsum);
         counts[k] = c > GRADED_MIN_POINTS ? c : GRADED_MIN_POINTS;
         total += counts[k];
         if (counts[k] > counts[big]) big = k;
@@ -403,37 +411,30 @@ DDH_BFN void layout_lane(const LayoutArgs &a, int d) {
     if (over > 0) counts[big] -= over;
     int start = 0;
     for (int k = 0; k < nl; ++k) {
-        starts[k] = start;
-        ends[k] = k == nl - 1 ? P - 1 : start + counts[k];
-        start = ends[k] + 1;
-        a.i0[(size_t)k * n + d] = starts[k];
+        D.starts[k] = start;
+        D.ends[k] = k == nl - 1 ? P - 1 : start + counts[k];
+        start = D.ends[k] + 1;
     }
-    if (ends[nl - 1] < starts[nl - 1]) {  // the counts overfill the grid
+    D.fits = D.ends[nl - 1] >= D.starts[nl - 1];
+}
+
+// Pass 1, one lane per device: each layer's first point, each point's layer, the segments
+// (a new one wherever a packed field may change), the interfaces.
+DDH_BFN void layout_structure(const LayoutArgs &a, int d) {
+    const size_t n = (size_t)a.n;
+    const int nl = a.nl, P = a.NP + 2;
+    LayoutDev D;
+    layout_device(a, d, D);
+    for (int k = 0; k < nl; ++k) a.i0[(size_t)k * n + d] = D.starts[k];
+    if (!D.fits) {
         a.nseg[d] = -1;
         return;
     }
-    for (int k = 0; k < nl; ++k) {
-        const int m = ends[k] - starts[k];
-        const double t_end = w[k] * (m + 1) / (m + 1);
-        double prev = 0.0, usum = 0.0;
-        for (int i = 1; i <= m + 1; ++i) {
-            const double u = i == m + 1 ? L[k]
-                                        : graded_invert(w[k] * i / (m + 1), t_end, L[k], H,
-                                                        a.beta, a.d0);
-            const int j = starts[k] + i - 1;
-            a.unit[(size_t)j * n + d] = u - prev;
-            a.lid[(size_t)j * n + d] = (uint8_t)k;
-            usum += u - prev;
-            prev = u;
-        }
-        a.unit_sum[(size_t)k * n + d] = usum;
-    }
-    // the segments (a new one wherever a packed field may change) and the interfaces
-    int s = 0, key_prev = -1;
+    int s = 0, key_prev = -1, k = 0;
     for (int j = 0; j < P; ++j) {
-        const int k = a.lid[(size_t)j * n + d];
-        bool crossing = false;
-        for (int q = 0; q < nl - 1; ++q) crossing = crossing || j == ends[q];
+        while (k < nl - 1 && j > D.ends[k]) ++k;
+        a.lid[(size_t)j * n + d] = (uint8_t)k;
+        const bool crossing = k < nl - 1 && j == D.ends[k];
         const int key = k * 8 + (j == 0 ? 1 : 0) + (j == P - 1 ? 2 : 0) + (crossing ? 4 : 0);
         if (j > 0 && key != key_prev) ++s;
         key_prev = key;
@@ -441,6 +442,34 @@ DDH_BFN void layout_lane(const LayoutArgs &a, int d) {
         if (j < P - 1) a.iface[(size_t)j * n + d] = crossing ? 1 : 0;
     }
     a.nseg[d] = s + 1;
+}
+
+// Pass 2, one lane per (device, point j): the spacing after point j, the difference of the
+// two positions around it (the layer's first position 0, its last its thickness).
+DDH_BFN void layout_point(const LayoutArgs &a, int d, int j) {
+    if (a.nseg[d] < 0) return;
+    const size_t n = (size_t)a.n;
+    LayoutDev D;
+    layout_device(a, d, D);
+    const int k = a.lid[(size_t)j * n + d];
+    const int m = D.ends[k] - D.starts[k], i = j - D.starts[k] + 1;
+    const double t_end = D.w[k] * (m + 1) / (m + 1);
+    auto position = [&](int q) {
+        if (q == 0) return 0.0;
+        if (q == m + 1) return D.L[k];
+        return graded_invert(D.w[k] * q / (m + 1), t_end, D.L[k], D.H, a.beta, a.d0);
+    };
+    a.unit[(size_t)j * n + d] = position(i) - position(i - 1);
+}
+
+// Pass 3, one lane per device: each layer's spacing sum.
+DDH_BFN void layout_sums(const LayoutArgs &a, int d) {
+    if (a.nseg[d] < 0) return;
+    const size_t n = (size_t)a.n;
+    const int nl = a.nl, P = a.NP + 2;
+    double sum[BUILD_MAX_LAYERS] = {};
+    for (int j = 0; j < P; ++j) sum[a.lid[(size_t)j * n + d]] += a.unit[(size_t)j * n + d];
+    for (int k = 0; k < nl; ++k) a.unit_sum[(size_t)k * n + d] = sum[k];
 }
 
 }  // namespace ddh

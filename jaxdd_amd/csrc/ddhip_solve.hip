@@ -261,10 +261,21 @@ __global__ void __launch_bounds__(64) build_kernel(BuildArgs a) {
     build_lane(a, (int)(gid % a.n), a.K > 0 ? (int)(gid / a.n) : -1);
 }
 
-// The graded grid's layout of every device (ddhip_build.h): one lane per device.
-__global__ void __launch_bounds__(64) layout_kernel(LayoutArgs a) {
+// The graded grid's layout of every device (ddhip_build.h): the structure and the layers' sums
+// one lane per device, the spacings one lane per (device, point), devices innermost.
+__global__ void __launch_bounds__(64) layout_structure_kernel(LayoutArgs a) {
     const int d = blockIdx.x * 64 + threadIdx.x;
-    if (d < a.n) layout_lane(a, d);
+    if (d < a.n) layout_structure(a, d);
+}
+
+__global__ void __launch_bounds__(256) layout_point_kernel(LayoutArgs a) {
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid < (long)a.n * (a.NP + 2)) layout_point(a, (int)(gid % a.n), (int)(gid / a.n));
+}
+
+__global__ void __launch_bounds__(64) layout_sums_kernel(LayoutArgs a) {
+    const int d = blockIdx.x * 64 + threadIdx.x;
+    if (d < a.n) layout_sums(a, d);
 }
 
 // The transient (ddhip_transient.h): one lane per device through all of its time steps, one
@@ -899,10 +910,15 @@ extern "C" int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double
     a.n = n; a.NP = NP; a.nl = nl; a.beta = beta; a.d0 = d0; a.L = L;
     a.lid = lid; a.sid = sid; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum; a.iface = iface;
     a.nseg = nseg;
-    hipLaunchKernelGGL(ddh::layout_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
-                       (hipStream_t)stream, a);
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 dev_grid((unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL(ddh::layout_structure_kernel, dev_grid, dim3(64), 0, s, a);
+    const long points = (long)n * (NP + 2);
+    hipLaunchKernelGGL(ddh::layout_point_kernel, dim3((unsigned)((points + 255) / 256)), dim3(256),
+                       0, s, a);
+    hipLaunchKernelGGL(ddh::layout_sums_kernel, dev_grid, dim3(64), 0, s, a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "layout_kernel launch");
+    if (e != hipSuccess) return hip_fail(e, "layout kernels launch");
     return DDH_OK;
 }
 

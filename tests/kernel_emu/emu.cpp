@@ -444,6 +444,9 @@ extern "C" int emu_graded_layout(int n, int NP, int nl, const double *L, double 
     a.n = n; a.NP = NP; a.nl = nl; a.beta = beta; a.d0 = d0; a.L = L;
     a.lid = lid; a.sid = sid; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum; a.iface = iface;
     a.nseg = nseg;
-    for (int d = 0; d < n; ++d) layout_lane(a, d);
+    for (int d = 0; d < n; ++d) layout_structure(a, d);
+    for (int j = 0; j < NP + 2; ++j)
+        for (int d = 0; d < n; ++d) layout_point(a, d, j);
+    for (int d = 0; d < n; ++d) layout_sums(a, d);
     return 0;
 }
