@@ -117,15 +117,19 @@ int ddh_solve_batch_noalloc(const ddh_batch *b, const double *V, int32_t nV, con
 size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
 
 /* dJ(V)/dtheta at every converged voltage by the implicit function theorem: replaces
- * src/jaxdd/batch.py:650-737 (`_converged_jvp`, `_tangent`: the transposed bordered solve of
- * reverse mode) for devices without mobile ions. theta reaches the kernel as the tangents of
- * the packed batch arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][22][n],
- * dsc [ntheta][20][n] (d array / d theta_k). states and ok: from ddh_solve_batch_states on the
- * same batch and ascending V. grad: [n][nV][ntheta]; 0 where a voltage did not converge (the
- * reference's zero gradient there); NaN and status[d] = 1 where device d has mobile ions.
- * nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each device's arrays and tangents are staged
- * in the local data share). Up to 16 directions take one reverse pass over the parameters,
- * more one forward pass each. bernoulli: as ddh_opts. */
+ * src/jaxdd/batch.py:606-737 (`_transposed`, `_converged_jvp`, `_tangent`: the transposed
+ * bordered solve of reverse mode), free mobile ions included (each species' normalisation and
+ * number constraint border the system; the normalisations of a converged state follow from
+ * its potential in closed form). theta reaches the kernel as the tangents of the packed batch
+ * arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][22][n], dsc [ntheta][20][n]
+ * (d array / d theta_k; every field the steady rows, the current and the ions' constraints
+ * read, V_T and the generation's factor included). states and ok: from ddh_solve_batch_states
+ * on the same batch and ascending V. grad: [n][nV][ntheta], of the current without the shunt
+ * (the caller adds V d(1/R_shunt)/dtheta with the shunt's current); 0 where a voltage did not
+ * converge (the reference's zero gradient there); NaN and status[d] = 1 where device d's
+ * mobile ions are frozen at a pre-bias. nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each
+ * device's arrays and tangents are staged in the local data share). Up to 16 directions take
+ * one reverse pass over the parameters, more one forward pass each. bernoulli: as ddh_opts. */
 int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
                 const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
                 const double *dsc, int32_t bernoulli, double *grad, int32_t *status,
@@ -139,8 +143,8 @@ size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_
  * 650-737): theta_bar [n][ntheta], theta_bar[d][k] = sum_v dJbar[d][v] dJ(V_v)/dtheta_k, with
  * dJbar [n][nV] the loss's derivative in each curve's current. Arguments otherwise as
  * ddh_adjoint (its gradient kept in the workspace and contracted on the device); status and
- * the zero contribution of unconverged voltages as ddh_adjoint's (NaN for a device with
- * mobile ions). workspace: ddh_adjoint_vjp_workspace_bytes(n, NP, nV, ntheta) bytes. */
+ * the zero contribution of unconverged voltages as ddh_adjoint's (NaN for a device whose
+ * mobile ions are frozen at a pre-bias). workspace: ddh_adjoint_vjp_workspace_bytes(n, NP, nV, ntheta) bytes. */
 int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV, const double *states,
                     const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
                     const double *dsc, int32_t bernoulli, const double *dJbar,

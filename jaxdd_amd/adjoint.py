@@ -14,7 +14,11 @@ layer, their unscaled spacing) is the structure's, fixed at the device's paramet
 thicknesses move the points only by scaling each layer's spacing (src/jaxdd/device.py:225-233
 `_spacing`).
 The HIP adjoint kernel (`ddh_adjoint`) then linearises each converged voltage's residual and
-current in those directions and solves one transposed block-tridiagonal system per voltage.
+current in those directions and solves one transposed block-tridiagonal system per voltage;
+with free mobile ions that system is bordered by each species' normalisation and number
+constraint (src/jaxdd/batch.py:595-618 `_transposed`), and `param_gradients` takes any of the
+builder's named numbers, the ions' densities and the temperature among them. Ions frozen at a
+pre-bias are refused (`NotImplementedError`).
 
 `curves(theta, V)` is the differentiable entry point: a `torch.autograd.Function` whose
 forward solves the curves and whose backward contracts the incoming dL/dJ with dJ/dtheta.
@@ -265,17 +269,50 @@ def resident(packed, tan, device="cuda"):
     return batch, dtan
 
 
+def _lanes(batch, lanes):
+    """The solve's lanes per device: ``lanes``, or for None 4 where the four-lane kernel takes
+    the batch (no mobile ions, NP >= 16), else 2."""
+    if lanes is not None:
+        return lanes
+    try:
+        check_four_lanes(batch)
+    except ValueError:
+        return 2
+    return 4
+
+
+FROZEN_REFUSED = ("gradients of devices whose mobile ions are frozen at a pre-bias are outside "
+                  "this adjoint (the scan's fixed charge is the pre-bias's solution)")
+
+
+def _check_not_frozen(batch) -> None:
+    sc = batch["_sc"]
+    ions = (sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)
+    frozen = (sc[SC_FIELDS.index("pre_cond")] > 0) & (sc[SC_FIELDS.index("fix_ions")] > 0)
+    if bool((ions & frozen).any()):
+        raise NotImplementedError(FROZEN_REFUSED)
+
+
+def _shunt_tangents(dsc):
+    """(K, n): the tangents of G_shunt = 1/R_shunt. The shunt's current G_shunt V is added
+    outside the kernels (as the reference adds it to every voltage's J), and so its part of
+    dJ/dtheta, V dG_shunt/dtheta, here."""
+    return dsc[:, SC_FIELDS.index("G_shunt"), :]
+
+
 def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
                  bernoulli_name="exact", device="cuda", return_states=False, events=None,
-                 lanes=4, precision="fp64"):
+                 lanes=None, precision="fp64"):
     """J (n, nV), converged, iterations and dJ/dtheta (n, nV, 16) on the GPU for ascending
     voltages: the solve keeping each voltage's final state, then the adjoint kernel.
-    Unconverged voltages get zero gradients. ``packed``, ``tan``: host arrays (`tangents`), or
+    Unconverged voltages get zero gradients (but for the shunt's current G_shunt V, added to
+    every voltage's J). ``packed``, ``tan``: host arrays (`tangents`), or
     the device-resident pair `resident` returns. ``return_states``: also the linearisation
     points (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream
     around the solve kernel and around the adjoint kernel. ``lanes``: GPU lanes per device of
     the solve (4: the grid in quarters, the fastest for the batches gradients are taken of;
-    2; 1), as `jaxdd_amd.batch.solve_batch`'s. ``precision``: the solve's, as solve_batch's
+    2; 1; None: 4 where the four-lane kernel takes the batch, else 2), as
+    `jaxdd_amd.batch.solve_batch`'s. ``precision``: the solve's, as solve_batch's
     ("mixed": the Newton correction's factors in FP32; the residual, the states and the
     adjoint stay FP64, and the states converge to the same ``tol``)."""
     lib = _lib.load()
@@ -284,14 +321,13 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
         raise ValueError("jv_gradients takes ascending voltages")
     batch = packed if "_pt" in packed else from_packed(packed, device)
     n, NP, nV = batch["n"], batch["NP"], len(V)
+    lanes = _lanes(batch, lanes)
     if lanes not in (1, 2, 4):
         raise ValueError(f"lanes {lanes!r}: 1, 2 or 4")
     if precision not in _lib.PRECISIONS:
         raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
     prec = _lib.PRECISIONS[precision]
-    sc = batch["_sc"]
-    if bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any()):
-        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+    _check_not_frozen(batch)
     if lanes == 4:
         check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
@@ -335,9 +371,10 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
         events[3].record()
     torch.cuda.current_stream(dev).synchronize()
     if int(astatus.max()) != 0:
-        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+        raise NotImplementedError(FROZEN_REFUSED)
     del sc, ws, aws  # the launches that read them are done
     J = J + batch["G_shunt"][:, None] * Vd[None, :]
+    grad = grad + Vd[None, :, None] * _shunt_tangents(dsc).T[:, None, :]
     if return_states:
         return J, ok.bool(), its.long(), grad, states
     return J, ok.bool(), its.long(), grad
@@ -366,7 +403,7 @@ def directions_for(wrt, n: int, nl: int) -> tuple[np.ndarray, np.ndarray]:
     return dl, ds
 
 
-def param_gradients(structures, params, wrt, voltages, tol=1e-10, max_iter=60, lanes=4,
+def param_gradients(structures, params, wrt, voltages, tol=1e-10, max_iter=60, lanes=None,
                     precision="fp64", device="cuda"):
     """dJ(V)/d(number) of each device's curve in the numbers ``wrt`` (`directions_for`): the
     counterpart of `jax.grad` over `device_tree(structure, params)` (reference:
@@ -383,7 +420,7 @@ def param_gradients(structures, params, wrt, voltages, tol=1e-10, max_iter=60, l
 
 
 def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="exact",
-           lanes=4):
+           lanes=None):
     """The loss's gradient in the parameters from its derivative in the curves: theta_bar
     (n, K) = sum over voltages of dJbar (n, nV) times dJ/dtheta, contracted on the device
     (ddh_adjoint_vjp); also J (n, nV) and converged. ``batch``, ``tan``: device-resident
@@ -398,6 +435,8 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
     Jb = torch.as_tensor(np.asarray(dJbar, dtype=float), device=dev).contiguous()
     if tuple(Jb.shape) != (n, nV):
         raise ValueError(f"dJbar: ({n}, {nV})")
+    _check_not_frozen(batch)
+    lanes = _lanes(batch, lanes)
     if lanes == 4:
         check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), 0,
@@ -429,7 +468,8 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
                                    stream), "ddh_adjoint_vjp")
     torch.cuda.current_stream(dev).synchronize()
     if int(astatus.max()) != 0:
-        raise NotImplementedError("gradients of devices with mobile ions are outside this adjoint")
+        raise NotImplementedError(FROZEN_REFUSED)
+    tbar = tbar + (Jb @ Vd)[:, None] * _shunt_tangents(tan["sc"]).T
     return tbar, J + batch["G_shunt"][:, None] * Vd[None, :], ok.bool()
 
 

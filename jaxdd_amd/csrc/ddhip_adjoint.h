@@ -14,10 +14,12 @@
 // the current in dual numbers along the packed coefficients' tangents dc/dtheta_k, which
 // gives dF/dtheta_k|_y (dotted with lambda) and dJ/dtheta_k|_y.
 //
-// The parameters reach the kernel only as those tangents (the host differentiates the device
-// builder), so any parameter of the packed arrays can be differentiated. Devices with mobile
-// ions are not differentiated here (status 1, gradients NaN): their border and the frozen
-// ions' dependence on the pre-bias are outside this adjoint.
+// The parameters reach the kernel only as those tangents (the device builder's forward mode),
+// so any parameter of the packed arrays can be differentiated. With mobile ions the system is
+// bordered by each species' normalisation and number constraint, and the transposed solve
+// gives the constraints' multipliers mu beside lambda (adj_lambda): dJ/dtheta then also takes
+// -mu . dg/dtheta. Ions frozen at a pre-bias are not differentiated here (status 1, gradients
+// NaN): the scan's fixed charge then depends on the pre-bias's solution.
 #pragma once
 
 #include "ddhip_core.h"
@@ -36,6 +38,8 @@ DDH_FN D1 operator*(D1 a, D1 b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 DDH_FN D1 operator*(double a, D1 b) { return {a * b.v, a * b.d}; }
 DDH_FN D1 operator*(D1 a, double b) { return {a.v * b, a.d * b}; }
 DDH_FN D1 operator+(D1 a, double b) { return {a.v + b, a.d}; }
+DDH_FN D1 operator+(double a, D1 b) { return {a + b.v, b.d}; }
+DDH_FN D1 operator-(double a, D1 b) { return {a - b.v, -b.d}; }
 DDH_FN D1 operator-(D1 a, double b) { return {a.v - b, a.d}; }
 DDH_FN D1 operator/(D1 a, D1 b) {
     const double q = a.v / b.v;
@@ -66,10 +70,12 @@ struct AdjArgs {
     double *lam;            // [NP][3][ls]
     double *iu;             // [NP + 1][ls] the current's running integral of U
     double *ts;             // [NP + 2][2][ls] the current's suffix sums of the point weights
+    double *ion;            // [4][ls] the ions' normalisations l_a, l_c and their multipliers
     size_t ls;
     int passes;             // 1: one pass per direction even where a reverse pass would do
 };
-constexpr int ADJ_F = 21;
+// Per row: Mi (9), C (9), w (3), and with mobile ions each constraint's gamma (2 x 3).
+constexpr int ADJ_F = 27;
 // Directions contracted in one reverse pass (more go through one pass per direction), and the
 // points whose tangents the wavefront stages in LDS at a time for the contraction.
 constexpr int ADJ_MAX_K = 16;
@@ -81,9 +87,10 @@ DDH_MFN int adj_sid_words(int NP) { return (NP + 2 + 7) / 8; }
 constexpr int ADJ_SLOTS = 1024;
 
 // A point of a dual pass: its potential and densities (tangents only at the contacts, whose
-// values are parameters) and its spacing.
+// values are parameters), its mobile ions' densities (through V_T: the normalisations are
+// unknowns, held) and its spacing.
 struct PD {
-    D1 V, n, p, h;
+    D1 V, n, p, h, an, ca;
     int sid;
 };
 
@@ -147,7 +154,7 @@ DDH_FN D1 recomb_d1(D1 n, D1 p, const SegD &sd, int id) {
 }
 
 // The edge fluxes Fn, Fp over (l, r) along the direction (src/jaxdd/solver.py:1069-1108).
-DDH_FN void edge_d1(const PD &l, const PD &r, const SegD &sd, double Vt, double iVt, int bern,
+DDH_FN void edge_d1(const PD &l, const PD &r, const SegD &sd, D1 Vt, D1 iVt, int bern,
                     D1 &Fn, D1 &Fp) {
     const D1 lcn = Vt * sd(l.sid, SEG_MUN), lcp = Vt * sd(l.sid, SEG_MUP);
     const D1 xn = ((r.V + sd(r.sid, SEG_OFFN)) - (l.V + sd(l.sid, SEG_OFFN))) * iVt;
@@ -161,9 +168,9 @@ DDH_FN void edge_d1(const PD &l, const PD &r, const SegD &sd, double Vt, double 
     Fp = lcp * (Bpf * l.p - Bpb * r.p);
 }
 
-// Row m's scaled residual F along the direction (row_blocks' F; no mobile ions).
-DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, double gfr,
-                   const SegD &sd, double Vt, double iVt, int bern, D1 F[3]) {
+// Row m's scaled residual F along the direction (row_blocks' F).
+DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, D1 gfr,
+                   const SegD &sd, D1 Vt, D1 iVt, int bern, D1 F[3]) {
     const D1 hm = l.h, hp = m.h, hs = hp + hm, L2 = Ld * Ld;
     const D1 D = dc(1.0) / (hp * hm * hs * L2);
     const D1 fac_ = 0.5 * L2 * hp * hm * hs;
@@ -184,7 +191,7 @@ DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, d
     const D1 elec = hm * FnR - hp * FnL + fac_ * (gen - R);
     const D1 hole = hp * FpL - hm * FpR + fac_ * (gen - R);
     const D1 pois = fac_l * l.V + fac_m * m.V + fu * r.V -
-                    Q_E * (m.n - m.p + sd(m.sid, SEG_DOP) + fix);
+                    Q_E * (m.n - m.p + sd(m.sid, SEG_DOP) + fix + m.an - m.ca);
     F[0] = pois * sP;
     F[1] = elec * sC;
     F[2] = hole * sC;
@@ -239,8 +246,26 @@ DDH_FN void adj_stage_direction(const AdjArgs &a, int d, int k, ldouble *tab, in
 #define TS(j, f) a.ts[((size_t)(j) * 2 + (f)) * ls + sl]
 #define YS(j, f) a.states[(((size_t)v * NP + ((j) - 1)) * 3 + (f)) * n + d]
 
-// Device d at converged voltage v, by the launched lane sl: lambda (T^T lambda = dJ/dy) and
-// the state's V, n, p, into the lane's workspace.
+#define ION(f) a.ion[(size_t)(f) * ls + sl]
+
+// Whether device d's gradients are refused: its mobile ions frozen at a pre-bias (the scan's
+// fixed charge then depends on the pre-bias's solution, outside this adjoint).
+DDH_FN bool adj_refused(const AdjArgs &a, int d) {
+    const size_t n = (size_t)a.n;
+    const bool mobile = a.sc[(size_t)SC_AT * n + d] > 0 || a.sc[(size_t)SC_CT * n + d] > 0;
+    return mobile && a.sc[(size_t)SC_PRE * n + d] > 0 && a.sc[(size_t)SC_FIX * n + d] > 0;
+}
+
+// Device d at converged voltage v, by the launched lane sl: lambda and, with mobile ions, the
+// multipliers mu of their number constraints, and the state's V, n, p, into the lane's
+// workspace. With mobile ions the system is bordered (src/jaxdd/batch.py:595-618): unknowns y
+// and each species' normalisation l, Jacobian [T U; W^T D] (U = dF/dl, W = dg/dy, D = dg/dl
+// = 1 at a converged state), and its transpose [T^T W; U^T D] (lambda, mu) = (dJ/dy, 0) is
+//     mu = (D - U^T T^-T W)^-1 (-U^T T^-T dJ/dy),    lambda = T^-T (dJ/dy - W mu),
+// where U^T T^-T v = v^T X with X = T^-1 U. The one right-to-left elimination gives all of it:
+// X's d', the gammas of W (gamma_i = W_i - c'_{i+1}^T gamma_{i+1}, as w of dJ/dy), and by
+// sum_i v_i . x_i = sum_i gamma(v)_i . d'_i the contractions (dJ/dy)^T X and W^T X; lambda's
+// substitution then takes w - mu_a gamma_a - mu_c gamma_c.
 DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
     const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
     const int NP = a.NP;
@@ -250,18 +275,31 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
     const double Vint = a.V[v];
     const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
-    // the current's sums (the Newton sweep's order), the state's densities kept for the passes
+    const double tot0 = TSC(SC_AT), tot1 = TSC(SC_CT);
+    const bool ions = tot0 > 0 || tot1 > 0;  // (the same for every lane of the wavefront)
+    // the current's sums (the Newton sweep's order), the state's densities kept for the passes;
+    // with ions each species' number at l = 0 (the constraint's trapezoid, solver.py:1241-1265)
     const double Vg0n = V0c + TSEG(sid0, SEG_OFFN), Vg0p = V0c + TSEG(sid0, SEG_OFFP);
     double wn_prev = exp(-Vg0n * iVt) * TSEG(sid0, SEG_IMUN);
     double wp_prev = exp(Vg0p * iVt) * TSEG(sid0, SEG_IMUP);
     const double endn0 = exp(TSC(SC_LNL) - Vg0n * iVt), endp0 = exp(TSC(SC_LPL) + Vg0p * iVt);
     double U_prev = 0.0, iu_prev = 0.0, x_prev = TPT(0, PT_X);
     double Sn = 0.0, Dn = 0.0, Sp = 0.0, Dp = 0.0;
+    double num0 = 0.0, num1 = 0.0, V_prev = V0c;
+    auto count = [&](int jl, int jr, double Vl, double Vr) {  // the edge (jl, jr)'s numbers
+        const double aml = TSEG(TSID(jl), SEG_AM), amr = TSEG(TSID(jr), SEG_AM);
+        const double cml = TSEG(TSID(jl), SEG_CM), cmr = TSEG(TSID(jr), SEG_CM);
+        const double hL = TPT(jl, PT_H) * L;
+        num0 += 0.5 * (aml * exp(Vl * iVt) + amr * exp(Vr * iVt)) * (aml * amr) * hL;
+        num1 += 0.5 * (cml * exp(-Vl * iVt) + cmr * exp(-Vr * iVt)) * (cml * cmr) * hL;
+    };
     IU(0) = 0.0;
     for (int j = 1; j <= NP; ++j) {
         const int id = TSID(j);
         const double y0 = YS(j, 0), nn = exp(YS(j, 1)), pp = exp(YS(j, 2));
         STV(j, 0) = y0; STV(j, 1) = nn; STV(j, 2) = pp;
+        if (ions) count(j - 1, j, V_prev, y0);
+        V_prev = y0;
         const double R = recombination_value(nn, pp, TSEG(id, SEG_KD), TSEG(id, SEG_NI2),
                                              TSEG(id, SEG_NT), TSEG(id, SEG_CN), TSEG(id, SEG_CP),
                                              TSEG(id, SEG_NT0), TSEG(id, SEG_PT0));
@@ -277,6 +315,16 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         IU(j) = iu;
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
     }
+    // the normalisations of the converged state in closed form (its numbers equal the totals:
+    // solver.py:1695-1712's refit); a species without ions is held at l = 0
+    double lg0 = 0.0, lg1 = 0.0;
+    if (ions) {
+        count(NP, NP + 1, V_prev, VLc);
+        lg0 = tot0 > 0 ? log(tot0 / num0) : 0.0;
+        lg1 = tot1 > 0 ? log(tot1 / num1) : 0.0;
+        ION(0) = lg0;
+        ION(1) = lg1;
+    }
     const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
     const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
     const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
@@ -291,30 +339,37 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         q.sid = TSID(j);
         q.offn = TSEG(q.sid, SEG_OFFN); q.offp = TSEG(q.sid, SEG_OFFP);
         q.mun = TSEG(q.sid, SEG_MUN); q.mup = TSEG(q.sid, SEG_MUP);
-        q.am = q.cm = 0.0;
         q.h = TPT(j, PT_H);
-        q.an = q.ca = 0.0;
         if (j == 0 || j == NP + 1) {
             q.V = j == 0 ? V0c : VLc;
             q.n = exp(j == 0 ? TSC(SC_LNL) : TSC(SC_LNR));
             q.p = exp(j == 0 ? TSC(SC_LPL) : TSC(SC_LPR));
             q.G = q.fix = q.R = q.dRn = q.dRp = 0.0;
-            return;
+        } else {
+            q.V = STV(j, 0);
+            q.n = STV(j, 1);
+            q.p = STV(j, 2);
+            q.G = TPT(j, PT_G);
+            q.fix = TPT(j, PT_FIX);
+            q.R = recombination(q.n, q.p, TSEG(q.sid, SEG_KD), TSEG(q.sid, SEG_NI2),
+                                TSEG(q.sid, SEG_NT), TSEG(q.sid, SEG_CN), TSEG(q.sid, SEG_CP),
+                                TSEG(q.sid, SEG_NT0), TSEG(q.sid, SEG_PT0), q.dRn, q.dRp);
         }
-        q.V = STV(j, 0);
-        q.n = STV(j, 1);
-        q.p = STV(j, 2);
-        q.G = TPT(j, PT_G);
-        q.fix = TPT(j, PT_FIX);
-        q.R = recombination(q.n, q.p, TSEG(q.sid, SEG_KD), TSEG(q.sid, SEG_NI2),
-                            TSEG(q.sid, SEG_NT), TSEG(q.sid, SEG_CN), TSEG(q.sid, SEG_CP),
-                            TSEG(q.sid, SEG_NT0), TSEG(q.sid, SEG_PT0), q.dRn, q.dRp);
+        q.am = ions ? TSEG(q.sid, SEG_AM) : 0.0;
+        q.cm = ions ? TSEG(q.sid, SEG_CM) : 0.0;
+        q.an = q.am != 0.0 ? q.am * exp(lg0 + q.V * iVt) : 0.0;
+        q.ca = q.cm != 0.0 ? q.cm * exp(lg1 - q.V * iVt) : 0.0;
     };
     Pt right, mid, cur;
     point(NP + 1, right);
     point(NP, mid);
     Edge eR = edge_flux_pt(mid, right, Vt, iVt, a.bern);
     double cpn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, wn_[3] = {0, 0, 0};
+    // the border: X's d' ([row][species]) and the constraints' gammas ([species][row]) of row
+    // i + 1, and the contractions (dJ/dy)^T X, W^T X ([constraint][species])
+    double dun[3][2] = {{0, 0}, {0, 0}, {0, 0}}, gwn[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double cX[2] = {0, 0}, WX[2][2] = {{0, 0}, {0, 0}};
+    const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
     double Tn1 = 0.0, Tp1 = 0.0;  // the current's suffix sums from point i + 1
     double dx_next = 0.0;         // no interval after NP in the sums
     for (int i = NP; i >= 1; --i) {
@@ -359,6 +414,51 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         for (int q = 0; q < 9; ++q) FAC(i, 9 + q) = C[q];
 #pragma unroll
         for (int q = 0; q < 3; ++q) FAC(i, 18 + q) = w[q];
+        if (ions) {
+            // U_i: dF/dl in the potential's row (row_blocks_s' border columns); W_i: dg/dV_i
+            // (the elimination sweep's border), both in the potential's row only
+            const double U0[2] = {-Q_E * mid.an * g.sP, Q_E * mid.ca * g.sP};
+            const double hLm = cur.h * L, hLp = mid.h * L;
+            const double W0[2] = {
+                0.5 * (mid.am * cur.am * hLm + right.am * mid.am * hLp) * mid.an * iVt * itot0,
+                -0.5 * (mid.cm * cur.cm * hLm + right.cm * mid.cm * hLp) * mid.ca * iVt * itot1};
+            double rh[3][2], du[3][2], gw[2][3];
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    rh[p][m] = (p == 0 ? U0[m] : 0.0) -
+                               (C[3 * p] * dun[0][m] + C[3 * p + 1] * dun[1][m] + C[3 * p + 2] * dun[2][m]);
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    du[p][m] = Mi[3 * p] * rh[0][m] + Mi[3 * p + 1] * rh[1][m] + Mi[3 * p + 2] * rh[2][m];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    gw[k][q] = (q == 0 ? W0[k] : 0.0) -
+                               (cpn[q] * gwn[k][0] + cpn[3 + q] * gwn[k][1] + cpn[6 + q] * gwn[k][2]);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                cX[m] += w[0] * du[0][m] + w[1] * du[1][m] + w[2] * du[2][m];
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    WX[k][m] += gw[k][0] * du[0][m] + gw[k][1] * du[1][m] + gw[k][2] * du[2][m];
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    FAC(i, 21 + 3 * k + q) = gw[k][q];
+                    gwn[k][q] = gw[k][q];
+                }
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int m = 0; m < 2; ++m) dun[p][m] = du[p][m];
+        }
         // c'_i = Mi A_i
 #pragma unroll
         for (int p = 0; p < 3; ++p)
@@ -370,7 +470,17 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         Tn1 = Tn; Tp1 = Tp; dx_next = dx;
         right = mid; mid = cur; eR = eL;
     }
-    // lambda_i = Mi_i^T (w_i - C_{i-1}^T lambda_{i-1}), left to right
+    // the multipliers: (D - W^T X)^T mu = -(dJ/dy)^T X, D = 1 (a held species' row too: its
+    // W is zero, and so its constraint's derivatives)
+    double mu0 = 0.0, mu1 = 0.0;
+    if (ions) {
+        mu0 = -cX[0];
+        mu1 = -cX[1];
+        solve2(1.0 - WX[0][0], -WX[1][0], -WX[0][1], 1.0 - WX[1][1], mu0, mu1);
+        ION(2) = mu0;
+        ION(3) = mu1;
+    }
+    // lambda_i = Mi_i^T (w_i - mu . gamma_i - C_{i-1}^T lambda_{i-1}), left to right
     {
         double lp[3] = {0, 0, 0}, Cp_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int i = 1; i <= NP; ++i) {
@@ -378,6 +488,10 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 t[q] = FAC(i, 18 + q) - (Cp_[q] * lp[0] + Cp_[3 + q] * lp[1] + Cp_[6 + q] * lp[2]);
+            if (ions) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) t[q] -= mu0 * FAC(i, 21 + q) + mu1 * FAC(i, 24 + q);
+            }
             double Mi[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) Mi[q] = FAC(i, q);
@@ -390,20 +504,23 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
             for (int q = 0; q < 9; ++q) Cp_[q] = FAC(i, 9 + q);
         }
     }
-
 }
 
-// dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y at device d's converged voltage v
-// (adj_lambda's workspace of lane sl), with direction k's tangents staged in the LDS table.
+// dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y - mu . dg/dtheta_k|_y at device d's
+// converged voltage v (adj_lambda's workspace of lane sl), with direction k's tangents staged in
+// the LDS table.
 DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
+    (void)d;
     const int NP = a.NP;
     const size_t ls = a.ls;
     const AdjTab t(a.nseg, a.NP);
     const int S = t.S;
     lcdouble *tsc = tab + t.sct;
-    const double Vt = tsc[SC_VT], iVt = 1.0 / Vt, gfr = tsc[SC_GFRAC], elec = tsc[SC_ELEC];
+    const double elec = tsc[SC_ELEC];
     const double cJ = -elec * Q_E, Vint = a.V[v];
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
+    const bool ions = tsc[SC_AT] > 0 || tsc[SC_CT] > 0;
+    const double lg0 = ions ? ION(0) : 0.0, lg1 = ions ? ION(1) : 0.0;
     {
         SegD sd;
         sd.val = tab;
@@ -412,7 +529,7 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
             return {tab[t.ptv + j * PT_N + f], tab[t.ptt + j * PT_N + f]};
         };
         auto scd = [&](int f) -> D1 { return {tsc[f], tsc[SC_N + f]}; };
-        const D1 Ld = scd(SC_L);
+        const D1 Ld = scd(SC_L), Vt = scd(SC_VT), iVt = dc(1.0) / Vt, gfr = scd(SC_GFRAC);
         const D1 V0d = scd(SC_V0) + 0.5 * elec * Vint, VLd = scd(SC_VL) - 0.5 * elec * Vint;
         auto pointd = [&](int j, PD &q) {
             q.sid = TSID(j);
@@ -421,11 +538,21 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
                 q.V = j == 0 ? V0d : VLd;
                 q.n = dexp(scd(j == 0 ? SC_LNL : SC_LNR));
                 q.p = dexp(scd(j == 0 ? SC_LPL : SC_LPR));
-                return;
+            } else {
+                q.V = dc(STV(j, 0));
+                q.n = dc(STV(j, 1));
+                q.p = dc(STV(j, 2));
             }
-            q.V = dc(STV(j, 0));
-            q.n = dc(STV(j, 1));
-            q.p = dc(STV(j, 2));
+            const double am = ions ? TSEG(q.sid, SEG_AM) : 0.0, cm = ions ? TSEG(q.sid, SEG_CM) : 0.0;
+            q.an = am != 0.0 ? am * dexp(lg0 + q.V * iVt) : dc(0.0);
+            q.ca = cm != 0.0 ? cm * dexp(lg1 - q.V * iVt) : dc(0.0);
+        };
+        // each species' number along the direction, the edge (l, r) weighted by h_l L
+        D1 num0 = dc(0.0), num1 = dc(0.0);
+        auto count = [&](const PD &l, const PD &r) {
+            const D1 hL = l.h * Ld;
+            num0 = num0 + 0.5 * (l.an + r.an) * (TSEG(l.sid, SEG_AM) * TSEG(r.sid, SEG_AM)) * hL;
+            num1 = num1 + 0.5 * (l.ca + r.ca) * (TSEG(l.sid, SEG_CM) * TSEG(r.sid, SEG_CM)) * hL;
         };
         // the current along the direction
         const D1 Vg0nd = V0d + sd(sid0, SEG_OFFN), Vg0pd = V0d + sd(sid0, SEG_OFFP);
@@ -440,6 +567,7 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
         pointd(1, m);
         for (int j = 1; j <= NP; ++j) {
             pointd(j + 1, r);
+            if (ions) count(l, m);
             // row j
             D1 F[3];
             row_d1(l, m, r, ptd(j, PT_G), ptd(j, PT_FIX), Ld, gfr, sd, Vt, iVt, a.bern, F);
@@ -457,6 +585,11 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
             Dpd = Dpd + 0.5 * dx * (wpp * iup + wp * iu);
             wnp = wn; wpp = wp; Up = U; iup = iu; xp = xj;
             l = m; m = r;
+        }
+        if (ions) {  // mu . dg/dtheta_k, g_s = number_s / total_s - 1 (a held species: none)
+            count(l, m);
+            if (tsc[SC_AT] > 0) acc += ION(2) * (num0 / scd(SC_AT)).d;
+            if (tsc[SC_CT] > 0) acc += ION(3) * (num1 / scd(SC_CT)).d;
         }
         const D1 endnNd = dexp(scd(SC_LNR) - (VLd + sd(sidN, SEG_OFFN)) * iVt);
         const D1 endpNd = dexp(scd(SC_LPR) + (VLd + sd(sidN, SEG_OFFP)) * iVt);
@@ -484,6 +617,7 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
     const double Vint = a.V[v];
     const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
+    double bL = 0.0, bVt = 0.0, bgfr = 0.0;  // the partials in L, V_T and the generation's factor
     auto seg_now = [&](int id, int f, double b) {
         lcdouble *tg = tab + t.segT + id * SEG_N + f;
 #pragma unroll
@@ -531,14 +665,21 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
     const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
     const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
     {
+        const double VgNn = VLc + TSEG(sidN, SEG_OFFN), VgNp = VLc + TSEG(sidN, SEG_OFFP);
+        const double iVt2 = iVt * iVt;
+        bVt += cJ * ((endn0 - endnN) / Sn + (endpN - endp0) / Sp);  // Kn, Kp's factor V_T
         double e = cJ * Vt / Sn * endn0;  // endn0 = exp(LNL - (V0 + offn) / V_T)
         sc_add(SC_LNL, e); sc_add(SC_V0, -e * iVt); seg_add(sid0, SEG_OFFN, -e * iVt);
+        bVt += e * Vg0n * iVt2;
         e = -cJ * Vt / Sp * endp0;        // endp0 = exp(LPL + (V0 + offp) / V_T)
         sc_add(SC_LPL, e); sc_add(SC_V0, e * iVt); seg_add(sid0, SEG_OFFP, e * iVt);
+        bVt -= e * Vg0p * iVt2;
         e = -cJ * Vt / Sn * endnN;
         sc_add(SC_LNR, e); sc_add(SC_VL, -e * iVt); seg_add(sidN, SEG_OFFN, -e * iVt);
+        bVt += e * VgNn * iVt2;
         e = cJ * Vt / Sp * endpN;
         sc_add(SC_LPR, e); sc_add(SC_VL, e * iVt); seg_add(sidN, SEG_OFFP, e * iVt);
+        bVt -= e * VgNp * iVt2;
     }
     // the edge fluxes reversed: partials of bFn Fn + bFp Fp over (l, r) (edge_d1)
     auto edge_rev = [&](int sl_, int sr_, double Vl, double nl, double pl, double Vr, double nr,
@@ -554,6 +695,9 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         seg_add(sl_, SEG_MUP, bFp * (Bpf * pl - Bpb * pr) * Vt);
         const double bxn = bFn * lcn * nr * dBnf + bFn * lcn * nl * dBnb;
         const double bxp = bFp * lcp * pl * dBpf + bFp * lcp * pr * dBpb;
+        // V_T: the factor V_T mu and the arguments' 1 / V_T
+        bVt += bFn * mun * (Bnf * nr - Bnb * nl) + bFp * mup * (Bpf * pl - Bpb * pr) -
+               (bxn * xn + bxp * xp) * iVt;
         seg_add(sr_, SEG_OFFN, bxn * iVt); seg_add(sl_, SEG_OFFN, -bxn * iVt);
         seg_add(sr_, SEG_OFFP, bxp * iVt); seg_add(sl_, SEG_OFFP, -bxp * iVt);
         if (lc) {  // the left contact: V0 and its densities
@@ -576,11 +720,47 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         seg_add(sid0, SEG_IMUN, bwn * en); seg_add(sid0, SEG_IMUP, bwp * ep);
         sc_add(SC_V0, -bwn * wn * iVt + bwp * wp * iVt);
         seg_add(sid0, SEG_OFFN, -bwn * wn * iVt); seg_add(sid0, SEG_OFFP, bwp * wp * iVt);
+        bVt += (bwn * wn * Vg0n - bwp * wp * Vg0p) * iVt * iVt;
     }
+    // mobile ions: the rows' densities at the held normalisations, and the constraints'
+    // partials, -mu . dg/dtheta with g_s = number_s / total_s - 1 (a held species: none)
+    const double tot0 = TSC(SC_AT), tot1 = TSC(SC_CT);
+    const bool ions = tot0 > 0 || tot1 > 0;
+    const double lg0 = ions ? ION(0) : 0.0, lg1 = ions ? ION(1) : 0.0;
+    const double ma = ions && tot0 > 0 ? -ION(2) / tot0 : 0.0;  // -mu_a / total_a
+    const double mc = ions && tot1 > 0 ? -ION(3) / tot1 : 0.0;
+    double nsum0 = 0.0, nsum1 = 0.0;
+    auto ion_an = [&](int id, double V) {
+        const double m = TSEG(id, SEG_AM);
+        return ions && m != 0.0 ? m * exp(lg0 + V * iVt) : 0.0;
+    };
+    auto ion_ca = [&](int id, double V) {
+        const double m = TSEG(id, SEG_CM);
+        return ions && m != 0.0 ? m * exp(lg1 - V * iVt) : 0.0;
+    };
+    // the edge (jl, jl + 1) of the numbers: its partials in L, V_T and the contacts'
+    // potentials; returns the one in h_jl
+    auto ion_edge = [&](int jl, double Vl, double Vr) -> double {
+        const int il = TSID(jl), ir = TSID(jl + 1);
+        const double mma = TSEG(il, SEG_AM) * TSEG(ir, SEG_AM);
+        const double mmc = TSEG(il, SEG_CM) * TSEG(ir, SEG_CM);
+        const double anl = ion_an(il, Vl), anr = ion_an(ir, Vr);
+        const double cal = ion_ca(il, Vl), car = ion_ca(ir, Vr);
+        const double h = TPT(jl, PT_H), hL = h * L;
+        const double na = 0.5 * (anl + anr) * mma, nc = 0.5 * (cal + car) * mmc;
+        nsum0 += na * hL;
+        nsum1 += nc * hL;
+        bL += (ma * na + mc * nc) * h;
+        // d an / d V_T = -an V / V_T^2, d ca / d V_T = ca V / V_T^2
+        bVt += 0.5 * hL * iVt * iVt *
+               (ma * mma * (-anl * Vl - anr * Vr) + mc * mmc * (cal * Vl + car * Vr));
+        if (jl == 0) sc_add(SC_V0, 0.5 * hL * iVt * (ma * mma * anl - mc * mmc * cal));
+        if (jl == NP) sc_add(SC_VL, 0.5 * hL * iVt * (ma * mma * anr - mc * mmc * car));
+        return (ma * na + mc * nc) * L;
+    };
     double wn_prev = exp(-Vg0n * iVt) * TSEG(sid0, SEG_IMUN);
     double wp_prev = exp(Vg0p * iVt) * TSEG(sid0, SEG_IMUP);
     double U_prev = 0.0, iu_prev = 0.0, x_prev = x0;
-    double bL = 0.0;
     for (int j = 1; j <= NP; ++j) {
         const bool lc = j == 1, rc = j == NP;
         const int sl_ = TSID(j - 1), sm = TSID(j), sr_ = TSID(j + 1);
@@ -624,8 +804,9 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         }
         const double elec_ = hm * FnR - hp * FnL + fac_ * (gen - R);
         const double hole = hp * FpL - hm * FpR + fac_ * (gen - R);
+        const double anm = ion_an(sm, Vm), cam = ion_ca(sm, Vm);
         const double pois = fac_l * Vl + fac_m * Vm + fu * Vr -
-                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + TPT(j, PT_FIX));
+                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + TPT(j, PT_FIX) + anm - cam);
         // the row reversed, seeded with -lambda_j
         const double b0 = -LAM(j, 0), b1 = -LAM(j, 1), b2 = -LAM(j, 2);
         const double bpois = b0 * sP;
@@ -639,6 +820,7 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         if (rc) sc_add(SC_VL, bpois * fu);
         seg_add(sm, SEG_DOP, -Q_E * bpois);
         bp(j, PT_FIX) = -Q_E * bpois;
+        bVt += bpois * Q_E * (anm + cam) * Vm * iVt * iVt;
         const double bjump = bfac_m - bfac_l;
         bfu -= bfac_m;
         const double bfl0 = bfac_l - bfac_m;
@@ -675,14 +857,16 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         const double bwn = (bSn + bDn * iu) * aj, bwp = (bSp + bDp * iu) * aj;
         seg_add(sm, SEG_IMUN, bwn * en); seg_add(sm, SEG_IMUP, bwp * ep);
         seg_add(sm, SEG_OFFN, -bwn * wn * iVt); seg_add(sm, SEG_OFFP, bwp * wp * iVt);
+        bVt += (bwn * wn * (Vm + TSEG(sm, SEG_OFFN)) - bwp * wp * (Vm + TSEG(sm, SEG_OFFP))) * iVt * iVt;
         const double bU = bDn * (0.5 * dx * Tn + 0.5 * dxn * Tn1) + bDp * (0.5 * dx * Tp + 0.5 * dxn * Tp1);
         bR += bU;
         bG -= bU * gfr;
+        bgfr += ((belec + bhole) * fac_ - bU) * G;
         // interval j (x_{j-1}, x_j)
         const double bdx = bSn * 0.5 * (wn_prev + wn) + bDn * (0.5 * (wn_prev * iu_prev + wn * iu) + 0.5 * (U_prev + U) * Tn) +
                            bSp * 0.5 * (wp_prev + wp) + bDp * (0.5 * (wp_prev * iu_prev + wp * iu) + 0.5 * (U_prev + U) * Tp);
         bp(j, PT_G) = bG;
-        bp(j - 1, PT_H) = pendH + bhm;
+        bp(j - 1, PT_H) = pendH + bhm + (ions ? ion_edge(j - 1, Vl, Vm) : 0.0);
         bp(j - 1, PT_X) = pendX - bdx;
         pendH = bhp;
         pendX = bdx;
@@ -701,10 +885,16 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         }
         wn_prev = wn; wp_prev = wp; U_prev = U; iu_prev = iu; x_prev = xj;
     }
-    bp(NP, PT_H) = pendH;
+    bp(NP, PT_H) = pendH + (ions ? ion_edge(NP, STV(NP, 0), VLc) : 0.0);
     bp(NP, PT_X) = pendX;
     flush();
+    if (ions) {
+        if (tot0 > 0) sc_add(SC_AT, ION(2) * nsum0 / (tot0 * tot0));
+        if (tot1 > 0) sc_add(SC_CT, ION(3) * nsum1 / (tot1 * tot1));
+    }
     sc_add(SC_L, bL);
+    sc_add(SC_VT, bVt);
+    sc_add(SC_GFRAC, bgfr);
 }
 
 // The points' part of the contraction, for points c0 .. c0 + ADJ_CHUNK - 1 (at most NP): their
@@ -748,6 +938,7 @@ DDH_FN void adj_stage_points(const AdjArgs &a, int d, int c0, ldouble *tab, int 
 #undef LAM
 #undef IU
 #undef YS
+#undef ION
 
 
 }  // namespace ddh

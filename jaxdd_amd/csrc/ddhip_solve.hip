@@ -312,17 +312,17 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
     for (int k = 0;; ++k) {
         const int d = xcd_device(map, k, a.n);
         if (d < 0) break;
-        const bool mobile = a.sc[(size_t)SC_AT * a.n + d] > 0 || a.sc[(size_t)SC_CT * a.n + d] > 0;
-        if (lane == 0) a.status[d] = mobile ? 1 : 0;
+        const bool refused = adj_refused(a, d);
+        if (lane == 0) a.status[d] = refused ? 1 : 0;
         __syncthreads();  // every lane is done with the previous device's table
         adj_stage_values(a, d, tab, lane, 64);
         __syncthreads();
         for (int v0 = 0; v0 < nV; v0 += 64) {
             const int v = v0 + lane;
-            const bool act = v < nV && !mobile && a.ok[(size_t)d * nV + v];
+            const bool act = v < nV && !refused && a.ok[(size_t)d * nV + v];
             double *out = a.grad + ((size_t)d * nV + (v < nV ? v : 0)) * K;
             if (v < nV && !act)
-                for (int k = 0; k < K; ++k) out[k] = mobile ? NAN : 0.0;
+                for (int k = 0; k < K; ++k) out[k] = refused ? NAN : 0.0;
             if (act) adj_lambda(a, d, v, sl, tab);
             if (REV) {  // every direction in one reverse pass
                 double g[ADJ_MAX_K];
@@ -652,7 +652,7 @@ extern "C" int ddh_solve_batch_noalloc(const ddh_batch *b, const double *V, int3
 }
 
 struct AdjLayout {
-    size_t fac, st, lam, iu, ts, total, slots, ls;
+    size_t fac, st, lam, iu, ts, ion, total, slots, ls;
 };
 
 // Per launched lane (ADJ_SLOTS workgroups of 64 at most): the factors, the state's V, n, p,
@@ -668,6 +668,7 @@ static AdjLayout adj_layout(int32_t n, int32_t NP, int32_t nV) {
     w.lam = off; off = align256(off + (size_t)NP * 3 * w.ls * sizeof(double));
     w.iu = off; off = align256(off + (size_t)(NP + 1) * w.ls * sizeof(double));
     w.ts = off; off = align256(off + (size_t)(NP + 2) * 2 * w.ls * sizeof(double));
+    w.ion = off; off = align256(off + (size_t)4 * w.ls * sizeof(double));
     w.total = off;
     return w;
 }
@@ -703,6 +704,7 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     a.lam = (double *)(base + w.lam);
     a.iu = (double *)(base + w.iu);
     a.ts = (double *)(base + w.ts);
+    a.ion = (double *)(base + w.ion);
     a.ls = w.ls;
     a.passes = 0;
     hipStream_t s = (hipStream_t)stream;

@@ -121,3 +121,47 @@ def short_points(devs, V, its, its_ref, wild):
     mask = (its_ref <= SHORT_STEPS) & (its < CHAOTIC_STEPS) & ~np.asarray(wild)[:, None]
     mask[np.arange(len(devs)), np.minimum(first, len(V) - 1)] = False
     return mask
+
+
+#: Named numbers of the free-ion stack (jaxdd_amd.builder's): the ions' densities, thicknesses,
+#: a mobility, a permittivity, a doping, traps, generation, the work function, temperature and
+#: the generation's factor.
+ION_WRT = [("N_anion", 1), ("N_cation", 1), ("L", 1), ("L", 0), ("mu_n", 1), ("eps_r", 1),
+           ("N_A", 2), ("N_t_bulk", 1), ("G_ehp", 1), "W_L", "T", "G_frac"]
+
+
+def bumped(p, w, h):
+    """The numbers ``p`` with ``w`` (a name or a (name, layer) pair) moved by ``h``."""
+    q = {k: (np.array(v, dtype=float) if isinstance(v, np.ndarray) else v) for k, v in p.items()}
+    if isinstance(w, str):
+        q[w] = float(q[w]) + h
+    else:
+        arr = np.array(q[w[0]], dtype=float)
+        arr[w[1]] += h
+        q[w[0]] = arr
+    return q
+
+
+def number_of(p, w):
+    """The number ``w`` of ``p``."""
+    return float(p[w]) if isinstance(w, str) else float(np.asarray(p[w[0]])[w[1]])
+
+
+def named_fd_errors(oracle, structs, params, wrt, V, J, grad):
+    """Per device and number, max over V of |dJ/dx - central differences| |x| / the curve's
+    current scale (relative step 1e-5, the oracle's curves at tol 1e-12)."""
+    from jaxdd_amd.batch import j_scale
+    from jaxdd_amd.device import device_arrays
+
+    err = np.zeros((len(structs), len(wrt)))
+    for d, (st, p) in enumerate(zip(structs, params)):
+        cscale = j_scale(device_arrays(st, p)) + np.max(np.abs(J[d]))
+        for k, w in enumerate(wrt):
+            x = number_of(p, w)
+            h = 1e-5 * abs(x)
+            Jpm = [oracle.solve_batch([device_arrays(st, bumped(p, w, s * h))], V, tol=1e-12,
+                                      strict=True, first_max_iter=None, max_iter=200)[0][0]
+                   for s in (1.0, -1.0)]
+            fd = (Jpm[0] - Jpm[1]) / (2 * h)
+            err[d, k] = np.max(np.abs(grad[d, :, k] - fd)) * abs(x) / cscale
+    return err

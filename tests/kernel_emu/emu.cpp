@@ -235,28 +235,30 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
                            double *grad, int32_t *status) {
     const size_t ls = 64;  // one workgroup: the devices in turn, a lane per voltage
     std::vector<double> fac((size_t)NP * ADJ_F * ls), st((size_t)NP * 3 * ls),
-        lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls), ts((size_t)(NP + 2) * 2 * ls);
+        lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls), ts((size_t)(NP + 2) * 2 * ls),
+        ion((size_t)4 * ls);
     AdjArgs a;
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg; a.ntheta = ntheta; a.bern = bern;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
     a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
     a.fac = fac.data(); a.st = st.data(); a.lam = lam.data(); a.iu = iu.data(); a.ls = ls;
     a.ts = ts.data();
+    a.ion = ion.data();
     a.passes = g_adj_passes;
     const int Kc = adj_k(ntheta, a.passes);
     std::vector<double> tab(adj_table_size(nseg, NP, Kc));
     for (int d = 0; d < n; ++d) {
-        const bool mobile = sc[(size_t)SC_AT * n + d] > 0 || sc[(size_t)SC_CT * n + d] > 0;
-        if (mobile) status[d] = 1;
+        const bool refused = adj_refused(a, d);
+        if (refused) status[d] = 1;
         adj_stage_values(a, d, tab.data(), 0, 1);
         for (int v0 = 0; v0 < nV; v0 += 64) {
             const int v1 = v0 + 64 < nV ? v0 + 64 : nV;
             std::vector<char> act(64, 0);
             for (int v = v0; v < v1; ++v) {
-                act[v - v0] = !mobile && ok[(size_t)d * nV + v];
+                act[v - v0] = !refused && ok[(size_t)d * nV + v];
                 double *out = grad + ((size_t)d * nV + v) * ntheta;
                 if (!act[v - v0])
-                    for (int k = 0; k < ntheta; ++k) out[k] = mobile ? NAN : 0.0;
+                    for (int k = 0; k < ntheta; ++k) out[k] = refused ? NAN : 0.0;
                 else
                     adj_lambda(a, d, v, (size_t)(v - v0), tab.data());
             }

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+from conftest import ION_WRT, named_fd_errors, number_of
 
 N_DEV = 4
 V_SUB = slice(None, None, 6)  # 11 of the 61 voltages
@@ -145,11 +146,13 @@ def test_unconverged_voltages_have_zero_gradient(emu):
     assert np.all(grad[~ok] == 0.0) and np.any(grad[ok] != 0.0)
 
 
-def test_devices_with_mobile_ions_are_refused(emu):
+def test_devices_with_frozen_ions_are_refused(emu):
+    """Ions frozen at a pre-bias: the scan's fixed charge is the pre-bias's solution, whose
+    dependence on the parameters this adjoint does not take (status 1, NaN)."""
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import pack
 
-    packed = pack(S.devices("c3", 1))
+    packed = pack(S.devices("c3", 1, variant="frozen"))
     V = S.VOLTAGES[::20].copy()
     states = np.zeros((len(V), 400, 3, 1))
     J, ok, its, _ = emu.solve_packed(packed, V, states=states)
@@ -157,6 +160,51 @@ def test_devices_with_mobile_ions_are_refused(emu):
            "sc": np.zeros((1,) + packed["sc"].shape)}
     grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
     assert status[0] == 1 and np.isnan(grad).all()
+
+
+@pytest.fixture(scope="module")
+def ion_case(emu):
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(2)
+    structs, params = zip(*[S.row_inputs(t, k, 100, ions=True) for k in range(2)])
+    dirs = A.directions_for(ION_WRT, 2, len(structs[0].i0))
+    packed, tan = emu.build_packed(list(structs), list(params), dirs)
+    V = S.VOLTAGES[::10].copy()
+    states = np.zeros((len(V), 100, 3, 2))
+    J, ok, its, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                     states=states)
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    return structs, params, packed, tan, V, J, ok, states, grad, status
+
+
+def test_mobile_ions_gradients_are_finite_differences(ion_case, oracle):
+    """Free mobile ions (the border of each species' normalisation and number, transposed:
+    reference src/jaxdd/batch.py:595-618 `_transposed`): the gradients in the named numbers,
+    the ions' own densities among them, against central differences of the oracle's curves."""
+    structs, params, _, _, V, J, ok, _, grad, status = ion_case
+    assert not status.any() and ok.all()
+    err = named_fd_errors(oracle, structs, params, ION_WRT, V, J, grad)
+    assert np.max(err) < 1e-5, err
+    assert np.median(err) < 3e-7, err
+
+
+def test_mobile_ions_reverse_pass_is_the_directions_passes(ion_case, emu):
+    """The reverse pass (the constraints' partials reversed with the rows') against one dual
+    pass per direction (the numbers' tangents forward), per device, voltage and number."""
+    from jaxdd_amd.batch import j_scale
+    from jaxdd_amd.device import device_arrays
+
+    structs, params, packed, tan, V, J, ok, states, grad, _ = ion_case
+    by_pass, st2 = emu.adjoint_packed(packed, V, states, ok, tan, passes=True)
+    assert not st2.any()
+    unit = np.array([[abs(number_of(p, w)) for w in ION_WRT] for p in params])
+    scale = np.array([j_scale(device_arrays(st, p)) for st, p in zip(structs, params)])
+    scale = scale + np.max(np.abs(J), axis=1)
+    err = np.abs(grad - by_pass) * unit[:, None, :] / scale[:, None, None]
+    # (the temperature's terms, through every exponential, cancel to 1e-11 of the scale)
+    assert np.max(err) < 5e-11, np.unravel_index(np.argmax(err), err.shape)
 
 
 def test_reverse_pass_is_the_directions_passes(c4_case, emu):

@@ -263,3 +263,50 @@ def test_gpu_named_gradients_and_the_vjp(torch_cuda):
     tbar, J3, ok3 = A.jv_vjp(batch, tan, V, w, tol=1e-12, max_iter=200)
     expect = np.einsum("nv,nvk->nk", w, g2)
     np.testing.assert_allclose(_np(tbar), expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max())
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_with_mobile_ions(torch_cuda, oracle, emu):
+    """Free mobile ions on the GPU (the bordered transposed solve): gradients in named numbers,
+    the ions' own densities, the temperature and the generation's factor among them, against
+    the adjoint's CPU replay at the GPU's own states and against central differences of the
+    oracle's curves. As the ion-free gradient's tests: the replay on a random weighting of the
+    curves and directions (1e-6), since at high forward bias the transposed system's
+    conditioning turns the GPU's and the host's last-bit differences of exp and log into
+    differences of single components; the differences per component (1e-5 of the curve's
+    scale, as the CPU replay's test)."""
+    from conftest import ION_WRT, named_fd_errors, number_of
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.builder import build
+
+    n, NP = 8, 100
+    t = S.draw(n)
+    structs, params = zip(*[S.row_inputs(t, k, NP, ions=True) for k in range(n)])
+    V = S.VOLTAGES[::10].copy()
+    print(f"gradients with mobile ions: {n} devices, {len(ION_WRT)} numbers", flush=True)
+    dirs = A.directions_for(ION_WRT, n, len(structs[0].i0))
+    batch, tan = build(list(structs), list(params), dirs)
+    J, ok, _, g, states = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200, strict=True,
+                                         return_states=True)
+    J, ok, g = _np(J), _np(ok), _np(g)
+    assert ok.all() and np.isfinite(g).all()
+    packed = {"pt": _np(batch["_pt"]), "sid": _np(batch["_sid"]), "seg": _np(batch["_seg"]),
+              "sc": _np(batch["_sc"]), "nseg": batch["nseg"]}
+    rtan = {k: _np(tan[k]) for k in ("pt", "seg", "sc")}
+    rg, rst = emu.adjoint_packed(packed, V, np.ascontiguousarray(_np(states)), ok, rtan)
+    assert not rst.any()
+    unit = np.array([[abs(number_of(p, w)) for w in ION_WRT] for p in params])
+    rng = np.random.default_rng(0)
+    w, d = rng.normal(size=ok.shape), rng.normal(size=unit.shape)
+    a = np.sum(w[:, :, None] * g * (unit * d)[:, None, :])
+    b = np.sum(w[:, :, None] * rg * (unit * d)[:, None, :])
+    scale = np.max(np.abs(J), axis=1)[:, None, None]
+    comp = np.abs(g - rg) * unit[:, None, :] / scale
+    print(f"GPU vs replay: weighted sum {abs(a - b) / abs(b):.2e}, components max "
+          f"{comp.max():.2e} median {np.median(comp):.2e}", flush=True)
+    assert abs(a - b) < 1e-6 * abs(b)
+    err = named_fd_errors(oracle, structs[:3], params[:3], ION_WRT, V, J, g)
+    print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
+    assert np.max(err) < 1e-5, err
+    assert np.median(err) < 3e-7, err

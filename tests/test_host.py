@@ -220,7 +220,8 @@ def test_library_rejects_bad_shapes_before_any_launch():
 
 def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
     """solve_batch(lanes=4) and jv_gradients check the batch before any launch: mobile ions
-    (ValueError for the solve, NotImplementedError for gradients) and NP < 16."""
+    with four lanes and NP < 16 (ValueError), ions frozen at a pre-bias for gradients
+    (NotImplementedError)."""
     from jaxdd_amd import adjoint as A
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import from_packed, pack, solve_batch
@@ -235,7 +236,10 @@ def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
     with pytest.raises(ValueError, match="lanes"):
         solve_batch(coarse, [0.0, 0.5], lanes=3)
     tan = {"pt": np.zeros((1,) + tuple(ions["_pt"].shape)), "seg": None, "sc": None}
-    with pytest.raises(NotImplementedError, match="mobile ions"):
-        A.jv_gradients(ions, tan, [0.0, 0.5])
+    with pytest.raises(ValueError, match="mobile ions"):
+        A.jv_gradients(ions, tan, [0.0, 0.5], lanes=4)
+    frozen = from_packed(pack([S.device_from_row(t, k, 40, True, True) for k in range(3)]), "cpu")
+    with pytest.raises(NotImplementedError, match="frozen"):
+        A.jv_gradients(frozen, tan, [0.0, 0.5])
     with pytest.raises(ValueError, match="precision"):
         A.jv_gradients(ions, tan, [0.0, 0.5], precision="fp32")
