@@ -353,15 +353,16 @@ struct Edge {  // Scharfetter-Gummel fluxes over one interval and their derivati
 };
 
 // The segment table of a wavefront's lanes in LDS, copied once per launch: field f of
-// segment s of the lane at lseg[(s * SEG_N + f) * WAVE]. A segment change then reloads from
+// segment s of the lane at lseg[(s * SEG_N + f) * ls], ls the wavefront's device slots (64
+// for one lane per device, 16 for four). A segment change then reloads from
 // LDS, whose wait counter is not the one global stores count on, so the reload does not wait
 // for the sweep's outstanding factor stores.
 constexpr int WAVE = 64;
 typedef DDH_LDS double ldouble;
 typedef DDH_LDS const double lcdouble;
 
-DDH_FN void fill_segments(const Args &a, int lane, ldouble *lseg) {
-    for (int q = 0; q < a.nseg * SEG_N; ++q) lseg[(size_t)q * WAVE] = a.seg[(size_t)q * a.n + lane];
+DDH_FN void fill_segments(const Args &a, int lane, ldouble *lseg, int ls = WAVE) {
+    for (int q = 0; q < a.nseg * SEG_N; ++q) lseg[(size_t)q * ls] = a.seg[(size_t)q * a.n + lane];
 }
 
 struct SweepCtl {  // what one lane's sweeps need of its state
@@ -380,14 +381,17 @@ typedef DDH_GLOBAL const uint8_t gcuint8;
 struct Lane {
     gcdouble *pt;   // + j*PT_N*n + f*n
     gcuint8 *sid;   // + j*n
-    lcdouble *seg;  // + (id*SEG_N + f)*WAVE (LDS)
+    lcdouble *seg;  // + (id*SEG_N + f)*ls (LDS)
     gdouble *yw, *ys1, *ys2;  // state buffers: + (j-1)*3n + f*n
     gdouble *fix;   // + (j-1)*n
     size_t n;
+    int ls;         // the LDS table's row stride: the wavefront's device slots (a constant of
+                    // each sweep, so that field offsets are immediates)
 };
 
-DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
+DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c, int ls = WAVE) {
     Lane L;
+    L.ls = ls;
     const size_t n = (size_t)DDH_UNIFORM(a.n), ny = (size_t)DDH_UNIFORM(a.NP) * 3 * n;
     L.n = n;
     L.pt = (gcdouble *)(a.pt + lane);
@@ -401,14 +405,15 @@ DDH_FN Lane lane_view(const Args &a, int lane, const SweepCtl &c) {
 }
 
 // Field f of this lane's segment id (LDS).
-DDH_FN double segv(const Lane &L, int id, int f) { return L.seg[(id * SEG_N + f) * WAVE]; }
+DDH_FN double segv(const Lane &L, int id, int f) { return L.seg[(id * SEG_N + f) * L.ls]; }
 
 // The segment fields a point carries (q.sid set); the ion masks only while ions move.
 DDH_FN void seg_fields(const Lane &L, bool mobile, Pt &q) {
-    lcdouble *s = L.seg + q.sid * SEG_N * WAVE;
-    q.offn = s[SEG_OFFN * WAVE]; q.offp = s[SEG_OFFP * WAVE];
-    q.mun = s[SEG_MUN * WAVE]; q.mup = s[SEG_MUP * WAVE];
-    const double am = s[SEG_AM * WAVE], cm = s[SEG_CM * WAVE];
+    const int w = L.ls;
+    lcdouble *s = L.seg + q.sid * SEG_N * w;
+    q.offn = s[SEG_OFFN * w]; q.offp = s[SEG_OFFP * w];
+    q.mun = s[SEG_MUN * w]; q.mup = s[SEG_MUP * w];
+    const double am = s[SEG_AM * w], cm = s[SEG_CM * w];
     q.am = mobile ? am : 0.0;
     q.cm = mobile ? cm : 0.0;
 }
@@ -419,12 +424,13 @@ struct CurSeg {
 };
 
 DDH_FN CurSeg cur_seg(const Lane &L, int sid) {
-    lcdouble *s = L.seg + sid * SEG_N * WAVE;
+    const int w = L.ls;
+    lcdouble *s = L.seg + sid * SEG_N * w;
     CurSeg g;
-    g.kd = s[SEG_KD * WAVE]; g.ni2 = s[SEG_NI2 * WAVE]; g.nt = s[SEG_NT * WAVE];
-    g.cn = s[SEG_CN * WAVE]; g.cp = s[SEG_CP * WAVE]; g.nt0 = s[SEG_NT0 * WAVE];
-    g.pt0 = s[SEG_PT0 * WAVE]; g.offn = s[SEG_OFFN * WAVE]; g.offp = s[SEG_OFFP * WAVE];
-    g.imun = s[SEG_IMUN * WAVE]; g.imup = s[SEG_IMUP * WAVE];
+    g.kd = s[SEG_KD * w]; g.ni2 = s[SEG_NI2 * w]; g.nt = s[SEG_NT * w];
+    g.cn = s[SEG_CN * w]; g.cp = s[SEG_CP * w]; g.nt0 = s[SEG_NT0 * w];
+    g.pt0 = s[SEG_PT0 * w]; g.offn = s[SEG_OFFN * w]; g.offp = s[SEG_OFFP * w];
+    g.imun = s[SEG_IMUN * w]; g.imup = s[SEG_IMUP * w];
     return g;
 }
 
@@ -494,10 +500,11 @@ DDH_FN void make_point(const Lane &L, int j, bool contact, const Contact &ct, co
     const size_t n = L.n;
     q.sid = sid;
     seg_fields(L, c.mobile, q);
-    lcdouble *s = L.seg + sid * SEG_N * WAVE;
-    const double kd = s[SEG_KD * WAVE], ni2 = s[SEG_NI2 * WAVE], nt = s[SEG_NT * WAVE];
-    const double cn = s[SEG_CN * WAVE], cpp = s[SEG_CP * WAVE], nt0 = s[SEG_NT0 * WAVE];
-    const double pt0 = s[SEG_PT0 * WAVE];
+    const int w = L.ls;
+    lcdouble *s = L.seg + sid * SEG_N * w;
+    const double kd = s[SEG_KD * w], ni2 = s[SEG_NI2 * w], nt = s[SEG_NT * w];
+    const double cn = s[SEG_CN * w], cpp = s[SEG_CP * w], nt0 = s[SEG_NT0 * w];
+    const double pt0 = s[SEG_PT0 * w];
     const double am = q.am, cm = q.cm;
     if (contact) {
         q.h = ct.h;
@@ -623,11 +630,12 @@ template <int K>
 DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
                        const Edge &eR, bool last, double L2, double iVt, double share,
                        double g_frac, double *A, double *B, double *C, double *r) {
-    lcdouble *sm = Lv.seg + mid.sid * SEG_N * WAVE;
+    const int w = Lv.ls;
+    lcdouble *sm = Lv.seg + mid.sid * SEG_N * w;
     RowSeg g;
-    g.meps = sm[SEG_EPS * WAVE];
-    g.ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * WAVE];
-    g.sP = sm[SEG_SP * WAVE]; g.scs = sm[SEG_SCS * WAVE]; g.dop = sm[SEG_DOP * WAVE];
+    g.meps = sm[SEG_EPS * w];
+    g.ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * w];
+    g.sP = sm[SEG_SP * w]; g.scs = sm[SEG_SCS * w]; g.dop = sm[SEG_DOP * w];
     row_blocks_s<K>(g, cur, mid, rV, eL, eR, last, L2, iVt, share, g_frac, A, B, C, r);
 }
 

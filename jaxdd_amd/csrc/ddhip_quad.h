@@ -82,13 +82,21 @@ constexpr int QUAD_MIN_NP = 16;
 #define DDH_QUADFN __device__ __attribute__((noinline))
 #endif
 
-// Where lane q of the device in slot `slot` of a wavefront keeps its prefix (lanes 1 and 2):
-// columns 16..47 of the first 21 rows of the wavefront's segment table, which the 16 devices'
-// tables (columns 0..15) leave free; element e at pre[e * WAVE]. The prefix is read and written
-// once a row, off the row's critical path, and in the local data share its waits do not drain
-// the row's memory loads in flight.
+// The wavefront's local data share: the 16 devices' segment tables (row stride QUAD_LS), then
+// the prefixes of lanes 1 and 2 of each device (QUAD_PRE rows of QUAD_PRE_LS columns): 25 KB
+// for seven segments, so that four workgroups (every SIMD of a CU) fit in one CU's LDS. Lane
+// q of the device in slot `slot` keeps its prefix at column 2 slot + (q - 1), element e at
+// pre[e * QUAD_PRE_LS]. The prefix is read and written once a row, off the row's critical
+// path, and in the local data share its waits do not drain the row's memory loads in flight.
+constexpr int QUAD_LS = 16;
 constexpr int QUAD_PRE = 21;
-DDH_FN int quad_pre_column(int slot, int q) { return 16 + 2 * slot + ((q - 1) & 1); }
+constexpr int QUAD_PRE_LS = 32;
+DDH_MFN size_t quad_lds_doubles(int nseg) {
+    return (size_t)nseg * SEG_N * QUAD_LS + (size_t)QUAD_PRE * QUAD_PRE_LS;
+}
+DDH_FN int quad_pre_column(int nseg, int slot, int q) {
+    return nseg * SEG_N * QUAD_LS + 2 * slot + ((q - 1) & 1);
+}
 
 // The elimination of one chunk (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination).
 template <typename T, typename S>
@@ -100,7 +108,7 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
     const int dir = ch.dir, j0 = ch.j0, nvis = ch.rows + 2;
     const bool spike = ch.spike != 0;
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, L = SC(SC_L), L2 = L * L;
-    const Lane Lv = lane_view(a, lane, c);
+    const Lane Lv = lane_view(a, lane, c, QUAD_LS);
     const size_t n = Lv.n;
     DDH_GLOBAL S *fac = (DDH_GLOBAL S *)(factors<S>(a) + lane);
     gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
@@ -118,7 +126,7 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
             dp[4 * row + col] = (spike && col == row + 1) ? T(-1) : T(0);
     if (spike) {  // R = I, acc = 0
 #pragma unroll
-        for (int k = 0; k < QUAD_PRE; ++k) pre[k * WAVE] = (k < 9 && k % 4 == 0) ? 1.0 : 0.0;
+        for (int k = 0; k < QUAD_PRE; ++k) pre[k * QUAD_PRE_LS] = (k < 9 && k % 4 == 0) ? 1.0 : 0.0;
     }
 
     Pt mid, far, pn;
@@ -218,15 +226,15 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
                 // the prefix: acc += R d'_i, R = -R c'_i
                 double R[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) R[k] = pre[k * WAVE];
+                for (int k = 0; k < 9; ++k) R[k] = pre[k * QUAD_PRE_LS];
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
 #pragma unroll
                     for (int col = 0; col < 4; ++col)
-                        pre[(9 + 4 * p + col) * WAVE] += R[3 * p] * (double)dn_[col] + R[3 * p + 1] * (double)dn_[4 + col] + R[3 * p + 2] * (double)dn_[8 + col];
+                        pre[(9 + 4 * p + col) * QUAD_PRE_LS] += R[3 * p] * (double)dn_[col] + R[3 * p + 1] * (double)dn_[4 + col] + R[3 * p + 2] * (double)dn_[8 + col];
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
-                        pre[(3 * p + k) * WAVE] = -(R[3 * p] * (double)cn_[k] + R[3 * p + 1] * (double)cn_[3 + k] + R[3 * p + 2] * (double)cn_[6 + k]);
+                        pre[(3 * p + k) * QUAD_PRE_LS] = -(R[3 * p] * (double)cn_[k] + R[3 * p + 1] * (double)cn_[3 + k] + R[3 * p + 2] * (double)cn_[6 + k]);
                 }
             }
 #pragma unroll
@@ -248,12 +256,12 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         out.m.cf[k] = (double)cp[k];
-        out.R[k] = spike ? pre[k * WAVE] : (k % 4 == 0 ? 1.0 : 0.0);
+        out.R[k] = spike ? pre[k * QUAD_PRE_LS] : (k % 4 == 0 ? 1.0 : 0.0);
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
         out.m.df[k] = (double)dp[k];
-        out.acc[k] = spike ? pre[(9 + k) * WAVE] : 0.0;
+        out.acc[k] = spike ? pre[(9 + k) * QUAD_PRE_LS] : 0.0;
     }
 }
 
@@ -340,7 +348,7 @@ DDH_QUADFN void backsub_quarter(const Args &a, int lane, int q, const SweepCtl &
     const int cnt = ch.rows;
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, nfl = SC(SC_NFLOOR), lnfloor = log(nfl);
     const double capv = CAP_V * Vt;
-    const Lane Lv = lane_view(a, lane, c);
+    const Lane Lv = lane_view(a, lane, c, QUAD_LS);
     const size_t n = Lv.n;
     DDH_GLOBAL const S *fac = (DDH_GLOBAL const S *)(factors<S>(a) + lane);
     const double s0 = sv[0], s1 = sv[1], s2 = sv[2];

@@ -109,8 +109,9 @@ __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
 }
 
 // Four lanes per device (ddhip_quad.h): 16 devices per wavefront, devices without mobile ions
-// (a device with them is refused: status 2); the segment table's free columns hold lanes 1 and
-// 2's prefixes (the table has nseg * SEG_N >= 22 rows). Lane 0 of each device runs the Poisson start;
+// (a device with them is refused: status 2); lanes 4d..4d+3 hold device d. The local data share
+// holds the 16 segment tables and lanes 1 and 2's prefixes (quad_lds_doubles), 25 KB for seven
+// segments, so that four workgroups fit one CU. Lane 0 of each device runs the Poisson start;
 // the four lanes then run the same state machine. A step's records move by shuffles: the
 // meeting records within lanes (0, 1) and (2, 3), the middle's two maps from lanes 1 and 2 to
 // all four, the chunks' current sums within the pairs and then the halves' across them.
@@ -119,18 +120,18 @@ struct QuadMid {
 };
 
 __global__ void __launch_bounds__(64) solve_quad_kernel(Args a) {
-    extern __shared__ double lds[];  // a.nseg * SEG_N * WAVE doubles (16 device slots used)
-    const int t = threadIdx.x, q = t & 3, base = t & ~3;
-    const int lane = blockIdx.x * 16 + (t >> 2);  // the device
+    extern __shared__ double lds[];  // quad_lds_doubles(a.nseg) doubles
+    const int t = threadIdx.x, q = t & 3, base = t & ~3, slot = t >> 2;
+    const int lane = blockIdx.x * 16 + slot;  // the device
     const bool inb = lane < a.n;
     const bool refused = inb && quad_refused(a, lane);
     const bool valid = inb && !refused;
-    ldouble *lseg = (ldouble *)lds + (t >> 2);
-    ldouble *pre = (ldouble *)lds + quad_pre_column(t >> 2, q);
+    ldouble *lseg = (ldouble *)lds + slot;
+    ldouble *pre = (ldouble *)lds + quad_pre_column(a.nseg, slot, q);
     LaneState &S = a.st[valid ? 4 * (size_t)lane + q : 0];
     if (refused && q == 0) quad_refuse(a, lane);
     if (valid && q == 0) {
-        fill_segments(a, lane, lseg);
+        fill_segments(a, lane, lseg, QUAD_LS);
         poisson_lane(a, lane, S);
     }
     __syncthreads();  // the segment table and the Poisson start, for lanes 1..3
@@ -559,9 +560,10 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
     const size_t lds = (size_t)b->nseg * ddh::SEG_N * ddh::WAVE * sizeof(double);
     if (lanes == 4) {
         dim3 grid((b->n + 15) / 16), block(64);
+        const size_t qlds = ddh::quad_lds_doubles(b->nseg) * sizeof(double);
         (void)hipFuncSetAttribute((const void *)ddh::solve_quad_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(ddh::solve_quad_kernel, grid, block, lds, s, a);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)qlds);
+        hipLaunchKernelGGL(ddh::solve_quad_kernel, grid, block, qlds, s, a);
     } else if (lanes == 2) {
         dim3 grid((b->n + 31) / 32), block(64);
         (void)hipFuncSetAttribute((const void *)ddh::solve_pair_kernel,

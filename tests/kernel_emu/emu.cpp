@@ -121,13 +121,14 @@ static void run_pair(Args &a) {
 static void run_quad(Args &a) {
     const int nw = (a.n + 15) / 16;
     std::vector<char> started(nw, 0), done(nw, 0);
-    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * WAVE);
+    const size_t per = quad_lds_doubles(a.nseg);
+    std::vector<double> lds_all((size_t)nw * per);
     int remaining = nw;
     while (remaining > 0) {
         for (int w = 0; w < nw; ++w) {
             if (done[w]) continue;
             const int d0 = w * 16, d1 = d0 + 16 < a.n ? d0 + 16 : a.n;
-            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
+            double *lds = &lds_all[(size_t)w * per];
             if (!started[w]) {
                 for (int d = d0; d < d1; ++d) {
                     if (quad_refused(a, d)) {
@@ -135,7 +136,7 @@ static void run_quad(Args &a) {
                         for (int q = 0; q < 4; ++q) a.st[4 * d + q].phase = PH_DONE;
                         continue;
                     }
-                    fill_segments(a, d, lds + (d - d0));
+                    fill_segments(a, d, lds + (d - d0), QUAD_LS);
                     poisson_lane(a, d, a.st[4 * d]);
                     for (int q = 1; q < 4; ++q) a.st[4 * d + q] = a.st[4 * d];
                     for (int q = 0; q < 4; ++q) {
@@ -159,7 +160,7 @@ static void run_quad(Args &a) {
                 QuadStep st[4];
                 const double *l = lds + (d - d0);
                 for (int q = 0; q < 4; ++q) {
-                    double *pre = lds + quad_pre_column(d - d0, q);
+                    double *pre = lds + quad_pre_column(a.nseg, d - d0, q);
                     if (f64) quad_phase1<double>(a, d, q, l, pre, S[q], st[q]);
                     else quad_phase1<float>(a, d, q, l, pre, S[q], st[q]);
                 }
