@@ -380,17 +380,25 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     return J, ok.bool(), its.long(), grad
 
 
+#: The name of the applied voltage among the numbers a gradient is taken in: the reference
+#: differentiates in the voltages too (`jax.grad` with respect to `solve_batch`'s V).
+VOLTAGE = "V"
+
+
 def directions_for(wrt, n: int, nl: int) -> tuple[np.ndarray, np.ndarray]:
     """Unit directions in the builder's numbers: ``wrt`` names per-device numbers
     (`builder.SCALAR_FIELDS`, e.g. "W_L") or (name, layer) pairs of per-layer numbers
-    (`builder.LAYER_FIELDS`, e.g. ("mu_n", 1)); returns (dlayers (K, nl, 21, n), dscalars
-    (K, 9, n))."""
+    (`builder.LAYER_FIELDS`, e.g. ("mu_n", 1)), or "V" (`VOLTAGE`, a zero row here: it is no
+    number of the builder's, and `voltage_tangents` sets its tangents); returns (dlayers
+    (K, nl, 21, n), dscalars (K, 9, n))."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
 
     K = len(wrt)
     dl = np.zeros((K, nl, len(LAYER_FIELDS), n))
     ds = np.zeros((K, len(SCALAR_FIELDS), n))
     for k, w in enumerate(wrt):
+        if isinstance(w, str) and w == VOLTAGE:
+            continue  # no number of the builder's: `voltage_tangents` sets its row
         if isinstance(w, str):
             if w not in SCALAR_FIELDS:
                 raise ValueError(f"{w!r}: not one of the per-device numbers {SCALAR_FIELDS}")
@@ -410,13 +418,37 @@ def param_gradients(structures, params, wrt, voltages, tol=1e-10, max_iter=60, l
     src/jaxdd/batch.py:142-148, 650-737) for any of the numbers the GPU builder takes, each
     device's structure held. ``structures``, ``params``: the devices' (`device_parameters`);
     returns J (n, nV), converged, iterations and the gradient (n, nV, len(wrt)) as
-    `jv_gradients`."""
+    `jv_gradients`; "V" in ``wrt`` gives each voltage's dJ/dV (the shunt's G_shunt included)."""
     from .builder import build
 
     dl, ds = directions_for(wrt, len(structures), len(structures[0].i0))
     batch, tan = build(list(structures), list(params), (dl, ds), device=device)
-    return jv_gradients(batch, tan, voltages, tol=tol, max_iter=max_iter, lanes=lanes,
-                        precision=precision)
+    vk = [k for k, w in enumerate(wrt) if isinstance(w, str) and w == VOLTAGE]
+    if vk:
+        tan = voltage_tangents(batch, tan, vk)
+    J, ok, its, grad = jv_gradients(batch, tan, voltages, tol=tol, max_iter=max_iter,
+                                    lanes=lanes, precision=precision)
+    if vk:  # the shunt's current G_shunt V
+        grad[:, :, vk] += batch["G_shunt"][:, None, None].to(grad.dtype)
+    return J, ok, its, grad
+
+
+def voltage_tangents(batch, tan, rows):
+    """The tangents ``tan`` with directions ``rows`` set to the applied voltage's: V enters a
+    voltage's solve only through the contacts' potentials, V0 + elec V / 2 and VL - elec V / 2
+    (src/jaxdd/batch.py:255-259 `_bounds`), so its tangent is elec / 2 on the left contact's
+    and -elec / 2 on the right's, and each voltage's dJ/dV is that direction's derivative
+    (J(V) depends on its own voltage only)."""
+    sc = batch["_sc"]
+    elec = sc[SC_FIELDS.index("left_elec")]
+    out = {k: (v.clone() if isinstance(v, torch.Tensor) else np.array(v)) for k, v in tan.items()}
+    for k in rows:
+        for key in ("pt", "seg", "sc"):
+            out[key][k] = 0
+        e = elec if isinstance(out["sc"], torch.Tensor) else elec.cpu().numpy()
+        out["sc"][k][SC_FIELDS.index("V0")] = 0.5 * e
+        out["sc"][k][SC_FIELDS.index("VL")] = -0.5 * e
+    return out
 
 
 def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="exact",

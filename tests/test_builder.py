@@ -310,3 +310,32 @@ def test_gpu_gradients_with_mobile_ions(torch_cuda, oracle, emu):
     print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
     assert np.max(err) < 1e-5, err
     assert np.median(err) < 3e-7, err
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_in_the_voltage(torch_cuda):
+    """`param_gradients` with "V": each voltage's dJ/dV of devices with free mobile ions (the
+    shunt's G_shunt included) against central differences of the GPU's own curves in the
+    voltage (h = 1e-5 V), to 1e-5 of each curve's largest slope."""
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.batch import solve_batch
+    from jaxdd_amd.builder import build
+
+    n, NP = 16, 100
+    t = S.draw(n)
+    structs, params = zip(*[S.row_inputs(t, k, NP, ions=True) for k in range(n)])
+    V = S.VOLTAGES[::10].copy()
+    print(f"the gradient in the voltage: {n} devices", flush=True)
+    J, ok, _, g = A.param_gradients(structs, params, ["V", ("mu_n", 1)], V, tol=1e-12,
+                                    max_iter=200)
+    batch = build(list(structs), list(params))
+    h = 1e-5
+    Jp, okp, _ = solve_batch(batch, V + h, tol=1e-12, max_iter=200, strict=True)
+    Jm, okm, _ = solve_batch(batch, V - h, tol=1e-12, max_iter=200, strict=True)
+    assert _np(ok).all() and _np(okp).all() and _np(okm).all()
+    fd = (_np(Jp) - _np(Jm)) / (2 * h)
+    err = np.abs(_np(g)[:, :, 0] - fd) / np.max(np.abs(fd), axis=1, keepdims=True)
+    print(f"dJ/dV against differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
+    assert err.max() < 1e-5 and np.median(err) < 1e-6
+
