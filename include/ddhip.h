@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define DDH_ABI_VERSION 2
+#define DDH_ABI_VERSION 3
 
 /* Segments (runs of points of one material) per device: the solve keeps each wavefront's
  * segment tables in the compute unit's 160 KB local data share. */
@@ -88,10 +88,13 @@ int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_o
 
 /* ddh_solve_batch that also keeps each voltage's final state: states [nV][NP][3][n] doubles
  * (device memory), (V, ln n, ln p) of every interior point; voltages in the solve's order
- * (ascending in batch mode). The linearisation points of ddh_adjoint. */
+ * (ascending in batch mode). pre_states: NULL, or [NP][3][n] doubles that receive the pre-bias
+ * solution of each device with a pre-bias (batch mode; left as they are for the others). The
+ * linearisation points of ddh_adjoint (the pre-bias's for devices whose ions it freezes). */
 int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *opts,
                            double *J, uint8_t *ok, int32_t *its, int32_t *status, double *states,
-                           void *workspace, size_t workspace_bytes, void *hip_stream);
+                           double *pre_states, void *workspace, size_t workspace_bytes,
+                           void *hip_stream);
 
 /* The device memory the library holds for ddh_solve_batch_noalloc, per GPU it runs on: 256 MiB,
  * ddh_solve_batch's workspace for about 1,500 devices of NP = 400 (mixed precision). */
@@ -120,20 +123,22 @@ size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
  * src/jaxdd/batch.py:606-737 (`_transposed`, `_converged_jvp`, `_tangent`: the transposed
  * bordered solve of reverse mode), free mobile ions included (each species' normalisation and
  * number constraint border the system; the normalisations of a converged state follow from
- * its potential in closed form). theta reaches the kernel as the tangents of the packed batch
+ * its potential in closed form), and ions frozen at a pre-bias (batch.py:937-960: the scan's
+ * fixed charge is the pre-bias solution's ions, differentiated through that solution by a
+ * second transposed bordered solve at it; pre_states from ddh_solve_batch_states). theta reaches the kernel as the tangents of the packed batch
  * arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][22][n], dsc [ntheta][20][n]
  * (d array / d theta_k; every field the steady rows, the current and the ions' constraints
  * read, V_T and the generation's factor included). states and ok: from ddh_solve_batch_states
  * on the same batch and ascending V. grad: [n][nV][ntheta], of the current without the shunt
  * (the caller adds V d(1/R_shunt)/dtheta with the shunt's current); 0 where a voltage did not
  * converge (the reference's zero gradient there); NaN and status[d] = 1 where device d's
- * mobile ions are frozen at a pre-bias. nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each
+ * mobile ions are frozen at a pre-bias and pre_states is NULL. nseg <= DDH_ADJOINT_MAX_SEGMENTS; NP <= 2200 (each
  * device's arrays and tangents are staged in the local data share). Up to 16 directions take
  * one reverse pass over the parameters, more one forward pass each. bernoulli: as ddh_opts. */
 int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
-                const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
-                const double *dsc, int32_t bernoulli, double *grad, int32_t *status,
-                void *workspace, size_t workspace_bytes, void *hip_stream);
+                const double *pre_states, const uint8_t *ok, int32_t ntheta, const double *dpt,
+                const double *dseg, const double *dsc, int32_t bernoulli, double *grad,
+                int32_t *status, void *workspace, size_t workspace_bytes, void *hip_stream);
 
 /* Workspace bytes ddh_adjoint_vjp needs. */
 size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t ntheta);
@@ -144,11 +149,11 @@ size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_
  * dJbar [n][nV] the loss's derivative in each curve's current. Arguments otherwise as
  * ddh_adjoint (its gradient kept in the workspace and contracted on the device); status and
  * the zero contribution of unconverged voltages as ddh_adjoint's (NaN for a device whose
- * mobile ions are frozen at a pre-bias). workspace: ddh_adjoint_vjp_workspace_bytes(n, NP, nV, ntheta) bytes. */
+ * mobile ions are frozen at a pre-bias, without pre_states). workspace: ddh_adjoint_vjp_workspace_bytes(n, NP, nV, ntheta) bytes. */
 int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV, const double *states,
-                    const uint8_t *ok, int32_t ntheta, const double *dpt, const double *dseg,
-                    const double *dsc, int32_t bernoulli, const double *dJbar,
-                    double *theta_bar, int32_t *status, void *workspace,
+                    const double *pre_states, const uint8_t *ok, int32_t ntheta,
+                    const double *dpt, const double *dseg, const double *dsc, int32_t bernoulli,
+                    const double *dJbar, double *theta_bar, int32_t *status, void *workspace,
                     size_t workspace_bytes, void *hip_stream);
 
 /* The batched block-tridiagonal sweep: replaces src/jaxdd/sweep.py:313-330 (`block_sweep`,

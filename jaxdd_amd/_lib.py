@@ -72,12 +72,12 @@ def load():
                                     vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_solve_batch_states.restype = ctypes.c_int
     lib.ddh_solve_batch_states.argtypes = [ctypes.POINTER(Batch), vp, i32, ctypes.POINTER(Opts),
-                                           vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+                                           vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_adjoint_workspace_bytes.restype = ctypes.c_size_t
     lib.ddh_adjoint_workspace_bytes.argtypes = [i32, i32, i32]
     lib.ddh_adjoint.restype = ctypes.c_int
-    lib.ddh_adjoint.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, i32, vp, vp, vp, i32, vp,
-                                vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_adjoint.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, vp, i32, vp, vp, vp, i32,
+                                vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_assemble.restype = ctypes.c_int
     lib.ddh_assemble.argtypes = [ctypes.POINTER(Batch), vp, vp, vp, i32, vp, vp, vp, vp, vp, vp,
                                  vp]
@@ -108,8 +108,8 @@ def load():
     lib.ddh_adjoint_vjp_workspace_bytes.restype = ctypes.c_size_t
     lib.ddh_adjoint_vjp_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.ddh_adjoint_vjp.restype = ctypes.c_int
-    lib.ddh_adjoint_vjp.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, i32, vp, vp, vp, i32,
-                                    vp, vp, vp, vp, ctypes.c_size_t, vp]
+    lib.ddh_adjoint_vjp.argtypes = [ctypes.POINTER(Batch), vp, i32, vp, vp, vp, i32, vp, vp, vp,
+                                    i32, vp, vp, vp, vp, ctypes.c_size_t, vp]
     lib.ddh_graded_layout.restype = ctypes.c_int
     lib.ddh_graded_layout.argtypes = [i32, i32, i32, vp, ctypes.c_double, ctypes.c_double, vp,
                                       vp, vp, vp, vp, vp, vp, vp]
@@ -118,8 +118,8 @@ def load():
                               vp, vp, vp, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
-    if lib.ddh_abi_version() != 2:
-        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 2")
+    if lib.ddh_abi_version() != 3:
+        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 3")
     _lib = lib
     return lib
 

@@ -17,8 +17,9 @@ The HIP adjoint kernel (`ddh_adjoint`) then linearises each converged voltage's 
 current in those directions and solves one transposed block-tridiagonal system per voltage;
 with free mobile ions that system is bordered by each species' normalisation and number
 constraint (src/jaxdd/batch.py:595-618 `_transposed`), and `param_gradients` takes any of the
-builder's named numbers, the ions' densities and the temperature among them. Ions frozen at a
-pre-bias are refused (`NotImplementedError`).
+builder's named numbers, the ions' densities and the temperature among them, and the applied
+voltage. Ions frozen at a pre-bias are differentiated through the pre-bias solution (a second
+transposed bordered solve at it; the solve keeps that solution).
 
 `curves(theta, V)` is the differentiable entry point: a `torch.autograd.Function` whose
 forward solves the curves and whose backward contracts the incoming dL/dJ with dJ/dtheta.
@@ -281,16 +282,19 @@ def _lanes(batch, lanes):
     return 4
 
 
-FROZEN_REFUSED = ("gradients of devices whose mobile ions are frozen at a pre-bias are outside "
-                  "this adjoint (the scan's fixed charge is the pre-bias's solution)")
+FROZEN_REFUSED = "the adjoint refused a device whose ions are frozen at a pre-bias"
 
 
-def _check_not_frozen(batch) -> None:
+def _pre_states(batch):
+    """Room for the pre-bias solutions (NP, 3, n) where some device of the batch has its
+    mobile ions frozen at its pre-bias (the adjoint's second linearisation point), else None."""
     sc = batch["_sc"]
     ions = (sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)
     frozen = (sc[SC_FIELDS.index("pre_cond")] > 0) & (sc[SC_FIELDS.index("fix_ions")] > 0)
-    if bool((ions & frozen).any()):
-        raise NotImplementedError(FROZEN_REFUSED)
+    if not bool((ions & frozen).any()):
+        return None
+    return torch.zeros((batch["NP"], 3, batch["n"]), dtype=torch.float64,
+                       device=batch["_pt"].device)
 
 
 def _shunt_tangents(dsc):
@@ -308,7 +312,8 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     Unconverged voltages get zero gradients (but for the shunt's current G_shunt V, added to
     every voltage's J). ``packed``, ``tan``: host arrays (`tangents`), or
     the device-resident pair `resident` returns. ``return_states``: also the linearisation
-    points (nV, NP, 3, n). ``events``: four torch.cuda.Event recorded on the launch stream
+    points (nV, NP, 3, n) and the pre-bias solutions (NP, 3, n) of a batch with ions frozen
+    at a pre-bias (None for another). ``events``: four torch.cuda.Event recorded on the launch stream
     around the solve kernel and around the adjoint kernel. ``lanes``: GPU lanes per device of
     the solve (4: the grid in quarters, the fastest for the batches gradients are taken of;
     2; 1; None: 4 where the four-lane kernel takes the batch, else 2), as
@@ -327,7 +332,6 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     if precision not in _lib.PRECISIONS:
         raise ValueError(f"precision {precision!r}: one of {tuple(_lib.PRECISIONS)}")
     prec = _lib.PRECISIONS[precision]
-    _check_not_frozen(batch)
     if lanes == 4:
         check_four_lanes(batch)
     opts = _lib.Opts(float(tol), int(max_iter), int(max_iter), int(bool(strict)),
@@ -339,6 +343,7 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
     status = torch.zeros(n, dtype=torch.int32, device=dev)
     states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
+    pre = _pre_states(batch)
     nbytes = lib.ddh_workspace_bytes(n, NP, nV, prec)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     # the per-device keys the caller may have replaced (g_frac, J_scale, V_pre), as solve_batch
@@ -356,14 +361,15 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     aws = torch.empty(abytes, dtype=torch.uint8, device=dev)
     if events is not None:
         events[0].record()
+    pp = None if pre is None else pre.data_ptr()
     _lib.check(lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
                                           J.data_ptr(), ok.data_ptr(), its.data_ptr(),
-                                          status.data_ptr(), states.data_ptr(), ws.data_ptr(),
+                                          status.data_ptr(), states.data_ptr(), pp, ws.data_ptr(),
                                           nbytes, stream), "ddh_solve_batch_states")
     if events is not None:
         events[1].record()
         events[2].record()
-    _lib.check(lib.ddh_adjoint(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(),
+    _lib.check(lib.ddh_adjoint(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(), pp,
                                ok.data_ptr(), K, dpt.data_ptr(), dseg.data_ptr(), dsc.data_ptr(),
                                opts.bernoulli, grad.data_ptr(), astatus.data_ptr(), aws.data_ptr(),
                                abytes, stream), "ddh_adjoint")
@@ -376,7 +382,7 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     J = J + batch["G_shunt"][:, None] * Vd[None, :]
     grad = grad + Vd[None, :, None] * _shunt_tangents(dsc).T[:, None, :]
     if return_states:
-        return J, ok.bool(), its.long(), grad, states
+        return J, ok.bool(), its.long(), grad, states, pre
     return J, ok.bool(), its.long(), grad
 
 
@@ -467,7 +473,6 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
     Jb = torch.as_tensor(np.asarray(dJbar, dtype=float), device=dev).contiguous()
     if tuple(Jb.shape) != (n, nV):
         raise ValueError(f"dJbar: ({n}, {nV})")
-    _check_not_frozen(batch)
     lanes = _lanes(batch, lanes)
     if lanes == 4:
         check_four_lanes(batch)
@@ -479,6 +484,8 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
     its = torch.zeros((n, nV), dtype=torch.int32, device=dev)
     status = torch.zeros(n, dtype=torch.int32, device=dev)
     states = torch.empty((nV, NP, 3, n), dtype=torch.float64, device=dev)
+    pre = _pre_states(batch)
+    pp = None if pre is None else pre.data_ptr()
     nbytes = lib.ddh_workspace_bytes(n, NP, nV, _lib.DDH_FP64)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     sc = _sc_now(batch)
@@ -487,13 +494,13 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
     stream = torch.cuda.current_stream(dev).cuda_stream
     _lib.check(lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
                                           J.data_ptr(), ok.data_ptr(), its.data_ptr(),
-                                          status.data_ptr(), states.data_ptr(), ws.data_ptr(),
+                                          status.data_ptr(), states.data_ptr(), pp, ws.data_ptr(),
                                           nbytes, stream), "ddh_solve_batch_states")
     tbar = torch.empty((n, K), dtype=torch.float64, device=dev)
     astatus = torch.zeros(n, dtype=torch.int32, device=dev)
     vbytes = lib.ddh_adjoint_vjp_workspace_bytes(n, NP, nV, K)
     vws = torch.empty(vbytes, dtype=torch.uint8, device=dev)
-    _lib.check(lib.ddh_adjoint_vjp(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(),
+    _lib.check(lib.ddh_adjoint_vjp(ctypes.byref(b), Vd.data_ptr(), nV, states.data_ptr(), pp,
                                    ok.data_ptr(), K, tan["pt"].data_ptr(), tan["seg"].data_ptr(),
                                    tan["sc"].data_ptr(), opts.bernoulli, Jb.data_ptr(),
                                    tbar.data_ptr(), astatus.data_ptr(), vws.data_ptr(), vbytes,

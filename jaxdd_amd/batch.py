@@ -232,8 +232,8 @@ def _run(batch: dict, V: np.ndarray, opts: _lib.Opts, precision: int, events=Non
             raise ValueError(f"states: float64 of shape {(nV, NP, 3, n)}")
         rc = lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), nV, ctypes.byref(opts),
                                         J.data_ptr(), ok.data_ptr(), its.data_ptr(),
-                                        status.data_ptr(), states.data_ptr(), ws.data_ptr(),
-                                        nbytes, stream)
+                                        status.data_ptr(), states.data_ptr(), None,
+                                        ws.data_ptr(), nbytes, stream)
         _lib.check(rc, "ddh_solve_batch_states")
     if events is not None:
         events[1].record()

@@ -18,8 +18,10 @@
 // so any parameter of the packed arrays can be differentiated. With mobile ions the system is
 // bordered by each species' normalisation and number constraint, and the transposed solve
 // gives the constraints' multipliers mu beside lambda (adj_lambda): dJ/dtheta then also takes
-// -mu . dg/dtheta. Ions frozen at a pre-bias are not differentiated here (status 1, gradients
-// NaN): the scan's fixed charge then depends on the pre-bias's solution.
+// -mu . dg/dtheta. Ions frozen at a pre-bias make the scan's fixed charge the pre-bias
+// solution's ions: a second pass per voltage solves the same bordered transpose at that
+// solution (adj_lambda's PRE) and adds its part; without the pre-bias solutions (pre NULL)
+// such a device is refused (status 1, gradients NaN).
 #pragma once
 
 #include "ddhip_core.h"
@@ -58,6 +60,8 @@ struct AdjArgs {
     const double *sc;
     const double *V;        // [nV] (ascending, as the solve's)
     const double *states;   // [nV][NP][3][n] the solve's final states
+    const double *pre;      // [NP][3][n] the pre-bias solutions (devices whose ions are frozen
+                            // at them; NULL: such devices refused)
     const uint8_t *ok;      // [n][nV]
     const double *dpt;      // [ntheta][NP + 2][PT_N][n]  tangents of the packed arrays
     const double *dseg;     // [ntheta][nseg][SEG_N][n]
@@ -71,6 +75,7 @@ struct AdjArgs {
     double *iu;             // [NP + 1][ls] the current's running integral of U
     double *ts;             // [NP + 2][2][ls] the current's suffix sums of the point weights
     double *ion;            // [4][ls] the ions' normalisations l_a, l_c and their multipliers
+    double *nu;             // [NP][ls] dJ/d(fixed charge) of a scan with frozen ions
     size_t ls;
     int passes;             // 1: one pass per direction even where a reverse pass would do
 };
@@ -247,13 +252,57 @@ DDH_FN void adj_stage_direction(const AdjArgs &a, int d, int k, ldouble *tab, in
 #define YS(j, f) a.states[(((size_t)v * NP + ((j) - 1)) * 3 + (f)) * n + d]
 
 #define ION(f) a.ion[(size_t)(f) * ls + sl]
+#define NU(j) a.nu[(size_t)((j) - 1) * ls + sl]
+#define PS(j, f) a.pre[(((size_t)(j) - 1) * 3 + (f)) * n + d]
 
-// Whether device d's gradients are refused: its mobile ions frozen at a pre-bias (the scan's
-// fixed charge then depends on the pre-bias's solution, outside this adjoint).
-DDH_FN bool adj_refused(const AdjArgs &a, int d) {
+// Whether device d's mobile ions are frozen at its pre-bias (src/jaxdd/batch.py:859-872,
+// 955-956): the scan then has no mobile ions, and its fixed charge is the pre-bias solution's
+// ions, a function of that solution and so of the parameters through it.
+DDH_FN bool adj_frozen(const AdjArgs &a, int d) {
     const size_t n = (size_t)a.n;
     const bool mobile = a.sc[(size_t)SC_AT * n + d] > 0 || a.sc[(size_t)SC_CT * n + d] > 0;
     return mobile && a.sc[(size_t)SC_PRE * n + d] > 0 && a.sc[(size_t)SC_FIX * n + d] > 0;
+}
+
+// Whether device d's gradients are refused: its ions frozen, and no pre-bias solution given.
+DDH_FN bool adj_refused(const AdjArgs &a, int d) { return adj_frozen(a, d) && !a.pre; }
+
+// The closed-form normalisations of device d's pre-bias solution (as adj_lambda's of a scan
+// state): the logarithms its frozen ions' densities take.
+DDH_FN void adj_pre_logs(const AdjArgs &a, int d, lcdouble *tab, double lg[2]) {
+    const int NP = a.NP;
+    const size_t n = (size_t)a.n;
+    const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
+    const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), elec = TSC(SC_ELEC);
+    const double Vpre = TSC(SC_VPRE);
+    const double V0c = TSC(SC_V0) + 0.5 * elec * Vpre, VLc = TSC(SC_VL) - 0.5 * elec * Vpre;
+    double num0 = 0.0, num1 = 0.0, Vl = V0c;
+    for (int j = 1; j <= NP + 1; ++j) {
+        const double Vr = j <= NP ? PS(j, 0) : VLc;
+        const int il = TSID(j - 1), ir = TSID(j);
+        const double aml = TSEG(il, SEG_AM), amr = TSEG(ir, SEG_AM);
+        const double cml = TSEG(il, SEG_CM), cmr = TSEG(ir, SEG_CM);
+        const double hL = TPT(j - 1, PT_H) * L;
+        num0 += 0.5 * (aml * exp(Vl * iVt) + amr * exp(Vr * iVt)) * (aml * amr) * hL;
+        num1 += 0.5 * (cml * exp(-Vl * iVt) + cmr * exp(-Vr * iVt)) * (cml * cmr) * hL;
+        Vl = Vr;
+    }
+    lg[0] = TSC(SC_AT) > 0 ? log(TSC(SC_AT) / num0) : 0.0;
+    lg[1] = TSC(SC_CT) > 0 ? log(TSC(SC_CT) / num1) : 0.0;
+}
+
+// Point j's frozen ions (the anions' density less the cations'), from the pre-bias solution's
+// potential there and its normalisations lg (the solver's freeze: make_point).
+DDH_FN double adj_frozen_ions(const AdjArgs &a, int d, lcdouble *tab, int j, const double lg[2],
+                              double &an, double &ca) {
+    const size_t n = (size_t)a.n;
+    const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
+    const double iVt = 1.0 / TSC(SC_VT), V = PS(j, 0);
+    const int id = TSID(j);
+    const double am = TSEG(id, SEG_AM), cm = TSEG(id, SEG_CM);
+    an = am != 0.0 ? am * exp(lg[0] + V * iVt) : 0.0;
+    ca = cm != 0.0 ? cm * exp(lg[1] - V * iVt) : 0.0;
+    return an - ca;
 }
 
 // Device d at converged voltage v, by the launched lane sl: lambda and, with mobile ions, the
@@ -266,17 +315,36 @@ DDH_FN bool adj_refused(const AdjArgs &a, int d) {
 // X's d', the gammas of W (gamma_i = W_i - c'_{i+1}^T gamma_{i+1}, as w of dJ/dy), and by
 // sum_i v_i . x_i = sum_i gamma(v)_i . d'_i the contractions (dJ/dy)^T X and W^T X; lambda's
 // substitution then takes w - mu_a gamma_a - mu_c gamma_c.
-DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
+// PRE (a device whose ions are frozen at its pre-bias, after the scan's pass at v): the same
+// transposed bordered system at the pre-bias solution, with the right-hand side the scan's
+// dJ/d(fixed charge) b_j = q sP_j lambda_j,V (kept in NU) carried through the frozen ions'
+// dependence on that solution: b_j (a_j + c_j) / V_T in the potential's rows, and
+// (sum b_j a_j, -sum b_j c_j) in the border's, so that the pre-bias solution's part of
+// dJ/dtheta is -lambda_pre . dF_pre/dtheta - mu_pre . dg_pre/dtheta (no current's part).
+DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab,
+                       bool pre = false) {
     const AdjTab t(a.nseg, a.NP, adj_k(a.ntheta, a.passes));
     const int NP = a.NP;
     const size_t n = (size_t)a.n, ls = a.ls;
     const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), L2 = L * L;
     const double gfr = TSC(SC_GFRAC), elec = TSC(SC_ELEC), cJ = -elec * Q_E;
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
-    const double Vint = a.V[v];
+    const double Vint = pre ? TSC(SC_VPRE) : a.V[v];
     const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
     const double tot0 = TSC(SC_AT), tot1 = TSC(SC_CT);
-    const bool ions = tot0 > 0 || tot1 > 0;  // (the same for every lane of the wavefront)
+    // (all of these the same for every lane of the wavefront: one device)
+    const bool frozen = adj_frozen(a, d);
+    const bool ions = (tot0 > 0 || tot1 > 0) && (pre || !frozen);  // the pass's mobile ions
+    auto ystate = [&](int j, int f) -> double { return pre ? PS(j, f) : YS(j, f); };
+    double lpre[2] = {0.0, 0.0};  // a frozen scan's fixed ions: the pre-bias's normalisations
+    if (frozen && !pre) {
+        adj_pre_logs(a, d, tab, lpre);
+        ION(0) = lpre[0];
+        ION(1) = lpre[1];
+    }
+    if (pre) {  // the scan's dJ/d(fixed charge), before its lambda is overwritten
+        for (int j = 1; j <= NP; ++j) NU(j) = Q_E * TSEG(TSID(j), SEG_SP) * LAM(j, 0);
+    }
     // the current's sums (the Newton sweep's order), the state's densities kept for the passes;
     // with ions each species' number at l = 0 (the constraint's trapezoid, solver.py:1241-1265)
     const double Vg0n = V0c + TSEG(sid0, SEG_OFFN), Vg0p = V0c + TSEG(sid0, SEG_OFFP);
@@ -293,13 +361,14 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         num0 += 0.5 * (aml * exp(Vl * iVt) + amr * exp(Vr * iVt)) * (aml * amr) * hL;
         num1 += 0.5 * (cml * exp(-Vl * iVt) + cmr * exp(-Vr * iVt)) * (cml * cmr) * hL;
     };
-    IU(0) = 0.0;
+    if (!pre) IU(0) = 0.0;
     for (int j = 1; j <= NP; ++j) {
         const int id = TSID(j);
-        const double y0 = YS(j, 0), nn = exp(YS(j, 1)), pp = exp(YS(j, 2));
+        const double y0 = ystate(j, 0), nn = exp(ystate(j, 1)), pp = exp(ystate(j, 2));
         STV(j, 0) = y0; STV(j, 1) = nn; STV(j, 2) = pp;
         if (ions) count(j - 1, j, V_prev, y0);
         V_prev = y0;
+        if (pre) continue;  // no current at the pre-bias
         const double R = recombination_value(nn, pp, TSEG(id, SEG_KD), TSEG(id, SEG_NI2),
                                              TSEG(id, SEG_NT), TSEG(id, SEG_CN), TSEG(id, SEG_CP),
                                              TSEG(id, SEG_NT0), TSEG(id, SEG_PT0));
@@ -325,14 +394,17 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         ION(0) = lg0;
         ION(1) = lg1;
     }
-    const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
-    const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
-    const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
-    const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
     // J = cJ (Kn + Kp): its partials in the four sums (the sums kept for the reverse pass in
-    // the suffix sums' unused ends)
-    const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
-    TS(0, 0) = Sn; TS(0, 1) = Sp; TS(NP + 1, 0) = Dn; TS(NP + 1, 1) = Dp;
+    // the suffix sums' unused ends); none at the pre-bias
+    double bSn = 0.0, bDn = 0.0, bSp = 0.0, bDp = 0.0;
+    if (!pre) {
+        const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
+        const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
+        const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
+        const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
+        bSn = -cJ * Kn / Sn; bDn = cJ / Sn; bSp = -cJ * Kp / Sp; bDp = -cJ / Sp;
+        TS(0, 0) = Sn; TS(0, 1) = Sp; TS(NP + 1, 0) = Dn; TS(NP + 1, 1) = Dp;
+    }
 
     // elimination, right to left, with dJ/dy and w = L'^-T dJ/dy
     auto point = [&](int j, Pt &q) {
@@ -351,6 +423,10 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
             q.p = STV(j, 2);
             q.G = TPT(j, PT_G);
             q.fix = TPT(j, PT_FIX);
+            if (frozen && !pre) {
+                double fa, fc;
+                q.fix += adj_frozen_ions(a, d, tab, j, lpre, fa, fc);
+            }
             q.R = recombination(q.n, q.p, TSEG(q.sid, SEG_KD), TSEG(q.sid, SEG_NI2),
                                 TSEG(q.sid, SEG_NT), TSEG(q.sid, SEG_CN), TSEG(q.sid, SEG_CP),
                                 TSEG(q.sid, SEG_NT0), TSEG(q.sid, SEG_PT0), q.dRn, q.dRp);
@@ -369,6 +445,7 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
     // i + 1, and the contractions (dJ/dy)^T X, W^T X ([constraint][species])
     double dun[3][2] = {{0, 0}, {0, 0}, {0, 0}}, gwn[2][3] = {{0, 0, 0}, {0, 0, 0}};
     double cX[2] = {0, 0}, WX[2][2] = {{0, 0}, {0, 0}};
+    double rl[2] = {0, 0};  // the border's right-hand side (the pre-bias pass's)
     const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
     double Tn1 = 0.0, Tp1 = 0.0;  // the current's suffix sums from point i + 1
     double dx_next = 0.0;         // no interval after NP in the sums
@@ -389,21 +466,31 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
                 M[3 * p + q] = B[3 * p + q] - (C[3 * p] * cpn[q] + C[3 * p + 1] * cpn[3 + q] +
                                                C[3 * p + 2] * cpn[6 + q]);
         inverse3d(M, Mi);
-        // dJ/dy_i by the current's reverse mode
+        // the right-hand side: dJ/dy_i by the current's reverse mode; at the pre-bias, the
+        // scan's dJ/d(fixed charge) through the frozen ions
         const double xi = TPT(i, PT_X), dx = xi - TPT(i - 1, PT_X);
-        const double ai = 0.5 * (dx + dx_next);
-        const double wn = exp(-(mid.V + mid.offn) * iVt) * TSEG(mid.sid, SEG_IMUN);
-        const double wp = exp((mid.V + mid.offp) * iVt) * TSEG(mid.sid, SEG_IMUP);
-        const double Tn = ai * wn + Tn1, Tp = ai * wp + Tp1;
-        TS(i, 0) = Tn;
-        TS(i, 1) = Tp;
-        const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
-                          bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
-        const double iu = IU(i);
-        double gv[3];
-        gv[0] = (bSn + bDn * iu) * ai * (-wn * iVt) + (bSp + bDp * iu) * ai * (wp * iVt);
-        gv[1] = bU * mid.dRn;
-        gv[2] = bU * mid.dRp;
+        double gv[3], Tn = 0.0, Tp = 0.0;
+        if (pre) {
+            const double b = NU(i);
+            gv[0] = b * (mid.an + mid.ca) * iVt;
+            gv[1] = gv[2] = 0.0;
+            rl[0] += b * mid.an;
+            rl[1] -= b * mid.ca;
+        } else {
+            const double ai = 0.5 * (dx + dx_next);
+            const double wn = exp(-(mid.V + mid.offn) * iVt) * TSEG(mid.sid, SEG_IMUN);
+            const double wp = exp((mid.V + mid.offp) * iVt) * TSEG(mid.sid, SEG_IMUP);
+            Tn = ai * wn + Tn1;
+            Tp = ai * wp + Tp1;
+            TS(i, 0) = Tn;
+            TS(i, 1) = Tp;
+            const double bU = bDn * (0.5 * dx * Tn + 0.5 * dx_next * Tn1) +
+                              bDp * (0.5 * dx * Tp + 0.5 * dx_next * Tp1);
+            const double iu = IU(i);
+            gv[0] = (bSn + bDn * iu) * ai * (-wn * iVt) + (bSp + bDp * iu) * ai * (wp * iVt);
+            gv[1] = bU * mid.dRn;
+            gv[2] = bU * mid.dRp;
+        }
         // w_i = g_i - c'_{i+1}^T w_{i+1}
         double w[3];
 #pragma unroll
@@ -470,12 +557,12 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
         Tn1 = Tn; Tp1 = Tp; dx_next = dx;
         right = mid; mid = cur; eR = eL;
     }
-    // the multipliers: (D - W^T X)^T mu = -(dJ/dy)^T X, D = 1 (a held species' row too: its
-    // W is zero, and so its constraint's derivatives)
+    // the multipliers: (D - W^T X)^T mu = r_l - (dJ/dy)^T X, D = 1 (a held species' row too:
+    // its W is zero, and so its constraint's derivatives)
     double mu0 = 0.0, mu1 = 0.0;
     if (ions) {
-        mu0 = -cX[0];
-        mu1 = -cX[1];
+        mu0 = rl[0] - cX[0];
+        mu1 = rl[1] - cX[1];
         solve2(1.0 - WX[0][0], -WX[1][0], -WX[0][1], 1.0 - WX[1][1], mu0, mu1);
         ION(2) = mu0;
         ION(3) = mu1;
@@ -508,19 +595,22 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab)
 
 // dJ/dtheta_k = dJ/dtheta_k|_y - lambda . dF/dtheta_k|_y - mu . dg/dtheta_k|_y at device d's
 // converged voltage v (adj_lambda's workspace of lane sl), with direction k's tangents staged in
-// the LDS table.
-DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab) {
-    (void)d;
+// the LDS table. PRE: the pre-bias solution's part, -lambda . dF/dtheta_k - mu . dg/dtheta_k
+// at it (adj_lambda's PRE pass), the pre-bias voltage's tangent included.
+DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab,
+                            bool pre = false) {
     const int NP = a.NP;
-    const size_t ls = a.ls;
+    const size_t n = (size_t)a.n, ls = a.ls;
     const AdjTab t(a.nseg, a.NP);
     const int S = t.S;
     lcdouble *tsc = tab + t.sct;
     const double elec = tsc[SC_ELEC];
-    const double cJ = -elec * Q_E, Vint = a.V[v];
+    const double cJ = -elec * Q_E;
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
-    const bool ions = tsc[SC_AT] > 0 || tsc[SC_CT] > 0;
+    const bool frozen = adj_frozen(a, d);
+    const bool ions = (tsc[SC_AT] > 0 || tsc[SC_CT] > 0) && (pre || !frozen);
     const double lg0 = ions ? ION(0) : 0.0, lg1 = ions ? ION(1) : 0.0;
+    const double lpre[2] = {frozen ? ION(0) : 0.0, frozen ? ION(1) : 0.0};
     {
         SegD sd;
         sd.val = tab;
@@ -530,7 +620,20 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
         };
         auto scd = [&](int f) -> D1 { return {tsc[f], tsc[SC_N + f]}; };
         const D1 Ld = scd(SC_L), Vt = scd(SC_VT), iVt = dc(1.0) / Vt, gfr = scd(SC_GFRAC);
+        const D1 Vint = pre ? scd(SC_VPRE) : dc(a.V[v]);
         const D1 V0d = scd(SC_V0) + 0.5 * elec * Vint, VLd = scd(SC_VL) - 0.5 * elec * Vint;
+        // a frozen scan's fixed charge at point j: the pre-bias solution's ions (its potential
+        // and normalisations held), through V_T
+        auto fixd = [&](int j) -> D1 {
+            D1 f = ptd(j, PT_FIX);
+            if (frozen && !pre) {
+                const int id = TSID(j);
+                const double am = TSEG(id, SEG_AM), cm = TSEG(id, SEG_CM), Vp = PS(j, 0);
+                if (am != 0.0) f = f + am * dexp(lpre[0] + Vp * iVt);
+                if (cm != 0.0) f = f - cm * dexp(lpre[1] - Vp * iVt);
+            }
+            return f;
+        };
         auto pointd = [&](int j, PD &q) {
             q.sid = TSID(j);
             q.h = ptd(j, PT_H);
@@ -570,7 +673,7 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
             if (ions) count(l, m);
             // row j
             D1 F[3];
-            row_d1(l, m, r, ptd(j, PT_G), ptd(j, PT_FIX), Ld, gfr, sd, Vt, iVt, a.bern, F);
+            row_d1(l, m, r, ptd(j, PT_G), fixd(j), Ld, gfr, sd, Vt, iVt, a.bern, F);
             acc += LAM(j, 0) * F[0].d + LAM(j, 1) * F[1].d + LAM(j, 2) * F[2].d;
             // the current's terms at point j
             const D1 R = recomb_d1(m.n, m.p, sd, m.sid);
@@ -591,6 +694,7 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
             if (tsc[SC_AT] > 0) acc += ION(2) * (num0 / scd(SC_AT)).d;
             if (tsc[SC_CT] > 0) acc += ION(3) * (num1 / scd(SC_CT)).d;
         }
+        if (pre) return -acc;  // no current at the pre-bias
         const D1 endnNd = dexp(scd(SC_LNR) - (VLd + sd(sidN, SEG_OFFN)) * iVt);
         const D1 endpNd = dexp(scd(SC_LPR) + (VLd + sd(sidN, SEG_OFFP)) * iVt);
         const D1 Knd = (Vt * (endn0d - endnNd) + Dnd) / Snd;
@@ -606,18 +710,19 @@ DDH_FN double adj_direction(const AdjArgs &a, int d, int v, size_t sl, lcdouble 
 // directions' tangents (from the LDS table) into gk; the points' are kept in the lane's
 // workspace for adj_contract_points. The same derivative as adj_direction's K passes: the rows
 // of row_d1 and the current of adj_lambda, reversed.
-DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab, double *gk) {
-    (void)d;
+DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab, double *gk,
+                         bool pre = false) {
     const int NP = a.NP, K = adj_k(a.ntheta, a.passes);
-    const size_t ls = a.ls;
+    const size_t n = (size_t)a.n, ls = a.ls;
     const AdjTab t(a.nseg, NP, K);
     const int S = t.S;
     const double Vt = TSC(SC_VT), iVt = 1.0 / Vt, L = TSC(SC_L), L2 = L * L;
     const double gfr = TSC(SC_GFRAC), elec = TSC(SC_ELEC), cJ = -elec * Q_E;
     const int sid0 = TSID(0), sidN = TSID(NP + 1);
-    const double Vint = a.V[v];
+    const double Vint = pre ? TSC(SC_VPRE) : a.V[v];
     const double V0c = TSC(SC_V0) + 0.5 * elec * Vint, VLc = TSC(SC_VL) - 0.5 * elec * Vint;
     double bL = 0.0, bVt = 0.0, bgfr = 0.0;  // the partials in L, V_T and the generation's factor
+    const bool frozen = adj_frozen(a, d);
     auto seg_now = [&](int id, int f, double b) {
         lcdouble *tg = tab + t.segT + id * SEG_N + f;
 #pragma unroll
@@ -643,11 +748,16 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         if (id == run) acc[f] += b;
         else seg_now(id, f, b);
     };
-    auto sc_add = [&](int f, double b) {
+    auto sc_one = [&](int f, double b) {
         lcdouble *tg = tab + t.scT + f;
 #pragma unroll
         for (int k = 0; k < ADJ_MAX_K; ++k)
             if (k < K) gk[k] += b * tg[k * SC_N];
+    };
+    // at the pre-bias the contacts' potentials also move with the pre-bias voltage
+    auto sc_add = [&](int f, double b) {
+        sc_one(f, b);
+        if (pre && (f == SC_V0 || f == SC_VL)) sc_one(SC_VPRE, (f == SC_V0 ? 0.5 : -0.5) * elec * b);
     };
     // the points' partials: into the lane's workspace (the elimination's factors, free once
     // lambda is known), [point 0..NP][field], contracted by adj_contract_points
@@ -655,16 +765,18 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
     double pendH = 0.0, pendX = 0.0;  // point j - 1's h and x partials, from rows j - 1, j
     bp(0, PT_G) = 0.0;
     bp(0, PT_FIX) = 0.0;
-    // J = cJ (Kn + Kp) and its partials in the sums and the contacts' terms
-    const double Sn = TS(0, 0), Sp = TS(0, 1), Dn = TS(NP + 1, 0), Dp = TS(NP + 1, 1);
+    // J = cJ (Kn + Kp) and its partials in the sums and the contacts' terms (none at the
+    // pre-bias: every partial of the current below carries one of the four seeds)
     const double Vg0n = V0c + TSEG(sid0, SEG_OFFN), Vg0p = V0c + TSEG(sid0, SEG_OFFP);
-    const double endn0 = exp(TSC(SC_LNL) - Vg0n * iVt), endp0 = exp(TSC(SC_LPL) + Vg0p * iVt);
-    const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
-    const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
-    const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
-    const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
-    const double bSn = -cJ * Kn / Sn, bDn = cJ / Sn, bSp = -cJ * Kp / Sp, bDp = -cJ / Sp;
-    {
+    double bSn = 0.0, bDn = 0.0, bSp = 0.0, bDp = 0.0;
+    if (!pre) {
+        const double Sn = TS(0, 0), Sp = TS(0, 1), Dn = TS(NP + 1, 0), Dp = TS(NP + 1, 1);
+        const double endn0 = exp(TSC(SC_LNL) - Vg0n * iVt), endp0 = exp(TSC(SC_LPL) + Vg0p * iVt);
+        const double endnN = exp(TSC(SC_LNR) - (VLc + TSEG(sidN, SEG_OFFN)) * iVt);
+        const double endpN = exp(TSC(SC_LPR) + (VLc + TSEG(sidN, SEG_OFFP)) * iVt);
+        const double Kn = (Vt * (-endnN + endn0) + Dn) / Sn;
+        const double Kp = (Vt * (endpN - endp0) - Dp) / Sp;
+        bSn = -cJ * Kn / Sn; bDn = cJ / Sn; bSp = -cJ * Kp / Sp; bDp = -cJ / Sp;
         const double VgNn = VLc + TSEG(sidN, SEG_OFFN), VgNp = VLc + TSEG(sidN, SEG_OFFP);
         const double iVt2 = iVt * iVt;
         bVt += cJ * ((endn0 - endnN) / Sn + (endpN - endp0) / Sp);  // Kn, Kp's factor V_T
@@ -725,8 +837,9 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
     // mobile ions: the rows' densities at the held normalisations, and the constraints'
     // partials, -mu . dg/dtheta with g_s = number_s / total_s - 1 (a held species: none)
     const double tot0 = TSC(SC_AT), tot1 = TSC(SC_CT);
-    const bool ions = tot0 > 0 || tot1 > 0;
+    const bool ions = (tot0 > 0 || tot1 > 0) && (pre || !frozen);  // the pass's mobile ions
     const double lg0 = ions ? ION(0) : 0.0, lg1 = ions ? ION(1) : 0.0;
+    const double lpre[2] = {frozen ? ION(0) : 0.0, frozen ? ION(1) : 0.0};
     const double ma = ions && tot0 > 0 ? -ION(2) / tot0 : 0.0;  // -mu_a / total_a
     const double mc = ions && tot1 > 0 ? -ION(3) / tot1 : 0.0;
     double nsum0 = 0.0, nsum1 = 0.0;
@@ -805,8 +918,10 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         const double elec_ = hm * FnR - hp * FnL + fac_ * (gen - R);
         const double hole = hp * FpL - hm * FpR + fac_ * (gen - R);
         const double anm = ion_an(sm, Vm), cam = ion_ca(sm, Vm);
+        double fixm = TPT(j, PT_FIX), fan = 0.0, fca = 0.0;  // a frozen scan's: with the ions
+        if (frozen && !pre) fixm += adj_frozen_ions(a, d, tab, j, lpre, fan, fca);
         const double pois = fac_l * Vl + fac_m * Vm + fu * Vr -
-                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + TPT(j, PT_FIX) + anm - cam);
+                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + fixm + anm - cam);
         // the row reversed, seeded with -lambda_j
         const double b0 = -LAM(j, 0), b1 = -LAM(j, 1), b2 = -LAM(j, 2);
         const double bpois = b0 * sP;
@@ -821,6 +936,8 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         seg_add(sm, SEG_DOP, -Q_E * bpois);
         bp(j, PT_FIX) = -Q_E * bpois;
         bVt += bpois * Q_E * (anm + cam) * Vm * iVt * iVt;
+        // the frozen ions through V_T (the pre-bias solution held; through it: the PRE pass)
+        if (frozen && !pre) bVt += -Q_E * bpois * (-(fan + fca) * PS(j, 0) * iVt * iVt);
         const double bjump = bfac_m - bfac_l;
         bfu -= bfac_m;
         const double bfl0 = bfac_l - bfac_m;
@@ -851,9 +968,10 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         const double aj = 0.5 * (dx + dxn);
         const double en = exp(-(Vm + TSEG(sm, SEG_OFFN)) * iVt), ep = exp((Vm + TSEG(sm, SEG_OFFP)) * iVt);
         const double wn = en * TSEG(sm, SEG_IMUN), wp = ep * TSEG(sm, SEG_IMUP);
-        const double U = R - G * gfr, iu = IU(j);
-        const double Tn = TS(j, 0), Tp = TS(j, 1);
-        const double Tn1 = j < NP ? TS(j + 1, 0) : 0.0, Tp1 = j < NP ? TS(j + 1, 1) : 0.0;
+        const double U = R - G * gfr, iu = pre ? 0.0 : IU(j);
+        const double Tn = pre ? 0.0 : TS(j, 0), Tp = pre ? 0.0 : TS(j, 1);
+        const double Tn1 = (!pre && j < NP) ? TS(j + 1, 0) : 0.0;
+        const double Tp1 = (!pre && j < NP) ? TS(j + 1, 1) : 0.0;
         const double bwn = (bSn + bDn * iu) * aj, bwp = (bSp + bDp * iu) * aj;
         seg_add(sm, SEG_IMUN, bwn * en); seg_add(sm, SEG_IMUP, bwp * ep);
         seg_add(sm, SEG_OFFN, -bwn * wn * iVt); seg_add(sm, SEG_OFFP, bwp * wp * iVt);

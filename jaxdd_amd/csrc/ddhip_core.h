@@ -85,6 +85,8 @@ struct Args {
     double *fix;      // [NP][n] net fixed ions in use (frozen after the pre-bias)
     LaneState *st;    // [n]
     double *states;   // [nV][NP][3][n] each voltage's final state (NULL: not kept)
+    double *pre;      // [NP][3][n] the pre-bias solution of a device with a pre-bias (NULL:
+                      // not kept)
     double *J;        // [n][nV]
     uint8_t *ok;
     int32_t *its;
@@ -1007,8 +1009,28 @@ DDH_FN void begin_scan_voltage(const Args &a, int lane, LaneState &S) {
     begin_solve(a, S, Vi, 1.0, a.max_iter, S.b_last, S.b_before, ratio, lg, work);
 }
 
+// Copies state buffer `buf` of the device to dst [NP][3][n]: each of the device's lanes its
+// share (the points its own sweeps wrote), eight loads in flight at a time.
+DDH_FN void keep_state(const Args &a, int lane, const LaneState &S, int buf, double *dst0) {
+    const size_t n = (size_t)a.n, m = (size_t)a.NP * 3;
+    const double *src = a.Y + (size_t)buf * m * n + lane;
+    double *dst = dst0 + lane;
+    const size_t hi = m * (size_t)(S.part + 1) / (size_t)S.parts;
+    size_t q = m * (size_t)S.part / (size_t)S.parts;
+    for (; q + 8 <= hi; q += 8) {
+        double t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = src[(q + k) * n];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dst[(q + k) * n] = t[k];
+    }
+    for (; q < hi; ++q) dst[q * n] = src[q * n];
+}
+
 DDH_FN void scan_setup(const Args &a, int lane, LaneState &S, int y_buf,
                                   const double ly[2]) {
+    // the pre-bias solution (the adjoint's second linearisation point where it freezes ions)
+    if (a.pre && !a.jv && SC(SC_PRE) > 0) keep_state(a, lane, S, y_buf, a.pre);
     S.b_start = y_buf;
     S.lstart[0] = ly[0]; S.lstart[1] = ly[1];
     bool frozen = SC(SC_PRE) > 0 && SC(SC_FIX) > 0;
@@ -1143,22 +1165,8 @@ DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
         a.J[o] = J;
         a.ok[o] = (uint8_t)((ok && S.pre_ok) ? 1 : 0);
         a.its[o] = S.it;
-        if (a.states) {  // the state the voltage ended with (the adjoint's linearisation point)
-            // each of the device's lanes copies its share, eight loads in flight at a time
-            const size_t n = (size_t)a.n, m = (size_t)a.NP * 3;
-            const double *src = a.Y + (size_t)S.b_work * m * n + lane;
-            double *dst = a.states + (size_t)idx * m * n + lane;
-            const size_t hi = m * (size_t)(S.part + 1) / (size_t)S.parts;
-            size_t q = m * (size_t)S.part / (size_t)S.parts;
-            for (; q + 8 <= hi; q += 8) {
-                double t[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) t[k] = src[(q + k) * n];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) dst[(q + k) * n] = t[k];
-            }
-            for (; q < hi; ++q) dst[q * n] = src[q * n];
-        }
+        if (a.states)  // the state the voltage ended with (the adjoint's linearisation point)
+            keep_state(a, lane, S, S.b_work, a.states + (size_t)idx * a.NP * 3 * a.n);
         if (ok && fin) {  // only a converged, finite solution is carried on
             S.b_before = S.b_last; S.b_last = S.b_work;
             S.lbefore[0] = S.llast[0]; S.lbefore[1] = S.llast[1];

@@ -314,6 +314,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
         const int d = xcd_device(map, k, a.n);
         if (d < 0) break;
         const bool refused = adj_refused(a, d);
+        const bool frozen = adj_frozen(a, d);  // a second pass at the pre-bias solution
         if (lane == 0) a.status[d] = refused ? 1 : 0;
         __syncthreads();  // every lane is done with the previous device's table
         adj_stage_values(a, d, tab, lane, 64);
@@ -329,12 +330,15 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
                 double g[ADJ_MAX_K];
 #pragma unroll
                 for (int k = 0; k < ADJ_MAX_K; ++k) g[k] = 0.0;
-                if (act) adj_contract(a, d, v, sl, tab, g);
-                for (int c0 = 0; c0 <= a.NP; c0 += ADJ_CHUNK) {
-                    __syncthreads();  // every lane is done with the previous chunk
-                    adj_stage_points(a, d, c0, tab, lane, 64);
-                    __syncthreads();
-                    if (act) adj_contract_points(a, c0, sl, tab, g);
+                for (int pass = 0; pass < (frozen ? 2 : 1); ++pass) {
+                    if (act && pass) adj_lambda(a, d, v, sl, tab, true);
+                    if (act) adj_contract(a, d, v, sl, tab, g, pass == 1);
+                    for (int c0 = 0; c0 <= a.NP; c0 += ADJ_CHUNK) {
+                        __syncthreads();  // every lane is done with the previous chunk
+                        adj_stage_points(a, d, c0, tab, lane, 64);
+                        __syncthreads();
+                        if (act) adj_contract_points(a, c0, sl, tab, g);
+                    }
                 }
                 if (act) {
 #pragma unroll
@@ -343,11 +347,14 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs a) {
                 }
                 continue;
             }
-            for (int k = 0; k < K; ++k) {
-                __syncthreads();  // every lane is done with the previous direction's tangents
-                adj_stage_direction(a, d, k, tab, lane, 64);
-                __syncthreads();
-                if (act) out[k] = adj_direction(a, d, v, sl, tab);
+            for (int pass = 0; pass < (frozen ? 2 : 1); ++pass) {
+                if (act && pass) adj_lambda(a, d, v, sl, tab, true);
+                for (int k = 0; k < K; ++k) {
+                    __syncthreads();  // every lane is done with the previous direction's tangents
+                    adj_stage_direction(a, d, k, tab, lane, 64);
+                    __syncthreads();
+                    if (act) out[k] = (pass ? out[k] : 0.0) + adj_direction(a, d, v, sl, tab, pass == 1);
+                }
             }
         }
     }
@@ -523,6 +530,7 @@ extern "C" size_t ddh_workspace_bytes(int32_t n, int32_t NP, int32_t nV, int32_t
 
 static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const ddh_opts *o,
                        double *J, uint8_t *ok, int32_t *its, int32_t *status, double *states,
+                       double *pre,
                        void *workspace, size_t workspace_bytes, void *stream) {
     if (!b || !V || !o || !J || !ok || !its || !status || !workspace)
         return fail(DDH_E_ARG, "ddh_solve_batch: NULL argument");
@@ -552,6 +560,7 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
     a.fix = (double *)(base + w.fix);
     a.st = (ddh::LaneState *)(base + w.st);
     a.states = states;
+    a.pre = pre;
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = o->tol; a.max_iter = o->max_iter; a.start_iter = o->start_iter;
     a.strict = o->strict; a.bern = o->bernoulli; a.jv = o->jv_mode;
@@ -584,15 +593,17 @@ extern "C" int ddh_solve_batch(const ddh_batch *b, const double *V, int32_t nV,
                                const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
                                int32_t *status, void *workspace, size_t workspace_bytes,
                                void *stream) {
-    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, workspace, workspace_bytes, stream);
+    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, nullptr, workspace,
+                       workspace_bytes, stream);
 }
 
 extern "C" int ddh_solve_batch_states(const ddh_batch *b, const double *V, int32_t nV,
                                       const ddh_opts *o, double *J, uint8_t *ok, int32_t *its,
-                                      int32_t *status, double *states, void *workspace,
-                                      size_t workspace_bytes, void *stream) {
+                                      int32_t *status, double *states, double *pre_states,
+                                      void *workspace, size_t workspace_bytes, void *stream) {
     if (!states) return fail(DDH_E_ARG, "ddh_solve_batch_states: NULL states");
-    return solve_batch(b, V, nV, o, J, ok, its, status, states, workspace, workspace_bytes, stream);
+    return solve_batch(b, V, nV, o, J, ok, its, status, states, pre_states, workspace,
+                       workspace_bytes, stream);
 }
 
 // ddh_solve_batch without an allocation: the workspace is this library's own device array of
@@ -649,12 +660,12 @@ extern "C" int ddh_solve_batch_noalloc(const ddh_batch *b, const double *V, int3
     void *ws = nullptr;
     const int rc = scratch_address(&ws);
     if (rc != DDH_OK) return rc;
-    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, ws,
+    return solve_batch(b, V, nV, o, J, ok, its, status, nullptr, nullptr, ws,
                        (size_t)DDH_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 struct AdjLayout {
-    size_t fac, st, lam, iu, ts, ion, total, slots, ls;
+    size_t fac, st, lam, iu, ts, ion, nu, total, slots, ls;
 };
 
 // Per launched lane (ADJ_SLOTS workgroups of 64 at most): the factors, the state's V, n, p,
@@ -671,6 +682,7 @@ static AdjLayout adj_layout(int32_t n, int32_t NP, int32_t nV) {
     w.iu = off; off = align256(off + (size_t)(NP + 1) * w.ls * sizeof(double));
     w.ts = off; off = align256(off + (size_t)(NP + 2) * 2 * w.ls * sizeof(double));
     w.ion = off; off = align256(off + (size_t)4 * w.ls * sizeof(double));
+    w.nu = off; off = align256(off + (size_t)NP * w.ls * sizeof(double));
     w.total = off;
     return w;
 }
@@ -681,7 +693,8 @@ extern "C" size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV)
 }
 
 extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, const double *states,
-                           const uint8_t *ok, int32_t ntheta, const double *dpt,
+                           const double *pre_states, const uint8_t *ok, int32_t ntheta,
+                           const double *dpt,
                            const double *dseg, const double *dsc, int32_t bernoulli,
                            double *grad, int32_t *status, void *workspace,
                            size_t workspace_bytes, void *stream) {
@@ -699,7 +712,8 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     ddh::AdjArgs a;
     a.n = b->n; a.NP = b->NP; a.nV = nV; a.nseg = b->nseg; a.ntheta = ntheta; a.bern = bernoulli;
     a.pt = b->pt; a.sid = b->sid; a.seg = b->seg; a.sc = b->sc;
-    a.V = V; a.states = states; a.ok = ok; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
+    a.V = V; a.states = states; a.pre = pre_states; a.ok = ok;
+    a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
     a.grad = grad; a.status = status;
     a.fac = (double *)(base + w.fac);
     a.st = (double *)(base + w.st);
@@ -707,6 +721,7 @@ extern "C" int ddh_adjoint(const ddh_batch *b, const double *V, int32_t nV, cons
     a.iu = (double *)(base + w.iu);
     a.ts = (double *)(base + w.ts);
     a.ion = (double *)(base + w.ion);
+    a.nu = (double *)(base + w.nu);
     a.ls = w.ls;
     a.passes = 0;
     hipStream_t s = (hipStream_t)stream;
@@ -747,7 +762,8 @@ extern "C" size_t ddh_adjoint_vjp_workspace_bytes(int32_t n, int32_t NP, int32_t
 }
 
 extern "C" int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV,
-                               const double *states, const uint8_t *ok, int32_t ntheta,
+                               const double *states, const double *pre_states,
+                               const uint8_t *ok, int32_t ntheta,
                                const double *dpt, const double *dseg, const double *dsc,
                                int32_t bernoulli, const double *dJbar, double *theta_bar,
                                int32_t *status, void *workspace, size_t workspace_bytes,
@@ -761,8 +777,8 @@ extern "C" int ddh_adjoint_vjp(const ddh_batch *b, const double *V, int32_t nV,
         return fail(DDH_E_WORKSPACE, "ddh_adjoint_vjp: workspace smaller than ddh_adjoint_vjp_workspace_bytes");
     const size_t abytes = align256(ddh_adjoint_workspace_bytes(b->n, b->NP, nV));
     double *grad = (double *)((char *)workspace + abytes);
-    const int rc = ddh_adjoint(b, V, nV, states, ok, ntheta, dpt, dseg, dsc, bernoulli, grad,
-                               status, workspace, abytes, stream);
+    const int rc = ddh_adjoint(b, V, nV, states, pre_states, ok, ntheta, dpt, dseg, dsc,
+                               bernoulli, grad, status, workspace, abytes, stream);
     if (rc != DDH_OK) return rc;
     const long threads = (long)b->n * ntheta;
     hipLaunchKernelGGL(ddh::vjp_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,

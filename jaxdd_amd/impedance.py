@@ -135,8 +135,8 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
         events[0].record()
     _lib.check(lib.ddh_solve_batch_states(ctypes.byref(b), Vd.data_ptr(), 1, ctypes.byref(opts),
                                           J.data_ptr(), ok.data_ptr(), its.data_ptr(),
-                                          status.data_ptr(), states.data_ptr(), ws.data_ptr(),
-                                          nbytes, stream), "ddh_solve_batch_states")
+                                          status.data_ptr(), states.data_ptr(), None,
+                                          ws.data_ptr(), nbytes, stream), "ddh_solve_batch_states")
     if events is not None:
         events[1].record()
         events[2].record()

@@ -147,13 +147,14 @@ def number_of(p, w):
     return float(p[w]) if isinstance(w, str) else float(np.asarray(p[w[0]])[w[1]])
 
 
-def named_fd_errors(oracle, structs, params, wrt, V, J, grad):
+def named_fd_errors(oracle, structs, params, wrt, V, J, grad, per_voltage=False):
     """Per device and number, max over V of |dJ/dx - central differences| |x| / the curve's
-    current scale (relative step 1e-5, the oracle's curves at tol 1e-12)."""
+    current scale (relative step 1e-5, the oracle's curves at tol 1e-12); ``per_voltage``: the
+    errors (devices, numbers, voltages) instead of their maxima."""
     from jaxdd_amd.batch import j_scale
     from jaxdd_amd.device import device_arrays
 
-    err = np.zeros((len(structs), len(wrt)))
+    err = np.zeros((len(structs), len(wrt)) + ((len(V),) if per_voltage else ()))
     for d, (st, p) in enumerate(zip(structs, params)):
         cscale = j_scale(device_arrays(st, p)) + np.max(np.abs(J[d]))
         for k, w in enumerate(wrt):
@@ -163,5 +164,6 @@ def named_fd_errors(oracle, structs, params, wrt, V, J, grad):
                                       strict=True, first_max_iter=None, max_iter=200)[0][0]
                    for s in (1.0, -1.0)]
             fd = (Jpm[0] - Jpm[1]) / (2 * h)
-            err[d, k] = np.max(np.abs(grad[d, :, k] - fd)) * abs(x) / cscale
+            e = np.abs(grad[d, :, k] - fd) * abs(x) / cscale
+            err[d, k] = e if per_voltage else np.max(e)
     return err

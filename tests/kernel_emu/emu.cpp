@@ -190,7 +190,8 @@ static int g_poison = 0;  // fill the workspace with this byte before a solve (0
 extern "C" void emu_set_poison(int byte) { g_poison = byte; }
 
 static double *g_states = nullptr;  // where the next solve keeps each voltage's final state
-extern "C" void emu_keep_states(double *states) { g_states = states; }
+static double *g_pre = nullptr;     // and the pre-bias solutions
+extern "C" void emu_keep_states(double *states, double *pre) { g_states = states; g_pre = pre; }
 
 extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                                const double *seg, const double *sc, const double *V, int nV,
@@ -215,7 +216,7 @@ extern "C" int emu_solve_batch(int n, int NP, int nseg, const double *pt, const 
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V;
     a.Y = Y.data(); a.fac64 = fac64.data(); a.fac32 = fac32.data(); a.pfac = pfac.data();
-    a.fix = fix.data(); a.st = st.data(); a.states = g_states;
+    a.fix = fix.data(); a.st = st.data(); a.states = g_states; a.pre = g_pre;
     a.J = J; a.ok = ok; a.its = its; a.status = status;
     a.tol = tol; a.max_iter = max_iter; a.start_iter = start_iter; a.strict = strict;
     a.bern = bern; a.jv = jv; a.mixed = precision ? 1 : 0;
@@ -231,25 +232,28 @@ extern "C" void emu_set_adjoint_passes(int passes) { g_adj_passes = passes; }
 
 extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint8_t *sid,
                            const double *seg, const double *sc, const double *V, int nV,
-                           const double *states, const uint8_t *ok, int ntheta,
+                           const double *states, const double *pre, const uint8_t *ok, int ntheta,
                            const double *dpt, const double *dseg, const double *dsc, int bern,
                            double *grad, int32_t *status) {
     const size_t ls = 64;  // one workgroup: the devices in turn, a lane per voltage
     std::vector<double> fac((size_t)NP * ADJ_F * ls), st((size_t)NP * 3 * ls),
         lam((size_t)NP * 3 * ls), iu((size_t)(NP + 1) * ls), ts((size_t)(NP + 2) * 2 * ls),
-        ion((size_t)4 * ls);
+        ion((size_t)4 * ls), nu((size_t)NP * ls);
     AdjArgs a;
     a.n = n; a.NP = NP; a.nV = nV; a.nseg = nseg; a.ntheta = ntheta; a.bern = bern;
     a.pt = pt; a.sid = sid; a.seg = seg; a.sc = sc; a.V = V; a.states = states; a.ok = ok;
+    a.pre = pre;
     a.dpt = dpt; a.dseg = dseg; a.dsc = dsc; a.grad = grad; a.status = status;
     a.fac = fac.data(); a.st = st.data(); a.lam = lam.data(); a.iu = iu.data(); a.ls = ls;
     a.ts = ts.data();
     a.ion = ion.data();
+    a.nu = nu.data();
     a.passes = g_adj_passes;
     const int Kc = adj_k(ntheta, a.passes);
     std::vector<double> tab(adj_table_size(nseg, NP, Kc));
     for (int d = 0; d < n; ++d) {
         const bool refused = adj_refused(a, d);
+        const bool frozen = adj_frozen(a, d);
         if (refused) status[d] = 1;
         adj_stage_values(a, d, tab.data(), 0, 1);
         for (int v0 = 0; v0 < nV; v0 += 64) {
@@ -265,13 +269,18 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
             }
             if (Kc > 0) {  // every direction in one reverse pass
                 std::vector<double> g((size_t)64 * ADJ_MAX_K, 0.0);
-                for (int v = v0; v < v1; ++v)
-                    if (act[v - v0]) adj_contract(a, d, v, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K]);
-                for (int c0 = 0; c0 <= NP; c0 += ADJ_CHUNK) {
-                    adj_stage_points(a, d, c0, tab.data(), 0, 1);
-                    for (int v = v0; v < v1; ++v)
-                        if (act[v - v0])
-                            adj_contract_points(a, c0, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K]);
+                for (int pass = 0; pass < (frozen ? 2 : 1); ++pass) {
+                    for (int v = v0; v < v1; ++v) {
+                        if (!act[v - v0]) continue;
+                        if (pass) adj_lambda(a, d, v, (size_t)(v - v0), tab.data(), true);
+                        adj_contract(a, d, v, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K], pass == 1);
+                    }
+                    for (int c0 = 0; c0 <= NP; c0 += ADJ_CHUNK) {
+                        adj_stage_points(a, d, c0, tab.data(), 0, 1);
+                        for (int v = v0; v < v1; ++v)
+                            if (act[v - v0])
+                                adj_contract_points(a, c0, (size_t)(v - v0), tab.data(), &g[(size_t)(v - v0) * ADJ_MAX_K]);
+                    }
                 }
                 for (int v = v0; v < v1; ++v)
                     if (act[v - v0])
@@ -279,12 +288,18 @@ extern "C" int emu_adjoint(int n, int NP, int nseg, const double *pt, const uint
                             grad[((size_t)d * nV + v) * ntheta + k] = g[(size_t)(v - v0) * ADJ_MAX_K + k];
                 continue;
             }
-            for (int k = 0; k < ntheta; ++k) {
-                adj_stage_direction(a, d, k, tab.data(), 0, 1);
-                for (int v = v0; v < v1; ++v)
-                    if (act[v - v0])
-                        grad[((size_t)d * nV + v) * ntheta + k] =
-                            adj_direction(a, d, v, (size_t)(v - v0), tab.data());
+            for (int pass = 0; pass < (frozen ? 2 : 1); ++pass) {
+                if (pass)
+                    for (int v = v0; v < v1; ++v)
+                        if (act[v - v0]) adj_lambda(a, d, v, (size_t)(v - v0), tab.data(), true);
+                for (int k = 0; k < ntheta; ++k) {
+                    adj_stage_direction(a, d, k, tab.data(), 0, 1);
+                    for (int v = v0; v < v1; ++v)
+                        if (act[v - v0]) {
+                            double &o = grad[((size_t)d * nV + v) * ntheta + k];
+                            o = (pass ? o : 0.0) + adj_direction(a, d, v, (size_t)(v - v0), tab.data(), pass == 1);
+                        }
+                }
             }
         }
     }

@@ -30,11 +30,11 @@ def lib():
         vp, i = ctypes.c_void_p, ctypes.c_int
         _lib.emu_solve_batch.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, ctypes.c_double, i, i,
                                          i, i, i, i, vp, vp, vp, vp]
-        _lib.emu_keep_states.argtypes = [vp]
+        _lib.emu_keep_states.argtypes = [vp, vp]
         _lib.emu_set_lanes.argtypes = [i]
         _lib.emu_set_adjoint_passes.argtypes = [i]
-        _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, i, vp, vp, vp, i,
-                                     vp, vp]
+        _lib.emu_adjoint.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, vp, vp, vp, i, vp, vp, vp,
+                                     i, vp, vp]
         _lib.emu_assemble.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp,
                                       vp, vp]
         _lib.emu_impedance.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, i,
@@ -54,10 +54,10 @@ def _p(a):
 
 
 def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=False,
-                 bernoulli="exact", precision="fp64", jv=False, states=None, lanes=1):
+                 bernoulli="exact", precision="fp64", jv=False, states=None, lanes=1, pre=None):
     """(J without the shunt, ok, its, status) for packed devices at voltages V (ascending in
     batch mode, the given order in jv mode). ``states``: an array (nV, NP, 3, n) that receives
-    each voltage's final state."""
+    each voltage's final state; ``pre``: one (NP, 3, n) that receives the pre-bias solutions."""
     n = packed["sc"].shape[1]
     NP = packed["pt"].shape[0] - 2
     V = np.ascontiguousarray(V, dtype=np.float64)
@@ -70,7 +70,9 @@ def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=Fals
     if states is not None:
         assert states.shape == (nV, NP, 3, n) and states.dtype == np.float64
         assert states.flags.c_contiguous
-    lib().emu_keep_states(None if states is None else _p(states))
+    if pre is not None:
+        assert pre.shape == (NP, 3, n) and pre.dtype == np.float64 and pre.flags.c_contiguous
+    lib().emu_keep_states(None if states is None else _p(states), None if pre is None else _p(pre))
     lib().emu_set_lanes(int(lanes))
     lib().emu_solve_batch(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV, tol,
                           max_iter, start_iter or max_iter, int(strict),
@@ -79,9 +81,10 @@ def solve_packed(packed, V, tol=1e-10, max_iter=60, start_iter=None, strict=Fals
     return J, ok.astype(bool), its, status
 
 
-def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact", passes=False):
+def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact", passes=False, pre=None):
     """dJ/dtheta (n, nV, ntheta) and status (n,) of the adjoint kernel's lanes, replayed;
-    ``passes``: one pass per direction instead of the reverse pass over the parameters."""
+    ``passes``: one pass per direction instead of the reverse pass over the parameters;
+    ``pre``: the pre-bias solutions (NP, 3, n) of devices whose ions are frozen at them."""
     n = packed["sc"].shape[1]
     NP = packed["pt"].shape[0] - 2
     V = np.ascontiguousarray(V, dtype=np.float64)
@@ -94,7 +97,9 @@ def adjoint_packed(packed, V, states, ok, tangents, bernoulli="exact", passes=Fa
     okb = np.ascontiguousarray(ok, dtype=np.uint8)
     lib().emu_set_adjoint_passes(int(bool(passes)))
     lib().emu_adjoint(n, NP, packed["nseg"], *(_p(a) for a in arrs), _p(V), nV,
-                      _p(np.ascontiguousarray(states)), _p(okb), K, _p(dpt), _p(dseg), _p(dsc),
+                      _p(np.ascontiguousarray(states)),
+                      None if pre is None else _p(np.ascontiguousarray(pre)), _p(okb), K, _p(dpt),
+                      _p(dseg), _p(dsc),
                       0 if bernoulli == "exact" else 1, _p(grad), _p(status))
     return grad, status
 

@@ -146,9 +146,9 @@ def test_unconverged_voltages_have_zero_gradient(emu):
     assert np.all(grad[~ok] == 0.0) and np.any(grad[ok] != 0.0)
 
 
-def test_devices_with_frozen_ions_are_refused(emu):
-    """Ions frozen at a pre-bias: the scan's fixed charge is the pre-bias's solution, whose
-    dependence on the parameters this adjoint does not take (status 1, NaN)."""
+def test_devices_with_frozen_ions_need_the_pre_bias_solution(emu):
+    """Ions frozen at a pre-bias: the scan's fixed charge is the pre-bias solution's ions, so
+    the adjoint needs that solution; without it such a device is refused (status 1, NaN)."""
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import pack
 
@@ -160,6 +160,60 @@ def test_devices_with_frozen_ions_are_refused(emu):
            "sc": np.zeros((1,) + packed["sc"].shape)}
     grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
     assert status[0] == 1 and np.isnan(grad).all()
+
+
+#: The frozen stack's numbers: the free-ion ones and the pre-bias voltage the ions freeze at.
+FROZEN_WRT = ION_WRT + ["V_pre"]
+
+
+@pytest.fixture(scope="module")
+def frozen_case(emu):
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(2)
+    structs, params = zip(*[S.row_inputs(t, k, 100, ions=True, frozen=True) for k in range(2)])
+    dirs = A.directions_for(FROZEN_WRT, 2, len(structs[0].i0))
+    packed, tan = emu.build_packed(list(structs), list(params), dirs)
+    V = S.VOLTAGES[::10].copy()
+    states = np.zeros((len(V), 100, 3, 2))
+    pre = np.zeros((100, 3, 2))
+    J, ok, its, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                     states=states, pre=pre)
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan, pre=pre)
+    return structs, params, packed, tan, V, J, ok, states, pre, grad, status
+
+
+def test_frozen_ions_gradients_are_finite_differences(frozen_case, oracle):
+    """Ions frozen at a pre-bias of 1.2 V (reference: src/jaxdd/batch.py:937-960, differentiated
+    through the pre-bias solution as `_freeze` is): the scan's adjoint and a second transposed
+    bordered solve at the pre-bias solution, against central differences of the oracle's curves
+    in the named numbers and the pre-bias voltage. 1e-5 of the curve's scale below 1.3 V; at
+    1.3 V, where a 1e-15 relative change of the scan's state moves a thickness's or the work
+    function's component by 5e-5, 1e-4."""
+    structs, params, _, _, V, J, ok, _, pre, grad, status = frozen_case
+    assert not status.any() and ok.all() and np.abs(pre).max() > 0
+    err = named_fd_errors(oracle, structs, params, FROZEN_WRT, V, J, grad, per_voltage=True)
+    low = V < 1.25
+    assert np.max(err[:, :, low]) < 1e-5, np.max(err[:, :, low], axis=2)
+    assert np.max(err) < 1e-4, np.max(err, axis=2)
+    assert np.median(err) < 3e-8, np.median(err)
+
+
+def test_frozen_ions_reverse_pass_is_the_directions_passes(frozen_case, emu):
+    """The two passes of a frozen device (the scan's, then the pre-bias solution's) by the
+    reverse pass and by one dual pass per direction."""
+    from jaxdd_amd.batch import j_scale
+    from jaxdd_amd.device import device_arrays
+
+    structs, params, packed, tan, V, J, ok, states, pre, grad, _ = frozen_case
+    by_pass, st2 = emu.adjoint_packed(packed, V, states, ok, tan, passes=True, pre=pre)
+    assert not st2.any()
+    unit = np.array([[abs(number_of(p, w)) for w in FROZEN_WRT] for p in params])
+    scale = np.array([j_scale(device_arrays(st, p)) for st, p in zip(structs, params)])
+    scale = scale + np.max(np.abs(J), axis=1)
+    err = np.abs(grad - by_pass) * unit[:, None, :] / scale[:, None, None]
+    assert np.max(err) < 5e-11, np.unravel_index(np.argmax(err), err.shape)
 
 
 @pytest.fixture(scope="module")

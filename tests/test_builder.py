@@ -287,8 +287,8 @@ def test_gpu_gradients_with_mobile_ions(torch_cuda, oracle, emu):
     print(f"gradients with mobile ions: {n} devices, {len(ION_WRT)} numbers", flush=True)
     dirs = A.directions_for(ION_WRT, n, len(structs[0].i0))
     batch, tan = build(list(structs), list(params), dirs)
-    J, ok, _, g, states = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200, strict=True,
-                                         return_states=True)
+    J, ok, _, g, states, _ = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200, strict=True,
+                                            return_states=True)
     J, ok, g = _np(J), _np(ok), _np(g)
     assert ok.all() and np.isfinite(g).all()
     packed = {"pt": _np(batch["_pt"]), "sid": _np(batch["_sid"]), "seg": _np(batch["_seg"]),
@@ -338,4 +338,47 @@ def test_gpu_gradient_in_the_voltage(torch_cuda):
     err = np.abs(_np(g)[:, :, 0] - fd) / np.max(np.abs(fd), axis=1, keepdims=True)
     print(f"dJ/dV against differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
     assert err.max() < 1e-5 and np.median(err) < 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_with_frozen_ions(torch_cuda, oracle, emu):
+    """Ions frozen at a 1.2 V pre-bias on the GPU: the solve keeps the pre-bias solutions
+    (ddh_solve_batch_states' pre_states) and the adjoint adds its second pass at them. Against
+    the CPU replay at the GPU's own states (a random weighting of the components, 1e-6) and
+    against central differences of the oracle's curves (the CPU replay's bars: 1e-5 of the
+    curve's scale below 1.3 V, 1e-4 at 1.3 V)."""
+    from conftest import ION_WRT, named_fd_errors, number_of
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+    from jaxdd_amd.builder import build
+
+    wrt = ION_WRT + ["V_pre"]
+    n, NP = 8, 100
+    t = S.draw(n)
+    structs, params = zip(*[S.row_inputs(t, k, NP, ions=True, frozen=True) for k in range(n)])
+    V = S.VOLTAGES[::10].copy()
+    print(f"gradients with frozen ions: {n} devices, {len(wrt)} numbers", flush=True)
+    batch, tan = build(list(structs), list(params), A.directions_for(wrt, n, len(structs[0].i0)))
+    J, ok, _, g, states, pre = A.jv_gradients(batch, tan, V, tol=1e-12, max_iter=200,
+                                              strict=True, return_states=True)
+    J, ok, g = _np(J), _np(ok), _np(g)
+    assert ok.all() and np.isfinite(g).all() and pre is not None
+    packed = {"pt": _np(batch["_pt"]), "sid": _np(batch["_sid"]), "seg": _np(batch["_seg"]),
+              "sc": _np(batch["_sc"]), "nseg": batch["nseg"]}
+    rtan = {k: _np(tan[k]) for k in ("pt", "seg", "sc")}
+    rg, rst = emu.adjoint_packed(packed, V, np.ascontiguousarray(_np(states)), ok, rtan,
+                                 pre=np.ascontiguousarray(_np(pre)))
+    assert not rst.any()
+    unit = np.array([[abs(number_of(p, w)) for w in wrt] for p in params])
+    rng = np.random.default_rng(1)
+    w, d = rng.normal(size=ok.shape), rng.normal(size=unit.shape)
+    a = np.sum(w[:, :, None] * g * (unit * d)[:, None, :])
+    b = np.sum(w[:, :, None] * rg * (unit * d)[:, None, :])
+    print(f"GPU vs replay: weighted sum {abs(a - b) / abs(b):.2e}", flush=True)
+    assert abs(a - b) < 1e-6 * abs(b)
+    err = named_fd_errors(oracle, structs[:2], params[:2], wrt, V, J, g, per_voltage=True)
+    low = V < 1.25
+    print(f"GPU vs differences: below 1.3 V max {err[:, :, low].max():.2e}, at 1.3 V max "
+          f"{err[:, :, ~low].max():.2e}, median {np.median(err):.2e}", flush=True)
+    assert np.max(err[:, :, low]) < 1e-5 and np.max(err) < 1e-4 and np.median(err) < 1e-7
 

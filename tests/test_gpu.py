@@ -390,8 +390,8 @@ def c4_gpu(torch_cuda):
     th = A.theta_from_table(S.draw(4), range(4))
     packed, tan = A.tangents(th)
     V = S.VOLTAGES[::6].copy()
-    J, ok, its, grad, states = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True,
-                                              max_iter=200, return_states=True)
+    J, ok, its, grad, states, _ = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True,
+                                                 max_iter=200, return_states=True)
     return th, packed, tan, V, _np(J), _np(ok), _np(grad), _np(states)
 
 
@@ -505,8 +505,8 @@ def test_gradients_of_the_mixed_precision_solve(c4_gpu):
     from jaxdd_amd import adjoint as A
 
     th, packed, tan, V, J, ok, grad, states = c4_gpu
-    Jm, okm, _, gm, sm = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True, max_iter=200,
-                                        precision="mixed", return_states=True)
+    Jm, okm, _, gm, sm, _ = A.jv_gradients(packed, tan, V, tol=1e-12, strict=True, max_iter=200,
+                                           precision="mixed", return_states=True)
     assert _np(okm).all()
     assert np.max(np.abs(_np(sm) - states)) < 1e-9
     assert np.max(np.abs(_np(Jm) - J)) <= 1e-9 * np.max(np.abs(J))

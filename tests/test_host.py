@@ -172,7 +172,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     lib.ddh_abi_version.restype = ctypes.c_int
-    assert lib.ddh_abi_version() == 2
+    assert lib.ddh_abi_version() == 3
     lib.ddh_workspace_bytes.restype = ctypes.c_size_t
     assert lib.ddh_workspace_bytes(16384, 400, 61, 1) > 0
     assert lib.ddh_workspace_bytes(0, 400, 61, 1) == 0
@@ -214,14 +214,14 @@ def test_library_rejects_bad_shapes_before_any_launch():
     assert rc == -1 and b"NP >= 16" in lib.ddh_last_error()
     # the adjoint stages a device's arrays in LDS: a grid too fine for it is refused
     b = _lib.Batch(64, 5000, 7, 0, 1, 1, 1, 1)
-    rc = lib.ddh_adjoint(ctypes.byref(b), 1, 61, 1, 1, 16, 1, 1, 1, 0, 1, 1, 1, 1 << 45, None)
+    rc = lib.ddh_adjoint(ctypes.byref(b), 1, 61, 1, None, 1, 16, 1, 1, 1, 0, 1, 1, 1, 1 << 45,
+                         None)
     assert rc == -1 and b"LDS" in lib.ddh_last_error()
 
 
 def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
     """solve_batch(lanes=4) and jv_gradients check the batch before any launch: mobile ions
-    with four lanes and NP < 16 (ValueError), ions frozen at a pre-bias for gradients
-    (NotImplementedError)."""
+    with four lanes and NP < 16 (ValueError)."""
     from jaxdd_amd import adjoint as A
     from jaxdd_amd import synthetic as S
     from jaxdd_amd.batch import from_packed, pack, solve_batch
@@ -238,8 +238,5 @@ def test_four_lanes_are_refused_for_mobile_ions_on_the_host():
     tan = {"pt": np.zeros((1,) + tuple(ions["_pt"].shape)), "seg": None, "sc": None}
     with pytest.raises(ValueError, match="mobile ions"):
         A.jv_gradients(ions, tan, [0.0, 0.5], lanes=4)
-    frozen = from_packed(pack([S.device_from_row(t, k, 40, True, True) for k in range(3)]), "cpu")
-    with pytest.raises(NotImplementedError, match="frozen"):
-        A.jv_gradients(frozen, tan, [0.0, 0.5])
     with pytest.raises(ValueError, match="precision"):
         A.jv_gradients(ions, tan, [0.0, 0.5], precision="fp32")
