@@ -91,7 +91,7 @@ def device_from_theta(theta, NP: int = 400, structure=None):
 
 def directions_of(thetas) -> tuple[np.ndarray, np.ndarray]:
     """The 16 parameters as directions in the GPU builder's numbers (`builder.LAYER_FIELDS`,
-    `SCALAR_FIELDS`): dlayers (16, 3, 21, n), dscalars (16, 9, n). Each is its number's unit
+    `SCALAR_FIELDS`): dlayers (16, 3, 22, n), dscalars (16, 9, n). Each is its number's unit
     direction, but for the trap density and the lifetime, which also move the perovskite's
     capture coefficients C = 1/(N_t tau)."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
@@ -147,7 +147,7 @@ STACK_FLAGS = (0, 1, 0)
 
 
 def numbers_of(thetas) -> tuple[np.ndarray, np.ndarray]:
-    """The GPU builder's numbers (`builder.number_arrays`' layout: layers [3][21][n], scalars
+    """The GPU builder's numbers (`builder.number_arrays`' layout: layers [3][22][n], scalars
     [9][n]) of parameter vectors (n, 16), vectorised: `params_of` for every device at once."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
 
@@ -159,6 +159,7 @@ def numbers_of(thetas) -> tuple[np.ndarray, np.ndarray]:
         if f in base:
             layers[:, i, :] = np.asarray(base[f], dtype=float)[:, None]
     F = {f: i for i, f in enumerate(LAYER_FIELDS)}
+    layers[:, F["preLangevin"], :] = 1.0
     layers[:, F["L"], :] = th[:, 0:3].T
     layers[:, F["mu_n"], :] = th[:, 3:9:2].T
     layers[:, F["mu_p"], :] = th[:, 4:9:2].T

@@ -27,12 +27,13 @@ from .grid import GRADED_POINTS_PER_EFOLD, GRADED_SMALLEST
 #: The per-layer numbers of ddh_build, in its order.
 LAYER_FIELDS = ("L", "eps_r", "E_c", "E_v", "N_c", "N_D", "N_A", "mu_n", "mu_p", "nu_int_n",
                 "nu_int_p", "k_direct", "N_t_bulk", "C_n_bulk", "C_p_bulk", "E_t_bulk", "G_ehp",
-                "N_anion", "N_cation", "mu_anion", "mu_cation")
+                "N_anion", "N_cation", "mu_anion", "mu_cation", "preLangevin")
 #: The per-device numbers of ddh_build, in its order (G_shunt = 1/R_shunt, 0 without a shunt).
 SCALAR_FIELDS = ("T", "W_L", "W_R", "G_frac", "V_pre", "G_shunt", "pre_cond", "fix_ions",
                  "left_elec")
 MAX_LAYERS = 8
 _TRAPS, _A_MOBILE, _A_FIXED, _C_MOBILE, _C_FIXED = 1, 2, 4, 8, 16
+_LANGEVIN, _LANGEVIN_INTERVAL = 32, 64
 
 
 def check_buildable(st: Structure) -> None:
@@ -47,6 +48,16 @@ def check_buildable(st: Structure) -> None:
         raise NotImplementedError(f"optional modules outside the GPU builder: {sorted(st.extra)}")
     if len(st.i0) > MAX_LAYERS:
         raise NotImplementedError(f"at most {MAX_LAYERS} layers")
+
+
+def layer_flags(st: Structure, k: int) -> int:
+    """Layer ``k``'s bits for ddh_build: bulk traps, mobile and fixed anions and cations,
+    Langevin's rate (at the interval's mobilities with ``langevin_mobility`` "simss")."""
+    langevin = st.langevin is not None and bool(st.langevin[k])
+    return ((_TRAPS if st.traps[k] else 0) | (_A_MOBILE if st.anion_mobile[k] else 0)
+            | (_A_FIXED if st.anion_fixed[k] else 0) | (_C_MOBILE if st.cation_mobile[k] else 0)
+            | (_C_FIXED if st.cation_fixed[k] else 0) | (_LANGEVIN if langevin else 0)
+            | (_LANGEVIN_INTERVAL if langevin and st.langevin_mobility == "simss" else 0))
 
 
 def segment_ids(st: Structure) -> np.ndarray:
@@ -74,11 +85,8 @@ def structure_arrays(structures) -> dict:
     nseg = int(max(s[-1] for s in sids)) + 1
     if nseg > MAX_SEGMENTS:
         raise NotImplementedError(f"{nseg} segments: at most {MAX_SEGMENTS}")
-    flags = np.array([[(_TRAPS if st.traps[k] else 0) | (_A_MOBILE if st.anion_mobile[k] else 0)
-                       | (_A_FIXED if st.anion_fixed[k] else 0)
-                       | (_C_MOBILE if st.cation_mobile[k] else 0)
-                       | (_C_FIXED if st.cation_fixed[k] else 0) for k in range(nl)]
-                      for st in structures], dtype=np.uint8).T
+    flags = np.array([[layer_flags(st, k) for k in range(nl)] for st in structures],
+                     dtype=np.uint8).T
     iface = np.zeros((NP + 1, len(structures)), dtype=np.uint8)
     for k, st in enumerate(structures):
         iface[list(st.i1[:-1]), k] = 1
@@ -96,10 +104,15 @@ def structure_arrays(structures) -> dict:
 
 
 def number_arrays(structures, params) -> tuple[np.ndarray, np.ndarray]:
-    """(layers [nl][21][n], scalars [9][n]) of the devices' numbers (`device_parameters`'
-    dicts: per-layer arrays and scalars)."""
-    layers = np.stack([np.stack([np.asarray(p[f], dtype=float) for f in LAYER_FIELDS], -1)
-                       for p in params], -1)  # (nl, 21, n)
+    """(layers [nl][22][n], scalars [9][n]) of the devices' numbers (`device_parameters`'
+    dicts: per-layer arrays and scalars; a missing `preLangevin` is 1)."""
+    def number(p, f):
+        if f == "preLangevin" and f not in p:
+            return np.ones(len(np.atleast_1d(p["L"])))
+        return np.asarray(p[f], dtype=float)
+
+    layers = np.stack([np.stack([number(p, f) for f in LAYER_FIELDS], -1)
+                       for p in params], -1)  # (nl, 22, n)
     scal = []
     for st, p in zip(structures, params):
         R_shunt = float(p.get("R_shunt", -1.0))
@@ -111,7 +124,7 @@ def number_arrays(structures, params) -> tuple[np.ndarray, np.ndarray]:
 
 def build(structures, params, directions=None, device="cuda"):
     """The batch (`batch.prepare`'s dict, on ``device``) of devices given by their structures
-    and numbers, built by ddh_build. ``directions``: None, or (dlayers [K][nl][21][n], dscalars
+    and numbers, built by ddh_build. ``directions``: None, or (dlayers [K][nl][22][n], dscalars
     [K][9][n]) host arrays, directions in the numbers; then also the tangents {"pt", "seg",
     "sc"} (torch, on the device) along each, as `adjoint.jv_gradients` takes them."""
     dev = _cuda(device)
@@ -141,7 +154,8 @@ def build_graded(L, flags, layers, scalars, directions=None, NP: int = 400,
     """The batch of devices on graded grids built entirely on the GPU: each device's layout
     from its thicknesses (ddh_graded_layout; ``L`` (n, nl)), then its arrays (ddh_build).
     ``flags`` (nl,) or (nl, n): each layer's bits (1 bulk traps, 2 mobile anions, 4 fixed
-    anions, 8 mobile cations, 16 fixed cations); ``layers`` [nl][21][n] and ``scalars`` [9][n]
+    anions, 8 mobile cations, 16 fixed cations, 32 Langevin recombination, 64 Langevin at the
+    interval's mobilities); ``layers`` [nl][22][n] and ``scalars`` [9][n]
     the numbers (`number_arrays`' layout); ``directions`` as `build`'s. Each point stops its
     Newton steps when they no longer move it; the host restatement (`grid.graded_layout`) stops
     a layer's points together: positions agree to the last bits, the smallest spacings to

@@ -7,7 +7,7 @@
 // The path's subset of the physics: Boltzmann carriers, each layer's uniform generation
 // (`G_ehp`, the reference's generation mode "none": every interior point of the layer), ohmic
 // contacts at the given work functions, bulk traps of one level, mobile ions' masks, totals and
-// fixed densities. What is discrete (the grid's layout, each point's layer and segment, which
+// fixed densities, direct recombination by each layer's k_direct or by Langevin's rate. What is discrete (the grid's layout, each point's layer and segment, which
 // layers generate light, hold traps or ions) is the structure's and has no derivative; the
 // thicknesses move the points by scaling each layer's unscaled spacing, with the mean of the
 // three spacings around each interface (SIMsalabim's rule).
@@ -45,12 +45,17 @@ constexpr double BUILD_DENSITY_FLOOR = 1e-12;  // src/jaxdd/solver.py:54
 // Per-layer numbers [nl][BL_N][n] (the reference's `LAYER_PARAMETERS` of the path).
 enum BuildLayer {
     BL_L, BL_EPSR, BL_EC, BL_EV, BL_NC, BL_ND, BL_NA, BL_MUN, BL_MUP, BL_NUN, BL_NUP, BL_KD,
-    BL_NT, BL_CN, BL_CP, BL_ET, BL_G, BL_NANION, BL_NCATION, BL_MUA, BL_MUC, BL_N
+    BL_NT, BL_CN, BL_CP, BL_ET, BL_G, BL_NANION, BL_NCATION, BL_MUA, BL_MUC, BL_PRELANGEVIN, BL_N
 };
 // Per-device numbers [BS_N][n].
 enum BuildScalar { BS_T, BS_WL, BS_WR, BS_GFRAC, BS_VPRE, BS_GSH, BS_PRE, BS_FIX, BS_ELEC, BS_N };
-// Per-layer flags [nl][n] (bits): bulk traps, mobile and fixed anions and cations.
-enum BuildFlag { BF_TRAPS = 1, BF_AMOBILE = 2, BF_AFIXED = 4, BF_CMOBILE = 8, BF_CFIXED = 16 };
+// Per-layer flags [nl][n] (bits): bulk traps, mobile and fixed anions and cations, Langevin's
+// rate for direct recombination, and that rate at the mobilities of the interval after each
+// point (SIMsalabim's) rather than the layer's.
+enum BuildFlag {
+    BF_TRAPS = 1, BF_AMOBILE = 2, BF_AFIXED = 4, BF_CMOBILE = 8, BF_CFIXED = 16,
+    BF_LANGEVIN = 32, BF_LANGEVIN_INTERVAL = 64
+};
 
 // The packed arrays' fields (batch.py SEG_FIELDS, SC_FIELDS, the points' PT fields).
 enum BuildSeg {
@@ -102,6 +107,12 @@ DDH_BFN BD operator/(double c, BD b) {
 DDH_BFN BD dexp1(BD a) { const double e = exp(a.v); return bd(e, e * a.d); }
 DDH_BFN BD dlog1(BD a) { return bd(log(a.v), a.d / a.v); }
 DDH_BFN BD dmax1(BD a, BD b) { return b.v > a.v ? b : a; }
+// max(a, b) as `jnp.maximum` differentiates it: at a tie the mean of the two derivatives
+// (what central differences see at the kink).
+DDH_BFN BD dmax_mean(BD a, BD b) {
+    if (a.v == b.v) return bd(a.v, 0.5 * (a.d + b.d));
+    return b.v > a.v ? b : a;
+}
 
 // A device's view for one lane.
 struct BuildLane {
@@ -164,7 +175,15 @@ DDH_BFN void bsegment(const BuildArgs &a, const BuildLane &L, int j, int l, BD t
     f[BG_OFFP] = lv.offp;
     f[BG_NC] = Nc;
     f[BG_DOP] = blayer(a, L, l, BL_NA) - blayer(a, L, l, BL_ND);
-    f[BG_KD] = interior ? blayer(a, L, l, BL_KD) : zero;
+    const BD mun = crossing ? total * hj * blayer(a, L, l, BL_NUN) / Vt : blayer(a, L, l, BL_MUN);
+    const BD mup = crossing ? total * hj * blayer(a, L, l, BL_NUP) / Vt : blayer(a, L, l, BL_MUP);
+    BD kd = blayer(a, L, l, BL_KD);
+    if (fl & BF_LANGEVIN) {  // preLangevin q (mu_n + mu_p) / eps (device.py:900-911)
+        const BD rate = blayer(a, L, l, BL_PRELANGEVIN) * BUILD_Q / (blayer(a, L, l, BL_EPSR) * BUILD_EPS0);
+        kd = (fl & BF_LANGEVIN_INTERVAL) ? rate * (mun + mup)
+                                          : rate * (blayer(a, L, l, BL_MUN) + blayer(a, L, l, BL_MUP));
+    }
+    f[BG_KD] = interior ? kd : zero;
     f[BG_NI2] = lv.ni * lv.ni;
     const bool trap = interior && (fl & BF_TRAPS);
     f[BG_NT] = trap ? blayer(a, L, l, BL_NT) : zero;
@@ -174,14 +193,12 @@ DDH_BFN void bsegment(const BuildArgs &a, const BuildLane &L, int j, int l, BD t
     f[BG_PT0] = trap ? lv.pt0 : zero;
     f[BG_AM] = bd((fl & BF_AMOBILE) ? 1.0 : 0.0);
     f[BG_CM] = bd((fl & BF_CMOBILE) ? 1.0 : 0.0);
-    const BD mun = crossing ? total * hj * blayer(a, L, l, BL_NUN) / Vt : blayer(a, L, l, BL_MUN);
-    const BD mup = crossing ? total * hj * blayer(a, L, l, BL_NUP) / Vt : blayer(a, L, l, BL_MUP);
     f[BG_MUN] = mun;
     f[BG_MUP] = mup;
     f[BG_EC] = blayer(a, L, l, BL_EC);
     f[BG_NI] = lv.ni;
     f[BG_SP] = 1.0 / (Nc * BUILD_Q);
-    f[BG_SCS] = 1.0 / (dmax1(mun, mup) * Vt * Nc);
+    f[BG_SCS] = 1.0 / (dmax_mean(mun, mup) * Vt * Nc);
     f[BG_IMUN] = 1.0 / mun;
     f[BG_IMUP] = 1.0 / mup;
 }

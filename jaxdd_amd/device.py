@@ -39,6 +39,7 @@ __all__ = [
     "GRADED_SMALLEST",
     "GRIDS",
     "GRID_SPLITS",
+    "LANGEVIN_MOBILITIES",
     "LAYER_PARAMETERS",
     "Device",
     "Structure",
@@ -145,6 +146,8 @@ class Structure:
     gen_mode: str = "file"  # "file" (a profile) or "none" (each layer's G_ehp)
     left_elec: int = -1
     extra: dict = field(default_factory=dict)
+    langevin: np.ndarray | None = None  # per layer: direct recombination by Langevin's rate
+    langevin_mobility: str = "layer"  # `LANGEVIN_MOBILITIES`
 
     @property
     def layout(self) -> Layout:
@@ -162,10 +165,14 @@ class Structure:
 LAYER_PARAMETERS = (
     "L", "eps_r", "E_c", "E_v", "N_c", "N_D", "N_A", "mu_n", "mu_p", "nu_int_n", "nu_int_p",
     "k_direct", "N_t_bulk", "C_n_bulk", "C_p_bulk", "E_t_bulk", "N_anion", "N_cation",
-    "mu_anion", "mu_cation", "G_ehp",
+    "mu_anion", "mu_cation", "G_ehp", "preLangevin",
 )
-#: Layer parameters an input may leave out (0 then).
+#: Layer parameters an input may leave out (0 then; `preLangevin` 1).
 _DEFAULT_ZERO = frozenset({"nu_int_n", "nu_int_p", "C_n_bulk", "C_p_bulk", "E_t_bulk", "G_ehp"})
+#: The mobilities Langevin's rate `preLangevin q (mu_n + mu_p) / eps` takes at a point: "layer"
+#: its own layer's; "simss" those of the interval after it (at a layer's last point before an
+#: interface the interface's, L h nu / V_T), as SIMsalabim takes them.
+LANGEVIN_MOBILITIES = ("layer", "simss")
 
 
 # ---- stage 1: the input as a structure and numbers ----------------------------------------
@@ -188,14 +195,17 @@ def _mobile_layers(setup: Setup, species: str, held: bool) -> np.ndarray:
     return out
 
 
-def device_parameters(setup: Setup, grid_split: str = "exact",
-                      grid: str = "graded") -> tuple[Structure, dict]:
+def device_parameters(setup: Setup, grid_split: str = "exact", grid: str = "graded",
+                      langevin: str = "layer") -> tuple[Structure, dict]:
     """A SIMsalabim input as its `Structure` and numbers (reference: device.py:576-801): a dict
     of `LAYER_PARAMETERS` (arrays over the layers) and the device's ``T``, ``W_L``, ``W_R``,
     ``G_frac``, ``V_pre``, ``gen_x``, ``gen_G``, ``R_series``, ``R_shunt``. ``grid``: one of
-    `GRIDS`; ``grid_split``: the SIMsalabim grid's share rule (`layer_points`)."""
+    `GRIDS`; ``grid_split``: the SIMsalabim grid's share rule (`layer_points`); ``langevin``:
+    the mobilities of Langevin's rate in layers with `useLangevin` (`LANGEVIN_MOBILITIES`)."""
     if grid not in GRIDS:
         raise ValueError(f"grid {grid!r}: one of {GRIDS}")
+    if langevin not in LANGEVIN_MOBILITIES:
+        raise ValueError(f"langevin {langevin!r}: one of {LANGEVIN_MOBILITIES}")
     layers = setup.layers
     if setup.i("leftElec") not in (-1, 1):
         raise ValueError("leftElec must be -1 (the left electrode the cathode) or 1 (the anode)")
@@ -208,6 +218,8 @@ def device_parameters(setup: Setup, grid_split: str = "exact",
               else simss_layout(thicknesses, NP, setup.f("grad"), grid_split))
 
     def number(ly, name):
+        if name == "preLangevin" and name not in ly.raw:
+            return 1.0
         if name in _DEFAULT_ZERO and name not in ly.raw:
             return 0.0
         return ly.f(name)
@@ -228,6 +240,8 @@ def device_parameters(setup: Setup, grid_split: str = "exact",
         cation_mobile=_mobile_layers(setup, "cation", held), cation_fixed=fixed("cation"),
         pre_cond=bool(setup.i("preCond", 0)), fix_ions=bool(setup.i("fixIons", 0)),
         sfb=sfb, gen_mode=gen_mode, left_elec=setup.i("leftElec"),
+        langevin=np.array([int(float(ly.raw.get("useLangevin", "0"))) == 1 for ly in layers]),
+        langevin_mobility=langevin,
     )
     for name, ly, flat in (("W_L", layers[0], sfb[0]), ("W_R", layers[-1], sfb[1])):
         if not flat and not ly.f("E_c") <= setup.f(name) <= ly.f("E_v"):
@@ -330,6 +344,18 @@ def device_arrays(structure: Structure, p: dict) -> Device:
     def in_traps(a):
         return np.where(trap, on_points(a), 0.0)
 
+    # direct recombination: each layer's k_direct, or where the layer uses Langevin's rate,
+    # preLangevin q (mu_n + mu_p) / eps (SIMsalabim's Calc_Langevin_Factor); none at the contacts
+    k_direct = on_points(p["k_direct"])
+    if st.langevin is not None and np.any(st.langevin):
+        if st.langevin_mobility == "simss":
+            mob_n = interval_mobility("mu_n", "nu_int_n")
+            mob_p = interval_mobility("mu_p", "nu_int_p")
+        else:
+            mob_n, mob_p = on_points(p["mu_n"]), on_points(p["mu_p"])
+        rate = on_points(p["preLangevin"]) * Q / (on_points(p["eps_r"]) * EPS_0)
+        k_direct = np.where(st.langevin[lid], rate * (mob_n + mob_p), k_direct)
+
     ions = {}
     for sp in ("anion", "cation"):
         mobile, fixed = getattr(st, f"{sp}_mobile"), getattr(st, f"{sp}_fixed")
@@ -343,7 +369,7 @@ def device_arrays(structure: Structure, p: dict) -> Device:
         eps=EPS_0 * on_points(p["eps_r"]), E_c=Ec, E_v=Ev, N_c=Nc, ni=lv.ni[lid],
         N_D=on_points(p["N_D"]), N_A=on_points(p["N_A"]),
         mu_n=interval_mobility("mu_n", "nu_int_n"), mu_p=interval_mobility("mu_p", "nu_int_p"),
-        k_direct=np.where(interior, on_points(p["k_direct"]), 0.0),
+        k_direct=np.where(interior, k_direct, 0.0),
         srh_Nt=in_traps(p["N_t_bulk"]), srh_Cn=in_traps(p["C_n_bulk"]),
         srh_Cp=in_traps(p["C_p_bulk"]), srh_nt0=in_traps(lv.nt0), srh_pt0=in_traps(lv.pt0),
         G=_generation(st, p, x, total),
@@ -364,11 +390,12 @@ def device_arrays(structure: Structure, p: dict) -> Device:
     )
 
 
-def build_device(setup: Setup, grid_split: str = "exact", grid: str = "graded") -> Device:
+def build_device(setup: Setup, grid_split: str = "exact", grid: str = "graded",
+                 langevin: str = "layer") -> Device:
     """The device of a SIMsalabim input (reference: device.py:1066-1079): ``grid`` "graded"
     (the default) or "simss" (SIMsalabim's own grid, to compare point by point with SimSS),
-    ``grid_split`` as in `layer_points`."""
-    return device_arrays(*device_parameters(setup, grid_split, grid))
+    ``grid_split`` as in `layer_points`, ``langevin`` as in `device_parameters`."""
+    return device_arrays(*device_parameters(setup, grid_split, grid, langevin))
 
 
 def with_parameters(structure: Structure, params: dict, **changes) -> Device:

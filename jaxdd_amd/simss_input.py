@@ -235,7 +235,7 @@ def ion_region_problems(setup: Setup) -> list[str]:
 # ---- what this path solves -----------------------------------------------------------------
 #: Layer switches whose other values turn on physics outside this path, with their off value.
 _LAYER_SWITCHES = {
-    "mobnDep": "0", "mobpDep": "0", "fieldDepG": "0", "useLangevin": "0",
+    "mobnDep": "0", "mobpDep": "0", "fieldDepG": "0",
     "bulkTrapType": "0", "intTrapFile": "none", "bulkTrapFile": "none",
 }
 

@@ -130,6 +130,30 @@ ION_WRT = [("N_anion", 1), ("N_cation", 1), ("L", 1), ("L", 0), ("mu_n", 1), ("e
            ("N_A", 2), ("N_t_bulk", 1), ("G_ehp", 1), "W_L", "T", "G_frac"]
 
 
+#: Named numbers of Langevin's rate (jaxdd_amd.builder's): the prefactors, the mobilities and
+#: permittivity it takes, an interface velocity (the interval's mobility at a layer's last
+#: point), a thickness.
+LANGEVIN_WRT = [("preLangevin", 1), ("preLangevin", 0), ("mu_n", 1), ("mu_p", 2), ("eps_r", 1),
+                ("nu_int_n", 0), ("L", 1)]
+
+
+def langevin_rows(n, NP, mode, ions=False):
+    """(structure, numbers) of ``n`` drawn stacks with Langevin's rate in every layer
+    (`preLangevin` 1e-3, 1e-2, 1e-3, as the SimSS run `langevin_light`) at the mobilities
+    ``mode`` ("layer" or "simss")."""
+    import dataclasses
+
+    from jaxdd_amd import synthetic as S
+
+    t = S.draw(n)
+    out = []
+    for k in range(n):
+        st, p = S.row_inputs(t, k, NP, ions, False)
+        st = dataclasses.replace(st, langevin=np.ones(len(st.i0), bool), langevin_mobility=mode)
+        out.append((st, dict(p, preLangevin=np.array([1e-3, 1e-2, 1e-3]))))
+    return out
+
+
 def bumped(p, w, h):
     """The numbers ``p`` with ``w`` (a name or a (name, layer) pair) moved by ``h``."""
     q = {k: (np.array(v, dtype=float) if isinstance(v, np.ndarray) else v) for k, v in p.items()}

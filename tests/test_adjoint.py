@@ -216,6 +216,31 @@ def test_frozen_ions_reverse_pass_is_the_directions_passes(frozen_case, emu):
     assert np.max(err) < 5e-11, np.unravel_index(np.argmax(err), err.shape)
 
 
+@pytest.mark.parametrize("mode", ["layer", "simss"])
+def test_langevin_gradients_are_finite_differences(emu, oracle, mode):
+    """Langevin's rate (reference: src/jaxdd/device.py:900-911, its gradient in `preLangevin`
+    tests/test_solver.py:1104-1122): the adjoint replay's gradients in the rate's numbers
+    (prefactors, mobilities, permittivity, an interface velocity, a thickness) against central
+    differences of the oracle's curves, each device's structure held."""
+    from conftest import LANGEVIN_WRT, langevin_rows
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd import synthetic as S
+
+    inputs = langevin_rows(2, 100, mode)
+    structs, params = zip(*inputs)
+    dirs = A.directions_for(LANGEVIN_WRT, 2, len(structs[0].i0))
+    packed, tan = emu.build_packed(list(structs), list(params), dirs)
+    V = S.VOLTAGES[::10].copy()
+    states = np.zeros((len(V), 100, 3, 2))
+    J, ok, its, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                     states=states)
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    assert not status.any() and ok.all()
+    err = named_fd_errors(oracle, structs, params, LANGEVIN_WRT, V, J, grad)
+    assert np.max(err) < 1e-5, err
+    assert np.median(err) < 3e-7, err
+
+
 @pytest.fixture(scope="module")
 def ion_case(emu):
     from jaxdd_amd import adjoint as A

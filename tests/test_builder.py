@@ -6,6 +6,8 @@ differences of the host builder on each device's structure; the GPU against both
 import numpy as np
 import pytest
 
+V_FULL = np.round(np.arange(-0.2, 1.3 + 1e-9, 0.025), 6)
+
 
 def _np(t):
     return t.detach().cpu().numpy()
@@ -48,6 +50,42 @@ def test_replay_builds_the_host_batch(emu, config):
     structs, params = zip(*inputs)
     ref = pack([device_arrays(st, p) for st, p in inputs])
     _same_packed(emu.build_packed(list(structs), list(params)), ref)
+
+
+@pytest.mark.parametrize("mode", ["layer", "simss"])
+def test_replay_builds_langevin_devices(emu, mode):
+    """Langevin's rate in every layer (reference: src/jaxdd/device.py:900-911), at each layer's
+    mobilities or SIMsalabim's (the interface's at a layer's last point): the builder's lanes
+    give the host builder's packed arrays (1e-14 of each field's scale), and their tangents in
+    the rate's numbers the host builder's central differences (relative step 1e-6; 1e-8 of
+    each field's scale per unit of ln x)."""
+    from conftest import LANGEVIN_WRT, bumped, langevin_rows, number_of
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import pack
+    from jaxdd_amd.device import device_arrays
+
+    inputs = langevin_rows(4, 80, mode)
+    structs, params = zip(*inputs)
+    ref = pack([device_arrays(st, p) for st, p in inputs])
+    dirs = A.directions_for(LANGEVIN_WRT, len(inputs), len(structs[0].i0))
+    got, tan = emu.build_packed(list(structs), list(params), dirs)
+    _same_packed(got, ref)
+    # the rate (SEG_KD) at every interior point
+    kd = np.take_along_axis(ref["seg"][:, 5, :], ref["sid"].astype(np.int64), axis=0)
+    assert np.all(kd[1:-1] > 0)
+    for key in ("pt", "seg", "sc"):
+        axis = tuple(i for i in range(ref[key].ndim) if i != ref[key].ndim - 2)
+        scale = np.max(np.abs(ref[key]), axis=axis, keepdims=True)[..., 0]
+        for k, w in enumerate(LANGEVIN_WRT):
+            for d, (st, p) in enumerate(inputs):
+                x = number_of(p, w)
+                h = 1e-6 * abs(x)
+                up, dn = (pack([device_arrays(st, bumped(p, w, s * h))])[key][..., 0]
+                          for s in (1.0, -1.0))
+                fd = (up - dn) / (2 * h)
+                g = tan[key][k][..., d][:fd.shape[0]]
+                err = np.abs(g - fd) * abs(x)
+                assert np.all(err <= 1e-8 * scale[:fd.shape[0]] + 1e-300), (key, w, d)
 
 
 def test_replay_tangents_are_the_host_builders_differences(emu):
@@ -307,6 +345,57 @@ def test_gpu_gradients_with_mobile_ions(torch_cuda, oracle, emu):
           f"{comp.max():.2e} median {np.median(comp):.2e}", flush=True)
     assert abs(a - b) < 1e-6 * abs(b)
     err = named_fd_errors(oracle, structs[:3], params[:3], ION_WRT, V, J, g)
+    print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
+    assert np.max(err) < 1e-5, err
+    assert np.median(err) < 3e-7, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["layer", "simss"])
+def test_gpu_langevin(torch_cuda, oracle, emu, mode):
+    """Langevin's rate on the GPU: the SimSS run `langevin_light` through `solve_jv` (within
+    1e-9 of its scale at SIMsalabim's mobilities, the reference's bar; the oracle's steps and
+    flags, 1e-12) and through `solve_batch` (the oracle's flags and steps, 1e-12); drawn stacks
+    built by ddh_build with the rate in every layer against the host builder, and their
+    gradients in the rate's numbers (`param_gradients`) against central differences of the
+    oracle's curves (1e-5 of each curve's scale)."""
+    from conftest import GOLDEN, LANGEVIN_WRT, fixture_jv, langevin_rows, named_fd_errors
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import pack, prepare, solve_batch
+    from jaxdd_amd.builder import build
+    from jaxdd_amd.device import build_device, device_arrays
+    from jaxdd_amd.simss_input import read_setup
+    from jaxdd_amd.solver import solve_jv
+
+    print(f"Langevin ({mode}): the SimSS run on the GPU", flush=True)
+    dev = build_device(read_setup(GOLDEN / "langevin_light" / "simulation_setup.txt"),
+                       grid="simss", langevin=mode)
+    ref = fixture_jv("langevin_light")
+    scale = np.max(np.abs(ref["Jext"]))
+    res = solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    Jo, oko, itso = oracle.solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    assert res.converged.all() and np.array_equal(res.iterations, itso)
+    assert np.max(np.abs(res.J - Jo)) < 1e-12 * scale
+    if mode == "simss":
+        assert np.max(np.abs(res.J - ref["Jext"])) < 1e-9 * scale
+    J, ok, its = (_np(x) for x in solve_batch(prepare([dev]), V_FULL, bernoulli_name="simss"))
+    Jb, okb, itsb = oracle.solve_batch([dev], V_FULL, bernoulli_name="simss", first_max_iter=None)
+    assert np.array_equal(ok, okb) and np.array_equal(its, itsb)
+    assert np.max(np.abs(J - Jb)) < 1e-12 * scale
+
+    inputs = langevin_rows(8, 100, mode)
+    structs, params = zip(*inputs)
+    print(f"Langevin ({mode}): 8 built devices, {len(LANGEVIN_WRT)} numbers", flush=True)
+    batch = build(list(structs), list(params))
+    want = pack([device_arrays(st, p) for st, p in inputs])
+    got = {"pt": _np(batch["_pt"]), "seg": _np(batch["_seg"]), "sc": _np(batch["_sc"]),
+           "imu": _np(batch["_imu"]), "sid": _np(batch["_sid"]), "nseg": batch["nseg"]}
+    _same_packed(got, want, rtol=1e-13)
+    V = V_FULL[::10].copy()
+    J, ok, _, g = A.param_gradients(structs, params, LANGEVIN_WRT, V, tol=1e-12, max_iter=200)
+    J, ok, g = _np(J), _np(ok), _np(g)
+    assert ok.all() and np.isfinite(g).all()
+    err = named_fd_errors(oracle, structs[:3], params[:3], LANGEVIN_WRT, V, J, g)
     print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
     assert np.max(err) < 1e-5, err
     assert np.median(err) < 3e-7, err

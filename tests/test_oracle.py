@@ -6,7 +6,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, curve_scale, fixture_device, fixture_jv
+from conftest import FIXTURES, GOLDEN, curve_scale, fixture_device, fixture_jv
 
 
 # ---- Bernoulli (src/jaxdd/solver.py:95-121; reference tests test_solver.py:50-64) ----------
@@ -157,6 +157,36 @@ def test_oracle_batch_reproduces_simss(oracle, case):
     assert ok.all()
     scale = np.max(np.abs(ref["Jext"]))
     assert np.max(np.abs(J[0] - ref["Jext"])) < FIXTURES[case] * scale
+
+
+def test_langevin_recombination_reproduces_simss(oracle):
+    """SimSS's Langevin recombination in all three layers (`useLangevin`, `preLangevin` 1e-3,
+    1e-2, 1e-3; reference tests/test_solver.py:1083-1101): at SIMsalabim's mobilities (those of
+    the interval after each point, `langevin="simss"`) within 1e-9 of the curve's scale; at
+    each point's own layer's (the default) 1e-5 to 1e-4 off, the points before an interface
+    taking the layer's mobility where SimSS takes the interface's. The rate moves the curve by
+    per cent: without it the curve is far off."""
+    import dataclasses
+
+    from jaxdd_amd.device import build_device, device_arrays, device_parameters
+    from jaxdd_amd.simss_input import check_supported, read_setup
+
+    setup = read_setup(GOLDEN / "langevin_light" / "simulation_setup.txt")
+    check_supported(setup)
+    ref = fixture_jv("langevin_light")
+    scale = np.max(np.abs(ref["Jext"]))
+    gap = {}
+    for mode in ("simss", "layer"):
+        J, ok, _ = oracle.solve_jv(build_device(setup, grid="simss", langevin=mode), ref["Vint"],
+                                   bernoulli_name="simss")
+        assert ok.all()
+        gap[mode] = np.max(np.abs(J - ref["Jext"])) / scale
+    assert gap["simss"] < 1e-9
+    assert 1e-5 < gap["layer"] < 1e-4
+    st, p = device_parameters(setup, grid="simss")
+    off = device_arrays(dataclasses.replace(st, langevin=np.zeros(3, bool)), p)
+    J, _, _ = oracle.solve_jv(off, ref["Vint"], bernoulli_name="simss")
+    assert np.max(np.abs(J - ref["Jext"])) / scale > 1e-3
 
 
 def test_dark_current_is_monotone_and_zero_at_zero(oracle):
