@@ -437,8 +437,12 @@ DDH_FN CurSeg cur_seg(const Lane &L, int sid) {
 }
 
 // The fluxes over the interval from l to r (src/jaxdd/solver.py:1069-1108); the interval's
-// mobilities are those of l's segment (the interval after the point).
+// mobilities are those of l's segment (the interval after the point). KIND: the Bernoulli
+// function's kind fixed at compile time (0 exact, 1 SIMsalabim's; a sweep instantiated for
+// one kind carries no code or registers of the other), or -1 for `bern` at run time.
+template <int KIND = -1>
 DDH_FN Edge edge_flux_pt(const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
+    if (KIND >= 0) bern = KIND;
     Edge e;
     const double lcn = Vt * l.mun, lcp = Vt * l.mup;
     double xn = ((r.V + r.offn) - (l.V + l.offn)) * iVt;
@@ -457,9 +461,10 @@ DDH_FN Edge edge_flux_pt(const Pt &l, const Pt &r, double Vt, double iVt, int be
     return e;
 }
 
+template <int KIND = -1>
 DDH_FN Edge edge_flux(const Lane &L, const Pt &l, const Pt &r, double Vt, double iVt, int bern) {
     (void)L;
-    return edge_flux_pt(l, r, Vt, iVt, bern);
+    return edge_flux_pt<KIND>(l, r, Vt, iVt, bern);
 }
 
 // An interior point's loads. A sweep issues them one point ahead of their use, so that their

@@ -45,11 +45,12 @@ struct PairBack {
 #define DDH_PAIRFN __device__ __attribute__((noinline))
 #endif
 
-// The elimination of one half (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination).
-template <typename T, typename S, int K>
+// The elimination of one half (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination),
+// for the Bernoulli function of kind BK (0 exact, 1 SIMsalabim's; the launch's `bern`).
+template <typename T, typename S, int K, int BK>
 DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cref, PairElim &out) {
     const SweepCtl c = cref;
-    const int NP = DDH_UNIFORM(a.NP), bern = DDH_UNIFORM(a.bern);
+    const int NP = DDH_UNIFORM(a.NP), bern = BK;
     const int m = NP / 2;
     const int dir = half ? -1 : 1;
     const int j0 = half ? NP + 1 : 0;
@@ -91,7 +92,7 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
         make_point(Lv, j, false, ct, now, now.sid, c, Vt, iVt, pn);
         // the edge between mid and the new point, left to right
         const Pt el = dir > 0 ? mid : pn, er = dir > 0 ? pn : mid;
-        const Edge en = edge_flux(Lv, el, er, Vt, iVt, bern);
+        const Edge en = edge_flux<BK>(Lv, el, er, Vt, iVt, bern);
         if constexpr (K > 1) {
             // the ion numbers over the half's intervals: left 0..m-1, right m..NP
             if (dir > 0 ? j <= m : j >= m) {
@@ -459,20 +460,26 @@ struct PairStep {
     double z[2];
 };
 
+template <typename T, int BK>
+DDH_FN void pair_elim(const Args &a, int lane, int half, bool ions, PairStep &st) {
+    if constexpr (sizeof(T) == 4) {
+        if (ions) elim_half<float, float, 3, BK>(a, lane, half, st.c, st.e);
+        else elim_half<float, float, 1, BK>(a, lane, half, st.c, st.e);
+    } else if (a.mixed) {
+        if (ions) elim_half<double, float, 3, BK>(a, lane, half, st.c, st.e);
+        else elim_half<double, float, 1, BK>(a, lane, half, st.c, st.e);
+    } else {
+        if (ions) elim_half<double, double, 3, BK>(a, lane, half, st.c, st.e);
+        else elim_half<double, double, 1, BK>(a, lane, half, st.c, st.e);
+    }
+}
+
 template <typename T>
 DDH_FN void pair_phase1(const Args &a, int lane, int half, bool ions, lcdouble *lseg,
                         const LaneState &S, PairStep &st) {
     st.c = sweep_ctl(a, lane, S, lseg);
-    if constexpr (sizeof(T) == 4) {
-        if (ions) elim_half<float, float, 3>(a, lane, half, st.c, st.e);
-        else elim_half<float, float, 1>(a, lane, half, st.c, st.e);
-    } else if (a.mixed) {
-        if (ions) elim_half<double, float, 3>(a, lane, half, st.c, st.e);
-        else elim_half<double, float, 1>(a, lane, half, st.c, st.e);
-    } else {
-        if (ions) elim_half<double, double, 3>(a, lane, half, st.c, st.e);
-        else elim_half<double, double, 1>(a, lane, half, st.c, st.e);
-    }
+    if (DDH_UNIFORM(a.bern)) pair_elim<T, 1>(a, lane, half, ions, st);
+    else pair_elim<T, 0>(a, lane, half, ions, st);
 }
 
 template <typename T>

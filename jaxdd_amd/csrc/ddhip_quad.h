@@ -98,12 +98,13 @@ DDH_FN int quad_pre_column(int nseg, int slot, int q) {
     return nseg * SEG_N * QUAD_LS + 2 * slot + ((q - 1) & 1);
 }
 
-// The elimination of one chunk (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination).
-template <typename T, typename S>
+// The elimination of one chunk (src/jaxdd/solver.py:1021-1187 rows, :1534-1560 elimination),
+// for the Bernoulli function of kind BK (0 exact, 1 SIMsalabim's; the launch's `bern`).
+template <typename T, typename S, int BK>
 DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cref, ldouble *pre,
                              QuadElim &out) {
     const SweepCtl c = cref;
-    const int NP = DDH_UNIFORM(a.NP), bern = DDH_UNIFORM(a.bern);
+    const int NP = DDH_UNIFORM(a.NP), bern = BK;
     const QuadChunk ch = quad_chunk(NP, q);
     const int dir = ch.dir, j0 = ch.j0, nvis = ch.rows + 2;
     const bool spike = ch.spike != 0;
@@ -149,7 +150,7 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
         fetch_point(Lv, s1, s2, v + 1 < nvis ? j + dir : j, nxt);
         make_point(Lv, j, false, ct, now, now.sid, c, Vt, iVt, pn);
         const Pt el = dir > 0 ? mid : pn, er = dir > 0 ? pn : mid;
-        const Edge en = edge_flux(Lv, el, er, Vt, iVt, bern);
+        const Edge en = edge_flux<BK>(Lv, el, er, Vt, iVt, bern);
         if (has_row) {
             const int i = j - dir;
             const Pt lft = dir > 0 ? far : pn, rgt = dir > 0 ? pn : far;
@@ -515,9 +516,17 @@ template <typename T>
 DDH_FN void quad_phase1(const Args &a, int lane, int q, lcdouble *lseg, ldouble *pre,
                         const LaneState &S, QuadStep &st) {
     st.c = sweep_ctl(a, lane, S, lseg);
-    if constexpr (sizeof(T) == 4) elim_quarter<float, float>(a, lane, q, st.c, pre, st.e);
-    else if (a.mixed) elim_quarter<double, float>(a, lane, q, st.c, pre, st.e);
-    else elim_quarter<double, double>(a, lane, q, st.c, pre, st.e);
+    const bool simss = DDH_UNIFORM(a.bern) != 0;
+    if constexpr (sizeof(T) == 4) {
+        if (simss) elim_quarter<float, float, 1>(a, lane, q, st.c, pre, st.e);
+        else elim_quarter<float, float, 0>(a, lane, q, st.c, pre, st.e);
+    } else if (a.mixed) {
+        if (simss) elim_quarter<double, float, 1>(a, lane, q, st.c, pre, st.e);
+        else elim_quarter<double, float, 0>(a, lane, q, st.c, pre, st.e);
+    } else {
+        if (simss) elim_quarter<double, double, 1>(a, lane, q, st.c, pre, st.e);
+        else elim_quarter<double, double, 0>(a, lane, q, st.c, pre, st.e);
+    }
 }
 
 template <typename T>
