@@ -39,6 +39,10 @@ struct PairBack {
     double endn, endp;          // left: the left contact's; right: the right contact's
 };
 
+// The wavefront's segment tables in the local data share: 32 device slots, row stride 32 (a
+// pair's two lanes read their device's column), 39 KB for seven segments.
+constexpr int PAIR_LS = 32;
+
 #ifdef DDH_CPU_EMULATION
 #define DDH_PAIRFN inline
 #else
@@ -59,7 +63,7 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
     const double tot0 = c.mobile ? SC(SC_AT) * c.share : 0.0;
     const double tot1 = c.mobile ? SC(SC_CT) * c.share : 0.0;
     const double itot0 = tot0 > 0 ? 1.0 / tot0 : 0.0, itot1 = tot1 > 0 ? 1.0 / tot1 : 0.0;
-    const Lane Lv = lane_view(a, lane, c);
+    const Lane Lv = lane_view(a, lane, c, PAIR_LS);
     const size_t n = Lv.n;
     DDH_GLOBAL S *fac = (DDH_GLOBAL S *)(factors<S>(a) + lane);
     gdouble *s1 = c.fresh ? Lv.ys1 : Lv.yw;
@@ -305,7 +309,7 @@ DDH_PAIRFN void backsub_half(const Args &a, int lane, int half, const SweepCtl &
     const int cnt = half ? NP - m : m;
     const double Vt = SC(SC_VT), iVt = 1.0 / Vt, nfl = SC(SC_NFLOOR), lnfloor = log(nfl);
     const double capv = CAP_V * Vt;
-    const Lane Lv = lane_view(a, lane, c);
+    const Lane Lv = lane_view(a, lane, c, PAIR_LS);
     const size_t n = Lv.n;
     DDH_GLOBAL const S *fac = (DDH_GLOBAL const S *)(factors<S>(a) + lane);
     const double z0 = zin[0], z1 = zin[1];

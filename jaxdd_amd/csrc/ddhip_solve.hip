@@ -74,14 +74,14 @@ __device__ inline R lane_record(const R &r, int from) {
 // Two lanes per device (ddhip_pair.h): 32 devices per wavefront. The left lane of each pair
 // runs the Poisson start; both lanes then run the same state machine on their own state.
 __global__ void __launch_bounds__(64) solve_pair_kernel(Args a) {
-    extern __shared__ double lds[];  // a.nseg * SEG_N * WAVE doubles (32 device slots used)
+    extern __shared__ double lds[];  // a.nseg * SEG_N * PAIR_LS doubles (32 device slots)
     const int t = threadIdx.x, half = t & 1;
     const int lane = blockIdx.x * 32 + (t >> 1);  // the device
     const bool valid = lane < a.n;
     ldouble *lseg = (ldouble *)lds + (t >> 1);
     LaneState &S = a.st[valid ? 2 * (size_t)lane + half : 0];
     if (valid && half == 0) {
-        fill_segments(a, lane, lseg);
+        fill_segments(a, lane, lseg, PAIR_LS);
         poisson_lane(a, lane, S);
     }
     __syncthreads();  // the segment table and the Poisson start, for the right lanes
@@ -575,9 +575,10 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
         hipLaunchKernelGGL(ddh::solve_quad_kernel, grid, block, qlds, s, a);
     } else if (lanes == 2) {
         dim3 grid((b->n + 31) / 32), block(64);
+        const size_t plds = (size_t)b->nseg * ddh::SEG_N * ddh::PAIR_LS * sizeof(double);
         (void)hipFuncSetAttribute((const void *)ddh::solve_pair_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(ddh::solve_pair_kernel, grid, block, lds, s, a);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
+        hipLaunchKernelGGL(ddh::solve_pair_kernel, grid, block, plds, s, a);
     } else {
         dim3 grid((b->n + 63) / 64), block(64);
         (void)hipFuncSetAttribute((const void *)ddh::solve_kernel,

@@ -65,16 +65,16 @@ static void run(Args &a) {
 static void run_pair(Args &a) {
     const int nw = (a.n + 31) / 32;
     std::vector<char> started(nw, 0), done(nw, 0);
-    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * WAVE);
+    std::vector<double> lds_all((size_t)nw * a.nseg * SEG_N * PAIR_LS);
     int remaining = nw;
     while (remaining > 0) {
         for (int w = 0; w < nw; ++w) {
             if (done[w]) continue;
             const int d0 = w * 32, d1 = d0 + 32 < a.n ? d0 + 32 : a.n;
-            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * WAVE];
+            double *lds = &lds_all[(size_t)w * a.nseg * SEG_N * PAIR_LS];
             if (!started[w]) {
                 for (int d = d0; d < d1; ++d) {
-                    fill_segments(a, d, lds + (d - d0));
+                    fill_segments(a, d, lds + (d - d0), PAIR_LS);
                     poisson_lane(a, d, a.st[2 * d]);
                     a.st[2 * d + 1] = a.st[2 * d];
                     for (int h = 0; h < 2; ++h) {
