@@ -140,6 +140,34 @@ def test_solve_jv_reproduces_simss(torch_cuda, case, precision):
         assert np.max(np.abs(res.V_ext - ref["Vext"])) < 1e-4
 
 
+def test_zimt_steady_states_through_the_steady_solver(torch_cuda, oracle):
+    """ZimT 5.36's steady states (`zimt_steady_light`: t = 0 on each of seven lines from -0.2
+    to 1.1 V, the last at half light) by `solve_jv` on the GPU, one line at a time as the
+    reference's own test solves them (tests/test_solver.py:454-470): ZimT's Jext to 1e-9 of
+    its scale, and the oracle's steps and currents (1e-12)."""
+    import dataclasses
+
+    from conftest import GOLDEN
+    from jaxdd_amd.device import build_device
+    from jaxdd_amd.simss_input import check_supported, read_setup, read_tvg
+    from jaxdd_amd.solver import solve_jv
+
+    setup = read_setup(GOLDEN / "zimt_steady_light" / "simulation_setup.txt")
+    check_supported(setup)
+    tvg = read_tvg(setup)
+    assert np.all(tvg.t == 0) and not tvg.oc.any()
+    ref = np.genfromtxt(GOLDEN / "zimt_steady_light" / "tj.dat", names=True)
+    dev = build_device(setup, grid="simss")
+    scale = np.max(np.abs(ref["Jext"]))
+    for v, g, want in zip(tvg.Vext, tvg.G_frac, ref["Jext"]):
+        d = dataclasses.replace(dev, G_frac=float(g))
+        res = solve_jv(d, np.array([v]), bernoulli_name="simss")
+        Jo, oko, itso = oracle.solve_jv(d, np.array([v]), bernoulli_name="simss")
+        assert res.converged.all() and np.array_equal(res.iterations, itso)
+        assert abs(res.J[0] - Jo[0]) < 1e-12 * scale
+        assert abs(res.J[0] - want) < 1e-9 * scale, (v, g)
+
+
 def test_solve_jv_matches_oracle(torch_cuda, oracle):
     from jaxdd_amd.solver import solve_jv
 

@@ -13,11 +13,15 @@ import pytest
 from conftest import GOLDEN
 
 #: ZimT 5.36 runs copied from the reference's tests/data, with the tolerance (of the largest
-#: |Jext|) the reference's own tests hold them to (tests/test_solver.py:504-580): electrons
-#: and holes to 1e-9, bulk traps in time to 1e-9, mobile ions (5 x 5) to 1e-7.
+#: |Jext|) the reference's own tests hold them to (tests/test_solver.py:454-470, 504-616):
+#: electrons and holes to 1e-9 (`zimt_sweep_spv_light` is the sweep with ZimT's layer voltages
+#: written, `zimt_steady_light` seven steady states, t = 0 on every line, the last at half
+#: light), bulk traps in time to 1e-9, mobile ions (5 x 5) to 1e-7.
 ZIMT = {
     "zimt_photo_light": 1e-9,
     "zimt_sweep_light": 1e-9,
+    "zimt_sweep_spv_light": 1e-9,
+    "zimt_steady_light": 1e-9,
     "zimt_traps_photo_light": 1e-9,
     "zimt_ions_slow_light": 1e-7,
     "zimt_ions_fast_light": 1e-7,
