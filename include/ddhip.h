@@ -221,26 +221,34 @@ int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L, double
 /* The batch's packed arrays from its structures and numbers on the GPU: replaces
  * src/jaxdd/device.py:877-1063 (`device_arrays`, the grid placed by `_spacing` :225-233) and
  * the packing of src/jaxdd/batch.py:151-223 (`prepare`), for the path's physics (Boltzmann
- * carriers, uniform generation per layer, ohmic contacts, one bulk trap level, mobile and fixed
- * ions' masks and totals, direct recombination by k_direct or Langevin's rate, device.py:900-911).
+ * carriers, uniform generation per layer or a generation profile, ohmic contacts, one bulk trap
+ * level, mobile and fixed ions' masks and totals, direct recombination by k_direct or Langevin's
+ * rate, device.py:900-911).
  * layers [nl][22][n]: per layer L, eps_r, E_c, E_v, N_c, N_D, N_A, mu_n, mu_p, nu_int_n,
  * nu_int_p, k_direct, N_t_bulk, C_n_bulk, C_p_bulk, E_t_bulk, G_ehp, N_anion, N_cation, mu_anion,
  * mu_cation, preLangevin; structure->flags [nl][n] bits: 1 bulk traps, 2 mobile anions, 4 fixed
  * anions, 8 mobile cations, 16 fixed cations, 32 Langevin's rate preLangevin q (mu_n + mu_p) /
  * eps in place of k_direct, 64 that rate at the interval's mobilities (the interface's at a
- * layer's last point) instead of the layer's; scalars [9][n]: T, W_L, W_R, G_frac, V_pre, 1/R_shunt
- * (0: none), pre_cond, fix_ions, left_elec. Outputs the ddh_batch arrays pt [NP + 2][4][n]
+ * layer's last point) instead of the layer's, 128 the layer generates (a profile's `layerGen`);
+ * scalars [10][n]: T, W_L, W_R, G_frac, V_pre, 1/R_shunt (0: none), pre_cond, fix_ions,
+ * left_elec, gen_profile (1: the device's generation is its profile, 0: each layer's G_ehp).
+ * gen [2][gen_m][n]: each device's generation profile, positions (ascending, their last the
+ * profile's length, stretched to the stack) then G (m^-3 s^-1) at G_frac = 1, a device's
+ * shorter table padded by repeating its last node (device.py:804-814 `_generation`: NumPy's
+ * interp, 0 at and beyond the last position, in generating layers only); gen_m 0 when no device
+ * takes a profile, else >= 2. Outputs the ddh_batch arrays pt [NP + 2][4][n]
  * (h, x, G, fixed anions - cations), seg [nseg][22][n] (each segment's fields, a device's
  * segments beyond its own count repeating its last), sc [20][n], and ion_mu [nseg][2][n] (each
  * segment's anion and cation mobility). With K > 0 directions dlayers [K][nl][22][n] and
- * dscalars [K][9][n] (each a direction in the numbers), also their derivatives along each,
+ * dscalars [K][10][n] (each a direction in the numbers), also their derivatives along each,
  * forward mode: dpt [K][NP + 2][4][n], dseg [K][nseg][22][n], dsc [K][20][n], the tangents
- * ddh_adjoint takes; the structure is held (the thicknesses scale each layer's spacing), as
- * the reference differentiates `device_tree(structure, params)` (batch.py:142-148). */
+ * ddh_adjoint takes; the structure is held (the thicknesses scale each layer's spacing and
+ * stretch the profile), as the reference differentiates `device_tree(structure, params)`
+ * (batch.py:142-148). */
 int ddh_build(const ddh_structure *structure, const double *layers, const double *scalars,
-              int32_t K, const double *dlayers, const double *dscalars, double *pt, double *seg,
-              double *sc, double *ion_mu, double *dpt, double *dseg, double *dsc,
-              void *hip_stream);
+              const double *gen, int32_t gen_m, int32_t K, const double *dlayers,
+              const double *dscalars, double *pt, double *seg, double *sc, double *ion_mu,
+              double *dpt, double *dseg, double *dsc, void *hip_stream);
 
 /* Device bytes of ddh_impedance's workspace (the per-device tables) for n devices of NP
  * interior points. */

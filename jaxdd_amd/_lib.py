@@ -114,8 +114,8 @@ def load():
     lib.ddh_graded_layout.argtypes = [i32, i32, i32, vp, ctypes.c_double, ctypes.c_double, vp,
                                       vp, vp, vp, vp, vp, vp, vp]
     lib.ddh_build.restype = ctypes.c_int
-    lib.ddh_build.argtypes = [ctypes.POINTER(Structure), vp, vp, i32, vp, vp, vp, vp, vp, vp,
-                              vp, vp, vp, vp]
+    lib.ddh_build.argtypes = [ctypes.POINTER(Structure), vp, vp, vp, i32, i32, vp, vp, vp, vp,
+                              vp, vp, vp, vp, vp, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
     if lib.ddh_abi_version() != 4:

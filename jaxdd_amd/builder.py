@@ -29,19 +29,21 @@ LAYER_FIELDS = ("L", "eps_r", "E_c", "E_v", "N_c", "N_D", "N_A", "mu_n", "mu_p",
                 "nu_int_p", "k_direct", "N_t_bulk", "C_n_bulk", "C_p_bulk", "E_t_bulk", "G_ehp",
                 "N_anion", "N_cation", "mu_anion", "mu_cation", "preLangevin")
 #: The per-device numbers of ddh_build, in its order (G_shunt = 1/R_shunt, 0 without a shunt).
+#: gen_profile: 1 where the device's generation is its profile (`gen_x`, `gen_G`), else 0.
 SCALAR_FIELDS = ("T", "W_L", "W_R", "G_frac", "V_pre", "G_shunt", "pre_cond", "fix_ions",
-                 "left_elec")
+                 "left_elec", "gen_profile")
 MAX_LAYERS = 8
 _TRAPS, _A_MOBILE, _A_FIXED, _C_MOBILE, _C_FIXED = 1, 2, 4, 8, 16
-_LANGEVIN, _LANGEVIN_INTERVAL = 32, 64
+_LANGEVIN, _LANGEVIN_INTERVAL, _LAYER_GEN = 32, 64, 128
 
 
 def check_buildable(st: Structure) -> None:
-    """`NotImplementedError` for a structure outside ddh_build's physics: a generation profile
-    file, flat-band contacts, more than eight layers, optional modules."""
-    if st.gen_mode != "none":
-        raise NotImplementedError("the GPU builder takes each layer's uniform generation "
-                                  "(generation mode 'none'), not a profile")
+    """`NotImplementedError` for a structure outside ddh_build's physics: generation other than
+    each layer's or a profile file's, flat-band contacts, more than eight layers, optional
+    modules."""
+    if st.gen_mode not in ("none", "file"):
+        raise NotImplementedError(f"generation mode {st.gen_mode!r}: the GPU builder takes "
+                                  "each layer's G_ehp or a profile")
     if any(st.sfb):
         raise NotImplementedError("flat-band work functions are outside the GPU builder")
     if st.extra:
@@ -52,12 +54,15 @@ def check_buildable(st: Structure) -> None:
 
 def layer_flags(st: Structure, k: int) -> int:
     """Layer ``k``'s bits for ddh_build: bulk traps, mobile and fixed anions and cations,
-    Langevin's rate (at the interval's mobilities with ``langevin_mobility`` "simss")."""
+    Langevin's rate (at the interval's mobilities with ``langevin_mobility`` "simss"), and
+    whether it takes the generation profile (its `layerGen`)."""
     langevin = st.langevin is not None and bool(st.langevin[k])
+    gen = st.gen_mode == "file" and bool(st.layer_gen[k])
     return ((_TRAPS if st.traps[k] else 0) | (_A_MOBILE if st.anion_mobile[k] else 0)
             | (_A_FIXED if st.anion_fixed[k] else 0) | (_C_MOBILE if st.cation_mobile[k] else 0)
             | (_C_FIXED if st.cation_fixed[k] else 0) | (_LANGEVIN if langevin else 0)
-            | (_LANGEVIN_INTERVAL if langevin and st.langevin_mobility == "simss" else 0))
+            | (_LANGEVIN_INTERVAL if langevin and st.langevin_mobility == "simss" else 0)
+            | (_LAYER_GEN if gen else 0))
 
 
 def segment_ids(st: Structure) -> np.ndarray:
@@ -104,7 +109,7 @@ def structure_arrays(structures) -> dict:
 
 
 def number_arrays(structures, params) -> tuple[np.ndarray, np.ndarray]:
-    """(layers [nl][22][n], scalars [9][n]) of the devices' numbers (`device_parameters`'
+    """(layers [nl][22][n], scalars [10][n]) of the devices' numbers (`device_parameters`'
     dicts: per-layer arrays and scalars; a missing `preLangevin` is 1)."""
     def number(p, f):
         if f == "preLangevin" and f not in p:
@@ -118,14 +123,37 @@ def number_arrays(structures, params) -> tuple[np.ndarray, np.ndarray]:
         R_shunt = float(p.get("R_shunt", -1.0))
         scal.append([float(p["T"]), float(p["W_L"]), float(p["W_R"]), float(p["G_frac"]),
                      float(p["V_pre"]), 1.0 / R_shunt if R_shunt > 0 else 0.0,
-                     float(st.pre_cond), float(st.fix_ions), float(st.left_elec)])
+                     float(st.pre_cond), float(st.fix_ions), float(st.left_elec),
+                     float(st.gen_mode == "file")])
     return np.ascontiguousarray(layers), np.ascontiguousarray(np.array(scal).T)
+
+
+def profile_arrays(structures, params) -> np.ndarray | None:
+    """ddh_build's generation profiles ``gen`` [2][m][n] (positions, then G), m the longest
+    table, a shorter one padded by repeating its last node (the same G at every point); a
+    device without a profile holds the positions 0..1 and G 0 (unread). None when no device
+    takes a profile."""
+    tables = [(np.asarray(p["gen_x"], dtype=float), np.asarray(p["gen_G"], dtype=float))
+              if st.gen_mode == "file" else None for st, p in zip(structures, params)]
+    m = max((len(t[0]) for t in tables if t is not None), default=0)
+    if m == 0:
+        return None
+    if m < 2:
+        raise ValueError("a generation profile needs two nodes at least")
+    gen = np.zeros((2, m, len(tables)))
+    gen[0] = np.linspace(0.0, 1.0, m)[:, None]
+    for d, t in enumerate(tables):
+        if t is not None:
+            k = len(t[0])
+            gen[0, :k, d], gen[1, :k, d] = t
+            gen[0, k:, d], gen[1, k:, d] = t[0][-1], t[1][-1]
+    return np.ascontiguousarray(gen)
 
 
 def build(structures, params, directions=None, device="cuda"):
     """The batch (`batch.prepare`'s dict, on ``device``) of devices given by their structures
     and numbers, built by ddh_build. ``directions``: None, or (dlayers [K][nl][22][n], dscalars
-    [K][9][n]) host arrays, directions in the numbers; then also the tangents {"pt", "seg",
+    [K][10][n]) host arrays, directions in the numbers; then also the tangents {"pt", "seg",
     "sc"} (torch, on the device) along each, as `adjoint.jv_gradients` takes them."""
     dev = _cuda(device)
     sa = structure_arrays(structures)
@@ -138,7 +166,8 @@ def build(structures, params, directions=None, device="cuda"):
     keep["i0"] = t(sa["i0"], torch.int32)
     keep["unit"], keep["unit_sum"] = t(sa["unit"]), t(sa["unit_sum"])
     R_series = [float(p.get("R_series", 0.0)) for p in params]
-    return _build(keep, sa["NP"], sa["nl"], sa["nseg"], layers, scalars, directions, R_series, dev)
+    return _build(keep, sa["NP"], sa["nl"], sa["nseg"], layers, scalars, directions, R_series, dev,
+                  profile_arrays(structures, params))
 
 
 def _cuda(device):
@@ -150,13 +179,15 @@ def _cuda(device):
 
 def build_graded(L, flags, layers, scalars, directions=None, NP: int = 400,
                  beta: float = GRADED_POINTS_PER_EFOLD, d0: float = GRADED_SMALLEST,
-                 R_series=None, device="cuda"):
+                 R_series=None, profile=None, device="cuda"):
     """The batch of devices on graded grids built entirely on the GPU: each device's layout
     from its thicknesses (ddh_graded_layout; ``L`` (n, nl)), then its arrays (ddh_build).
     ``flags`` (nl,) or (nl, n): each layer's bits (1 bulk traps, 2 mobile anions, 4 fixed
     anions, 8 mobile cations, 16 fixed cations, 32 Langevin recombination, 64 Langevin at the
-    interval's mobilities); ``layers`` [nl][22][n] and ``scalars`` [9][n]
-    the numbers (`number_arrays`' layout); ``directions`` as `build`'s. Each point stops its
+    interval's mobilities, 128 the layer takes the profile); ``layers`` [nl][22][n] and
+    ``scalars`` [10][n] the numbers (`number_arrays`' layout); ``profile`` None or the profiles
+    [2][m][n] (`profile_arrays`' layout) of the devices whose `gen_profile` is 1; ``directions``
+    as `build`'s. Each point stops its
     Newton steps when they no longer move it; the host restatement (`grid.graded_layout`) stops
     a layer's points together: positions agree to the last bits, the smallest spacings to
     1e-11 of themselves."""
@@ -193,10 +224,10 @@ def build_graded(L, flags, layers, scalars, directions=None, NP: int = 400,
     keep["flags"] = t(np.broadcast_to(fl.reshape(nl, -1), (nl, n)), torch.uint8)
     R = np.zeros(n) if R_series is None else np.asarray(R_series, dtype=float)
     return _build(keep, NP, nl, int(counts.max()), np.asarray(layers, dtype=float),
-                  np.asarray(scalars, dtype=float), directions, R, dev)
+                  np.asarray(scalars, dtype=float), directions, R, dev, profile)
 
 
-def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev):
+def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev, profile=None):
     lib = _lib.load()
     n = keep["lid"].shape[1]
     P = NP + 2
@@ -204,13 +235,26 @@ def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev):
     def t(a, dtype=torch.float64):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
 
+    if layers.shape != (nl, len(LAYER_FIELDS), n) or scalars.shape != (len(SCALAR_FIELDS), n):
+        raise ValueError(f"layers ({nl}, {len(LAYER_FIELDS)}, {n}) and scalars "
+                         f"({len(SCALAR_FIELDS)}, {n})")
+    gen_m = 0
+    if profile is not None:
+        profile = np.asarray(profile, dtype=float)
+        gen_m = profile.shape[1] if profile.ndim == 3 else 0
+        if profile.shape != (2, gen_m, n) or gen_m < 2:
+            raise ValueError(f"profile: (2, m >= 2, {n})")
+        keep["gen"] = t(profile)
+    elif np.any(scalars[SCALAR_FIELDS.index("gen_profile")] > 0):
+        raise ValueError("gen_profile 1 without a profile")
     keep["layers"], keep["scalars"] = t(layers), t(scalars)
     K = 0
     if directions is not None:
         dl, ds = (np.asarray(x, dtype=float) for x in directions)
         K = dl.shape[0]
         if dl.shape != (K, nl, len(LAYER_FIELDS), n) or ds.shape != (K, len(SCALAR_FIELDS), n):
-            raise ValueError("directions: (K, nl, 21, n) and (K, 9, n)")
+            raise ValueError(f"directions: (K, {nl}, {len(LAYER_FIELDS)}, {n}) and "
+                             f"(K, {len(SCALAR_FIELDS)}, {n})")
         keep["dlayers"], keep["dscalars"] = t(dl), t(ds)
     pt = torch.empty((P, 4, n), dtype=torch.float64, device=dev)
     seg = torch.empty((nseg, 22, n), dtype=torch.float64, device=dev)
@@ -227,7 +271,8 @@ def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev):
     ptr = (lambda x: x.data_ptr() if x is not None else None)
     stream = torch.cuda.current_stream(dev).cuda_stream
     _lib.check(lib.ddh_build(ctypes.byref(s), keep["layers"].data_ptr(),
-                             keep["scalars"].data_ptr(), K, ptr(keep.get("dlayers")),
+                             keep["scalars"].data_ptr(), ptr(keep.get("gen")), gen_m, K,
+                             ptr(keep.get("dlayers")),
                              ptr(keep.get("dscalars")), pt.data_ptr(), seg.data_ptr(),
                              sc.data_ptr(), imu.data_ptr(),
                              *(ptr(tan[k]) if tan else None for k in ("pt", "seg", "sc")),
@@ -250,4 +295,4 @@ def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev):
 
 
 __all__ = ["LAYER_FIELDS", "SCALAR_FIELDS", "build", "build_graded", "check_buildable",
-           "number_arrays", "segment_ids", "structure_arrays"]
+           "number_arrays", "profile_arrays", "segment_ids", "structure_arrays"]

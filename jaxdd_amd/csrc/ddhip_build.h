@@ -5,7 +5,8 @@
 // of central differences of the host builder.
 //
 // The path's subset of the physics: Boltzmann carriers, each layer's uniform generation
-// (`G_ehp`, the reference's generation mode "none": every interior point of the layer), ohmic
+// (`G_ehp`, the reference's generation mode "none": every interior point of the layer) or a
+// generation profile (a table of G over the position, stretched to the stack), ohmic
 // contacts at the given work functions, bulk traps of one level, mobile ions' masks, totals and
 // fixed densities, direct recombination by each layer's k_direct or by Langevin's rate. What is discrete (the grid's layout, each point's layer and segment, which
 // layers generate light, hold traps or ions) is the structure's and has no derivative; the
@@ -48,13 +49,16 @@ enum BuildLayer {
     BL_NT, BL_CN, BL_CP, BL_ET, BL_G, BL_NANION, BL_NCATION, BL_MUA, BL_MUC, BL_PRELANGEVIN, BL_N
 };
 // Per-device numbers [BS_N][n].
-enum BuildScalar { BS_T, BS_WL, BS_WR, BS_GFRAC, BS_VPRE, BS_GSH, BS_PRE, BS_FIX, BS_ELEC, BS_N };
+// BS_GENPROF: 1 for a device whose generation is its profile (`gen`), 0 for each layer's G_ehp.
+enum BuildScalar {
+    BS_T, BS_WL, BS_WR, BS_GFRAC, BS_VPRE, BS_GSH, BS_PRE, BS_FIX, BS_ELEC, BS_GENPROF, BS_N
+};
 // Per-layer flags [nl][n] (bits): bulk traps, mobile and fixed anions and cations, Langevin's
 // rate for direct recombination, and that rate at the mobilities of the interval after each
 // point (SIMsalabim's) rather than the layer's.
 enum BuildFlag {
     BF_TRAPS = 1, BF_AMOBILE = 2, BF_AFIXED = 4, BF_CMOBILE = 8, BF_CFIXED = 16,
-    BF_LANGEVIN = 32, BF_LANGEVIN_INTERVAL = 64
+    BF_LANGEVIN = 32, BF_LANGEVIN_INTERVAL = 64, BF_LAYERGEN = 128
 };
 
 // The packed arrays' fields (batch.py SEG_FIELDS, SC_FIELDS, the points' PT fields).
@@ -78,6 +82,8 @@ struct BuildArgs {
     const double *unit_sum;   // [nl][n] its sum over each layer
     const double *layers;     // [nl][BL_N][n]
     const double *scalars;    // [BS_N][n]
+    const double *gen;        // [2][gen_m][n] each device's generation profile: positions, G
+    int gen_m;                // its nodes (0: no device takes a profile)
     const double *dlayers;    // [K][nl][BL_N][n] the directions (K = 0: none)
     const double *dscalars;   // [K][BS_N][n]
     double *pt, *seg, *sc, *imu;     // [NP + 2][4][n], [nseg][22][n], [20][n], [nseg][2][n]
@@ -222,11 +228,32 @@ struct BuildPoint {
 
 struct BuildRun {
     BD x, total, Vt, hG, hsum;  // the running position, sums for the current scale
+    BD gscale;                  // the profile's stretch to the stack: total / its last position
+    int gk;                     // the profile's interval at the running position
     BD Ncmax;
     int seg_prev;
     BD last[BG_N];              // the device's last segment (its padding)
     BD last_mua, last_muc;
 };
+
+// The generation profile at the running position (device.py:804-814 `_generation`): the
+// profile's positions a_k stretched to the stack; the interval k = #{a_1..a_{m-1} <= x}
+// (walked forward as the points are emitted in order), its value
+// G_k + (x - a_k)(G_{k+1} - G_k) / (a_{k+1} - a_k), extrapolated before a_1; none at or beyond
+// the last position. A shorter table padded by repeating its last node takes the same values.
+DDH_BFN BD bprofile(const BuildArgs &a, const BuildLane &L, BuildRun &R) {
+    const int m = a.gen_m;
+    const size_t n = L.n;
+    const double *gx = a.gen + L.d, *gg = a.gen + (size_t)m * n + L.d;
+    auto axis = [&](int k) { return gx[(size_t)k * n] * R.gscale; };
+    const BD x = R.x;
+    if (!(x.v < axis(m - 1).v)) return bd(0.0);
+    while (R.gk + 1 < m - 1 && axis(R.gk + 1).v <= x.v) ++R.gk;
+    const int k = R.gk;
+    const BD ak = axis(k), ak1 = axis(k + 1);
+    const double Gk = gg[(size_t)k * n], Gk1 = gg[(size_t)(k + 1) * n];
+    return bd(Gk) + (x - ak) * bd(Gk1 - Gk) / (ak1 - ak);
+}
 
 DDH_BFN void bemit(const BuildArgs &a, const BuildLane &L, const BuildPoint &p, BuildRun &R) {
     const int P = a.NP + 2, j = p.j, l = p.l;
@@ -235,7 +262,16 @@ DDH_BFN void bemit(const BuildArgs &a, const BuildLane &L, const BuildPoint &p, 
     const bool interior = j > 0 && j < P - 1;
     bput(L, a.pt, a.dpt, ((size_t)j * BP_N + BP_H) * L.n + L.d, pslice, p.h);
     bput(L, a.pt, a.dpt, ((size_t)j * BP_N + BP_X) * L.n + L.d, pslice, R.x);
-    const BD G = interior ? blayer(a, L, l, BL_G) : bd(0.0);
+    BD G = bd(0.0);
+    if (interior) {
+        if (a.scalars[(size_t)BS_GENPROF * L.n + L.d] > 0) {
+            // a profile asked for without one handed over: NaN, not a read past `gen`
+            if (a.gen_m < 2) G = bd(__builtin_nan(""));
+            else if (fl & BF_LAYERGEN) G = bprofile(a, L, R);
+        } else {
+            G = blayer(a, L, l, BL_G);
+        }
+    }
     bput(L, a.pt, a.dpt, ((size_t)j * BP_N + BP_G) * L.n + L.d, pslice, G);
     const BD af = (fl & BF_AFIXED) ? blayer(a, L, l, BL_NANION) : bd(0.0);
     const BD cf = (fl & BF_CFIXED) ? blayer(a, L, l, BL_NCATION) : bd(0.0);
@@ -270,6 +306,8 @@ DDH_BFN void build_lane(const BuildArgs &a, int d, int k) {
     R.total = bd(0.0);
     for (int l = 0; l < nl; ++l) R.total = R.total + blayer(a, L, l, BL_L);
     R.x = bd(0.0); R.hG = bd(0.0); R.hsum = bd(0.0);
+    R.gk = 0;
+    R.gscale = a.gen_m > 0 ? R.total / a.gen[(size_t)(a.gen_m - 1) * L.n + d] : bd(0.0);
     R.Ncmax = blayer(a, L, 0, BL_NC);
     R.seg_prev = -1;
     // each layer's scale of its unscaled spacing: L_l / (total unit_sum_l)

@@ -944,12 +944,14 @@ extern "C" int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double
 }
 
 extern "C" int ddh_build(const ddh_structure *st, const double *layers, const double *scalars,
-                         int32_t K, const double *dlayers, const double *dscalars, double *pt,
-                         double *seg, double *sc, double *ion_mu, double *dpt, double *dseg,
-                         double *dsc, void *stream) {
+                         const double *gen, int32_t gen_m, int32_t K, const double *dlayers,
+                         const double *dscalars, double *pt, double *seg, double *sc,
+                         double *ion_mu, double *dpt, double *dseg, double *dsc, void *stream) {
     if (!st || !layers || !scalars || !pt || !seg || !sc || !ion_mu || !st->lid || !st->sid ||
         !st->flags || !st->i0 || !st->unit || !st->unit_sum)
         return fail(DDH_E_ARG, "ddh_build: NULL argument");
+    if (gen_m < 0 || gen_m == 1 || (gen_m > 0 && !gen))
+        return fail(DDH_E_ARG, "ddh_build: gen_m 0 (no profile) or >= 2 nodes with gen");
     if (K < 0 || K > 1024 || (K > 0 && (!dlayers || !dscalars || !dpt || !dseg || !dsc)))
         return fail(DDH_E_ARG, "ddh_build: 0 <= K <= 1024, and the directions' arrays when K > 0");
     if (st->n <= 0 || st->NP < 2 || st->NP > 65536 || st->nl < 1 || st->nl > ddh::BUILD_MAX_LAYERS ||
@@ -960,6 +962,7 @@ extern "C" int ddh_build(const ddh_structure *st, const double *layers, const do
     a.lid = st->lid; a.sid = st->sid; a.flags = st->flags; a.i0 = st->i0;
     a.unit = st->unit; a.unit_sum = st->unit_sum;
     a.layers = layers; a.scalars = scalars; a.dlayers = dlayers; a.dscalars = dscalars;
+    a.gen = gen; a.gen_m = gen_m;
     a.pt = pt; a.seg = seg; a.sc = sc; a.imu = ion_mu; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
     const long lanes = (long)st->n * (K > 0 ? K : 1);
     hipLaunchKernelGGL(ddh::build_kernel, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0,

@@ -300,15 +300,19 @@ class _LayerValues:
 # ---- stage 3: the arrays on the grid's points ---------------------------------------------
 def _generation(st: Structure, p: dict, x: np.ndarray, total: float) -> np.ndarray:
     """Generation rate per point at G_frac = 1: each layer's uniform ``G_ehp``, or the profile
-    file's rate interpolated linearly on its axis stretched to the stack (zero beyond its end
-    and in layers that do not generate); none at the contacts."""
+    file's rate interpolated linearly on its axis stretched to the stack, each point in the
+    interval k = #{a_1..a_{m-1} <= x} (extrapolated before a_1), zero at or beyond the axis's
+    end and in layers that do not generate; none at the contacts (reference:
+    device.py:804-814)."""
     if st.gen_mode == "none":
         G = np.asarray(p["G_ehp"], dtype=float)[st.lid]
     else:
-        axis = np.asarray(p["gen_x"], dtype=float)
-        axis = axis * (total / axis[-1])
-        G = np.interp(x, axis, np.asarray(p["gen_G"], dtype=float), right=0.0)
-        G = np.where(st.layer_gen[st.lid] & (x < axis[-1]), G, 0.0)
+        gx, gG = (np.asarray(p[k], dtype=float) for k in ("gen_x", "gen_G"))
+        axis = gx * (total / gx[-1])
+        j = np.searchsorted(axis[1:], x, side="right")
+        k = np.minimum(j, len(axis) - 2)
+        G = gG[k] + (x - axis[k]) * (gG[k + 1] - gG[k]) / (axis[k + 1] - axis[k])
+        G = np.where(st.layer_gen[st.lid] & (j < len(axis) - 1), G, 0.0)
     G = G.copy()
     G[0] = G[-1] = 0.0
     return G

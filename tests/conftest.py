@@ -40,6 +40,23 @@ def fixture_device(case: str):
     return build_device(read_setup(GOLDEN / case / "simulation_setup.txt"), grid="simss")
 
 
+def fixture_inputs(case: str):
+    """(structure, numbers) of a SimSS fixture's device on SIMsalabim's grid."""
+    from jaxdd_amd.device import device_parameters
+    from jaxdd_amd.simss_input import read_setup
+
+    return device_parameters(read_setup(GOLDEN / case / "simulation_setup.txt"), grid="simss")
+
+
+#: The SimSS runs of one stack (3 layers, 400 points), eight of them generating by the
+#: profile file, one by each layer's G_ehp.
+SIMSS_RUNS = ("frozen_1e17_x1_thin_light", "frozen_thick_htl_dark", "held_thin_light",
+              "langevin_light", "logv_light", "none_light", "resist_light",
+              "steady_1e17_thin_light", "thin_l1_light")
+#: Numbers the profile's stretch and scale depend on: the thicknesses and G_frac.
+PROFILE_WRT = (("L", 0), ("L", 1), ("L", 2), "G_frac")
+
+
 def fixture_jv(case: str):
     return np.genfromtxt(GOLDEN / case / "JV.dat", names=True)
 

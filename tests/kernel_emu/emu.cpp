@@ -441,13 +441,15 @@ extern "C" int emu_transient(int n, int NP, int nseg, int nt, const double *pt, 
 extern "C" int emu_build(int n, int NP, int nl, int nseg, int K, const uint8_t *lid,
                          const uint8_t *sid, const uint8_t *flags, const int32_t *i0,
                          const double *unit, const double *unit_sum, const double *layers,
-                         const double *scalars, const double *dlayers, const double *dscalars,
+                         const double *scalars, const double *gen, int gen_m,
+                         const double *dlayers, const double *dscalars,
                          double *pt, double *seg, double *sc, double *imu, double *dpt,
                          double *dseg, double *dsc) {
     BuildArgs a = {};
     a.n = n; a.NP = NP; a.nl = nl; a.nseg = nseg; a.K = K;
     a.lid = lid; a.sid = sid; a.flags = flags; a.i0 = i0; a.unit = unit; a.unit_sum = unit_sum;
     a.layers = layers; a.scalars = scalars; a.dlayers = dlayers; a.dscalars = dscalars;
+    a.gen = gen; a.gen_m = gen_m;
     a.pt = pt; a.seg = seg; a.sc = sc; a.imu = imu; a.dpt = dpt; a.dseg = dseg; a.dsc = dsc;
     for (int k = (K > 0 ? 0 : -1); k < (K > 0 ? K : 0); ++k)
         for (int d = 0; d < n; ++d) build_lane(a, d, k);

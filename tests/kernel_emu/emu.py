@@ -42,7 +42,7 @@ def lib():
         _lib.emu_assemble_ions.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                            vp, vp]
         _lib.emu_xcd_hits.argtypes = [i, i, vp]
-        _lib.emu_build.argtypes = [i, i, i, i, i] + [vp] * 17
+        _lib.emu_build.argtypes = [i, i, i, i, i] + [vp] * 8 + [vp, i] + [vp] * 9
         _lib.emu_graded_layout.argtypes = [i, i, i, vp, ctypes.c_double, ctypes.c_double] + [vp] * 7
         _lib.emu_transient.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        ctypes.c_double, i, i, i, i, i, vp, vp, vp, vp, vp, vp]
@@ -220,6 +220,8 @@ def build_packed(structures, params, directions=None):
 
     sa = B.structure_arrays(structures)
     layers, scalars = B.number_arrays(structures, params)
+    gen = B.profile_arrays(structures, params)
+    gen_m = 0 if gen is None else gen.shape[1]
     n, NP, nl, nseg = len(structures), sa["NP"], sa["nl"], sa["nseg"]
     P = NP + 2
     K = 0 if directions is None else np.asarray(directions[0]).shape[0]
@@ -231,7 +233,7 @@ def build_packed(structures, params, directions=None):
     ds = np.ascontiguousarray(directions[1], dtype=float) if K else np.zeros(1)
     lib().emu_build(n, NP, nl, nseg, K, _p(sa["lid"]), _p(sa["sid"]), _p(sa["flags"]),
                     _p(sa["i0"]), _p(sa["unit"]), _p(sa["unit_sum"]), _p(layers), _p(scalars),
-                    _p(dl), _p(ds), _p(pt), _p(seg), _p(sc), _p(imu), _p(dpt), _p(dseg), _p(dsc))
+                    _p(gen) if gen_m else None, gen_m, _p(dl), _p(ds), _p(pt), _p(seg), _p(sc), _p(imu), _p(dpt), _p(dseg), _p(dsc))
     out = {"pt": pt, "seg": seg, "sc": sc, "imu": imu, "sid": sa["sid"], "nseg": nseg}
     if K:
         return out, {"pt": dpt, "seg": dseg, "sc": dsc}

@@ -88,6 +88,55 @@ def test_replay_builds_langevin_devices(emu, mode):
                 assert np.all(err <= 1e-8 * scale[:fd.shape[0]] + 1e-300), (key, w, d)
 
 
+def test_replay_builds_the_simss_runs_devices(emu):
+    """The nine SimSS runs' devices in one batch, eight generating by the profile file
+    (reference: src/jaxdd/device.py:804-814 `_generation`, its axis stretched to the stack),
+    one by each layer's G_ehp: the builder's lanes give the host builder's packed arrays (1e-14
+    of each field's scale), the profile's G at every point to 1e-14 of itself, and the
+    tangents in the thicknesses and G_frac the host builder's central differences (relative
+    step 1e-6; 1e-8 of each field's scale per unit of ln x). G has a kink at each of the
+    profile's nodes, where grid points often lie: there the reference's derivative is the
+    slope of the interval the point belongs to (the one after the node), a one-sided
+    difference (second order: steps h and 2h), which G's tangent, and that of the current scale
+    (a sum over G), match instead. G_frac 0 (a dark run) takes the absolute step 1e-6."""
+    from conftest import PROFILE_WRT, SIMSS_RUNS, bumped, fixture_inputs, number_of
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import SC_FIELDS, pack
+    from jaxdd_amd.device import device_arrays
+
+    inputs = [fixture_inputs(c) for c in SIMSS_RUNS]
+    structs, params = zip(*inputs)
+    assert sum(st.gen_mode == "file" for st in structs) == 8
+    ref = pack([device_arrays(st, p) for st, p in inputs])
+    dirs = A.directions_for(PROFILE_WRT, len(inputs), 3)
+    got, tan = emu.build_packed(list(structs), list(params), dirs)
+    _same_packed(got, ref)
+    G, Gr = got["pt"][:, 2, :], ref["pt"][:, 2, :]
+    assert np.all(np.abs(G - Gr) <= 1e-14 * np.abs(Gr))
+    assert np.all(G[1:-1, SIMSS_RUNS.index("logv_light")] >= 0)
+    for key in ("pt", "seg", "sc"):
+        axis = tuple(i for i in range(ref[key].ndim) if i != ref[key].ndim - 2)
+        scale = np.max(np.abs(ref[key]), axis=axis, keepdims=True)[..., 0]
+        for k, w in enumerate(PROFILE_WRT):
+            for d, (st, p) in enumerate(inputs):
+                x = number_of(p, w)
+                unit = abs(x) if x else 1.0
+                h = 1e-6 * unit
+                f = {s: pack([device_arrays(st, bumped(p, w, s * h))])[key][..., 0]
+                     for s in (2.0, 1.0, -1.0, -2.0)}
+                fd = (f[1.0] - f[-1.0]) / (2 * h)
+                g = tan[key][k][..., d][:fd.shape[0]]
+                err = np.abs(g - fd) * unit
+                if key != "seg":
+                    f0 = ref[key][..., d][:fd.shape[0]]
+                    right = (4 * f[1.0] - f[2.0] - 3 * f0) / (2 * h)
+                    left = (3 * f0 - 4 * f[-1.0] + f[-2.0]) / (2 * h)
+                    one_sided = np.minimum(np.abs(g - right), np.abs(g - left)) * unit
+                    kinked = (np.s_[:, 2] if key == "pt" else SC_FIELDS.index("J_scale"))
+                    err[kinked] = np.minimum(err[kinked], one_sided[kinked])
+                assert np.all(err <= 1e-8 * scale[:fd.shape[0]] + 1e-300), (key, w, SIMSS_RUNS[d])
+
+
 def test_replay_tangents_are_the_host_builders_differences(emu):
     """The forward-mode tangents of the 16 parameters against `adjoint.tangents`' central
     differences (relative step 1e-6) of the host builder on each device's structure, in the
@@ -160,14 +209,14 @@ def test_vectorised_numbers_are_the_devices_numbers():
 
 
 def test_the_builder_refuses_what_it_does_not_build():
-    """A generation profile, flat-band contacts, optional modules: before any launch."""
+    """Computed generation, flat-band contacts, optional modules: before any launch."""
     import dataclasses
 
     from jaxdd_amd import builder as B
 
     st, _ = _rows(1, 40)[0]
     B.check_buildable(st)
-    for change, match in ((dict(gen_mode="file"), "profile"), (dict(sfb=(True, False)), "flat-band"),
+    for change, match in ((dict(gen_mode="calc"), "generation"), (dict(sfb=(True, False)), "flat-band"),
                           (dict(extra={"langevin": True}), "optional")):
         with pytest.raises(NotImplementedError, match=match):
             B.check_buildable(dataclasses.replace(st, **change))
@@ -399,6 +448,51 @@ def test_gpu_langevin(torch_cuda, oracle, emu, mode):
     print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
     assert np.max(err) < 1e-5, err
     assert np.median(err) < 3e-7, err
+
+
+@pytest.mark.gpu
+def test_gpu_simss_runs_built_with_their_profiles(torch_cuda, oracle, emu):
+    """ddh_build of the nine SimSS runs' devices in one batch (eight generating by the profile
+    file, one by each layer's G_ehp): the packed arrays against the host builder's to 1e-14 and
+    the tangents in the thicknesses and G_frac against the CPU replay's to 1e-13; then
+    `param_gradients` of three of them against central differences of the oracle's curves
+    (relative step 1e-5), to 1e-7 of each curve's scale (measured: 3e-9 at most, 3e-10 at the
+    median). The curves' derivatives in the thicknesses have kinks where a grid point crosses
+    one of the profile's nodes (several points lie on nodes), too small in the curves to show
+    at these bounds."""
+    from conftest import PROFILE_WRT, SIMSS_RUNS, fixture_inputs, named_fd_errors
+    from jaxdd_amd import adjoint as A
+    from jaxdd_amd.batch import pack
+    from jaxdd_amd.builder import build
+    from jaxdd_amd.device import device_arrays
+
+    inputs = [fixture_inputs(c) for c in SIMSS_RUNS]
+    structs, params = zip(*inputs)
+    print(f"ddh_build: the {len(inputs)} SimSS runs' devices, {len(PROFILE_WRT)} directions",
+          flush=True)
+    dirs = A.directions_for(PROFILE_WRT, len(inputs), 3)
+    batch, tan = build(list(structs), list(params), dirs)
+    got = {"pt": _np(batch["_pt"]), "seg": _np(batch["_seg"]), "sc": _np(batch["_sc"]),
+           "imu": _np(batch["_imu"]), "sid": _np(batch["_sid"]), "nseg": batch["nseg"]}
+    _same_packed(got, pack([device_arrays(st, p) for st, p in inputs]))
+    _, rtan = emu.build_packed(list(structs), list(params), dirs)
+    for key in ("pt", "seg", "sc"):
+        g, r = _np(tan[key]), rtan[key]
+        assert np.max(np.abs(g - r)) <= 1e-13 * (np.max(np.abs(r)) + 1e-300), key
+
+    runs = ("held_thin_light", "steady_1e17_thin_light", "langevin_light")
+    sub = [inputs[SIMSS_RUNS.index(c)] for c in runs]
+    structs, params = zip(*sub)
+    V = V_FULL[::10].copy()
+    print(f"param_gradients: {runs}", flush=True)
+    J, ok, _, g = A.param_gradients(structs, params, PROFILE_WRT, V, tol=1e-12, max_iter=200)
+    J, ok, g = _np(J), _np(ok), _np(g)
+    assert ok.all() and np.isfinite(g).all()
+    err = named_fd_errors(oracle, structs, params, PROFILE_WRT, V, J, g)
+    print(f"GPU vs differences: max {err.max():.2e} median {np.median(err):.2e}\n{err}",
+          flush=True)
+    assert np.max(err) < 1e-7, err
+    assert np.median(err) < 1e-8, err
 
 
 @pytest.mark.gpu
