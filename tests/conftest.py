@@ -26,7 +26,22 @@ FIXTURES = {
     "held_thin_light": 1e-9,
     "frozen_thick_htl_dark": 2e-5,
     "resist_light": 1e-5,
+    # six voltages 0.1 mV apart around open circuit (SimSS at tolPois 1e-12): the curve's
+    # largest |J| is 1.1 A/m^2, 4e-3 of the photocurrent; 3e-7 of it is 1e-9 of the latter
+    "thin_l1_voc_light": 3e-7,
 }
+
+
+def open_circuit_voltage(V, J):
+    """The open-circuit voltage of a curve sampled around it: the root, inside the samples, of
+    the cubic least-squares fit of J(V) (the reference's reading of the Voc run,
+    tests/test_solver.py:750-764)."""
+    m = np.mean(V)
+    r = np.roots(np.polyfit(np.asarray(V) - m, J, 3))
+    r = r[np.abs(r.imag) == 0].real + m
+    r = r[(r > np.min(V)) & (r < np.max(V))]
+    assert r.size == 1, r
+    return float(r[0])
 
 
 def pytest_configure(config):

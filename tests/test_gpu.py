@@ -138,6 +138,11 @@ def test_solve_jv_reproduces_simss(torch_cuda, case, precision):
     assert np.max(np.abs(res.J - ref["Jext"])) < FIXTURES[case] * scale
     if case == "resist_light":
         assert np.max(np.abs(res.V_ext - ref["Vext"])) < 1e-4
+    if case == "thin_l1_voc_light" and precision == "fp64":
+        from conftest import open_circuit_voltage
+
+        assert abs(open_circuit_voltage(ref["Vext"], res.J)
+                   - open_circuit_voltage(ref["Vext"], ref["Jext"])) < 1e-10
 
 
 def test_zimt_steady_states_through_the_steady_solver(torch_cuda, oracle):

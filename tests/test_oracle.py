@@ -159,6 +159,20 @@ def test_oracle_batch_reproduces_simss(oracle, case):
     assert np.max(np.abs(J[0] - ref["Jext"])) < FIXTURES[case] * scale
 
 
+def test_open_circuit_voltage_is_simss(oracle):
+    """SimSS's run at 0.1 mV steps around open circuit: the oracle's curve to 2e-11 A/m^2
+    (Jphoto 257 A/m^2), and its Voc (the cubic through the six points) SimSS's to 1e-10 V,
+    the reference's bar for its own open circuit."""
+    from conftest import fixture_device, fixture_jv, open_circuit_voltage
+
+    dev, ref = fixture_device("thin_l1_voc_light"), fixture_jv("thin_l1_voc_light")
+    J, ok, _ = oracle.solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    assert ok.all()
+    assert np.max(np.abs(J - ref["Jext"])) < 5e-11
+    assert abs(open_circuit_voltage(ref["Vext"], J)
+               - open_circuit_voltage(ref["Vext"], ref["Jext"])) < 1e-10
+
+
 def test_langevin_recombination_reproduces_simss(oracle):
     """SimSS's Langevin recombination in all three layers (`useLangevin`, `preLangevin` 1e-3,
     1e-2, 1e-3; reference tests/test_solver.py:1083-1101): at SIMsalabim's mobilities (those of
