@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define DDH_ABI_VERSION 4
+#define DDH_ABI_VERSION 5
 
 /* Segments (runs of points of one material) per device: the solve keeps each wavefront's
  * segment tables in the compute unit's 160 KB local data share. */
@@ -39,7 +39,11 @@ enum ddh_precision { DDH_FP64 = 0, DDH_MIXED = 1 };
 
 /* A packed batch of n devices on NP interior points each, lane-innermost
  * (jaxdd_amd/csrc/ddhip_layout.h documents every field). Built by jaxdd_amd.batch.prepare,
- * the counterpart of src/jaxdd/batch.py:151-223 (`prepare`). */
+ * the counterpart of src/jaxdd/batch.py:151-223 (`prepare`). A segment's 23rd field is its bulk
+ * traps' charge type (1 donor, -1 acceptor, 0 neutral or none; device.py:940-948): their charge
+ * enters the Poisson rows of ddh_solve_batch, ddh_solve_batch_states and ddh_assemble (the
+ * start then solves first with them neutral, batch.py:898-900). ddh_adjoint, ddh_adjoint_vjp,
+ * ddh_impedance, ddh_impedance_ions and ddh_transient take no charged traps (the field 0). */
 typedef struct {
     int32_t n;        /* devices */
     int32_t NP;       /* interior grid points */
@@ -47,7 +51,7 @@ typedef struct {
     int32_t reserved;
     const double *pt;   /* [NP + 2][4][n] */
     const uint8_t *sid; /* [NP + 2][n] */
-    const double *seg;  /* [nseg][22][n] */
+    const double *seg;  /* [nseg][23][n], the last field the bulk traps' charge type */
     const double *sc;   /* [20][n] */
 } ddh_batch;
 
@@ -126,7 +130,7 @@ size_t ddh_adjoint_workspace_bytes(int32_t n, int32_t NP, int32_t nV);
  * its potential in closed form), and ions frozen at a pre-bias (batch.py:937-960: the scan's
  * fixed charge is the pre-bias solution's ions, differentiated through that solution by a
  * second transposed bordered solve at it; pre_states from ddh_solve_batch_states). theta reaches the kernel as the tangents of the packed batch
- * arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][22][n], dsc [ntheta][20][n]
+ * arrays: dpt [ntheta][NP + 2][4][n], dseg [ntheta][nseg][23][n], dsc [ntheta][20][n]
  * (d array / d theta_k; every field the steady rows, the current and the ions' constraints
  * read, V_T and the generation's factor included). states and ok: from ddh_solve_batch_states
  * on the same batch and ascending V. grad: [n][nV][ntheta], of the current without the shunt
@@ -237,11 +241,11 @@ int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L, double
  * shorter table padded by repeating its last node (device.py:804-814 `_generation`: NumPy's
  * interp, 0 at and beyond the last position, in generating layers only); gen_m 0 when no device
  * takes a profile, else >= 2. Outputs the ddh_batch arrays pt [NP + 2][4][n]
- * (h, x, G, fixed anions - cations), seg [nseg][22][n] (each segment's fields, a device's
+ * (h, x, G, fixed anions - cations), seg [nseg][23][n] (each segment's fields, a device's
  * segments beyond its own count repeating its last), sc [20][n], and ion_mu [nseg][2][n] (each
  * segment's anion and cation mobility). With K > 0 directions dlayers [K][nl][22][n] and
  * dscalars [K][10][n] (each a direction in the numbers), also their derivatives along each,
- * forward mode: dpt [K][NP + 2][4][n], dseg [K][nseg][22][n], dsc [K][20][n], the tangents
+ * forward mode: dpt [K][NP + 2][4][n], dseg [K][nseg][23][n], dsc [K][20][n], the tangents
  * ddh_adjoint takes; the structure is held (the thicknesses scale each layer's spacing and
  * stretch the profile), as the reference differentiates `device_tree(structure, params)`
  * (batch.py:142-148). */

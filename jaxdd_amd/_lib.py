@@ -118,8 +118,8 @@ def load():
                               vp, vp, vp, vp, vp, vp]
     lib.ddh_block_sweep.restype = ctypes.c_int
     lib.ddh_block_sweep.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]
-    if lib.ddh_abi_version() != 4:
-        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 4")
+    if lib.ddh_abi_version() != 5:
+        raise RuntimeError(f"{LIB}: ABI version {lib.ddh_abi_version()}, expected 5")
     _lib = lib
     return lib
 

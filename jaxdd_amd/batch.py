@@ -27,7 +27,7 @@ __all__ = ["external_voltages", "from_packed", "merge", "pack", "prepare", "solv
 # Field orders of jaxdd_amd/csrc/ddhip_layout.h.
 PT_FIELDS = ("h", "x", "G", "fix")
 SEG_FIELDS = ("eps", "offn", "offp", "Nc", "dop", "kd", "ni2", "Nt", "Cn", "Cp", "nt0", "pt0",
-              "am", "cm", "mun", "mup", "Ec", "ni", "sP", "sCs", "imun", "imup")
+              "am", "cm", "mun", "mup", "Ec", "ni", "sP", "sCs", "imun", "imup", "tt")
 SC_FIELDS = ("L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_right",
              "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
              "G_shunt", "n_floor", "pre_cond", "fix_ions")
@@ -36,9 +36,10 @@ MAX_SEGMENTS = 14  # DDH_MAX_SEGMENTS of include/ddhip.h
 
 
 def _point_fields(dev: Device) -> np.ndarray:
-    """(NP + 2, 22) material parameters per point, in `SEG_FIELDS` order: the device's own
+    """(NP + 2, 23) material parameters per point, in `SEG_FIELDS` order: the device's own
     and, derived from them, the row scales (src/jaxdd/solver.py:245-248, the continuity
-    scale without its 1/h) and the inverse interval mobilities."""
+    scale without its 1/h), the inverse interval mobilities and the bulk traps' charge type
+    (1 donor, -1 acceptor, 0 neutral or none)."""
     with np.errstate(divide="ignore"):
         return np.stack([
             dev.eps, dev.offset_n, dev.offset_p, dev.N_c, dev.N_A - dev.N_D, dev.k_direct,
@@ -47,6 +48,7 @@ def _point_fields(dev: Device) -> np.ndarray:
             dev.mu_n, dev.mu_p, dev.E_c, dev.ni,
             1.0 / (Q * dev.N_c), 1.0 / (np.maximum(dev.mu_n, dev.mu_p) * dev.Vt * dev.N_c),
             1.0 / dev.mu_n, 1.0 / dev.mu_p,
+            np.zeros(len(dev.x)) if dev.srh_type is None else dev.srh_type,
         ], axis=-1).astype(np.float64)
 
 
@@ -66,6 +68,23 @@ def _scalars(dev: Device) -> list[float]:
     ]
 
 
+def charged_traps(batch) -> bool:
+    """Whether any device of a batch (`prepare`'s dict, or a list of devices) holds charged
+    bulk traps."""
+    if isinstance(batch, dict):
+        seg = batch.get("_seg")
+        return seg is not None and bool((seg[:, SEG_FIELDS.index("tt"), :] != 0).any())
+    return any(d.srh_type is not None and np.any(d.srh_type != 0) for d in batch)
+
+
+def refuse_charged_traps(batch, what: str) -> None:
+    """`NotImplementedError` where a batch holds charged bulk traps and ``what`` does not take
+    them (their charge enters the steady solve's Poisson rows only)."""
+    if charged_traps(batch):
+        raise NotImplementedError(f"{what}: charged bulk traps (bulkTrapType 1 or -1) are "
+                                  "solved in the steady state only")
+
+
 def _check(devices) -> None:
     for d in devices:
         if d.statistics != "boltzmann" or any(d.free) or d.curr_diff:
@@ -81,7 +100,7 @@ def pack(devices: list[Device]) -> dict[str, np.ndarray]:
     _check(devices)
     n = len(devices)
     P = next(iter(sizes))
-    fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 22)
+    fields = np.stack([_point_fields(d) for d in devices])  # (n, P, 23)
     # the mobile ions' mobilities (read by the transient only) also split segments
     ion_mu = np.stack([np.stack([np.zeros(P) if d.mu_anion is None else d.mu_anion,
                                  np.zeros(P) if d.mu_cation is None else d.mu_cation], -1)

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batch import MAX_SEGMENTS, SC_FIELDS
+from .batch import MAX_SEGMENTS, SC_FIELDS, SEG_FIELDS
 from .device import Structure
 from .grid import GRADED_POINTS_PER_EFOLD, GRADED_SMALLEST
 
@@ -46,6 +46,8 @@ def check_buildable(st: Structure) -> None:
                                   "each layer's G_ehp or a profile")
     if any(st.sfb):
         raise NotImplementedError("flat-band work functions are outside the GPU builder")
+    if st.trap_type is not None and np.any(st.traps & (st.trap_type != 0)):
+        raise NotImplementedError("charged bulk traps are outside the GPU builder")
     if st.extra:
         raise NotImplementedError(f"optional modules outside the GPU builder: {sorted(st.extra)}")
     if len(st.i0) > MAX_LAYERS:
@@ -257,13 +259,14 @@ def _build(keep, NP, nl, nseg, layers, scalars, directions, R_series, dev, profi
                              f"(K, {len(SCALAR_FIELDS)}, {n})")
         keep["dlayers"], keep["dscalars"] = t(dl), t(ds)
     pt = torch.empty((P, 4, n), dtype=torch.float64, device=dev)
-    seg = torch.empty((nseg, 22, n), dtype=torch.float64, device=dev)
+    seg = torch.empty((nseg, len(SEG_FIELDS), n), dtype=torch.float64, device=dev)
     sc = torch.empty((len(SC_FIELDS), n), dtype=torch.float64, device=dev)
     imu = torch.empty((nseg, 2, n), dtype=torch.float64, device=dev)
     tan = None
     if K:
         tan = {"pt": torch.empty((K, P, 4, n), dtype=torch.float64, device=dev),
-               "seg": torch.empty((K, nseg, 22, n), dtype=torch.float64, device=dev),
+               "seg": torch.empty((K, nseg, len(SEG_FIELDS), n), dtype=torch.float64,
+                                  device=dev),
                "sc": torch.empty((K, len(SC_FIELDS), n), dtype=torch.float64, device=dev)}
     s = _lib.Structure(n, NP, nl, nseg, keep["lid"].data_ptr(), keep["sid"].data_ptr(),
                        keep["flags"].data_ptr(), keep["i0"].data_ptr(), keep["unit"].data_ptr(),

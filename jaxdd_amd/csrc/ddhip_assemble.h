@@ -108,6 +108,10 @@ DDH_FN void assemble_lane(const AsmArgs &a, int d) {
         g.meps = KSEG(mid.sid, SEG_EPS);
         g.ceps = KSEG(cur.sid, SEG_EPS);
         g.sP = KSEG(mid.sid, SEG_SP); g.scs = KSEG(mid.sid, SEG_SCS); g.dop = KSEG(mid.sid, SEG_DOP);
+        const double tt = KSEG(mid.sid, SEG_TT), Nt = KSEG(mid.sid, SEG_NT);
+        if (tt != 0.0 && Nt > 0.0)  // charged bulk traps (solver.py:1136-1140)
+            g.qt = trap_charge(tt, mid.n, mid.p, Nt, KSEG(mid.sid, SEG_CN), KSEG(mid.sid, SEG_CP),
+                               KSEG(mid.sid, SEG_NT0), KSEG(mid.sid, SEG_PT0), g.dqn, g.dqp);
         double Ab[9], Bb[9], Cb[9], r[9];
         row_blocks_s<3>(g, cur, mid, rgt.V, eL, eR, false, L2, iVt, 1.0, gfr, Ab, Bb, Cb, r);
         const size_t o = (size_t)(i - 1);

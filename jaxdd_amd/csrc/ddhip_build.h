@@ -64,8 +64,10 @@ enum BuildFlag {
 // The packed arrays' fields (batch.py SEG_FIELDS, SC_FIELDS, the points' PT fields).
 enum BuildSeg {
     BG_EPS, BG_OFFN, BG_OFFP, BG_NC, BG_DOP, BG_KD, BG_NI2, BG_NT, BG_CN, BG_CP, BG_NT0,
-    BG_PT0, BG_AM, BG_CM, BG_MUN, BG_MUP, BG_EC, BG_NI, BG_SP, BG_SCS, BG_IMUN, BG_IMUP, BG_N
+    BG_PT0, BG_AM, BG_CM, BG_MUN, BG_MUP, BG_EC, BG_NI, BG_SP, BG_SCS, BG_IMUN, BG_IMUP,
+    BG_TT, BG_N  // BG_TT: the traps' charge type, 0 (the builder's traps are neutral)
 };
+static_assert((int)BG_N == (int)SEG_N, "the builder writes every packed segment field");
 enum BuildSc {
     BC_L, BC_VT, BC_GFRAC, BC_V0, BC_VL, BC_WL, BC_WR, BC_LNL, BC_LNR, BC_LPL, BC_LPR, BC_ELEC,
     BC_AT, BC_CT, BC_VPRE, BC_JS, BC_GSH, BC_NFLOOR, BC_PRE, BC_FIX, BC_N
@@ -207,6 +209,7 @@ DDH_BFN void bsegment(const BuildArgs &a, const BuildLane &L, int j, int l, BD t
     f[BG_SCS] = 1.0 / (dmax_mean(mun, mup) * Vt * Nc);
     f[BG_IMUN] = 1.0 / mun;
     f[BG_IMUP] = 1.0 / mup;
+    f[BG_TT] = bd(0.0);
 }
 
 DDH_BFN void bput_segment(const BuildArgs &a, const BuildLane &L, int s, const BD *f, BD mua,

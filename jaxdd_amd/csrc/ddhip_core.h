@@ -50,10 +50,24 @@ using std::isfinite;
 
 namespace ddh {
 
+// The charged bulk traps' charge density (m^-3) at a point and its derivatives in ln n and
+// ln p (src/jaxdd/solver.py:762-773 `_trap_charge`, one level; device.py:940-948): Nt (cwe -
+// f), cwe 1 for donors (tt 1) and 0 for acceptors (tt -1), f = (Cn n + Cp pt0) / (Cn (n + nt0)
+// + Cp (p + pt0)) the level's steady occupation.
+DDH_FN double trap_charge(double tt, double n, double p, double Nt, double Cn, double Cp,
+                          double nt0, double pt0, double &dqn, double &dqp) {
+    const double num = Cn * n + Cp * pt0, den = Cn * (n + nt0) + Cp * (p + pt0);
+    const double f = num / den, cwe = tt > 0 ? 1.0 : 0.0;
+    const double iden2 = 1.0 / (den * den);
+    dqn = -Nt * n * Cn * (Cn * nt0 + Cp * p) * iden2;
+    dqp = Nt * p * Cp * num * iden2;
+    return Nt * (cwe - f);
+}
+
 // ---------------------------------------------------------------------------------------
 // Kernel arguments and per-lane state
 // ---------------------------------------------------------------------------------------
-enum Phase { PH_F = 0, PH_STAGE, PH_PRE, PH_SCAN, PH_DONE };
+enum Phase { PH_F = 0, PH_STAGE, PH_PRE, PH_SCAN, PH_DONE, PH_NEUTRAL };
 
 struct LaneState {
     int phase, stage, j, below, it, maxit, err;
@@ -62,6 +76,8 @@ struct LaneState {
     int noise_state;  // jv mode: 0 no noise floor yet, 1 pending, 2 known
     int need64;       // mixed precision: this solve has switched to FP64 factors
     int part, parts;  // this lane's share of its device's state copies (lanes per device)
+    int charged;      // the device holds charged bulk traps
+    int neutral;      // this solve leaves their charge out (the start's first solve)
     double Vint, share, ratio, first, V_last, V_before, before_share, noise_floor;
     double lw[2], lpois[2], lf[2], lc[2], llast[2], lbefore[2], lstart[2];
     double hist[4], Jh[4];
@@ -227,9 +243,16 @@ DDH_FN void poisson_lane(const Args &a, int lane, LaneState &S) {
             double an = am * exp(logs[0] + V / Vt), ca = cm * exp(logs[1] - V / Vt);
             double F = fac_l * Vm1 + fac_m * V + fac_u * Vp -
                        Q_E * (n - p + s2.v[SEG_DOP] + FIX(i) + an - ca);
+            double dtq = 0.0;  // the charged traps' charge, the densities moving with V
+            if (s2.v[SEG_TT] != 0.0 && s2.v[SEG_NT] > 0.0) {  // (solver.py:1672-1673)
+                double dqn, dqp;
+                F = F + Q_E * trap_charge(s2.v[SEG_TT], n, p, s2.v[SEG_NT], s2.v[SEG_CN],
+                                          s2.v[SEG_CP], s2.v[SEG_NT0], s2.v[SEG_PT0], dqn, dqp);
+                dtq = Q_E * (dqn - dqp) / Vt;
+            }
             double lo = (i == 1) ? 0.0 : fac_l * sP;
             double up = (i == NP) ? 0.0 : fac_u * sP;
-            double mid = (fac_m - Q_E * (n / Vt + p / Vt + an / Vt + ca / Vt)) * sP;
+            double mid = (fac_m - Q_E * (n / Vt + p / Vt + an / Vt + ca / Vt) + dtq) * sP;
             double r0 = -F * sP, r1 = -Q_E * an * sP * am, r2 = Q_E * ca * sP * cm;
             double m = mid - up * cprev;
             double rm = 1.0 / m;
@@ -329,6 +352,10 @@ DDH_FN void poisson_lane(const Args &a, int lane, LaneState &S) {
     S.first = first;
     S.b_pois = 0;
     S.mobile0 = mobile ? 1 : 0;
+    S.charged = 0;
+    S.neutral = 0;
+    for (int q = 0; q < a.nseg; ++q)
+        if (a.seg[((size_t)q * SEG_N + SEG_TT) * a.n + lane] != 0.0) S.charged = 1;
     a.status[lane] = 0;
 }
 
@@ -372,6 +399,7 @@ struct SweepCtl {  // what one lane's sweeps need of its state
     double logs[2];     // normalisations at the state being assembled
     double lstart[2];   // the pre-bias's (for the freeze)
     int fresh, freeze_now, mobile, src1, src2, work;
+    int trapq;          // the charged bulk traps' charge in the Poisson rows
     lcdouble *lseg;     // the lane's segment table in LDS
 };
 
@@ -581,6 +609,7 @@ DDH_FN void to_t(const double *in, T *out, int m) {
 // and its point's Poisson row scale, continuity row scale and doping.
 struct RowSeg {
     double meps, ceps, sP, scs, dop;
+    double qt = 0.0, dqn = 0.0, dqp = 0.0;  // the charged traps' charge, d/d ln n, d/d ln p
 };
 
 template <int K>
@@ -604,14 +633,15 @@ DDH_FN void row_blocks_s(const RowSeg &g, const Pt &cur, const Pt &mid, double r
     const double gen = mid.G * g_frac;
     const double elec = hm * eR.Fn - hp * eL.Fn + fac_ * (gen - mid.R);
     const double hole = hp * eL.Fp - hm * eR.Fp + fac_ * (gen - mid.R);
-    const double pois = fac_l * cur.V + fac_m * mid.V + fu * rV -
-                        Q_E * (mid.n - mid.p + dop + share * mid.fix + mid.an - mid.ca);
+    const double pois = (fac_l * cur.V + fac_m * mid.V + fu * rV -
+                         Q_E * (mid.n - mid.p + dop + share * mid.fix + mid.an - mid.ca)) +
+                        Q_E * g.qt;  // solver.py:1136-1140
     A[0] = fac_l * sP; A[1] = 0.0; A[2] = 0.0;
     A[3] = hp * eL.gn * sC; A[4] = -hp * eL.an0 * sC; A[5] = 0.0;
     A[6] = -hp * eL.gp * sC; A[7] = 0.0; A[8] = hp * eL.ap0 * sC;
     B[0] = (fac_m - Q_E * (mid.an + mid.ca) * iVt) * sP;
-    B[1] = -Q_E * mid.n * sP;
-    B[2] = Q_E * mid.p * sP;
+    B[1] = (-Q_E * mid.n + Q_E * g.dqn) * sP;
+    B[2] = (Q_E * mid.p + Q_E * g.dqp) * sP;
     B[3] = (-hp * eL.gn - hm * eR.gn) * sC;
     B[4] = (-hp * eL.an1 + hm * eR.an0 - fac_ * mid.dRn) * sC;
     B[5] = (-fac_ * mid.dRp) * sC;
@@ -636,13 +666,20 @@ DDH_FN void row_blocks_s(const RowSeg &g, const Pt &cur, const Pt &mid, double r
 template <int K>
 DDH_FN void row_blocks(const Lane &Lv, const Pt &cur, const Pt &mid, double rV, const Edge &eL,
                        const Edge &eR, bool last, double L2, double iVt, double share,
-                       double g_frac, double *A, double *B, double *C, double *r) {
+                       double g_frac, double *A, double *B, double *C, double *r,
+                       bool trapq = false) {
     const int w = Lv.ls;
     lcdouble *sm = Lv.seg + mid.sid * SEG_N * w;
     RowSeg g;
     g.meps = sm[SEG_EPS * w];
     g.ceps = Lv.seg[(cur.sid * SEG_N + SEG_EPS) * w];
     g.sP = sm[SEG_SP * w]; g.scs = sm[SEG_SCS * w]; g.dop = sm[SEG_DOP * w];
+    if (trapq) {  // the lane's device holds charged traps, and this solve counts their charge
+        const double tt = sm[SEG_TT * w], Nt = sm[SEG_NT * w];
+        if (tt != 0.0 && Nt > 0.0)
+            g.qt = trap_charge(tt, mid.n, mid.p, Nt, sm[SEG_CN * w], sm[SEG_CP * w],
+                               sm[SEG_NT0 * w], sm[SEG_PT0 * w], g.dqn, g.dqp);
+    }
     row_blocks_s<K>(g, cur, mid, rV, eL, eR, last, L2, iVt, share, g_frac, A, B, C, r);
 }
 
@@ -704,7 +741,7 @@ DDH_SWEEP void elim_sweep(const Args &a, int lane, const SweepCtl &cref, double 
             double A[9], B[9], C[9], r[3 * K];
             // (the noise's right-hand side takes the last row's coupling to the contact too)
             row_blocks<K>(Lv, cur, mid, rV, eL, eR, !NOISE && i == NP, L2, iVt, c.share, c.g_frac,
-                          A, B, C, r);
+                          A, B, C, r, c.trapq != 0);
             if constexpr (NOISE) {
                 // eps * sum |dF/dy| |y| over the row's three points, y = (V, ln n, ln p)
                 const double yl[3] = {cur.V, log(cur.n), log(cur.p)};
@@ -1079,12 +1116,22 @@ DDH_FN void begin_stage(const Args &a, LaneState &S) {
     begin_solve(a, S, S.first, share, a.jv ? a.max_iter : a.start_iter, S.b_c, S.b_c, 0.0, lg, work);
 }
 
-// The lane's first solve after Poisson's start.
-DDH_FN void start_lane(const Args &a, int lane, LaneState &S) {
+// The lane's first solve after Poisson's start. With charged bulk traps, first the solve with
+// them neutral, whose solution (where it converges) stands in for Poisson's start
+// (src/jaxdd/batch.py:898-900, solver.py:2192-2200): from Poisson's, Newton finds no way to the
+// charged solution.
+DDH_FN void start_lane(const Args &a, int lane, LaneState &S, bool neutral_done = false) {
     S.err = 0;
     S.mobile_now = S.mobile0;
     S.freeze_now = 0;
     S.pre_ok = 1;
+    if (S.charged && !neutral_done) {
+        S.phase = PH_NEUTRAL;
+        S.neutral = 1;
+        begin_solve(a, S, S.first, 1.0, a.jv ? a.max_iter : a.start_iter, S.b_pois, S.b_pois,
+                    0.0, S.lpois, 4);
+        return;
+    }
     bool pre = SC(SC_PRE) > 0;
     if (a.jv && !S.mobile0) {
         // solve_jv: without mobile ions no solve at the first voltage (solver.py:2211-2226)
@@ -1100,6 +1147,15 @@ DDH_FN void start_lane(const Args &a, int lane, LaneState &S) {
 DDH_FN void end_solve(const Args &a, int lane, LaneState &S, int ok, double J) {
     const int fin = S.fin_work && isfinite(S.lw[0]) && isfinite(S.lw[1]);
     switch (S.phase) {
+    case PH_NEUTRAL: {
+        S.neutral = 0;
+        if (ok) {  // buffer 4: no later solve of the start writes it
+            S.b_pois = S.b_work;
+            S.lpois[0] = S.lw[0]; S.lpois[1] = S.lw[1];
+        }
+        start_lane(a, lane, S, true);
+        return;
+    }
     case PH_F: {
         S.ok_f = ok;
         S.fin_f = fin;
@@ -1244,6 +1300,7 @@ DDH_FN SweepCtl sweep_ctl(const Args &a, int lane, const LaneState &S, lcdouble 
     c.lstart[0] = S.lstart[0]; c.lstart[1] = S.lstart[1];
     c.fresh = S.fresh; c.freeze_now = S.freeze_now && S.fresh; c.mobile = S.mobile_now;
     c.src1 = S.b_src1; c.src2 = S.b_src2; c.work = S.b_work;
+    c.trapq = S.charged && !S.neutral;
     return c;
 }
 

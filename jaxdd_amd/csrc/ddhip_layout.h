@@ -52,6 +52,7 @@ enum SegField {
     SEG_SCS,      // the continuity rows' scale times h: 1/(max(mu_n, mu_p) V_T N_c)
     SEG_IMUN,     // 1/mu_n, 1/mu_p of the interval after the point (the current's integrals)
     SEG_IMUP,
+    SEG_TT,       // the bulk traps' charge type: 1 donor, -1 acceptor, 0 neutral or none
     SEG_N
 };
 

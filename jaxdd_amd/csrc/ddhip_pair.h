@@ -112,7 +112,7 @@ DDH_PAIRFN void elim_half(const Args &a, int lane, int half, const SweepCtl &cre
             const Edge eL = dir > 0 ? eold : en, eR = dir > 0 ? en : eold;
             double A[9], B[9], C[9], r[3 * K];
             row_blocks<K>(Lv, lft, mid, rgt.V, eL, eR, i == NP, L2, iVt, c.share, c.g_frac, A,
-                          B, C, r);
+                          B, C, r, c.trapq != 0);
             if constexpr (sizeof(T) == 4) {
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {

@@ -157,7 +157,7 @@ DDH_QUADFN void elim_quarter(const Args &a, int lane, int q, const SweepCtl &cre
             const Edge eL = dir > 0 ? eold : en, eR = dir > 0 ? en : eold;
             double A[9], B[9], C[9], r[3];
             row_blocks<1>(Lv, lft, mid, rgt.V, eL, eR, i == NP, L2, iVt, c.share, c.g_frac, A, B,
-                          C, r);
+                          C, r, c.trapq != 0);
             if constexpr (sizeof(T) == 4) {
                 // row equilibration by powers of two (exact) before the rounding to FP32
 #pragma unroll

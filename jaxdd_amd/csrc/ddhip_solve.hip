@@ -580,6 +580,9 @@ static int solve_batch(const ddh_batch *b, const double *V, int32_t nV, const dd
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
         hipLaunchKernelGGL(ddh::solve_pair_kernel, grid, block, plds, s, a);
     } else {
+        if (lds > 160 * 1024)
+            return fail(DDH_E_ARG, "ddh_solve_batch: one lane per device takes at most 13 "
+                                   "segments (its segment tables fill the 160 KiB of LDS)");
         dim3 grid((b->n + 63) / 64), block(64);
         (void)hipFuncSetAttribute((const void *)ddh::solve_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

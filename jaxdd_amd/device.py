@@ -109,6 +109,9 @@ class Device:
     curr_diff: bool = False
     free: tuple[bool, bool] = (False, False)
     statistics: str = "boltzmann"
+    # per point: the bulk traps' charge type where they are (1 donor, -1 acceptor, 0 neutral
+    # and where there are none; reference: device.py:940-948 `trap_charge`, `trap_cwe`)
+    srh_type: np.ndarray | None = None
 
     @property
     def NP(self) -> int:
@@ -148,6 +151,7 @@ class Structure:
     extra: dict = field(default_factory=dict)
     langevin: np.ndarray | None = None  # per layer: direct recombination by Langevin's rate
     langevin_mobility: str = "layer"  # `LANGEVIN_MOBILITIES`
+    trap_type: np.ndarray | None = None  # per layer: the bulk traps' `bulkTrapType` (1, -1, 0)
 
     @property
     def layout(self) -> Layout:
@@ -242,6 +246,7 @@ def device_parameters(setup: Setup, grid_split: str = "exact", grid: str = "grad
         sfb=sfb, gen_mode=gen_mode, left_elec=setup.i("leftElec"),
         langevin=np.array([int(float(ly.raw.get("useLangevin", "0"))) == 1 for ly in layers]),
         langevin_mobility=langevin,
+        trap_type=np.array([int(float(ly.raw.get("bulkTrapType", "0"))) for ly in layers]),
     )
     for name, ly, flat in (("W_L", layers[0], sfb[0]), ("W_R", layers[-1], sfb[1])):
         if not flat and not ly.f("E_c") <= setup.f(name) <= ly.f("E_v"):
@@ -391,6 +396,8 @@ def device_arrays(structure: Structure, p: dict) -> Device:
         R_series=float(p.get("R_series", 0.0)), R_shunt=float(p.get("R_shunt", -1.0)),
         left_elec=st.left_elec,
         mu_anion=on_points(p["mu_anion"]), mu_cation=on_points(p["mu_cation"]),
+        srh_type=(None if st.trap_type is None or not np.any(st.traps & (st.trap_type != 0))
+                  else in_traps(np.where(st.traps, st.trap_type, 0).astype(float))),
     )
 
 

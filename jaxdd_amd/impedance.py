@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batch import SC_FIELDS, _sc_now
+from .batch import SC_FIELDS, _sc_now, refuse_charged_traps
 
 #: The ions' fluxes the reference's transients take (src/jaxdd/solver.py:1312).
 ION_FLUXES = ("electrochemical", "modified-drift")
@@ -86,6 +86,7 @@ def solve_impedance(batch, frequencies, voltage=0.0, tol=1e-10, max_iter=60,
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
     if dt is not None and not (np.isfinite(dt) and dt > 0):
         raise ValueError(f"dt {dt!r}: a positive step (s), or None for the continuous response")
+    refuse_charged_traps(batch, "the impedance")
     sc = batch["_sc"]
     ions = bool(((sc[SC_FIELDS.index("a_total")] > 0) | (sc[SC_FIELDS.index("c_total")] > 0)).any())
     if ions and bool(((batch["pre_cond"] > 0) & (batch["fix_ions"] > 0)).any()):

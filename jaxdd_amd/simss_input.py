@@ -236,7 +236,7 @@ def ion_region_problems(setup: Setup) -> list[str]:
 #: Layer switches whose other values turn on physics outside this path, with their off value.
 _LAYER_SWITCHES = {
     "mobnDep": "0", "mobpDep": "0", "fieldDepG": "0",
-    "bulkTrapType": "0", "intTrapFile": "none", "bulkTrapFile": "none",
+    "intTrapFile": "none", "bulkTrapFile": "none",
 }
 
 
@@ -267,9 +267,10 @@ def _device_problems(setup: Setup) -> list[str]:
 
 def _layer_problems(j: int, ly: Layer, last: bool) -> list[str]:
     out = []
+    kind = ly.raw.get("bulkTrapType", "0").strip()
+    if float(kind) not in (-1.0, 0.0, 1.0):
+        out.append(f"layer {j}: bulkTrapType = {kind} (-1 acceptor, 0 neutral, 1 donor)")
     for name, off in _LAYER_SWITCHES.items():
-        if name == "bulkTrapType" and float(ly.raw.get("N_t_bulk", "0")) == 0:
-            continue  # a trap type without traps changes nothing
         value = ly.raw.get(name, off)
         if _switched_on(value, off):
             out.append(f"layer {j}: {name} = {value.strip()} is not on this path")

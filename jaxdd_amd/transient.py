@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batch import SC_FIELDS, _run, _sc_now
+from .batch import SC_FIELDS, _run, _sc_now, refuse_charged_traps
 from .simss_input import TVG
 
 __all__ = ["TransientResult", "solve_transient"]
@@ -79,6 +79,7 @@ def solve_transient(batch, tvg: TVG | list[TVG], tol=1e-10, max_iter=200,
     (`solve_batch`'s start, the same kernels)."""
     if bernoulli_name not in ("exact", "simss"):
         raise ValueError(f"bernoulli_name {bernoulli_name!r}: 'exact' or 'simss'")
+    refuse_charged_traps(batch, "the transient")
     if not GPU_TRANSIENT_ENABLED:
         raise NotImplementedError(
             "the GPU transient kernel is switched off: its first run on the MI355X ended in a "

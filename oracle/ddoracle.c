@@ -75,7 +75,9 @@ typedef struct {
     int NP, bern;
     /* stencils on the interior points, solver.py:210-249 */
     double *fac_l, *fac_m, *fac_u, *hm, *hp, *fac, *dop, *kd, *ni2, *Nt, *Cn, *Cp, *nt0, *pt0;
-    double *Gs, *am, *cm, *afix, *cfix, *sP, *sC;
+    double *Gs, *am, *cm, *afix, *cfix, *sP, *sC, *tt;
+    int charged;  /* some point holds charged bulk traps */
+    int neutral;  /* their charge left out (the start's first solve, batch.py:898-900) */
     double n_floor;
     /* ions over the full grid (one slot per species): masks, totals, hL = h*L per interval */
     double *amask_full, *cmask_full, *hL;
@@ -100,7 +102,8 @@ static void problem_init(problem *P, const ddo_device *d, int bern) {
     P->bern = bern;
     double **fields[] = {&P->fac_l, &P->fac_m, &P->fac_u, &P->hm, &P->hp, &P->fac, &P->dop,
                          &P->kd,    &P->ni2,   &P->Nt,    &P->Cn, &P->Cp, &P->nt0, &P->pt0,
-                         &P->Gs,    &P->am,    &P->cm,    &P->afix, &P->cfix, &P->sP, &P->sC};
+                         &P->Gs,    &P->am,    &P->cm,    &P->afix, &P->cfix, &P->sP, &P->sC,
+                         &P->tt};
     for (size_t f = 0; f < sizeof(fields) / sizeof(fields[0]); ++f) *fields[f] = alloc_d(NP);
     P->amask_full = alloc_d(NP + 2);
     P->cmask_full = alloc_d(NP + 2);
@@ -132,6 +135,8 @@ static void problem_init(problem *P, const ddo_device *d, int bern) {
         P->Cp[r] = d->Cp[i];
         P->nt0[r] = d->nt0[i];
         P->pt0[r] = d->pt0[i];
+        P->tt[r] = d->tt ? d->tt[i] : 0.0;
+        if (P->tt[r] != 0.0) P->charged = 1;
         P->Gs[r] = d->G[i];
         P->am[r] = d->a_mask[i];
         P->cm[r] = d->c_mask[i];
@@ -150,7 +155,7 @@ static void problem_init(problem *P, const ddo_device *d, int bern) {
 static void problem_free(problem *P) {
     double *fields[] = {P->fac_l, P->fac_m, P->fac_u, P->hm,  P->hp,  P->fac,  P->dop,
                         P->kd,    P->ni2,   P->Nt,    P->Cn,  P->Cp,  P->nt0,  P->pt0,
-                        P->Gs,    P->am,    P->cm,    P->afix, P->cfix, P->sP, P->sC,
+                        P->Gs,    P->am,    P->cm,    P->afix, P->cfix, P->sP, P->sC, P->tt,
                         P->amask_full, P->cmask_full, P->hL, P->ni2_full, P->work};
     for (size_t f = 0; f < sizeof(fields) / sizeof(fields[0]); ++f) free(fields[f]);
 }
@@ -178,6 +183,21 @@ static double recomb(double n, double p, double kd, double ni2, double Nt, doubl
     if (dln_n) *dln_n = dn * n;
     if (dln_p) *dln_p = dp * p;
     return R;
+}
+
+/* The charged bulk traps' charge density (m^-3) at a point, and its derivatives in ln n and
+   ln p (solver.py:762-773 `_trap_charge`, one level; device.py:940-948): Nt (cwe - f), cwe 1
+   for donors and 0 for acceptors, f = (Cn n + Cp pt0) / (Cn (n + nt0) + Cp (p + pt0)) the
+   level's steady occupation; 0 for neutral traps and where there are none. */
+static double trap_charge(double tt, double n, double p, double Nt, double Cn, double Cp,
+                          double nt0, double pt0, double *dln_n, double *dln_p) {
+    *dln_n = *dln_p = 0.0;
+    if (tt == 0.0 || !(Nt > 0)) return 0.0;
+    double num = Cn * n + Cp * pt0, den = Cn * (n + nt0) + Cp * (p + pt0);
+    double f = num / den, cwe = tt > 0 ? 1.0 : 0.0;
+    *dln_n = -Nt * n * Cn * (Cn * nt0 + Cp * p) / (den * den);
+    *dln_p = Nt * p * Cp * num / (den * den);
+    return Nt * (cwe - f);
 }
 
 /* ---- assembly: solver.py:1021-1187 (Boltzmann, no optional modules) --------------------- */
@@ -228,6 +248,11 @@ static void assemble_pt(const problem *P, const double *full, double g_frac, dou
         double poisson = P->fac_l[r] * full[3 * (i - 1)] + P->fac_m[r] * Vm +
                          P->fac_u[r] * full[3 * (i + 1)] -
                          Q_E * (nm - pm + P->dop[r] + afix + anion - cfix - cation);
+        double dQn, dQp;
+        double tq = P->neutral ? 0.0 : P->tt[r];
+        double Qt = trap_charge(tq, nm, pm, P->Nt[r], P->Cn[r], P->Cp[r], P->nt0[r], P->pt0[r],
+                                &dQn, &dQp);
+        if (tq != 0.0) poisson = poisson + Q_E * Qt;  /* solver.py:1136-1140 */
         double *a = A + 9 * r, *b = B + 9 * r, *c = C + 9 * r;
         const double *l0 = dFn0 + 3 * (i - 1), *l1 = dFn1 + 3 * (i - 1);
         const double *r0 = dFn0 + 3 * i, *r1 = dFn1 + 3 * i;
@@ -237,8 +262,8 @@ static void assemble_pt(const problem *P, const double *full, double g_frac, dou
         a[0] = P->fac_l[r] * sP; a[1] = 0; a[2] = 0;
         c[0] = P->fac_u[r] * sP; c[1] = 0; c[2] = 0;
         b[0] = (P->fac_m[r] - Q_E * (anion + cation) / Vt) * sP;
-        b[1] = -Q_E * nm * sP;
-        b[2] = Q_E * pm * sP;
+        b[1] = (-Q_E * nm + Q_E * dQn) * sP;
+        b[2] = (Q_E * pm + Q_E * dQp) * sP;
         for (int col = 0; col < 3; ++col) {
             a[3 + col] = -hp * l0[col] * sC;
             a[6 + col] = hp * q0[col] * sC;
@@ -704,10 +729,15 @@ static void poisson_solve(const problem *P, scratch *s, double *y, double Vint, 
             double afix = P->afix[r] * sc->share, cfix = P->cfix[r] * sc->share;
             double F = P->fac_l[r] * Vf[i - 1] + P->fac_m[r] * Vm + P->fac_u[r] * Vf[i + 1] -
                        Q_E * (n - p + P->dop[r] + afix + anion - cfix - cation);
+            double dQn, dQp;  /* the traps' charge, the densities moving with V (solver.py:1672) */
+            double Qt = trap_charge(P->tt[r], n, p, P->Nt[r], P->Cn[r], P->Cp[r], P->nt0[r],
+                                    P->pt0[r], &dQn, &dQp);
+            if (P->tt[r] != 0.0) F = F + Q_E * Qt;
             rhs[3 * r] = -F * P->sP[r];
             lo[r] = r == 0 ? 0.0 : P->fac_l[r] * P->sP[r];
             up[r] = r == NP - 1 ? 0.0 : P->fac_u[r] * P->sP[r];
-            mid[r] = (P->fac_m[r] - Q_E * (n / Vt + p / Vt + anion / Vt + cation / Vt)) * P->sP[r];
+            mid[r] = (P->fac_m[r] - Q_E * (n / Vt + p / Vt + anion / Vt + cation / Vt) +
+                      Q_E * (dQn - dQp) / Vt) * P->sP[r];
             rhs[3 * r + 1] = -Q_E * anion * P->sP[r] * P->am[r];
             rhs[3 * r + 2] = Q_E * cation * P->sP[r] * P->cm[r];
         }
@@ -811,6 +841,15 @@ static int start_state(problem *P, scratch *s, const ddo_device *d, double first
     poisson_solve(P, s, y, first, &full_sc, o->start_iter);
     full_V(d, first, y, Vf);
     ion_logs(P, Vf, &full_sc, logs);
+    if (P->charged) {  /* charged bulk traps: from the solution with them neutral (batch.py:898) */
+        double logs_n[2] = {logs[0], logs[1]};
+        memcpy(yf, y, ny * sizeof(double));
+        P->neutral = 1;
+        int ok_n = newton_loop(P, s, yf, logs_n, first, g_frac, &full_sc, o->tol, o->start_iter,
+                               o->strict, 0, &Jdummy, &itdummy);
+        P->neutral = 0;
+        if (ok_n) { memcpy(y, yf, ny * sizeof(double)); logs[0] = logs_n[0]; logs[1] = logs_n[1]; }
+    }
     memcpy(yf, y, ny * sizeof(double));
     double logs_f[2] = {logs[0], logs[1]};
     int ok_f = newton_loop(P, s, yf, logs_f, first, g_frac, &full_sc, o->tol, o->start_iter,
@@ -1033,6 +1072,14 @@ int ddo_solve_jv(const ddo_device *d, const double *Vs, int nV, double tol, int 
     poisson_solve(&P, &s, y, first, &sc, max_iter);
     full_V(d, first, y, Vf);
     ion_logs(&P, Vf, &sc, logs);
+    if (P.charged) {  /* charged bulk traps: from the solution with them neutral (solver.py:2192) */
+        memcpy(yn, y, ny * sizeof(double));
+        double ln_[2] = {logs[0], logs[1]};
+        P.neutral = 1;
+        int ok = settle(&P, &s, yn, ln_, first, &sc, tol, max_iter, &steps);
+        P.neutral = 0;
+        if (ok) { memcpy(y, yn, ny * sizeof(double)); logs[0] = ln_[0]; logs[1] = ln_[1]; }
+    }
     if (mobile) {
         memcpy(yn, y, ny * sizeof(double));
         double lf[2] = {logs[0], logs[1]};
@@ -1592,6 +1639,9 @@ int ddo_transient(const ddo_device *d, const double *t, const double *V, const d
     int NP = d->NP;
     if (NP < 2 || NP > 4096 || nt < 1) return -1;
     if (d->pre_cond) return -3;
+    if (d->tt)  /* charged traps' occupations in time are outside the restatement */
+        for (int i = 0; i < NP + 2; ++i)
+            if (d->tt[i] != 0.0) return -5;
     problem P;
     problem_init(&P, d, o->bernoulli);
     int ions = d->a_total > 0 || d->c_total > 0;

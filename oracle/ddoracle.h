@@ -42,6 +42,8 @@ typedef struct {
     double left_elec, a_total, c_total, V_pre, J_scale, G_shunt;
     int32_t pre_cond, fix_ions;
     const double *mu_a, *mu_c; /* the mobile ions' mobility per point (transients) */
+    const double *tt; /* per point: the bulk traps' charge type, 1 donor, -1 acceptor, 0
+                         neutral or none (NULL: none charged); the steady state only */
 } ddo_device;
 
 typedef struct {

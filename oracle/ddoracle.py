@@ -34,7 +34,8 @@ class _Device(ctypes.Structure):
             "L", "Vt", "g_frac", "V0", "VL", "W_L", "W_R", "ln_n_left", "ln_n_right",
             "ln_p_left", "ln_p_right", "left_elec", "a_total", "c_total", "V_pre", "J_scale",
             "G_shunt")
-    ] + [("pre_cond", ctypes.c_int32), ("fix_ions", ctypes.c_int32), ("mu_a", _P), ("mu_c", _P)]
+    ] + [("pre_cond", ctypes.c_int32), ("fix_ions", ctypes.c_int32), ("mu_a", _P), ("mu_c", _P),
+         ("tt", _P)]
 
 
 class _Opts(ctypes.Structure):
@@ -128,6 +129,7 @@ class _Pinned:
             "a_fixed": dev.anion_fixed, "c_fixed": dev.cation_fixed,
             "mu_a": np.zeros(NP + 2) if dev.mu_anion is None else dev.mu_anion,
             "mu_c": np.zeros(NP + 2) if dev.mu_cation is None else dev.mu_cation,
+            "tt": np.zeros(NP + 2) if dev.srh_type is None else dev.srh_type,
         }
         self.keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in arrays.items()}
         iface = np.zeros(NP + 1, dtype=np.uint8)
@@ -310,6 +312,8 @@ def transient(dev, t, V, G_frac, tol=1e-10, max_iter=200, bernoulli_name="exact"
                              ctypes.byref(o), _ptr(out["J"]), _ptr(out["J_ext"]),
                              out["converged"].ctypes.data, out["iterations"].ctypes.data,
                              out["kept"].ctypes.data)
+    if rc == -5:
+        raise NotImplementedError("the transient restatement takes no charged bulk traps")
     if rc != 0:
         raise ValueError(f"ddo_transient: error {rc}")
     out["converged"] = out["converged"].astype(bool)

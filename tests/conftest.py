@@ -29,6 +29,9 @@ FIXTURES = {
     # six voltages 0.1 mV apart around open circuit (SimSS at tolPois 1e-12): the curve's
     # largest |J| is 1.1 A/m^2, 4e-3 of the photocurrent; 3e-7 of it is 1e-9 of the latter
     "thin_l1_voc_light": 3e-7,
+    # acceptor bulk traps (3.9e21 m^-3, mid-gap) charged in the absorber, SimSS at tolerances
+    # of 1e-9 (tests/test_solver.py:909-922)
+    "bulk_acceptor_light": 1e-9,
 }
 
 

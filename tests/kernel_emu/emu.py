@@ -225,9 +225,9 @@ def build_packed(structures, params, directions=None):
     n, NP, nl, nseg = len(structures), sa["NP"], sa["nl"], sa["nseg"]
     P = NP + 2
     K = 0 if directions is None else np.asarray(directions[0]).shape[0]
-    pt, seg = np.zeros((P, 4, n)), np.zeros((nseg, 22, n))
+    pt, seg = np.zeros((P, 4, n)), np.zeros((nseg, 23, n))
     sc, imu = np.zeros((len(SC_FIELDS), n)), np.zeros((nseg, 2, n))
-    dpt, dseg = np.zeros((max(K, 1), P, 4, n)), np.zeros((max(K, 1), nseg, 22, n))
+    dpt, dseg = np.zeros((max(K, 1), P, 4, n)), np.zeros((max(K, 1), nseg, 23, n))
     dsc = np.zeros((max(K, 1), len(SC_FIELDS), n))
     dl = np.ascontiguousarray(directions[0], dtype=float) if K else np.zeros(1)
     ds = np.ascontiguousarray(directions[1], dtype=float) if K else np.zeros(1)

@@ -568,3 +568,17 @@ def test_autograd_function_backward_is_the_adjoint(c4_gpu):
     expect = np.einsum("nv,nvk->nk", w.numpy(), _np(g))
     got = theta.grad.numpy()
     assert np.allclose(got, expect, rtol=1e-9, atol=0.0)
+
+
+def test_charged_traps_are_refused_by_the_gradients_and_the_impedance(torch_cuda):
+    """A batch holding charged bulk traps: the steady solve takes it (the fixture tests), the
+    gradients and the impedance refuse it before any launch."""
+    from jaxdd_amd.adjoint import jv_gradients
+    from jaxdd_amd.batch import prepare
+    from jaxdd_amd.impedance import solve_impedance
+
+    batch = prepare([fixture_device("bulk_acceptor_light")])
+    with pytest.raises(NotImplementedError, match="charged"):
+        solve_impedance(batch, [1e3])
+    with pytest.raises(NotImplementedError, match="charged"):
+        jv_gradients(batch, None, [0.0, 0.5])

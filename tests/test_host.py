@@ -172,7 +172,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     lib.ddh_abi_version.restype = ctypes.c_int
-    assert lib.ddh_abi_version() == 4
+    assert lib.ddh_abi_version() == 5
     lib.ddh_workspace_bytes.restype = ctypes.c_size_t
     assert lib.ddh_workspace_bytes(16384, 400, 61, 1) > 0
     assert lib.ddh_workspace_bytes(0, 400, 61, 1) == 0

@@ -84,7 +84,8 @@ def _perturbed_state(dev, seed=1):
     return full + noise
 
 
-@pytest.mark.parametrize("case", ["thin_l1_light", "steady_1e17_thin_light"])
+@pytest.mark.parametrize("case", ["thin_l1_light", "steady_1e17_thin_light",
+                                  "bulk_acceptor_light"])
 @pytest.mark.parametrize("bern", ["exact", "simss"])
 def test_assembly_jacobian_is_the_residuals_derivative(oracle, case, bern):
     """A, B, C and the ion columns U against central differences of the residual F
@@ -157,6 +158,48 @@ def test_oracle_batch_reproduces_simss(oracle, case):
     assert ok.all()
     scale = np.max(np.abs(ref["Jext"]))
     assert np.max(np.abs(J[0] - ref["Jext"])) < FIXTURES[case] * scale
+
+
+def test_charged_traps_move_the_curve_as_simss(oracle):
+    """Acceptor bulk traps charged in the absorber (reference: solver.py:762-773, 1136-1140; the
+    start from the solution with them neutral, batch.py:898-900): the oracle's curve is SimSS's
+    to 1e-9 of its scale, in at most 8 steps a voltage, where the same traps left neutral miss it
+    by 4 % (their charge is what moves it); the transient's restatement refuses them."""
+    import dataclasses
+
+    from conftest import fixture_device, fixture_jv
+
+    dev, ref = fixture_device("bulk_acceptor_light"), fixture_jv("bulk_acceptor_light")
+    assert set(np.unique(dev.srh_type)) == {-1.0, 0.0}
+    scale = np.max(np.abs(ref["Jext"]))
+    J, ok, its = oracle.solve_jv(dev, ref["Vint"], bernoulli_name="simss")
+    assert ok.all() and its.max() <= 8
+    assert np.max(np.abs(J - ref["Jext"])) < 1e-9 * scale
+    J0, ok0, _ = oracle.solve_jv(dataclasses.replace(dev, srh_type=None), ref["Vint"],
+                                 bernoulli_name="simss")
+    assert ok0.all() and np.max(np.abs(J0 - ref["Jext"])) > 0.03 * scale
+    with pytest.raises(NotImplementedError, match="charged"):
+        oracle.transient(dev, np.array([0.0, 1e-9]), np.array([0.0, 0.1]), np.ones(2))
+
+
+def test_charged_traps_are_refused_outside_the_steady_solve():
+    """The GPU builder, the gradients, the impedance and the transient take no charged traps:
+    `NotImplementedError` before any launch."""
+    from conftest import GOLDEN
+    from jaxdd_amd import builder as B
+    from jaxdd_amd.batch import charged_traps, refuse_charged_traps
+    from jaxdd_amd.device import build_device, device_parameters
+    from jaxdd_amd.simss_input import check_supported, read_setup
+
+    setup = read_setup(GOLDEN / "bulk_acceptor_light" / "simulation_setup.txt")
+    check_supported(setup)
+    st, _ = device_parameters(setup, grid="simss")
+    with pytest.raises(NotImplementedError, match="charged"):
+        B.check_buildable(st)
+    dev = build_device(setup, grid="simss")
+    assert charged_traps([dev]) and not charged_traps([fixture_device("thin_l1_light")])
+    with pytest.raises(NotImplementedError, match="the impedance: charged"):
+        refuse_charged_traps([dev], "the impedance")
 
 
 def test_open_circuit_voltage_is_simss(oracle):
