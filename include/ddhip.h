@@ -42,8 +42,9 @@ enum ddh_precision { DDH_FP64 = 0, DDH_MIXED = 1 };
  * the counterpart of src/jaxdd/batch.py:151-223 (`prepare`). A segment's 23rd field is its bulk
  * traps' charge type (1 donor, -1 acceptor, 0 neutral or none; device.py:940-948): their charge
  * enters the Poisson rows of ddh_solve_batch, ddh_solve_batch_states and ddh_assemble (the
- * start then solves first with them neutral, batch.py:898-900). ddh_adjoint, ddh_adjoint_vjp,
- * ddh_impedance, ddh_impedance_ions and ddh_transient take no charged traps (the field 0). */
+ * start then solves first with them neutral, batch.py:898-900), ddh_assemble, ddh_adjoint and
+ * ddh_adjoint_vjp (the charge's partials in the traps' fields). ddh_impedance,
+ * ddh_impedance_ions and ddh_transient take no charged traps (the field 0). */
 typedef struct {
     int32_t n;        /* devices */
     int32_t NP;       /* interior grid points */

@@ -34,8 +34,7 @@ import torch
 
 from . import _lib
 from . import synthetic as S
-from .batch import (SC_FIELDS, _sc_now, check_four_lanes, from_packed, pack,
-                    refuse_charged_traps)
+from .batch import SC_FIELDS, _sc_now, check_four_lanes, from_packed, pack
 
 THETA = ("L_etl", "L_pvk", "L_htl",
          "mu_n_etl", "mu_p_etl", "mu_n_pvk", "mu_p_pvk", "mu_n_htl", "mu_p_htl",
@@ -327,7 +326,6 @@ def jv_gradients(packed, tan, voltages, tol=1e-10, max_iter=60, strict=False,
     if np.any(np.diff(V) <= 0):
         raise ValueError("jv_gradients takes ascending voltages")
     batch = packed if "_pt" in packed else from_packed(packed, device)
-    refuse_charged_traps(batch, "the gradients")
     n, NP, nV = batch["n"], batch["NP"], len(V)
     lanes = _lanes(batch, lanes)
     if lanes not in (1, 2, 4):
@@ -470,7 +468,6 @@ def jv_vjp(batch, tan, voltages, dJbar, tol=1e-10, max_iter=60, bernoulli_name="
     V = np.asarray(voltages, dtype=float)
     if np.any(np.diff(V) <= 0):
         raise ValueError("jv_vjp takes ascending voltages")
-    refuse_charged_traps(batch, "the gradients")
     n, NP, nV = batch["n"], batch["NP"], len(V)
     K = tan["pt"].shape[0]
     dev = batch["_pt"].device

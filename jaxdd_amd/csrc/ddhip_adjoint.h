@@ -158,6 +158,16 @@ DDH_FN D1 recomb_d1(D1 n, D1 p, const SegD &sd, int id) {
     return R;
 }
 
+// The charged bulk traps' charge along the direction (trap_charge's value, solver.py:762-773).
+DDH_FN D1 trap_charge_d1(D1 n, D1 p, const SegD &sd, int id) {
+    const double tt = sd(id, SEG_TT).v;
+    const D1 Nt = sd(id, SEG_NT);
+    if (tt == 0.0 || !(Nt.v > 0)) return dc(0.0);
+    const D1 Cn = sd(id, SEG_CN), Cp = sd(id, SEG_CP), pt0 = sd(id, SEG_PT0);
+    const D1 f = (Cn * n + Cp * pt0) / (Cn * (n + sd(id, SEG_NT0)) + Cp * (p + pt0));
+    return Nt * (dc(tt > 0 ? 1.0 : 0.0) - f);
+}
+
 // The edge fluxes Fn, Fp over (l, r) along the direction (src/jaxdd/solver.py:1069-1108).
 DDH_FN void edge_d1(const PD &l, const PD &r, const SegD &sd, D1 Vt, D1 iVt, int bern,
                     D1 &Fn, D1 &Fp) {
@@ -195,8 +205,9 @@ DDH_FN void row_d1(const PD &l, const PD &m, const PD &r, D1 G, D1 fix, D1 Ld, D
     edge_d1(m, r, sd, Vt, iVt, bern, FnR, FpR);
     const D1 elec = hm * FnR - hp * FnL + fac_ * (gen - R);
     const D1 hole = hp * FpL - hm * FpR + fac_ * (gen - R);
-    const D1 pois = fac_l * l.V + fac_m * m.V + fu * r.V -
-                    Q_E * (m.n - m.p + sd(m.sid, SEG_DOP) + fix + m.an - m.ca);
+    const D1 pois = (fac_l * l.V + fac_m * m.V + fu * r.V -
+                     Q_E * (m.n - m.p + sd(m.sid, SEG_DOP) + fix + m.an - m.ca)) +
+                    Q_E * trap_charge_d1(m.n, m.p, sd, m.sid);
     F[0] = pois * sP;
     F[1] = elec * sC;
     F[2] = hole * sC;
@@ -457,6 +468,10 @@ DDH_FN void adj_lambda(const AdjArgs &a, int d, int v, size_t sl, lcdouble *tab,
         g.meps = TSEG(mid.sid, SEG_EPS);
         g.ceps = TSEG(cur.sid, SEG_EPS);
         g.sP = TSEG(mid.sid, SEG_SP); g.scs = TSEG(mid.sid, SEG_SCS); g.dop = TSEG(mid.sid, SEG_DOP);
+        if (TSEG(mid.sid, SEG_TT) != 0.0 && TSEG(mid.sid, SEG_NT) > 0.0)  // charged bulk traps
+            g.qt = trap_charge(TSEG(mid.sid, SEG_TT), mid.n, mid.p, TSEG(mid.sid, SEG_NT),
+                               TSEG(mid.sid, SEG_CN), TSEG(mid.sid, SEG_CP),
+                               TSEG(mid.sid, SEG_NT0), TSEG(mid.sid, SEG_PT0), g.dqn, g.dqp);
         row_blocks_s<1>(g, cur, mid, right.V, eL, eR, i == NP, L2, iVt, 1.0, gfr, A, B, C, r);
         double M[9], Mi[9];
 #pragma unroll
@@ -920,11 +935,31 @@ DDH_FN void adj_contract(const AdjArgs &a, int d, int v, size_t sl, lcdouble *ta
         const double anm = ion_an(sm, Vm), cam = ion_ca(sm, Vm);
         double fixm = TPT(j, PT_FIX), fan = 0.0, fca = 0.0;  // a frozen scan's: with the ions
         if (frozen && !pre) fixm += adj_frozen_ions(a, d, tab, j, lpre, fan, fca);
-        const double pois = fac_l * Vl + fac_m * Vm + fu * Vr -
-                            Q_E * (nm - pm + TSEG(sm, SEG_DOP) + fixm + anm - cam);
+        const double ttq = TSEG(sm, SEG_TT), Ntq = TSEG(sm, SEG_NT);
+        const bool charged = ttq != 0.0 && Ntq > 0.0;  // charged bulk traps (solver.py:1136)
+        double qt = 0.0;
+        if (charged) {
+            double dqn_, dqp_;
+            qt = trap_charge(ttq, nm, pm, Ntq, TSEG(sm, SEG_CN), TSEG(sm, SEG_CP),
+                             TSEG(sm, SEG_NT0), TSEG(sm, SEG_PT0), dqn_, dqp_);
+        }
+        const double pois = (fac_l * Vl + fac_m * Vm + fu * Vr -
+                             Q_E * (nm - pm + TSEG(sm, SEG_DOP) + fixm + anm - cam)) +
+                            Q_E * qt;
         // the row reversed, seeded with -lambda_j
         const double b0 = -LAM(j, 0), b1 = -LAM(j, 1), b2 = -LAM(j, 2);
         const double bpois = b0 * sP;
+        if (charged) {  // Nt (cwe - f), f = N / D: its partials in the level's fields
+            const double Cn = TSEG(sm, SEG_CN), Cp = TSEG(sm, SEG_CP);
+            const double nt0 = TSEG(sm, SEG_NT0), pt0 = TSEG(sm, SEG_PT0);
+            const double N = Cn * nm + Cp * pt0, D = Cn * (nm + nt0) + Cp * (pm + pt0);
+            const double iD2 = 1.0 / (D * D), bq = Q_E * bpois;
+            seg_add(sm, SEG_NT, bq * ((ttq > 0 ? 1.0 : 0.0) - N / D));
+            seg_add(sm, SEG_CN, -bq * Ntq * (nm * D - N * (nm + nt0)) * iD2);
+            seg_add(sm, SEG_CP, -bq * Ntq * (pt0 * D - N * (pm + pt0)) * iD2);
+            seg_add(sm, SEG_NT0, bq * Ntq * N * Cn * iD2);
+            seg_add(sm, SEG_PT0, -bq * Ntq * Cp * (D - N) * iD2);
+        }
         seg_add(sm, SEG_SP, b0 * pois);
         const double belec = b1 * sC, bhole = b2 * sC, bsC = b1 * elec_ + b2 * hole;
         seg_add(sm, SEG_SCS, bsC / hp);

@@ -346,3 +346,38 @@ def test_gradient_in_the_voltage_is_the_curves_slope(ions, emu, oracle):
         assert np.max(np.abs(dJdV[d] - fd)) < 1e-5 * np.max(np.abs(fd)), (d, dJdV[d], fd)
         assert np.median(np.abs(dJdV[d] - fd)) < 1e-6 * np.max(np.abs(fd))
 
+
+
+#: Numbers the charged traps' charge depends on: their density, capture coefficients and
+#: level, and a mobility beside them.
+CHARGED_WRT = (("N_t_bulk", 1), ("C_n_bulk", 1), ("C_p_bulk", 1), ("E_t_bulk", 1), ("mu_n", 1))
+
+
+def test_charged_trap_gradients_are_finite_differences(emu, oracle):
+    """Acceptor bulk traps charged in the absorber (SimSS's run `bulk_acceptor_light`): the
+    adjoint replay's gradients in the traps' numbers, their charge's partials included
+    (reference: solver.py:762-773, 1136-1140), with the host builder's tangents (central
+    differences, relative step 1e-6), against central differences of the oracle's curves."""
+    from conftest import bumped, fixture_inputs, named_fd_errors, number_of
+    from jaxdd_amd.batch import pack
+    from jaxdd_amd.device import device_arrays
+
+    st, p = fixture_inputs("bulk_acceptor_light")
+    packed = pack([device_arrays(st, p)])
+    assert np.any(packed["seg"][:, -1, 0] != 0)
+    tan = {key: np.zeros((len(CHARGED_WRT),) + packed[key].shape) for key in ("pt", "seg", "sc")}
+    for k, w in enumerate(CHARGED_WRT):
+        h = 1e-6 * abs(number_of(p, w))
+        up, dn = (pack([device_arrays(st, bumped(p, w, s * h))]) for s in (1.0, -1.0))
+        for key in tan:
+            tan[key][k] = (up[key] - dn[key]) / (2 * h)
+    V = np.array([-0.2, 0.3, 0.6, 0.8, 0.95, 1.05])
+    states = np.zeros((len(V), st.NP, 3, 1))
+    J, ok, _, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                   states=states)
+    grad, status = emu.adjoint_packed(packed, V, states, ok, tan)
+    assert not status.any() and ok.all()
+    err = named_fd_errors(oracle, [st], [p], CHARGED_WRT, V, J, grad)
+    print(f"charged traps, replay vs differences: {err}")
+    assert np.max(err) < 1e-8, err  # measured: 5e-10 at most
+    assert np.median(err) < 1e-9, err

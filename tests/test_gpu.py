@@ -570,15 +570,35 @@ def test_autograd_function_backward_is_the_adjoint(c4_gpu):
     assert np.allclose(got, expect, rtol=1e-9, atol=0.0)
 
 
-def test_charged_traps_are_refused_by_the_gradients_and_the_impedance(torch_cuda):
-    """A batch holding charged bulk traps: the steady solve takes it (the fixture tests), the
-    gradients and the impedance refuse it before any launch."""
+def test_charged_traps_gradients_and_the_impedance_refusal(torch_cuda, emu):
+    """A batch holding charged bulk traps (SimSS's run `bulk_acceptor_light`): its gradients
+    in the traps' numbers (`jv_gradients`, host tangents) are the CPU replay's to 1e-9 of their
+    scale (the replay's against the oracle's differences: test_adjoint.py); the impedance
+    refuses the batch before any launch."""
+    from conftest import bumped, fixture_inputs, number_of
     from jaxdd_amd.adjoint import jv_gradients
-    from jaxdd_amd.batch import prepare
+    from jaxdd_amd.batch import pack, prepare
+    from jaxdd_amd.device import device_arrays
     from jaxdd_amd.impedance import solve_impedance
+    from test_adjoint import CHARGED_WRT
 
-    batch = prepare([fixture_device("bulk_acceptor_light")])
+    st, p = fixture_inputs("bulk_acceptor_light")
+    packed = pack([device_arrays(st, p)])
+    tan = {key: np.zeros((len(CHARGED_WRT),) + packed[key].shape) for key in ("pt", "seg", "sc")}
+    for k, w in enumerate(CHARGED_WRT):
+        h = 1e-6 * abs(number_of(p, w))
+        up, dn = (pack([device_arrays(st, bumped(p, w, s * h))]) for s in (1.0, -1.0))
+        for key in tan:
+            tan[key][k] = (up[key] - dn[key]) / (2 * h)
+    V = np.array([-0.2, 0.3, 0.6, 0.8, 0.95, 1.05])
+    print("charged traps: jv_gradients on the GPU", flush=True)
+    J, ok, _, g = jv_gradients(packed, tan, V, tol=1e-12, strict=True, max_iter=200, lanes=2)
+    J, ok, g = _np(J), _np(ok), _np(g)
+    states = np.zeros((len(V), st.NP, 3, 1))
+    Jr, okr, _, _ = emu.solve_packed(packed, V, tol=1e-12, strict=True, max_iter=200,
+                                     states=states)
+    gr, status = emu.adjoint_packed(packed, V, states, okr, tan)
+    assert ok.all() and okr.all() and not status.any()
+    assert np.max(np.abs(g - gr)) <= 1e-9 * np.max(np.abs(gr))
     with pytest.raises(NotImplementedError, match="charged"):
-        solve_impedance(batch, [1e3])
-    with pytest.raises(NotImplementedError, match="charged"):
-        jv_gradients(batch, None, [0.0, 0.5])
+        solve_impedance(prepare([device_arrays(st, p)]), [1e3])
