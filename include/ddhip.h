@@ -229,9 +229,10 @@ int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L, double
  * carriers, uniform generation per layer or a generation profile, ohmic contacts, one bulk trap
  * level, mobile and fixed ions' masks and totals, direct recombination by k_direct or Langevin's
  * rate, device.py:900-911).
- * layers [nl][22][n]: per layer L, eps_r, E_c, E_v, N_c, N_D, N_A, mu_n, mu_p, nu_int_n,
+ * layers [nl][23][n]: per layer L, eps_r, E_c, E_v, N_c, N_D, N_A, mu_n, mu_p, nu_int_n,
  * nu_int_p, k_direct, N_t_bulk, C_n_bulk, C_p_bulk, E_t_bulk, G_ehp, N_anion, N_cation, mu_anion,
- * mu_cation, preLangevin; structure->flags [nl][n] bits: 1 bulk traps, 2 mobile anions, 4 fixed
+ * mu_cation, preLangevin, bulkTrapType (the traps' charge type: 1 donor, -1 acceptor, 0
+ * neutral; held, its direction ignored); structure->flags [nl][n] bits: 1 bulk traps, 2 mobile anions, 4 fixed
  * anions, 8 mobile cations, 16 fixed cations, 32 Langevin's rate preLangevin q (mu_n + mu_p) /
  * eps in place of k_direct, 64 that rate at the interval's mobilities (the interface's at a
  * layer's last point) instead of the layer's, 128 the layer generates (a profile's `layerGen`);
@@ -244,7 +245,7 @@ int ddh_graded_layout(int32_t n, int32_t NP, int32_t nl, const double *L, double
  * takes a profile, else >= 2. Outputs the ddh_batch arrays pt [NP + 2][4][n]
  * (h, x, G, fixed anions - cations), seg [nseg][23][n] (each segment's fields, a device's
  * segments beyond its own count repeating its last), sc [20][n], and ion_mu [nseg][2][n] (each
- * segment's anion and cation mobility). With K > 0 directions dlayers [K][nl][22][n] and
+ * segment's anion and cation mobility). With K > 0 directions dlayers [K][nl][23][n] and
  * dscalars [K][10][n] (each a direction in the numbers), also their derivatives along each,
  * forward mode: dpt [K][NP + 2][4][n], dseg [K][nseg][23][n], dsc [K][20][n], the tangents
  * ddh_adjoint takes; the structure is held (the thicknesses scale each layer's spacing and

@@ -91,7 +91,7 @@ def device_from_theta(theta, NP: int = 400, structure=None):
 
 def directions_of(thetas) -> tuple[np.ndarray, np.ndarray]:
     """The 16 parameters as directions in the GPU builder's numbers (`builder.LAYER_FIELDS`,
-    `SCALAR_FIELDS`): dlayers (16, 3, 22, n), dscalars (16, 10, n). Each is its number's unit
+    `SCALAR_FIELDS`): dlayers (16, 3, 23, n), dscalars (16, 10, n). Each is its number's unit
     direction, but for the trap density and the lifetime, which also move the perovskite's
     capture coefficients C = 1/(N_t tau)."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
@@ -147,7 +147,7 @@ STACK_FLAGS = (0, 1, 0)
 
 
 def numbers_of(thetas) -> tuple[np.ndarray, np.ndarray]:
-    """The GPU builder's numbers (`builder.number_arrays`' layout: layers [3][22][n], scalars
+    """The GPU builder's numbers (`builder.number_arrays`' layout: layers [3][23][n], scalars
     [10][n]) of parameter vectors (n, 16), vectorised: `params_of` for every device at once."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
 
@@ -397,7 +397,7 @@ def directions_for(wrt, n: int, nl: int) -> tuple[np.ndarray, np.ndarray]:
     (`builder.SCALAR_FIELDS`, e.g. "W_L") or (name, layer) pairs of per-layer numbers
     (`builder.LAYER_FIELDS`, e.g. ("mu_n", 1)), or "V" (`VOLTAGE`, a zero row here: it is no
     number of the builder's, and `voltage_tangents` sets its tangents); returns (dlayers
-    (K, nl, 22, n), dscalars (K, 10, n))."""
+    (K, nl, 23, n), dscalars (K, 10, n))."""
     from .builder import LAYER_FIELDS, SCALAR_FIELDS
 
     K = len(wrt)

@@ -27,7 +27,7 @@ from .grid import GRADED_POINTS_PER_EFOLD, GRADED_SMALLEST
 #: The per-layer numbers of ddh_build, in its order.
 LAYER_FIELDS = ("L", "eps_r", "E_c", "E_v", "N_c", "N_D", "N_A", "mu_n", "mu_p", "nu_int_n",
                 "nu_int_p", "k_direct", "N_t_bulk", "C_n_bulk", "C_p_bulk", "E_t_bulk", "G_ehp",
-                "N_anion", "N_cation", "mu_anion", "mu_cation", "preLangevin")
+                "N_anion", "N_cation", "mu_anion", "mu_cation", "preLangevin", "bulkTrapType")
 #: The per-device numbers of ddh_build, in its order (G_shunt = 1/R_shunt, 0 without a shunt).
 #: gen_profile: 1 where the device's generation is its profile (`gen_x`, `gen_G`), else 0.
 SCALAR_FIELDS = ("T", "W_L", "W_R", "G_frac", "V_pre", "G_shunt", "pre_cond", "fix_ions",
@@ -46,8 +46,6 @@ def check_buildable(st: Structure) -> None:
                                   "each layer's G_ehp or a profile")
     if any(st.sfb):
         raise NotImplementedError("flat-band work functions are outside the GPU builder")
-    if st.trap_type is not None and np.any(st.traps & (st.trap_type != 0)):
-        raise NotImplementedError("charged bulk traps are outside the GPU builder")
     if st.extra:
         raise NotImplementedError(f"optional modules outside the GPU builder: {sorted(st.extra)}")
     if len(st.i0) > MAX_LAYERS:
@@ -111,15 +109,19 @@ def structure_arrays(structures) -> dict:
 
 
 def number_arrays(structures, params) -> tuple[np.ndarray, np.ndarray]:
-    """(layers [nl][22][n], scalars [10][n]) of the devices' numbers (`device_parameters`'
-    dicts: per-layer arrays and scalars; a missing `preLangevin` is 1)."""
-    def number(p, f):
+    """(layers [nl][23][n], scalars [10][n]) of the devices' numbers (`device_parameters`'
+    dicts: per-layer arrays and scalars; a missing `preLangevin` is 1) and the bulk traps'
+    charge type of each structure (`bulkTrapType`, where the layer holds traps)."""
+    def number(st, p, f):
         if f == "preLangevin" and f not in p:
             return np.ones(len(np.atleast_1d(p["L"])))
+        if f == "bulkTrapType":
+            kind = np.zeros(len(st.i0)) if st.trap_type is None else st.trap_type
+            return np.where(st.traps, kind, 0).astype(float)
         return np.asarray(p[f], dtype=float)
 
-    layers = np.stack([np.stack([number(p, f) for f in LAYER_FIELDS], -1)
-                       for p in params], -1)  # (nl, 22, n)
+    layers = np.stack([np.stack([number(st, p, f) for f in LAYER_FIELDS], -1)
+                       for st, p in zip(structures, params)], -1)  # (nl, 23, n)
     scal = []
     for st, p in zip(structures, params):
         R_shunt = float(p.get("R_shunt", -1.0))
@@ -154,7 +156,7 @@ def profile_arrays(structures, params) -> np.ndarray | None:
 
 def build(structures, params, directions=None, device="cuda"):
     """The batch (`batch.prepare`'s dict, on ``device``) of devices given by their structures
-    and numbers, built by ddh_build. ``directions``: None, or (dlayers [K][nl][22][n], dscalars
+    and numbers, built by ddh_build. ``directions``: None, or (dlayers [K][nl][23][n], dscalars
     [K][10][n]) host arrays, directions in the numbers; then also the tangents {"pt", "seg",
     "sc"} (torch, on the device) along each, as `adjoint.jv_gradients` takes them."""
     dev = _cuda(device)
@@ -186,7 +188,7 @@ def build_graded(L, flags, layers, scalars, directions=None, NP: int = 400,
     from its thicknesses (ddh_graded_layout; ``L`` (n, nl)), then its arrays (ddh_build).
     ``flags`` (nl,) or (nl, n): each layer's bits (1 bulk traps, 2 mobile anions, 4 fixed
     anions, 8 mobile cations, 16 fixed cations, 32 Langevin recombination, 64 Langevin at the
-    interval's mobilities, 128 the layer takes the profile); ``layers`` [nl][22][n] and
+    interval's mobilities, 128 the layer takes the profile); ``layers`` [nl][23][n] and
     ``scalars`` [10][n] the numbers (`number_arrays`' layout); ``profile`` None or the profiles
     [2][m][n] (`profile_arrays`' layout) of the devices whose `gen_profile` is 1; ``directions``
     as `build`'s. Each point stops its

@@ -46,7 +46,9 @@ constexpr double BUILD_DENSITY_FLOOR = 1e-12;  // src/jaxdd/solver.py:54
 // Per-layer numbers [nl][BL_N][n] (the reference's `LAYER_PARAMETERS` of the path).
 enum BuildLayer {
     BL_L, BL_EPSR, BL_EC, BL_EV, BL_NC, BL_ND, BL_NA, BL_MUN, BL_MUP, BL_NUN, BL_NUP, BL_KD,
-    BL_NT, BL_CN, BL_CP, BL_ET, BL_G, BL_NANION, BL_NCATION, BL_MUA, BL_MUC, BL_PRELANGEVIN, BL_N
+    BL_NT, BL_CN, BL_CP, BL_ET, BL_G, BL_NANION, BL_NCATION, BL_MUA, BL_MUC, BL_PRELANGEVIN,
+    BL_TRAPTYPE,  // the bulk traps' charge type (1 donor, -1 acceptor, 0 neutral): no derivative
+    BL_N
 };
 // Per-device numbers [BS_N][n].
 // BS_GENPROF: 1 for a device whose generation is its profile (`gen`), 0 for each layer's G_ehp.
@@ -65,7 +67,7 @@ enum BuildFlag {
 enum BuildSeg {
     BG_EPS, BG_OFFN, BG_OFFP, BG_NC, BG_DOP, BG_KD, BG_NI2, BG_NT, BG_CN, BG_CP, BG_NT0,
     BG_PT0, BG_AM, BG_CM, BG_MUN, BG_MUP, BG_EC, BG_NI, BG_SP, BG_SCS, BG_IMUN, BG_IMUP,
-    BG_TT, BG_N  // BG_TT: the traps' charge type, 0 (the builder's traps are neutral)
+    BG_TT, BG_N  // BG_TT: the traps' charge type (held: no tangent)
 };
 static_assert((int)BG_N == (int)SEG_N, "the builder writes every packed segment field");
 enum BuildSc {
@@ -209,7 +211,7 @@ DDH_BFN void bsegment(const BuildArgs &a, const BuildLane &L, int j, int l, BD t
     f[BG_SCS] = 1.0 / (dmax_mean(mun, mup) * Vt * Nc);
     f[BG_IMUN] = 1.0 / mun;
     f[BG_IMUP] = 1.0 / mup;
-    f[BG_TT] = bd(0.0);
+    f[BG_TT] = trap ? bd(blayer(a, L, l, BL_TRAPTYPE).v) : zero;
 }
 
 DDH_BFN void bput_segment(const BuildArgs &a, const BuildLane &L, int s, const BD *f, BD mua,

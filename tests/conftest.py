@@ -66,10 +66,10 @@ def fixture_inputs(case: str):
     return device_parameters(read_setup(GOLDEN / case / "simulation_setup.txt"), grid="simss")
 
 
-#: The SimSS runs of one stack (3 layers, 400 points), eight of them generating by the
-#: profile file, one by each layer's G_ehp.
-SIMSS_RUNS = ("frozen_1e17_x1_thin_light", "frozen_thick_htl_dark", "held_thin_light",
-              "langevin_light", "logv_light", "none_light", "resist_light",
+#: The SimSS runs of one stack (3 layers, 400 points), nine of them generating by the
+#: profile file, one by each layer's G_ehp; one with charged (acceptor) bulk traps.
+SIMSS_RUNS = ("bulk_acceptor_light", "frozen_1e17_x1_thin_light", "frozen_thick_htl_dark",
+              "held_thin_light", "langevin_light", "logv_light", "none_light", "resist_light",
               "steady_1e17_thin_light", "thin_l1_light")
 #: Numbers the profile's stretch and scale depend on: the thicknesses and G_frac.
 PROFILE_WRT = (("L", 0), ("L", 1), ("L", 2), "G_frac")

@@ -89,7 +89,8 @@ def test_replay_builds_langevin_devices(emu, mode):
 
 
 def test_replay_builds_the_simss_runs_devices(emu):
-    """The nine SimSS runs' devices in one batch, eight generating by the profile file
+    """The ten SimSS runs' devices in one batch (one with charged bulk traps, their type a
+    segment field), nine generating by the profile file
     (reference: src/jaxdd/device.py:804-814 `_generation`, its axis stretched to the stack),
     one by each layer's G_ehp: the builder's lanes give the host builder's packed arrays (1e-14
     of each field's scale), the profile's G at every point to 1e-14 of itself, and the
@@ -106,7 +107,7 @@ def test_replay_builds_the_simss_runs_devices(emu):
 
     inputs = [fixture_inputs(c) for c in SIMSS_RUNS]
     structs, params = zip(*inputs)
-    assert sum(st.gen_mode == "file" for st in structs) == 8
+    assert sum(st.gen_mode == "file" for st in structs) == 9
     ref = pack([device_arrays(st, p) for st, p in inputs])
     dirs = A.directions_for(PROFILE_WRT, len(inputs), 3)
     got, tan = emu.build_packed(list(structs), list(params), dirs)
@@ -452,8 +453,8 @@ def test_gpu_langevin(torch_cuda, oracle, emu, mode):
 
 @pytest.mark.gpu
 def test_gpu_simss_runs_built_with_their_profiles(torch_cuda, oracle, emu):
-    """ddh_build of the nine SimSS runs' devices in one batch (eight generating by the profile
-    file, one by each layer's G_ehp): the packed arrays against the host builder's to 1e-14 and
+    """ddh_build of the ten SimSS runs' devices in one batch (nine generating by the profile
+    file, one by each layer's G_ehp, one with charged bulk traps): the packed arrays against the host builder's to 1e-14 and
     the tangents in the thicknesses and G_frac against the CPU replay's to 1e-13; then
     `param_gradients` of three of them against central differences of the oracle's curves
     (relative step 1e-5), to 1e-7 of each curve's scale (measured: 3e-9 at most, 3e-10 at the
@@ -480,7 +481,7 @@ def test_gpu_simss_runs_built_with_their_profiles(torch_cuda, oracle, emu):
         g, r = _np(tan[key]), rtan[key]
         assert np.max(np.abs(g - r)) <= 1e-13 * (np.max(np.abs(r)) + 1e-300), key
 
-    runs = ("held_thin_light", "steady_1e17_thin_light", "langevin_light")
+    runs = ("held_thin_light", "steady_1e17_thin_light", "bulk_acceptor_light")
     sub = [inputs[SIMSS_RUNS.index(c)] for c in runs]
     structs, params = zip(*sub)
     V = V_FULL[::10].copy()

@@ -182,9 +182,9 @@ def test_charged_traps_move_the_curve_as_simss(oracle):
         oracle.transient(dev, np.array([0.0, 1e-9]), np.array([0.0, 0.1]), np.ones(2))
 
 
-def test_charged_traps_are_refused_outside_the_steady_solve():
-    """The GPU builder, the gradients, the impedance and the transient take no charged traps:
-    `NotImplementedError` before any launch."""
+def test_charged_traps_are_refused_by_the_impedance_and_the_transient():
+    """The impedance and the transient take no charged traps (`NotImplementedError` before any
+    launch); the GPU builder takes them (their type a held per-layer number)."""
     from conftest import GOLDEN
     from jaxdd_amd import builder as B
     from jaxdd_amd.batch import charged_traps, refuse_charged_traps
@@ -193,13 +193,15 @@ def test_charged_traps_are_refused_outside_the_steady_solve():
 
     setup = read_setup(GOLDEN / "bulk_acceptor_light" / "simulation_setup.txt")
     check_supported(setup)
-    st, _ = device_parameters(setup, grid="simss")
-    with pytest.raises(NotImplementedError, match="charged"):
-        B.check_buildable(st)
+    st, p = device_parameters(setup, grid="simss")
+    B.check_buildable(st)
+    layers, _ = B.number_arrays([st], [p])
+    assert list(layers[:, B.LAYER_FIELDS.index("bulkTrapType"), 0]) == [0.0, -1.0, 0.0]
     dev = build_device(setup, grid="simss")
     assert charged_traps([dev]) and not charged_traps([fixture_device("thin_l1_light")])
-    with pytest.raises(NotImplementedError, match="the impedance: charged"):
-        refuse_charged_traps([dev], "the impedance")
+    for what in ("the impedance", "the transient"):
+        with pytest.raises(NotImplementedError, match=f"{what}: charged"):
+            refuse_charged_traps([dev], what)
 
 
 def test_open_circuit_voltage_is_simss(oracle):
